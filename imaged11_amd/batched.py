@@ -1,0 +1,186 @@
+"""Batched device API: many grains per launch on HBM-resident torch tensors.
+
+Where ImageD11 loops in Python over grains and calls one C routine per
+grain (``indexing.py:818-820`` for score, ``refinegrains.py:718-734`` for
+compute_gv + score_and_assign, ``refinegrains.py:459-498`` for
+score_and_refine), these functions run the whole loop in one kernel launch
+with the same per-call semantics.  Every function takes CUDA (HIP) tensors,
+enqueues on torch's current stream and returns without synchronising.
+"""
+import numpy as np
+import torch
+
+from . import _lib, geometry
+from ._lib import dptr, darr, stream_ptr
+
+
+def _need(t, dtype, name, shape=None):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("%s must be a GPU tensor" % name)
+    if t.dtype != dtype:
+        raise TypeError("%s must be %s, got %s" % (name, dtype, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    if shape is not None:
+        if len(shape) != t.dim() or any(s is not None and s != d for s, d in zip(shape, t.shape)):
+            raise ValueError("%s: expected shape %s, got %s" % (name, shape, tuple(t.shape)))
+    return t
+
+
+def _ready():
+    lib = _lib.load()
+    lib.gs_clear_error()
+    return lib
+
+
+def score_many(ubis, gv, tol, peaks_per_lane=0, out=None):
+    """counts[g] = score(ubis[g], gv, tol) for every grain g.
+
+    ubis (K,3,3) float64, gv (N,3) float64 -> int32 (K,)."""
+    lib = _ready()
+    _need(ubis, torch.float64, "ubis", (None, 3, 3))
+    _need(gv, torch.float64, "gv", (None, 3))
+    K = ubis.shape[0]
+    if out is None:
+        out = torch.empty(K, dtype=torch.int32, device=gv.device)
+    _need(out, torch.int32, "out", (K,))
+    _lib.check(lib.gs_score_many(dptr(ubis), K, dptr(gv), gv.shape[0], float(tol), dptr(out),
+                                 int(peaks_per_lane), stream_ptr()), "score_many")
+    return out
+
+
+def assign_many(ubis, gv, tol, drlv2, labels, grain_labels=None, out=None):
+    """score_and_assign(ubis[g], gv, tol, drlv2, labels, grain_labels[g]) for
+    g = 0..K-1 in order; drlv2 (N,) float64 and labels (N,) int32 are
+    updated in place.  grain_labels defaults to 0..K-1.  Returns the int32
+    (K,) per-call counts."""
+    lib = _ready()
+    _need(ubis, torch.float64, "ubis", (None, 3, 3))
+    _need(gv, torch.float64, "gv", (None, 3))
+    N = gv.shape[0]
+    K = ubis.shape[0]
+    _need(drlv2, torch.float64, "drlv2", (N,))
+    _need(labels, torch.int32, "labels", (N,))
+    if grain_labels is None:
+        grain_labels = torch.arange(K, dtype=torch.int32, device=gv.device)
+    _need(grain_labels, torch.int32, "grain_labels", (K,))
+    if out is None:
+        out = torch.empty(K, dtype=torch.int32, device=gv.device)
+    _lib.check(lib.gs_assign_many(dptr(ubis), dptr(grain_labels), K, dptr(gv), N, float(tol),
+                                  dptr(drlv2), dptr(labels), dptr(out), stream_ptr()),
+               "assign_many")
+    return out
+
+
+def xlylzl(sc, fc, p, r, dist, out=None):
+    """compute_xlylzl on the device: (N,) sc, fc -> (N,3) lab positions."""
+    lib = _ready()
+    _need(sc, torch.float64, "sc", (None,))
+    n = sc.shape[0]
+    _need(fc, torch.float64, "fc", (n,))
+    if out is None:
+        out = torch.empty((n, 3), dtype=torch.float64, device=sc.device)
+    _lib.check(lib.gs_compute_xlylzl_dev(dptr(sc), dptr(fc), darr(p), darr(np.ravel(r)), darr(dist),
+                                         dptr(out), n, stream_ptr()), "xlylzl")
+    return out
+
+
+def gvectors(xl, omega, omegasign, wvln, wedge, chi, t=(0.0, 0.0, 0.0), out=None):
+    """compute_gv on the device: (N,3) lab positions, (N,) omega -> (N,3)."""
+    lib = _ready()
+    _need(xl, torch.float64, "xlylzl", (None, 3))
+    n = xl.shape[0]
+    _need(omega, torch.float64, "omega", (n,))
+    if out is None:
+        out = torch.empty((n, 3), dtype=torch.float64, device=xl.device)
+    _lib.check(lib.gs_compute_gv_dev(dptr(xl), dptr(omega), float(omegasign), float(wvln),
+                                     float(wedge), float(chi), darr(t), dptr(out), n, stream_ptr()),
+               "gvectors")
+    return out
+
+
+def gvectors_trig(xl, so, co, mat, wvln, t=(0.0, 0.0, 0.0), out=None):
+    """compute_gv arithmetic with the per-peak sin/cos and wedge/chi matrix
+    supplied (used to check the device arithmetic independently of how the
+    sines were evaluated)."""
+    lib = _ready()
+    _need(xl, torch.float64, "xlylzl", (None, 3))
+    n = xl.shape[0]
+    _need(so, torch.float64, "so", (n,))
+    _need(co, torch.float64, "co", (n,))
+    if out is None:
+        out = torch.empty((n, 3), dtype=torch.float64, device=xl.device)
+    _lib.check(lib.gs_compute_gv_trig_dev(dptr(xl), dptr(so), dptr(co), darr(np.ravel(mat)),
+                                          float(wvln), darr(t), dptr(out), n, stream_ptr()),
+               "gvectors_trig")
+    return out
+
+
+def sincos_omega(omega, omegasign):
+    """Per-peak sin, cos of ((PI/180)*omega)*omegasign."""
+    lib = _ready()
+    _need(omega, torch.float64, "omega", (None,))
+    so = torch.empty_like(omega)
+    co = torch.empty_like(omega)
+    _lib.check(lib.gs_sincos_omega(dptr(omega), float(omegasign), dptr(so), dptr(co),
+                                   omega.shape[0], stream_ptr()), "sincos_omega")
+    return so, co
+
+
+def sf2gv(sc, fc, omega, pars, t=(0.0, 0.0, 0.0), keep=False):
+    """Raw peaks (sc, fc, omega) to g-vectors in one fused pass, with the
+    detector/goniometer of an ImageD11 parameter dict (``Ctransform.sf2gv``,
+    ``ImageD11/transform.py``).  keep=True also returns the lab positions
+    and per-peak sin/cos: (gv, xl, so, co)."""
+    lib = _ready()
+    _need(sc, torch.float64, "sc", (None,))
+    n = sc.shape[0]
+    _need(fc, torch.float64, "fc", (n,))
+    _need(omega, torch.float64, "omega", (n,))
+    p, r, dist = geometry.pack_xlylzl(pars)
+    sign, wvln, wedge, chi = geometry.pack_gv(pars)
+    gv = torch.empty((n, 3), dtype=torch.float64, device=sc.device)
+    xl = so = co = None
+    if keep:
+        xl = torch.empty((n, 3), dtype=torch.float64, device=sc.device)
+        so = torch.empty(n, dtype=torch.float64, device=sc.device)
+        co = torch.empty(n, dtype=torch.float64, device=sc.device)
+    _lib.check(lib.gs_sf2gv(dptr(sc), dptr(fc), dptr(omega), darr(p), darr(r), darr(dist),
+                            float(sign), float(wvln), float(wedge), float(chi), darr(t),
+                            dptr(xl) if keep else None, dptr(so) if keep else None,
+                            dptr(co) if keep else None, dptr(gv), n, stream_ptr()), "sf2gv")
+    if keep:
+        return gv, xl, so, co
+    return gv
+
+
+def list_order(off):
+    """Grain visiting order for the persistent refine: longest list first."""
+    lengths = off[1:] - off[:-1]
+    return torch.argsort(lengths, descending=True, stable=True).to(torch.int32)
+
+
+def refine_lists(ubis, gv, idx, off, tol, n_iter=2, order=None, lanes_per_grain=0,
+                 waves_per_cu=0, inplace=False):
+    """n_iter successive score_and_refine calls per grain on CSR peak lists.
+
+    ubis (K,3,3) float64; gv (N,3); idx int32 peak indices; off int64 (K+1,)
+    offsets.  Returns (ubis_out, n int32 (n_iter,K), mean float64 (n_iter,K));
+    n/mean of iteration i are those the i-th call returns (kept peaks and
+    mean drlv2 of the UBI entering that call)."""
+    lib = _ready()
+    _need(ubis, torch.float64, "ubis", (None, 3, 3))
+    _need(gv, torch.float64, "gv", (None, 3))
+    K = ubis.shape[0]
+    _need(idx, torch.int32, "idx", (None,))
+    _need(off, torch.int64, "off", (K + 1,))
+    if order is None:
+        order = list_order(off)
+    _need(order, torch.int32, "order", (K,))
+    u = ubis if inplace else ubis.clone()
+    n = torch.empty((n_iter, K), dtype=torch.int32, device=gv.device)
+    m = torch.empty((n_iter, K), dtype=torch.float64, device=gv.device)
+    _lib.check(lib.gs_refine_lists(dptr(u), K, dptr(gv), dptr(idx), dptr(off), dptr(order),
+                                   float(tol), int(n_iter), dptr(n), dptr(m), int(lanes_per_grain),
+                                   int(waves_per_cu), stream_ptr()), "refine_lists")
+    return u, n, m

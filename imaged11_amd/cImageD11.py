@@ -1,0 +1,307 @@
+"""Drop-in for ``ImageD11.cImageD11`` on the grain-scoring path.
+
+Same names, arguments, return values and in-place behaviour as the f2py
+wrappers of ``src/_cImageD11.pyf:106-170,551-583`` (re-exported by
+``ImageD11/cImageD11.py:10``), so callers such as ``indexing.py`` and
+``refinegrains.py`` work unchanged once these names are bound in place of
+the extension's:
+
+    n = score(ubi, gv, tol)
+    n, mean = score_and_refine(ubi, gv, tol)          # ubi updated in place
+    n = score_and_assign(ubi, gv, tol, drlv2, labels, label)
+    npk, mean = refine_assigned(ubi, gv, labels, label)
+    compute_gv(xlylzl, omega, omegasign, wvln, wedge, chi, t, gv)
+    compute_xlylzl(s, f, p, r, dist, xlylzl)
+    verify_rounding(n)
+
+Argument handling follows f2py: ``intent(in)`` arrays of another dtype or
+layout are converted (copied); ``intent(inout)`` arrays (``gv`` of
+compute_gv, ``xlylzl``, ``drlv2``, ``labels``, and the ``ubi`` of
+score_and_refine) must already be C-contiguous with the exact dtype, or
+ValueError is raised before anything runs.  ``refine_assigned`` writes the
+refined matrix into ``ubi`` only when ``ubi`` is a C-contiguous float64
+(3, 3) array, as the reference's ``intent(c,in)`` does.
+
+Every array argument may also be a torch tensor resident on the GPU: the
+work then runs on the device arrays directly (no staging), on torch's
+current stream, and returns once the results are available.  numpy inputs
+go through the library's host-pointer entry points, which stage to the GPU.
+All computation runs in the HIP kernels; there is no CPU path.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import GscoreError, dptr, hptr, darr, stream_ptr
+
+__all__ = [
+    "verify_rounding", "score", "score_and_refine", "score_and_assign", "refine_assigned",
+    "compute_gv", "compute_xlylzl", "cimaged11_omp_set_num_threads",
+    "cimaged11_omp_get_max_threads", "GscoreError",
+]
+
+_omp_threads = 1
+
+
+def cimaged11_omp_set_num_threads(n):
+    """Kept for callers that pin host threads (``ImageD11/indexing.py:1402``);
+    the kernels run on the GPU, so the value is only recorded."""
+    global _omp_threads
+    _omp_threads = max(1, int(n))
+
+
+def cimaged11_omp_get_max_threads():
+    """Returns the value recorded by cimaged11_omp_set_num_threads."""
+    return _omp_threads
+
+
+def _is_dev(x):
+    return isinstance(x, torch.Tensor) and x.is_cuda
+
+
+def _in_array(x, dtype, shape, name):
+    """intent(in): convert to a C-contiguous array of dtype/shape (copy if needed)."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=dtype))
+    if shape is not None and not _shape_ok(a.shape, shape):
+        raise ValueError("%s: expected shape %s, got %s" % (name, shape, a.shape))
+    return a
+
+
+def _shape_ok(got, want):
+    if len(got) != len(want):
+        return False
+    return all(w is None or g == w for g, w in zip(got, want))
+
+
+def _inout_array(x, dtype, shape, name):
+    """intent(inout): must already be an ndarray of that dtype, C-contiguous."""
+    if not isinstance(x, np.ndarray):
+        raise ValueError("%s: intent(inout) argument must be a numpy array" % name)
+    if x.dtype != np.dtype(dtype):
+        raise ValueError("%s: intent(inout) array must have dtype %s, got %s"
+                         % (name, np.dtype(dtype), x.dtype))
+    if not x.flags.c_contiguous:
+        raise ValueError("%s: intent(inout) array is not C-contiguous" % name)
+    if shape is not None and not _shape_ok(x.shape, shape):
+        raise ValueError("%s: expected shape %s, got %s" % (name, shape, x.shape))
+    return x
+
+
+def _dev_in(x, dtype, shape, name, device=None):
+    """A device tensor input converted to a contiguous tensor of dtype."""
+    if _is_dev(x):
+        t = x.to(dtype=dtype).contiguous()
+    else:
+        t = torch.as_tensor(np.ascontiguousarray(np.asarray(x, dtype=_NP[dtype])),
+                            device=device)
+    if shape is not None and not _shape_ok(tuple(t.shape), shape):
+        raise ValueError("%s: expected shape %s, got %s" % (name, shape, tuple(t.shape)))
+    return t
+
+
+def _dev_inout(x, dtype, shape, name):
+    if x.dtype != dtype:
+        raise ValueError("%s: intent(inout) tensor must have dtype %s, got %s" % (name, dtype, x.dtype))
+    if not x.is_contiguous():
+        raise ValueError("%s: intent(inout) tensor is not contiguous" % name)
+    if shape is not None and not _shape_ok(tuple(x.shape), shape):
+        raise ValueError("%s: expected shape %s, got %s" % (name, shape, tuple(x.shape)))
+    return x
+
+
+_NP = {torch.float64: np.float64, torch.int32: np.int32}
+
+
+def _lib_ready():
+    lib = _lib.load()
+    _lib.require_gpu()
+    lib.gs_clear_error()
+    return lib
+
+
+
+def verify_rounding(n):
+    """Number of disagreements between the device's MAGIC rounding and
+    floor(x + 0.5) on 600 integer-valued points (``src/closest.c:100-119``);
+    0 when the rounding is right."""
+    lib = _lib_ready()
+    r = lib.verify_rounding(int(n))
+    _lib.check_host_call("verify_rounding")
+    return r
+
+
+def score(ubi, gv, tol):
+    """Count the g-vectors indexed by ubi within tol (``src/closest.c:227-248``)."""
+    if _is_dev(gv):
+        g = _dev_in(gv, torch.float64, (None, 3), "gv")
+        u = _dev_in(ubi, torch.float64, (3, 3), "ubi", device=g.device)
+        lib = _lib_ready()
+        out = torch.empty(1, dtype=torch.int32, device=g.device)
+        _lib.check(lib.gs_score_many(dptr(u), 1, dptr(g), g.shape[0], float(tol), dptr(out), 0,
+                                     stream_ptr()), "score")
+        return int(out.item())
+    u = _in_array(ubi, np.float64, (3, 3), "ubi")
+    g = _in_array(gv, np.float64, (None, 3), "gv")
+    lib = _lib_ready()
+    n = lib.score(hptr(u), hptr(g), float(tol), g.shape[0])
+    _lib.check_host_call("score")
+    return n
+
+
+def _device_scalars(device):
+    """(int32 view, float64 view) of one 16-byte device result buffer."""
+    res = torch.empty(2, dtype=torch.float64, device=device)
+    return res, res[0:1].view(torch.int32)[0:1], res[1:2]
+
+
+def _read_scalars(res):
+    r = res.cpu()
+    return int(r[0:1].view(torch.int32)[0]), float(r[1])
+
+
+def score_and_refine(ubi, gv, tol):
+    """Score, then least-squares refine ubi in place on the indexed peaks
+    (``src/closest.c:267-345``).  Returns (n, mean drlv2 before refinement)."""
+    if _is_dev(ubi) or _is_dev(gv):
+        g = _dev_in(gv, torch.float64, (None, 3), "gv")
+        if _is_dev(ubi):
+            host = None
+            u = _dev_inout(ubi, torch.float64, (3, 3), "ubi")
+        else:
+            host = _inout_array(ubi, np.float64, (3, 3), "ubi")
+            u = torch.as_tensor(host, device=g.device).clone()
+        lib = _lib_ready()
+        res, n, m = _device_scalars(g.device)
+        _lib.check(lib.gs_score_and_refine_dev(dptr(u), dptr(g), g.shape[0], float(tol), dptr(n),
+                                               dptr(m), stream_ptr()), "score_and_refine")
+        out = _read_scalars(res)
+        if host is not None:
+            host[...] = u.cpu().numpy()
+        return out
+    u = _inout_array(ubi, np.float64, (3, 3), "ubi")
+    g = _in_array(gv, np.float64, (None, 3), "gv")
+    lib = _lib_ready()
+    n = ctypes.c_int(0)
+    m = ctypes.c_double(0.0)
+    lib.score_and_refine(hptr(u), hptr(g), float(tol), ctypes.byref(n), ctypes.byref(m), g.shape[0])
+    _lib.check_host_call("score_and_refine")
+    return n.value, m.value
+
+
+def score_and_assign(ubi, gv, tol, drlv2, labels, label):
+    """Assign peaks to this ubi where they fit better than drlv2
+    (``src/closest.c:366-397``); drlv2 and labels are updated in place.
+    Returns the number of peaks assigned by this call."""
+    if _is_dev(gv) or _is_dev(drlv2) or _is_dev(labels):
+        g = _dev_in(gv, torch.float64, (None, 3), "gv")
+        ng = g.shape[0]
+        d = _dev_inout(drlv2, torch.float64, (ng,), "drlv2")
+        lab = _dev_inout(labels, torch.int32, (ng,), "labels")
+        u = _dev_in(ubi, torch.float64, (3, 3), "ubi", device=g.device)
+        lib = _lib_ready()
+        gl = torch.tensor([int(label)], dtype=torch.int32, device=g.device)
+        out = torch.empty(1, dtype=torch.int32, device=g.device)
+        _lib.check(lib.gs_assign_many(dptr(u), dptr(gl), 1, dptr(g), ng, float(tol), dptr(d),
+                                      dptr(lab), dptr(out), stream_ptr()), "score_and_assign")
+        return int(out.item())
+    u = _in_array(ubi, np.float64, (3, 3), "ubi")
+    g = _in_array(gv, np.float64, (None, 3), "gv")
+    ng = g.shape[0]
+    d = _inout_array(drlv2, np.float64, (ng,), "drlv2")
+    lab = _inout_array(labels, np.int32, (ng,), "labels")
+    lib = _lib_ready()
+    n = lib.score_and_assign(hptr(u), hptr(g), float(tol), hptr(d), hptr(lab), int(label), ng)
+    _lib.check_host_call("score_and_assign")
+    return n
+
+
+def refine_assigned(ubi, gv, labels, label):
+    """Least-squares fit on the peaks with labels == label
+    (``src/closest.c:416-474``).  Returns (npk, mean drlv2 before
+    refinement); the fitted matrix is written into ubi only if ubi is a
+    C-contiguous float64 (3, 3) array (or a float64 device tensor)."""
+    writable = (isinstance(ubi, np.ndarray) and ubi.dtype == np.float64 and
+                ubi.flags.c_contiguous and ubi.shape == (3, 3))
+    if _is_dev(gv) or _is_dev(labels) or _is_dev(ubi):
+        g = _dev_in(gv, torch.float64, (None, 3), "gv")
+        ng = g.shape[0]
+        lab = _dev_in(labels, torch.int32, (ng,), "labels", device=g.device)
+        if _is_dev(ubi) and ubi.dtype == torch.float64 and ubi.is_contiguous():
+            u = _dev_inout(ubi, torch.float64, (3, 3), "ubi")
+        else:
+            u = _dev_in(ubi, torch.float64, (3, 3), "ubi", device=g.device).clone()
+        lib = _lib_ready()
+        res, n, m = _device_scalars(g.device)
+        _lib.check(lib.gs_refine_assigned_dev(dptr(u), dptr(g), ng, dptr(lab), int(label), dptr(n),
+                                              dptr(m), stream_ptr()), "refine_assigned")
+        out = _read_scalars(res)
+        if writable:
+            ubi[...] = u.cpu().numpy()
+        return out
+    u = ubi if writable else _in_array(ubi, np.float64, (3, 3), "ubi")
+    g = _in_array(gv, np.float64, (None, 3), "gv")
+    ng = g.shape[0]
+    lab = _in_array(labels, np.int32, (ng,), "labels")
+    lib = _lib_ready()
+    npk = ctypes.c_int(0)
+    m = ctypes.c_double(0.0)
+    lib.refine_assigned(hptr(u), hptr(g), hptr(lab), int(label), ctypes.byref(npk), ctypes.byref(m),
+                        ng)
+    _lib.check_host_call("refine_assigned")
+    return npk.value, m.value
+
+
+def compute_gv(xlylzl, omega, omegasign, wvln, wedge, chi, t, gv):
+    """g-vectors from lab-frame spot positions (``src/cdiffraction.c:112-168``),
+    written into gv (n, 3)."""
+    tt = _in_array(np.asarray(t, dtype=np.float64).reshape(-1), np.float64, (3,), "t")
+    if _is_dev(gv) or _is_dev(xlylzl) or _is_dev(omega):
+        x = _dev_in(xlylzl, torch.float64, (None, 3), "xlylzl")
+        n = x.shape[0]
+        w = _dev_in(omega, torch.float64, (n,), "omega", device=x.device)
+        out = _dev_inout(gv, torch.float64, (n, 3), "gv")
+        lib = _lib_ready()
+        _lib.check(lib.gs_compute_gv_dev(dptr(x), dptr(w), float(omegasign), float(wvln),
+                                         float(wedge), float(chi), darr(tt), dptr(out), n,
+                                         stream_ptr()), "compute_gv")
+        torch.cuda.current_stream().synchronize()
+        return None
+    x = _in_array(xlylzl, np.float64, (None, 3), "xlylzl")
+    n = x.shape[0]
+    w = _in_array(omega, np.float64, (n,), "omega")
+    out = _inout_array(gv, np.float64, (n, 3), "gv")
+    lib = _lib_ready()
+    lib.compute_gv(hptr(x), hptr(w), float(omegasign), float(wvln), float(wedge), float(chi),
+                   hptr(tt), hptr(out), n)
+    _lib.check_host_call("compute_gv")
+    return None
+
+
+def compute_xlylzl(s, f, p, r, dist, xlylzl):
+    """Lab-frame spot positions from detector pixel coordinates
+    (``src/cdiffraction.c:190-224``), written into xlylzl (n, 3).
+    p = [s_cen, f_cen, s_size, f_size], r = (tilt . flip).ravel(),
+    dist = [dx, dy, dz]."""
+    pp = _in_array(p, np.float64, (4,), "p")
+    rr = _in_array(np.asarray(r, dtype=np.float64).reshape(-1), np.float64, (9,), "r")
+    dd = _in_array(dist, np.float64, (3,), "dist")
+    if _is_dev(xlylzl) or _is_dev(s) or _is_dev(f):
+        ss = _dev_in(s, torch.float64, (None,), "s")
+        n = ss.shape[0]
+        ff = _dev_in(f, torch.float64, (n,), "f", device=ss.device)
+        out = _dev_inout(xlylzl, torch.float64, (n, 3), "xlylzl")
+        lib = _lib_ready()
+        _lib.check(lib.gs_compute_xlylzl_dev(dptr(ss), dptr(ff), darr(pp), darr(rr), darr(dd),
+                                             dptr(out), n, stream_ptr()), "compute_xlylzl")
+        torch.cuda.current_stream().synchronize()
+        return None
+    ss = _in_array(s, np.float64, (None,), "s")
+    n = ss.shape[0]
+    ff = _in_array(f, np.float64, (n,), "f")
+    out = _inout_array(xlylzl, np.float64, (n, 3), "xlylzl")
+    lib = _lib_ready()
+    lib.compute_xlylzl(hptr(ss), hptr(ff), hptr(pp), hptr(rr), hptr(dd), hptr(out), n)
+    _lib.check_host_call("compute_xlylzl")
+    return None

@@ -1,0 +1,280 @@
+// Device arithmetic shared by the grain-scoring kernels (gfx950, FP64 VALU).
+//
+// Everything that decides a count, a label or a drlv2 value is written as
+// the reference writes it: left-to-right sums of separately rounded
+// products (this file is compiled with -ffp-contract=off), rounding to the
+// nearest integer by two literal adds of 1.5*2^52, and the wedge/chi and
+// geometry expressions in the reference's macro-expansion order.  FMA is
+// used only where the result is not compared with the reference bit for
+// bit: inside the double-double sincos, which is accurate to ~2^-80 and
+// then rounded once.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gs_trig_constants.h"
+
+#pragma clang fp contract(off)
+
+#define GS_HD __host__ __device__
+#define GS_MAGIC 6755399441055744.0
+#define GS_PI 3.14159265358979323846
+#define GS_WAVE 64
+
+// Round to nearest, ties to even, for |x| < 2^51; bit-identical to
+// (x + MAGIC) - MAGIC for every input (src/closest.c:84-85).
+GS_HD __forceinline__ double gs_round(double x) {
+    return (x + GS_MAGIC) - GS_MAGIC;
+}
+
+// h_i = (u_i0*g0 + u_i1*g1) + u_i2*g2 and its squared distance to the
+// nearest integer triple (src/closest.c:236-242, :288-294).
+struct gs_ubi {
+    double u00, u01, u02, u10, u11, u12, u20, u21, u22;
+};
+
+GS_HD __forceinline__ gs_ubi gs_load_ubi(const double* __restrict__ u) {
+    gs_ubi r;
+    r.u00 = u[0]; r.u01 = u[1]; r.u02 = u[2];
+    r.u10 = u[3]; r.u11 = u[4]; r.u12 = u[5];
+    r.u20 = u[6]; r.u21 = u[7]; r.u22 = u[8];
+    return r;
+}
+
+GS_HD __forceinline__ double gs_drlv2(const gs_ubi& u, double x, double y, double z,
+                                           double& ih0, double& ih1, double& ih2) {
+    double h0 = u.u00 * x + u.u01 * y + u.u02 * z;
+    double h1 = u.u10 * x + u.u11 * y + u.u12 * z;
+    double h2 = u.u20 * x + u.u21 * y + u.u22 * z;
+    ih0 = gs_round(h0);
+    ih1 = gs_round(h1);
+    ih2 = gs_round(h2);
+    double t0 = h0 - ih0, t1 = h1 - ih1, t2 = h2 - ih2;
+    return t0 * t0 + t1 * t1 + t2 * t2;
+}
+
+GS_HD __forceinline__ double gs_drlv2(const gs_ubi& u, double x, double y, double z) {
+    double a, b, c;
+    return gs_drlv2(u, x, y, z, a, b, c);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 inverse in place, the cofactor formula and operation order of
+// src/closest.c:37-76; returns 0, or -1 when det == 0 exactly.
+GS_HD __forceinline__ int gs_inverse3x3(double* A) {
+    double det = A[0] * (A[8] * A[4] - A[7] * A[5]) - A[3] * (A[8] * A[1] - A[7] * A[2]) +
+                 A[6] * (A[5] * A[1] - A[4] * A[2]);
+    if (!(det != 0.)) return -1;
+    double B[9];
+    B[0] = (A[8] * A[4] - A[7] * A[5]) / det;
+    B[1] = -(A[8] * A[1] - A[7] * A[2]) / det;
+    B[2] = (A[5] * A[1] - A[4] * A[2]) / det;
+    B[3] = -(A[8] * A[3] - A[6] * A[5]) / det;
+    B[4] = (A[8] * A[0] - A[6] * A[2]) / det;
+    B[5] = -(A[5] * A[0] - A[3] * A[2]) / det;
+    B[6] = (A[7] * A[3] - A[6] * A[4]) / det;
+    B[7] = -(A[7] * A[0] - A[6] * A[1]) / det;
+    B[8] = (A[4] * A[0] - A[3] * A[1]) / det;
+    for (int i = 0; i < 9; i++) A[i] = B[i];
+    return 0;
+}
+
+// UB = R . H^-1 then UBI = UB^-1, written to ubi only if both inverses
+// exist (src/closest.c:320-336).  H is the full symmetric matrix.
+GS_HD __forceinline__ void gs_solve_ubi(double* ubi, const double* R, double* H) {
+    double UB[9];
+    for (int i = 0; i < 9; i++) UB[i] = 0.;
+    int k = gs_inverse3x3(H);
+    if (k == 0) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int l = 0; l < 3; l++) UB[3 * i + j] = UB[3 * i + j] + R[3 * i + l] * H[3 * l + j];
+    }
+    if (k == 0 && gs_inverse3x3(UB) == 0)
+        for (int i = 0; i < 9; i++) ubi[i] = UB[i];
+}
+
+// ---------------------------------------------------------------------------
+// Double-double helpers (error-free transformations with FMA).
+struct gs_dd {
+    double hi, lo;
+};
+
+GS_HD __forceinline__ gs_dd gs_two_sum(double a, double b) {
+    double s = a + b;
+    double bb = s - a;
+    double e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+
+GS_HD __forceinline__ gs_dd gs_fast_two_sum(double a, double b) {
+    double s = a + b;
+    return {s, b - (s - a)};
+}
+
+GS_HD __forceinline__ gs_dd gs_two_prod(double a, double b) {
+    double p = a * b;
+    return {p, __builtin_fma(a, b, -p)};
+}
+
+// a*b for double-double a, b (relative error ~2^-104)
+GS_HD __forceinline__ gs_dd gs_dd_mul(gs_dd a, gs_dd b) {
+    gs_dd p = gs_two_prod(a.hi, b.hi);
+    p.lo = __builtin_fma(a.hi, b.lo, p.lo);
+    p.lo = __builtin_fma(a.lo, b.hi, p.lo);
+    return gs_fast_two_sum(p.hi, p.lo);
+}
+
+// a+b when |b| is well below |a| (Horner steps of a convergent series)
+GS_HD __forceinline__ gs_dd gs_dd_add_small(gs_dd a, gs_dd b) {
+    gs_dd s = gs_two_sum(a.hi, b.hi);
+    s.lo += a.lo + b.lo;
+    return gs_fast_two_sum(s.hi, s.lo);
+}
+
+// a - b for general double-double values
+GS_HD __forceinline__ gs_dd gs_dd_sub(gs_dd a, gs_dd b) {
+    gs_dd s = gs_two_sum(a.hi, -b.hi);
+    gs_dd t = gs_two_sum(a.lo, -b.lo);
+    s.lo += t.hi;
+    s = gs_fast_two_sum(s.hi, s.lo);
+    s.lo += t.lo;
+    return gs_fast_two_sum(s.hi, s.lo);
+}
+
+// sin and cos of a double argument x (radians), each rounded once from a
+// double-double value accurate to ~2^-80 relative, so the results are the
+// correctly rounded values except in extremely rare near-midpoint cases.
+// Cody-Waite reduction by pi/2 held in four pieces (exact for |k| < 2^20),
+// then Taylor series on |r| <= pi/4 with the leading terms in
+// double-double.  Arguments beyond 2^19*pi use the device library.
+GS_HD __noinline__ void gs_sincos(double x, double* s_out, double* c_out) {
+    if (!(fabs(x) < 823549.6)) {  // 2^19 * pi, also catches NaN/Inf
+        *s_out = sin(x);
+        *c_out = cos(x);
+        return;
+    }
+    if (x == 0.0) {  // keeps the sign of a zero argument
+        *s_out = x;
+        *c_out = 1.0;
+        return;
+    }
+    double kf = rint(x * GS_2_OVER_PI);
+    int q = ((int)kf) & 3;
+    gs_dd r;
+    {
+        double t = x - kf * GS_PIO2_1;  // exact: k*PIO2_1 has <= 53 bits, Sterbenz
+        gs_dd p2 = gs_two_prod(kf, GS_PIO2_2);
+        r = gs_dd_sub({t, 0.0}, p2);
+        r = gs_dd_sub(r, {kf * GS_PIO2_3, 0.0});  // exact product (33+20 bits)
+        r = gs_dd_sub(r, {kf * GS_PIO2_4, 0.0});
+    }
+    gs_dd r2 = gs_dd_mul(r, r);
+    double z = r2.hi;
+    // sin(r) = r * (1 + r2 * S(r2)),   cos(r) = 1 + r2 * C(r2)
+    double ts = GS_SIN_TAIL[7];
+#pragma unroll
+    for (int i = 6; i >= 0; i--) ts = __builtin_fma(ts, z, GS_SIN_TAIL[i]);
+    double tc = GS_COS_TAIL[7];
+#pragma unroll
+    for (int i = 6; i >= 0; i--) tc = __builtin_fma(tc, z, GS_COS_TAIL[i]);
+    gs_dd S = {ts, 0.0};
+#pragma unroll
+    for (int i = 3; i >= 0; i--) S = gs_dd_add_small({GS_SIN_DD[i][0], GS_SIN_DD[i][1]}, gs_dd_mul(S, r2));
+    gs_dd C = {tc, 0.0};
+#pragma unroll
+    for (int i = 4; i >= 0; i--) C = gs_dd_add_small({GS_COS_DD[i][0], GS_COS_DD[i][1]}, gs_dd_mul(C, r2));
+    gs_dd sr = gs_dd_mul(r, gs_dd_add_small({1.0, 0.0}, gs_dd_mul(r2, S)));
+    gs_dd cr = gs_dd_add_small({1.0, 0.0}, gs_dd_mul(r2, C));
+    double sv = sr.hi + sr.lo;
+    double cv = cr.hi + cr.lo;
+    double so, co;
+    switch (q) {
+        case 0: so = sv; co = cv; break;
+        case 1: so = cv; co = -sv; break;
+        case 2: so = -sv; co = -cv; break;
+        default: so = -cv; co = sv; break;
+    }
+    *s_out = so;
+    *c_out = co;
+}
+
+// ---------------------------------------------------------------------------
+// Diffraction geometry (src/cdiffraction.c).
+
+// mat = W(wedge).C(chi), formed with the operand order of the matmat macro
+// (src/cdiffraction.h:20-29) from sin((wedge*PI)/180.0) etc.
+// (src/cdiffraction.c:119-142).
+GS_HD __forceinline__ void gs_wedge_chi_matrix(double wedge, double chi, double* mat) {
+    double sw, cw, sc, cc;
+    gs_sincos(wedge * GS_PI / 180.0, &sw, &cw);
+    gs_sincos(chi * GS_PI / 180.0, &sc, &cc);
+    double w[9] = {cw, 0.0, -sw, 0., 1.0, 0., sw, 0.0, cw};
+    double c[9] = {1., 0.0, 0., 0., cc, -sc, 0., sc, cc};
+    for (int col = 0; col < 3; col++)
+        for (int row = 0; row < 3; row++)
+            mat[3 * col + row] = c[row] * w[3 * col + 0] + c[row + 3] * w[3 * col + 1] +
+                                 c[row + 6] * w[3 * col + 2];
+}
+
+// the omega argument exactly as the reference forms it:
+// ((PI/180.0)*omega)*omegasign (src/cdiffraction.c:146-147)
+GS_HD __forceinline__ double gs_omega_rad(double omega, double omegasign) {
+    return GS_PI / 180.0 * omega * omegasign;
+}
+
+// one peak of compute_gv given sin/cos of its omega (src/cdiffraction.c:149-166)
+GS_HD __forceinline__ void gs_gv_one(double xl0, double xl1, double xl2, double so, double co,
+                                          const double* mat, double ds, double t0, double t1,
+                                          double t2, double& g0, double& g1, double& g2) {
+    double u0 = co * t0 - so * t1;
+    double u1 = so * t0 + co * t1;
+    double u2 = t2;
+    double o0 = mat[0] * u0 + mat[1] * u1 + mat[2] * u2;
+    double o1 = mat[3] * u0 + mat[4] * u1 + mat[5] * u2;
+    double o2 = mat[6] * u0 + mat[7] * u1 + mat[8] * u2;
+    double d0 = xl0 - o0, d1 = xl1 - o1, d2 = xl2 - o2;
+    double modyz = 1. / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    double k0 = ds * (d0 * modyz - 1.);
+    double k1 = ds * d1 * modyz;
+    double k2 = ds * d2 * modyz;
+    double v0 = mat[0] * k0 + mat[3] * k1 + mat[6] * k2;
+    double v1 = mat[1] * k0 + mat[4] * k1 + mat[7] * k2;
+    double v2 = mat[2] * k0 + mat[5] * k1 + mat[8] * k2;
+    g0 = co * v0 + so * v1;
+    g1 = -so * v0 + co * v1;
+    g2 = v2;
+}
+
+// one peak of compute_xlylzl (src/cdiffraction.c:213-222)
+GS_HD __forceinline__ void gs_xlylzl_one(double s, double f, const double* p, const double* r,
+                                              const double* dist, double& x0, double& x1,
+                                              double& x2) {
+    double v1 = (f - p[1]) * p[3];
+    double v2 = (s - p[0]) * p[2];
+    x0 = r[1] * v1 + r[2] * v2 + dist[0];
+    x1 = r[4] * v1 + r[5] * v2 + dist[1];
+    x2 = r[7] * v1 + r[8] * v2 + dist[2];
+}
+
+// ---------------------------------------------------------------------------
+// Reductions.
+__device__ __forceinline__ double gs_wave_sum(double v, int width) {
+    for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, GS_WAVE);
+    return v;
+}
+
+__device__ __forceinline__ int gs_wave_sum_int(int v, int width) {
+    for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, GS_WAVE);
+    return v;
+}
+
+// Map a dispatch-order block id to a logical tile id so that consecutive
+// logical tiles land on the same XCD (blocks are dispatched round-robin
+// over the 8 XCDs of the MI355X).  A bijection on [0, nb).
+GS_HD __forceinline__ int gs_xcd_remap(int id, int nb) {
+    const int nx = 8;
+    int q = nb / nx, r = nb % nx;
+    int x = id % nx, slot = id / nx;
+    return x * q + (x < r ? x : r) + slot;
+}

@@ -1,0 +1,425 @@
+// HIP kernels of the grain-scoring path, written for gfx950 (wave64,
+// FP64 on the vector ALUs, 256 CUs in 8 XCDs).
+//
+// Data layout in HBM: g-vectors and lab positions are (n, 3) row-major
+// float64 exactly as the reference passes them (an array of 24-byte rows);
+// UBIs are (K, 3, 3) row-major; per-peak drlv2 (float64) and labels (int32)
+// are flat arrays; per-grain peak lists are CSR (int32 peak index, int64
+// offsets).  All-pairs kernels keep UBIs wave-uniform so they live in
+// scalar registers, hold a few peaks per lane in VGPRs, and count with a
+// wave ballot + popcount feeding LDS counters, one global atomic per
+// (block, grain) with a non-zero count.
+#pragma once
+#include "gs_device.h"
+
+#define GS_BLOCK 256
+#define GS_GRAIN_CHUNK 1024  // grains per LDS counter tile
+#define GS_NPART 17          // n, sum drlv2, R[9], H[6]
+
+// ---------------------------------------------------------------------------
+// small utility kernels (used instead of hipMemset so every launch is
+// graph-capture safe)
+__global__ void k_fill_i32(int* __restrict__ p, int v, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_f64(double* __restrict__ p, double v, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// verify_rounding on the device (src/closest.c:100-119): 600 integer-valued
+// points, MAGIC rounding vs floor(x + 0.5).
+__global__ void k_verify_rounding(int n, int* __restrict__ bad) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 600) return;
+    int base = t < 300 ? n : -n;
+    int i = (t % 300) - 100;
+    double v = base + i * 50.0;
+    int a = (int)gs_round(v);
+    int b = (int)floor(v + 0.5);
+    if (a != b) atomicAdd(bad, 1);
+}
+
+// ---------------------------------------------------------------------------
+// score for K UBIs x ng g-vectors -> counts[K] (counts must be zeroed).
+// Logical tile = (peak chunk of GS_BLOCK*P rows, grain split of gsplit
+// UBIs); the tiles of one chunk are adjacent after the XCD remap so the
+// chunk is fetched into one XCD's L2.
+template <int P>
+__global__ __launch_bounds__(GS_BLOCK) void k_score_many(const double* __restrict__ ubis, int K,
+                                                         int gsplit, int nsplit,
+                                                         const double* __restrict__ gv, int ng,
+                                                         double tolsq, int* __restrict__ counts) {
+    __shared__ int lcnt[GS_GRAIN_CHUNK];
+    const int tile = gs_xcd_remap(blockIdx.x, gridDim.x);
+    const int chunk = tile / nsplit;
+    const int split = tile - chunk * nsplit;
+    const int g0 = split * gsplit;
+    const int g1 = min(K, g0 + gsplit);
+    for (int i = threadIdx.x; i < g1 - g0; i += GS_BLOCK) lcnt[i] = 0;
+
+    double x[P], y[P], z[P];
+    const long long base = (long long)chunk * (GS_BLOCK * P) + threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        long long k = base + (long long)p * GS_BLOCK;
+        if (k < ng) {
+            x[p] = gv[3 * k + 0];
+            y[p] = gv[3 * k + 1];
+            z[p] = gv[3 * k + 2];
+        } else {
+            x[p] = y[p] = z[p] = __builtin_nan("");  // never counted
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (GS_WAVE - 1);
+    for (int g = g0; g < g1; g++) {
+        const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)g);
+        int c = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            double s = gs_drlv2(u, x[p], y[p], z[p]);
+            c += __popcll(__ballot(s < tolsq));
+        }
+        if (lane == 0 && c) atomicAdd(&lcnt[g - g0], c);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < g1 - g0; i += GS_BLOCK) {
+        int c = lcnt[i];
+        if (c) atomicAdd(&counts[g0 + i], c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// score_and_assign for K grains in order (src/closest.c:366-397 called once
+// per grain, as refinegrains.assignlabels and indexer.fight_over_peaks do).
+// Each lane owns P peaks and runs every grain in order on their (drlv2,
+// label) state, so the per-peak state machine is exactly the sequential
+// one.  counts[g] (zeroed by the caller) = the value the g-th call returns.
+template <int P>
+__global__ __launch_bounds__(GS_BLOCK) void k_assign_many(
+    const double* __restrict__ ubis, const int* __restrict__ glabels, int K,
+    const double* __restrict__ gv, int ng, double tolsq, double* __restrict__ drlv2,
+    int* __restrict__ labels, int* __restrict__ counts) {
+    __shared__ int lcnt[GS_GRAIN_CHUNK];
+    double x[P], y[P], z[P], d[P];
+    int lab[P];
+    const long long base = (long long)blockIdx.x * (GS_BLOCK * P) + threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        long long k = base + (long long)p * GS_BLOCK;
+        if (k < ng) {
+            x[p] = gv[3 * k + 0];
+            y[p] = gv[3 * k + 1];
+            z[p] = gv[3 * k + 2];
+            d[p] = drlv2[k];
+            lab[p] = labels[k];
+        } else {
+            x[p] = y[p] = z[p] = __builtin_nan("");
+            d[p] = 0.0;
+            lab[p] = -1;
+        }
+    }
+    const int lane = threadIdx.x & (GS_WAVE - 1);
+    for (int c0 = 0; c0 < K; c0 += GS_GRAIN_CHUNK) {
+        const int c1 = min(K, c0 + GS_GRAIN_CHUNK);
+        for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) lcnt[i] = 0;
+        __syncthreads();
+        for (int g = c0; g < c1; g++) {
+            const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)g);
+            const int L = glabels[g];
+            int c = 0;
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                double s = gs_drlv2(u, x[p], y[p], z[p]);
+                bool take = (s < tolsq) && (s < d[p]);
+                if (take) {
+                    lab[p] = L;
+                    d[p] = s;
+                } else if (lab[p] == L) {
+                    lab[p] = -1;
+                }
+                c += __popcll(__ballot(take));
+            }
+            if (lane == 0 && c) atomicAdd(&lcnt[g - c0], c);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) {
+            int c = lcnt[i];
+            if (c) atomicAdd(&counts[c0 + i], c);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        long long k = base + (long long)p * GS_BLOCK;
+        if (k < ng) {
+            drlv2[k] = d[p];
+            labels[k] = lab[p];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One-grain least-squares refinement over all ng peaks, two passes:
+// per-block partial sums (n, sum drlv2, R, H) then a one-wave finalize that
+// adds the block partials in block order (deterministic for a given grid),
+// solves UB = R.H^-1, UBI = UB^-1 in place and writes n and the mean drlv2.
+// MODE 0: peaks with drlv2 < tol^2 (score_and_refine, src/closest.c:267-345)
+// MODE 1: peaks with labels == label (refine_assigned, src/closest.c:416-474)
+template <int MODE>
+__global__ __launch_bounds__(GS_BLOCK) void k_refine_partials(
+    const double* __restrict__ ubi, const double* __restrict__ gv, int ng, double tolsq,
+    const int* __restrict__ labels, int label, double* __restrict__ partials) {
+    const gs_ubi u = gs_load_ubi(ubi);
+    double a[GS_NPART];
+#pragma unroll
+    for (int i = 0; i < GS_NPART; i++) a[i] = 0.0;
+    for (long long k = (long long)blockIdx.x * GS_BLOCK + threadIdx.x; k < ng;
+         k += (long long)gridDim.x * GS_BLOCK) {
+        double g0 = gv[3 * k + 0], g1 = gv[3 * k + 1], g2 = gv[3 * k + 2];
+        double i0, i1, i2;
+        double s = gs_drlv2(u, g0, g1, g2, i0, i1, i2);
+        bool keep = (MODE == 0) ? (s < tolsq) : (labels[k] == label);
+        if (keep) {
+            a[0] += 1.0;
+            a[1] += s;
+            a[2] += i0 * g0; a[3] += i1 * g0; a[4] += i2 * g0;   // R[0][j] += ih[j]*g[0]
+            a[5] += i0 * g1; a[6] += i1 * g1; a[7] += i2 * g1;
+            a[8] += i0 * g2; a[9] += i1 * g2; a[10] += i2 * g2;
+            a[11] += i0 * i0; a[12] += i1 * i0; a[13] += i2 * i0;  // H[0][0..2]
+            a[14] += i1 * i1; a[15] += i2 * i1; a[16] += i2 * i2;  // H[1][1..2], H[2][2]
+        }
+    }
+    __shared__ double red[GS_BLOCK / GS_WAVE][GS_NPART];
+    const int lane = threadIdx.x & (GS_WAVE - 1), wave = threadIdx.x / GS_WAVE;
+#pragma unroll
+    for (int i = 0; i < GS_NPART; i++) {
+        double v = gs_wave_sum(a[i], GS_WAVE);
+        if (lane == 0) red[wave][i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < GS_NPART) {
+        double v = red[0][threadIdx.x];
+        for (int w = 1; w < GS_BLOCK / GS_WAVE; w++) v += red[w][threadIdx.x];
+        partials[(size_t)blockIdx.x * GS_NPART + threadIdx.x] = v;
+    }
+}
+
+// R and H (full 3x3) from the 17 partial sums
+__device__ __forceinline__ void gs_unpack_RH(const double* t, double* R, double* H) {
+    for (int i = 0; i < 9; i++) R[i] = t[2 + i];
+    H[0] = t[11]; H[1] = t[12]; H[2] = t[13];
+    H[3] = t[12]; H[4] = t[14]; H[5] = t[15];
+    H[6] = t[13]; H[7] = t[15]; H[8] = t[16];
+}
+
+__global__ void k_refine_finalize(const double* __restrict__ partials, int nblocks,
+                                  double* __restrict__ ubi, int* __restrict__ n_out,
+                                  double* __restrict__ mean_out) {
+    __shared__ double tot[GS_NPART];
+    if (threadIdx.x < GS_NPART) {
+        double v = 0.0;
+        for (int b = 0; b < nblocks; b++) v += partials[(size_t)b * GS_NPART + threadIdx.x];
+        tot[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double R[9], H[9], u[9];
+        gs_unpack_RH(tot, R, H);
+        for (int i = 0; i < 9; i++) u[i] = ubi[i];
+        gs_solve_ubi(u, R, H);
+        for (int i = 0; i < 9; i++) ubi[i] = u[i];
+        int n = (int)tot[0];
+        double s = tot[1];
+        *n_out = n;
+        *mean_out = n > 0 ? s / n : s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched score_and_refine over CSR peak lists: a persistent grid whose
+// waves pull groups of 64/LPG grains from a work queue (grains visited in
+// `order`, longest list first), LPG lanes per grain.  Each grain runs
+// n_iter successive score_and_refine calls on its own list (the calls of
+// different grains are independent).  Sums are butterfly-reduced across
+// the LPG lanes, which leaves bit-identical totals in every lane, so each
+// lane solves the 3x3 system itself and keeps the updated UBI in registers.
+// n_out/mean_out are [n_iter][K]; queue[0] must be zero at launch.
+template <int LPG>
+__global__ __launch_bounds__(GS_BLOCK) void k_refine_lists(
+    double* __restrict__ ubis, int K, const double* __restrict__ gv,
+    const int* __restrict__ idx, const long long* __restrict__ off,
+    const int* __restrict__ order, double tolsq, int n_iter, int* __restrict__ n_out,
+    double* __restrict__ mean_out, int* __restrict__ queue) {
+    constexpr int GPW = GS_WAVE / LPG;
+    const int lane = threadIdx.x & (GS_WAVE - 1);
+    const int sub = lane / LPG;
+    const int sl = lane - sub * LPG;
+    for (;;) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(queue, GPW);
+        base = __shfl(base, 0, GS_WAVE);
+        if (base >= K) break;
+        const int slot = base + sub;
+        const bool active = slot < K;
+        const int g = active ? order[slot] : 0;
+        const long long o0 = active ? off[g] : 0;
+        const long long o1 = active ? off[g + 1] : 0;
+        double u[9];
+        for (int i = 0; i < 9; i++) u[i] = active ? ubis[9 * (size_t)g + i] : 0.0;
+        for (int it = 0; it < n_iter; it++) {
+            const gs_ubi U = {u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8]};
+            double a[GS_NPART];
+#pragma unroll
+            for (int i = 0; i < GS_NPART; i++) a[i] = 0.0;
+            for (long long q = o0 + sl; q < o1; q += LPG) {
+                const long long k = idx[q];
+                double g0 = gv[3 * k + 0], g1 = gv[3 * k + 1], g2 = gv[3 * k + 2];
+                double i0, i1, i2;
+                double s = gs_drlv2(U, g0, g1, g2, i0, i1, i2);
+                if (s < tolsq) {
+                    a[0] += 1.0;
+                    a[1] += s;
+                    a[2] += i0 * g0; a[3] += i1 * g0; a[4] += i2 * g0;
+                    a[5] += i0 * g1; a[6] += i1 * g1; a[7] += i2 * g1;
+                    a[8] += i0 * g2; a[9] += i1 * g2; a[10] += i2 * g2;
+                    a[11] += i0 * i0; a[12] += i1 * i0; a[13] += i2 * i0;
+                    a[14] += i1 * i1; a[15] += i2 * i1; a[16] += i2 * i2;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < GS_NPART; i++) a[i] = gs_wave_sum(a[i], LPG);
+            double R[9], H[9];
+            gs_unpack_RH(a, R, H);
+            gs_solve_ubi(u, R, H);
+            if (active && sl == 0) {
+                const int n = (int)a[0];
+                n_out[(size_t)it * K + g] = n;
+                mean_out[(size_t)it * K + g] = n > 0 ? a[1] / n : a[1];
+            }
+        }
+        if (active && sl == 0)
+            for (int i = 0; i < 9; i++) ubis[9 * (size_t)g + i] = u[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Geometry kernels.
+struct gs_detector {
+    double p[4];     // z_center, y_center, z_size, y_size
+    double r[9];     // detector rotation . flip, row-major
+    double dist[3];  // distance vector
+};
+
+struct gs_goniometer {
+    double omegasign, wvln, wedge, chi;
+    double t[3];    // grain translation
+    double mat[9];  // wedge/chi matrix, used only when mat_given
+    int mat_given;
+};
+
+// per-block wedge/chi matrix and 1/wavelength in LDS
+__device__ __forceinline__ void gs_block_setup(const gs_goniometer& gp, double* smat, double* sds) {
+    if (threadIdx.x == 0) {
+        if (gp.mat_given) {
+            for (int i = 0; i < 9; i++) smat[i] = gp.mat[i];
+        } else {
+            gs_wedge_chi_matrix(gp.wedge, gp.chi, smat);
+        }
+        *sds = 1. / gp.wvln;
+    }
+    __syncthreads();
+}
+
+// compute_xlylzl (src/cdiffraction.c:190-224)
+__global__ __launch_bounds__(GS_BLOCK) void k_xlylzl(const double* __restrict__ s,
+                                                     const double* __restrict__ f, gs_detector det,
+                                                     double* __restrict__ out, int n) {
+    const long long i = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double x0, x1, x2;
+    gs_xlylzl_one(s[i], f[i], det.p, det.r, det.dist, x0, x1, x2);
+    out[3 * i + 0] = x0;
+    out[3 * i + 1] = x1;
+    out[3 * i + 2] = x2;
+}
+
+// sin/cos of ((PI/180)*omega)*omegasign per peak
+__global__ __launch_bounds__(GS_BLOCK) void k_sincos_omega(const double* __restrict__ omega,
+                                                           double omegasign, double* __restrict__ so,
+                                                           double* __restrict__ co, int n) {
+    const long long i = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double a, b;
+    gs_sincos(gs_omega_rad(omega[i], omegasign), &a, &b);
+    so[i] = a;
+    co[i] = b;
+}
+
+// compute_gv (src/cdiffraction.c:112-168).  TRIG: sin/cos per peak are
+// read from so/co instead of being evaluated from omega.
+template <bool TRIG>
+__global__ __launch_bounds__(GS_BLOCK) void k_gv(const double* __restrict__ xl,
+                                                 const double* __restrict__ omega,
+                                                 const double* __restrict__ so_in,
+                                                 const double* __restrict__ co_in,
+                                                 gs_goniometer gp, double* __restrict__ gv, int n) {
+    __shared__ double smat[9];
+    __shared__ double sds;
+    gs_block_setup(gp, smat, &sds);
+    const long long i = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double so, co;
+    if (TRIG) {
+        so = so_in[i];
+        co = co_in[i];
+    } else {
+        gs_sincos(gs_omega_rad(omega[i], gp.omegasign), &so, &co);
+    }
+    double mat[9];
+    for (int j = 0; j < 9; j++) mat[j] = smat[j];
+    double g0, g1, g2;
+    gs_gv_one(xl[3 * i + 0], xl[3 * i + 1], xl[3 * i + 2], so, co, mat, sds, gp.t[0], gp.t[1],
+              gp.t[2], g0, g1, g2);
+    gv[3 * i + 0] = g0;
+    gv[3 * i + 1] = g1;
+    gv[3 * i + 2] = g2;
+}
+
+// compute_xlylzl -> compute_gv fused: raw (s, f, omega) to g-vectors in one
+// pass; the lab positions and per-peak sin/cos are optionally stored too.
+__global__ __launch_bounds__(GS_BLOCK) void k_sf2gv(
+    const double* __restrict__ s, const double* __restrict__ f, const double* __restrict__ omega,
+    gs_detector det, gs_goniometer gp, double* __restrict__ xl_out, double* __restrict__ so_out,
+    double* __restrict__ co_out, double* __restrict__ gv, int n) {
+    __shared__ double smat[9];
+    __shared__ double sds;
+    gs_block_setup(gp, smat, &sds);
+    const long long i = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double x0, x1, x2;
+    gs_xlylzl_one(s[i], f[i], det.p, det.r, det.dist, x0, x1, x2);
+    double so, co;
+    gs_sincos(gs_omega_rad(omega[i], gp.omegasign), &so, &co);
+    double mat[9];
+    for (int j = 0; j < 9; j++) mat[j] = smat[j];
+    double g0, g1, g2;
+    gs_gv_one(x0, x1, x2, so, co, mat, sds, gp.t[0], gp.t[1], gp.t[2], g0, g1, g2);
+    if (xl_out) {
+        xl_out[3 * i + 0] = x0;
+        xl_out[3 * i + 1] = x1;
+        xl_out[3 * i + 2] = x2;
+    }
+    if (so_out) {
+        so_out[i] = so;
+        co_out[i] = co;
+    }
+    gv[3 * i + 0] = g0;
+    gv[3 * i + 1] = g1;
+    gv[3 * i + 2] = g2;
+}

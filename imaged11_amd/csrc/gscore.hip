@@ -1,0 +1,581 @@
+// libgscore: MI355X (gfx950) grain-scoring kernels behind a C ABI.
+//
+// Two families of entry points (declared in include/gscore.h):
+//   * the reference's own C prototypes (score, score_and_refine, ...) on
+//     host pointers: inputs are staged to device memory, the HIP kernels
+//     run, outputs are copied back, the call returns when done;
+//   * gs_* entry points on device pointers, enqueued on a caller's stream
+//     (a hipStream_t passed as void*), with an int status.
+// Errors: gs_* return a non-zero status; the reference prototypes, which
+// have no status in the reference, record the error (gs_last_error) and
+// return -1 from the int-returning ones.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gs_kernels.h"
+#include "../../include/gscore.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define GS_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_err(GS_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));    \
+    } while (0)
+
+#define GS_LAUNCHED()                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = hipGetLastError();                                                 \
+        if (e_ != hipSuccess)                                                              \
+            return set_err(GS_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+#define GS_CHECK(code)              \
+    do {                            \
+        int c_ = (code);            \
+        if (c_ != GS_OK) return c_; \
+    } while (0)
+
+// Per-device state: a partial-sum buffer for the one-grain refinement and
+// a few int counters (work queues), allocated once per device.
+struct DeviceCtx {
+    bool ready = false;
+    int cus = 0;
+    double* partials = nullptr;  // GS_MAX_PARTIAL_BLOCKS * GS_NPART
+    int* counters = nullptr;     // GS_NCOUNTERS ints
+};
+
+constexpr int GS_MAX_PARTIAL_BLOCKS = 1024;
+constexpr int GS_NCOUNTERS = 4096;
+std::mutex g_ctx_mutex;
+std::vector<DeviceCtx> g_ctx;
+
+int get_ctx(DeviceCtx** out) {
+    int dev = 0, n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return set_err(GS_ERR_NODEVICE, "no HIP device available (%s)",
+                       e != hipSuccess ? hipGetErrorString(e) : "count is 0");
+    GS_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ctx_mutex);
+    if ((int)g_ctx.size() < n) g_ctx.resize(n);
+    DeviceCtx& c = g_ctx[dev];
+    if (!c.ready) {
+        GS_HIP(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
+        GS_HIP(hipMalloc(&c.partials, sizeof(double) * GS_MAX_PARTIAL_BLOCKS * GS_NPART));
+        GS_HIP(hipMalloc(&c.counters, sizeof(int) * GS_NCOUNTERS));
+        c.ready = true;
+    }
+    *out = &c;
+    return GS_OK;
+}
+
+hipStream_t S(gs_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+int blocks_for(long long n, int per_block) {
+    long long b = (n + per_block - 1) / per_block;
+    return (int)std::max<long long>(1, b);
+}
+
+int fill_i32(int* p, int v, long long n, hipStream_t st) {
+    if (n <= 0) return GS_OK;
+    int nb = (int)std::min<long long>(blocks_for(n, GS_BLOCK), 4096);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(GS_BLOCK), 0, st, p, v, n);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int fill_f64(double* p, double v, long long n, hipStream_t st) {
+    if (n <= 0) return GS_OK;
+    int nb = (int)std::min<long long>(blocks_for(n, GS_BLOCK), 4096);
+    hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(GS_BLOCK), 0, st, p, v, n);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+// Host-pointer staging: one growable device buffer per host thread.
+struct Staging {
+    int dev = -1;
+    char* buf = nullptr;
+    size_t cap = 0;
+};
+thread_local Staging g_stage;
+
+int stage_reserve(size_t bytes, char** out) {
+    int dev = 0;
+    GS_HIP(hipGetDevice(&dev));
+    if (g_stage.dev != dev) {  // buffers belong to one device
+        g_stage = Staging();
+        g_stage.dev = dev;
+    }
+    if (bytes > g_stage.cap) {
+        if (g_stage.buf) GS_HIP(hipFree(g_stage.buf));
+        g_stage.buf = nullptr;
+        size_t cap = std::max<size_t>(bytes, 1 << 20);
+        GS_HIP(hipMalloc(&g_stage.buf, cap));
+        g_stage.cap = cap;
+    }
+    *out = g_stage.buf;
+    return GS_OK;
+}
+
+// carve 256-byte aligned slices out of a staging buffer
+struct Carver {
+    char* base;
+    size_t used = 0;
+    explicit Carver(char* b) : base(b) {}
+    template <class T>
+    T* take(size_t count) {
+        T* p = reinterpret_cast<T*>(base + used);
+        used += (count * sizeof(T) + 255) & ~size_t(255);
+        return p;
+    }
+    static size_t bytes(size_t count, size_t elem) { return (count * elem + 255) & ~size_t(255); }
+};
+
+int check_ng(long long ng) {
+    if (ng < 0) return set_err(GS_ERR_ARG, "negative array length %lld", ng);
+    return GS_OK;
+}
+
+// ---- launchers shared by both families ------------------------------------
+
+int launch_score_many(const double* ubis, int K, const double* gv, int ng, double tol, int* counts,
+                      int peaks_per_lane, hipStream_t st) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (K < 0 || ng < 0) return set_err(GS_ERR_ARG, "negative size K=%d ng=%d", K, ng);
+    GS_CHECK(fill_i32(counts, 0, K, st));
+    if (K == 0 || ng == 0) return GS_OK;
+    const double tolsq = tol * tol;
+    int P = peaks_per_lane;
+    if (P != 1 && P != 2 && P != 4 && P != 8) P = 4;
+    const long long nchunks = (ng + (long long)GS_BLOCK * P - 1) / ((long long)GS_BLOCK * P);
+    // enough tiles to fill every CU several times over; a split holds at
+    // most GS_GRAIN_CHUNK grains (the LDS counter tile)
+    const long long want = 32LL * ctx->cus;
+    long long nsplit = std::max<long long>(1, (want + nchunks - 1) / nchunks);
+    nsplit = std::min<long long>(nsplit, std::max(1, K / 16));
+    nsplit = std::max<long long>(nsplit, (K + GS_GRAIN_CHUNK - 1) / GS_GRAIN_CHUNK);
+    const int gsplit = (int)((K + nsplit - 1) / nsplit);
+    nsplit = (K + gsplit - 1) / gsplit;
+    const long long nb = nchunks * nsplit;
+    if (nb > 0x7fffffffLL) return set_err(GS_ERR_ARG, "grid too large");
+    switch (P) {
+        case 1: hipLaunchKernelGGL(k_score_many<1>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+        case 2: hipLaunchKernelGGL(k_score_many<2>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+        case 8: hipLaunchKernelGGL(k_score_many<8>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+        default: hipLaunchKernelGGL(k_score_many<4>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+    }
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int launch_assign_many(const double* ubis, const int* glabels, int K, const double* gv, int ng,
+                       double tol, double* drlv2, int* labels, int* counts, hipStream_t st) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (K < 0 || ng < 0) return set_err(GS_ERR_ARG, "negative size K=%d ng=%d", K, ng);
+    GS_CHECK(fill_i32(counts, 0, K, st));
+    if (K == 0 || ng == 0) return GS_OK;
+    const double tolsq = tol * tol;
+    constexpr int P = 2;
+    const long long nb = (ng + (long long)GS_BLOCK * P - 1) / ((long long)GS_BLOCK * P);
+    hipLaunchKernelGGL(k_assign_many<P>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis, glabels,
+                       K, gv, ng, tolsq, drlv2, labels, counts);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int launch_refine_one(int mode, double* ubi, const double* gv, int ng, double tol,
+                      const int* labels, int label, int* n_out, double* mean_out, hipStream_t st) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(ng));
+    const int nb = std::min(GS_MAX_PARTIAL_BLOCKS, blocks_for(ng, GS_BLOCK * 4));
+    const double tolsq = tol * tol;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_refine_partials<0>, dim3(nb), dim3(GS_BLOCK), 0, st, ubi, gv, ng,
+                           tolsq, labels, label, ctx->partials);
+    else
+        hipLaunchKernelGGL(k_refine_partials<1>, dim3(nb), dim3(GS_BLOCK), 0, st, ubi, gv, ng,
+                           tolsq, labels, label, ctx->partials);
+    GS_LAUNCHED();
+    hipLaunchKernelGGL(k_refine_finalize, dim3(1), dim3(GS_WAVE), 0, st, ctx->partials, nb, ubi,
+                       n_out, mean_out);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+gs_detector make_detector(const double* p, const double* r, const double* dist) {
+    gs_detector d;
+    for (int i = 0; i < 4; i++) d.p[i] = p[i];
+    for (int i = 0; i < 9; i++) d.r[i] = r[i];
+    for (int i = 0; i < 3; i++) d.dist[i] = dist[i];
+    return d;
+}
+
+gs_goniometer make_gonio(double omegasign, double wvln, double wedge, double chi, const double* t,
+                         const double* mat) {
+    gs_goniometer g;
+    g.omegasign = omegasign;
+    g.wvln = wvln;
+    g.wedge = wedge;
+    g.chi = chi;
+    for (int i = 0; i < 3; i++) g.t[i] = t ? t[i] : 0.0;
+    g.mat_given = mat != nullptr;
+    for (int i = 0; i < 9; i++) g.mat[i] = mat ? mat[i] : 0.0;
+    return g;
+}
+
+int launch_xlylzl(const double* s, const double* f, const double* p, const double* r,
+                  const double* dist, double* xl, int n, hipStream_t st) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(n));
+    if (n == 0) return GS_OK;
+    hipLaunchKernelGGL(k_xlylzl, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, st, s, f,
+                       make_detector(p, r, dist), xl, n);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int launch_gv(const double* xl, const double* omega, const double* so, const double* co,
+              const gs_goniometer& gp, double* gv, int n, hipStream_t st) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(n));
+    if (n == 0) return GS_OK;
+    if (so)
+        hipLaunchKernelGGL(k_gv<true>, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, st, xl,
+                           omega, so, co, gp, gv, n);
+    else
+        hipLaunchKernelGGL(k_gv<false>, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, st, xl,
+                           omega, so, co, gp, gv, n);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* gs_last_error(void) { return g_err.c_str(); }
+
+int gs_version(void) { return GS_VERSION; }
+
+int gs_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int gs_init(void) {
+    DeviceCtx* ctx;
+    return get_ctx(&ctx);
+}
+
+// ---- reference prototypes on host pointers ---------------------------------
+
+int verify_rounding(int n) {
+    DeviceCtx* ctx;
+    if (get_ctx(&ctx) != GS_OK) return -1;
+    char* buf;
+    if (stage_reserve(256, &buf) != GS_OK) return -1;
+    int* bad = reinterpret_cast<int*>(buf);
+    hipStream_t st = nullptr;
+    if (fill_i32(bad, 0, 1, st) != GS_OK) return -1;
+    hipLaunchKernelGGL(k_verify_rounding, dim3(3), dim3(GS_BLOCK), 0, st, n, bad);
+    int h = -1;
+    if (hipGetLastError() != hipSuccess || hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "verify_rounding: HIP error");
+        return -1;
+    }
+    return h;
+}
+
+int score(double ubi[3][3], double gv[][3], double tol, int ng) {
+    if (check_ng(ng) != GS_OK) return -1;
+    char* buf;
+    size_t need = Carver::bytes(9, 8) + Carver::bytes((size_t)ng * 3, 8) + Carver::bytes(1, 4);
+    if (stage_reserve(need, &buf) != GS_OK) return -1;
+    Carver c(buf);
+    double* du = c.take<double>(9);
+    double* dg = c.take<double>((size_t)ng * 3);
+    int* dn = c.take<int>(1);
+    hipStream_t st = nullptr;
+    int h = -1;
+    if (hipMemcpyAsync(du, ubi, 72, hipMemcpyHostToDevice, st) != hipSuccess ||
+        (ng > 0 && hipMemcpyAsync(dg, gv, (size_t)ng * 24, hipMemcpyHostToDevice, st) != hipSuccess)) {
+        set_err(GS_ERR_HIP, "score: copy to device failed");
+        return -1;
+    }
+    if (launch_score_many(du, 1, dg, ng, tol, dn, 4, st) != GS_OK) return -1;
+    if (hipMemcpyAsync(&h, dn, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "score: copy back failed");
+        return -1;
+    }
+    return h;
+}
+
+static int refine_host(int mode, double* ubi, const double* gv, const int* labels, int label,
+                       double tol, int* n_out, double* mean_out, int ng) {
+    GS_CHECK(check_ng(ng));
+    char* buf;
+    size_t need = Carver::bytes(9, 8) + Carver::bytes((size_t)ng * 3, 8) + Carver::bytes(2, 8) +
+                  (mode ? Carver::bytes((size_t)ng, 4) : 0);
+    GS_CHECK(stage_reserve(need, &buf));
+    Carver c(buf);
+    double* du = c.take<double>(9);
+    double* dg = c.take<double>((size_t)ng * 3);
+    double* dm = c.take<double>(1);
+    int* dn = reinterpret_cast<int*>(c.take<double>(1));
+    int* dl = mode ? c.take<int>((size_t)ng) : nullptr;
+    hipStream_t st = nullptr;
+    GS_HIP(hipMemcpyAsync(du, ubi, 72, hipMemcpyHostToDevice, st));
+    if (ng > 0) GS_HIP(hipMemcpyAsync(dg, gv, (size_t)ng * 24, hipMemcpyHostToDevice, st));
+    if (mode && ng > 0) GS_HIP(hipMemcpyAsync(dl, labels, (size_t)ng * 4, hipMemcpyHostToDevice, st));
+    GS_CHECK(launch_refine_one(mode, du, dg, ng, tol, dl, label, dn, dm, st));
+    GS_HIP(hipMemcpyAsync(ubi, du, 72, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipMemcpyAsync(n_out, dn, 4, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipMemcpyAsync(mean_out, dm, 8, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipStreamSynchronize(st));
+    return GS_OK;
+}
+
+void score_and_refine(double ubi[3][3], double gv[][3], double tol, int* n_arg,
+                      double* sumdrlv2_arg, int ng) {
+    if (refine_host(0, &ubi[0][0], &gv[0][0], nullptr, 0, tol, n_arg, sumdrlv2_arg, ng) != GS_OK) {
+        *n_arg = -1;
+        *sumdrlv2_arg = NAN;
+    }
+}
+
+void refine_assigned(double ubi[3][3], double gv[][3], int* labels, int label, int* npk,
+                     double* sumdrlv2, int ng) {
+    if (refine_host(1, &ubi[0][0], &gv[0][0], labels, label, 0.0, npk, sumdrlv2, ng) != GS_OK) {
+        *npk = -1;
+        *sumdrlv2 = NAN;
+    }
+}
+
+int score_and_assign(double ubi[3][3], double gv[][3], double tol, double* drlv2, int* labels,
+                     int label, int ng) {
+    if (check_ng(ng) != GS_OK) return -1;
+    char* buf;
+    size_t need = Carver::bytes(9, 8) + Carver::bytes((size_t)ng * 3, 8) +
+                  Carver::bytes((size_t)ng, 8) + Carver::bytes((size_t)ng, 4) + Carver::bytes(2, 4);
+    if (stage_reserve(need, &buf) != GS_OK) return -1;
+    Carver c(buf);
+    double* du = c.take<double>(9);
+    double* dg = c.take<double>((size_t)ng * 3);
+    double* dd = c.take<double>((size_t)ng);
+    int* dl = c.take<int>((size_t)ng);
+    int* dn = c.take<int>(2);  // [0] count, [1] the label
+    hipStream_t st = nullptr;
+    int h = -1;
+    bool ok = hipMemcpyAsync(du, ubi, 72, hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(dn + 1, &label, 4, hipMemcpyHostToDevice, st) == hipSuccess;
+    if (ok && ng > 0)
+        ok = hipMemcpyAsync(dg, gv, (size_t)ng * 24, hipMemcpyHostToDevice, st) == hipSuccess &&
+             hipMemcpyAsync(dd, drlv2, (size_t)ng * 8, hipMemcpyHostToDevice, st) == hipSuccess &&
+             hipMemcpyAsync(dl, labels, (size_t)ng * 4, hipMemcpyHostToDevice, st) == hipSuccess;
+    if (!ok) {
+        set_err(GS_ERR_HIP, "score_and_assign: copy to device failed");
+        return -1;
+    }
+    if (launch_assign_many(du, dn + 1, 1, dg, ng, tol, dd, dl, dn, st) != GS_OK) return -1;
+    ok = hipMemcpyAsync(&h, dn, 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+    if (ok && ng > 0)
+        ok = hipMemcpyAsync(drlv2, dd, (size_t)ng * 8, hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipMemcpyAsync(labels, dl, (size_t)ng * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+    if (!ok || hipStreamSynchronize(st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "score_and_assign: copy back failed");
+        return -1;
+    }
+    return h;
+}
+
+void compute_gv(double xlylzl[][3], double omega[], double omegasign, double wvln, double wedge,
+                double chi, double t[3], double gv[][3], int n) {
+    if (check_ng(n) != GS_OK || n == 0) return;
+    char* buf;
+    size_t need = 2 * Carver::bytes((size_t)n * 3, 8) + Carver::bytes((size_t)n, 8);
+    if (stage_reserve(need, &buf) != GS_OK) return;
+    Carver c(buf);
+    double* dx = c.take<double>((size_t)n * 3);
+    double* dw = c.take<double>((size_t)n);
+    double* dg = c.take<double>((size_t)n * 3);
+    hipStream_t st = nullptr;
+    if (hipMemcpyAsync(dx, xlylzl, (size_t)n * 24, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dw, omega, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "compute_gv: copy to device failed");
+        return;
+    }
+    gs_goniometer gp = make_gonio(omegasign, wvln, wedge, chi, t, nullptr);
+    if (launch_gv(dx, dw, nullptr, nullptr, gp, dg, n, st) != GS_OK) return;
+    if (hipMemcpyAsync(gv, dg, (size_t)n * 24, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        set_err(GS_ERR_HIP, "compute_gv: copy back failed");
+}
+
+void compute_xlylzl(double s[], double f[], double p[4], double r[9], double dist[3],
+                    double xlylzl[][3], int n) {
+    if (check_ng(n) != GS_OK || n == 0) return;
+    char* buf;
+    size_t need = 2 * Carver::bytes((size_t)n, 8) + Carver::bytes((size_t)n * 3, 8);
+    if (stage_reserve(need, &buf) != GS_OK) return;
+    Carver c(buf);
+    double* ds = c.take<double>((size_t)n);
+    double* df = c.take<double>((size_t)n);
+    double* dx = c.take<double>((size_t)n * 3);
+    hipStream_t st = nullptr;
+    if (hipMemcpyAsync(ds, s, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(df, f, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "compute_xlylzl: copy to device failed");
+        return;
+    }
+    if (launch_xlylzl(ds, df, p, r, dist, dx, n, st) != GS_OK) return;
+    if (hipMemcpyAsync(xlylzl, dx, (size_t)n * 24, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        set_err(GS_ERR_HIP, "compute_xlylzl: copy back failed");
+}
+
+// ---- device-pointer entry points -------------------------------------------
+
+// The device sincos algorithm (gs_sincos) compiled for the host CPU: lets
+// the accuracy of the exact same code be checked without a GPU.  No entry
+// point of the library calls it.
+void gs_selftest_sincos_host(const double* x, double* s, double* c, int n) {
+    for (int i = 0; i < n; i++) gs_sincos(x[i], &s[i], &c[i]);
+}
+
+int gs_clear_error(void) {
+    g_err.clear();
+    return GS_OK;
+}
+
+int gs_score_many(const double* ubis, int K, const double* gv, int ng, double tol, int* counts,
+                  int peaks_per_lane, gs_stream_t stream) {
+    return launch_score_many(ubis, K, gv, ng, tol, counts, peaks_per_lane, S(stream));
+}
+
+int gs_assign_many(const double* ubis, const int* grain_labels, int K, const double* gv, int ng,
+                   double tol, double* drlv2, int* labels, int* counts, gs_stream_t stream) {
+    return launch_assign_many(ubis, grain_labels, K, gv, ng, tol, drlv2, labels, counts, S(stream));
+}
+
+int gs_score_and_refine_dev(double* ubi, const double* gv, int ng, double tol, int* n_out,
+                            double* mean_out, gs_stream_t stream) {
+    return launch_refine_one(0, ubi, gv, ng, tol, nullptr, 0, n_out, mean_out, S(stream));
+}
+
+int gs_refine_assigned_dev(double* ubi, const double* gv, int ng, const int* labels, int label,
+                           int* n_out, double* mean_out, gs_stream_t stream) {
+    return launch_refine_one(1, ubi, gv, ng, 0.0, labels, label, n_out, mean_out, S(stream));
+}
+
+int gs_compute_xlylzl_dev(const double* s, const double* f, const double p[4], const double r[9],
+                          const double dist[3], double* xlylzl, int n, gs_stream_t stream) {
+    return launch_xlylzl(s, f, p, r, dist, xlylzl, n, S(stream));
+}
+
+int gs_compute_gv_dev(const double* xlylzl, const double* omega, double omegasign, double wvln,
+                      double wedge, double chi, const double t[3], double* gv, int n,
+                      gs_stream_t stream) {
+    gs_goniometer gp = make_gonio(omegasign, wvln, wedge, chi, t, nullptr);
+    return launch_gv(xlylzl, omega, nullptr, nullptr, gp, gv, n, S(stream));
+}
+
+int gs_compute_gv_trig_dev(const double* xlylzl, const double* so, const double* co,
+                           const double mat[9], double wvln, const double t[3], double* gv, int n,
+                           gs_stream_t stream) {
+    if (!so || !co || !mat) return set_err(GS_ERR_ARG, "gs_compute_gv_trig_dev: null so/co/mat");
+    gs_goniometer gp = make_gonio(1.0, wvln, 0.0, 0.0, t, mat);
+    return launch_gv(xlylzl, nullptr, so, co, gp, gv, n, S(stream));
+}
+
+int gs_sincos_omega(const double* omega, double omegasign, double* so, double* co, int n,
+                    gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(n));
+    if (n == 0) return GS_OK;
+    hipLaunchKernelGGL(k_sincos_omega, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
+                       omega, omegasign, so, co, n);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_sf2gv(const double* s, const double* f, const double* omega, const double p[4],
+             const double r[9], const double dist[3], double omegasign, double wvln, double wedge,
+             double chi, const double t[3], double* xlylzl_out, double* so_out, double* co_out,
+             double* gv, int n, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(n));
+    if ((so_out == nullptr) != (co_out == nullptr))
+        return set_err(GS_ERR_ARG, "gs_sf2gv: so_out and co_out must both be given or both NULL");
+    if (n == 0) return GS_OK;
+    gs_goniometer gp = make_gonio(omegasign, wvln, wedge, chi, t, nullptr);
+    hipLaunchKernelGGL(k_sf2gv, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream), s, f,
+                       omega, make_detector(p, r, dist), gp, xlylzl_out, so_out, co_out, gv, n);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_refine_lists(double* ubis, int K, const double* gv, const int* idx, const long long* off,
+                    const int* order, double tol, int n_iter, int* n_out, double* mean_out,
+                    int lanes_per_grain, int waves_per_cu, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (K < 0 || n_iter < 0) return set_err(GS_ERR_ARG, "negative K or n_iter");
+    if (K == 0 || n_iter == 0) return GS_OK;
+    hipStream_t st = S(stream);
+    int* queue = ctx->counters;
+    GS_CHECK(fill_i32(queue, 0, 1, st));
+    const double tolsq = tol * tol;
+    int lpg = lanes_per_grain;
+    if (lpg != 8 && lpg != 16 && lpg != 32 && lpg != 64) lpg = 16;
+    int wpc = waves_per_cu > 0 ? waves_per_cu : 8;
+    const int gpw = GS_WAVE / lpg;
+    long long waves_needed = (K + gpw - 1) / gpw;
+    long long waves = std::min<long long>(waves_needed, (long long)wpc * ctx->cus);
+    int nb = (int)std::max<long long>(1, (waves + GS_BLOCK / GS_WAVE - 1) / (GS_BLOCK / GS_WAVE));
+    switch (lpg) {
+        case 8: hipLaunchKernelGGL(k_refine_lists<8>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        case 32: hipLaunchKernelGGL(k_refine_lists<32>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        case 64: hipLaunchKernelGGL(k_refine_lists<64>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        default: hipLaunchKernelGGL(k_refine_lists<16>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+    }
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+}  // extern "C"

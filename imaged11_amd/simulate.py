@@ -1,0 +1,254 @@
+"""Seeded synthetic inputs for the grain-scoring path.
+
+The reference publishes no benchmark inputs for these functions (its only
+timed simulation needs the external PolyXSim tool), so the workloads are
+generated here from fixed seeds:
+
+* ``lattice_gvectors`` — one grain's g-vectors plus random ones (the
+  one-grain ``score_and_refine`` case: a=4.05 A cubic, 12 000 lattice
+  g-vectors with N(0, 1e-3) noise + 8 000 uniform in a 1.6 A^-1 ball);
+* ``simulate_peaks`` — a forward model that places the diffraction spots of
+  many grains on a tilted 2048x2048 detector over an omega scan, with pixel
+  jitter and spurious peaks, returning the raw (sc, fc, omega) peak table
+  that compute_xlylzl -> compute_gv -> score consume;
+* ``refine_lists`` — uneven (lognormal) per-grain peak lists for the batched
+  refinement.
+
+The forward model inverts compute_gv (``src/cdiffraction.c:112-168``): for a
+sample-frame g it solves the Bragg condition for omega, forms the scattered
+beam direction, and intersects it with the detector plane of
+compute_xlylzl (``src/cdiffraction.c:190-224``).  Everything is numpy on the
+host; it only produces inputs.
+"""
+import math
+
+import numpy as np
+
+from . import geometry
+
+
+def random_rotations(n, rng):
+    """Uniformly distributed rotation matrices (unit quaternions)."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    w, x, y, z = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def ub_from_cell(U, a, b, c):
+    """UB for an orthogonal cell: g = U . diag(1/a, 1/b, 1/c) . hkl."""
+    return U * np.array([1.0 / a, 1.0 / b, 1.0 / c])[None, :]
+
+
+def lattice_gvectors(seed=1, a=4.05, n_lattice=12000, n_random=8000, gmax=1.6,
+                     hmax=8, noise=1e-3):
+    """One grain's UBI and a g-vector table (n_lattice + n_random, 3).
+
+    Lattice rows are UB.hkl for integer |h|,|k|,|l| <= hmax with
+    |g| <= gmax, cycled to n_lattice rows, each with N(0, noise) added per
+    component; random rows are uniform in the ball |g| <= gmax.  The rows
+    are shuffled.  Returns (ubi, gv, is_lattice).
+    """
+    rng = np.random.default_rng(seed)
+    U = random_rotations(1, rng)[0]
+    ub = ub_from_cell(U, a, a, a)
+    r = np.arange(-hmax, hmax + 1)
+    hkl = np.array(np.meshgrid(r, r, r, indexing="ij")).reshape(3, -1).T
+    hkl = hkl[np.any(hkl != 0, axis=1)]
+    g = hkl @ ub.T
+    g = g[np.linalg.norm(g, axis=1) <= gmax]
+    reps = int(math.ceil(n_lattice / len(g)))
+    g = np.tile(g, (reps, 1))[:n_lattice]
+    g = g + rng.normal(scale=noise, size=g.shape)
+    v = rng.normal(size=(n_random, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    v *= gmax * rng.random(n_random)[:, None] ** (1.0 / 3.0)
+    gv = np.concatenate([g, v])
+    lat = np.concatenate([np.ones(len(g), bool), np.zeros(n_random, bool)])
+    perm = rng.permutation(len(gv))
+    ubi = np.linalg.inv(ub)
+    return np.ascontiguousarray(ubi), np.ascontiguousarray(gv[perm]), lat[perm]
+
+
+def detector_parameters(rng, npx=2048, pixel=50.0, distance=150000.0, wavelength=0.2,
+                        tilt=0.01, wedge=0.0, chi=0.0):
+    """An ImageD11 parameter dict for a 2048^2 detector at 50 um, 150 mm,
+    tilts U[-tilt, tilt] rad, wavelength 0.2 A (lengths in microns)."""
+    return {
+        "y_center": npx / 2 + rng.uniform(-5, 5), "z_center": npx / 2 + rng.uniform(-5, 5),
+        "y_size": pixel, "z_size": pixel, "distance": distance, "wavelength": wavelength,
+        "omegasign": 1.0,
+        "tilt_x": rng.uniform(-tilt, tilt), "tilt_y": rng.uniform(-tilt, tilt),
+        "tilt_z": rng.uniform(-tilt, tilt),
+        "o11": 1.0, "o12": 0.0, "o21": 0.0, "o22": -1.0, "wedge": wedge, "chi": chi,
+    }
+
+
+def random_grains(K, rng, cubic=(3.0, 6.0), ortho=(3.0, 9.0)):
+    """K UB matrices: the first half cubic a~U[cubic], the rest orthorhombic
+    a,b,c~U[ortho].  Returns (ub[K,3,3], cells[K,3])."""
+    U = random_rotations(K, rng)
+    cells = np.empty((K, 3))
+    nc = K // 2
+    cells[:nc] = rng.uniform(*cubic, size=nc)[:, None]
+    cells[nc:] = rng.uniform(*ortho, size=(K - nc, 3))
+    ub = U / cells[:, None, :]
+    return ub, cells
+
+
+def _grain_spots(ub, t, pars, gmax, omega_range, omega_step):
+    """Spots of one grain: (sc, fc, omega) for every hkl within gmax whose
+    Bragg solution lands on the detector inside the omega range."""
+    p, r, dist = geometry.pack_xlylzl(pars)
+    sign, wvln, wedge, chi = geometry.pack_gv(pars)
+    mat = geometry.wedge_chi_matrix(wedge, chi).reshape(3, 3)
+    inv_len = np.linalg.norm(np.linalg.inv(ub), axis=1)  # cell edge lengths
+    hm = np.floor(gmax * inv_len).astype(int)
+    axes = [np.arange(-m, m + 1) for m in hm]
+    hkl = np.array(np.meshgrid(*axes, indexing="ij")).reshape(3, -1).T
+    hkl = hkl[np.any(hkl != 0, axis=1)]
+    g = hkl @ ub.T
+    gg = np.einsum("ij,ij->i", g, g)
+    keep = gg <= gmax * gmax
+    g, gg = g[keep], gg[keep]
+    # k_x(theta) = A cos(theta) + B sin(theta) + m02 g2 must equal -wvln |g|^2 / 2
+    A = mat[0, 0] * g[:, 0] + mat[0, 1] * g[:, 1]
+    B = mat[0, 1] * g[:, 0] - mat[0, 0] * g[:, 1]
+    C = -0.5 * wvln * gg - mat[0, 2] * g[:, 2]
+    R = np.hypot(A, B)
+    ok = (R > 0) & (np.abs(C) < R)
+    g, A, B, C, R = g[ok], A[ok], B[ok], C[ok], R[ok]
+    phi = np.arctan2(B, A)
+    acos = np.arccos(C / R)
+    sols = []
+    for sgn in (1.0, -1.0):
+        theta = phi + sgn * acos
+        om = np.degrees(theta) * sign
+        om = (om - omega_range[0]) % 360.0 + omega_range[0]
+        inr = (om >= omega_range[0]) & (om < omega_range[1])
+        if not np.any(inr):
+            continue
+        gi = g[inr]
+        th = np.radians(om[inr] * sign)
+        co, so = np.cos(th), np.sin(th)
+        w = np.stack([co * gi[:, 0] - so * gi[:, 1], so * gi[:, 0] + co * gi[:, 1], gi[:, 2]], 1)
+        k = w @ mat.T
+        nhat = wvln * k
+        nhat[:, 0] += 1.0
+        u = np.stack([co * t[0] - so * t[1], so * t[0] + co * t[1], np.full_like(co, t[2])], 1)
+        o = u @ mat.T
+        R3 = r.reshape(3, 3)
+        c1, c2 = R3[:, 1], R3[:, 2]
+        # o + s nhat = dist + v1 c1 + v2 c2  ->  [nhat, -c1, -c2] (s, v1, v2) = dist - o
+        M = np.empty((len(nhat), 3, 3))
+        M[:, :, 0] = nhat
+        M[:, :, 1] = -c1[None, :]
+        M[:, :, 2] = -c2[None, :]
+        rhs = dist[None, :] - o
+        sol = np.linalg.solve(M, rhs[:, :, None])[:, :, 0]
+        good = sol[:, 0] > 0
+        fc = sol[:, 1] / p[3] + p[1]
+        sc = sol[:, 2] / p[2] + p[0]
+        sols.append(np.stack([sc[good], fc[good], om[inr][good]], 1))
+    if not sols:
+        return np.zeros((0, 3))
+    return np.concatenate(sols)
+
+
+def simulate_peaks(ub, translations, pars, n_peaks, seed=2, spurious=0.10, gmax=1.6,
+                   omega_range=(0.0, 180.0), omega_step=0.25, jitter_px=0.5, npx=2048):
+    """Forward-simulate a raw peak table of exactly n_peaks rows.
+
+    Spots of every grain that land on the npx x npx detector are collected,
+    omega is binned to the centre of its omega_step frame, (sc, fc) get
+    N(0, jitter_px) jitter, the grain spots are subsampled to
+    (1 - spurious) * n_peaks and uniform spurious peaks fill the rest.
+    Returns dict(sc, fc, omega, grain) with grain = -1 for spurious rows;
+    rows are shuffled.
+    """
+    rng = np.random.default_rng(seed)
+    K = ub.shape[0]
+    rows, owner = [], []
+    for g in range(K):
+        s = _grain_spots(ub[g], translations[g], pars, gmax, omega_range, omega_step)
+        on = (s[:, 0] >= 0) & (s[:, 0] < npx) & (s[:, 1] >= 0) & (s[:, 1] < npx)
+        s = s[on]
+        rows.append(s)
+        owner.append(np.full(len(s), g, np.int32))
+    rows = np.concatenate(rows)
+    owner = np.concatenate(owner)
+    n_real = int(round(n_peaks * (1.0 - spurious)))
+    if len(rows) < n_real:
+        raise ValueError("simulation produced %d spots, %d needed" % (len(rows), n_real))
+    pick = rng.choice(len(rows), n_real, replace=False)
+    rows, owner = rows[pick], owner[pick]
+    rows[:, 0] += rng.normal(scale=jitter_px, size=n_real)
+    rows[:, 1] += rng.normal(scale=jitter_px, size=n_real)
+    nbin = np.floor((rows[:, 2] - omega_range[0]) / omega_step)
+    rows[:, 2] = omega_range[0] + (nbin + 0.5) * omega_step
+    n_sp = n_peaks - n_real
+    nfr = int(round((omega_range[1] - omega_range[0]) / omega_step))
+    sp = np.stack([rng.uniform(0, npx, n_sp), rng.uniform(0, npx, n_sp),
+                   omega_range[0] + (rng.integers(0, nfr, n_sp) + 0.5) * omega_step], 1)
+    allrows = np.concatenate([rows, sp])
+    allown = np.concatenate([owner, np.full(n_sp, -1, np.int32)])
+    perm = rng.permutation(n_peaks)
+    allrows, allown = allrows[perm], allown[perm]
+    return {
+        "sc": np.ascontiguousarray(allrows[:, 0]),
+        "fc": np.ascontiguousarray(allrows[:, 1]),
+        "omega": np.ascontiguousarray(allrows[:, 2]),
+        "grain": allown,
+    }
+
+
+def scoring_workload(K=1000, N=1_000_000, seed=2, spurious=0.10, translations=False):
+    """The many-grain scoring workload: K grains (half cubic a~U[3,6],
+    half orthorhombic a,b,c~U[3,9]), N raw peaks simulated on the detector,
+    10% spurious.  Returns dict(pars, ubi[K,3,3], ub, trans[K,3], peaks)."""
+    rng = np.random.default_rng(seed)
+    pars = detector_parameters(rng)
+    ub, cells = random_grains(K, rng)
+    if translations:
+        trans = rng.uniform(-200.0, 200.0, size=(K, 3))
+        trans[:, 2] *= 0.25
+    else:
+        trans = np.zeros((K, 3))
+    peaks = simulate_peaks(ub, trans, pars, N, seed=seed + 1000, spurious=spurious)
+    ubi = np.linalg.inv(ub)
+    return {"pars": pars, "ubi": np.ascontiguousarray(ubi), "ub": ub, "trans": trans,
+            "cells": cells, "peaks": peaks}
+
+
+def refine_lists(owner, K, seed=3, median=300, sigma=0.8, lo=20, hi=2000):
+    """Per-grain peak lists with lognormal lengths (median, sigma) clipped
+    to [lo, hi].  Each list holds the grain's own peaks first (up to its
+    length), then random other rows of the table.  Returns CSR (idx, off)."""
+    rng = np.random.default_rng(seed)
+    n = len(owner)
+    lengths = np.clip(np.round(rng.lognormal(math.log(median), sigma, K)), lo, hi).astype(np.int64)
+    order = np.argsort(owner, kind="stable")
+    starts = np.searchsorted(owner[order], np.arange(K))
+    ends = np.searchsorted(owner[order], np.arange(K), side="right")
+    idx = np.empty(int(lengths.sum()), np.int32)
+    off = np.zeros(K + 1, np.int64)
+    off[1:] = np.cumsum(lengths)
+    for g in range(K):
+        own = order[starts[g]:ends[g]]
+        m = int(lengths[g])
+        take = own[:m]
+        if len(take) < m:
+            extra = rng.integers(0, n, m - len(take))
+            take = np.concatenate([take, extra])
+        idx[off[g]:off[g + 1]] = rng.permutation(take)
+    return idx, off

@@ -1,0 +1,148 @@
+/*
+ * gscore.h — C ABI of libgscore, the MI355X (gfx950) grain-scoring path.
+ *
+ * Two families of entry points:
+ *
+ *  1. The reference's own prototypes, on host pointers, so that existing
+ *     bindings (the f2py signatures of src/_cImageD11.pyf:106-170,551-583,
+ *     or a ctypes/cffi stub) can load this library in place of the
+ *     ImageD11._cImageD11 extension.  Arrays are row-major float64 / int32
+ *     exactly as the reference expects; inputs are staged to the GPU, the
+ *     kernels run, outputs are copied back before the call returns.  The
+ *     reference has no error codes; on a HIP error these record a message
+ *     (gs_last_error) and return -1 (int results), set *n = -1 (refine), or
+ *     leave the output untouched (geometry).
+ *
+ *  2. gs_* entry points on device pointers, enqueued on a caller's stream
+ *     (a hipStream_t passed as void*, NULL = the null stream), returning a
+ *     gs_status.  Small parameter arrays (p, r, dist, t, mat) are host
+ *     pointers read at launch time.
+ *
+ * All sizes are C int as in the reference (n < 2^31).
+ */
+#ifndef GSCORE_H
+#define GSCORE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_VERSION 1
+
+typedef void *gs_stream_t;
+
+enum gs_status {
+    GS_OK = 0,
+    GS_ERR_ARG = 1,      /* invalid argument (sizes, null pointers)     */
+    GS_ERR_HIP = 2,      /* a HIP runtime call or kernel launch failed  */
+    GS_ERR_NODEVICE = 3  /* no HIP device is visible                    */
+};
+
+/* ---- library state -------------------------------------------------- */
+const char *gs_last_error(void);  /* message of the last failure on this thread */
+int gs_clear_error(void);
+int gs_version(void);
+int gs_device_count(void);
+int gs_init(void);               /* allocate the per-device state now */
+
+/* The device sin/cos algorithm evaluated on the host CPU, for checking its
+ * accuracy without a GPU; no other entry point calls it. */
+void gs_selftest_sincos_host(const double *x, double *s, double *c, int n);
+
+/* ---- 1. reference prototypes (host pointers) ------------------------ */
+
+/* src/closest.c:100-119 — MAGIC rounding vs floor(x+0.5) on 600 points,
+ * evaluated on the GPU; returns the number of mismatches (0). */
+int verify_rounding(int n);
+
+/* src/closest.c:227-248 — peaks with sum((h - round h)^2) < tol^2,
+ * h = ubi . gv[k]. */
+int score(double ubi[3][3], double gv[][3], double tol, int ng);
+
+/* src/closest.c:267-345 — score, then UB = R.H^-1 over the kept peaks,
+ * ubi := UB^-1 unless singular; *n = kept peaks, *sumdrlv2 = their mean
+ * drlv2 under the input ubi. */
+void score_and_refine(double ubi[3][3], double gv[][3], double tol, int *n_arg,
+                      double *sumdrlv2_arg, int ng);
+
+/* src/closest.c:366-397 — peaks fitting better than drlv2[k] take
+ * `label`; a peak that had `label` and no longer fits gets -1. */
+int score_and_assign(double ubi[3][3], double gv[][3], double tol, double *drlv2,
+                     int *labels, int label, int ng);
+
+/* src/closest.c:416-474 — refine on the peaks with labels[k] == label. */
+void refine_assigned(double ubi[3][3], double gv[][3], int *labels, int label, int *npk,
+                     double *sumdrlv2, int ng);
+
+/* src/cdiffraction.c:112-168 — g-vectors from lab positions and omega. */
+void compute_gv(double xlylzl[][3], double omega[], double omegasign, double wvln,
+                double wedge, double chi, double t[3], double gv[][3], int n);
+
+/* src/cdiffraction.c:190-224 — lab positions from detector (s, f). */
+void compute_xlylzl(double s[], double f[], double p[4], double r[9], double dist[3],
+                    double xlylzl[][3], int n);
+
+/* ---- 2. device-pointer entry points --------------------------------- */
+
+/* score for K UBIs (K,3,3) against ng g-vectors -> counts[K] (int32);
+ * the loop indexer.score runs over trial UBIs (ImageD11/indexing.py:1090).
+ * peaks_per_lane: 1, 2, 4 or 8 (tile height; 0 = default). */
+int gs_score_many(const double *ubis, int K, const double *gv, int ng, double tol,
+                  int *counts, int peaks_per_lane, gs_stream_t stream);
+
+/* score_and_assign for K grains in order, label grain_labels[g] for grain
+ * g, on shared drlv2/labels (ImageD11/refinegrains.py:718-734,
+ * ImageD11/indexing.py:885-889); counts[g] = value of the g-th call. */
+int gs_assign_many(const double *ubis, const int *grain_labels, int K, const double *gv,
+                   int ng, double tol, double *drlv2, int *labels, int *counts,
+                   gs_stream_t stream);
+
+/* one-grain score_and_refine / refine_assigned on device memory; ubi (9
+ * doubles) is updated in place, n_out/mean_out are device scalars. */
+int gs_score_and_refine_dev(double *ubi, const double *gv, int ng, double tol, int *n_out,
+                            double *mean_out, gs_stream_t stream);
+int gs_refine_assigned_dev(double *ubi, const double *gv, int ng, const int *labels,
+                           int label, int *n_out, double *mean_out, gs_stream_t stream);
+
+/* compute_xlylzl / compute_gv on device memory. */
+int gs_compute_xlylzl_dev(const double *s, const double *f, const double p[4],
+                          const double r[9], const double dist[3], double *xlylzl, int n,
+                          gs_stream_t stream);
+int gs_compute_gv_dev(const double *xlylzl, const double *omega, double omegasign,
+                      double wvln, double wedge, double chi, const double t[3], double *gv,
+                      int n, gs_stream_t stream);
+
+/* compute_gv with per-peak sin/cos of omega and the wedge/chi matrix given
+ * (the arithmetic of src/cdiffraction.c:149-166 after the libm calls). */
+int gs_compute_gv_trig_dev(const double *xlylzl, const double *so, const double *co,
+                           const double mat[9], double wvln, const double t[3], double *gv,
+                           int n, gs_stream_t stream);
+
+/* sin and cos of ((PI/180)*omega)*omegasign per peak. */
+int gs_sincos_omega(const double *omega, double omegasign, double *so, double *co, int n,
+                    gs_stream_t stream);
+
+/* compute_xlylzl -> compute_gv fused (ImageD11/transform.py:729-738,
+ * Ctransform.sf2gv); xlylzl_out and so_out/co_out may be NULL. */
+int gs_sf2gv(const double *s, const double *f, const double *omega, const double p[4],
+             const double r[9], const double dist[3], double omegasign, double wvln,
+             double wedge, double chi, const double t[3], double *xlylzl_out, double *so_out,
+             double *co_out, double *gv, int n, gs_stream_t stream);
+
+/* Batched score_and_refine: grain g refines on the peaks gv[idx[off[g]]]
+ * .. gv[idx[off[g+1]-1]], n_iter successive calls (refinegrains.refine,
+ * ImageD11/refinegrains.py:459-498, makes two).  ubis (K,3,3) updated in
+ * place; n_out/mean_out are [n_iter][K].  `order` (K ints) is the order in
+ * which the persistent waves take grains (longest list first balances
+ * best).  lanes_per_grain 8/16/32/64 and waves_per_cu tune the launch
+ * (0 = defaults). */
+int gs_refine_lists(double *ubis, int K, const double *gv, const int *idx,
+                    const long long *off, const int *order, double tol, int n_iter,
+                    int *n_out, double *mean_out, int lanes_per_grain, int waves_per_cu,
+                    gs_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSCORE_H */

@@ -1,0 +1,220 @@
+"""Write the golden vectors under tests/golden/ from the reference C.
+
+The reference's own C (src/closest.c, src/cdiffraction.c) is compiled from
+its sources into oracle/_ref/libref.so by oracle/Makefile; this script feeds
+it seeded inputs and stores inputs + outputs as .npz fixtures.  It also
+extracts the ken_simul test data (simAl.gve g-vectors, test.ubi matrices,
+test.idx peak counts) that the reference's own tests hold, as plain arrays.
+
+    make -C oracle && python tests/golden/make_golden.py [REFERENCE_ROOT]
+
+Nothing here is imported by the product; the tests read the .npz files.
+"""
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from imaged11_amd import geometry, simulate  # noqa: E402
+from oracle import oracle  # noqa: E402
+
+REFROOT = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+
+
+def ref_lib():
+    r = oracle.load_reference()
+    if r is None:
+        raise SystemExit("oracle/_ref/libref.so missing: run `make -C oracle` first")
+    return r
+
+
+def scoring_cases(ref):
+    """score / score_and_refine / score_and_assign / refine_assigned."""
+    out = {}
+    ubi, gv, lat = simulate.lattice_gvectors(seed=1, n_lattice=1500, n_random=1000)
+    rng = np.random.default_rng(101)
+    # a few more UBIs: perturbed copies and unrelated ones
+    ubis = [ubi]
+    for s in (1e-3, 1e-2):
+        ubis.append(ubi + rng.normal(scale=s, size=(3, 3)))
+    for _ in range(3):
+        U = simulate.random_rotations(1, rng)[0]
+        ubis.append(np.linalg.inv(simulate.ub_from_cell(U, *rng.uniform(3, 9, 3))))
+    ubis = np.ascontiguousarray(np.array(ubis))
+    tols = np.array([0.01, 0.05, 0.2, 0.5])
+    out["gv"] = gv
+    out["ubis"] = ubis
+    out["tols"] = tols
+    out["score"] = np.array([[ref.score(u, gv, t) for t in tols] for u in ubis], np.int32)
+    # three successive score_and_refine calls per (ubi, tol)
+    sar_n = np.zeros((len(ubis), len(tols), 3), np.int32)
+    sar_m = np.zeros((len(ubis), len(tols), 3))
+    sar_u = np.zeros((len(ubis), len(tols), 3, 3, 3))
+    for i, u0 in enumerate(ubis):
+        for j, t in enumerate(tols):
+            u = u0.copy()
+            for k in range(3):
+                sar_n[i, j, k], sar_m[i, j, k] = ref.score_and_refine(u, gv, t)
+                sar_u[i, j, k] = u
+    out["sar_n"], out["sar_mean"], out["sar_ubi"] = sar_n, sar_m, sar_u
+    # score_and_assign sequences under the three caller conventions
+    for name, d0, l0 in (("rg", 1.0, -1), ("nb", 1.0, 0), ("fop", 2.0, -1)):
+        for tname, t in (("t05", 0.05), ("t2", 0.2)):
+            d = np.full(len(gv), d0)
+            lab = np.full(len(gv), l0, np.int32)
+            ns = []
+            for g, u in enumerate(ubis):
+                ns.append(ref.score_and_assign(u, gv, t, d, lab, g))
+            # a second pass re-assigning grain 2 exercises the "-1" branch
+            ns.append(ref.score_and_assign(ubis[3], gv, t, d, lab, 2))
+            key = "saa_%s_%s" % (name, tname)
+            out[key + "_n"] = np.array(ns, np.int32)
+            out[key + "_drlv2"] = d
+            out[key + "_labels"] = lab
+    # refine_assigned on the refinegrains-convention labels
+    d = out["saa_rg_t2_labels"]
+    ra_n = np.zeros(len(ubis), np.int32)
+    ra_m = np.zeros(len(ubis))
+    ra_u = np.zeros((len(ubis), 3, 3))
+    for g, u0 in enumerate(ubis):
+        u = u0.copy()
+        ra_n[g], ra_m[g] = ref.refine_assigned(u, gv, d, g)
+        ra_u[g] = u
+    out["ra_n"], out["ra_mean"], out["ra_ubi"] = ra_n, ra_m, ra_u
+    out["verify_rounding"] = np.array([ref.verify_rounding(n) for n in (0, 1, 20, 12345)], np.int32)
+    return out
+
+
+def geometry_cases(ref):
+    """compute_xlylzl over flips x tilts, compute_gv over the parameter sets
+    of the reference's test/test_cImageD11.py (wedge, chi, sign, t)."""
+    rng = np.random.default_rng(202)
+    out = {}
+    npk = 48
+    sc = rng.random(npk) * 2048
+    fc = rng.random(npk) * 2048
+    p = np.array([980.0, 1010.0, 48.0, 49.0])
+    dist = np.array([100100.0, 0.0, 0.0])
+    flips = [[1, 0, 0, 1], [-1, 0, 0, 1], [-1, 0, 0, -1], [1, 0, 0, -1],
+             [0, 1, 1, 0], [0, -1, 1, 0], [0, -1, -1, 0], [0, 1, -1, 0]]
+    tilts = [(x, y, z) for x in (-0.5, 0.0, 0.24) for y in (-0.5, 0.0, 0.24)
+             for z in (-0.5, 0.0, 0.24)]
+    rs, xls = [], []
+    for o11, o12, o21, o22 in flips:
+        for tx, ty, tz in tilts:
+            r = np.dot(geometry.detector_rotation_matrix(tx, ty, tz),
+                       geometry.flip_matrix(o11, o12, o21, o22)).ravel()
+            xl = np.zeros((npk, 3))
+            ref.compute_xlylzl(sc, fc, p, r, dist, xl)
+            rs.append(r)
+            xls.append(xl)
+    out.update(xl_sc=sc, xl_fc=fc, xl_p=p, xl_dist=dist, xl_r=np.array(rs), xl_out=np.array(xls))
+    N = 203
+    xyz = np.ascontiguousarray(np.array([np.linspace(0, 2000, N), np.linspace(10, 2010, N),
+                                         np.ones(N)]).T)
+    xyz_rand = np.ascontiguousarray(np.stack([rng.uniform(1e5, 2e5, N), rng.uniform(-5e4, 5e4, N),
+                                              rng.uniform(-5e4, 5e4, N)], 1))
+    omega = np.linspace(-180, 180, N)
+    omega_rand = rng.uniform(-400, 400, N)
+    sets = [(5., 10., 1.0, 0.125, 20., 30., 40.), (5., 0., 1.0, 0.125, 20., 30., 40.),
+            (0., 5., 1.0, 0.125, 20., 30., 40.), (5., 10., -1.0, 0.125, 200., 300., 40.),
+            (5., 0., 1.0, 0.125, 200., 300., -400.), (0., 0., 1.0, 0.125, 0., 0., 0.),
+            (0., 0., 1.0, 0.125, 200., 300., 400.), (-3.3, 7.1, -1.0, 0.2, -55., 12., 3.)]
+    gvs = []
+    for wedge, chi, sign, wvln, tx, ty, tz in sets:
+        for x, w in ((xyz, omega), (xyz_rand, omega_rand)):
+            g = np.zeros((N, 3))
+            ref.compute_gv(x, w, sign, wvln, wedge, chi, np.array([tx, ty, tz]), g)
+            gvs.append(g)
+    out.update(gv_xyz=xyz, gv_xyz_rand=xyz_rand, gv_omega=omega, gv_omega_rand=omega_rand,
+               gv_sets=np.array(sets), gv_out=np.array(gvs))
+    return out
+
+
+def workload_case(ref):
+    """A small raw-peak scoring workload through the reference's three calls."""
+    wl = simulate.scoring_workload(K=24, N=6000, seed=7, translations=True)
+    pk, pars = wl["peaks"], wl["pars"]
+    p, r, dist = geometry.pack_xlylzl(pars)
+    sign, wvln, wedge, chi = geometry.pack_gv(pars)
+    n = len(pk["sc"])
+    xl = np.zeros((n, 3))
+    ref.compute_xlylzl(pk["sc"], pk["fc"], p, r, dist, xl)
+    gv0 = np.zeros((n, 3))
+    ref.compute_gv(xl, pk["omega"], sign, wvln, wedge, chi, np.zeros(3), gv0)
+    counts = np.array([ref.score(u, gv0, 0.05) for u in wl["ubi"]], np.int32)
+    # assignlabels loop with per-grain translations (refinegrains.py:718-734)
+    d = np.ones(n)
+    lab = np.full(n, -1, np.int32)
+    an = []
+    g = np.zeros((n, 3))
+    for k, u in enumerate(wl["ubi"]):
+        ref.compute_gv(xl, pk["omega"], sign, wvln, wedge, chi, wl["trans"][k], g)
+        an.append(ref.score_and_assign(u, g, 0.05, d, lab, k))
+    return dict(sc=pk["sc"], fc=pk["fc"], omega=pk["omega"], ubis=wl["ubi"], trans=wl["trans"],
+                p=p, r=r, dist=dist, gonio=np.array([sign, wvln, wedge, chi]), xl=xl, gv=gv0,
+                counts=counts, assign_n=np.array(an, np.int32), assign_drlv2=d, assign_labels=lab)
+
+
+def ken_simul(ref):
+    """simAl.gve + test.ubi + test.idx from the reference's test/ken_simul."""
+    base = os.path.join(REFROOT, "test", "ken_simul")
+    rows = []
+    with open(os.path.join(base, "simAl.gve")) as f:
+        started = False
+        for line in f:
+            if line.startswith("#"):
+                if "xr yr zr" in line:
+                    started = True
+                continue
+            if started and line.strip():
+                rows.append([float(v) for v in line.split()[:3]])
+    gv = np.ascontiguousarray(np.array(rows))
+    ubis = []
+    with open(os.path.join(base, "test.ubi")) as f:
+        block = []
+        for line in f:
+            v = line.split()
+            if len(v) == 3:
+                block.append([float(x) for x in v])
+                if len(block) == 3:
+                    ubis.append(block)
+                    block = []
+    ubis = np.ascontiguousarray(np.array(ubis))
+    npeaks = []
+    with open(os.path.join(base, "test.idx")) as f:
+        for line in f:
+            m = re.match(r"Grain:\s*(\d+)\s+Npeaks=(\d+)", line)
+            if m:
+                npeaks.append(int(m.group(2)))
+    # fight_over_peaks as indexing.py:876-889 does it, with hkl_tol 0.01
+    d = np.full(len(gv), 2.0)
+    lab = np.full(len(gv), -1, np.int32)
+    ns = [ref.score_and_assign(u, gv, 0.01, d, lab, i) for i, u in enumerate(ubis)]
+    ra = []
+    for i, u in enumerate(ubis):
+        uu = u.copy()
+        ra.append(ref.refine_assigned(uu, gv, lab, i))
+    return dict(gv=gv, ubis=ubis, idx_npeaks=np.array(npeaks, np.int32),
+                fop_n=np.array(ns, np.int32), fop_drlv2=d, fop_labels=lab,
+                ra_n=np.array([a for a, _ in ra], np.int32), ra_mean=np.array([b for _, b in ra]))
+
+
+def main():
+    ref = ref_lib()
+    np.savez_compressed(os.path.join(HERE, "scoring_cases.npz"), **scoring_cases(ref))
+    np.savez_compressed(os.path.join(HERE, "geometry_cases.npz"), **geometry_cases(ref))
+    np.savez_compressed(os.path.join(HERE, "workload_small.npz"), **workload_case(ref))
+    np.savez_compressed(os.path.join(HERE, "ken_simul.npz"), **ken_simul(ref))
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

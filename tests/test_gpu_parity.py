@@ -1,0 +1,310 @@
+"""GPU parity: the HIP kernels, through the C ABI (drop-in host pointers and
+device-pointer entry points), against the oracle and the golden vectors
+written by the reference C.
+
+Bars: counts, labels, drlv2 of score_and_assign and everything of
+compute_xlylzl bit-exact; score_and_refine n exact, mean drlv2 within
+1e-12 relative, refined UBI within 1e-10 (summation order of R differs);
+compute_gv bit-exact given the same sin/cos, and within 2 ulp of the
+reference with the device's correctly rounded sincos (glibc's sincos, which
+the reference calls, is not correctly rounded for ~0.16% of arguments)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from imaged11_amd import _lib, batched, geometry, simulate
+from imaged11_amd import cImageD11 as gpu
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+MEAN_RTOL = 1e-12
+UBI_TOL = 1e-10
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def T(a, dev, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+
+
+def ulps(a, b):
+    a = np.ascontiguousarray(a, np.float64).view(np.int64)
+    b = np.ascontiguousarray(b, np.float64).view(np.int64)
+    a = np.where(a < 0, np.int64(-0x8000000000000000) - a, a)
+    b = np.where(b < 0, np.int64(-0x8000000000000000) - b, b)
+    return np.abs(a - b)
+
+
+def test_library_loaded_native(dev):
+    lib = _lib.load()
+    assert lib.gs_device_count() >= 1
+    assert len(_lib.hip_runtimes_loaded()) == 1
+    assert lib.gs_init() == 0
+
+
+def test_verify_rounding(dev):
+    for n in (0, 1, 20, 12345):
+        assert gpu.verify_rounding(n) == 0
+
+
+@pytest.mark.parametrize("mode", ["host", "device"])
+def test_score_golden(dev, mode):
+    s = load("scoring_cases.npz")
+    gv = s["gv"] if mode == "host" else T(s["gv"], dev)
+    for i, u in enumerate(s["ubis"]):
+        for j, t in enumerate(s["tols"]):
+            assert gpu.score(u, gv, t) == s["score"][i, j]
+
+
+@pytest.mark.parametrize("mode", ["host", "device"])
+def test_score_and_refine_golden(dev, mode):
+    s = load("scoring_cases.npz")
+    gv = s["gv"] if mode == "host" else T(s["gv"], dev)
+    for i, u0 in enumerate(s["ubis"]):
+        for j, t in enumerate(s["tols"]):
+            u = u0.copy()
+            for k in range(3):
+                n, m = gpu.score_and_refine(u, gv, t)
+                assert n == s["sar_n"][i, j, k]
+                want_m = s["sar_mean"][i, j, k]
+                assert abs(m - want_m) <= MEAN_RTOL * abs(want_m)
+                np.testing.assert_allclose(u, s["sar_ubi"][i, j, k], rtol=0, atol=UBI_TOL)
+
+
+@pytest.mark.parametrize("mode", ["host", "device"])
+@pytest.mark.parametrize("conv,d0,l0", [("rg", 1.0, -1), ("nb", 1.0, 0), ("fop", 2.0, -1)])
+@pytest.mark.parametrize("tname,tol", [("t05", 0.05), ("t2", 0.2)])
+def test_score_and_assign_golden(dev, mode, conv, d0, l0, tname, tol):
+    s = load("scoring_cases.npz")
+    gv, ubis = s["gv"], s["ubis"]
+    d = np.full(len(gv), d0)
+    lab = np.full(len(gv), l0, np.int32)
+    if mode == "device":
+        gv, d, lab = T(gv, dev), T(d, dev), T(lab, dev, torch.int32)
+    ns = [gpu.score_and_assign(u, gv, tol, d, lab, g) for g, u in enumerate(ubis)]
+    ns.append(gpu.score_and_assign(ubis[3], gv, tol, d, lab, 2))
+    key = "saa_%s_%s" % (conv, tname)
+    if mode == "device":
+        d, lab = d.cpu().numpy(), lab.cpu().numpy()
+    assert ns == s[key + "_n"].tolist()
+    assert np.array_equal(d.view(np.int64), s[key + "_drlv2"].view(np.int64))
+    assert np.array_equal(lab, s[key + "_labels"])
+
+
+@pytest.mark.parametrize("mode", ["host", "device"])
+def test_refine_assigned_golden(dev, mode):
+    s = load("scoring_cases.npz")
+    lab = s["saa_rg_t2_labels"]
+    gv = s["gv"] if mode == "host" else T(s["gv"], dev)
+    for g, u0 in enumerate(s["ubis"]):
+        u = u0.copy()
+        n, m = gpu.refine_assigned(u, gv, lab, g)
+        assert n == s["ra_n"][g]
+        assert abs(m - s["ra_mean"][g]) <= MEAN_RTOL * abs(s["ra_mean"][g])
+        np.testing.assert_allclose(u, s["ra_ubi"][g], rtol=0, atol=UBI_TOL)
+
+
+def test_refine_assigned_readonly_ubi(dev):
+    """ubi that is not C-contiguous float64 is copied, not overwritten."""
+    s = load("scoring_cases.npz")
+    u = np.asfortranarray(s["ubis"][0])
+    before = u.copy()
+    gpu.refine_assigned(u, s["gv"], s["saa_rg_t2_labels"], 0)
+    assert np.array_equal(u, before)
+
+
+@pytest.mark.parametrize("mode", ["host", "device"])
+def test_compute_xlylzl_golden(dev, mode):
+    s = load("geometry_cases.npz")
+    for r, want in zip(s["xl_r"], s["xl_out"]):
+        if mode == "host":
+            out = np.zeros_like(want)
+            gpu.compute_xlylzl(s["xl_sc"], s["xl_fc"], s["xl_p"], r, s["xl_dist"], out)
+        else:
+            o = torch.zeros(want.shape, dtype=torch.float64, device=dev)
+            gpu.compute_xlylzl(T(s["xl_sc"], dev), T(s["xl_fc"], dev), s["xl_p"], r, s["xl_dist"], o)
+            out = o.cpu().numpy()
+        assert np.array_equal(out, want)
+
+
+def test_compute_gv_bitexact_given_trig(dev, orc):
+    """Everything of compute_gv after the libm calls (products, sums, the
+    correctly rounded division and sqrt) is bit-identical on the device."""
+    s = load("geometry_cases.npz")
+    outs = iter(s["gv_out"])
+    for wedge, chi, sign, wvln, tx, ty, tz in s["gv_sets"]:
+        for x, w in ((s["gv_xyz"], s["gv_omega"]), (s["gv_xyz_rand"], s["gv_omega_rand"])):
+            so, co = orc.sincos_omega(w, sign)          # glibc sincos, as the reference
+            mat = orc.wedge_chi_matrix(wedge, chi)
+            got = batched.gvectors_trig(T(x, dev), T(so, dev), T(co, dev), mat, wvln,
+                                        (tx, ty, tz)).cpu().numpy()
+            assert np.array_equal(got, next(outs))
+
+
+def test_sincos_device_equals_host_build(dev):
+    """The device sincos and the same code compiled for the host agree bit
+    for bit (the host build is checked against 113-bit sinq/cosq on CPU)."""
+    rng = np.random.default_rng(0)
+    omega = np.concatenate([rng.uniform(-1000, 1000, 200000), (np.arange(1440) + 0.5) * 0.25,
+                            np.arange(-720, 721, 1.0), [0.0, -0.0, 1e-300, 90.0, 180.0]])
+    so, co = batched.sincos_omega(T(omega, dev), 1.0)
+    x = np.pi / 180.0 * omega * 1.0
+    hs, hc = np.zeros_like(x), np.zeros_like(x)
+    _lib.load().gs_selftest_sincos_host(_lib.hptr(x), _lib.hptr(hs), _lib.hptr(hc), len(x))
+    assert np.array_equal(so.cpu().numpy().view(np.int64), hs.view(np.int64))
+    assert np.array_equal(co.cpu().numpy().view(np.int64), hc.view(np.int64))
+
+
+@pytest.mark.parametrize("mode", ["host", "device"])
+def test_compute_gv_golden_ulps(dev, mode):
+    s = load("geometry_cases.npz")
+    outs = iter(s["gv_out"])
+    frac = []
+    for wedge, chi, sign, wvln, tx, ty, tz in s["gv_sets"]:
+        for x, w in ((s["gv_xyz"], s["gv_omega"]), (s["gv_xyz_rand"], s["gv_omega_rand"])):
+            want = next(outs)
+            if mode == "host":
+                g = np.zeros_like(want)
+                gpu.compute_gv(x, w, sign, wvln, wedge, chi, (tx, ty, tz), g)
+            else:
+                gt = torch.zeros(want.shape, dtype=torch.float64, device=dev)
+                gpu.compute_gv(T(x, dev), T(w, dev), sign, wvln, wedge, chi, (tx, ty, tz), gt)
+                g = gt.cpu().numpy()
+            scale = np.abs(want).max(axis=1, keepdims=True)
+            err = np.abs(g - want) / scale
+            assert err.max() <= 8 * 2.0 ** -53
+            frac.append(np.mean(np.any(g != want, axis=1)))
+    assert np.mean(frac) < 0.05
+
+
+# ---- batched kernels against the oracle --------------------------------------
+
+@pytest.fixture(scope="module")
+def workload():
+    wl = simulate.scoring_workload(K=96, N=60000, seed=21, translations=True)
+    return wl
+
+
+def _oracle_gv(orc, wl, t=np.zeros(3)):
+    pk, pars = wl["peaks"], wl["pars"]
+    p, r, dist = geometry.pack_xlylzl(pars)
+    sign, wvln, wedge, chi = geometry.pack_gv(pars)
+    n = len(pk["sc"])
+    xl = np.zeros((n, 3))
+    orc.compute_xlylzl(pk["sc"], pk["fc"], p, r, dist, xl)
+    gv = np.zeros((n, 3))
+    orc.compute_gv(xl, pk["omega"], sign, wvln, wedge, chi, t, gv)
+    return xl, gv
+
+
+@pytest.mark.parametrize("ppl", [1, 2, 4, 8])
+def test_score_many_pipeline(dev, orc, workload, ppl):
+    wl = workload
+    _, gv = _oracle_gv(orc, wl)
+    want = orc.score_many(wl["ubi"], gv, 0.05)
+    pk = wl["peaks"]
+    g = batched.sf2gv(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev), wl["pars"])
+    got = batched.score_many(T(wl["ubi"], dev), g, 0.05, peaks_per_lane=ppl).cpu().numpy()
+    assert np.array_equal(got, want)
+    # and on the oracle's own g-vectors (no sincos difference at all)
+    got2 = batched.score_many(T(wl["ubi"], dev), T(gv, dev), 0.05, peaks_per_lane=ppl)
+    assert np.array_equal(got2.cpu().numpy(), want)
+
+
+def test_sf2gv_matches_separate_kernels(dev, orc, workload):
+    pk, pars = workload["peaks"], workload["pars"]
+    p, r, dist = geometry.pack_xlylzl(pars)
+    sign, wvln, wedge, chi = geometry.pack_gv(pars)
+    sc, fc, om = T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev)
+    gv, xl, so, co = batched.sf2gv(sc, fc, om, pars, keep=True)
+    xl2 = batched.xlylzl(sc, fc, p, r, dist)
+    gv2 = batched.gvectors(xl2, om, sign, wvln, wedge, chi)
+    assert torch.equal(xl, xl2) and torch.equal(gv, gv2)
+    xl_o, _ = _oracle_gv(orc, workload)
+    assert np.array_equal(xl.cpu().numpy(), xl_o)
+
+
+@pytest.mark.parametrize("d0,l0", [(1.0, -1), (1.0, 0), (2.0, -1)])
+def test_assign_many(dev, orc, workload, d0, l0):
+    wl = workload
+    _, gv = _oracle_gv(orc, wl)
+    n = len(gv)
+    d = np.full(n, d0)
+    lab = np.full(n, l0, np.int32)
+    labels_of = np.arange(len(wl["ubi"]), dtype=np.int32)[::-1].copy()
+    want_c = orc.assign_many(wl["ubi"], labels_of, gv, 0.05, d, lab)
+    dd, ll = T(np.full(n, d0), dev), T(np.full(n, l0), dev, torch.int32)
+    c = batched.assign_many(T(wl["ubi"], dev), T(gv, dev), 0.05, dd, ll,
+                            grain_labels=T(labels_of, dev, torch.int32))
+    assert np.array_equal(c.cpu().numpy(), want_c)
+    assert np.array_equal(ll.cpu().numpy(), lab)
+    assert np.array_equal(dd.cpu().numpy().view(np.int64), d.view(np.int64))
+
+
+@pytest.mark.parametrize("lpg", [8, 16, 32, 64])
+def test_refine_lists(dev, orc, workload, lpg):
+    wl = workload
+    _, gv = _oracle_gv(orc, wl)
+    K = len(wl["ubi"])
+    idx, off = simulate.refine_lists(wl["peaks"]["grain"], K, seed=4, median=200, lo=3, hi=2000)
+    rng = np.random.default_rng(8)
+    start = wl["ubi"] @ (np.eye(3) + rng.normal(scale=3e-3, size=(K, 3, 3)))
+    u_o, n_o, m_o = orc.refine_lists(start, gv, idx, off, 0.05, 4)
+    u, n, m = batched.refine_lists(T(start, dev), T(gv, dev), T(idx, dev, torch.int32),
+                                   T(off, dev, torch.int64), 0.05, 4, lanes_per_grain=lpg)
+    assert np.array_equal(n.cpu().numpy(), n_o)
+    m = m.cpu().numpy()
+    assert np.all(np.abs(m - m_o) <= MEAN_RTOL * np.abs(m_o))
+    np.testing.assert_allclose(u.cpu().numpy(), u_o, rtol=0, atol=UBI_TOL)
+
+
+def test_ken_simul_fight_over_peaks(dev):
+    k = load("ken_simul.npz")
+    gv = T(k["gv"], dev)
+    d = T(np.full(len(k["gv"]), 2.0), dev)
+    lab = T(np.full(len(k["gv"]), -1), dev, torch.int32)
+    c = batched.assign_many(T(k["ubis"], dev), gv, 0.01, d, lab)
+    assert c.cpu().numpy().tolist() == k["fop_n"].tolist()
+    assert np.array_equal(lab.cpu().numpy(), k["fop_labels"])
+    labs = lab.cpu().numpy()
+    for i, u in enumerate(k["ubis"]):
+        n, _ = gpu.refine_assigned(u.copy(), k["gv"], labs, i)
+        assert n == k["idx_npeaks"][i]
+
+
+def test_edge_cases(dev):
+    ubi = np.eye(3)
+    empty = np.zeros((0, 3))
+    assert gpu.score(ubi, empty, 0.1) == 0
+    u = ubi.copy()
+    assert gpu.score_and_refine(u, empty, 0.1) == (0, 0.0) and np.array_equal(u, ubi)
+    gv = np.array([[np.nan, 0, 0], [np.inf, 0, 0], [0.5, 1.5, 2.5], [2.0 ** 51 + 0.5, 0, 0],
+                   [2.0 ** 52 + 1, 3, 0], [-2.0 ** 51 - 1.5, 0, 0], [1e300, 0, 0], [-0.5, 0, 0]])
+    from oracle import oracle
+    orc = oracle.load_oracle()
+    for tol in (0.0, 0.1, 0.8, 1.5, 1e10):
+        assert gpu.score(ubi, gv, tol) == orc.score(ubi, gv, tol)
+        d1, d2 = np.ones(len(gv)), np.ones(len(gv))
+        l1, l2 = np.zeros(len(gv), np.int32), np.zeros(len(gv), np.int32)
+        assert gpu.score_and_assign(ubi, gv, tol, d1, l1, 0) == \
+            orc.score_and_assign(ubi, gv, tol, d2, l2, 0)
+        assert np.array_equal(d1.view(np.int64), d2.view(np.int64)) and np.array_equal(l1, l2)
+    K0 = batched.score_many(T(np.zeros((0, 3, 3)), dev), T(gv, dev), 0.1)
+    assert K0.numel() == 0
+
+
+def test_full_scoring_workload_counts(dev, orc):
+    """compute_xlylzl -> compute_gv -> score, 1000 grains x 1e6 peaks: the
+    1000 per-grain counts are identical to the oracle's."""
+    wl = simulate.scoring_workload(K=1000, N=1_000_000, seed=2)
+    _, gv = _oracle_gv(orc, wl)
+    want = orc.score_many(wl["ubi"], gv, 0.05, nthreads=min(16, os.cpu_count() or 1))
+    pk = wl["peaks"]
+    g = batched.sf2gv(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev), wl["pars"])
+    got = batched.score_many(T(wl["ubi"], dev), g, 0.05).cpu().numpy()
+    assert np.array_equal(got, want)

@@ -1,0 +1,148 @@
+"""CPU-side checks of the product library: the C ABI loads and exports
+every declared symbol, the host logic of the drop-in (f2py argument rules),
+the accuracy of the device sincos algorithm (its host build), the
+simulator's invariants.  No kernel is launched."""
+import math
+
+import mpmath
+import numpy as np
+import pytest
+import torch
+
+from imaged11_amd import _lib, geometry, simulate
+from imaged11_amd import cImageD11 as gpu
+
+
+def test_exports_every_header_symbol():
+    lib = _lib.load()
+    names = _lib.header_symbols()
+    assert {"score", "score_and_refine", "score_and_assign", "refine_assigned", "compute_gv",
+            "compute_xlylzl", "verify_rounding", "gs_score_many", "gs_refine_lists"} <= set(names)
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _lib.SIGNATURES, "no ctypes signature for %s" % n
+    assert lib.gs_version() == 1
+
+
+def test_single_hip_runtime():
+    _lib.load()
+    assert len(_lib.hip_runtimes_loaded()) == 1
+
+
+def _host_sincos(x):
+    x = np.ascontiguousarray(x, np.float64)
+    s, c = np.zeros_like(x), np.zeros_like(x)
+    _lib.load().gs_selftest_sincos_host(_lib.hptr(x), _lib.hptr(s), _lib.hptr(c), len(x))
+    return s, c
+
+
+def test_device_sincos_correctly_rounded():
+    """The device sincos algorithm (compiled for the host) returns the
+    correctly rounded sin and cos, checked against 200-bit mpmath, on
+    omega grids, random angles and awkward arguments."""
+    mpmath.mp.prec = 200
+    rng = np.random.default_rng(1)
+    omega = np.concatenate([(np.arange(720) + 0.5) * 0.25, np.arange(-360, 361, 15.0),
+                            rng.uniform(-720, 720, 1500)])
+    x = np.concatenate([np.pi / 180.0 * omega,
+                        [1e-300, -1e-300, 5e-324, 1e-8, np.pi / 2, np.pi, 3 * np.pi / 2, 1e5,
+                         -123456.789, 800000.0]])
+    s, c = _host_sincos(x)
+    for xi, si, ci in zip(x, s, c):
+        m = mpmath.mpf(float(xi))
+        assert si == float(mpmath.sin(m)), xi
+        assert ci == float(mpmath.cos(m)), xi
+
+
+def test_device_sincos_special():
+    s, c = _host_sincos(np.array([0.0, -0.0, np.inf, np.nan]))
+    assert s[0] == 0.0 and not math.copysign(1, s[0]) < 0 and c[0] == 1.0
+    assert s[1] == 0.0 and math.copysign(1, s[1]) < 0 and c[1] == 1.0
+    assert np.isnan(s[2]) and np.isnan(c[2]) and np.isnan(s[3])
+
+
+def test_device_sincos_vs_glibc_sincos(orc):
+    """glibc's sincos (what the reference's compute_gv calls) is not
+    correctly rounded everywhere: the two agree except in the last bit of a
+    small fraction of arguments."""
+    rng = np.random.default_rng(2)
+    omega = rng.uniform(-360, 360, 200000)
+    so, co = orc.sincos_omega(omega, 1.0)
+    s, c = _host_sincos(np.pi / 180.0 * omega * 1.0)
+    diff = (s != so) | (c != co)
+    assert diff.mean() < 0.01
+    assert np.abs(s - so).max() <= 2.0 ** -53 and np.abs(c - co).max() <= 2.0 ** -53
+
+
+def test_dropin_argument_rules():
+    """f2py rules of src/_cImageD11.pyf: intent(inout) arrays must have the
+    exact dtype and be C-contiguous; shapes are checked; checked before any
+    device work."""
+    gv = np.zeros((10, 3))
+    with pytest.raises(ValueError):
+        gpu.score_and_refine(np.eye(3, dtype=np.float32), gv, 0.1)
+    with pytest.raises(ValueError):
+        gpu.score_and_refine(np.asfortranarray(np.arange(9.0).reshape(3, 3)), gv, 0.1)
+    with pytest.raises(ValueError):
+        gpu.score_and_refine([[1, 0, 0], [0, 1, 0], [0, 0, 1]], gv, 0.1)
+    with pytest.raises(ValueError):
+        gpu.score(np.eye(3), np.zeros((10, 2)), 0.1)
+    with pytest.raises(ValueError):
+        gpu.score(np.eye(2), gv, 0.1)
+    with pytest.raises(ValueError):
+        gpu.score_and_assign(np.eye(3), gv, 0.1, np.ones(10, np.float32), np.zeros(10, np.int32), 1)
+    with pytest.raises(ValueError):
+        gpu.score_and_assign(np.eye(3), gv, 0.1, np.ones(10), np.zeros(10, np.int64), 1)
+    with pytest.raises(ValueError):
+        gpu.score_and_assign(np.eye(3), gv, 0.1, np.ones(9), np.zeros(10, np.int32), 1)
+    with pytest.raises(ValueError):
+        gpu.compute_gv(gv, np.zeros(10), 1, 0.2, 0, 0, (0, 0, 0), np.zeros((10, 3), np.float32))
+    with pytest.raises(ValueError):
+        gpu.compute_gv(gv, np.zeros(10), 1, 0.2, 0, 0, (0, 0), np.zeros((10, 3)))
+    with pytest.raises(ValueError):
+        gpu.compute_xlylzl(np.zeros(5), np.zeros(5), np.zeros(4), np.eye(3), np.zeros(3),
+                           np.zeros((5, 3))[:, ::-1])
+    with pytest.raises(ValueError):
+        gpu.compute_xlylzl(np.zeros(5), np.zeros(4), np.zeros(4), np.eye(3), np.zeros(3),
+                           np.zeros((5, 3)))
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU failure")
+def test_dropin_fails_loudly_without_gpu():
+    with pytest.raises(_lib.GscoreError):
+        gpu.score(np.eye(3), np.zeros((4, 3)), 0.1)
+    with pytest.raises(_lib.GscoreError):
+        gpu.verify_rounding(20)
+
+
+def test_omp_thread_knob():
+    gpu.cimaged11_omp_set_num_threads(3)
+    assert gpu.cimaged11_omp_get_max_threads() == 3
+    gpu.cimaged11_omp_set_num_threads(1)
+
+
+def test_geometry_packing_matches_ctransform_formula():
+    pars = {"z_center": 980.0, "y_center": 1010.0, "z_size": 48.0, "y_size": 49.0,
+            "distance": 100100.0, "tilt_x": 0.01, "tilt_y": -0.02, "tilt_z": 0.03,
+            "o11": 1, "o12": 0, "o21": 0, "o22": -1}
+    p, r, dist = geometry.pack_xlylzl(pars)
+    assert p.tolist() == [980.0, 1010.0, 48.0, 49.0] and dist.tolist() == [100100.0, 0, 0]
+    R = r.reshape(3, 3)
+    np.testing.assert_allclose(R @ R.T * np.diag([1, 1, 1]), np.diag([1, 1, 1]), atol=1e-15)
+    # Rz first, then Ry, then Rx: a detector vector along y picks up tilt_z first
+    rz = geometry.detector_rotation_matrix(0, 0, 0.03)
+    assert rz[1, 0] > 0 and rz[0, 1] < 0
+    m = geometry.wedge_chi_matrix(3.0, 4.0).reshape(3, 3)
+    np.testing.assert_allclose(m @ m.T, np.eye(3), atol=1e-15)
+
+
+def test_simulator_invariants():
+    wl = simulate.scoring_workload(K=10, N=5000, seed=3)
+    pk = wl["peaks"]
+    assert len(pk["sc"]) == 5000 and (pk["grain"] == -1).sum() == 500
+    assert set(np.unique(pk["omega"] / 0.25 - 0.5)) <= set(np.arange(720.0))
+    idx, off = simulate.refine_lists(pk["grain"], 10, median=40, lo=20, hi=80)
+    L = np.diff(off)
+    assert L.min() >= 20 and L.max() <= 80 and off[-1] == len(idx)
+    ubi, gv, lat = simulate.lattice_gvectors(seed=1)
+    assert gv.shape == (20000, 3) and lat.sum() == 12000
