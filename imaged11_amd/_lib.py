@@ -53,7 +53,8 @@ SIGNATURES = {
     "gs_sincos_omega": (_i, [_vp, _d, _vp, _vp, _i, _vp]),
     "gs_sf2gv": (_i, [_vp, _vp, _vp, _dp, _dp, _dp, _d, _d, _d, _d, _dp, _vp, _vp, _vp, _vp, _i,
                       _vp]),
-    "gs_refine_lists": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _d, _i, _vp, _vp, _i, _i, _vp]),
+    "gs_refine_lists": (_i, [_vp, _i, _vp, _i, _vp, ctypes.c_longlong, _vp, _vp, _d, _i, _vp, _vp,
+                             _i, _i, _vp]),
 }
 
 _lib = None

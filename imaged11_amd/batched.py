@@ -180,7 +180,8 @@ def refine_lists(ubis, gv, idx, off, tol, n_iter=2, order=None, lanes_per_grain=
     u = ubis if inplace else ubis.clone()
     n = torch.empty((n_iter, K), dtype=torch.int32, device=gv.device)
     m = torch.empty((n_iter, K), dtype=torch.float64, device=gv.device)
-    _lib.check(lib.gs_refine_lists(dptr(u), K, dptr(gv), dptr(idx), dptr(off), dptr(order),
+    _lib.check(lib.gs_refine_lists(dptr(u), K, dptr(gv), gv.shape[0], dptr(idx), idx.numel(),
+                                   dptr(off), dptr(order),
                                    float(tol), int(n_iter), dptr(n), dptr(m), int(lanes_per_grain),
                                    int(waves_per_cu), stream_ptr()), "refine_lists")
     return u, n, m

@@ -250,10 +250,13 @@ __global__ void k_refine_finalize(const double* __restrict__ partials, int nbloc
 // the LPG lanes, which leaves bit-identical totals in every lane, so each
 // lane solves the 3x3 system itself and keeps the updated UBI in registers.
 // n_out/mean_out are [n_iter][K]; queue[0] must be zero at launch.
+// Out-of-range entries are ignored rather than trusted: a grain number
+// outside [0, K) in `order` is skipped, list bounds are clipped to
+// [0, n_idx], and peak indices outside [0, ng) are not kept.
 template <int LPG>
 __global__ __launch_bounds__(GS_BLOCK) void k_refine_lists(
-    double* __restrict__ ubis, int K, const double* __restrict__ gv,
-    const int* __restrict__ idx, const long long* __restrict__ off,
+    double* __restrict__ ubis, int K, const double* __restrict__ gv, int ng,
+    const int* __restrict__ idx, long long n_idx, const long long* __restrict__ off,
     const int* __restrict__ order, double tolsq, int n_iter, int* __restrict__ n_out,
     double* __restrict__ mean_out, int* __restrict__ queue) {
     constexpr int GPW = GS_WAVE / LPG;
@@ -266,10 +269,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_refine_lists(
         base = __shfl(base, 0, GS_WAVE);
         if (base >= K) break;
         const int slot = base + sub;
-        const bool active = slot < K;
-        const int g = active ? order[slot] : 0;
-        const long long o0 = active ? off[g] : 0;
-        const long long o1 = active ? off[g + 1] : 0;
+        const int gq = slot < K ? order[slot] : -1;
+        const bool active = gq >= 0 && gq < K;
+        const int g = active ? gq : 0;
+        long long o1 = active ? off[g + 1] : 0;
+        o1 = o1 < 0 ? 0 : (o1 > n_idx ? n_idx : o1);
+        long long o0 = active ? off[g] : 0;
+        o0 = o0 < 0 ? 0 : (o0 > o1 ? o1 : o0);
         double u[9];
         for (int i = 0; i < 9; i++) u[i] = active ? ubis[9 * (size_t)g + i] : 0.0;
         for (int it = 0; it < n_iter; it++) {
@@ -279,6 +285,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_refine_lists(
             for (int i = 0; i < GS_NPART; i++) a[i] = 0.0;
             for (long long q = o0 + sl; q < o1; q += LPG) {
                 const long long k = idx[q];
+                if (k < 0 || k >= ng) continue;
                 double g0 = gv[3 * k + 0], g1 = gv[3 * k + 1], g2 = gv[3 * k + 2];
                 double i0, i1, i2;
                 double s = gs_drlv2(U, g0, g1, g2, i0, i1, i2);

@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -140,19 +141,24 @@ int stage_reserve(size_t bytes, char** out) {
     return GS_OK;
 }
 
-// carve 256-byte aligned slices out of a staging buffer
-struct Carver {
-    char* base;
-    size_t used = 0;
-    explicit Carver(char* b) : base(b) {}
-    template <class T>
-    T* take(size_t count) {
-        T* p = reinterpret_cast<T*>(base + used);
-        used += (count * sizeof(T) + 255) & ~size_t(255);
-        return p;
+// Reserve one staging buffer holding consecutive 256-byte aligned slices
+// of the given sizes; the reservation is computed from the same list that
+// is carved, so no slice can run past it.
+size_t slice_bytes(size_t b) { return (b + 255) & ~size_t(255); }
+
+int stage_slices(std::initializer_list<size_t> sizes, std::vector<char*>& out) {
+    size_t need = 0;
+    for (size_t b : sizes) need += slice_bytes(b);
+    char* buf;
+    GS_CHECK(stage_reserve(need, &buf));
+    out.clear();
+    size_t at = 0;
+    for (size_t b : sizes) {
+        out.push_back(buf + at);
+        at += slice_bytes(b);
     }
-    static size_t bytes(size_t count, size_t elem) { return (count * elem + 255) & ~size_t(255); }
-};
+    return GS_OK;
+}
 
 int check_ng(long long ng) {
     if (ng < 0) return set_err(GS_ERR_ARG, "negative array length %lld", ng);
@@ -321,13 +327,11 @@ int verify_rounding(int n) {
 
 int score(double ubi[3][3], double gv[][3], double tol, int ng) {
     if (check_ng(ng) != GS_OK) return -1;
-    char* buf;
-    size_t need = Carver::bytes(9, 8) + Carver::bytes((size_t)ng * 3, 8) + Carver::bytes(1, 4);
-    if (stage_reserve(need, &buf) != GS_OK) return -1;
-    Carver c(buf);
-    double* du = c.take<double>(9);
-    double* dg = c.take<double>((size_t)ng * 3);
-    int* dn = c.take<int>(1);
+    std::vector<char*> sl;
+    if (stage_slices({72, (size_t)ng * 24, 4}, sl) != GS_OK) return -1;
+    double* du = reinterpret_cast<double*>(sl[0]);
+    double* dg = reinterpret_cast<double*>(sl[1]);
+    int* dn = reinterpret_cast<int*>(sl[2]);
     hipStream_t st = nullptr;
     int h = -1;
     if (hipMemcpyAsync(du, ubi, 72, hipMemcpyHostToDevice, st) != hipSuccess ||
@@ -347,16 +351,13 @@ int score(double ubi[3][3], double gv[][3], double tol, int ng) {
 static int refine_host(int mode, double* ubi, const double* gv, const int* labels, int label,
                        double tol, int* n_out, double* mean_out, int ng) {
     GS_CHECK(check_ng(ng));
-    char* buf;
-    size_t need = Carver::bytes(9, 8) + Carver::bytes((size_t)ng * 3, 8) + Carver::bytes(2, 8) +
-                  (mode ? Carver::bytes((size_t)ng, 4) : 0);
-    GS_CHECK(stage_reserve(need, &buf));
-    Carver c(buf);
-    double* du = c.take<double>(9);
-    double* dg = c.take<double>((size_t)ng * 3);
-    double* dm = c.take<double>(1);
-    int* dn = reinterpret_cast<int*>(c.take<double>(1));
-    int* dl = mode ? c.take<int>((size_t)ng) : nullptr;
+    std::vector<char*> sl;
+    GS_CHECK(stage_slices({72, (size_t)ng * 24, 8, 4, mode ? (size_t)ng * 4 : 0}, sl));
+    double* du = reinterpret_cast<double*>(sl[0]);
+    double* dg = reinterpret_cast<double*>(sl[1]);
+    double* dm = reinterpret_cast<double*>(sl[2]);
+    int* dn = reinterpret_cast<int*>(sl[3]);
+    int* dl = mode ? reinterpret_cast<int*>(sl[4]) : nullptr;
     hipStream_t st = nullptr;
     GS_HIP(hipMemcpyAsync(du, ubi, 72, hipMemcpyHostToDevice, st));
     if (ng > 0) GS_HIP(hipMemcpyAsync(dg, gv, (size_t)ng * 24, hipMemcpyHostToDevice, st));
@@ -388,16 +389,13 @@ void refine_assigned(double ubi[3][3], double gv[][3], int* labels, int label, i
 int score_and_assign(double ubi[3][3], double gv[][3], double tol, double* drlv2, int* labels,
                      int label, int ng) {
     if (check_ng(ng) != GS_OK) return -1;
-    char* buf;
-    size_t need = Carver::bytes(9, 8) + Carver::bytes((size_t)ng * 3, 8) +
-                  Carver::bytes((size_t)ng, 8) + Carver::bytes((size_t)ng, 4) + Carver::bytes(2, 4);
-    if (stage_reserve(need, &buf) != GS_OK) return -1;
-    Carver c(buf);
-    double* du = c.take<double>(9);
-    double* dg = c.take<double>((size_t)ng * 3);
-    double* dd = c.take<double>((size_t)ng);
-    int* dl = c.take<int>((size_t)ng);
-    int* dn = c.take<int>(2);  // [0] count, [1] the label
+    std::vector<char*> sl;
+    if (stage_slices({72, (size_t)ng * 24, (size_t)ng * 8, (size_t)ng * 4, 8}, sl) != GS_OK) return -1;
+    double* du = reinterpret_cast<double*>(sl[0]);
+    double* dg = reinterpret_cast<double*>(sl[1]);
+    double* dd = reinterpret_cast<double*>(sl[2]);
+    int* dl = reinterpret_cast<int*>(sl[3]);
+    int* dn = reinterpret_cast<int*>(sl[4]);  // [0] count, [1] the label
     hipStream_t st = nullptr;
     int h = -1;
     bool ok = hipMemcpyAsync(du, ubi, 72, hipMemcpyHostToDevice, st) == hipSuccess &&
@@ -425,13 +423,11 @@ int score_and_assign(double ubi[3][3], double gv[][3], double tol, double* drlv2
 void compute_gv(double xlylzl[][3], double omega[], double omegasign, double wvln, double wedge,
                 double chi, double t[3], double gv[][3], int n) {
     if (check_ng(n) != GS_OK || n == 0) return;
-    char* buf;
-    size_t need = 2 * Carver::bytes((size_t)n * 3, 8) + Carver::bytes((size_t)n, 8);
-    if (stage_reserve(need, &buf) != GS_OK) return;
-    Carver c(buf);
-    double* dx = c.take<double>((size_t)n * 3);
-    double* dw = c.take<double>((size_t)n);
-    double* dg = c.take<double>((size_t)n * 3);
+    std::vector<char*> sl;
+    if (stage_slices({(size_t)n * 24, (size_t)n * 8, (size_t)n * 24}, sl) != GS_OK) return;
+    double* dx = reinterpret_cast<double*>(sl[0]);
+    double* dw = reinterpret_cast<double*>(sl[1]);
+    double* dg = reinterpret_cast<double*>(sl[2]);
     hipStream_t st = nullptr;
     if (hipMemcpyAsync(dx, xlylzl, (size_t)n * 24, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(dw, omega, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
@@ -448,13 +444,11 @@ void compute_gv(double xlylzl[][3], double omega[], double omegasign, double wvl
 void compute_xlylzl(double s[], double f[], double p[4], double r[9], double dist[3],
                     double xlylzl[][3], int n) {
     if (check_ng(n) != GS_OK || n == 0) return;
-    char* buf;
-    size_t need = 2 * Carver::bytes((size_t)n, 8) + Carver::bytes((size_t)n * 3, 8);
-    if (stage_reserve(need, &buf) != GS_OK) return;
-    Carver c(buf);
-    double* ds = c.take<double>((size_t)n);
-    double* df = c.take<double>((size_t)n);
-    double* dx = c.take<double>((size_t)n * 3);
+    std::vector<char*> sl;
+    if (stage_slices({(size_t)n * 8, (size_t)n * 8, (size_t)n * 24}, sl) != GS_OK) return;
+    double* ds = reinterpret_cast<double*>(sl[0]);
+    double* df = reinterpret_cast<double*>(sl[1]);
+    double* dx = reinterpret_cast<double*>(sl[2]);
     hipStream_t st = nullptr;
     if (hipMemcpyAsync(ds, s, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(df, f, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
@@ -550,12 +544,13 @@ int gs_sf2gv(const double* s, const double* f, const double* omega, const double
     return GS_OK;
 }
 
-int gs_refine_lists(double* ubis, int K, const double* gv, const int* idx, const long long* off,
-                    const int* order, double tol, int n_iter, int* n_out, double* mean_out,
-                    int lanes_per_grain, int waves_per_cu, gs_stream_t stream) {
+int gs_refine_lists(double* ubis, int K, const double* gv, int ng, const int* idx, long long n_idx,
+                    const long long* off, const int* order, double tol, int n_iter, int* n_out,
+                    double* mean_out, int lanes_per_grain, int waves_per_cu, gs_stream_t stream) {
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
-    if (K < 0 || n_iter < 0) return set_err(GS_ERR_ARG, "negative K or n_iter");
+    if (K < 0 || n_iter < 0 || ng < 0 || n_idx < 0)
+        return set_err(GS_ERR_ARG, "negative K, n_iter, ng or n_idx");
     if (K == 0 || n_iter == 0) return GS_OK;
     hipStream_t st = S(stream);
     int* queue = ctx->counters;
@@ -569,10 +564,10 @@ int gs_refine_lists(double* ubis, int K, const double* gv, const int* idx, const
     long long waves = std::min<long long>(waves_needed, (long long)wpc * ctx->cus);
     int nb = (int)std::max<long long>(1, (waves + GS_BLOCK / GS_WAVE - 1) / (GS_BLOCK / GS_WAVE));
     switch (lpg) {
-        case 8: hipLaunchKernelGGL(k_refine_lists<8>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
-        case 32: hipLaunchKernelGGL(k_refine_lists<32>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
-        case 64: hipLaunchKernelGGL(k_refine_lists<64>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
-        default: hipLaunchKernelGGL(k_refine_lists<16>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        case 8: hipLaunchKernelGGL(k_refine_lists<8>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        case 32: hipLaunchKernelGGL(k_refine_lists<32>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        case 64: hipLaunchKernelGGL(k_refine_lists<64>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        default: hipLaunchKernelGGL(k_refine_lists<16>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
     }
     GS_LAUNCHED();
     return GS_OK;

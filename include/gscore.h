@@ -135,11 +135,13 @@ int gs_sf2gv(const double *s, const double *f, const double *omega, const double
  * place; n_out/mean_out are [n_iter][K].  `order` (K ints) is the order in
  * which the persistent waves take grains (longest list first balances
  * best).  lanes_per_grain 8/16/32/64 and waves_per_cu tune the launch
- * (0 = defaults). */
-int gs_refine_lists(double *ubis, int K, const double *gv, const int *idx,
-                    const long long *off, const int *order, double tol, int n_iter,
-                    int *n_out, double *mean_out, int lanes_per_grain, int waves_per_cu,
-                    gs_stream_t stream);
+ * (0 = defaults).  Entries out of range are ignored, never dereferenced:
+ * grains outside [0, K) in order, offsets outside [0, n_idx], peak
+ * indices outside [0, ng). */
+int gs_refine_lists(double *ubis, int K, const double *gv, int ng, const int *idx,
+                    long long n_idx, const long long *off, const int *order, double tol,
+                    int n_iter, int *n_out, double *mean_out, int lanes_per_grain,
+                    int waves_per_cu, gs_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,7 @@ pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 MEAN_RTOL = 1e-12
+CHAIN_MEAN_RTOL = 1e-9
 UBI_TOL = 1e-10
 
 
@@ -62,17 +63,27 @@ def test_score_golden(dev, mode):
 
 
 @pytest.mark.parametrize("mode", ["host", "device"])
-def test_score_and_refine_golden(dev, mode):
+def test_score_and_refine_golden(dev, orc, mode):
+    """Three chained calls.  Per call (same input UBI on both sides): n
+    exact, mean within 1e-12, UBI within 1e-10.  Along the chain against
+    the reference's golden outputs: n exact, UBI within 1e-10; the mean of
+    a later call is evaluated at an input UBI that already differs by
+    ~1e-14 (R summed in another order), and drlv2 = t^2 amplifies that to
+    ~1e-11 relative, so the chained mean bar is CHAIN_MEAN_RTOL."""
     s = load("scoring_cases.npz")
     gv = s["gv"] if mode == "host" else T(s["gv"], dev)
     for i, u0 in enumerate(s["ubis"]):
         for j, t in enumerate(s["tols"]):
             u = u0.copy()
             for k in range(3):
+                u_in = u.copy()
                 n, m = gpu.score_and_refine(u, gv, t)
-                assert n == s["sar_n"][i, j, k]
+                n_o, m_o = orc.score_and_refine(u_in, s["gv"], t)
+                assert n == n_o == s["sar_n"][i, j, k]
+                assert abs(m - m_o) <= MEAN_RTOL * abs(m_o)
+                np.testing.assert_allclose(u, u_in, rtol=0, atol=UBI_TOL)
                 want_m = s["sar_mean"][i, j, k]
-                assert abs(m - want_m) <= MEAN_RTOL * abs(want_m)
+                assert abs(m - want_m) <= (MEAN_RTOL if k == 0 else CHAIN_MEAN_RTOL) * abs(want_m)
                 np.testing.assert_allclose(u, s["sar_ubi"][i, j, k], rtol=0, atol=UBI_TOL)
 
 
@@ -254,12 +265,20 @@ def test_refine_lists(dev, orc, workload, lpg):
     idx, off = simulate.refine_lists(wl["peaks"]["grain"], K, seed=4, median=200, lo=3, hi=2000)
     rng = np.random.default_rng(8)
     start = wl["ubi"] @ (np.eye(3) + rng.normal(scale=3e-3, size=(K, 3, 3)))
+    gvd, idxd, offd = T(gv, dev), T(idx, dev, torch.int32), T(off, dev, torch.int64)
+    # per call: one iteration from the same UBIs
+    u_o, n_o, m_o = orc.refine_lists(start, gv, idx, off, 0.05, 1)
+    u, n, m = batched.refine_lists(T(start, dev), gvd, idxd, offd, 0.05, 1, lanes_per_grain=lpg)
+    assert np.array_equal(n.cpu().numpy(), n_o)
+    assert np.all(np.abs(m.cpu().numpy() - m_o) <= MEAN_RTOL * np.abs(m_o))
+    np.testing.assert_allclose(u.cpu().numpy(), u_o, rtol=0, atol=UBI_TOL)
+    # chained: four iterations inside one launch
     u_o, n_o, m_o = orc.refine_lists(start, gv, idx, off, 0.05, 4)
-    u, n, m = batched.refine_lists(T(start, dev), T(gv, dev), T(idx, dev, torch.int32),
-                                   T(off, dev, torch.int64), 0.05, 4, lanes_per_grain=lpg)
+    u, n, m = batched.refine_lists(T(start, dev), gvd, idxd, offd, 0.05, 4, lanes_per_grain=lpg)
     assert np.array_equal(n.cpu().numpy(), n_o)
     m = m.cpu().numpy()
-    assert np.all(np.abs(m - m_o) <= MEAN_RTOL * np.abs(m_o))
+    assert np.all(np.abs(m[0] - m_o[0]) <= MEAN_RTOL * np.abs(m_o[0]))
+    assert np.all(np.abs(m - m_o) <= CHAIN_MEAN_RTOL * np.abs(m_o))
     np.testing.assert_allclose(u.cpu().numpy(), u_o, rtol=0, atol=UBI_TOL)
 
 
