@@ -36,6 +36,7 @@ SIGNATURES = {
     "gs_device_count": (_i, []),
     "gs_init": (_i, []),
     "gs_selftest_sincos_host": (None, [_vp, _vp, _vp, _i]),
+    "gs_selftest_certified_host": (None, [_vp, _vp, _i, _d, _d, _vp]),
     "verify_rounding": (_i, [_i]),
     "score": (_i, [_vp, _vp, _d, _i]),
     "score_and_refine": (None, [_vp, _vp, _d, _ip, _dp, _i]),
@@ -43,7 +44,7 @@ SIGNATURES = {
     "refine_assigned": (None, [_vp, _vp, _vp, _i, _ip, _dp, _i]),
     "compute_gv": (None, [_vp, _vp, _d, _d, _d, _d, _vp, _vp, _i]),
     "compute_xlylzl": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _i]),
-    "gs_score_many": (_i, [_vp, _i, _vp, _i, _d, _vp, _i, _vp]),
+    "gs_score_many": (_i, [_vp, _i, _vp, _i, _d, _vp, _i, _i, _vp]),
     "gs_assign_many": (_i, [_vp, _vp, _i, _vp, _i, _d, _vp, _vp, _vp, _vp]),
     "gs_score_and_refine_dev": (_i, [_vp, _vp, _i, _d, _vp, _vp, _vp]),
     "gs_refine_assigned_dev": (_i, [_vp, _vp, _i, _vp, _i, _vp, _vp, _vp]),
@@ -54,8 +55,23 @@ SIGNATURES = {
     "gs_sf2gv": (_i, [_vp, _vp, _vp, _dp, _dp, _dp, _d, _d, _d, _d, _dp, _vp, _vp, _vp, _vp, _i,
                       _vp]),
     "gs_refine_lists": (_i, [_vp, _i, _vp, _i, _vp, ctypes.c_longlong, _vp, _vp, _d, _i, _vp, _vp,
-                             _i, _i, _vp]),
+                             _i, _i, _vp, _vp]),
+    "gs_refine_graph_create": (_i, [ctypes.POINTER(_vp), _vp, _i, _vp, _i, _vp, ctypes.c_longlong,
+                                    _vp, _vp, _d, _i, _vp, _vp, _i, _i, _vp, _vp]),
+    "gs_graph_launch": (_i, [_vp, _vp]),
+    "gs_gv_assign_many": (_i, [_vp, _vp, _vp, _i, _d, _d, _d, _dp, _vp, _vp, _vp, _i, _d, _vp, _vp,
+                               _vp, _vp]),
+    "gs_gv_lists": (_i, [_vp, _vp, _vp, _i, _d, _d, _d, _dp, _vp, _i, _vp, _vp, ctypes.c_longlong,
+                         _vp, _vp]),
+    "gs_joint_hkl": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _i, _vp, _vp, ctypes.c_longlong,
+                          _vp, _vp]),
+    "gs_joint_adjoint": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _i, _vp, ctypes.c_longlong,
+                              _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
+    "gs_graph_destroy": (_i, [_vp]),
 }
+
+SCORE_CERTIFIED = 0
+SCORE_EXACT = 1
 
 _lib = None
 

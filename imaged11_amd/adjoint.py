@@ -1,0 +1,197 @@
+"""Gradient of the joint grain + detector refinement objective.
+
+ImageD11 fits detector geometry by Nelder-Mead on ``refinegrains.gof``
+(``ImageD11/refinegrains.py:517-624``, ``ImageD11/simplex.py:51-95``), one
+Python loop over grains per objective evaluation.  Here the objective
+
+    L(UBIs, translations, geometry) =
+        sum_g sum_{q in list g} || UBI_g . G_q(geometry, t_g) - hkl_q ||^2
+
+with G_q = compute_gv(compute_xlylzl(sc, fc), omega) for grain g's
+translation and assignments/hkl held fixed, is evaluated together with its
+exact gradient in ONE kernel pass (``gs_joint_adjoint``: a hand-written
+reverse sweep through compute_gv and compute_xlylzl per (grain, peak)
+pair).  `JointLoss` exposes it as a ``torch.autograd.Function`` of the
+UBIs, the translations and the 26 packed geometry numbers; `pack_geometry`
+maps the physical parameters (distance, y/z centre, tilts, wavelength,
+wedge, chi) onto that packing with differentiable torch operations, so
+autograd carries the gradient to them.
+"""
+import numpy as np
+import torch
+
+from . import _lib, batched, geometry
+from ._lib import dptr, stream_ptr
+
+GEO = 26
+FREE = ("distance", "y_center", "z_center", "tilt_x", "tilt_y", "tilt_z", "wavelength", "wedge",
+        "chi")
+
+
+def pack_geometry(params, fixed):
+    """26-vector [p(4), r(9), dist(3), wavelength, mat(9)] from a dict of
+    0-d tensors ``params`` (any of FREE) and plain numbers ``fixed`` (the
+    rest of an ImageD11 parameter dict: pixel sizes, o11..o22, ...).
+
+    Same conventions as transform.Ctransform (``ImageD11/transform.py:
+    685-693``) and the wedge/chi matrix of compute_gv
+    (``src/cdiffraction.c:119-142``); differentiable in ``params``."""
+    q = geometry.full_parameters(fixed)
+    dev = next(iter(params.values())).device if params else torch.device("cpu")
+
+    def P(name):
+        if name in params:
+            return params[name]
+        return torch.tensor(float(q[name]), dtype=torch.float64, device=dev)
+
+    one = torch.ones((), dtype=torch.float64, device=dev)
+    zero = torch.zeros((), dtype=torch.float64, device=dev)
+    tx, ty, tz = P("tilt_x"), P("tilt_y"), P("tilt_z")
+    rz = torch.stack([torch.stack([torch.cos(tz), -torch.sin(tz), zero]),
+                      torch.stack([torch.sin(tz), torch.cos(tz), zero]),
+                      torch.stack([zero, zero, one])])
+    ry = torch.stack([torch.stack([torch.cos(ty), zero, torch.sin(ty)]),
+                      torch.stack([zero, one, zero]),
+                      torch.stack([-torch.sin(ty), zero, torch.cos(ty)])])
+    rx = torch.stack([torch.stack([one, zero, zero]),
+                      torch.stack([zero, torch.cos(tx), -torch.sin(tx)]),
+                      torch.stack([zero, torch.sin(tx), torch.cos(tx)])])
+    flip = torch.tensor(geometry.flip_matrix(q["o11"], q["o12"], q["o21"], q["o22"]),
+                        dtype=torch.float64, device=dev)
+    r = (rx @ ry @ rz @ flip).reshape(9)
+    p = torch.stack([P("z_center"), P("y_center"), P("z_size"), P("y_size")])
+    dist = torch.stack([P("distance"), zero, zero])
+    w = P("wedge") * (np.pi / 180.0)
+    c = P("chi") * (np.pi / 180.0)
+    W = torch.stack([torch.stack([torch.cos(w), zero, -torch.sin(w)]),
+                     torch.stack([zero, one, zero]),
+                     torch.stack([torch.sin(w), zero, torch.cos(w)])])
+    C = torch.stack([torch.stack([one, zero, zero]),
+                     torch.stack([zero, torch.cos(c), -torch.sin(c)]),
+                     torch.stack([zero, torch.sin(c), torch.cos(c)])])
+    mat = (W @ C).reshape(9)
+    return torch.cat([p, r, dist, P("wavelength").reshape(1), mat])
+
+
+class JointProblem:
+    """Device-resident data of a joint refinement: the raw peak table
+    (sc, fc and the per-peak sin/cos of omega), per-grain CSR peak lists and
+    the fixed integer hkl of every list entry."""
+
+    def __init__(self, sc, fc, omega, omegasign, idx, off):
+        batched._need(sc, torch.float64, "sc", (None,))
+        n = sc.shape[0]
+        batched._need(fc, torch.float64, "fc", (n,))
+        batched._need(omega, torch.float64, "omega", (n,))
+        batched._need(idx, torch.int32, "idx", (None,))
+        batched._need(off, torch.int64, "off", (None,))
+        self.sc, self.fc = sc, fc
+        self.so, self.co = batched.sincos_omega(omega, omegasign)
+        self.idx, self.off = idx, off
+        self.K = off.numel() - 1
+        lengths = off[1:] - off[:-1]
+        self.owner = torch.repeat_interleave(
+            torch.arange(self.K, dtype=torch.int32, device=sc.device), lengths)
+        self.order = batched.list_order(off)
+        self.hkl = torch.zeros((idx.numel(), 3), dtype=torch.int32, device=sc.device)
+        self.queue = torch.zeros(1, dtype=torch.int32, device=sc.device)
+        dev = sc.device
+        self.loss_g = torch.empty(self.K, dtype=torch.float64, device=dev)
+        self.geo_part = torch.empty((self.K, GEO), dtype=torch.float64, device=dev)
+        self.out = torch.empty(GEO + 1, dtype=torch.float64, device=dev)
+
+    def assign_hkl(self, geo, ubis, trans):
+        """Fix hkl_q = round(UBI_g . G_q) at the given state."""
+        lib = _lib.load()
+        lib.gs_clear_error()
+        geo = geo.detach().contiguous()
+        _lib.check(lib.gs_joint_hkl(dptr(self.sc), dptr(self.fc), dptr(self.so), dptr(self.co),
+                                    self.sc.shape[0], dptr(geo), dptr(ubis.detach().contiguous()),
+                                    dptr(trans.detach().contiguous()), self.K, dptr(self.idx),
+                                    dptr(self.owner), self.idx.numel(), dptr(self.hkl),
+                                    stream_ptr()), "joint hkl")
+
+    def evaluate(self, geo, ubis, trans, gspace=False, want_sums=False, lanes_per_grain=16):
+        """One fused forward + reverse pass.  Returns (loss, dL/dUBI (K,3,3),
+        dL/dt (K,3), dL/dgeo (26,), sums) with sums = (K,15) Gauss-Newton
+        sums or None.  In g-space mode the loss is 1/2 sum ||UB.h - G||^2
+        and the UBI gradient slot holds dE/dUB."""
+        lib = _lib.load()
+        lib.gs_clear_error()
+        geo = geo.detach().contiguous()
+        ubis = ubis.detach().contiguous()
+        trans = trans.detach().contiguous()
+        dev = self.sc.device
+        gu = torch.empty((self.K, 3, 3), dtype=torch.float64, device=dev)
+        gt = torch.empty((self.K, 3), dtype=torch.float64, device=dev)
+        sums = torch.empty((self.K, 15), dtype=torch.float64, device=dev) if want_sums else None
+        _lib.check(lib.gs_joint_adjoint(
+            dptr(self.sc), dptr(self.fc), dptr(self.so), dptr(self.co), self.sc.shape[0],
+            dptr(geo), dptr(ubis), dptr(trans), self.K, dptr(self.idx), self.idx.numel(),
+            dptr(self.off), dptr(self.hkl), dptr(self.order), int(bool(gspace)),
+            dptr(self.loss_g), dptr(gu), dptr(gt), dptr(self.geo_part),
+            dptr(sums) if want_sums else None, dptr(self.out), int(lanes_per_grain),
+            dptr(self.queue), stream_ptr()), "joint adjoint")
+        out = self.out.clone()
+        return out[GEO], gu, gt, out[:GEO], sums
+
+
+class JointLoss(torch.autograd.Function):
+    """L(UBIs, translations, packed geometry) with its hand-written adjoint."""
+
+    @staticmethod
+    def forward(ctx, ubis, trans, geo, problem):
+        loss, gu, gt, gg, _ = problem.evaluate(geo, ubis, trans)
+        ctx.save_for_backward(gu, gt, gg)
+        return loss
+
+    @staticmethod
+    def backward(ctx, grad):
+        gu, gt, gg = ctx.saved_tensors
+        return grad * gu, grad * gt, grad * gg, None
+
+
+def joint_loss(problem, ubis, trans, params, fixed):
+    """L as a differentiable function of UBIs, translations and the
+    physical geometry parameters in ``params`` (dict of 0-d tensors)."""
+    geo = pack_geometry(params, fixed)
+    return JointLoss.apply(ubis, trans, geo, problem)
+
+
+def gauss_newton_ubis(sums):
+    """Per-grain least-squares UBI for fixed g-vectors and hkl:
+    UBI = (sum h G^T)(sum G G^T)^-1, from the (K,15) sums."""
+    A = sums[:, :9].reshape(-1, 3, 3)
+    s = sums[:, 9:]
+    M = torch.stack([torch.stack([s[:, 0], s[:, 1], s[:, 2]], -1),
+                     torch.stack([s[:, 1], s[:, 3], s[:, 4]], -1),
+                     torch.stack([s[:, 2], s[:, 4], s[:, 5]], -1)], -2)
+    return torch.linalg.solve(M, A.transpose(1, 2)).transpose(1, 2)
+
+
+def refine_joint(problem, ubis, trans, params, fixed, steps=50, lr=1e-3, scales=None):
+    """Joint refinement: each step is one adjoint pass; the UBIs take the
+    Gauss-Newton (exact least-squares) update for the current geometry and
+    the geometry parameters an Adam step on dL/dparams (per-parameter step
+    scales in ``scales``).  hkl and the peak lists stay fixed.  Returns
+    (ubis, params, loss history)."""
+    ubis = ubis.detach().clone()
+    names = sorted(params)
+    scale = {k: float((scales or {}).get(k, 1.0)) for k in names}
+    # optimise dimensionless offsets x_k with value = start + scale_k * x_k
+    start = {k: params[k].detach().clone() for k in names}
+    x = {k: torch.zeros((), dtype=torch.float64, device=ubis.device, requires_grad=True)
+         for k in names}
+    opt = torch.optim.Adam([x[k] for k in names], lr=lr)
+    history = []
+    for _ in range(steps):
+        cur = {k: start[k] + scale[k] * x[k] for k in names}
+        geo = pack_geometry(cur, fixed)
+        loss, gu, gt, gg, sums = problem.evaluate(geo, ubis, trans, want_sums=True)
+        history.append(float(loss))
+        ubis = gauss_newton_ubis(sums)
+        opt.zero_grad()
+        geo.backward(gg)
+        opt.step()
+    final = {k: (start[k] + scale[k] * x[k]).detach() for k in names}
+    return ubis, final, history

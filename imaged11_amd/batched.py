@@ -33,10 +33,12 @@ def _ready():
     return lib
 
 
-def score_many(ubis, gv, tol, peaks_per_lane=0, out=None):
+def score_many(ubis, gv, tol, peaks_per_lane=0, out=None, exact=False):
     """counts[g] = score(ubis[g], gv, tol) for every grain g.
 
-    ubis (K,3,3) float64, gv (N,3) float64 -> int32 (K,)."""
+    ubis (K,3,3) float64, gv (N,3) float64 -> int32 (K,).  exact=True uses
+    the reference's arithmetic for every pair instead of the certified FMA
+    path; both return the reference's counts."""
     lib = _ready()
     _need(ubis, torch.float64, "ubis", (None, 3, 3))
     _need(gv, torch.float64, "gv", (None, 3))
@@ -44,8 +46,9 @@ def score_many(ubis, gv, tol, peaks_per_lane=0, out=None):
     if out is None:
         out = torch.empty(K, dtype=torch.int32, device=gv.device)
     _need(out, torch.int32, "out", (K,))
+    mode = _lib.SCORE_EXACT if exact else _lib.SCORE_CERTIFIED
     _lib.check(lib.gs_score_many(dptr(ubis), K, dptr(gv), gv.shape[0], float(tol), dptr(out),
-                                 int(peaks_per_lane), stream_ptr()), "score_many")
+                                 int(peaks_per_lane), mode, stream_ptr()), "score_many")
     return out
 
 
@@ -180,8 +183,76 @@ def refine_lists(ubis, gv, idx, off, tol, n_iter=2, order=None, lanes_per_grain=
     u = ubis if inplace else ubis.clone()
     n = torch.empty((n_iter, K), dtype=torch.int32, device=gv.device)
     m = torch.empty((n_iter, K), dtype=torch.float64, device=gv.device)
+    queue = torch.empty(1, dtype=torch.int32, device=gv.device)
     _lib.check(lib.gs_refine_lists(dptr(u), K, dptr(gv), gv.shape[0], dptr(idx), idx.numel(),
-                                   dptr(off), dptr(order),
-                                   float(tol), int(n_iter), dptr(n), dptr(m), int(lanes_per_grain),
-                                   int(waves_per_cu), stream_ptr()), "refine_lists")
+                                   dptr(off), dptr(order), float(tol), int(n_iter), dptr(n),
+                                   dptr(m), int(lanes_per_grain), int(waves_per_cu), dptr(queue),
+                                   stream_ptr()), "refine_lists")
     return u, n, m
+
+
+def _mat_arg(mat):
+    return None if mat is None else darr(np.ravel(mat))
+
+
+def gv_assign_many(xl, so, co, wvln, ubis, trans, tol, drlv2, labels, grain_labels=None,
+                   wedge=0.0, chi=0.0, mat=None, out=None):
+    """For g = 0..K-1 in order: compute_gv with grain g's translation, then
+    score_and_assign with label grain_labels[g] (refinegrains.assignlabels,
+    ``ImageD11/refinegrains.py:718-734``), fused in one launch.  xl (N,3)
+    lab positions, so/co (N,) sin/cos of omega*omegasign; drlv2, labels
+    updated in place.  Returns the int32 (K,) per-call counts."""
+    lib = _ready()
+    _need(xl, torch.float64, "xlylzl", (None, 3))
+    N = xl.shape[0]
+    _need(so, torch.float64, "so", (N,))
+    _need(co, torch.float64, "co", (N,))
+    _need(ubis, torch.float64, "ubis", (None, 3, 3))
+    K = ubis.shape[0]
+    _need(trans, torch.float64, "trans", (K, 3))
+    _need(drlv2, torch.float64, "drlv2", (N,))
+    _need(labels, torch.int32, "labels", (N,))
+    if grain_labels is None:
+        grain_labels = torch.arange(K, dtype=torch.int32, device=xl.device)
+    _need(grain_labels, torch.int32, "grain_labels", (K,))
+    if out is None:
+        out = torch.empty(K, dtype=torch.int32, device=xl.device)
+    _lib.check(lib.gs_gv_assign_many(dptr(xl), dptr(so), dptr(co), N, float(wvln), float(wedge),
+                                     float(chi), _mat_arg(mat), dptr(ubis), dptr(trans),
+                                     dptr(grain_labels), K, float(tol), dptr(drlv2), dptr(labels),
+                                     dptr(out), stream_ptr()), "gv_assign_many")
+    return out
+
+
+def lists_from_labels(labels, K):
+    """CSR peak lists of grains 0..K-1 from per-peak labels (peaks in
+    ascending order inside each list, label -1 and others dropped):
+    returns (idx int32, off int64 (K+1,), owner int32)."""
+    lab = labels.long()
+    keep = (lab >= 0) & (lab < K)
+    peaks = torch.nonzero(keep, as_tuple=False).flatten()
+    order = torch.argsort(lab[peaks], stable=True)
+    idx = peaks[order].to(torch.int32)
+    counts = torch.bincount(lab[peaks], minlength=K)
+    off = torch.zeros(K + 1, dtype=torch.int64, device=labels.device)
+    off[1:] = torch.cumsum(counts, 0)
+    owner = torch.repeat_interleave(torch.arange(K, dtype=torch.int32, device=labels.device),
+                                    counts)
+    return idx, off, owner
+
+
+def gv_lists(xl, so, co, wvln, trans, idx, owner, wedge=0.0, chi=0.0, mat=None):
+    """g-vector of every list entry with its grain's translation: (n_idx, 3)."""
+    lib = _ready()
+    _need(xl, torch.float64, "xlylzl", (None, 3))
+    N = xl.shape[0]
+    _need(so, torch.float64, "so", (N,))
+    _need(co, torch.float64, "co", (N,))
+    _need(trans, torch.float64, "trans", (None, 3))
+    _need(idx, torch.int32, "idx", (None,))
+    _need(owner, torch.int32, "owner", (idx.numel(),))
+    out = torch.empty((idx.numel(), 3), dtype=torch.float64, device=xl.device)
+    _lib.check(lib.gs_gv_lists(dptr(xl), dptr(so), dptr(co), N, float(wvln), float(wedge),
+                               float(chi), _mat_arg(mat), dptr(trans), trans.shape[0], dptr(idx),
+                               dptr(owner), idx.numel(), dptr(out), stream_ptr()), "gv_lists")
+    return out

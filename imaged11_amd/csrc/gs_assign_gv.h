@@ -1,0 +1,174 @@
+// compute_gv for each grain's translation fused with score_and_assign, for
+// every grain in order: the loop of refinegrains.assignlabels
+// (ImageD11/refinegrains.py:718-734), one peak per lane, the (drlv2,
+// label) state of a peak carried through all grains in registers.
+//
+// Fast path.  With M_k = Omega(omega_k) . mat^T, the reference's g-vector
+// (src/cdiffraction.c:149-166) is, in exact arithmetic,
+//     G = ds (a_k - t) / |a_k - t| - c_k,
+//     a_k = M_k . xl_k,  c_k = ds M_k . (1, 0, 0),
+// because d = xl - mat.R(omega).t and M_k.mat.R(omega) = Omega.R = I while
+// M_k is orthogonal.  a_k, c_k are per-peak constants, so a pair costs one
+// subtraction, a norm and a scaling before h = UBI.G.
+//
+// Certification.  Both the reference's G and the fast G are within
+// 90 u ds (u = 2^-53, ds = 1/wavelength) of the exact G in every component
+// when |t| <= |xl|/2 (so a - t and xl - o cannot cancel): k0 = ds (d0/|d|
+// - 1) is the only subtraction of like quantities and |d0/|d|| <= 1, and
+// everything else is a few rounded products of magnitude <= 2 ds.  h =
+// UBI.G then moves by at most ||u_i||_1 (90 u ds) + 6.7e-16 A_i per row,
+// and drlv2 by the sum of that
+// over rows (1-Lipschitz, see k_score_many_fast) plus 5e-16 from the sums
+// of squares.  E = U1 (1e-13 ds + 1e-15 Gmax) + 1e-15 + 2e-15 tol^2 bounds
+// it with margin (Gmax = 2 ds bounds |G|).  A pair whose fast drlv2 is
+// >= tol^2 + E certainly fails `drlv2 < tol^2`, so its only effect is the
+// reference's else branch (label == this label -> -1), which needs no
+// arithmetic.  Every other pair is recomputed with the reference's own
+// compute_gv and score arithmetic (gs_gv_one, gs_drlv2) and the reference's
+// rule is applied to that exact value, so labels and stored drlv2 are
+// those of the reference loop.  A NaN or infinite fast drlv2 is never
+// ">= tol^2 + E", so non-finite inputs take the exact route; a grain with
+// |t| > |xl|/2 for some peak of the wave takes it for the whole wave.
+#pragma once
+#include "gs_kernels.h"
+
+template <int P>
+__global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
+    const double* __restrict__ xl, const double* __restrict__ so_in,
+    const double* __restrict__ co_in, int ng, gs_goniometer gp,
+    const double* __restrict__ ubis, const double* __restrict__ trans,
+    const int* __restrict__ glabels, int K, double tolsq, double* __restrict__ drlv2,
+    int* __restrict__ labels, int* __restrict__ counts) {
+    __shared__ int lcnt[GS_GRAIN_CHUNK];
+    __shared__ double smat[9];
+    __shared__ double sds;
+    gs_block_setup(gp, smat, &sds);
+    double mat[9];
+    for (int j = 0; j < 9; j++) mat[j] = smat[j];
+    const double ds = sds;
+
+    // per-peak constants of the fast path and the (drlv2, label) state
+    double ax[P], ay[P], az[P], cx[P], cy[P], cz[P], d[P];
+    int lab[P];
+    double n2min = __builtin_inf();
+    const long long base = (long long)blockIdx.x * (GS_BLOCK * P) + threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const long long k = base + (long long)p * GS_BLOCK;
+        ax[p] = ay[p] = az[p] = cx[p] = cy[p] = cz[p] = 0.0;
+        d[p] = 0.0;
+        lab[p] = -1;
+        if (k < ng) {
+            const double so = so_in[k], co = co_in[k];
+            const double x0 = xl[3 * k], x1 = xl[3 * k + 1], x2 = xl[3 * k + 2];
+            // w = mat^T xl, a = Omega w ; mat^T x_hat = first row of mat
+            const double w0 = mat[0] * x0 + mat[3] * x1 + mat[6] * x2;
+            const double w1 = mat[1] * x0 + mat[4] * x1 + mat[7] * x2;
+            const double w2 = mat[2] * x0 + mat[5] * x1 + mat[8] * x2;
+            ax[p] = co * w0 + so * w1;
+            ay[p] = -so * w0 + co * w1;
+            az[p] = w2;
+            cx[p] = ds * (co * mat[0] + so * mat[1]);
+            cy[p] = ds * (-so * mat[0] + co * mat[1]);
+            cz[p] = ds * mat[2];
+            d[p] = drlv2[k];
+            lab[p] = labels[k];
+            const double n2 = x0 * x0 + x1 * x1 + x2 * x2;
+            n2min = n2 == n2 ? fmin(n2min, n2) : n2min;
+        }
+    }
+    for (int o = GS_WAVE / 2; o > 0; o >>= 1) n2min = fmin(n2min, __shfl_xor(n2min, o, GS_WAVE));
+    const double gmax = 2.0 * fabs(ds);
+    const int lane = threadIdx.x & (GS_WAVE - 1);
+    for (int c0 = 0; c0 < K; c0 += GS_GRAIN_CHUNK) {
+        const int c1 = min(K, c0 + GS_GRAIN_CHUNK);
+        for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) lcnt[i] = 0;
+        __syncthreads();
+        for (int g = c0; g < c1; g++) {
+            const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)g);
+            const double t0 = trans[3 * (size_t)g], t1 = trans[3 * (size_t)g + 1],
+                         t2 = trans[3 * (size_t)g + 2];
+            const int L = glabels[g];
+            const double u1 = fabs(u.u00) + fabs(u.u01) + fabs(u.u02) + fabs(u.u10) +
+                              fabs(u.u11) + fabs(u.u12) + fabs(u.u20) + fabs(u.u21) + fabs(u.u22);
+            double e = u1 * (1e-13 * fabs(ds) + 1e-15 * gmax) + 1e-15 + 2e-15 * tolsq;
+            const double tt = t0 * t0 + t1 * t1 + t2 * t2;
+            if (!(u1 * gmax < 562949953421312.0) || !(4.0 * tt <= n2min)) e = __builtin_inf();
+            const double hi = tolsq + e;
+            int cnt = 0;
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                const double e0 = ax[p] - t0, e1 = ay[p] - t1, e2 = az[p] - t2;
+                const double inv = ds / sqrt(__builtin_fma(e2, e2, __builtin_fma(e1, e1, e0 * e0)));
+                const double G0 = __builtin_fma(e0, inv, -cx[p]);
+                const double G1 = __builtin_fma(e1, inv, -cy[p]);
+                const double G2 = __builtin_fma(e2, inv, -cz[p]);
+                const double sf = gs_drlv2_fma(u, G0, G1, G2);
+                // candidates (and non-finite values) take the exact route
+                const bool cand = !(sf >= hi);
+                bool take = false;
+                if (cand) {
+                    const long long k = base + (long long)p * GS_BLOCK;
+                    if (k < ng) {
+                        double g0, g1, g2;
+                        gs_gv_one(xl[3 * k], xl[3 * k + 1], xl[3 * k + 2], so_in[k], co_in[k],
+                                  mat, ds, t0, t1, t2, g0, g1, g2);
+                        const double se = gs_drlv2(u, g0, g1, g2);
+                        if ((se < tolsq) && (se < d[p])) {
+                            take = true;
+                            lab[p] = L;
+                            d[p] = se;
+                        } else if (lab[p] == L) {
+                            lab[p] = -1;
+                        }
+                    }
+                } else if (lab[p] == L) {
+                    lab[p] = -1;
+                }
+                cnt += __popcll(__ballot(take));
+            }
+            if (lane == 0 && cnt) atomicAdd(&lcnt[g - c0], cnt);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) {
+            const int c = lcnt[i];
+            if (c) atomicAdd(&counts[c0 + i], c);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const long long k = base + (long long)p * GS_BLOCK;
+        if (k < ng) {
+            drlv2[k] = d[p];
+            labels[k] = lab[p];
+        }
+    }
+}
+
+// g-vector of every CSR list entry with its grain's translation
+// (compute_gv per grain, gathered): out[q] = G(peak idx[q], t_owner[q]).
+// Used to refine grains whose g-vectors depend on their position.
+__global__ __launch_bounds__(GS_BLOCK) void k_gv_lists(
+    const double* __restrict__ xl, const double* __restrict__ so_in,
+    const double* __restrict__ co_in, int ng, gs_goniometer gp, const double* __restrict__ trans,
+    int K, const int* __restrict__ idx, const int* __restrict__ owner, long long n_idx,
+    double* __restrict__ out) {
+    __shared__ double smat[9];
+    __shared__ double sds;
+    gs_block_setup(gp, smat, &sds);
+    const long long q = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
+    if (q >= n_idx) return;
+    const int k = idx[q], g = owner[q];
+    double g0 = __builtin_nan(""), g1 = g0, g2 = g0;
+    if (k >= 0 && k < ng && g >= 0 && g < K) {
+        double mat[9];
+        for (int j = 0; j < 9; j++) mat[j] = smat[j];
+        gs_gv_one(xl[3 * (size_t)k], xl[3 * (size_t)k + 1], xl[3 * (size_t)k + 2], so_in[k],
+                  co_in[k], mat, sds, trans[3 * (size_t)g], trans[3 * (size_t)g + 1],
+                  trans[3 * (size_t)g + 2], g0, g1, g2);
+    }
+    out[3 * q] = g0;
+    out[3 * q + 1] = g1;
+    out[3 * q + 2] = g2;
+}

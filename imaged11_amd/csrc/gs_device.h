@@ -58,6 +58,33 @@ GS_HD __forceinline__ double gs_drlv2(const gs_ubi& u, double x, double y, doubl
     return gs_drlv2(u, x, y, z, a, b, c);
 }
 
+// The certified variant of gs_drlv2: h with FMA, rounding with rint
+// (v_rndne, equal to the MAGIC rounding for |h| < 2^51), squares summed
+// with FMA.  Used only together with gs_cert_window.
+GS_HD __forceinline__ double gs_drlv2_fma(const gs_ubi& u, double x, double y, double z) {
+    double h0 = __builtin_fma(u.u02, z, __builtin_fma(u.u01, y, u.u00 * x));
+    double h1 = __builtin_fma(u.u12, z, __builtin_fma(u.u11, y, u.u10 * x));
+    double h2 = __builtin_fma(u.u22, z, __builtin_fma(u.u21, y, u.u20 * x));
+    double t0 = h0 - rint(h0), t1 = h1 - rint(h1), t2 = h2 - rint(h2);
+    return __builtin_fma(t2, t2, __builtin_fma(t1, t1, t0 * t0));
+}
+
+// Half-width E of the window around tol^2 inside which gs_drlv2_fma cannot
+// decide `gs_drlv2 < tol^2`, for peaks whose components are bounded by
+// gmax in magnitude (derivation at k_score_many_fast).  +inf when the
+// bound does not apply (U1*gmax >= 2^49, or a non-finite gmax).
+GS_HD __forceinline__ double gs_cert_window(const gs_ubi& u, double gmax, double tolsq) {
+    const double u1 = fabs(u.u00) + fabs(u.u01) + fabs(u.u02) + fabs(u.u10) + fabs(u.u11) +
+                      fabs(u.u12) + fabs(u.u20) + fabs(u.u21) + fabs(u.u22);
+    const double span = u1 * gmax;
+    if (!(span < 562949953421312.0)) return __builtin_inf();  // 2^49
+    return 1e-15 * span + 1e-15 + 2e-15 * tolsq;
+}
+
+GS_HD __forceinline__ double gs_abs_finite(double a) {
+    return a == a ? fabs(a) : 0.0;  // NaN rows never count on either path
+}
+
 // ---------------------------------------------------------------------------
 // 3x3 inverse in place, the cofactor formula and operation order of
 // src/closest.c:37-76; returns 0, or -1 when det == 0 exactly.

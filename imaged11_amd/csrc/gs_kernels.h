@@ -95,6 +95,95 @@ __global__ __launch_bounds__(GS_BLOCK) void k_score_many(const double* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Certified fast score: the same counts as k_score_many for every input.
+//
+// h is formed with FMA and rounded with v_rndne (3 + 2 ops per row instead
+// of 5 + 3), the squares summed with FMA.  Let S_e be the sum the
+// reference computes (separately rounded products, MAGIC rounding) and S_f
+// this one.  With A_i = sum_j |u_ij| |g_j|, each h differs from the exact
+// dot product by at most gamma_3 A_i (gamma_3 = 3u/(1-3u), u = 2^-53), so
+// |h_f - h_e| <= 6.7e-16 A_i.  The
+// distance to the nearest integer is 1-Lipschitz and at most 1/2, so
+// x -> (x - round x)^2 changes by at most |dx| (it is continuous across
+// the half-integers): the rows move S by at most 6.7e-16 sum_i A_i <=
+// 6.7e-16 U1 Gmax, with U1 = sum |u_ij| and Gmax = max |g_j| over the
+// wave's peaks.  Both sums of squares are <= 3/4, so rounding them adds at
+// most 5e-16 in absolute terms.  E = 1e-15 U1 Gmax + 1e-15 + 2e-15 tol^2
+// covers all of it with margin (the last term absorbs the rounding of
+// tol^2 +- E).  S_f < tol^2 - E proves S_e < tol^2; S_f >= tol^2 + E proves
+// the opposite; lanes in between are recomputed with the reference
+// arithmetic.
+// v_rndne equals the MAGIC rounding for |h| < 2^51, so a (wave, grain)
+// with U1 Gmax >= 2^49 (or a non-finite g) takes the exact path entirely.
+__device__ __forceinline__ double gs_wave_max(double v) {
+    for (int o = GS_WAVE / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, GS_WAVE));
+    return v;
+}
+
+template <int P>
+__global__ __launch_bounds__(GS_BLOCK) void k_score_many_fast(
+    const double* __restrict__ ubis, int K, int gsplit, int nsplit,
+    const double* __restrict__ gv, int ng, double tolsq, int* __restrict__ counts) {
+    __shared__ int lcnt[GS_GRAIN_CHUNK];
+    const int tile = gs_xcd_remap(blockIdx.x, gridDim.x);
+    const int chunk = tile / nsplit;
+    const int split = tile - chunk * nsplit;
+    const int g0 = split * gsplit;
+    const int g1 = min(K, g0 + gsplit);
+    for (int i = threadIdx.x; i < g1 - g0; i += GS_BLOCK) lcnt[i] = 0;
+
+    double x[P], y[P], z[P];
+    double gm = 0.0;
+    const long long base = (long long)chunk * (GS_BLOCK * P) + threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        long long k = base + (long long)p * GS_BLOCK;
+        if (k < ng) {
+            x[p] = gv[3 * k + 0];
+            y[p] = gv[3 * k + 1];
+            z[p] = gv[3 * k + 2];
+        } else {
+            x[p] = y[p] = z[p] = __builtin_nan("");
+        }
+        gm = fmax(gm, fmax(gs_abs_finite(x[p]), fmax(gs_abs_finite(y[p]), gs_abs_finite(z[p]))));
+    }
+    const double gmax = gs_wave_max(gm);  // +inf if any peak of the wave is infinite
+    __syncthreads();
+    const int lane = threadIdx.x & (GS_WAVE - 1);
+    gs_ubi un = gs_load_ubi(ubis + 9 * (size_t)g0);
+    for (int g = g0; g < g1; g++) {
+        const gs_ubi u = un;
+        if (g + 1 < g1) un = gs_load_ubi(ubis + 9 * (size_t)(g + 1));  // prefetch
+        const double e = gs_cert_window(u, gmax, tolsq);
+        const double lo = tolsq - e, hi = tolsq + e;
+        int c = 0;
+        unsigned long long undecided = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const double s = gs_drlv2_fma(u, x[p], y[p], z[p]);
+            const unsigned long long in = __ballot(s < lo);
+            undecided |= __ballot(s < hi) ^ in;
+            c += __popcll(in);
+        }
+        if (undecided) {  // wave-uniform and rare: settle undecided lanes exactly
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                const double s = gs_drlv2_fma(u, x[p], y[p], z[p]);
+                const bool und = (s < hi) && !(s < lo);
+                const double se = gs_drlv2(u, x[p], y[p], z[p]);
+                c += __popcll(__ballot(und && se < tolsq));
+            }
+        }
+        if (lane == 0 && c) atomicAdd(&lcnt[g - g0], c);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < g1 - g0; i += GS_BLOCK) {
+        int c = lcnt[i];
+        if (c) atomicAdd(&counts[g0 + i], c);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // score_and_assign for K grains in order (src/closest.c:366-397 called once
 // per grain, as refinegrains.assignlabels and indexer.fight_over_peaks do).
 // Each lane owns P peaks and runs every grain in order on their (drlv2,

@@ -22,6 +22,8 @@
 #include <vector>
 
 #include "gs_kernels.h"
+#include "gs_adjoint.h"
+#include "gs_assign_gv.h"
 #include "../../include/gscore.h"
 
 namespace {
@@ -168,7 +170,7 @@ int check_ng(long long ng) {
 // ---- launchers shared by both families ------------------------------------
 
 int launch_score_many(const double* ubis, int K, const double* gv, int ng, double tol, int* counts,
-                      int peaks_per_lane, hipStream_t st) {
+                      int peaks_per_lane, int mode, hipStream_t st) {
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
     if (K < 0 || ng < 0) return set_err(GS_ERR_ARG, "negative size K=%d ng=%d", K, ng);
@@ -176,7 +178,7 @@ int launch_score_many(const double* ubis, int K, const double* gv, int ng, doubl
     if (K == 0 || ng == 0) return GS_OK;
     const double tolsq = tol * tol;
     int P = peaks_per_lane;
-    if (P != 1 && P != 2 && P != 4 && P != 8) P = 4;
+    if (P != 1 && P != 2 && P != 4 && P != 8 && P != 16) P = 8;
     const long long nchunks = (ng + (long long)GS_BLOCK * P - 1) / ((long long)GS_BLOCK * P);
     // enough tiles to fill every CU several times over; a split holds at
     // most GS_GRAIN_CHUNK grains (the LDS counter tile)
@@ -188,11 +190,23 @@ int launch_score_many(const double* ubis, int K, const double* gv, int ng, doubl
     nsplit = (K + gsplit - 1) / gsplit;
     const long long nb = nchunks * nsplit;
     if (nb > 0x7fffffffLL) return set_err(GS_ERR_ARG, "grid too large");
-    switch (P) {
-        case 1: hipLaunchKernelGGL(k_score_many<1>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-        case 2: hipLaunchKernelGGL(k_score_many<2>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-        case 8: hipLaunchKernelGGL(k_score_many<8>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-        default: hipLaunchKernelGGL(k_score_many<4>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+    const dim3 grid((unsigned)nb), block(GS_BLOCK);
+    if (mode == GS_SCORE_EXACT) {
+        switch (P) {
+            case 1: hipLaunchKernelGGL(k_score_many<1>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 2: hipLaunchKernelGGL(k_score_many<2>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 4: hipLaunchKernelGGL(k_score_many<4>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 16: hipLaunchKernelGGL(k_score_many<16>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            default: hipLaunchKernelGGL(k_score_many<8>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+        }
+    } else {
+        switch (P) {
+            case 1: hipLaunchKernelGGL(k_score_many_fast<1>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 2: hipLaunchKernelGGL(k_score_many_fast<2>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 4: hipLaunchKernelGGL(k_score_many_fast<4>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 16: hipLaunchKernelGGL(k_score_many_fast<16>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            default: hipLaunchKernelGGL(k_score_many_fast<8>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+        }
     }
     GS_LAUNCHED();
     return GS_OK;
@@ -283,6 +297,35 @@ int launch_gv(const double* xl, const double* omega, const double* so, const dou
     return GS_OK;
 }
 
+int check_refine_args(int K, int ng, long long n_idx, int n_iter) {
+    if (K < 0 || n_iter < 0 || ng < 0 || n_idx < 0)
+        return set_err(GS_ERR_ARG, "negative K, n_iter, ng or n_idx");
+    return GS_OK;
+}
+
+int launch_refine_lists(DeviceCtx* ctx, double* ubis, int K, const double* gv, int ng,
+                        const int* idx, long long n_idx, const long long* off, const int* order,
+                        double tol, int n_iter, int* n_out, double* mean_out, int lanes_per_grain,
+                        int waves_per_cu, int* queue, hipStream_t st) {
+    const double tolsq = tol * tol;
+    int lpg = lanes_per_grain;
+    if (lpg != 8 && lpg != 16 && lpg != 32 && lpg != 64) lpg = 16;
+    int wpc = waves_per_cu > 0 ? waves_per_cu : 8;
+    const int gpw = GS_WAVE / lpg;
+    long long waves_needed = (K + gpw - 1) / gpw;
+    long long waves = std::min<long long>(waves_needed, (long long)wpc * ctx->cus);
+    int nb = (int)std::max<long long>(1, (waves + GS_BLOCK / GS_WAVE - 1) / (GS_BLOCK / GS_WAVE));
+    const dim3 grid(nb), block(GS_BLOCK);
+    switch (lpg) {
+        case 8: hipLaunchKernelGGL(k_refine_lists<8>, grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        case 32: hipLaunchKernelGGL(k_refine_lists<32>, grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        case 64: hipLaunchKernelGGL(k_refine_lists<64>, grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        default: hipLaunchKernelGGL(k_refine_lists<16>, grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+    }
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -339,7 +382,7 @@ int score(double ubi[3][3], double gv[][3], double tol, int ng) {
         set_err(GS_ERR_HIP, "score: copy to device failed");
         return -1;
     }
-    if (launch_score_many(du, 1, dg, ng, tol, dn, 4, st) != GS_OK) return -1;
+    if (launch_score_many(du, 1, dg, ng, tol, dn, 0, GS_SCORE_CERTIFIED, st) != GS_OK) return -1;
     if (hipMemcpyAsync(&h, dn, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
         set_err(GS_ERR_HIP, "score: copy back failed");
@@ -470,14 +513,36 @@ void gs_selftest_sincos_host(const double* x, double* s, double* c, int n) {
     for (int i = 0; i < n; i++) gs_sincos(x[i], &s[i], &c[i]);
 }
 
+// The certified score decision (gs_drlv2_fma + gs_cert_window) compiled
+// for the host, next to the reference arithmetic (gs_drlv2): for each row
+// of gv, out[k] = 1 if `drlv2 < tol^2` is certain, 0 if certainly false,
+// 2 if undecided, plus 4 when the reference arithmetic says true.  gmax is
+// the bound on |gv| the window is computed for.  Checking aid only.
+void gs_selftest_certified_host(const double* ubi, const double* gv, int n, double tol,
+                                double gmax, int* out) {
+    const gs_ubi u = gs_load_ubi(ubi);
+    const double tolsq = tol * tol;
+    const double e = gs_cert_window(u, gmax, tolsq);
+    const double lo = tolsq - e, hi = tolsq + e;
+    for (int k = 0; k < n; k++) {
+        const double* g = gv + 3 * (size_t)k;
+        double s = gs_drlv2_fma(u, g[0], g[1], g[2]);
+        int d = s < lo ? 1 : (s < hi ? 2 : 0);
+        if (gs_drlv2(u, g[0], g[1], g[2]) < tolsq) d |= 4;
+        out[k] = d;
+    }
+}
+
 int gs_clear_error(void) {
     g_err.clear();
     return GS_OK;
 }
 
 int gs_score_many(const double* ubis, int K, const double* gv, int ng, double tol, int* counts,
-                  int peaks_per_lane, gs_stream_t stream) {
-    return launch_score_many(ubis, K, gv, ng, tol, counts, peaks_per_lane, S(stream));
+                  int peaks_per_lane, int mode, gs_stream_t stream) {
+    if (mode != GS_SCORE_CERTIFIED && mode != GS_SCORE_EXACT)
+        return set_err(GS_ERR_ARG, "gs_score_many: unknown mode %d", mode);
+    return launch_score_many(ubis, K, gv, ng, tol, counts, peaks_per_lane, mode, S(stream));
 }
 
 int gs_assign_many(const double* ubis, const int* grain_labels, int K, const double* gv, int ng,
@@ -546,29 +611,172 @@ int gs_sf2gv(const double* s, const double* f, const double* omega, const double
 
 int gs_refine_lists(double* ubis, int K, const double* gv, int ng, const int* idx, long long n_idx,
                     const long long* off, const int* order, double tol, int n_iter, int* n_out,
-                    double* mean_out, int lanes_per_grain, int waves_per_cu, gs_stream_t stream) {
+                    double* mean_out, int lanes_per_grain, int waves_per_cu, int* queue_ws,
+                    gs_stream_t stream) {
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
-    if (K < 0 || n_iter < 0 || ng < 0 || n_idx < 0)
-        return set_err(GS_ERR_ARG, "negative K, n_iter, ng or n_idx");
+    GS_CHECK(check_refine_args(K, ng, n_idx, n_iter));
     if (K == 0 || n_iter == 0) return GS_OK;
     hipStream_t st = S(stream);
-    int* queue = ctx->counters;
+    int* queue = queue_ws ? queue_ws : ctx->counters;
     GS_CHECK(fill_i32(queue, 0, 1, st));
-    const double tolsq = tol * tol;
-    int lpg = lanes_per_grain;
-    if (lpg != 8 && lpg != 16 && lpg != 32 && lpg != 64) lpg = 16;
-    int wpc = waves_per_cu > 0 ? waves_per_cu : 8;
-    const int gpw = GS_WAVE / lpg;
-    long long waves_needed = (K + gpw - 1) / gpw;
-    long long waves = std::min<long long>(waves_needed, (long long)wpc * ctx->cus);
-    int nb = (int)std::max<long long>(1, (waves + GS_BLOCK / GS_WAVE - 1) / (GS_BLOCK / GS_WAVE));
-    switch (lpg) {
-        case 8: hipLaunchKernelGGL(k_refine_lists<8>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
-        case 32: hipLaunchKernelGGL(k_refine_lists<32>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
-        case 64: hipLaunchKernelGGL(k_refine_lists<64>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
-        default: hipLaunchKernelGGL(k_refine_lists<16>, dim3(nb), dim3(GS_BLOCK), 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+    return launch_refine_lists(ctx, ubis, K, gv, ng, idx, n_idx, off, order, tol, n_iter, n_out,
+                               mean_out, lanes_per_grain, waves_per_cu, queue, st);
+}
+
+// A captured multi-iteration refinement: one persistent refine launch per
+// iteration, each on its own work-queue counter, all counters cleared by a
+// fill kernel at the head of the graph.
+struct gs_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int gs_refine_graph_create(gs_graph_t* out, double* ubis, int K, const double* gv, int ng,
+                           const int* idx, long long n_idx, const long long* off, const int* order,
+                           double tol, int n_iter, int* n_out, double* mean_out,
+                           int lanes_per_grain, int waves_per_cu, int* queue_ws,
+                           gs_stream_t stream) {
+    if (!out) return set_err(GS_ERR_ARG, "gs_refine_graph_create: null output handle");
+    *out = nullptr;
+    if (!queue_ws) return set_err(GS_ERR_ARG, "gs_refine_graph_create: queue_ws (n_iter ints) needed");
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_refine_args(K, ng, n_idx, n_iter));
+    if (K == 0 || n_iter == 0) return set_err(GS_ERR_ARG, "gs_refine_graph_create: empty problem");
+    hipStream_t st = S(stream);
+    if (st == nullptr) return set_err(GS_ERR_ARG, "gs_refine_graph_create: capture needs a non-null stream");
+    GS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = fill_i32(queue_ws, 0, n_iter, st);
+    for (int it = 0; rc == GS_OK && it < n_iter; it++)
+        rc = launch_refine_lists(ctx, ubis, K, gv, ng, idx, n_idx, off, order, tol, 1,
+                                 n_out + (size_t)it * K, mean_out + (size_t)it * K,
+                                 lanes_per_grain, waves_per_cu, queue_ws + it, st);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc != GS_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
     }
+    if (e != hipSuccess) return set_err(GS_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(graph);
+        return set_err(GS_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    gs_graph_s* g = new gs_graph_s;
+    g->graph = graph;
+    g->exec = exec;
+    *out = g;
+    return GS_OK;
+}
+
+int gs_graph_launch(gs_graph_t g, gs_stream_t stream) {
+    if (!g || !g->exec) return set_err(GS_ERR_ARG, "gs_graph_launch: invalid graph");
+    GS_HIP(hipGraphLaunch(g->exec, S(stream)));
+    return GS_OK;
+}
+
+int gs_graph_destroy(gs_graph_t g) {
+    if (!g) return GS_OK;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return GS_OK;
+}
+
+// ---- adjoint of the joint objective ------------------------------------------
+
+int gs_joint_hkl(const double* sc, const double* fc, const double* so, const double* co, int npk,
+                 const double* geo, const double* ubis, const double* trans, int K, const int* idx,
+                 const int* owner, long long n_idx, int* hkl, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (npk < 0 || K < 0 || n_idx < 0) return set_err(GS_ERR_ARG, "gs_joint_hkl: negative size");
+    if (n_idx == 0) return GS_OK;
+    hipLaunchKernelGGL(k_joint_hkl, dim3(blocks_for(n_idx, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
+                       sc, fc, so, co, npk, geo, ubis, trans, K, idx, owner, n_idx, hkl);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_joint_adjoint(const double* sc, const double* fc, const double* so, const double* co,
+                     int npk, const double* geo, const double* ubis, const double* trans, int K,
+                     const int* idx, long long n_idx, const long long* off, const int* hkl,
+                     const int* order, int gspace, double* loss_g, double* grad_ubi,
+                     double* grad_t, double* geo_part, double* nsums, double* out,
+                     int lanes_per_grain, int* queue_ws, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (npk < 0 || K < 0 || n_idx < 0) return set_err(GS_ERR_ARG, "gs_joint_adjoint: negative size");
+    hipStream_t st = S(stream);
+    int* queue = queue_ws ? queue_ws : ctx->counters + 1;
+    GS_CHECK(fill_i32(queue, 0, 1, st));
+    int lpg = lanes_per_grain;
+    if (lpg != 16 && lpg != 32 && lpg != 64) lpg = 16;
+    const int gpw = GS_WAVE / lpg;
+    long long waves = std::min<long long>((K + gpw - 1) / gpw, 8LL * ctx->cus);
+    int nb = (int)std::max<long long>(1, (waves + GS_BLOCK / GS_WAVE - 1) / (GS_BLOCK / GS_WAVE));
+    const dim3 grid(nb), block(GS_BLOCK);
+#define GS_ADJ_LAUNCH(L, GSP, NS)                                                              \
+    hipLaunchKernelGGL((k_joint_adjoint<L, GSP, NS>), grid, block, 0, st, sc, fc, so, co, npk, \
+                       geo, ubis, trans, K, idx, n_idx, off, hkl, order, loss_g, grad_ubi,     \
+                       grad_t, geo_part, nsums, queue)
+#define GS_ADJ_LPG(GSP, NS)                         \
+    switch (lpg) {                                  \
+        case 32: GS_ADJ_LAUNCH(32, GSP, NS); break; \
+        case 64: GS_ADJ_LAUNCH(64, GSP, NS); break; \
+        default: GS_ADJ_LAUNCH(16, GSP, NS); break; \
+    }
+    if (K > 0) {
+        if (gspace) {
+            if (nsums) { GS_ADJ_LPG(true, true) } else { GS_ADJ_LPG(true, false) }
+        } else {
+            if (nsums) { GS_ADJ_LPG(false, true) } else { GS_ADJ_LPG(false, false) }
+        }
+        GS_LAUNCHED();
+    }
+#undef GS_ADJ_LPG
+#undef GS_ADJ_LAUNCH
+    hipLaunchKernelGGL(k_joint_finalize, dim3(1), dim3(GS_WAVE), 0, st, geo_part, loss_g, K, geo,
+                       out);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+// ---- compute_gv per grain fused with score_and_assign -------------------------
+
+int gs_gv_assign_many(const double* xlylzl, const double* so, const double* co, int ng,
+                      double wvln, double wedge, double chi, const double* mat,
+                      const double* ubis, const double* trans, const int* grain_labels, int K,
+                      double tol, double* drlv2, int* labels, int* counts, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (K < 0 || ng < 0) return set_err(GS_ERR_ARG, "negative size K=%d ng=%d", K, ng);
+    hipStream_t st = S(stream);
+    GS_CHECK(fill_i32(counts, 0, K, st));
+    if (K == 0 || ng == 0) return GS_OK;
+    gs_goniometer gp = make_gonio(1.0, wvln, wedge, chi, nullptr, mat);
+    constexpr int P = 2;
+    const long long nb = (ng + (long long)GS_BLOCK * P - 1) / ((long long)GS_BLOCK * P);
+    hipLaunchKernelGGL(k_gv_assign_many<P>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, xlylzl, so,
+                       co, ng, gp, ubis, trans, grain_labels, K, tol * tol, drlv2, labels, counts);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_gv_lists(const double* xlylzl, const double* so, const double* co, int ng, double wvln,
+                double wedge, double chi, const double* mat, const double* trans, int K,
+                const int* idx, const int* owner, long long n_idx, double* out,
+                gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (K < 0 || ng < 0 || n_idx < 0) return set_err(GS_ERR_ARG, "gs_gv_lists: negative size");
+    if (n_idx == 0) return GS_OK;
+    gs_goniometer gp = make_gonio(1.0, wvln, wedge, chi, nullptr, mat);
+    hipLaunchKernelGGL(k_gv_lists, dim3(blocks_for(n_idx, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
+                       xlylzl, so, co, ng, gp, trans, K, idx, owner, n_idx, out);
     GS_LAUNCHED();
     return GS_OK;
 }

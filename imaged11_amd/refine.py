@@ -1,0 +1,181 @@
+"""Batched least-squares refinement of many grains on the GPU.
+
+``refinegrains.refine`` (``ImageD11/refinegrains.py:459-498``) calls
+``score_and_refine`` a few times per grain from a Python loop, and
+``refinegrains.gof`` repeats that for every grain at every simplex step.
+Here all grains are refined together: each grain owns a CSR list of peak
+indices into one HBM-resident g-vector table, and the n_iter successive
+``score_and_refine`` calls per grain run either
+
+* ``"fused"``: inside one persistent launch (each grain keeps its UBI in
+  registers between iterations), or
+* ``"graph"``: as n_iter single-iteration persistent launches captured
+  once into a hipGraph and replayed (`RefineGraph`),
+
+with launch parameters (lanes per grain, resident waves per CU) picked by
+`autotune` and cached per problem shape.
+"""
+import ctypes
+import json
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, batched
+from ._lib import dptr, stream_ptr
+
+LANES = (8, 16, 32, 64)
+WAVES_PER_CU = (4, 8, 16)
+MODES = ("fused", "graph")
+CACHE_ENV = "IMAGED11_AMD_TUNE_CACHE"
+
+
+class RefineGraph:
+    """n_iter score_and_refine passes over all grains, captured as a
+    hipGraph on a private stream.  ``ubis`` is refined in place by each
+    replay; ``n`` and ``mean`` (n_iter, K) hold the per-call results of the
+    last replay.  The tensors passed in must stay alive with the graph."""
+
+    def __init__(self, ubis, gv, idx, off, tol, n_iter, order=None, lanes_per_grain=0,
+                 waves_per_cu=0):
+        batched._need(ubis, torch.float64, "ubis", (None, 3, 3))
+        batched._need(gv, torch.float64, "gv", (None, 3))
+        K = ubis.shape[0]
+        batched._need(idx, torch.int32, "idx", (None,))
+        batched._need(off, torch.int64, "off", (K + 1,))
+        if order is None:
+            order = batched.list_order(off)
+        batched._need(order, torch.int32, "order", (K,))
+        self.ubis, self.gv, self.idx, self.off, self.order = ubis, gv, idx, off, order
+        dev = gv.device
+        self.n = torch.zeros((n_iter, K), dtype=torch.int32, device=dev)
+        self.mean = torch.zeros((n_iter, K), dtype=torch.float64, device=dev)
+        self.queue = torch.zeros(n_iter, dtype=torch.int32, device=dev)
+        self.stream = torch.cuda.Stream(device=dev)
+        lib = _lib.load()
+        lib.gs_clear_error()
+        handle = ctypes.c_void_p()
+        torch.cuda.current_stream(dev).synchronize()
+        _lib.check(lib.gs_refine_graph_create(
+            ctypes.byref(handle), dptr(ubis), K, dptr(gv), gv.shape[0], dptr(idx), idx.numel(),
+            dptr(off), dptr(order), float(tol), int(n_iter), dptr(self.n), dptr(self.mean),
+            int(lanes_per_grain), int(waves_per_cu), dptr(self.queue),
+            stream_ptr(self.stream)), "refine graph capture")
+        self.handle = handle
+
+    def replay(self, stream=None):
+        """Enqueue one replay on `stream` (default: torch's current stream)."""
+        _lib.check(_lib.load().gs_graph_launch(self.handle, stream_ptr(stream)), "graph launch")
+
+    def close(self):
+        if self.handle:
+            _lib.load().gs_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _cache_path():
+    return os.environ.get(CACHE_ENV) or os.path.join(
+        os.path.expanduser("~"), ".cache", "imaged11_amd", "refine_tune.json")
+
+
+def problem_key(off, n_iter):
+    """Shape of a refinement problem: arch, grain count and list-length
+    quantiles, each rounded to a power-of-two bucket."""
+    lengths = (off[1:] - off[:-1]).double()
+    q = torch.quantile(lengths, torch.tensor([0.5, 0.9, 1.0], dtype=torch.float64,
+                                              device=lengths.device)).cpu().numpy()
+    b = lambda v: int(2 ** round(np.log2(max(float(v), 1.0))))
+    props = torch.cuda.get_device_properties(off.device)
+    arch = getattr(props, "gcnArchName", props.name)
+    return "%s|K%d|L%d-%d-%d|it%d" % (arch, b(off.numel() - 1), b(q[0]), b(q[1]), b(q[2]), n_iter)
+
+
+def _time(fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def autotune(ubis, gv, idx, off, tol, n_iter, reps=5, use_cache=True, verbose=False):
+    """Pick (mode, lanes_per_grain, waves_per_cu) for this problem by
+    timing every candidate with HIP events; results are cached on disk by
+    `problem_key`.  Returns the best config dict (with its time in ms)."""
+    key = problem_key(off, n_iter)
+    path = _cache_path()
+    cache = {}
+    if use_cache and os.path.exists(path):
+        try:
+            cache = json.load(open(path))
+        except (OSError, ValueError):
+            cache = {}
+        if key in cache:
+            return cache[key]
+    order = batched.list_order(off)
+    work = ubis.clone()
+    results = []
+    for lanes in LANES:
+        for wpc in WAVES_PER_CU:
+            def fused():
+                work.copy_(ubis)
+                batched.refine_lists(work, gv, idx, off, tol, n_iter, order=order,
+                                     lanes_per_grain=lanes, waves_per_cu=wpc, inplace=True)
+            results.append(({"mode": "fused", "lanes_per_grain": lanes, "waves_per_cu": wpc},
+                            _time(fused, reps)))
+            g = RefineGraph(work, gv, idx, off, tol, n_iter, order=order, lanes_per_grain=lanes,
+                            waves_per_cu=wpc)
+
+            def graph():
+                work.copy_(ubis)
+                g.replay()
+            results.append(({"mode": "graph", "lanes_per_grain": lanes, "waves_per_cu": wpc},
+                             _time(graph, reps)))
+            g.close()
+    results.sort(key=lambda r: r[1])
+    if verbose:
+        for cfg, ms in results:
+            print("%-6s lanes %2d waves/CU %2d : %.3f ms" % (cfg["mode"], cfg["lanes_per_grain"],
+                                                             cfg["waves_per_cu"], ms))
+    best = dict(results[0][0], ms=results[0][1], key=key, tuned_at=time.time())
+    cache[key] = best
+    if use_cache:
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            tmp = path + ".tmp%d" % os.getpid()
+            json.dump(cache, open(tmp, "w"), indent=1)
+            os.replace(tmp, path)
+        except OSError:
+            pass
+    return best
+
+
+def refine(ubis, gv, idx, off, tol, n_iter=2, config=None):
+    """Refine every grain with the given (or autotuned) configuration.
+    Returns (ubis_out, n (n_iter,K), mean (n_iter,K))."""
+    if config is None:
+        config = autotune(ubis, gv, idx, off, tol, n_iter)
+    if config.get("mode", "fused") == "graph":
+        u = ubis.clone()
+        g = RefineGraph(u, gv, idx, off, tol, n_iter, lanes_per_grain=config["lanes_per_grain"],
+                        waves_per_cu=config["waves_per_cu"])
+        g.replay()
+        n, m = g.n.clone(), g.mean.clone()
+        torch.cuda.current_stream().synchronize()
+        g.close()
+        return u, n, m
+    return batched.refine_lists(ubis, gv, idx, off, tol, n_iter,
+                                lanes_per_grain=config["lanes_per_grain"],
+                                waves_per_cu=config["waves_per_cu"])

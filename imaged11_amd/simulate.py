@@ -106,9 +106,11 @@ def random_grains(K, rng, cubic=(3.0, 6.0), ortho=(3.0, 9.0)):
     return ub, cells
 
 
-def _grain_spots(ub, t, pars, gmax, omega_range, omega_step):
-    """Spots of one grain: (sc, fc, omega) for every hkl within gmax whose
-    Bragg solution lands on the detector inside the omega range."""
+def _grain_spots(ub, t, pars, gmax, omega_range, omega_step, dty_rng=None, dty=0.0):
+    """Spots of one grain: (sc, fc, omega, dty) for every hkl within gmax
+    whose Bragg solution lands on the detector inside the omega range.
+    With dty > 0 each spot is recorded at a sample y-translation drawn
+    U[-dty, dty] that shifts the scattering origin by (0, dty_k, 0)."""
     p, r, dist = geometry.pack_xlylzl(pars)
     sign, wvln, wedge, chi = geometry.pack_gv(pars)
     mat = geometry.wedge_chi_matrix(wedge, chi).reshape(3, 3)
@@ -147,6 +149,8 @@ def _grain_spots(ub, t, pars, gmax, omega_range, omega_step):
         nhat[:, 0] += 1.0
         u = np.stack([co * t[0] - so * t[1], so * t[0] + co * t[1], np.full_like(co, t[2])], 1)
         o = u @ mat.T
+        shift = dty_rng.uniform(-dty, dty, len(o)) if dty > 0 else np.zeros(len(o))
+        o[:, 1] += shift
         R3 = r.reshape(3, 3)
         c1, c2 = R3[:, 1], R3[:, 2]
         # o + s nhat = dist + v1 c1 + v2 c2  ->  [nhat, -c1, -c2] (s, v1, v2) = dist - o
@@ -159,28 +163,31 @@ def _grain_spots(ub, t, pars, gmax, omega_range, omega_step):
         good = sol[:, 0] > 0
         fc = sol[:, 1] / p[3] + p[1]
         sc = sol[:, 2] / p[2] + p[0]
-        sols.append(np.stack([sc[good], fc[good], om[inr][good]], 1))
+        sols.append(np.stack([sc[good], fc[good], om[inr][good], shift[good]], 1))
     if not sols:
-        return np.zeros((0, 3))
+        return np.zeros((0, 4))
     return np.concatenate(sols)
 
 
 def simulate_peaks(ub, translations, pars, n_peaks, seed=2, spurious=0.10, gmax=1.6,
-                   omega_range=(0.0, 180.0), omega_step=0.25, jitter_px=0.5, npx=2048):
+                   omega_range=(0.0, 180.0), omega_step=0.25, jitter_px=0.5, npx=2048, dty=0.0):
     """Forward-simulate a raw peak table of exactly n_peaks rows.
 
     Spots of every grain that land on the npx x npx detector are collected,
     omega is binned to the centre of its omega_step frame, (sc, fc) get
     N(0, jitter_px) jitter, the grain spots are subsampled to
     (1 - spurious) * n_peaks and uniform spurious peaks fill the rest.
-    Returns dict(sc, fc, omega, grain) with grain = -1 for spurious rows;
-    rows are shuffled.
+    Returns dict(sc, fc, omega, dty, grain) with grain = -1 for spurious
+    rows; rows are shuffled.  dty > 0: per-peak sample y-translations
+    U[-dty, dty] (same length unit as the detector) shift each spot's
+    scattering origin; the analysis removes them from yl.
     """
     rng = np.random.default_rng(seed)
     K = ub.shape[0]
     rows, owner = [], []
     for g in range(K):
-        s = _grain_spots(ub[g], translations[g], pars, gmax, omega_range, omega_step)
+        s = _grain_spots(ub[g], translations[g], pars, gmax, omega_range, omega_step,
+                         dty_rng=rng, dty=dty)
         on = (s[:, 0] >= 0) & (s[:, 0] < npx) & (s[:, 1] >= 0) & (s[:, 1] < npx)
         s = s[on]
         rows.append(s)
@@ -199,7 +206,8 @@ def simulate_peaks(ub, translations, pars, n_peaks, seed=2, spurious=0.10, gmax=
     n_sp = n_peaks - n_real
     nfr = int(round((omega_range[1] - omega_range[0]) / omega_step))
     sp = np.stack([rng.uniform(0, npx, n_sp), rng.uniform(0, npx, n_sp),
-                   omega_range[0] + (rng.integers(0, nfr, n_sp) + 0.5) * omega_step], 1)
+                   omega_range[0] + (rng.integers(0, nfr, n_sp) + 0.5) * omega_step,
+                   rng.uniform(-dty, dty, n_sp) if dty > 0 else np.zeros(n_sp)], 1)
     allrows = np.concatenate([rows, sp])
     allown = np.concatenate([owner, np.full(n_sp, -1, np.int32)])
     perm = rng.permutation(n_peaks)
@@ -208,14 +216,16 @@ def simulate_peaks(ub, translations, pars, n_peaks, seed=2, spurious=0.10, gmax=
         "sc": np.ascontiguousarray(allrows[:, 0]),
         "fc": np.ascontiguousarray(allrows[:, 1]),
         "omega": np.ascontiguousarray(allrows[:, 2]),
+        "dty": np.ascontiguousarray(allrows[:, 3]),
         "grain": allown,
     }
 
 
-def scoring_workload(K=1000, N=1_000_000, seed=2, spurious=0.10, translations=False):
+def scoring_workload(K=1000, N=1_000_000, seed=2, spurious=0.10, translations=False, dty=0.0):
     """The many-grain scoring workload: K grains (half cubic a~U[3,6],
     half orthorhombic a,b,c~U[3,9]), N raw peaks simulated on the detector,
-    10% spurious.  Returns dict(pars, ubi[K,3,3], ub, trans[K,3], peaks)."""
+    10% spurious.  Returns dict(pars, ubi[K,3,3], ub, trans[K,3], peaks).
+    dty > 0 adds per-peak sample y-translations U[-dty, dty] (scanning)."""
     rng = np.random.default_rng(seed)
     pars = detector_parameters(rng)
     ub, cells = random_grains(K, rng)
@@ -224,7 +234,7 @@ def scoring_workload(K=1000, N=1_000_000, seed=2, spurious=0.10, translations=Fa
         trans[:, 2] *= 0.25
     else:
         trans = np.zeros((K, 3))
-    peaks = simulate_peaks(ub, trans, pars, N, seed=seed + 1000, spurious=spurious)
+    peaks = simulate_peaks(ub, trans, pars, N, seed=seed + 1000, spurious=spurious, dty=dty)
     ubi = np.linalg.inv(ub)
     return {"pars": pars, "ubi": np.ascontiguousarray(ubi), "ub": ub, "trans": trans,
             "cells": cells, "peaks": peaks}

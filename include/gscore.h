@@ -49,6 +49,13 @@ int gs_init(void);               /* allocate the per-device state now */
  * accuracy without a GPU; no other entry point calls it. */
 void gs_selftest_sincos_host(const double *x, double *s, double *c, int n);
 
+/* The certified score decision evaluated on the host next to the
+ * reference arithmetic, for checking the error bound without a GPU:
+ * out[k] = 1 certain true, 0 certain false, 2 undecided; +4 when the
+ * reference arithmetic gives true.  No other entry point calls it. */
+void gs_selftest_certified_host(const double *ubi, const double *gv, int n, double tol,
+                                double gmax, int *out);
+
 /* ---- 1. reference prototypes (host pointers) ------------------------ */
 
 /* src/closest.c:100-119 — MAGIC rounding vs floor(x+0.5) on 600 points,
@@ -84,11 +91,18 @@ void compute_xlylzl(double s[], double f[], double p[4], double r[9], double dis
 
 /* ---- 2. device-pointer entry points --------------------------------- */
 
+/* score modes: both give the reference's counts for every input.
+ * CERTIFIED evaluates h with FMA and decides each pair from a proven error
+ * bound, recomputing the few undecidable pairs with the reference's own
+ * arithmetic; EXACT uses the reference's arithmetic for every pair. */
+#define GS_SCORE_CERTIFIED 0
+#define GS_SCORE_EXACT 1
+
 /* score for K UBIs (K,3,3) against ng g-vectors -> counts[K] (int32);
  * the loop indexer.score runs over trial UBIs (ImageD11/indexing.py:1090).
- * peaks_per_lane: 1, 2, 4 or 8 (tile height; 0 = default). */
+ * peaks_per_lane: 1, 2, 4, 8 or 16 (tile height; 0 = default 8). */
 int gs_score_many(const double *ubis, int K, const double *gv, int ng, double tol,
-                  int *counts, int peaks_per_lane, gs_stream_t stream);
+                  int *counts, int peaks_per_lane, int mode, gs_stream_t stream);
 
 /* score_and_assign for K grains in order, label grain_labels[g] for grain
  * g, on shared drlv2/labels (ImageD11/refinegrains.py:718-734,
@@ -131,17 +145,82 @@ int gs_sf2gv(const double *s, const double *f, const double *omega, const double
 
 /* Batched score_and_refine: grain g refines on the peaks gv[idx[off[g]]]
  * .. gv[idx[off[g+1]-1]], n_iter successive calls (refinegrains.refine,
- * ImageD11/refinegrains.py:459-498, makes two).  ubis (K,3,3) updated in
- * place; n_out/mean_out are [n_iter][K].  `order` (K ints) is the order in
- * which the persistent waves take grains (longest list first balances
- * best).  lanes_per_grain 8/16/32/64 and waves_per_cu tune the launch
- * (0 = defaults).  Entries out of range are ignored, never dereferenced:
- * grains outside [0, K) in order, offsets outside [0, n_idx], peak
- * indices outside [0, ng). */
+ * ImageD11/refinegrains.py:459-498, makes two) inside ONE persistent
+ * launch.  ubis (K,3,3) updated in place; n_out/mean_out are [n_iter][K].
+ * `order` (K ints) is the order in which the persistent waves take grains
+ * (longest list first balances best).  lanes_per_grain 8/16/32/64 and
+ * waves_per_cu tune the launch (0 = defaults).  queue_ws: one device int
+ * used as the work-queue counter (NULL = a per-device counter of the
+ * library, which calls overlapping on one device would share).  Entries
+ * out of range are ignored, never dereferenced: grains outside [0, K) in
+ * order, offsets outside [0, n_idx], peak indices outside [0, ng). */
 int gs_refine_lists(double *ubis, int K, const double *gv, int ng, const int *idx,
                     long long n_idx, const long long *off, const int *order, double tol,
                     int n_iter, int *n_out, double *mean_out, int lanes_per_grain,
-                    int waves_per_cu, gs_stream_t stream);
+                    int waves_per_cu, int *queue_ws, gs_stream_t stream);
+
+/* The same refinement as a hipGraph: n_iter single-iteration persistent
+ * launches (each a full score_and_refine pass over all grains) captured
+ * once on `stream` and replayed with gs_graph_launch.  queue_ws must hold
+ * n_iter device ints; all buffers must stay allocated while the graph is
+ * used.  Replays start from whatever ubis holds. */
+typedef struct gs_graph_s *gs_graph_t;
+int gs_refine_graph_create(gs_graph_t *out, double *ubis, int K, const double *gv, int ng,
+                           const int *idx, long long n_idx, const long long *off,
+                           const int *order, double tol, int n_iter, int *n_out,
+                           double *mean_out, int lanes_per_grain, int waves_per_cu,
+                           int *queue_ws, gs_stream_t stream);
+int gs_graph_launch(gs_graph_t graph, gs_stream_t stream);
+int gs_graph_destroy(gs_graph_t graph);
+
+/* ---- adjoint of the joint refinement objective ----------------------
+ * No counterpart in the reference (which fits geometry by Nelder-Mead on
+ * refinegrains.gof, ImageD11/refinegrains.py:517-624).  The objective is
+ *   L = sum_g sum_{q in list g} || UBI_g . G_q - hkl_q ||^2
+ * with G_q = compute_gv(compute_xlylzl(sc[k], fc[k]), omega_k) for grain g's
+ * translation, k = idx[q], and fixed integer hkl_q.  so/co are the
+ * per-peak sin/cos of omega (gs_sincos_omega).  geo (26 device doubles):
+ * p[4] (z_center, y_center, z_size, y_size), r[9], dist[3], wavelength,
+ * mat[9] (wedge/chi matrix).
+ *
+ * gs_joint_hkl: hkl[3q..] = round(UBI_g . G_q) for every list entry, owner[q]
+ * = the grain of entry q.
+ * gs_joint_adjoint: per grain loss_g[K], grad_ubi[K*9], grad_t[K*3],
+ * geo_part[K*26]; nsums[K*15] (optional, NULL to skip) = sum h G^T and
+ * sum G G^T for a Gauss-Newton UBI update; out[27] = dL/dgeo (26, slot 16
+ * is dL/dwavelength) and L, reduced over grains in grain order
+ * (deterministic).  gspace != 0: the g-space objective
+ * 1/2 sum ||UB.hkl - G||^2 (UB = UBI^-1), grad_ubi then holds dE/dUB =
+ * UB.H - R with score_and_refine's R and H (src/closest.c:310-316). */
+int gs_joint_hkl(const double *sc, const double *fc, const double *so, const double *co,
+                 int npk, const double *geo, const double *ubis, const double *trans, int K,
+                 const int *idx, const int *owner, long long n_idx, int *hkl,
+                 gs_stream_t stream);
+int gs_joint_adjoint(const double *sc, const double *fc, const double *so, const double *co,
+                     int npk, const double *geo, const double *ubis, const double *trans, int K,
+                     const int *idx, long long n_idx, const long long *off, const int *hkl,
+                     const int *order, int gspace, double *loss_g, double *grad_ubi,
+                     double *grad_t, double *geo_part, double *nsums, double *out,
+                     int lanes_per_grain, int *queue_ws, gs_stream_t stream);
+
+/* ---- compute_gv per grain + score_and_assign, all grains in order ----
+ * The loop of refinegrains.assignlabels (ImageD11/refinegrains.py:
+ * 718-734): for g = 0..K-1, compute_gv(xlylzl, omega, ..., trans[g]) then
+ * score_and_assign(ubis[g], gv, tol, drlv2, labels, grain_labels[g]).
+ * so/co: per-peak sin/cos of omega*omegasign (gs_sincos_omega); mat: the
+ * wedge/chi matrix (9 host doubles) or NULL to build it from wedge, chi.
+ * counts[g] = the value the g-th score_and_assign returns. */
+int gs_gv_assign_many(const double *xlylzl, const double *so, const double *co, int ng,
+                      double wvln, double wedge, double chi, const double *mat,
+                      const double *ubis, const double *trans, const int *grain_labels, int K,
+                      double tol, double *drlv2, int *labels, int *counts, gs_stream_t stream);
+
+/* out[q] = compute_gv of peak idx[q] for the translation of grain owner[q]
+ * (n_idx rows of 3); rows with an out-of-range index are NaN. */
+int gs_gv_lists(const double *xlylzl, const double *so, const double *co, int ng, double wvln,
+                double wedge, double chi, const double *mat, const double *trans, int K,
+                const int *idx, const int *owner, long long n_idx, double *out,
+                gs_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,175 @@
+"""The adjoint of the joint objective: value against the oracle, gradient
+against central finite differences (h = 1e-6 relative) for every physical
+geometry parameter and for UBI and translation entries, the g-space
+gradient against the normal equations of score_and_refine, determinism,
+and the joint Gauss-Newton/Adam refinement."""
+import numpy as np
+import pytest
+import torch
+
+from imaged11_amd import adjoint, geometry, simulate
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dev, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+
+
+@pytest.fixture(scope="module")
+def setup(dev):
+    wl = simulate.scoring_workload(K=120, N=60000, seed=41, translations=True)
+    pk, pars = wl["peaks"], wl["pars"]
+    pars = dict(pars, wedge=0.7, chi=-0.4)  # exercise the wedge/chi path too
+    idx, off = simulate.refine_lists(pk["grain"], 120, seed=42, median=200, lo=20, hi=1000)
+    prob = adjoint.JointProblem(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev),
+                                pars["omegasign"], T(idx, dev, torch.int32),
+                                T(off, dev, torch.int64))
+    params = {k: torch.tensor(float(pars[k]), dtype=torch.float64, device=dev)
+              for k in adjoint.FREE}
+    ubis = T(wl["ubi"], dev)
+    trans = T(wl["trans"], dev)
+    geo = adjoint.pack_geometry(params, pars)
+    prob.assign_hkl(geo, ubis, trans)
+    return dict(wl=wl, pars=pars, idx=idx, off=off, prob=prob, params=params, ubis=ubis,
+                trans=trans, geo=geo)
+
+
+def test_pack_geometry_matches_host_packing(setup):
+    p, r, dist = geometry.pack_xlylzl(setup["pars"])
+    geo = setup["geo"].cpu().numpy()
+    np.testing.assert_allclose(geo[:4], p, rtol=0, atol=0)
+    np.testing.assert_allclose(geo[4:13], r, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(geo[13:16], dist, rtol=0, atol=0)
+    assert geo[16] == setup["pars"]["wavelength"]
+    np.testing.assert_allclose(geo[17:], geometry.wedge_chi_matrix(0.7, -0.4), atol=1e-15)
+
+
+def test_loss_matches_oracle(setup, orc):
+    s = setup
+    loss, *_ = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"])
+    geo = s["geo"].cpu().numpy()
+    pk = s["wl"]["peaks"]
+    so, co = s["prob"].so.cpu().numpy(), s["prob"].co.cpu().numpy()
+    want = orc.joint_loss(pk["sc"], pk["fc"], so, co, geo[:4], geo[4:13], geo[13:16], geo[16],
+                          geo[17:], s["wl"]["ubi"], s["wl"]["trans"], s["idx"], s["off"],
+                          s["prob"].hkl.cpu().numpy())
+    assert abs(float(loss) - want) <= 1e-10 * abs(want)
+
+
+def _loss_at(s, params=None, ubis=None, trans=None):
+    geo = adjoint.pack_geometry(params or s["params"], s["pars"])
+    loss, *_ = s["prob"].evaluate(geo, s["ubis"] if ubis is None else ubis,
+                                  s["trans"] if trans is None else trans)
+    return float(loss)
+
+
+def test_geometry_gradient_finite_differences(setup):
+    s = setup
+    params = {k: v.clone().requires_grad_(True) for k, v in s["params"].items()}
+    loss = adjoint.joint_loss(s["prob"], s["ubis"], s["trans"], params, s["pars"])
+    loss.backward()
+    floors = {"tilt_x": 1e-3, "tilt_y": 1e-3, "tilt_z": 1e-3, "wedge": 1.0, "chi": 1.0}
+    for k in adjoint.FREE:
+        v = float(s["params"][k])
+        h = 1e-6 * max(abs(v), floors.get(k, 0.0))
+        up = {kk: vv.clone() for kk, vv in s["params"].items()}
+        dn = {kk: vv.clone() for kk, vv in s["params"].items()}
+        up[k] = up[k] + h
+        dn[k] = dn[k] - h
+        fd = (_loss_at(s, up) - _loss_at(s, dn)) / (2 * h)
+        g = float(params[k].grad)
+        assert abs(g - fd) <= 1e-5 * max(abs(fd), abs(g)) + 1e-9 * abs(float(loss)), (k, g, fd)
+
+
+def test_ubi_and_translation_gradient_finite_differences(setup):
+    s = setup
+    _, gu, gt, _, _ = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"])
+    rng = np.random.default_rng(0)
+    for _ in range(6):
+        g, i, j = int(rng.integers(0, 120)), int(rng.integers(0, 3)), int(rng.integers(0, 3))
+        h = 1e-6 * abs(float(s["ubis"][g, i, j])) + 1e-9
+        up, dn = s["ubis"].clone(), s["ubis"].clone()
+        up[g, i, j] += h
+        dn[g, i, j] -= h
+        fd = (_loss_at(s, ubis=up) - _loss_at(s, ubis=dn)) / (2 * h)
+        assert abs(float(gu[g, i, j]) - fd) <= 1e-6 * abs(fd) + 1e-8
+        c = int(rng.integers(0, 3))
+        h = 1e-6 * max(abs(float(s["trans"][g, c])), 1.0)
+        up, dn = s["trans"].clone(), s["trans"].clone()
+        up[g, c] += h
+        dn[g, c] -= h
+        fd = (_loss_at(s, trans=up) - _loss_at(s, trans=dn)) / (2 * h)
+        assert abs(float(gt[g, c]) - fd) <= 1e-5 * abs(fd) + 1e-9
+
+
+def test_gspace_gradient_vanishes_at_score_and_refine_solution(setup, orc):
+    """g-space gradient UB.H - R is zero at UB = R.H^-1, the matrix that
+    score_and_refine (src/closest.c:267-345) returns for the same peaks."""
+    s = setup
+    pk = s["wl"]["peaks"]
+    geo = s["geo"].cpu().numpy()
+    so, co = s["prob"].so.cpu().numpy(), s["prob"].co.cpu().numpy()
+    xl = np.zeros((len(pk["sc"]), 3))
+    orc.compute_xlylzl(pk["sc"], pk["fc"], geo[:4], geo[4:13], geo[13:16], xl)
+    ubis_new = s["wl"]["ubi"].copy()
+    hkl = s["prob"].hkl.cpu().numpy()
+    for g in range(120):
+        sl = slice(s["off"][g], s["off"][g + 1])
+        gv = orc.compute_gv_trig(xl[s["idx"][sl]], so[s["idx"][sl]], co[s["idx"][sl]], geo[17:],
+                                 geo[16], s["wl"]["trans"][g])
+        u = ubis_new[g].copy()
+        n, _ = orc.score_and_refine(u, gv, 1.0)  # tol 1: every list peak kept
+        assert n == s["off"][g + 1] - s["off"][g]
+        assert np.array_equal(np.rint(gv @ ubis_new[g].T), hkl[sl])
+        ubis_new[g] = u
+    _, gub, _, _, _ = s["prob"].evaluate(s["geo"], T(ubis_new, s["ubis"].device), s["trans"],
+                                         gspace=True)
+    gub = gub.cpu().numpy()
+    # scale: |R| ~ sum |h||g|
+    scale = np.array([np.abs(hkl[s["off"][g]:s["off"][g + 1]]).sum() for g in range(120)])
+    assert np.all(np.abs(gub).reshape(120, -1).max(1) <= 1e-11 * scale)
+    # away from the solution it equals UB.H - R computed in numpy
+    _, gub0, _, _, _ = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"], gspace=True)
+    g = 7
+    sl = slice(s["off"][g], s["off"][g + 1])
+    gv = orc.compute_gv_trig(xl[s["idx"][sl]], so[s["idx"][sl]], co[s["idx"][sl]], geo[17:],
+                             geo[16], s["wl"]["trans"][g])
+    h = hkl[sl].astype(float)
+    R = gv.T @ h
+    H = h.T @ h
+    UB = np.linalg.inv(s["wl"]["ubi"][g])
+    np.testing.assert_allclose(gub0[g].cpu().numpy(), UB @ H - R, rtol=1e-9, atol=1e-9 * np.abs(R).max())
+
+
+def test_adjoint_deterministic(setup):
+    s = setup
+    a = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"], want_sums=True)
+    b = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"], want_sums=True)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+
+
+def test_joint_refinement_recovers_geometry(setup):
+    """50 steps (Gauss-Newton UBIs + Adam geometry) from a perturbed
+    detector: the fit ends as good as the fit at the true geometry and the
+    beam centre moves back towards the truth.  (Distance trades off against
+    the cell scale of the UBIs, so it is not asserted.)"""
+    s = setup
+    truth = {k: float(v) for k, v in s["params"].items()}
+    names = ("distance", "y_center", "z_center", "tilt_z")
+    dev = s["ubis"].device
+    # the fit at the true geometry: Gauss-Newton UBIs, then the loss
+    _, _, _, _, sums = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"], want_sums=True)
+    best_loss = float(s["prob"].evaluate(s["geo"], adjoint.gauss_newton_ubis(sums),
+                                         s["trans"])[0])
+    start = dict(truth, distance=truth["distance"] + 150.0, y_center=truth["y_center"] + 0.6,
+                 z_center=truth["z_center"] - 0.4, tilt_z=truth["tilt_z"] + 2e-4)
+    params = {k: torch.tensor(start[k], dtype=torch.float64, device=dev) for k in names}
+    scales = {"distance": 100.0, "y_center": 1.0, "z_center": 1.0, "tilt_z": 1e-3}
+    ubis, final, hist = adjoint.refine_joint(s["prob"], s["ubis"], s["trans"], params,
+                                             dict(s["pars"]), steps=50, lr=0.05, scales=scales)
+    assert hist[-1] < hist[0]
+    assert hist[-1] <= 1.05 * best_loss
+    for k in ("y_center", "z_center"):
+        assert abs(float(final[k]) - truth[k]) < 0.5 * abs(start[k] - truth[k])

@@ -213,17 +213,20 @@ def _oracle_gv(orc, wl, t=np.zeros(3)):
     return xl, gv
 
 
+@pytest.mark.parametrize("exact", [False, True])
 @pytest.mark.parametrize("ppl", [1, 2, 4, 8])
-def test_score_many_pipeline(dev, orc, workload, ppl):
+def test_score_many_pipeline(dev, orc, workload, ppl, exact):
     wl = workload
     _, gv = _oracle_gv(orc, wl)
     want = orc.score_many(wl["ubi"], gv, 0.05)
     pk = wl["peaks"]
     g = batched.sf2gv(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev), wl["pars"])
-    got = batched.score_many(T(wl["ubi"], dev), g, 0.05, peaks_per_lane=ppl).cpu().numpy()
+    got = batched.score_many(T(wl["ubi"], dev), g, 0.05, peaks_per_lane=ppl,
+                             exact=exact).cpu().numpy()
     assert np.array_equal(got, want)
     # and on the oracle's own g-vectors (no sincos difference at all)
-    got2 = batched.score_many(T(wl["ubi"], dev), T(gv, dev), 0.05, peaks_per_lane=ppl)
+    got2 = batched.score_many(T(wl["ubi"], dev), T(gv, dev), 0.05, peaks_per_lane=ppl,
+                              exact=exact)
     assert np.array_equal(got2.cpu().numpy(), want)
 
 
@@ -327,3 +330,19 @@ def test_full_scoring_workload_counts(dev, orc):
     g = batched.sf2gv(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev), wl["pars"])
     got = batched.score_many(T(wl["ubi"], dev), g, 0.05).cpu().numpy()
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("tol,hmax", [(0.05, 3), (0.05, 40), (0.3, 200), (0.8, 20), (1e-4, 2)])
+def test_score_certified_boundary(dev, orc, tol, hmax):
+    """Pairs within 1e-17..1e-9 relative of tol^2: the certified kernel,
+    the exact kernel and the oracle count the same."""
+    from helpers import boundary_pairs
+    for seed in range(3):
+        ubi, gv = boundary_pairs(seed, 60000, tol, hmax)
+        want = orc.score(ubi, gv, tol)
+        u = T(ubi[None], dev)
+        g = T(gv, dev)
+        for ppl in (1, 4, 8):
+            fast = batched.score_many(u, g, tol, peaks_per_lane=ppl).item()
+            exact = batched.score_many(u, g, tol, peaks_per_lane=ppl, exact=True).item()
+            assert fast == exact == want

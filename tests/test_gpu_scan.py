@@ -1,0 +1,128 @@
+"""compute_gv per grain fused with score_and_assign (the assignlabels loop)
+and the loop drivers, against the oracle.  Labels, drlv2 and per-call
+counts bit-exact."""
+import numpy as np
+import pytest
+import torch
+
+from imaged11_amd import batched, drivers, geometry, simulate
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dev, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+
+
+@pytest.fixture(scope="module", params=[(0.0, 0.0), (1.5, -2.0)], ids=["flat", "wedge_chi"])
+def scan(request, orc):
+    wedge, chi = request.param
+    wl = simulate.scoring_workload(K=80, N=40000, seed=51, spurious=0.15, translations=True,
+                                   dty=500.0)
+    pars = dict(wl["pars"], wedge=wedge, chi=chi)
+    if wedge or chi:  # re-simulate spots for this goniometer
+        pk = simulate.simulate_peaks(wl["ub"], wl["trans"], pars, 40000, seed=52, spurious=0.15,
+                                     dty=500.0)
+        wl = dict(wl, peaks=pk)
+    pk = wl["peaks"]
+    p, r, dist = geometry.pack_xlylzl(pars)
+    sign, wvln, _, _ = geometry.pack_gv(pars)
+    xl = np.zeros((len(pk["sc"]), 3))
+    orc.compute_xlylzl(pk["sc"], pk["fc"], p, r, dist, xl)
+    xl[:, 1] -= pk["dty"]
+    so, co = orc.sincos_omega(pk["omega"], sign)
+    mat = orc.wedge_chi_matrix(wedge, chi)
+    return dict(wl=wl, pars=pars, xl=xl, so=so, co=co, mat=mat, wvln=wvln)
+
+
+@pytest.mark.parametrize("d0,l0", [(1.0, -1), (1.0, 0), (2.0, -1)])
+def test_gv_assign_bitexact_given_trig(dev, orc, scan, d0, l0):
+    s = scan
+    n, K = len(s["xl"]), len(s["wl"]["ubi"])
+    glab = np.arange(K, dtype=np.int32)[::-1].copy()
+    d, lab = np.full(n, d0), np.full(n, l0, np.int32)
+    want = orc.gv_assign_many(s["xl"], s["so"], s["co"], s["mat"], s["wvln"], s["wl"]["ubi"],
+                              s["wl"]["trans"], glab, 0.05, d, lab)
+    dd, ll = T(np.full(n, d0), dev), T(np.full(n, l0), dev, torch.int32)
+    got = batched.gv_assign_many(T(s["xl"], dev), T(s["so"], dev), T(s["co"], dev), s["wvln"],
+                                 T(s["wl"]["ubi"], dev), T(s["wl"]["trans"], dev), 0.05, dd, ll,
+                                 grain_labels=T(glab, dev, torch.int32), mat=s["mat"])
+    assert np.array_equal(got.cpu().numpy(), want)
+    assert np.array_equal(ll.cpu().numpy(), lab)
+    assert np.array_equal(dd.cpu().numpy().view(np.int64), d.view(np.int64))
+    assert want.sum() > 0.5 * n
+
+
+def test_assignlabels_driver(dev, orc, scan):
+    """The production path (device sincos, device wedge/chi matrix) gives
+    the reference loop's labels on this data."""
+    s = scan
+    pk = s["wl"]["peaks"]
+    sc = drivers.Scan(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev), s["pars"],
+                      dty=T(pk["dty"], dev))
+    np.testing.assert_array_equal(sc.xl.cpu().numpy(), s["xl"])
+    K = len(s["wl"]["ubi"])
+    labels, drlv2, counts = drivers.assignlabels(sc, T(s["wl"]["ubi"], dev),
+                                                 T(s["wl"]["trans"], dev), 0.05)
+    n = len(s["xl"])
+    d, lab = np.ones(n), np.full(n, -1, np.int32)
+    want = orc.gv_assign_many(s["xl"], s["so"], s["co"], s["mat"], s["wvln"], s["wl"]["ubi"],
+                              s["wl"]["trans"], np.arange(K), 0.05, d, lab)
+    assert np.array_equal(counts.cpu().numpy(), want)
+    assert np.array_equal(labels.cpu().numpy(), lab)
+    # drlv2 within the last bits of sin/cos of omega (device sincos is
+    # correctly rounded, glibc's is not for ~0.16% of angles)
+    np.testing.assert_allclose(drlv2.cpu().numpy(), d, rtol=1e-9, atol=1e-18)
+    # most peaks go back to the grain that made them
+    own = pk["grain"]
+    real = own >= 0
+    assert (labels.cpu().numpy()[real] == own[real]).mean() > 0.9
+
+
+def test_refine_after_assignment(dev, orc, scan):
+    s = scan
+    pk = s["wl"]["peaks"]
+    sc = drivers.Scan(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev), s["pars"],
+                      dty=T(pk["dty"], dev))
+    K = len(s["wl"]["ubi"])
+    ubis = T(s["wl"]["ubi"], dev)
+    trans = T(s["wl"]["trans"], dev)
+    labels, _, _ = drivers.assignlabels(sc, ubis, trans, 0.05)
+    gvl, idx, off = drivers.grain_gvectors(sc, trans, labels, K)
+    u, n, m = drivers.refine_grains(ubis, gvl, off, 0.05, n_iter=2)
+    gvl_h, off_h = gvl.cpu().numpy(), off.cpu().numpy()
+    idx_h = idx.cpu().numpy()
+    lab_h = labels.cpu().numpy()
+    for g in range(K):
+        sel = idx_h[off_h[g]:off_h[g + 1]]
+        assert np.all(lab_h[sel] == g) and np.all(np.diff(sel) > 0)
+        # the grain's g-vectors with its translation
+        want_g = orc.compute_gv_trig(s["xl"][sel], s["so"][sel], s["co"][sel], s["mat"],
+                                     s["wvln"], s["wl"]["trans"][g])
+        np.testing.assert_allclose(gvl_h[off_h[g]:off_h[g + 1]], want_g, rtol=0,
+                                   atol=1e-14)
+        uu = s["wl"]["ubi"][g].copy()
+        for it in range(2):
+            nn, mm = orc.score_and_refine(uu, gvl_h[off_h[g]:off_h[g + 1]], 0.05)
+            assert int(n[it, g]) == nn
+        np.testing.assert_allclose(u[g].cpu().numpy(), uu, rtol=0, atol=1e-10)
+
+
+def test_index_drivers(dev, orc):
+    k = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                           "ken_simul.npz"))
+    gv = T(k["gv"], dev)
+    ubis = T(k["ubis"], dev)
+    labels, drlv2, gas = drivers.fight_over_peaks(ubis, gv, 0.01)
+    assert gas.cpu().numpy().tolist() == k["idx_npeaks"].tolist()
+    assert np.array_equal(labels.cpu().numpy(), k["fop_labels"])
+    scores = drivers.indexer_scores(ubis, gv, 0.01).cpu().numpy()
+    assert scores.tolist() == [orc.score(u, k["gv"], 0.01) for u in k["ubis"]]
+    mask = drivers.getind(ubis[3], gv, 0.01).cpu().numpy()
+    d, lab = np.ones(len(k["gv"])), np.zeros(len(k["gv"]), np.int32)
+    orc.score_and_assign(k["ubis"][3], k["gv"], 0.01, d, lab, 1)
+    assert np.array_equal(mask, lab == 1)
+    l2, d2, c2 = drivers.assign_peaks_to_grains(ubis, gv, 0.01)
+    d, lab = np.ones(len(k["gv"])), np.zeros(len(k["gv"]), np.int32)
+    want = [orc.score_and_assign(u, k["gv"], 0.01, d, lab, i) for i, u in enumerate(k["ubis"])]
+    assert c2.cpu().numpy().tolist() == want and np.array_equal(l2.cpu().numpy(), lab)

@@ -12,6 +12,8 @@ import torch
 from imaged11_amd import _lib, geometry, simulate
 from imaged11_amd import cImageD11 as gpu
 
+from helpers import boundary_pairs
+
 
 def test_exports_every_header_symbol():
     lib = _lib.load()
@@ -146,3 +148,39 @@ def test_simulator_invariants():
     assert L.min() >= 20 and L.max() <= 80 and off[-1] == len(idx)
     ubi, gv, lat = simulate.lattice_gvectors(seed=1)
     assert gv.shape == (20000, 3) and lat.sum() == 12000
+
+
+@pytest.mark.parametrize("tol,hmax", [(0.05, 3), (0.05, 40), (0.01, 10), (0.3, 200),
+                                      (0.5, 5), (0.8, 20), (1e-4, 2)])
+def test_certified_decision_is_sound(tol, hmax):
+    """Whenever the certified FMA path decides, it decides as the
+    reference arithmetic does; undecided pairs are the rare exception."""
+    lib = _lib.load()
+    total_amb = 0
+    for seed in range(4):
+        ubi, gv = boundary_pairs(seed, 40000, tol, hmax)
+        gmax = np.abs(gv).max()
+        out = np.zeros(len(gv), np.int32)
+        lib.gs_selftest_certified_host(_lib.hptr(ubi), _lib.hptr(gv), len(gv), tol, gmax,
+                                       _lib.hptr(out))
+        dec, exact = out & 3, (out & 4) > 0
+        decided = dec != 2
+        assert np.array_equal(dec[decided] == 1, exact[decided])
+        # the cases straddle tol^2 and the window is exercised; for
+        # tol >= 0.01 some near-boundary pairs are decided as well
+        assert 0 < exact.sum() < len(gv)
+        assert (~decided).sum() > 0
+        assert tol < 0.01 or decided.sum() > 0
+        total_amb += int((~decided).sum())
+
+
+def test_certified_window_special():
+    lib = _lib.load()
+    ubi = np.eye(3)
+    gv = np.array([[np.nan, 0, 0], [0.5, 0.5, 0.5], [2.0 ** 51 + 0.5, 0, 0], [0.01, 0, 0]])
+    out = np.zeros(4, np.int32)
+    # huge gmax -> everything undecided (exact path)
+    lib.gs_selftest_certified_host(_lib.hptr(ubi), _lib.hptr(gv), 4, 0.1, 2.0 ** 52, _lib.hptr(out))
+    assert (out[1:] & 3 == 2).all() and out[0] == 0
+    lib.gs_selftest_certified_host(_lib.hptr(ubi), _lib.hptr(gv), 4, 0.1, np.inf, _lib.hptr(out))
+    assert (out[1:] & 3 == 2).all()
