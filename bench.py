@@ -9,7 +9,7 @@ at 50 um / 150 mm, wavelength 0.2 A, omega 0-180 deg in 0.25 deg frames,
 timed region.  value = K*N*steps / elapsed.
 
 Printed: ONE JSON line, with a roofline object for the dominant kernel
-(k_score_many, FP64 vector ALU bound) timed live with HIP events on its
+(k_score_many_fast, FP64 vector ALU bound) timed live with HIP events on its
 stream, and a cpu_baseline object: the reference C (oracle/_ref, compiled
 from its sources) running the same pipeline on the host cores, the grain
 loop spread over threads (ctypes releases the GIL), with its per-grain
@@ -28,7 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOPS_PER_PAIR = 23      # h = UBI.g: 9 mul + 6 add; 3 sub; sumsq: 3 mul + 2 add
-VALU_OPS_PER_PAIR = 30   # as issued: + 6 rounding adds + 1 compare (no FMA)
+# FP64 VALU instructions issued per pair by the certified kernel: h with
+# mul + 2 FMA per row (9), round + subtract per row (6), sum of squares
+# mul + 2 FMA (3), two compares (2)
+VALU_OPS_PER_PAIR = 20
 PEAK_FP64_VALU = 78.6e12  # MI355X FP64 vector, FMA counted as 2
 PEAK_HBM = 8.0e12
 
@@ -44,7 +47,27 @@ def parse():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-aux", action="store_true",
+                    help="skip the batched-refinement and adjoint measurements")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of "
+                         "known traffic that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_score_many_fast__traffic_1000x1e6.txt")
+
+
+def measured_traffic():
+    """HBM bytes per k_score_many_fast launch on the bench workload, from
+    the committed rocprofv3 PMC summary (null when absent)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(wl, tol, threads, gpu_counts):
@@ -89,6 +112,123 @@ def cpu_baseline(wl, tol, threads, gpu_counts):
     }
 
 
+def _events_ms(fn, reps=3):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def aux_refine_adjoint(threads, with_cpu):
+    """Batched refinement of 5000 grains with lognormal lists (median 300,
+    clipped [20, 2000]) from a 2e6-peak table, 10 score_and_refine
+    iterations: autotuned batched launch vs the per-grain drop-in loop on
+    the same GPU (device-resident lists) vs the reference C per grain on the
+    host; then one adjoint evaluation of the joint objective on the same
+    lists and 50 joint refinement steps."""
+    import numpy as np
+    import torch
+    from imaged11_amd import adjoint, batched, geometry, refine, simulate
+    from imaged11_amd import cImageD11 as gpu_c
+
+    dev = torch.device("cuda:0")
+    K, N, n_iter, tol = 5000, 2_000_000, 10, 0.05
+    t0 = time.perf_counter()
+    wl = simulate.scoring_workload(K=K, N=N, seed=3)
+    idx, off = simulate.refine_lists(wl["peaks"]["grain"], K, seed=3)
+    gen_s = time.perf_counter() - t0
+    pk, pars = wl["peaks"], wl["pars"]
+    T = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    gv = batched.sf2gv(T(pk["sc"]), T(pk["fc"]), T(pk["omega"]), pars)
+    rng = np.random.default_rng(4)
+    start = wl["ubi"] @ (np.eye(3) + rng.normal(scale=1e-3, size=(K, 3, 3)))
+    ubis0 = T(start)
+    idx_d, off_d = T(idx, torch.int32), T(off, torch.int64)
+    M = int(off[-1])
+    best = refine.autotune(ubis0, gv, idx_d, off_d, tol, n_iter, reps=3)
+    work = ubis0.clone()
+    if best["mode"] == "graph":
+        g = refine.RefineGraph(work, gv, idx_d, off_d, tol, n_iter,
+                               lanes_per_grain=best["lanes_per_grain"],
+                               waves_per_cu=best["waves_per_cu"])
+        run = lambda: (work.copy_(ubis0), g.replay())
+    else:
+        run = lambda: batched.refine_lists(ubis0, gv, idx_d, off_d, tol, n_iter,
+                                           lanes_per_grain=best["lanes_per_grain"],
+                                           waves_per_cu=best["waves_per_cu"])
+    batched_ms = _events_ms(run, reps=5)
+    # per-grain drop-in loop on the same GPU: lists gathered once, then the
+    # reference's call pattern (n_iter score_and_refine calls per grain)
+    lists = [gv[idx_d[off[k]:off[k + 1]].long()].contiguous() for k in range(K)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u_loop = start.copy()
+    for k in range(K):
+        u = u_loop[k]
+        for _ in range(n_iter):
+            gpu_c.score_and_refine(u, lists[k], tol)
+    loop_ms = (time.perf_counter() - t0) * 1e3
+    out = {"grains": K, "peaks": N, "list_entries": M, "n_iter": n_iter, "gen_s": gen_s,
+           "config": best, "batched_ms": batched_ms, "per_grain_gpu_loop_ms": loop_ms,
+           "speedup_vs_gpu_loop": loop_ms / batched_ms,
+           "refine_iterations_per_s": n_iter / (batched_ms * 1e-3),
+           "pairs_per_s": M * n_iter / (batched_ms * 1e-3)}
+    if with_cpu:
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import oracle
+        ref = oracle.load_reference() or oracle.load_oracle()
+        gvh = gv.cpu().numpy()
+        sub = [np.ascontiguousarray(gvh[idx[off[k]:off[k + 1]]]) for k in range(K)]
+
+        def one(k):
+            u = start[k].copy()
+            for _ in range(n_iter):
+                ref.score_and_refine(u, sub[k], tol)
+            return u
+        with ThreadPoolExecutor(max_workers=threads) as pool:
+            t0 = time.perf_counter()
+            list(pool.map(one, range(K)))
+            cpu_ms = (time.perf_counter() - t0) * 1e3
+        out["cpu_reference_ms"] = cpu_ms
+        out["cpu_threads"] = threads
+        out["speedup_vs_cpu"] = cpu_ms / batched_ms
+    # adjoint on the same lists (translations zero), then 50 joint steps
+    prob = adjoint.JointProblem(T(pk["sc"]), T(pk["fc"]), T(pk["omega"]), pars["omegasign"],
+                                idx_d, off_d)
+    params = {k: torch.tensor(float(pars[k]), dtype=torch.float64, device=dev)
+              for k in adjoint.FREE[:8]}
+    trans = torch.zeros((K, 3), dtype=torch.float64, device=dev)
+    geo = adjoint.pack_geometry(params, pars)
+    prob.assign_hkl(geo, ubis0, trans)
+    adj_ms = _events_ms(lambda: prob.evaluate(geo, ubis0, trans, want_sums=True), reps=5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p2 = dict(params, distance=params["distance"] + 100.0)
+    _, _, hist = adjoint.refine_joint(prob, ubis0, trans, p2, pars, steps=50, lr=0.05,
+                                      scales={k: 1.0 for k in p2} | {"distance": 100.0,
+                                                                     "tilt_x": 1e-3,
+                                                                     "tilt_y": 1e-3,
+                                                                     "tilt_z": 1e-3,
+                                                                     "wavelength": 1e-5,
+                                                                     "wedge": 0.01})
+    torch.cuda.synchronize()
+    out["adjoint"] = {"eval_ms": adj_ms, "pairs": M,
+                      "pairs_per_s": M / (adj_ms * 1e-3),
+                      "joint_steps": 50, "joint_refine_ms": (time.perf_counter() - t0) * 1e3,
+                      "loss_first": hist[0], "loss_last": hist[-1]}
+    out["batched_refine_plus_adjoint_ms"] = batched_ms + adj_ms
+    out["speedup_refine_plus_adjoint_vs_gpu_loop"] = loop_ms / (batched_ms + adj_ms)
+    if "cpu_reference_ms" in out:
+        out["speedup_refine_plus_adjoint_vs_cpu"] = out["cpu_reference_ms"] / (batched_ms + adj_ms)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -113,6 +253,12 @@ def main():
     def T(a):
         return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
 
+    if args.copy_mb:
+        src = torch.ones(args.copy_mb * 131072, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     sc, fc, om, ubis = T(pk["sc"]), T(pk["fc"]), T(pk["omega"]), T(wl["ubi"])
     counts = torch.empty(args.grains, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
@@ -162,13 +308,13 @@ def main():
                                "%d grains x %d peaks, tol %g" % (args.grains, args.peaks, args.tol),
                    "grains": args.grains, "peaks": args.peaks, "tol": args.tol},
         "roofline": {
-            "kernel": "k_score_many",
+            "kernel": "k_score_many_fast",
             "bound": "valu",
             "achieved": achieved / 1e12,
             "peak": PEAK_FP64_VALU / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_VALU,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel_ms": score_ms,
             "flops_per_pair": FLOPS_PER_PAIR,
             "valu_ops_per_pair": VALU_OPS_PER_PAIR,
@@ -177,9 +323,11 @@ def main():
         },
         "cpu_baseline": None,
     }
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(wl, args.tol, threads, gpu_counts)
+    if not args.no_aux:
+        out["aux"] = {"refine_5000_grains": aux_refine_adjoint(threads, not args.no_cpu)}
     print(json.dumps(out))
 
 

@@ -33,7 +33,7 @@ def _ready():
     return lib
 
 
-def score_many(ubis, gv, tol, peaks_per_lane=0, out=None, exact=False):
+def score_many(ubis, gv, tol, peaks_per_lane=0, out=None, exact=False, mode=None):
     """counts[g] = score(ubis[g], gv, tol) for every grain g.
 
     ubis (K,3,3) float64, gv (N,3) float64 -> int32 (K,).  exact=True uses
@@ -46,7 +46,8 @@ def score_many(ubis, gv, tol, peaks_per_lane=0, out=None, exact=False):
     if out is None:
         out = torch.empty(K, dtype=torch.int32, device=gv.device)
     _need(out, torch.int32, "out", (K,))
-    mode = _lib.SCORE_EXACT if exact else _lib.SCORE_CERTIFIED
+    if mode is None:
+        mode = _lib.SCORE_EXACT if exact else _lib.SCORE_CERTIFIED
     _lib.check(lib.gs_score_many(dptr(ubis), K, dptr(gv), gv.shape[0], float(tol), dptr(out),
                                  int(peaks_per_lane), mode, stream_ptr()), "score_many")
     return out

@@ -32,6 +32,44 @@
 #pragma once
 #include "gs_kernels.h"
 
+// per-peak constants of the fast path: a = Omega(omega) mat^T xl and
+// c = ds Omega(omega) mat^T (1,0,0)
+GS_HD __forceinline__ void gs_scan_consts(double x0, double x1, double x2, double so, double co,
+                                          const double* mat, double ds, double* a, double* c) {
+    const double w0 = mat[0] * x0 + mat[3] * x1 + mat[6] * x2;
+    const double w1 = mat[1] * x0 + mat[4] * x1 + mat[7] * x2;
+    const double w2 = mat[2] * x0 + mat[5] * x1 + mat[8] * x2;
+    a[0] = co * w0 + so * w1;
+    a[1] = -so * w0 + co * w1;
+    a[2] = w2;
+    c[0] = ds * (co * mat[0] + so * mat[1]);
+    c[1] = ds * (-so * mat[0] + co * mat[1]);
+    c[2] = ds * mat[2];
+}
+
+// fast drlv2 of one pair: G = ds (a - t)/|a - t| - c, then gs_drlv2_fma
+GS_HD __forceinline__ double gs_scan_fast_drlv2(const gs_ubi& u, const double* a,
+                                                const double* c, double t0, double t1, double t2,
+                                                double ds) {
+    const double e0 = a[0] - t0, e1 = a[1] - t1, e2 = a[2] - t2;
+    const double inv = ds / sqrt(__builtin_fma(e2, e2, __builtin_fma(e1, e1, e0 * e0)));
+    const double G0 = __builtin_fma(e0, inv, -c[0]);
+    const double G1 = __builtin_fma(e1, inv, -c[1]);
+    const double G2 = __builtin_fma(e2, inv, -c[2]);
+    return gs_drlv2_fma(u, G0, G1, G2);
+}
+
+// half-width of the undecided window for a grain (UBI u, |t|^2 = tt) and
+// peaks with |xl|^2 >= n2min; +inf where the bound does not apply
+GS_HD __forceinline__ double gs_scan_window(const gs_ubi& u, double ds, double tolsq, double tt,
+                                            double n2min) {
+    const double gmax = 2.0 * fabs(ds);
+    const double u1 = fabs(u.u00) + fabs(u.u01) + fabs(u.u02) + fabs(u.u10) + fabs(u.u11) +
+                      fabs(u.u12) + fabs(u.u20) + fabs(u.u21) + fabs(u.u22);
+    if (!(u1 * gmax < 562949953421312.0) || !(4.0 * tt <= n2min)) return __builtin_inf();
+    return u1 * (1e-13 * fabs(ds) + 1e-15 * gmax) + 1e-15 + 2e-15 * tolsq;
+}
+
 template <int P>
 __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
     const double* __restrict__ xl, const double* __restrict__ so_in,
@@ -59,18 +97,15 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
         d[p] = 0.0;
         lab[p] = -1;
         if (k < ng) {
-            const double so = so_in[k], co = co_in[k];
             const double x0 = xl[3 * k], x1 = xl[3 * k + 1], x2 = xl[3 * k + 2];
-            // w = mat^T xl, a = Omega w ; mat^T x_hat = first row of mat
-            const double w0 = mat[0] * x0 + mat[3] * x1 + mat[6] * x2;
-            const double w1 = mat[1] * x0 + mat[4] * x1 + mat[7] * x2;
-            const double w2 = mat[2] * x0 + mat[5] * x1 + mat[8] * x2;
-            ax[p] = co * w0 + so * w1;
-            ay[p] = -so * w0 + co * w1;
-            az[p] = w2;
-            cx[p] = ds * (co * mat[0] + so * mat[1]);
-            cy[p] = ds * (-so * mat[0] + co * mat[1]);
-            cz[p] = ds * mat[2];
+            double a[3], c[3];
+            gs_scan_consts(x0, x1, x2, so_in[k], co_in[k], mat, ds, a, c);
+            ax[p] = a[0];
+            ay[p] = a[1];
+            az[p] = a[2];
+            cx[p] = c[0];
+            cy[p] = c[1];
+            cz[p] = c[2];
             d[p] = drlv2[k];
             lab[p] = labels[k];
             const double n2 = x0 * x0 + x1 * x1 + x2 * x2;
@@ -78,7 +113,6 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
         }
     }
     for (int o = GS_WAVE / 2; o > 0; o >>= 1) n2min = fmin(n2min, __shfl_xor(n2min, o, GS_WAVE));
-    const double gmax = 2.0 * fabs(ds);
     const int lane = threadIdx.x & (GS_WAVE - 1);
     for (int c0 = 0; c0 < K; c0 += GS_GRAIN_CHUNK) {
         const int c1 = min(K, c0 + GS_GRAIN_CHUNK);
@@ -89,21 +123,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
             const double t0 = trans[3 * (size_t)g], t1 = trans[3 * (size_t)g + 1],
                          t2 = trans[3 * (size_t)g + 2];
             const int L = glabels[g];
-            const double u1 = fabs(u.u00) + fabs(u.u01) + fabs(u.u02) + fabs(u.u10) +
-                              fabs(u.u11) + fabs(u.u12) + fabs(u.u20) + fabs(u.u21) + fabs(u.u22);
-            double e = u1 * (1e-13 * fabs(ds) + 1e-15 * gmax) + 1e-15 + 2e-15 * tolsq;
             const double tt = t0 * t0 + t1 * t1 + t2 * t2;
-            if (!(u1 * gmax < 562949953421312.0) || !(4.0 * tt <= n2min)) e = __builtin_inf();
-            const double hi = tolsq + e;
+            const double hi = tolsq + gs_scan_window(u, ds, tolsq, tt, n2min);
             int cnt = 0;
 #pragma unroll
             for (int p = 0; p < P; p++) {
-                const double e0 = ax[p] - t0, e1 = ay[p] - t1, e2 = az[p] - t2;
-                const double inv = ds / sqrt(__builtin_fma(e2, e2, __builtin_fma(e1, e1, e0 * e0)));
-                const double G0 = __builtin_fma(e0, inv, -cx[p]);
-                const double G1 = __builtin_fma(e1, inv, -cy[p]);
-                const double G2 = __builtin_fma(e2, inv, -cz[p]);
-                const double sf = gs_drlv2_fma(u, G0, G1, G2);
+                const double a[3] = {ax[p], ay[p], az[p]}, c[3] = {cx[p], cy[p], cz[p]};
+                const double sf = gs_scan_fast_drlv2(u, a, c, t0, t1, t2, ds);
                 // candidates (and non-finite values) take the exact route
                 const bool cand = !(sf >= hi);
                 bool take = false;

@@ -60,13 +60,12 @@ int set_err(int code, const char* fmt, ...) {
         if (c_ != GS_OK) return c_; \
     } while (0)
 
-// Per-device state: a partial-sum buffer for the one-grain refinement and
-// a few int counters (work queues), allocated once per device.
+// Per-device state: a few int counters (default work queues), allocated
+// once per device.
 struct DeviceCtx {
     bool ready = false;
     int cus = 0;
-    double* partials = nullptr;  // GS_MAX_PARTIAL_BLOCKS * GS_NPART
-    int* counters = nullptr;     // GS_NCOUNTERS ints
+    int* counters = nullptr;  // GS_NCOUNTERS ints
 };
 
 constexpr int GS_MAX_PARTIAL_BLOCKS = 1024;
@@ -86,7 +85,6 @@ int get_ctx(DeviceCtx** out) {
     DeviceCtx& c = g_ctx[dev];
     if (!c.ready) {
         GS_HIP(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
-        GS_HIP(hipMalloc(&c.partials, sizeof(double) * GS_MAX_PARTIAL_BLOCKS * GS_NPART));
         GS_HIP(hipMalloc(&c.counters, sizeof(int) * GS_NCOUNTERS));
         c.ready = true;
     }
@@ -114,6 +112,27 @@ int fill_f64(double* p, double v, long long n, hipStream_t st) {
     int nb = (int)std::min<long long>(blocks_for(n, GS_BLOCK), 4096);
     hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(GS_BLOCK), 0, st, p, v, n);
     GS_LAUNCHED();
+    return GS_OK;
+}
+
+// Block partials of the one-grain refinement: one buffer per host thread
+// and device, so that calls from different host threads never share it
+// (score_and_refine and refine_assigned are thread-safe in the reference).
+struct PartialsTL {
+    int dev = -1;
+    double* buf = nullptr;
+};
+thread_local PartialsTL g_partials;
+
+int get_partials(double** out) {
+    int dev = 0;
+    GS_HIP(hipGetDevice(&dev));
+    if (g_partials.dev != dev || !g_partials.buf) {
+        g_partials = PartialsTL();
+        GS_HIP(hipMalloc(&g_partials.buf, sizeof(double) * GS_MAX_PARTIAL_BLOCKS * GS_NPART));
+        g_partials.dev = dev;
+    }
+    *out = g_partials.buf;
     return GS_OK;
 }
 
@@ -199,13 +218,21 @@ int launch_score_many(const double* ubis, int K, const double* gv, int ng, doubl
             case 16: hipLaunchKernelGGL(k_score_many<16>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
             default: hipLaunchKernelGGL(k_score_many<8>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
         }
-    } else {
+    } else if (mode == GS_SCORE_CERTIFIED) {
         switch (P) {
-            case 1: hipLaunchKernelGGL(k_score_many_fast<1>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 2: hipLaunchKernelGGL(k_score_many_fast<2>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 4: hipLaunchKernelGGL(k_score_many_fast<4>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 16: hipLaunchKernelGGL(k_score_many_fast<16>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            default: hipLaunchKernelGGL(k_score_many_fast<8>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 1: hipLaunchKernelGGL((k_score_many_fast<1, false>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 2: hipLaunchKernelGGL((k_score_many_fast<2, false>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 4: hipLaunchKernelGGL((k_score_many_fast<4, false>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 16: hipLaunchKernelGGL((k_score_many_fast<16, false>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            default: hipLaunchKernelGGL((k_score_many_fast<8, false>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+        }
+    } else {  // GS_SCORE_CERTIFIED_OUTLINE
+        switch (P) {
+            case 1: hipLaunchKernelGGL((k_score_many_fast<1, true>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 2: hipLaunchKernelGGL((k_score_many_fast<2, true>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 4: hipLaunchKernelGGL((k_score_many_fast<4, true>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 16: hipLaunchKernelGGL((k_score_many_fast<16, true>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            default: hipLaunchKernelGGL((k_score_many_fast<8, true>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
         }
     }
     GS_LAUNCHED();
@@ -235,14 +262,16 @@ int launch_refine_one(int mode, double* ubi, const double* gv, int ng, double to
     GS_CHECK(check_ng(ng));
     const int nb = std::min(GS_MAX_PARTIAL_BLOCKS, blocks_for(ng, GS_BLOCK * 4));
     const double tolsq = tol * tol;
+    double* partials;
+    GS_CHECK(get_partials(&partials));
     if (mode == 0)
         hipLaunchKernelGGL(k_refine_partials<0>, dim3(nb), dim3(GS_BLOCK), 0, st, ubi, gv, ng,
-                           tolsq, labels, label, ctx->partials);
+                           tolsq, labels, label, partials);
     else
         hipLaunchKernelGGL(k_refine_partials<1>, dim3(nb), dim3(GS_BLOCK), 0, st, ubi, gv, ng,
-                           tolsq, labels, label, ctx->partials);
+                           tolsq, labels, label, partials);
     GS_LAUNCHED();
-    hipLaunchKernelGGL(k_refine_finalize, dim3(1), dim3(GS_WAVE), 0, st, ctx->partials, nb, ubi,
+    hipLaunchKernelGGL(k_refine_finalize, dim3(1), dim3(GS_WAVE), 0, st, partials, nb, ubi,
                        n_out, mean_out);
     GS_LAUNCHED();
     return GS_OK;
@@ -533,6 +562,35 @@ void gs_selftest_certified_host(const double* ubi, const double* gv, int n, doub
     }
 }
 
+// The fast-path decision of gs_gv_assign_many for single pairs, on the
+// host, next to the reference arithmetic: row k has its own peak (xl, so,
+// co), UBI and translation.  out[k] = 1 if `drlv2 < tol^2` is decided by the
+// fast path as certainly false, 2 if left to the exact route; +4 when the
+// reference's compute_gv + score arithmetic gives drlv2 < tol^2.  Checking
+// aid only.
+void gs_selftest_scan_host(const double* xl, const double* so, const double* co,
+                           const double* ubis, const double* trans, int n, const double* mat,
+                           double wvln, double tol, int* out) {
+    const double ds = 1. / wvln;
+    const double tolsq = tol * tol;
+    for (int k = 0; k < n; k++) {
+        const double* x = xl + 3 * (size_t)k;
+        const double* t = trans + 3 * (size_t)k;
+        const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)k);
+        double a[3], c[3];
+        gs_scan_consts(x[0], x[1], x[2], so[k], co[k], mat, ds, a, c);
+        const double n2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+        const double tt = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        const double hi = tolsq + gs_scan_window(u, ds, tolsq, tt, n2);
+        const double sf = gs_scan_fast_drlv2(u, a, c, t[0], t[1], t[2], ds);
+        int dcs = (sf >= hi) ? 1 : 2;
+        double g0, g1, g2;
+        gs_gv_one(x[0], x[1], x[2], so[k], co[k], mat, ds, t[0], t[1], t[2], g0, g1, g2);
+        if (gs_drlv2(u, g0, g1, g2) < tolsq) dcs |= 4;
+        out[k] = dcs;
+    }
+}
+
 int gs_clear_error(void) {
     g_err.clear();
     return GS_OK;
@@ -540,7 +598,7 @@ int gs_clear_error(void) {
 
 int gs_score_many(const double* ubis, int K, const double* gv, int ng, double tol, int* counts,
                   int peaks_per_lane, int mode, gs_stream_t stream) {
-    if (mode != GS_SCORE_CERTIFIED && mode != GS_SCORE_EXACT)
+    if (mode != GS_SCORE_CERTIFIED && mode != GS_SCORE_EXACT && mode != GS_SCORE_CERTIFIED_OUTLINE)
         return set_err(GS_ERR_ARG, "gs_score_many: unknown mode %d", mode);
     return launch_score_many(ubis, K, gv, ng, tol, counts, peaks_per_lane, mode, S(stream));
 }

@@ -56,6 +56,15 @@ void gs_selftest_sincos_host(const double *x, double *s, double *c, int n);
 void gs_selftest_certified_host(const double *ubi, const double *gv, int n, double tol,
                                 double gmax, int *out);
 
+/* The fast-path decision of gs_gv_assign_many evaluated on the host for
+ * single pairs (row k: peak xl/so/co, UBI ubis[9k..], translation
+ * trans[3k..]), next to the reference arithmetic: out[k] = 1 certainly
+ * fails the tolerance, 2 left to the exact route; +4 when the reference
+ * arithmetic passes it.  No other entry point calls it. */
+void gs_selftest_scan_host(const double *xl, const double *so, const double *co,
+                           const double *ubis, const double *trans, int n, const double *mat,
+                           double wvln, double tol, int *out);
+
 /* ---- 1. reference prototypes (host pointers) ------------------------ */
 
 /* src/closest.c:100-119 — MAGIC rounding vs floor(x+0.5) on 600 points,
@@ -97,6 +106,7 @@ void compute_xlylzl(double s[], double f[], double p[4], double r[9], double dis
  * arithmetic; EXACT uses the reference's arithmetic for every pair. */
 #define GS_SCORE_CERTIFIED 0
 #define GS_SCORE_EXACT 1
+#define GS_SCORE_CERTIFIED_OUTLINE 2 /* CERTIFIED, undecided pairs settled out of line */
 
 /* score for K UBIs (K,3,3) against ng g-vectors -> counts[K] (int32);
  * the loop indexer.score runs over trial UBIs (ImageD11/indexing.py:1090).

@@ -184,3 +184,22 @@ def test_certified_window_special():
     assert (out[1:] & 3 == 2).all() and out[0] == 0
     lib.gs_selftest_certified_host(_lib.hptr(ubi), _lib.hptr(gv), 4, 0.1, np.inf, _lib.hptr(out))
     assert (out[1:] & 3 == 2).all()
+
+
+@pytest.mark.parametrize("tol,hmax", [(0.05, 2), (0.05, 30), (0.25, 10), (0.01, 3)])
+def test_scan_fast_path_is_sound(tol, hmax):
+    """The fused compute_gv + assign fast path only ever skips pairs that
+    the reference arithmetic also puts outside the tolerance."""
+    from helpers import scan_boundary_pairs
+    lib = _lib.load()
+    for seed in range(3):
+        xl, so, co, ubis, t, mat = scan_boundary_pairs(seed, 30000, tol, hmax)
+        out = np.zeros(len(xl), np.int32)
+        lib.gs_selftest_scan_host(_lib.hptr(xl), _lib.hptr(so), _lib.hptr(co), _lib.hptr(ubis),
+                                  _lib.hptr(t), len(xl), _lib.hptr(mat), 0.2, tol,
+                                  _lib.hptr(out))
+        skipped = (out & 3) == 1
+        exact_in = (out & 4) > 0
+        assert not np.any(skipped & exact_in)
+        assert 0 < exact_in.sum() < len(xl)
+        assert skipped.sum() > 0 and (~skipped).sum() > 0
