@@ -46,7 +46,7 @@ SIGNATURES = {
     "compute_gv": (None, [_vp, _vp, _d, _d, _d, _d, _vp, _vp, _i]),
     "compute_xlylzl": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _i]),
     "gs_score_many": (_i, [_vp, _i, _vp, _i, _d, _vp, _i, _i, _vp]),
-    "gs_assign_many": (_i, [_vp, _vp, _i, _vp, _i, _d, _vp, _vp, _vp, _vp]),
+    "gs_assign_many": (_i, [_vp, _vp, _i, _vp, _i, _d, _vp, _vp, _vp, _i, _vp]),
     "gs_score_and_refine_dev": (_i, [_vp, _vp, _i, _d, _vp, _vp, _vp]),
     "gs_refine_assigned_dev": (_i, [_vp, _vp, _i, _vp, _i, _vp, _vp, _vp]),
     "gs_compute_xlylzl_dev": (_i, [_vp, _vp, _dp, _dp, _dp, _vp, _i, _vp]),

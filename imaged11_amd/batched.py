@@ -53,11 +53,12 @@ def score_many(ubis, gv, tol, peaks_per_lane=0, out=None, exact=False, mode=None
     return out
 
 
-def assign_many(ubis, gv, tol, drlv2, labels, grain_labels=None, out=None):
+def assign_many(ubis, gv, tol, drlv2, labels, grain_labels=None, out=None, exact=False):
     """score_and_assign(ubis[g], gv, tol, drlv2, labels, grain_labels[g]) for
     g = 0..K-1 in order; drlv2 (N,) float64 and labels (N,) int32 are
     updated in place.  grain_labels defaults to 0..K-1.  Returns the int32
-    (K,) per-call counts."""
+    (K,) per-call counts.  exact=True evaluates every pair with the
+    reference arithmetic instead of the certified fast path (same results)."""
     lib = _ready()
     _need(ubis, torch.float64, "ubis", (None, 3, 3))
     _need(gv, torch.float64, "gv", (None, 3))
@@ -70,8 +71,9 @@ def assign_many(ubis, gv, tol, drlv2, labels, grain_labels=None, out=None):
     _need(grain_labels, torch.int32, "grain_labels", (K,))
     if out is None:
         out = torch.empty(K, dtype=torch.int32, device=gv.device)
+    mode = _lib.SCORE_EXACT if exact else _lib.SCORE_CERTIFIED
     _lib.check(lib.gs_assign_many(dptr(ubis), dptr(grain_labels), K, dptr(gv), N, float(tol),
-                                  dptr(drlv2), dptr(labels), dptr(out), stream_ptr()),
+                                  dptr(drlv2), dptr(labels), dptr(out), mode, stream_ptr()),
                "assign_many")
     return out
 

@@ -204,7 +204,8 @@ def score_and_assign(ubi, gv, tol, drlv2, labels, label):
         gl = torch.tensor([int(label)], dtype=torch.int32, device=g.device)
         out = torch.empty(1, dtype=torch.int32, device=g.device)
         _lib.check(lib.gs_assign_many(dptr(u), dptr(gl), 1, dptr(g), ng, float(tol), dptr(d),
-                                      dptr(lab), dptr(out), stream_ptr()), "score_and_assign")
+                                      dptr(lab), dptr(out), _lib.SCORE_CERTIFIED, stream_ptr()),
+                   "score_and_assign")
         return int(out.item())
     u = _in_array(ubi, np.float64, (3, 3), "ubi")
     g = _in_array(gv, np.float64, (None, 3), "gv")

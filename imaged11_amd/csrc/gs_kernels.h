@@ -266,6 +266,86 @@ __global__ __launch_bounds__(GS_BLOCK) void k_assign_many(
     }
 }
 
+// The same loop with the certified decision of k_score_many_fast: a pair
+// whose FMA drlv2 is >= tol^2 + E certainly fails `drlv2 < tol^2`, so only
+// the reference's else branch (label == L -> -1) can apply; every other
+// pair is recomputed with the reference arithmetic and the reference's
+// rule is applied to that value, so labels, stored drlv2 and counts are
+// those of k_assign_many.
+template <int P>
+__global__ __launch_bounds__(GS_BLOCK) void k_assign_many_fast(
+    const double* __restrict__ ubis, const int* __restrict__ glabels, int K,
+    const double* __restrict__ gv, int ng, double tolsq, double* __restrict__ drlv2,
+    int* __restrict__ labels, int* __restrict__ counts) {
+    __shared__ int lcnt[GS_GRAIN_CHUNK];
+    double x[P], y[P], z[P], d[P];
+    int lab[P];
+    double gm = 0.0;
+    const long long base = (long long)blockIdx.x * (GS_BLOCK * P) + threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        long long k = base + (long long)p * GS_BLOCK;
+        if (k < ng) {
+            x[p] = gv[3 * k + 0];
+            y[p] = gv[3 * k + 1];
+            z[p] = gv[3 * k + 2];
+            d[p] = drlv2[k];
+            lab[p] = labels[k];
+        } else {
+            x[p] = y[p] = z[p] = __builtin_nan("");
+            d[p] = 0.0;
+            lab[p] = -1;
+        }
+        gm = fmax(gm, fmax(gs_abs_finite(x[p]), fmax(gs_abs_finite(y[p]), gs_abs_finite(z[p]))));
+    }
+    const double gmax = gs_wave_max(gm);
+    const int lane = threadIdx.x & (GS_WAVE - 1);
+    for (int c0 = 0; c0 < K; c0 += GS_GRAIN_CHUNK) {
+        const int c1 = min(K, c0 + GS_GRAIN_CHUNK);
+        for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) lcnt[i] = 0;
+        __syncthreads();
+        for (int g = c0; g < c1; g++) {
+            const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)g);
+            const int L = glabels[g];
+            const double hi = tolsq + gs_cert_window(u, gmax, tolsq);
+            int c = 0;
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                const double sf = gs_drlv2_fma(u, x[p], y[p], z[p]);
+                bool take = false;
+                if (!(sf >= hi)) {  // candidate (or non-finite): reference arithmetic
+                    const double s = gs_drlv2(u, x[p], y[p], z[p]);
+                    take = (s < tolsq) && (s < d[p]);
+                    if (take) {
+                        lab[p] = L;
+                        d[p] = s;
+                    } else if (lab[p] == L) {
+                        lab[p] = -1;
+                    }
+                } else if (lab[p] == L) {
+                    lab[p] = -1;
+                }
+                c += __popcll(__ballot(take));
+            }
+            if (lane == 0 && c) atomicAdd(&lcnt[g - c0], c);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) {
+            int c = lcnt[i];
+            if (c) atomicAdd(&counts[c0 + i], c);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        long long k = base + (long long)p * GS_BLOCK;
+        if (k < ng) {
+            drlv2[k] = d[p];
+            labels[k] = lab[p];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // One-grain least-squares refinement over all ng peaks, two passes:
 // per-block partial sums (n, sum drlv2, R, H) then a one-wave finalize that

@@ -240,7 +240,8 @@ int launch_score_many(const double* ubis, int K, const double* gv, int ng, doubl
 }
 
 int launch_assign_many(const double* ubis, const int* glabels, int K, const double* gv, int ng,
-                       double tol, double* drlv2, int* labels, int* counts, hipStream_t st) {
+                       double tol, double* drlv2, int* labels, int* counts, int mode,
+                       hipStream_t st) {
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
     if (K < 0 || ng < 0) return set_err(GS_ERR_ARG, "negative size K=%d ng=%d", K, ng);
@@ -249,8 +250,12 @@ int launch_assign_many(const double* ubis, const int* glabels, int K, const doub
     const double tolsq = tol * tol;
     constexpr int P = 2;
     const long long nb = (ng + (long long)GS_BLOCK * P - 1) / ((long long)GS_BLOCK * P);
-    hipLaunchKernelGGL(k_assign_many<P>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis, glabels,
-                       K, gv, ng, tolsq, drlv2, labels, counts);
+    if (mode == GS_SCORE_EXACT)
+        hipLaunchKernelGGL(k_assign_many<P>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis,
+                           glabels, K, gv, ng, tolsq, drlv2, labels, counts);
+    else
+        hipLaunchKernelGGL(k_assign_many_fast<P>, dim3((unsigned)nb), dim3(GS_BLOCK), 0, st, ubis,
+                           glabels, K, gv, ng, tolsq, drlv2, labels, counts);
     GS_LAUNCHED();
     return GS_OK;
 }
@@ -480,7 +485,8 @@ int score_and_assign(double ubi[3][3], double gv[][3], double tol, double* drlv2
         set_err(GS_ERR_HIP, "score_and_assign: copy to device failed");
         return -1;
     }
-    if (launch_assign_many(du, dn + 1, 1, dg, ng, tol, dd, dl, dn, st) != GS_OK) return -1;
+    if (launch_assign_many(du, dn + 1, 1, dg, ng, tol, dd, dl, dn, GS_SCORE_CERTIFIED, st) != GS_OK)
+        return -1;
     ok = hipMemcpyAsync(&h, dn, 4, hipMemcpyDeviceToHost, st) == hipSuccess;
     if (ok && ng > 0)
         ok = hipMemcpyAsync(drlv2, dd, (size_t)ng * 8, hipMemcpyDeviceToHost, st) == hipSuccess &&
@@ -604,8 +610,12 @@ int gs_score_many(const double* ubis, int K, const double* gv, int ng, double to
 }
 
 int gs_assign_many(const double* ubis, const int* grain_labels, int K, const double* gv, int ng,
-                   double tol, double* drlv2, int* labels, int* counts, gs_stream_t stream) {
-    return launch_assign_many(ubis, grain_labels, K, gv, ng, tol, drlv2, labels, counts, S(stream));
+                   double tol, double* drlv2, int* labels, int* counts, int mode,
+                   gs_stream_t stream) {
+    if (mode != GS_SCORE_CERTIFIED && mode != GS_SCORE_EXACT)
+        return set_err(GS_ERR_ARG, "gs_assign_many: unknown mode %d", mode);
+    return launch_assign_many(ubis, grain_labels, K, gv, ng, tol, drlv2, labels, counts, mode,
+                              S(stream));
 }
 
 int gs_score_and_refine_dev(double* ubi, const double* gv, int ng, double tol, int* n_out,

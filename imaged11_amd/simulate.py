@@ -262,3 +262,125 @@ def refine_lists(owner, K, seed=3, median=300, sigma=0.8, lo=20, hi=2000):
             take = np.concatenate([take, extra])
         idx[off[g]:off[g + 1]] = rng.permutation(take)
     return idx, off
+
+
+def fast_peaks(ub, translations, pars, n_peaks, seed=5, spurious=0.15, gmax=1.6,
+               omega_range=(0.0, 180.0), omega_step=0.25, jitter_px=0.5, npx=2048, dty=0.0,
+               candidates=2000, chunk=1000, device="cpu"):
+    """Large peak tables quickly: the forward model of `simulate_peaks`,
+    vectorised over grains with torch (on `device`), drawing `candidates`
+    random hkl per grain from its bounding box instead of enumerating the
+    whole reciprocal-lattice ball.  Same outputs: dict(sc, fc, omega, dty,
+    grain) numpy arrays of exactly n_peaks rows."""
+    import torch
+    dev = torch.device(device)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    f64 = dict(dtype=torch.float64, device=dev)
+    p, r, dist = geometry.pack_xlylzl(pars)
+    sign, wvln, wedge, chi = geometry.pack_gv(pars)
+    mat = torch.tensor(geometry.wedge_chi_matrix(wedge, chi).reshape(3, 3), **f64)
+    R3 = torch.tensor(r.reshape(3, 3), **f64)
+    c1, c2 = R3[:, 1], R3[:, 2]
+    w = torch.linalg.cross(c1, c2)
+    distv = torch.tensor(dist, **f64)
+    ub_t = torch.as_tensor(ub, **f64)
+    tr_t = torch.as_tensor(np.asarray(translations, float), **f64)
+    K = ub_t.shape[0]
+    hm = torch.floor(gmax * torch.linalg.norm(torch.linalg.inv(ub_t), dim=2))  # (K, 3)
+    out_rows, out_own = [], []
+    for g0 in range(0, K, chunk):
+        g1 = min(K, g0 + chunk)
+        G = g1 - g0
+        u = torch.rand((G, candidates, 3), generator=gen, **f64)
+        hkl = torch.floor(u * (2 * hm[g0:g1, None, :] + 1)) - hm[g0:g1, None, :]
+        key = ((hkl + 512) * torch.tensor([1 << 20, 1 << 10, 1], **f64)).sum(-1)
+        gvec = torch.einsum("gij,gcj->gci", ub_t[g0:g1], hkl)
+        gg = (gvec * gvec).sum(-1)
+        own = torch.arange(g0, g1, device=dev)[:, None].expand(G, candidates)
+        keep = (gg <= gmax * gmax) & (hkl.abs().sum(-1) > 0)
+        gvec, gg, own, key = gvec[keep], gg[keep], own[keep], key[keep]
+        # one copy of each (grain, hkl)
+        uk = torch.unique(own.double() * 2.0 ** 31 + key, return_inverse=True)[1]
+        first = torch.full((int(uk.max()) + 1 if uk.numel() else 0,), len(uk), device=dev,
+                           dtype=torch.long)
+        first.scatter_reduce_(0, uk, torch.arange(len(uk), device=dev), reduce="amin")
+        gvec, gg, own = gvec[first], gg[first], own[first]
+        A = mat[0, 0] * gvec[:, 0] + mat[0, 1] * gvec[:, 1]
+        B = mat[0, 1] * gvec[:, 0] - mat[0, 0] * gvec[:, 1]
+        C = -0.5 * wvln * gg - mat[0, 2] * gvec[:, 2]
+        Rn = torch.hypot(A, B)
+        ok = (Rn > 0) & (C.abs() < Rn)
+        gvec, A, B, C, Rn, own = gvec[ok], A[ok], B[ok], C[ok], Rn[ok], own[ok]
+        phi = torch.atan2(B, A)
+        ac = torch.arccos(C / Rn)
+        for sgn in (1.0, -1.0):
+            om = torch.rad2deg(phi + sgn * ac) * sign
+            om = torch.remainder(om - omega_range[0], 360.0) + omega_range[0]
+            inr = (om >= omega_range[0]) & (om < omega_range[1])
+            gi, omi, owi = gvec[inr], om[inr], own[inr]
+            th = torch.deg2rad(omi * sign)
+            co, so = torch.cos(th), torch.sin(th)
+            wv = torch.stack([co * gi[:, 0] - so * gi[:, 1], so * gi[:, 0] + co * gi[:, 1],
+                              gi[:, 2]], 1)
+            nh = wvln * (wv @ mat.T)
+            nh[:, 0] += 1.0
+            t = tr_t[owi]
+            uu = torch.stack([co * t[:, 0] - so * t[:, 1], so * t[:, 0] + co * t[:, 1], t[:, 2]],
+                             1)
+            o = uu @ mat.T
+            if dty > 0:
+                shift = (torch.rand(len(o), generator=gen, **f64) * 2 - 1) * dty
+            else:
+                shift = torch.zeros(len(o), **f64)
+            o[:, 1] += shift
+            b = distv[None, :] - o
+            s = (b @ w) / (nh @ w)  # ray parameter to the detector plane
+            P = o + s[:, None] * nh - distv[None, :]
+            v1 = P @ c1
+            v2 = P @ c2
+            fc = v1 / p[3] + p[1]
+            sc = v2 / p[2] + p[0]
+            on = (s > 0) & (sc >= 0) & (sc < npx) & (fc >= 0) & (fc < npx)
+            out_rows.append(torch.stack([sc[on], fc[on], omi[on], shift[on]], 1))
+            out_own.append(owi[on])
+    rows = torch.cat(out_rows)
+    owner = torch.cat(out_own).to(torch.int32)
+    n_real = int(round(n_peaks * (1.0 - spurious)))
+    if len(rows) < n_real:
+        raise ValueError("simulation produced %d spots, %d needed" % (len(rows), n_real))
+    pick = torch.randperm(len(rows), generator=gen, device=dev)[:n_real]
+    rows, owner = rows[pick], owner[pick]
+    rows[:, 0] += torch.randn(n_real, generator=gen, **f64) * jitter_px
+    rows[:, 1] += torch.randn(n_real, generator=gen, **f64) * jitter_px
+    nbin = torch.floor((rows[:, 2] - omega_range[0]) / omega_step)
+    rows[:, 2] = omega_range[0] + (nbin + 0.5) * omega_step
+    n_sp = n_peaks - n_real
+    nfr = int(round((omega_range[1] - omega_range[0]) / omega_step))
+    sp = torch.stack([torch.rand(n_sp, generator=gen, **f64) * npx,
+                      torch.rand(n_sp, generator=gen, **f64) * npx,
+                      omega_range[0] + (torch.randint(0, nfr, (n_sp,), generator=gen,
+                                                      device=dev).double() + 0.5) * omega_step,
+                      (torch.rand(n_sp, generator=gen, **f64) * 2 - 1) * dty], 1)
+    allrows = torch.cat([rows, sp])
+    allown = torch.cat([owner, torch.full((n_sp,), -1, dtype=torch.int32, device=dev)])
+    perm = torch.randperm(n_peaks, generator=gen, device=dev)
+    allrows, allown = allrows[perm].cpu().numpy(), allown[perm].cpu().numpy()
+    return {"sc": np.ascontiguousarray(allrows[:, 0]), "fc": np.ascontiguousarray(allrows[:, 1]),
+            "omega": np.ascontiguousarray(allrows[:, 2]), "dty": np.ascontiguousarray(allrows[:, 3]),
+            "grain": allown}
+
+
+def scan_workload(K=20000, N=8_000_000, seed=5, spurious=0.15, dty=500.0, device="cpu"):
+    """The large scanning workload: K grains (cells as scoring_workload)
+    with translations, N peaks with per-peak dty in U[-dty, dty] and 15%
+    spurious peaks, generated with `fast_peaks`."""
+    rng = np.random.default_rng(seed)
+    pars = detector_parameters(rng)
+    ub, cells = random_grains(K, rng)
+    trans = rng.uniform(-200.0, 200.0, size=(K, 3))
+    trans[:, 2] *= 0.25
+    peaks = fast_peaks(ub, trans, pars, N, seed=seed + 1000, spurious=spurious, dty=dty,
+                       device=device)
+    return {"pars": pars, "ubi": np.ascontiguousarray(np.linalg.inv(ub)), "ub": ub,
+            "trans": trans, "cells": cells, "peaks": peaks}

@@ -116,9 +116,10 @@ int gs_score_many(const double *ubis, int K, const double *gv, int ng, double to
 
 /* score_and_assign for K grains in order, label grain_labels[g] for grain
  * g, on shared drlv2/labels (ImageD11/refinegrains.py:718-734,
- * ImageD11/indexing.py:885-889); counts[g] = value of the g-th call. */
+ * ImageD11/indexing.py:885-889); counts[g] = value of the g-th call.
+ * mode GS_SCORE_CERTIFIED or GS_SCORE_EXACT, same results. */
 int gs_assign_many(const double *ubis, const int *grain_labels, int K, const double *gv,
-                   int ng, double tol, double *drlv2, int *labels, int *counts,
+                   int ng, double tol, double *drlv2, int *labels, int *counts, int mode,
                    gs_stream_t stream);
 
 /* one-grain score_and_refine / refine_assigned on device memory; ubi (9

@@ -243,8 +243,9 @@ def test_sf2gv_matches_separate_kernels(dev, orc, workload):
     assert np.array_equal(xl.cpu().numpy(), xl_o)
 
 
+@pytest.mark.parametrize("exact", [False, True])
 @pytest.mark.parametrize("d0,l0", [(1.0, -1), (1.0, 0), (2.0, -1)])
-def test_assign_many(dev, orc, workload, d0, l0):
+def test_assign_many(dev, orc, workload, d0, l0, exact):
     wl = workload
     _, gv = _oracle_gv(orc, wl)
     n = len(gv)
@@ -254,7 +255,7 @@ def test_assign_many(dev, orc, workload, d0, l0):
     want_c = orc.assign_many(wl["ubi"], labels_of, gv, 0.05, d, lab)
     dd, ll = T(np.full(n, d0), dev), T(np.full(n, l0), dev, torch.int32)
     c = batched.assign_many(T(wl["ubi"], dev), T(gv, dev), 0.05, dd, ll,
-                            grain_labels=T(labels_of, dev, torch.int32))
+                            grain_labels=T(labels_of, dev, torch.int32), exact=exact)
     assert np.array_equal(c.cpu().numpy(), want_c)
     assert np.array_equal(ll.cpu().numpy(), lab)
     assert np.array_equal(dd.cpu().numpy().view(np.int64), d.view(np.int64))
@@ -346,3 +347,12 @@ def test_score_certified_boundary(dev, orc, tol, hmax):
             fast = batched.score_many(u, g, tol, peaks_per_lane=ppl).item()
             exact = batched.score_many(u, g, tol, peaks_per_lane=ppl, exact=True).item()
             assert fast == exact == want
+        # the assignment state machine on the same boundary pairs
+        for d0 in (1.0, tol * tol * (1 + 1e-13)):
+            d, lab = np.full(len(gv), d0), np.zeros(len(gv), np.int32)
+            n_o = orc.score_and_assign(ubi, gv, tol, d, lab, 0)
+            for ex in (False, True):
+                dd, ll = T(np.full(len(gv), d0), dev), T(np.zeros(len(gv)), dev, torch.int32)
+                c = batched.assign_many(u, g, tol, dd, ll, exact=ex)
+                assert int(c[0]) == n_o and np.array_equal(ll.cpu().numpy(), lab)
+                assert np.array_equal(dd.cpu().numpy().view(np.int64), d.view(np.int64))

@@ -126,3 +126,28 @@ def test_index_drivers(dev, orc):
     d, lab = np.ones(len(k["gv"])), np.zeros(len(k["gv"]), np.int32)
     want = [orc.score_and_assign(u, k["gv"], 0.01, d, lab, i) for i, u in enumerate(k["ubis"])]
     assert c2.cpu().numpy().tolist() == want and np.array_equal(l2.cpu().numpy(), lab)
+
+
+def test_scan_workload_labels_bitexact(dev, orc):
+    """2000 grains x 8e5 peaks (the large-scan generator at 1/10 scale):
+    labels, drlv2 and the 2000 per-call counts equal the oracle's loop."""
+    import os
+    wl = simulate.scan_workload(K=2000, N=800_000, seed=6, device="cuda")
+    pk, pars = wl["peaks"], wl["pars"]
+    p, r, dist = geometry.pack_xlylzl(pars)
+    sign, wvln, wedge, chi = geometry.pack_gv(pars)
+    n = len(pk["sc"])
+    xl = np.zeros((n, 3))
+    orc.compute_xlylzl(pk["sc"], pk["fc"], p, r, dist, xl)
+    xl[:, 1] -= pk["dty"]
+    so, co = orc.sincos_omega(pk["omega"], sign)
+    mat = orc.wedge_chi_matrix(wedge, chi)
+    d, lab = np.ones(n), np.full(n, -1, np.int32)
+    want = orc.gv_assign_many(xl, so, co, mat, wvln, wl["ubi"], wl["trans"], np.arange(2000),
+                              0.05, d, lab, nthreads=min(16, os.cpu_count() or 1))
+    dd, ll = T(np.ones(n), dev), T(np.full(n, -1), dev, torch.int32)
+    got = batched.gv_assign_many(T(xl, dev), T(so, dev), T(co, dev), wvln, T(wl["ubi"], dev),
+                                 T(wl["trans"], dev), 0.05, dd, ll, mat=mat)
+    assert np.array_equal(got.cpu().numpy(), want)
+    assert np.array_equal(ll.cpu().numpy(), lab)
+    assert np.array_equal(dd.cpu().numpy().view(np.int64), d.view(np.int64))
