@@ -69,6 +69,14 @@ SIGNATURES = {
     "gs_joint_adjoint": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _i, _vp, ctypes.c_longlong,
                               _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "gs_graph_destroy": (_i, [_vp]),
+    "closest": (None, [_vp, _vp, _ip, _dp, _i, _i]),
+    "quickorient": (None, [_vp, _vp]),
+    "gs_gv_ds": (_i, [_vp, _i, _vp, _vp]),
+    "gs_ring_assign": (_i, [_vp, _i, _vp, _i, _d, _vp, _vp]),
+    "gs_unit_rows": (_i, [_vp, _i, _vp, _i, _vp, _vp]),
+    "gs_find_hits": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _vp, _vp]),
+    "gs_find_all": (_i, [_vp, _i, _vp, _i, _vp, _i, _d, _vp, _vp, ctypes.c_longlong, _vp]),
+    "gs_orient_hits": (_i, [_vp, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp]),
 }
 
 SCORE_CERTIFIED = 0

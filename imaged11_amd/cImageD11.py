@@ -13,6 +13,8 @@ the extension's:
     compute_gv(xlylzl, omega, omegasign, wvln, wedge, chi, t, gv)
     compute_xlylzl(s, f, p, r, dist, xlylzl)
     verify_rounding(n)
+    ibest, best = closest(x, v)                       # indexer.find
+    quickorient(ubi, bt)                              # unitcell.orient
 
 Argument handling follows f2py: ``intent(in)`` arrays of another dtype or
 layout are converted (copied); ``intent(inout)`` arrays (``gv`` of
@@ -38,7 +40,7 @@ from ._lib import GscoreError, dptr, hptr, darr, stream_ptr
 
 __all__ = [
     "verify_rounding", "score", "score_and_refine", "score_and_assign", "refine_assigned",
-    "compute_gv", "compute_xlylzl", "cimaged11_omp_set_num_threads",
+    "compute_gv", "compute_xlylzl", "closest", "quickorient", "cimaged11_omp_set_num_threads",
     "cimaged11_omp_get_max_threads", "GscoreError",
 ]
 
@@ -305,4 +307,37 @@ def compute_xlylzl(s, f, p, r, dist, xlylzl):
     lib = _lib_ready()
     lib.compute_xlylzl(hptr(ss), hptr(ff), hptr(pp), hptr(rr), hptr(dd), hptr(out), n)
     _lib.check_host_call("compute_xlylzl")
+    return None
+
+
+def closest(x, v):
+    """(ibest, best): the index in x of the smallest |x[i] - v[j]| (first in
+    x-major order) and that difference; (0, 99.0) when none is below 99
+    (``src/closest.c:188-207``, f2py ``src/_cImageD11.pyf:88-103``).
+    indexer.find calls it with x = cosines between peaks, v = the cosines
+    the unit cell allows (``ImageD11/indexing.py:713``)."""
+    if _is_dev(x) or _is_dev(v):
+        raise ValueError("closest takes host arrays; the batched device search is "
+                         "imaged11_amd.indexing.find_hits")
+    xx = _in_array(x, np.float64, (None,), "x")
+    vv = _in_array(v, np.float64, (None,), "v")
+    lib = _lib_ready()
+    ib = ctypes.c_int(0)
+    best = ctypes.c_double(0.0)
+    lib.closest(hptr(xx), hptr(vv), ctypes.byref(ib), ctypes.byref(best), xx.shape[0],
+                vv.shape[0])
+    _lib.check_host_call("closest")
+    return ib.value, best.value
+
+
+def quickorient(ubi, bt):
+    """Orientation from two g-vectors held in rows 0 and 1 of ubi, written
+    over ubi in place: ubi = bt . [t1; t2; t3] with t1 along g1, t3 along
+    g1 x g2 (``src/cdiffraction.c:293-328``, f2py
+    ``src/_cImageD11.pyf:603-613``; ubi intent(inout), bt intent(in))."""
+    u = _inout_array(ubi, np.float64, (3, 3), "ubi")
+    b = _in_array(bt, np.float64, (3, 3), "bt")
+    lib = _lib_ready()
+    lib.quickorient(hptr(u), hptr(b))
+    _lib.check_host_call("quickorient")
     return None

@@ -24,6 +24,7 @@
 #include "gs_kernels.h"
 #include "gs_adjoint.h"
 #include "gs_assign_gv.h"
+#include "gs_index.h"
 #include "../../include/gscore.h"
 
 namespace {
@@ -723,14 +724,14 @@ int gs_refine_graph_create(gs_graph_t* out, double* ubis, int K, const double* g
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (rc != GS_OK) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc;
     }
     if (e != hipSuccess) return set_err(GS_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
     hipGraphExec_t exec = nullptr;
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         return set_err(GS_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
     gs_graph_s* g = new gs_graph_s;
@@ -845,6 +846,145 @@ int gs_gv_lists(const double* xlylzl, const double* so, const double* co, int ng
     gs_goniometer gp = make_gonio(1.0, wvln, wedge, chi, nullptr, mat);
     hipLaunchKernelGGL(k_gv_lists, dim3(blocks_for(n_idx, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
                        xlylzl, so, co, ng, gp, trans, K, idx, owner, n_idx, out);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+// ---- indexer ---------------------------------------------------------------
+
+void closest(double x[], double v[], int* ibest, double* best, int nx, int nv) {
+    if (nx < 0 || nv < 0) {
+        set_err(GS_ERR_ARG, "closest: negative length");
+        return;
+    }
+    std::vector<char*> sl;
+    if (stage_slices({(size_t)nx * 8, (size_t)nv * 8, 4, 8}, sl) != GS_OK) return;
+    double* dx = reinterpret_cast<double*>(sl[0]);
+    double* dv = reinterpret_cast<double*>(sl[1]);
+    int* di = reinterpret_cast<int*>(sl[2]);
+    double* db = reinterpret_cast<double*>(sl[3]);
+    hipStream_t st = nullptr;
+    if ((nx > 0 && hipMemcpyAsync(dx, x, (size_t)nx * 8, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (nv > 0 && hipMemcpyAsync(dv, v, (size_t)nv * 8, hipMemcpyHostToDevice, st) != hipSuccess)) {
+        set_err(GS_ERR_HIP, "closest: copy to device failed");
+        return;
+    }
+    hipLaunchKernelGGL(k_closest, dim3(1), dim3(GS_BLOCK), 0, st, dx, nx, dv, nv, di, db);
+    if (hipGetLastError() != hipSuccess) {
+        set_err(GS_ERR_HIP, "closest: kernel launch failed");
+        return;
+    }
+    int hi = 0;
+    double hb = 0.0;
+    if (hipMemcpyAsync(&hi, di, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&hb, db, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "closest: copy back failed");
+        return;
+    }
+    *ibest = hi;
+    *best = hb;
+}
+
+void quickorient(double UBI[9], double BT[9]) {
+    std::vector<char*> sl;
+    if (stage_slices({72, 72}, sl) != GS_OK) return;
+    double* du = reinterpret_cast<double*>(sl[0]);
+    double* db = reinterpret_cast<double*>(sl[1]);
+    hipStream_t st = nullptr;
+    if (hipMemcpyAsync(du, UBI, 72, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(db, BT, 72, hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "quickorient: copy to device failed");
+        return;
+    }
+    hipLaunchKernelGGL(k_quickorient, dim3(1), dim3(64), 0, st, du, db);
+    if (hipGetLastError() != hipSuccess) {
+        set_err(GS_ERR_HIP, "quickorient: kernel launch failed");
+        return;
+    }
+    if (hipMemcpyAsync(UBI, du, 72, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        set_err(GS_ERR_HIP, "quickorient: copy back failed");
+}
+
+int gs_gv_ds(const double* gv, int ng, double* ds, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(ng));
+    if (ng == 0) return GS_OK;
+    hipLaunchKernelGGL(k_gv_ds, dim3(blocks_for(ng, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream), gv,
+                       ng, ds);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_ring_assign(const double* ds, int n, const double* ringds, int nr, double tol, int* ra,
+                   gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (n < 0 || nr < 0) return set_err(GS_ERR_ARG, "gs_ring_assign: negative size");
+    if (n == 0) return GS_OK;
+    hipLaunchKernelGGL(k_ring_assign, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
+                       ds, n, ringds, nr, tol, ra);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_unit_rows(const double* gv, int ng, const int* idx, int n, double* out,
+                 gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (n < 0 || ng < 0) return set_err(GS_ERR_ARG, "gs_unit_rows: negative size");
+    if (n == 0) return GS_OK;
+    hipLaunchKernelGGL(k_unit_rows, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
+                       gv, ng, idx, n, out);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_find_hits(const double* n1, int N1, const double* n2, int N2, const double* cs, int nv,
+                 int* ibest, double* best, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (N1 < 0 || N2 < 0 || nv < 0) return set_err(GS_ERR_ARG, "gs_find_hits: negative size");
+    if (nv > GS_MAX_COSES)
+        return set_err(GS_ERR_ARG, "gs_find_hits: %d cosines (at most %d)", nv, GS_MAX_COSES);
+    if (N1 == 0) return GS_OK;
+    const int nb = (N1 + GS_FIND_ROWS - 1) / GS_FIND_ROWS;
+    hipLaunchKernelGGL(k_find_hits, dim3(nb), dim3(GS_BLOCK), 0, S(stream), n1, N1, n2, N2, cs, nv,
+                       ibest, best);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_find_all(const double* n1, int N1, const double* n2, int N2, const double* cs, int nv,
+                double mtol, unsigned long long* counter, long long* keys, long long cap,
+                gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (N1 < 0 || N2 < 0 || nv < 0 || cap < 0)
+        return set_err(GS_ERR_ARG, "gs_find_all: negative size");
+    if (nv > GS_MAX_COSES)
+        return set_err(GS_ERR_ARG, "gs_find_all: %d cosines (at most %d)", nv, GS_MAX_COSES);
+    if (!counter) return set_err(GS_ERR_ARG, "gs_find_all: counter is NULL");
+    GS_CHECK(fill_i32(reinterpret_cast<int*>(counter), 0, 2, S(stream)));
+    if (N1 == 0) return GS_OK;
+    hipLaunchKernelGGL(k_find_all, dim3(N1), dim3(GS_BLOCK), 0, S(stream), n1, N1, n2, N2, cs, nv,
+                       mtol, counter, keys, cap);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_orient_hits(const double* gv, int ng, const int* hi, const int* hj, int nh,
+                   const double* c2ab, int nc, const double* bt, double* ubis, int* choice,
+                   gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (ng < 0 || nh < 0) return set_err(GS_ERR_ARG, "gs_orient_hits: negative size");
+    if (nc < 1) return set_err(GS_ERR_ARG, "gs_orient_hits: empty cosine table");
+    if (nh == 0) return GS_OK;
+    hipLaunchKernelGGL(k_orient_hits, dim3(blocks_for(nh, GS_BLOCK)), dim3(GS_BLOCK), 0,
+                       S(stream), gv, ng, hi, hj, nh, c2ab, nc, bt, ubis, choice);
     GS_LAUNCHED();
     return GS_OK;
 }

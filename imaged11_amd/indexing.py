@@ -1,0 +1,342 @@
+"""Batched GPU indexer: ``ImageD11/indexing.py``'s ``indexer`` for the
+ring-pair search (``assigntorings``, ``find``, ``scorethem``, ``score``,
+``getind``).
+
+The reference runs ``find`` as a Python loop over ring-1 peaks, each a
+numpy dot product plus one ``closest`` call, and ``scorethem`` as a Python
+loop over the hits, each an ``orient`` plus one ``score`` over all peaks.
+Here:
+
+* ``assigntorings`` is one kernel (``gs_ring_assign``);
+* ``find`` is one kernel over all ring-1 x ring-2 peak pairs
+  (``gs_find_hits``, or ``gs_find_all`` for a negative cosine tolerance);
+* ``scorethem`` builds every hit's trial UBI in one kernel
+  (``gs_orient_hits``) and scores all of them in one launch of the
+  many-grain scorer (``gs_score_many``).  The reference's sequential
+  decision loop then runs on those precomputed scores: a hit's score does
+  not depend on earlier decisions, only whether it is looked at does, so
+  the loop visits the hits in the reference's order (last first), skips
+  the same ones, and makes the same expensive follow-ups (orientation
+  variants, ``score_and_refine``, ``getind``, uniqueness) on the GPU only
+  for hits above ``minpks``.
+
+Results (hits, grains, scores, peak assignments) are the reference's for
+the same g-vectors, unit-cell tables and parameters; see the parity tests.
+"""
+import logging
+
+import numpy as np
+import torch
+
+from . import _lib, batched, cImageD11
+from ._lib import dptr, stream_ptr
+from . import unitcell as unitcell_mod
+
+
+def _dev_f64(a, device):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+
+
+def _dev_i32(a, device):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
+
+
+def ring_cosines(uc, ring_1, ring_2):
+    """Distinct cosines between the hkls of two rings, largest first, with
+    +-1 removed and values within 1e-5 merged (ImageD11/indexing.py:
+    656-676)."""
+    h1s = uc.ringhkls[uc.ringds[int(ring_1)]]
+    h2s = uc.ringhkls[uc.ringds[int(ring_2)]]
+    cos = sorted(uc.anglehkls(a, b)[1] for a in h1s for b in h2s)
+    out = []
+    while cos:
+        c = cos.pop()
+        if abs(c - 1.0) < 1e-5 or abs(c + 1.0) < 1e-5:
+            continue
+        if not out or abs(out[-1] - c) > 1e-5:
+            out.append(c)
+    return out
+
+
+def find_hits(gv, i1, i2, cs, cosine_tol):
+    """The hits of indexer.find for ring-1 peaks i1 and ring-2 peaks i2
+    (ImageD11/indexing.py:693-728) as [diff, i, j] rows in the reference's
+    order.  gv: (N,3) float64 device tensor; i1, i2: int index arrays."""
+    lib = _lib.load()
+    _lib.require_gpu()
+    dev = gv.device
+    i1 = np.asarray(i1, dtype=np.int64)
+    i2 = np.asarray(i2, dtype=np.int64)
+    cs = np.ascontiguousarray(cs, dtype=np.float64)
+    N1, N2, nv = len(i1), len(i2), len(cs)
+    if N1 == 0 or N2 == 0:
+        return []
+    n1 = torch.empty((N1, 3), dtype=torch.float64, device=dev)
+    n2 = torch.empty((N2, 3), dtype=torch.float64, device=dev)
+    d1, d2 = _dev_i32(i1, dev), _dev_i32(i2, dev)
+    N = gv.shape[0]
+    _lib.check(lib.gs_unit_rows(dptr(gv), N, dptr(d1), N1, dptr(n1), stream_ptr()), "unit_rows")
+    _lib.check(lib.gs_unit_rows(dptr(gv), N, dptr(d2), N2, dptr(n2), stream_ptr()), "unit_rows")
+    dcs = _dev_f64(cs, dev)
+    tol = float(cosine_tol)
+    if tol > 0:
+        ib = torch.empty(N1, dtype=torch.int32, device=dev)
+        best = torch.empty(N1, dtype=torch.float64, device=dev)
+        _lib.check(lib.gs_find_hits(dptr(n1), N1, dptr(n2), N2, dptr(dcs), nv, dptr(ib),
+                                    dptr(best), stream_ptr()), "find_hits")
+        ib, best = ib.cpu().numpy(), best.cpu().numpy()
+        sel = np.nonzero(best < tol)[0]
+        return [[float(best[i]), int(i1[i]), int(i2[ib[i]])] for i in sel]
+    mtol = -tol
+    counter = torch.zeros(1, dtype=torch.int64, device=dev)
+    _lib.check(lib.gs_find_all(dptr(n1), N1, dptr(n2), N2, dptr(dcs), nv, mtol, dptr(counter),
+                               None, 0, stream_ptr()), "find_all")
+    n = int(counter.item())
+    if n == 0:
+        return []
+    keys = torch.empty(n, dtype=torch.int64, device=dev)
+    _lib.check(lib.gs_find_all(dptr(n1), N1, dptr(n2), N2, dptr(dcs), nv, mtol, dptr(counter),
+                               dptr(keys), n, stream_ptr()), "find_all")
+    keys = torch.sort(keys).values.cpu().numpy()
+    j = keys % N2
+    i = keys // (nv * N2)
+    return [[0.0, int(i1[a]), int(i2[b])] for a, b in zip(i, j)]
+
+
+def orient_hits(uc, ring_1, ring_2, gv, hits):
+    """Trial UBI of every hit as unitcell.orient(ring_1, gv[i], ring_2,
+    gv[j]) makes it (crange <= 0), in one kernel.  Returns (nh,3,3)."""
+    lib = _lib.load()
+    dev = gv.device
+    _, c2ab, matrs = uc.getanglehkls(ring_1, ring_2)
+    hi = _dev_i32([h[1] for h in hits], dev)
+    hj = _dev_i32([h[2] for h in hits], dev)
+    nh = len(hits)
+    ubis = torch.empty((nh, 3, 3), dtype=torch.float64, device=dev)
+    dc = _dev_f64(np.asarray(c2ab, dtype=np.float64), dev)
+    dbt = _dev_f64(np.asarray(matrs, dtype=np.float64).reshape(-1, 3, 3), dev)
+    if nh:
+        _lib.check(lib.gs_orient_hits(dptr(gv), gv.shape[0], dptr(hi), dptr(hj), nh, dptr(dc),
+                                      len(c2ab), dptr(dbt), dptr(ubis), None, stream_ptr()),
+                   "orient_hits")
+    return ubis
+
+
+class indexer:
+    """Finds grain orientations from pairs of peaks on two powder rings
+    (ImageD11/indexing.py:313-1183, the parts on the search path).  Same
+    parameters and attributes: gv, ds, ra, ga, hits, ubis, scores."""
+
+    def __init__(self, unitcell=None, gv=None, cosine_tol=0.002, minpks=10, hkl_tol=0.01,
+                 ring_1=1, ring_2=2, ds_tol=0.005, wavelength=-1, uniqueness=0.5,
+                 eta_range=0.0, max_grains=100, device="cuda"):
+        self.device = torch.device(device)
+        self.unitcell = unitcell
+        self.gv = None
+        self.ra = None
+        if gv is not None:
+            gv = np.asarray(gv)
+            assert gv.shape[1] == 3
+            self.gv = gv.astype(float)
+            self.ds = np.sqrt((gv * gv).sum(axis=1))
+            self.ga = np.zeros(len(self.ds), np.int32) - 1
+            self.gvflat = np.ascontiguousarray(gv, float)
+            self.wedge = 0.0
+        self.cosine_tol = cosine_tol
+        self.wavelength = wavelength
+        self.hkl_tol = hkl_tol
+        self.ring_1 = ring_1
+        self.ring_2 = ring_2
+        self.uniqueness = uniqueness
+        self.minpks = minpks
+        self.ds_tol = ds_tol
+        self.max_grains = max_grains
+        self.eta_range = eta_range
+        self.ubis = []
+        self.scores = []
+        self.hits = []
+        self.omega = None
+        self._gv_dev = None
+
+    # parameters given as strings, as from ImageD11's parameter files
+    _TYPES = {"cosine_tol": float, "hkl_tol": float, "ds_tol": float, "wavelength": float,
+              "uniqueness": float, "eta_range": float, "minpks": int, "ring_1": int,
+              "ring_2": int, "max_grains": int}
+
+    def set_parameters(self, pars):
+        """Set search parameters from a dict (values may be strings)."""
+        for k, v in pars.items():
+            if k in self._TYPES:
+                setattr(self, k, self._TYPES[k](v))
+
+    def _gvd(self):
+        if self._gv_dev is None or self._gv_dev.shape[0] != len(self.gv):
+            self._gv_dev = _dev_f64(self.gv, self.device)
+        return self._gv_dev
+
+    def out_of_eta_range(self, eta):
+        """True for peaks within eta_range of eta = 0 or +-180
+        (ImageD11/indexing.py:426-434); eta is first brought into
+        [-180, 180] by whole turns as mod_360(eta, 0) does (:172-183)."""
+        e = float(eta)
+        while e < -180:
+            e += 360
+        while e > 180:
+            e -= 360
+        er = abs(self.eta_range)
+        if -er < e < er:
+            return True
+        if e < -180.0 + er or e > 180.0 - er:
+            return True
+        return False
+
+    def readgvfile(self, filename, quiet=True):
+        """Read a .gve file: cell line, '#' header (wavelength, wedge), then
+        rows xr yr zr xc yc ds eta omega (ImageD11/indexing.py:1184-1248)."""
+        with open(filename) as f:
+            self.unitcell = unitcell_mod.cellfromstring(f.readline())
+            while True:
+                line = f.readline()
+                if line == "":
+                    raise ValueError("%s: no column header line" % filename)
+                if line[0] != "#":
+                    continue
+                if "wavelength" in line:
+                    self.wavelength = float(line.split()[-1])
+                    continue
+                if "wedge" in line:
+                    self.wedge = float(line.split()[-1])
+                    continue
+                if "ds h k l" in line:
+                    continue
+                if "omega" in line and "xc" in line:
+                    break
+            rows = []
+            for line in f:
+                v = [float(x) for x in line.split()]
+                if len(v) == 0:
+                    continue
+                if self.out_of_eta_range(v[6]):
+                    continue
+                rows.append(v[:8])
+        rows = np.array(rows, float).reshape(-1, 8)
+        self.xr, self.yr, self.zr = rows[:, 0], rows[:, 1], rows[:, 2]
+        self.xp, self.yp = rows[:, 3], rows[:, 4]
+        self.ds = rows[:, 5].copy()
+        self.eta = rows[:, 6].copy()
+        self.omega = rows[:, 7].copy()
+        self.gv = np.ascontiguousarray(rows[:, :3])
+        self.allgv = self.gv.copy()
+        self.ga = np.zeros(len(self.ds), np.int32) - 1
+        self.gvflat = np.ascontiguousarray(self.gv, float)
+        self.gv = self.gvflat.copy()
+        self._gv_dev = None
+
+    def assigntorings(self):
+        """ra[k] = nearest ring within ds_tol, -1 otherwise
+        (ImageD11/indexing.py:437-457), on the GPU."""
+        lib = _lib.load()
+        _lib.require_gpu()
+        limit = np.amax(self.ds)
+        self.unitcell.makerings(limit, tol=self.ds_tol)
+        dsr = np.asarray(self.unitcell.ringds, dtype=np.float64)
+        n = len(self.ds)
+        ds = _dev_f64(self.ds, self.device)
+        rd = _dev_f64(dsr, self.device)
+        ra = torch.empty(n, dtype=torch.int32, device=self.device)
+        _lib.check(lib.gs_ring_assign(dptr(ds), n, dptr(rd), len(dsr), float(self.ds_tol),
+                                      dptr(ra), stream_ptr()), "ring_assign")
+        self.ra = ra.cpu().numpy()
+        self.na = np.bincount(self.ra[self.ra >= 0], minlength=len(dsr)).astype(np.int32)
+
+    def find(self):
+        """Candidate peak pairs (ImageD11/indexing.py:633-728)."""
+        if self.ra is None:
+            self.assigntorings()
+        iall = np.arange(self.gv.shape[0])
+        i1 = iall[(self.ra == self.ring_1) & (self.ga == -1)]
+        i2 = iall[(self.ra == self.ring_2) & (self.ga == -1)]
+        self.hits = []
+        if len(i1) == 0 or len(i2) == 0:
+            logging.info("no peaks left for those rings")
+            return
+        cs = ring_cosines(self.unitcell, self.ring_1, self.ring_2)
+        self.cosangles = np.array(cs, "d")
+        self.hits = find_hits(self._gvd(), i1, i2, self.cosangles, self.cosine_tol)
+        logging.info("Number of trial orientations generated %d", len(self.hits))
+
+    def score(self, UBI, tol=None):
+        return cImageD11.score(UBI, self._gvd(), self.hkl_tol if tol is None else tol)
+
+    def getind(self, UBI, tol=None, drlv2tmp=None, labelstmp=None):
+        """Boolean mask of the peaks UBI indexes: score_and_assign from
+        drlv2 = 1, labels = 0, label 1 (ImageD11/indexing.py:1060-1088)."""
+        if tol is None:
+            tol = self.hkl_tol
+        g = self._gvd()
+        n = g.shape[0]
+        drlv2 = torch.ones(n, dtype=torch.float64, device=g.device)
+        labels = torch.zeros(n, dtype=torch.int32, device=g.device)
+        cImageD11.score_and_assign(UBI, g, tol, drlv2, labels, 1)
+        return (labels == 1).cpu().numpy()
+
+    def scorethem(self, fitb4=False):
+        """Decide which hits become grains (ImageD11/indexing.py:762-874)."""
+        if self.hits is None or len(self.hits) == 0:
+            logging.info("No hits to score")
+            return
+        if fitb4:
+            raise NotImplementedError("fitb4 (marked as not working in the reference)")
+        g = self._gvd()
+        tol = float(self.hkl_tol)
+        uc = self.unitcell
+        trial = orient_hits(uc, self.ring_1, self.ring_2, g, self.hits)
+        npk_all = batched.score_many(trial, g, tol).cpu().numpy()
+        trial = trial.cpu().numpy()
+        ng = 0
+        self.nuniq = 0
+        h = len(self.hits)
+        while h > 0 and ng < self.max_grains:
+            h -= 1
+            _, i, j = self.hits[h]
+            if self.ga[i] > -1 or self.ga[j] > -1 or i == j:
+                continue
+            npk = int(npk_all[h])
+            UBI = trial[h].copy()
+            if npk > self.minpks:
+                uc.orient(self.ring_1, self.gv[i, :], self.ring_2, self.gv[j, :], verbose=0,
+                          crange=abs(self.cosine_tol))
+                if len(uc.UBIlist) > 1:
+                    cand = torch.as_tensor(np.array(uc.UBIlist), device=g.device)
+                    npks = batched.score_many(cand, g, tol).cpu().numpy()
+                    choice = int(np.argmax(npks))
+                    if npks[choice] >= npk:
+                        UBI = uc.UBIlist[choice].copy()
+                        npk = int(npks[choice])
+                cImageD11.score_and_refine(UBI, g, tol)
+                ind = self.getind(UBI)
+                ga = self.ga[ind]
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    uniq = np.sum(np.where(ga == -1, 1, 0)) * 1.0 / ga.shape[0]
+                if uniq > self.uniqueness:
+                    self.ga[ind] = len(self.scores) + 1
+                    self.ubis.append(UBI)
+                    self.scores.append(npk)
+                    ng += 1
+                else:
+                    self.nuniq += 1
+        # the reference pops every visited hit off its list
+        self.hits = self.hits[:h]
+        logging.info("Number of orientations with more than %d peaks is %d",
+                     self.minpks, len(self.ubis))
+
+    def fight_over_peaks(self):
+        """Give each peak to the grain indexing it best, then refine each
+        grain on its peaks (ImageD11/indexing.py:876-905)."""
+        from . import drivers
+        g = self._gvd()
+        ubis = torch.as_tensor(np.array(self.ubis), device=g.device)
+        labels, drlv2, gas = drivers.fight_over_peaks(ubis, g, self.hkl_tol)
+        self.ga = labels.cpu().numpy()
+        self.gas = gas.cpu().numpy()
+        return self.ga

@@ -174,7 +174,8 @@ def simulate_peaks(ub, translations, pars, n_peaks, seed=2, spurious=0.10, gmax=
     """Forward-simulate a raw peak table of exactly n_peaks rows.
 
     Spots of every grain that land on the npx x npx detector are collected,
-    omega is binned to the centre of its omega_step frame, (sc, fc) get
+    omega is binned to the centre of its omega_step frame (kept exact when
+    omega_step <= 0), (sc, fc) get
     N(0, jitter_px) jitter, the grain spots are subsampled to
     (1 - spurious) * n_peaks and uniform spurious peaks fill the rest.
     Returns dict(sc, fc, omega, dty, grain) with grain = -1 for spurious
@@ -201,12 +202,15 @@ def simulate_peaks(ub, translations, pars, n_peaks, seed=2, spurious=0.10, gmax=
     rows, owner = rows[pick], owner[pick]
     rows[:, 0] += rng.normal(scale=jitter_px, size=n_real)
     rows[:, 1] += rng.normal(scale=jitter_px, size=n_real)
-    nbin = np.floor((rows[:, 2] - omega_range[0]) / omega_step)
-    rows[:, 2] = omega_range[0] + (nbin + 0.5) * omega_step
     n_sp = n_peaks - n_real
-    nfr = int(round((omega_range[1] - omega_range[0]) / omega_step))
-    sp = np.stack([rng.uniform(0, npx, n_sp), rng.uniform(0, npx, n_sp),
-                   omega_range[0] + (rng.integers(0, nfr, n_sp) + 0.5) * omega_step,
+    if omega_step > 0:
+        nbin = np.floor((rows[:, 2] - omega_range[0]) / omega_step)
+        rows[:, 2] = omega_range[0] + (nbin + 0.5) * omega_step
+        nfr = int(round((omega_range[1] - omega_range[0]) / omega_step))
+        sp_omega = omega_range[0] + (rng.integers(0, nfr, n_sp) + 0.5) * omega_step
+    else:  # exact omega, no frames
+        sp_omega = rng.uniform(omega_range[0], omega_range[1], n_sp)
+    sp = np.stack([rng.uniform(0, npx, n_sp), rng.uniform(0, npx, n_sp), sp_omega,
                    rng.uniform(-dty, dty, n_sp) if dty > 0 else np.zeros(n_sp)], 1)
     allrows = np.concatenate([rows, sp])
     allown = np.concatenate([owner, np.full(n_sp, -1, np.int32)])
@@ -384,3 +388,41 @@ def scan_workload(K=20000, N=8_000_000, seed=5, spurious=0.15, dty=500.0, device
                        device=device)
     return {"pars": pars, "ubi": np.ascontiguousarray(np.linalg.inv(ub)), "ub": ub,
             "trans": trans, "cells": cells, "peaks": peaks}
+
+
+def indexing_workload(K=200, cell=(4.05, 4.05, 4.05, 90.0, 90.0, 90.0), symmetry="F",
+                      dsmax=1.2, keep=0.5, noise=2e-4, spurious=0.1, seed=7):
+    """g-vectors for the ring-pair indexer: K randomly oriented grains of
+    one phase, each contributing a random `keep` fraction of its
+    reflections with d* < dsmax (g = U.B.hkl + N(0, noise) per component),
+    plus `spurious` uniform peaks in the d* shell.  The unit cell is built
+    from `cell`/`symmetry` with the reference's B convention.  Returns
+    dict(gv (N,3), ds (N,), grain (N,) with -1 for spurious, U (K,3,3),
+    cell, symmetry); rows are shuffled."""
+    from . import unitcell as uc_mod
+
+    class _NoExt:  # gethkls needs no compiled calls
+        pass
+
+    rng = np.random.default_rng(seed)
+    uc = uc_mod.unitcell(list(cell), symmetry, ext=_NoExt())
+    hkl = np.array([h for _, h in uc.gethkls(dsmax)], float)
+    Us = random_rotations(K, rng)
+    rows, owner = [], []
+    for k in range(K):
+        sel = rng.random(len(hkl)) < keep
+        g = (Us[k] @ uc.B @ hkl[sel].T).T
+        rows.append(g + rng.normal(scale=noise, size=g.shape))
+        owner.append(np.full(len(g), k, np.int32))
+    g = np.concatenate(rows)
+    own = np.concatenate(owner)
+    n_sp = int(round(len(g) * spurious / (1.0 - spurious)))
+    d = rng.normal(size=(n_sp, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r = rng.uniform(0.2, dsmax, n_sp)
+    g = np.concatenate([g, d * r[:, None]])
+    own = np.concatenate([own, np.full(n_sp, -1, np.int32)])
+    perm = rng.permutation(len(g))
+    g, own = np.ascontiguousarray(g[perm]), own[perm]
+    return {"gv": g, "ds": np.sqrt((g * g).sum(axis=1)), "grain": own, "U": Us,
+            "cell": tuple(cell), "symmetry": symmetry}

@@ -233,6 +233,57 @@ int gs_gv_lists(const double *xlylzl, const double *so, const double *co, int ng
                 const int *idx, const int *owner, long long n_idx, double *out,
                 gs_stream_t stream);
 
+/* ---- indexer: ring assignment, pair search, trial orientations ---- */
+
+/* closest (src/closest.c:188-207; f2py src/_cImageD11.pyf:88-103): the index in x of
+ * the smallest |x[i] - v[j]| (first in x-major order) and that value;
+ * 0 and 99 when none is below 99.  Host pointers. */
+void closest(double x[], double v[], int *ibest, double *best, int nx, int nv);
+
+/* quickorient (src/cdiffraction.c:293-328; src/_cImageD11.pyf:603-613): UBI rows 0, 1 hold g1, g2 on
+ * entry and the orientation BT . [t1; t2; t3] on return.  Host pointers. */
+void quickorient(double UBI[9], double BT[9]);
+
+/* ds[k] = |gv[k]| as np.sqrt((gv*gv).sum(axis=1))
+ * (ImageD11/indexing.py:345). */
+int gs_gv_ds(const double *gv, int ng, double *ds, gs_stream_t stream);
+
+/* indexer.assigntorings (ImageD11/indexing.py:447-457): ra[k] = the ring
+ * j with the smallest |ds[k] - ringds[j]| < tol (first on ties), else -1. */
+int gs_ring_assign(const double *ds, int n, const double *ringds, int nr, double tol, int *ra,
+                   gs_stream_t stream);
+
+/* out[r] = gv[idx[r]] / |gv[idx[r]]| (indexer.find, ImageD11/indexing.py:
+ * 696-706); NaN rows for out-of-range indices. */
+int gs_unit_rows(const double *gv, int ng, const int *idx, int n, double *out,
+                 gs_stream_t stream);
+
+/* indexer.find, cosine_tol > 0 (ImageD11/indexing.py:710-716): for each
+ * row i of n1 (N1 x 3 unit vectors of ring-1 peaks), closest(n2 . n1[i],
+ * cs) over the N2 ring-2 unit vectors -> ibest[i] (row of n2), best[i].
+ * nv <= 4096. */
+int gs_find_hits(const double *n1, int N1, const double *n2, int N2, const double *cs, int nv,
+                 int *ibest, double *best, gs_stream_t stream);
+
+/* indexer.find, cosine_tol < 0 (ImageD11/indexing.py:717-723): counts the
+ * (i, m, j) with |cs[m] - n2[j] . n1[i]| < mtol into *counter (device,
+ * zeroed here); when keys is not NULL also writes key = (i*nv + m)*N2 + j
+ * for the first cap of them, in no particular order (sort to get the
+ * reference's i, m, j order).  nv <= 4096. */
+int gs_find_all(const double *n1, int N1, const double *n2, int N2, const double *cs, int nv,
+                double mtol, unsigned long long *counter, long long *keys, long long cap,
+                gs_stream_t stream);
+
+/* unitcell.orient with crange <= 0 + quickorient (ImageD11/unitcell.py:
+ * 617-653) for nh peak pairs (hi[h], hj[h]): ubis[9h..] = the trial UBI
+ * from the sorted cosine table c2ab (nc entries) and its BT matrices
+ * bt[9b..]; choice[h] = b (may be NULL).  Feed ubis to gs_score_many to
+ * score every trial in one launch (indexer.scorethem,
+ * ImageD11/indexing.py:762-800). */
+int gs_orient_hits(const double *gv, int ng, const int *hi, const int *hj, int nh,
+                   const double *c2ab, int nc, const double *bt, double *ubis, int *choice,
+                   gs_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,7 +28,15 @@
  *   - central finite differences for the joint objective used by the
  *     adjoint (gs_oracle_joint_loss has no reference counterpart; it is
  *     pinned by brute-force recomputation through the other oracle
- *     functions and by the normal equations of score_and_refine).
+ *     functions and by the normal equations of score_and_refine);
+ *   - indexer functions (closest, quickorient, find, orient): closest and
+ *     quickorient bit-for-bit against the reference C; the two 3-term dot
+ *     orders (explicit fma calls, the only FMAs in this file) bit-for-bit
+ *     against numpy's np.dot; find against the reference's own loop run
+ *     with numpy and the reference C closest; orient against the numpy
+ *     unitcell.orient; the whole search against the reference's own
+ *     test (test/ken_simul/testken.py: 20 grains from simAl.gve) and the
+ *     test.idx peak counts.
  *
  * Row-major flat arrays: ubi[9] is ubi[i][j] at 3*i+j, gv[3*k+c].
  */
@@ -490,6 +498,140 @@ double orc_joint_loss(const double *sc, const double *fc, const double *so,
         loss += part;
     }
     return loss;
+}
+
+/* ---- indexer ----------------------------------------------------------- */
+
+/* src/closest.c:188-207: scan x-major, strict '<', start at 99 / index 0. */
+void orc_closest(const double *x, const double *v, int *ibest, double *best, int nx, int nv) {
+    double b = 99.0;
+    int ib = 0;
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < nv; j++)
+            if (fabs(x[i] - v[j]) < b) {
+                b = fabs(x[i] - v[j]);
+                ib = i;
+            }
+    *ibest = ib;
+    *best = b;
+}
+
+/* src/cdiffraction.c:293-328 (Busing & Levy from two observed g-vectors). */
+void orc_quickorient(double *u, const double *bt) {
+    double m[9];
+    m[6] = u[1] * u[5] - u[2] * u[4];
+    m[7] = u[2] * u[3] - u[0] * u[5];
+    m[8] = u[0] * u[4] - u[1] * u[3];
+    double s = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (int c = 0; c < 3; c++) m[c] = u[c] / s;
+    s = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
+    for (int c = 6; c < 9; c++) m[c] /= s;
+    m[3] = m[1] * m[8] - m[2] * m[7];
+    m[4] = m[2] * m[6] - m[0] * m[8];
+    m[5] = m[0] * m[7] - m[1] * m[6];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++)
+            u[3 * r + c] = bt[3 * r + 0] * m[c] + bt[3 * r + 1] * m[3 + c] + bt[3 * r + 2] * m[6 + c];
+}
+
+/* unit rows of gv[idx] as indexer.find builds n1, n2
+ * (ImageD11/indexing.py:696-706): |g| = sqrt((x*x + y*y) + z*z), then
+ * each component divided by it. */
+void orc_unit_rows(const double *gv, const int *idx, int n, double *out) {
+    for (int r = 0; r < n; r++) {
+        const double *g = gv + 3 * (long)idx[r];
+        double m = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        for (int c = 0; c < 3; c++) out[3 * r + c] = g[c] / m;
+    }
+}
+
+/* The indexer's 3-term dot products as numpy 2.2 / OpenBLAS 0.3.29
+ * (x86-64 Haswell/SkylakeX kernels) evaluates them: dgemv (np.dot of a
+ * matrix row and a vector) fma(x2,y2, fma(x0,y0, x1*y1)); ddot (np.dot of
+ * two vectors) fma(x2,y2, fma(x1,y1, x0*y0)).  Pinned against numpy in
+ * tests/test_indexer_cpu.py. */
+static double orc_dot3_gemv(const double *x, const double *y) {
+    return fma(x[2], y[2], fma(x[0], y[0], x[1] * y[1]));
+}
+static double orc_dot3_ddot(const double *x, const double *y) {
+    return fma(x[2], y[2], fma(x[1], y[1], x[0] * y[0]));
+}
+
+/* out[r] = x[r] . y[r] in the dgemv (mode 0) or ddot (mode 1) order, for
+ * pinning the two orders against numpy. */
+void orc_dot3_rows(const double *x, const double *y, int n, int mode, double *out) {
+    for (int r = 0; r < n; r++)
+        out[r] = mode == 0 ? orc_dot3_gemv(x + 3 * (long)r, y + 3 * (long)r)
+                           : orc_dot3_ddot(x + 3 * (long)r, y + 3 * (long)r);
+}
+
+/* indexer.find with cosine_tol > 0 (ImageD11/indexing.py:710-716): for
+ * every row i of n1, costheta[j] = np.dot(n2, n1[i])[j] and
+ * closest(costheta, cs). */
+void orc_find_hits(const double *n1, int N1, const double *n2, int N2, const double *cs, int nv,
+                   int *ibest, double *best) {
+    double *ct = (double *)malloc(sizeof(double) * (N2 > 0 ? N2 : 1));
+    for (int i = 0; i < N1; i++) {
+        const double *a = n1 + 3 * (long)i;
+        for (int j = 0; j < N2; j++) {
+            const double *b = n2 + 3 * (long)j;
+            ct[j] = orc_dot3_gemv(b, a);
+        }
+        orc_closest(ct, cs, ibest + i, best + i, N2, nv);
+    }
+    free(ct);
+}
+
+/* ImageD11/indexing.py:717-723 (cosine_tol < 0): keys (i*nv + m)*N2 + j of
+ * every |cs[m] - costheta_ij| < mtol in i, m, j order; returns the count
+ * (keys written while < cap). */
+long long orc_find_all(const double *n1, int N1, const double *n2, int N2, const double *cs,
+                       int nv, double mtol, long long *keys, long long cap) {
+    long long n = 0;
+    double *ct = (double *)malloc(sizeof(double) * (N2 > 0 ? N2 : 1));
+    for (int i = 0; i < N1; i++) {
+        const double *a = n1 + 3 * (long)i;
+        for (int j = 0; j < N2; j++) {
+            const double *b = n2 + 3 * (long)j;
+            ct[j] = orc_dot3_gemv(b, a);
+        }
+        for (int m = 0; m < nv; m++)
+            for (int j = 0; j < N2; j++)
+                if (fabs(cs[m] - ct[j]) < mtol) {
+                    if (n < cap) keys[n] = ((long long)i * nv + m) * N2 + j;
+                    n++;
+                }
+    }
+    free(ct);
+    return n;
+}
+
+/* unitcell.orient, crange <= 0 (ImageD11/unitcell.py:617-630, 649-653),
+ * for the peak pairs (hi[h], hj[h]). */
+void orc_orient_hits(const double *gv, const int *hi, const int *hj, int nh, const double *c2ab,
+                     int nc, const double *bt, double *ubis, int *choice) {
+    for (int h = 0; h < nh; h++) {
+        const double *g1 = gv + 3 * (long)hi[h], *g2 = gv + 3 * (long)hj[h];
+        double dot = orc_dot3_ddot(g1, g2);
+        double s1 = g1[0] * g1[0] + g1[1] * g1[1] + g1[2] * g1[2];
+        double s2 = g2[0] * g2[0] + g2[1] * g2[1] + g2[2] * g2[2];
+        double ct = dot / sqrt(s1 * s2);
+        int i = 0; /* searchsorted(c2ab, ct, side='left'); NaN sorts last */
+        if (isnan(ct))
+            i = nc;
+        else
+            while (i < nc && c2ab[i] < ct) i++;
+        int b = i;
+        if (i > 0 && (i == nc || fabs(ct - c2ab[i - 1]) < fabs(ct - c2ab[i]))) b = i - 1;
+        double *u = ubis + 9 * (long)h;
+        for (int c = 0; c < 3; c++) {
+            u[c] = g1[c];
+            u[3 + c] = g2[c];
+            u[6 + c] = 0.0;
+        }
+        orc_quickorient(u, bt + 9 * (long)b);
+        if (choice) choice[h] = b;
+    }
 }
 
 int orc_openmp_threads(void) {

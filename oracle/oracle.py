@@ -83,6 +83,12 @@ class _Base:
         self._xl = getattr(L, p + "compute_xlylzl")
         self._xl.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
         self._xl.restype = None
+        self._closest = getattr(L, p + "closest")
+        self._closest.argtypes = [_dp, _dp, _ip, _dp, ctypes.c_int, ctypes.c_int]
+        self._closest.restype = None
+        self._qo = getattr(L, p + "quickorient")
+        self._qo.argtypes = [_dp, _dp]
+        self._qo.restype = None
 
     # f2py-style wrappers -------------------------------------------------
     def verify_rounding(self, n):
@@ -136,6 +142,20 @@ class _Base:
         assert xlylzl.dtype == np.float64 and xlylzl.flags.c_contiguous
         self._xl(_d(s), _d(f), _d(p), _d(r), _d(dist), _d(xlylzl), s.shape[0])
 
+    def closest(self, x, v):
+        x = _f64(x)
+        v = _f64(v)
+        ib = ctypes.c_int(0)
+        b = ctypes.c_double(0)
+        self._closest(_d(x), _d(v), ctypes.byref(ib), ctypes.byref(b), x.shape[0], v.shape[0])
+        return ib.value, b.value
+
+    def quickorient(self, ubi, bt):
+        """ubi (3,3) C-contiguous float64, overwritten."""
+        assert ubi.dtype == np.float64 and ubi.flags.c_contiguous and ubi.shape == (3, 3)
+        bt = _f64(bt, (3, 3))
+        self._qo(_d(ubi), _d(bt))
+
 
 class Reference(_Base):
     """The reference C built from its own sources into ``_ref/libref.so``."""
@@ -178,6 +198,62 @@ class Oracle(_Base):
                                      _dp, ctypes.c_int, _ip, _llp, _ip]
         L.orc_joint_loss.restype = ctypes.c_double
         L.orc_openmp_threads.restype = ctypes.c_int
+        L.orc_unit_rows.argtypes = [_dp, _ip, ctypes.c_int, _dp]
+        L.orc_dot3_rows.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_find_hits.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip,
+                                    _dp]
+        L.orc_find_all.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
+                                   ctypes.c_double, _llp, ctypes.c_longlong]
+        L.orc_find_all.restype = ctypes.c_longlong
+        L.orc_orient_hits.argtypes = [_dp, _ip, _ip, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
+                                      _ip]
+
+    def dot3_rows(self, x, y, mode):
+        x = _f64(x, (-1, 3))
+        y = _f64(y, (-1, 3))
+        out = np.empty(len(x))
+        self.lib.orc_dot3_rows(_d(x), _d(y), len(x), int(mode), _d(out))
+        return out
+
+    def unit_rows(self, gv, idx):
+        gv = _f64(gv, (-1, 3))
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.empty((len(idx), 3))
+        self.lib.orc_unit_rows(_d(gv), _i(idx), len(idx), _d(out))
+        return out
+
+    def find_hits(self, n1, n2, cs):
+        n1 = _f64(n1, (-1, 3))
+        n2 = _f64(n2, (-1, 3))
+        cs = _f64(cs)
+        ib = np.zeros(len(n1), np.int32)
+        best = np.zeros(len(n1))
+        self.lib.orc_find_hits(_d(n1), len(n1), _d(n2), len(n2), _d(cs), len(cs), _i(ib),
+                               _d(best))
+        return ib, best
+
+    def find_all(self, n1, n2, cs, mtol):
+        n1 = _f64(n1, (-1, 3))
+        n2 = _f64(n2, (-1, 3))
+        cs = _f64(cs)
+        n = self.lib.orc_find_all(_d(n1), len(n1), _d(n2), len(n2), _d(cs), len(cs),
+                                  float(mtol), None, 0)
+        keys = np.zeros(max(n, 1), np.int64)
+        self.lib.orc_find_all(_d(n1), len(n1), _d(n2), len(n2), _d(cs), len(cs), float(mtol),
+                              keys.ctypes.data_as(_llp), n)
+        return keys[:n]
+
+    def orient_hits(self, gv, hi, hj, c2ab, bt):
+        gv = _f64(gv, (-1, 3))
+        hi = np.ascontiguousarray(hi, dtype=np.int32)
+        hj = np.ascontiguousarray(hj, dtype=np.int32)
+        c2ab = _f64(c2ab)
+        bt = _f64(bt, (-1, 3, 3))
+        ubis = np.zeros((len(hi), 3, 3))
+        choice = np.zeros(len(hi), np.int32)
+        self.lib.orc_orient_hits(_d(gv), _i(hi), _i(hj), len(hi), _d(c2ab), len(c2ab), _d(bt),
+                                 _d(ubis), _i(choice))
+        return ubis, choice
 
     def inverse3x3(self, a):
         a = _f64(a, (3, 3)).copy()

@@ -205,7 +205,16 @@ def ken_simul(ref):
                 ra_n=np.array([a for a, _ in ra], np.int32), ra_mean=np.array([b for _, b in ra]))
 
 
+def copy_data_files():
+    """Data files of the reference's own tests, kept verbatim: simAl.gve
+    (test/ken_simul, the input of testken.py's 20-grain indexing test)."""
+    import shutil
+    shutil.copyfile(os.path.join(REFROOT, "test", "ken_simul", "simAl.gve"),
+                    os.path.join(HERE, "simAl.gve"))
+
+
 def main():
+    copy_data_files()
     ref = ref_lib()
     np.savez_compressed(os.path.join(HERE, "scoring_cases.npz"), **scoring_cases(ref))
     np.savez_compressed(os.path.join(HERE, "geometry_cases.npz"), **geometry_cases(ref))

@@ -82,9 +82,19 @@ def test_geometry_gradient_finite_differences(setup):
         assert abs(g - fd) <= 1e-5 * max(abs(fd), abs(g)) + 1e-9 * abs(float(loss)), (k, g, fd)
 
 
+def _grain_loss_at(s, g, ubis=None, trans=None):
+    """Loss of grain g alone: a UBI or translation entry of grain g moves
+    only that term, so its finite difference is not buried in the rounding
+    noise of the other grains' terms."""
+    s["prob"].evaluate(s["geo"], s["ubis"] if ubis is None else ubis,
+                       s["trans"] if trans is None else trans)
+    return float(s["prob"].loss_g[g])
+
+
 def test_ubi_and_translation_gradient_finite_differences(setup):
     s = setup
     _, gu, gt, _, _ = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"])
+    gu, gt = gu.clone(), gt.clone()
     rng = np.random.default_rng(0)
     for _ in range(6):
         g, i, j = int(rng.integers(0, 120)), int(rng.integers(0, 3)), int(rng.integers(0, 3))
@@ -92,15 +102,18 @@ def test_ubi_and_translation_gradient_finite_differences(setup):
         up, dn = s["ubis"].clone(), s["ubis"].clone()
         up[g, i, j] += h
         dn[g, i, j] -= h
-        fd = (_loss_at(s, ubis=up) - _loss_at(s, ubis=dn)) / (2 * h)
+        fd = (_grain_loss_at(s, g, ubis=up) - _grain_loss_at(s, g, ubis=dn)) / (2 * h)
         assert abs(float(gu[g, i, j]) - fd) <= 1e-6 * abs(fd) + 1e-8
+        # translations move g-vectors by ~|t|/distance: a larger step keeps
+        # the difference well above rounding (the loss is smooth in t)
         c = int(rng.integers(0, 3))
-        h = 1e-6 * max(abs(float(s["trans"][g, c])), 1.0)
+        h = 1e-2
         up, dn = s["trans"].clone(), s["trans"].clone()
         up[g, c] += h
         dn[g, c] -= h
-        fd = (_loss_at(s, trans=up) - _loss_at(s, trans=dn)) / (2 * h)
-        assert abs(float(gt[g, c]) - fd) <= 1e-5 * abs(fd) + 1e-9
+        lg = _grain_loss_at(s, g)
+        fd = (_grain_loss_at(s, g, trans=up) - _grain_loss_at(s, g, trans=dn)) / (2 * h)
+        assert abs(float(gt[g, c]) - fd) <= 1e-5 * abs(fd) + 1e-13 * abs(lg) / h, (g, c)
 
 
 def test_gspace_gradient_vanishes_at_score_and_refine_solution(setup, orc):
@@ -150,11 +163,11 @@ def test_adjoint_deterministic(setup):
         assert torch.equal(x, y)
 
 
-def test_joint_refinement_recovers_geometry(setup):
+def test_joint_refinement_noisy_data(setup):
     """50 steps (Gauss-Newton UBIs + Adam geometry) from a perturbed
-    detector: the fit ends as good as the fit at the true geometry and the
-    beam centre moves back towards the truth.  (Distance trades off against
-    the cell scale of the UBIs, so it is not asserted.)"""
+    detector on jittered, omega-binned spots: the fit ends at least as good
+    as the fit at the true geometry.  (With this noise the true geometry is
+    not the minimum, so the parameters themselves are not asserted here.)"""
     s = setup
     truth = {k: float(v) for k, v in s["params"].items()}
     names = ("distance", "y_center", "z_center", "tilt_z")
@@ -171,5 +184,47 @@ def test_joint_refinement_recovers_geometry(setup):
                                              dict(s["pars"]), steps=50, lr=0.05, scales=scales)
     assert hist[-1] < hist[0]
     assert hist[-1] <= 1.05 * best_loss
+
+
+@pytest.fixture(scope="module")
+def exact_setup(dev):
+    """Noise-free spots (no pixel jitter, exact omega, no spurious peaks):
+    the true geometry is the exact minimum of the joint objective."""
+    wl = simulate.scoring_workload(K=60, N=30000, seed=43, translations=True)
+    pars = dict(wl["pars"], wedge=0.7, chi=-0.4)
+    pk = simulate.simulate_peaks(wl["ub"], wl["trans"], pars, 12000, seed=44, spurious=0.0,
+                                 omega_step=0.0, jitter_px=0.0)
+    # each grain's list = exactly its own spots
+    idx = np.argsort(pk["grain"], kind="stable").astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(np.bincount(pk["grain"], minlength=60))])
+    assert np.all(np.diff(off) >= 20)
+    prob = adjoint.JointProblem(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev),
+                                pars["omegasign"], T(idx, dev, torch.int32),
+                                T(off, dev, torch.int64))
+    params = {k: torch.tensor(float(pars[k]), dtype=torch.float64, device=dev)
+              for k in adjoint.FREE}
+    ubis, trans = T(wl["ubi"], dev), T(wl["trans"], dev)
+    prob.assign_hkl(adjoint.pack_geometry(params, pars), ubis, trans)
+    return dict(pars=pars, prob=prob, params=params, ubis=ubis, trans=trans)
+
+
+def test_joint_refinement_recovers_exact_geometry(exact_setup):
+    """On noise-free spots the refinement from a perturbed detector drives
+    the loss towards zero and the beam centre back towards the truth.
+    (Distance trades off against the cell scale of the UBIs, so it is not
+    asserted.)"""
+    s = exact_setup
+    truth = {k: float(v) for k, v in s["params"].items()}
+    geo = adjoint.pack_geometry(s["params"], s["pars"])
+    assert float(s["prob"].evaluate(geo, s["ubis"], s["trans"])[0]) < 1e-12
+    names = ("distance", "y_center", "z_center", "tilt_z")
+    dev = s["ubis"].device
+    start = dict(truth, distance=truth["distance"] + 150.0, y_center=truth["y_center"] + 0.6,
+                 z_center=truth["z_center"] - 0.4, tilt_z=truth["tilt_z"] + 2e-4)
+    params = {k: torch.tensor(start[k], dtype=torch.float64, device=dev) for k in names}
+    scales = {"distance": 100.0, "y_center": 1.0, "z_center": 1.0, "tilt_z": 1e-3}
+    ubis, final, hist = adjoint.refine_joint(s["prob"], s["ubis"], s["trans"], params,
+                                             dict(s["pars"]), steps=100, lr=0.05, scales=scales)
+    assert hist[-1] < 0.05 * hist[0]
     for k in ("y_center", "z_center"):
-        assert abs(float(final[k]) - truth[k]) < 0.5 * abs(start[k] - truth[k])
+        assert abs(float(final[k]) - truth[k]) < 0.5 * abs(start[k] - truth[k]), k

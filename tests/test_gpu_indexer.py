@@ -1,0 +1,153 @@
+"""The indexer on the GPU against its CPU checker: closest, quickorient,
+ring assignment, the pair search of indexer.find (both tolerance modes),
+batched trial orientations, and the whole assigntorings + find +
+scorethem search (hits, grains, scores, peak assignments) on simAl.gve
+(the reference's test/ken_simul/testken.py: 20 grains) and on a larger
+synthetic set."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from imaged11_amd import _lib, cImageD11, indexing, simulate
+from imaged11_amd import unitcell as U
+from oracle import indexer_oracle as IO
+from test_indexer_cpu import read_gve
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(__file__)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.int64)
+
+
+def test_closest_dropin(orc):
+    rng = np.random.default_rng(1)
+    for t in range(200):
+        x = rng.uniform(-1, 1, int(rng.integers(0, 3000)))
+        v = rng.uniform(-1, 1, int(rng.integers(0, 12)))
+        if t % 3 == 0 and len(x) > 600:
+            x[500] = x[20]  # tie across threads: the first index wins
+            x[7] = np.nan
+        if t % 4 == 0:
+            x, v = np.round(x, 2), np.round(v, 2)
+        assert cImageD11.closest(x, v) == orc.closest(x, v), t
+    assert cImageD11.closest(np.array([0.3, 0.2]), np.array([])) == (0, 99.0)
+    assert cImageD11.closest(np.array([]), np.array([0.2])) == (0, 99.0)
+
+
+def test_quickorient_dropin(orc):
+    rng = np.random.default_rng(2)
+    for _ in range(100):
+        u = np.zeros((3, 3))
+        u[:2] = rng.normal(size=(2, 3))
+        bt = rng.normal(size=(3, 3))
+        u2 = u.copy()
+        cImageD11.quickorient(u, bt)
+        orc.quickorient(u2, bt)
+        assert np.array_equal(bits(u), bits(u2))
+    with pytest.raises(ValueError):
+        cImageD11.quickorient(np.zeros((3, 3), np.float32), np.eye(3))
+
+
+def test_ring_assign_and_ds(dev, orc):
+    rows = read_gve()[2]
+    gv = np.ascontiguousarray(rows[:, :3])
+    ind = indexing.indexer(U.cellfromstring(read_gve()[0]), gv, ds_tol=0.005)
+    ind.assigntorings()
+    want = IO.assigntorings(ind.ds, ind.unitcell.ringds, 0.005)
+    assert np.array_equal(ind.ra, want)
+    lib = _lib.load()
+    g = torch.as_tensor(gv, device=dev)
+    ds = torch.empty(len(gv), dtype=torch.float64, device=dev)
+    _lib.check(lib.gs_gv_ds(g.data_ptr(), len(gv), ds.data_ptr(), _lib.stream_ptr()), "gv_ds")
+    torch.cuda.synchronize()
+    assert np.array_equal(bits(ds.cpu().numpy()), bits(np.sqrt((gv * gv).sum(axis=1))))
+
+
+def _unit(n, rng, quant=None):
+    v = rng.normal(size=(n, 3))
+    if quant:
+        v = np.round(v * quant) / quant  # many exactly equal cosines
+    return v
+
+
+@pytest.mark.parametrize("tol", [0.002, 0.05, -0.01])
+def test_find_hits_matches_oracle(dev, orc, tol):
+    rng = np.random.default_rng(5)
+    for quant in (None, 4):
+        gv = _unit(6000, rng, quant)
+        i1 = np.sort(rng.choice(6000, 900, replace=False))
+        i2 = np.sort(rng.choice(6000, 2500, replace=False))
+        cs = np.sort(rng.uniform(-0.97, 0.97, 11))[::-1].copy()
+        got = indexing.find_hits(torch.as_tensor(gv, device=dev), i1, i2, cs, tol)
+        n1, n2 = orc.unit_rows(gv, i1), orc.unit_rows(gv, i2)
+        if tol > 0:
+            ib, best = orc.find_hits(n1, n2, cs)
+            want = [[float(best[i]), int(i1[i]), int(i2[ib[i]])] for i in range(len(i1))
+                    if best[i] < tol]
+        else:
+            keys = orc.find_all(n1, n2, cs, -tol)
+            want = [[0.0, int(i1[k // (len(cs) * len(i2))]), int(i2[k % len(i2)])] for k in keys]
+        assert got == want
+        assert len(got) > 10
+
+
+def test_orient_hits_matches_oracle(dev, orc):
+    cell, _, rows = read_gve()
+    gv = np.ascontiguousarray(rows[:, :3])
+    uc = U.cellfromstring(cell, ext=orc)
+    uc.makerings(rows[:, 5].max(), tol=0.005)
+    rng = np.random.default_rng(9)
+    for r1, r2 in ((0, 1), (1, 3), (2, 2)):
+        hits = [[0.0, int(a), int(b)] for a, b in rng.integers(0, len(gv), size=(3000, 2))]
+        got = indexing.orient_hits(uc, r1, r2, torch.as_tensor(gv, device=dev), hits)
+        _, c2ab, matrs = uc.getanglehkls(r1, r2)
+        want, _ = orc.orient_hits(gv, [h[1] for h in hits], [h[2] for h in hits], c2ab, matrs)
+        assert np.array_equal(bits(got.cpu().numpy()), bits(want))
+
+
+def _compare(ind, res):
+    assert ind.hits == res["left"]
+    assert len(ind.ubis) == len(res["ubis"])
+    assert ind.scores == res["scores"]
+    for a, b in zip(ind.ubis, res["ubis"]):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-10)
+    assert np.array_equal(ind.ga, res["ga"])
+
+
+def test_indexer_simAl_20_grains(orc):
+    """test/ken_simul/testken.py with the GPU indexer: minpks 80, rings 0
+    and 1 -> 20 grains; every hit, score and assignment as the checker's."""
+    gve = os.path.join(HERE, "golden", "simAl.gve")
+    ind = indexing.indexer()
+    ind.readgvfile(gve)
+    ind.set_parameters({"minpks": "80", "ring_1": "0", "ring_2": "1"})
+    ind.assigntorings()
+    ind.find()
+    cell, _, rows = read_gve()
+    uc = U.cellfromstring(cell, ext=orc)
+    res = IO.index(orc, uc, np.ascontiguousarray(rows[:, :3]), rows[:, 5].copy(), 0, 1,
+                   minpks=80)
+    assert ind.hits == res["hits"]
+    ind.scorethem()
+    ind.assigntorings()
+    assert len(ind.ubis) == 20
+    _compare(ind, res)
+
+
+def test_indexer_synthetic_matches_oracle(orc):
+    w = simulate.indexing_workload(K=300, seed=4)
+    uc_gpu = U.unitcell(list(w["cell"]), w["symmetry"])
+    ind = indexing.indexer(uc_gpu, w["gv"], ring_1=0, ring_2=1, minpks=20, hkl_tol=0.02,
+                           max_grains=1000)
+    ind.assigntorings()
+    ind.find()
+    ind.scorethem()
+    uc = U.unitcell(list(w["cell"]), w["symmetry"], ext=orc)
+    res = IO.index(orc, uc, w["gv"], w["ds"], 0, 1, minpks=20, hkl_tol=0.02, max_grains=1000)
+    assert len(res["ubis"]) > 250
+    _compare(ind, res)

@@ -1,0 +1,182 @@
+"""The indexer's CPU checker and unit-cell tables, pinned without a GPU.
+
+* unit cell: the reflections and d* of simAl.gve's header (written by the
+  reference), Busing-Levy closed forms, ring multiplicities of fcc;
+* closest / quickorient: the oracle against the reference C;
+* the two numpy dot orders the kernels reproduce, against numpy itself;
+* find: the oracle against the reference's own loop (numpy + the
+  reference C closest), ImageD11/indexing.py:708-723;
+* orient: the oracle's batched form against unitcell.orient (numpy);
+* the whole search on simAl.gve: 20 grains, the assertion of the
+  reference's test/ken_simul/testken.py, and test.idx's peak counts.
+"""
+import os
+from collections import Counter
+
+import numpy as np
+import pytest
+
+from imaged11_amd import unitcell as U
+from oracle import indexer_oracle as IO
+
+HERE = os.path.dirname(__file__)
+GVE = os.path.join(HERE, "golden", "simAl.gve")
+
+
+def read_gve(path=GVE):
+    """(cell line, header [(ds, hkl)], rows xr yr zr xc yc ds eta omega)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    head, rows, started = [], [], False
+    for ln in lines[1:]:
+        if ln.startswith("#"):
+            if "xc" in ln and "omega" in ln:
+                started = True
+            continue
+        v = ln.split()
+        if not v:
+            continue
+        if started:
+            rows.append([float(x) for x in v])
+        else:
+            head.append((float(v[0]), tuple(int(x) for x in v[1:4])))
+    return lines[0], head, np.array(rows)
+
+
+@pytest.fixture(scope="module")
+def gve():
+    return read_gve()
+
+
+def test_gethkls_matches_gve_header(gve, orc):
+    cell, head, _ = gve
+    uc = U.cellfromstring(cell, ext=orc)
+    assert uc.symmetry == "F"
+    dsmax = max(d for d, _ in head)
+    pk = uc.gethkls(dsmax + 1e-6)
+    assert sorted(h for _, h in pk) == sorted(h for _, h in head)
+    mine = {h: d for d, h in pk}
+    for d, h in head:
+        assert abs(mine[h] - d) < 1e-6
+    assert [p[0] for p in pk] == sorted(p[0] for p in pk)
+
+
+def test_unitcell_closed_forms(orc):
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        cell = list(rng.uniform(3, 9, 3)) + list(rng.uniform(70, 110, 3))
+        uc = U.unitcell(cell, "P", ext=orc)
+        np.testing.assert_allclose(uc.B.T @ uc.B, uc.gi, rtol=1e-12, atol=1e-15)
+        assert uc.B[1, 0] == 0 and uc.B[2, 0] == 0 and uc.B[2, 1] == 0
+        for h in rng.integers(-4, 5, size=(10, 3)):
+            assert abs(uc.ds(h) - np.linalg.norm(uc.B @ h)) < 1e-12
+    # fcc ring multiplicities {111} 8, {200} 6, {220} 12, {311} 24, {222} 8
+    uc = U.unitcell([4.05] * 3 + [90.0] * 3, "F", ext=orc)
+    uc.makerings(1.0)
+    assert [len(uc.ringhkls[d]) for d in uc.ringds[:5]] == [8, 6, 12, 24, 8]
+    # an orientation from the ideal g-vectors of a ring pair indexes them
+    hab, c2ab, matrs = uc.getanglehkls(0, 1)
+    rot = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    for (h1, h2), bt in zip(hab, matrs):
+        g = np.zeros((3, 3))
+        g[0], g[1] = rot @ uc.B @ np.array(h1, float), rot @ uc.B @ np.array(h2, float)
+        orc.quickorient(g, bt)
+        np.testing.assert_allclose(g @ (rot @ uc.B @ np.array(h1, float)), h1, atol=1e-12)
+        np.testing.assert_allclose(g @ (rot @ uc.B @ np.array(h2, float)), h2, atol=1e-12)
+
+
+def test_closest_quickorient_match_reference(orc, ref):
+    rng = np.random.default_rng(7)
+    for t in range(3000):
+        x = rng.uniform(-1, 1, int(rng.integers(0, 40)))
+        v = rng.uniform(-1, 1, int(rng.integers(0, 6)))
+        if t % 5 == 0 and len(x) > 4:
+            x[3] = x[1]  # tie: the first index wins
+            x[4] = np.nan
+        if t % 11 == 0:
+            x = np.round(x, 1)
+            v = np.round(v, 1)
+        assert orc.closest(x, v) == ref.closest(x, v)
+        u = np.zeros((3, 3))
+        u[:2] = rng.normal(size=(2, 3))
+        bt = rng.normal(size=(3, 3))
+        u2 = u.copy()
+        orc.quickorient(u, bt)
+        ref.quickorient(u2, bt)
+        assert np.array_equal(u.view(np.int64), u2.view(np.int64))
+    assert orc.closest(np.array([0.5]), np.array([])) == (0, 99.0)
+
+
+def test_dot_orders_match_numpy(orc):
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=(20000, 3)) * rng.uniform(1e-3, 1e3, (20000, 1))
+    y = rng.normal(size=(20000, 3))
+    gemv = orc.dot3_rows(x, np.broadcast_to(y[5], x.shape), 0)
+    assert np.array_equal(gemv, np.dot(x, y[5]))
+    ddot = orc.dot3_rows(x, y, 1)
+    assert np.array_equal(ddot, np.array([np.dot(a, b) for a, b in zip(x, y)]))
+
+
+def _reference_find(ref, gv, i1, i2, cs, tol):
+    """ImageD11/indexing.py:693-723 as written: numpy and the compiled
+    reference closest."""
+    mp = np.sqrt(np.sum(gv * gv, 1))
+    n1 = gv[i1] / mp[i1][:, None]
+    n2 = gv[i2] / mp[i2][:, None]
+    hits = []
+    for i in range(len(i1)):
+        costheta = np.dot(n2, n1[i])
+        if tol > 0:
+            best, diff = ref.closest(costheta, cs)
+            if diff < tol:
+                hits.append([diff, int(i1[i]), int(i2[best])])
+        else:
+            for cval in cs:
+                for c in np.compress(abs(cval - costheta) < -tol, i2):
+                    hits.append([0.0, int(i1[i]), int(c)])
+    return hits
+
+
+@pytest.mark.parametrize("tol", [0.002, 0.02, -0.002])
+def test_find_matches_reference_loop(gve, orc, ref, tol):
+    cell, _, rows = gve
+    gv = np.ascontiguousarray(rows[:, :3])
+    uc = U.cellfromstring(cell, ext=orc)
+    uc.makerings(rows[:, 5].max(), tol=0.005)
+    ra = IO.assigntorings(rows[:, 5], uc.ringds, 0.005)
+    ga = np.zeros(len(gv), np.int32) - 1
+    for r1, r2 in ((0, 1), (1, 2), (2, 2)):
+        cs = IO.ring_cosines(uc, r1, r2)
+        want = _reference_find(ref, gv, np.nonzero(ra == r1)[0], np.nonzero(ra == r2)[0], cs, tol)
+        got = IO.find(orc, gv, ra, ga, r1, r2, cs, tol)
+        assert got == want
+        assert len(got) > 0
+
+
+def test_orient_batched_matches_unitcell(gve, orc):
+    cell, _, rows = gve
+    gv = np.ascontiguousarray(rows[:, :3])
+    uc = U.cellfromstring(cell, ext=orc)
+    uc.makerings(rows[:, 5].max(), tol=0.005)
+    ra = IO.assigntorings(rows[:, 5], uc.ringds, 0.005)
+    cs = IO.ring_cosines(uc, 0, 1)
+    hits = IO.find(orc, gv, ra, np.zeros(len(gv), np.int32) - 1, 0, 1, cs, 0.002)
+    _, c2ab, matrs = uc.getanglehkls(0, 1)
+    ubis, choice = orc.orient_hits(gv, [h[1] for h in hits], [h[2] for h in hits], c2ab, matrs)
+    for h, u in zip(hits, ubis):
+        uc.orient(0, gv[h[1]], 1, gv[h[2]])
+        assert np.array_equal(u.view(np.int64), uc.UBI.view(np.int64))
+
+
+def test_oracle_indexer_finds_20_grains(gve, orc):
+    """test/ken_simul/testken.py: minpks 80, rings 0 and 1 -> 20 grains;
+    fight_over_peaks then gives test.idx's per-grain peak counts."""
+    cell, _, rows = gve
+    gv = np.ascontiguousarray(rows[:, :3])
+    uc = U.cellfromstring(cell, ext=orc)
+    res = IO.index(orc, uc, gv, rows[:, 5].copy(), 0, 1, minpks=80)
+    assert len(res["ubis"]) == 20
+    assert all(s > 80 for s in res["scores"])
+    _, _, gas = IO.fight_over_peaks(orc, gv, res["ubis"], 0.01)
+    k = np.load(os.path.join(HERE, "golden", "ken_simul.npz"))
+    assert Counter(gas.tolist()) == Counter(k["idx_npeaks"].tolist())
