@@ -15,7 +15,10 @@ from its sources) running the same pipeline on the host cores, the grain
 loop spread over threads (ctypes releases the GIL), with its per-grain
 counts compared to the GPU's.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 50] [--no-cpu] [--no-aux]
+
+The warmup is long enough for the GPU clock to settle: the first few
+launches of the score kernel run up to 20% slower than the steady state.
 """
 import argparse
 import json
@@ -39,8 +42,8 @@ PEAK_HBM = 8.0e12
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--grains", type=int, default=1000)
     ap.add_argument("--peaks", type=int, default=1_000_000)
     ap.add_argument("--tol", type=float, default=0.05)
@@ -229,6 +232,53 @@ def aux_refine_adjoint(threads, with_cpu):
     return out
 
 
+def aux_index(with_cpu, K=3000, seed=11):
+    """Ring-pair indexing (indexer.assigntorings + find + scorethem) of K
+    fcc grains' g-vectors on the GPU, against the CPU checker's sequential
+    restatement of the same search (one thread) on the same input."""
+    import torch
+    from imaged11_amd import indexing, simulate, unitcell
+    t0 = time.perf_counter()
+    w = simulate.indexing_workload(K=K, seed=seed)
+    gen_s = time.perf_counter() - t0
+    pars = dict(ring_1=0, ring_2=1, minpks=20, hkl_tol=0.02, max_grains=2 * K)
+    uc = unitcell.unitcell(list(w["cell"]), w["symmetry"])
+
+    def run():
+        ind = indexing.indexer(uc, w["gv"], **pars)
+        marks = [time.perf_counter()]
+        nh = 0
+        for step in (ind.assigntorings, ind.find, ind.scorethem):
+            step()
+            torch.cuda.synchronize()
+            marks.append(time.perf_counter())
+            nh = max(nh, len(ind.hits))
+        return ind, nh, np.diff(marks) * 1e3
+
+    run()  # the ring tables and cosine caches are built once per unit cell
+    ind, nh, ms = run()
+    out = {"grains": K, "peaks": int(len(w["gv"])), "gen_s": gen_s, "hits": nh,
+           "grains_found": len(ind.ubis), "assigntorings_ms": ms[0], "find_ms": ms[1],
+           "scorethem_ms": ms[2], "total_ms": float(ms.sum()),
+           "trial_pairs_scored": nh * int(len(w["gv"]))}
+    if with_cpu:
+        from oracle import oracle as O, indexer_oracle as IO
+        orc = O.load_oracle()
+        ucc = unitcell.unitcell(list(w["cell"]), w["symmetry"], ext=orc)
+        t0 = time.perf_counter()
+        res = IO.index(orc, ucc, w["gv"], w["ds"], 0, 1, minpks=20, hkl_tol=0.02,
+                       max_grains=2 * K)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_hits"] = len(res["hits"])
+        out["cpu"] = {"kind": "port", "threads": 1, "seconds": cpu_s,
+                      "sample": "the whole search, sequential C restatement + numpy"}
+        out["speedup_vs_cpu"] = cpu_s * 1e3 / out["total_ms"]
+        out["same_grains_and_scores_as_cpu"] = bool(
+            len(res["ubis"]) == len(ind.ubis) and res["scores"] == ind.scores
+            and np.array_equal(res["ga"], ind.ga))
+    return out
+
+
 def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
     """Large scan: compute_gv per grain translation + score_and_assign for
     K grains x N peaks (per-peak dty, 15% spurious) in one fused launch,
@@ -394,7 +444,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(wl, args.tol, threads, gpu_counts)
     if not args.no_aux:
         out["aux"] = {"refine_5000_grains": aux_refine_adjoint(threads, not args.no_cpu),
-                      "scan_20000_grains": aux_scan(threads, not args.no_cpu)}
+                      "scan_20000_grains": aux_scan(threads, not args.no_cpu),
+                      "index_3000_grains": aux_index(not args.no_cpu)}
     print(json.dumps(out))
 
 

@@ -76,7 +76,8 @@ SIGNATURES = {
     "gs_unit_rows": (_i, [_vp, _i, _vp, _i, _vp, _vp]),
     "gs_find_hits": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _vp, _vp]),
     "gs_find_all": (_i, [_vp, _i, _vp, _i, _vp, _i, _d, _vp, _vp, ctypes.c_longlong, _vp]),
-    "gs_orient_hits": (_i, [_vp, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp]),
+    "gs_orient_hits": (_i, [_vp, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gs_select_many": (_i, [_vp, _i, _vp, _i, _d, _vp, _vp, _vp, _vp]),
 }
 
 SCORE_CERTIFIED = 0

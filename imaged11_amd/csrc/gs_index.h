@@ -247,22 +247,26 @@ __global__ __launch_bounds__(GS_BLOCK) void k_find_all(
     if (!keys && local) atomicAdd(counter, local);
 }
 
-// unitcell.orient with crange <= 0 (ImageD11/unitcell.py:617-630, 649-653)
-// for many (i, j) peak pairs: costheta = g1.g2 / sqrt(|g1|^2 |g2|^2), the
-// nearest entry b of the sorted table c2ab (searchsorted 'left', then the
-// lower neighbour when strictly closer), and UBI = quickorient([g1; g2],
-// bt[b]).  choice[h] = b when choice is given.  Out-of-range peak indices
-// give a NaN UBI and choice -1.
+// unitcell.orient (ImageD11/unitcell.py:617-653) for many (i, j) peak
+// pairs: costheta = g1.g2 / sqrt(|g1|^2 |g2|^2) and UBI = quickorient([g1;
+// g2], bt[b]).  Without bsel, b is the nearest entry of the sorted table
+// c2ab (searchsorted 'left', then the lower neighbour when strictly
+// closer: the crange <= 0 case); with bsel, b = bsel[h] (one of the
+// crange > 0 variants).  choice[h] = b and costheta[h] when given.
+// Out-of-range peak or table indices give a NaN UBI and choice -1.
 __global__ void k_orient_hits(const double* __restrict__ gv, int ng, const int* __restrict__ hi,
                               const int* __restrict__ hj, int nh, const double* __restrict__ c2ab,
-                              int nc, const double* __restrict__ bt, double* __restrict__ ubis,
-                              int* __restrict__ choice) {
+                              int nc, const double* __restrict__ bt, const int* __restrict__ bsel,
+                              double* __restrict__ ubis, int* __restrict__ choice,
+                              double* __restrict__ costheta) {
     const long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nh) return;
     const int i = hi[h], j = hj[h];
     double u[9];
     int b = -1;
-    if (i >= 0 && i < ng && j >= 0 && j < ng) {
+    double ct = __builtin_nan("");
+    const int bfix = bsel ? bsel[h] : 0;
+    if (i >= 0 && i < ng && j >= 0 && j < ng && bfix >= 0 && bfix < nc) {
         for (int c = 0; c < 3; c++) {
             u[c] = gv[3 * (long long)i + c];
             u[3 + c] = gv[3 * (long long)j + c];
@@ -270,9 +274,11 @@ __global__ void k_orient_hits(const double* __restrict__ gv, int ng, const int* 
         const double dot = gs_dot3_ddot(u, u + 3);
         const double s1 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
         const double s2 = u[3] * u[3] + u[4] * u[4] + u[5] * u[5];
-        const double ct = dot / sqrt(s1 * s2);
+        ct = dot / sqrt(s1 * s2);
         int lo = 0;
-        if (ct != ct) {
+        if (bsel) {
+            lo = nc;  // unused: the table entry is given
+        } else if (ct != ct) {
             lo = nc;  // numpy orders NaN after every number
         } else {
             int hi_ = nc;
@@ -285,7 +291,10 @@ __global__ void k_orient_hits(const double* __restrict__ gv, int ng, const int* 
             }
         }
         b = lo;
-        if (lo > 0 && (lo == nc || fabs(ct - c2ab[lo - 1]) < fabs(ct - c2ab[lo]))) b = lo - 1;
+        if (bsel)
+            b = bfix;
+        else if (lo > 0 && (lo == nc || fabs(ct - c2ab[lo - 1]) < fabs(ct - c2ab[lo])))
+            b = lo - 1;
         u[6] = u[7] = u[8] = 0.0;
         gs_quickorient(u, bt + 9 * (long long)b);
     } else {
@@ -293,4 +302,57 @@ __global__ void k_orient_hits(const double* __restrict__ gv, int ng, const int* 
     }
     for (int c = 0; c < 9; c++) ubis[9 * h + c] = u[c];
     if (choice) choice[h] = b;
+    if (costheta) costheta[h] = ct;
+}
+
+// Peaks each of K UBIs indexes, with the reference's arithmetic: the pairs
+// with gs_drlv2 < thr (thr = tol^2 for score_and_refine's selection,
+// min(tol^2, 1) for getind's score_and_assign from drlv2 = 1).  cursor[g]
+// (zeroed by the caller) counts grain g's peaks; when idx is given they are
+// also written at idx[off[g] + slot] for slot < off[g+1] - off[g], in no
+// particular order.  Tiles as k_score_many: P peaks per lane in registers,
+// a split of gsplit grains per block; lane 0 of a wave takes the wave's
+// slots with one atomic per grain.
+template <int P>
+__global__ __launch_bounds__(GS_BLOCK) void k_select_many(
+    const double* __restrict__ ubis, int K, int gsplit, int nsplit, const double* __restrict__ gv,
+    int ng, double thr, const long long* __restrict__ off, int* __restrict__ cursor,
+    int* __restrict__ idx) {
+    const int tile = gs_xcd_remap(blockIdx.x, gridDim.x);
+    const int chunk = tile / nsplit;
+    const int split = tile - chunk * nsplit;
+    const int g0 = split * gsplit;
+    const int g1 = min(K, g0 + gsplit);
+    double x[P], y[P], z[P];
+    const long long base = (long long)chunk * (GS_BLOCK * P) + threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const long long k = base + (long long)p * GS_BLOCK;
+        if (k < ng) {
+            x[p] = gv[3 * k + 0];
+            y[p] = gv[3 * k + 1];
+            z[p] = gv[3 * k + 2];
+        } else {
+            x[p] = y[p] = z[p] = __builtin_nan("");  // never selected
+        }
+    }
+    const int lane = threadIdx.x & (GS_WAVE - 1);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int g = g0; g < g1; g++) {
+        const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)g);
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const bool pass = gs_drlv2(u, x[p], y[p], z[p]) < thr;
+            const unsigned long long m = __ballot(pass);
+            if (m == 0) continue;
+            int first = 0;
+            if (lane == 0) first = atomicAdd(&cursor[g], __popcll(m));
+            if (idx) {
+                first = __shfl(first, 0, GS_WAVE);
+                const long long cap = off[g + 1] - off[g];
+                const long long slot = first + __popcll(m & below);
+                if (pass && slot < cap) idx[off[g] + slot] = (int)(base + (long long)p * GS_BLOCK);
+            }
+        }
+    }
 }

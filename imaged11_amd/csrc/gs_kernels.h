@@ -107,14 +107,16 @@ __global__ __launch_bounds__(GS_BLOCK) void k_score_many(const double* __restric
 // x -> (x - round x)^2 changes by at most |dx| (it is continuous across
 // the half-integers): the rows move S by at most 6.7e-16 sum_i A_i <=
 // 6.7e-16 U1 Gmax, with U1 = sum |u_ij| and Gmax = max |g_j| over the
-// wave's peaks.  Both sums of squares are <= 3/4, so rounding them adds at
+// block's peaks.  Both sums of squares are <= 3/4, so rounding them adds at
 // most 5e-16 in absolute terms.  E = 1e-15 U1 Gmax + 1e-15 + 2e-15 tol^2
 // covers all of it with margin (the last term absorbs the rounding of
 // tol^2 +- E).  S_f < tol^2 - E proves S_e < tol^2; S_f >= tol^2 + E proves
 // the opposite; lanes in between are recomputed with the reference
 // arithmetic.
-// v_rndne equals the MAGIC rounding for |h| < 2^51, so a (wave, grain)
+// v_rndne equals the MAGIC rounding for |h| < 2^51, so a (block, grain)
 // with U1 Gmax >= 2^49 (or a non-finite g) takes the exact path entirely.
+// The window (tol^2 - E, tol^2 + E) of each grain is computed once per
+// block into LDS, so the grain loop spends its VALU issue on the pairs.
 __device__ __forceinline__ double gs_wave_max(double v) {
     for (int o = GS_WAVE / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, GS_WAVE));
     return v;
@@ -134,6 +136,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_score_many_fast(
     const double* __restrict__ ubis, int K, int gsplit, int nsplit,
     const double* __restrict__ gv, int ng, double tolsq, int* __restrict__ counts) {
     __shared__ int lcnt[GS_GRAIN_CHUNK];
+    __shared__ double lwin[2 * GS_GRAIN_CHUNK];  // (lo, hi) per grain of the split
+    __shared__ double lgm[GS_BLOCK / GS_WAVE];
     const int tile = gs_xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = tile / nsplit;
     const int split = tile - chunk * nsplit;
@@ -156,15 +160,24 @@ __global__ __launch_bounds__(GS_BLOCK) void k_score_many_fast(
         }
         gm = fmax(gm, fmax(gs_abs_finite(x[p]), fmax(gs_abs_finite(y[p]), gs_abs_finite(z[p]))));
     }
-    const double gmax = gs_wave_max(gm);  // +inf if any peak of the wave is infinite
-    __syncthreads();
     const int lane = threadIdx.x & (GS_WAVE - 1);
+    gm = gs_wave_max(gm);  // +inf if any peak of the wave is infinite
+    if (lane == 0) lgm[threadIdx.x / GS_WAVE] = gm;
+    __syncthreads();
+    double gmax = lgm[0];
+#pragma unroll
+    for (int w = 1; w < GS_BLOCK / GS_WAVE; w++) gmax = fmax(gmax, lgm[w]);
+    for (int i = threadIdx.x; i < g1 - g0; i += GS_BLOCK) {
+        const double e = gs_cert_window(gs_load_ubi(ubis + 9 * (size_t)(g0 + i)), gmax, tolsq);
+        lwin[2 * i] = tolsq - e;
+        lwin[2 * i + 1] = tolsq + e;
+    }
+    __syncthreads();
     gs_ubi un = gs_load_ubi(ubis + 9 * (size_t)g0);
     for (int g = g0; g < g1; g++) {
         const gs_ubi u = un;
         if (g + 1 < g1) un = gs_load_ubi(ubis + 9 * (size_t)(g + 1));  // prefetch
-        const double e = gs_cert_window(u, gmax, tolsq);
-        const double lo = tolsq - e, hi = tolsq + e;
+        const double lo = lwin[2 * (g - g0)], hi = lwin[2 * (g - g0) + 1];
         int c = 0;
         unsigned long long undecided = 0;
 #pragma unroll
@@ -278,6 +291,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_assign_many_fast(
     const double* __restrict__ gv, int ng, double tolsq, double* __restrict__ drlv2,
     int* __restrict__ labels, int* __restrict__ counts) {
     __shared__ int lcnt[GS_GRAIN_CHUNK];
+    __shared__ double lhi[GS_GRAIN_CHUNK];  // tol^2 + E per grain of the chunk
+    __shared__ double lgm[GS_BLOCK / GS_WAVE];
     double x[P], y[P], z[P], d[P];
     int lab[P];
     double gm = 0.0;
@@ -298,16 +313,24 @@ __global__ __launch_bounds__(GS_BLOCK) void k_assign_many_fast(
         }
         gm = fmax(gm, fmax(gs_abs_finite(x[p]), fmax(gs_abs_finite(y[p]), gs_abs_finite(z[p]))));
     }
-    const double gmax = gs_wave_max(gm);
     const int lane = threadIdx.x & (GS_WAVE - 1);
+    gm = gs_wave_max(gm);
+    if (lane == 0) lgm[threadIdx.x / GS_WAVE] = gm;
+    __syncthreads();
+    double gmax = lgm[0];
+#pragma unroll
+    for (int w = 1; w < GS_BLOCK / GS_WAVE; w++) gmax = fmax(gmax, lgm[w]);
     for (int c0 = 0; c0 < K; c0 += GS_GRAIN_CHUNK) {
         const int c1 = min(K, c0 + GS_GRAIN_CHUNK);
-        for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) lcnt[i] = 0;
+        for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) {
+            lcnt[i] = 0;
+            lhi[i] = tolsq + gs_cert_window(gs_load_ubi(ubis + 9 * (size_t)(c0 + i)), gmax, tolsq);
+        }
         __syncthreads();
         for (int g = c0; g < c1; g++) {
             const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)g);
             const int L = glabels[g];
-            const double hi = tolsq + gs_cert_window(u, gmax, tolsq);
+            const double hi = lhi[g - c0];
             int c = 0;
 #pragma unroll
             for (int p = 0; p < P; p++) {

@@ -189,6 +189,26 @@ int check_ng(long long ng) {
 
 // ---- launchers shared by both families ------------------------------------
 
+// Tiles of the all-pairs kernels: chunks of GS_BLOCK*P peaks times splits of
+// at most GS_GRAIN_CHUNK grains (the LDS tile), with enough tiles to fill
+// every CU several times over.
+int pair_tiles(DeviceCtx* ctx, int K, int ng, int P, int* gsplit_out, int* nsplit_out,
+               unsigned* nb_out) {
+    const long long nchunks = (ng + (long long)GS_BLOCK * P - 1) / ((long long)GS_BLOCK * P);
+    const long long want = 32LL * ctx->cus;
+    long long nsplit = std::max<long long>(1, (want + nchunks - 1) / nchunks);
+    nsplit = std::min<long long>(nsplit, std::max(1, K / 16));
+    nsplit = std::max<long long>(nsplit, (K + GS_GRAIN_CHUNK - 1) / GS_GRAIN_CHUNK);
+    const int gsplit = (int)((K + nsplit - 1) / nsplit);
+    nsplit = (K + gsplit - 1) / gsplit;
+    const long long nb = nchunks * nsplit;
+    if (nb > 0x7fffffffLL) return set_err(GS_ERR_ARG, "grid too large");
+    *gsplit_out = gsplit;
+    *nsplit_out = (int)nsplit;
+    *nb_out = (unsigned)nb;
+    return GS_OK;
+}
+
 int launch_score_many(const double* ubis, int K, const double* gv, int ng, double tol, int* counts,
                       int peaks_per_lane, int mode, hipStream_t st) {
     DeviceCtx* ctx;
@@ -199,41 +219,33 @@ int launch_score_many(const double* ubis, int K, const double* gv, int ng, doubl
     const double tolsq = tol * tol;
     int P = peaks_per_lane;
     if (P != 1 && P != 2 && P != 4 && P != 8 && P != 16) P = 8;
-    const long long nchunks = (ng + (long long)GS_BLOCK * P - 1) / ((long long)GS_BLOCK * P);
-    // enough tiles to fill every CU several times over; a split holds at
-    // most GS_GRAIN_CHUNK grains (the LDS counter tile)
-    const long long want = 32LL * ctx->cus;
-    long long nsplit = std::max<long long>(1, (want + nchunks - 1) / nchunks);
-    nsplit = std::min<long long>(nsplit, std::max(1, K / 16));
-    nsplit = std::max<long long>(nsplit, (K + GS_GRAIN_CHUNK - 1) / GS_GRAIN_CHUNK);
-    const int gsplit = (int)((K + nsplit - 1) / nsplit);
-    nsplit = (K + gsplit - 1) / gsplit;
-    const long long nb = nchunks * nsplit;
-    if (nb > 0x7fffffffLL) return set_err(GS_ERR_ARG, "grid too large");
-    const dim3 grid((unsigned)nb), block(GS_BLOCK);
+    int gsplit, nsplit;
+    unsigned nb;
+    GS_CHECK(pair_tiles(ctx, K, ng, P, &gsplit, &nsplit, &nb));
+    const dim3 grid(nb), block(GS_BLOCK);
     if (mode == GS_SCORE_EXACT) {
         switch (P) {
-            case 1: hipLaunchKernelGGL(k_score_many<1>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 2: hipLaunchKernelGGL(k_score_many<2>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 4: hipLaunchKernelGGL(k_score_many<4>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 16: hipLaunchKernelGGL(k_score_many<16>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            default: hipLaunchKernelGGL(k_score_many<8>, grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 1: hipLaunchKernelGGL(k_score_many<1>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 2: hipLaunchKernelGGL(k_score_many<2>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 4: hipLaunchKernelGGL(k_score_many<4>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 16: hipLaunchKernelGGL(k_score_many<16>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            default: hipLaunchKernelGGL(k_score_many<8>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
         }
     } else if (mode == GS_SCORE_CERTIFIED) {
         switch (P) {
-            case 1: hipLaunchKernelGGL((k_score_many_fast<1, false>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 2: hipLaunchKernelGGL((k_score_many_fast<2, false>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 4: hipLaunchKernelGGL((k_score_many_fast<4, false>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 16: hipLaunchKernelGGL((k_score_many_fast<16, false>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            default: hipLaunchKernelGGL((k_score_many_fast<8, false>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 1: hipLaunchKernelGGL((k_score_many_fast<1, false>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 2: hipLaunchKernelGGL((k_score_many_fast<2, false>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 4: hipLaunchKernelGGL((k_score_many_fast<4, false>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 16: hipLaunchKernelGGL((k_score_many_fast<16, false>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            default: hipLaunchKernelGGL((k_score_many_fast<8, false>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
         }
     } else {  // GS_SCORE_CERTIFIED_OUTLINE
         switch (P) {
-            case 1: hipLaunchKernelGGL((k_score_many_fast<1, true>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 2: hipLaunchKernelGGL((k_score_many_fast<2, true>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 4: hipLaunchKernelGGL((k_score_many_fast<4, true>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            case 16: hipLaunchKernelGGL((k_score_many_fast<16, true>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
-            default: hipLaunchKernelGGL((k_score_many_fast<8, true>), grid, block, 0, st, ubis, K, gsplit, (int)nsplit, gv, ng, tolsq, counts); break;
+            case 1: hipLaunchKernelGGL((k_score_many_fast<1, true>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 2: hipLaunchKernelGGL((k_score_many_fast<2, true>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 4: hipLaunchKernelGGL((k_score_many_fast<4, true>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 16: hipLaunchKernelGGL((k_score_many_fast<16, true>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            default: hipLaunchKernelGGL((k_score_many_fast<8, true>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
         }
     }
     GS_LAUNCHED();
@@ -976,15 +988,34 @@ int gs_find_all(const double* n1, int N1, const double* n2, int N2, const double
 }
 
 int gs_orient_hits(const double* gv, int ng, const int* hi, const int* hj, int nh,
-                   const double* c2ab, int nc, const double* bt, double* ubis, int* choice,
-                   gs_stream_t stream) {
+                   const double* c2ab, int nc, const double* bt, const int* bsel, double* ubis,
+                   int* choice, double* costheta, gs_stream_t stream) {
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
     if (ng < 0 || nh < 0) return set_err(GS_ERR_ARG, "gs_orient_hits: negative size");
     if (nc < 1) return set_err(GS_ERR_ARG, "gs_orient_hits: empty cosine table");
     if (nh == 0) return GS_OK;
     hipLaunchKernelGGL(k_orient_hits, dim3(blocks_for(nh, GS_BLOCK)), dim3(GS_BLOCK), 0,
-                       S(stream), gv, ng, hi, hj, nh, c2ab, nc, bt, ubis, choice);
+                       S(stream), gv, ng, hi, hj, nh, c2ab, nc, bt, bsel, ubis, choice,
+                       costheta);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_select_many(const double* ubis, int K, const double* gv, int ng, double thr,
+                   const long long* off, int* cursor, int* idx, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (K < 0 || ng < 0) return set_err(GS_ERR_ARG, "gs_select_many: negative size");
+    if (idx && !off) return set_err(GS_ERR_ARG, "gs_select_many: idx needs off");
+    GS_CHECK(fill_i32(cursor, 0, K, S(stream)));
+    if (K == 0 || ng == 0) return GS_OK;
+    constexpr int P = 4;
+    int gsplit, nsplit;
+    unsigned nb;
+    GS_CHECK(pair_tiles(ctx, K, ng, P, &gsplit, &nsplit, &nb));
+    hipLaunchKernelGGL(k_select_many<P>, dim3(nb), dim3(GS_BLOCK), 0, S(stream), ubis, K, gsplit,
+                       nsplit, gv, ng, thr, off, cursor, idx);
     GS_LAUNCHED();
     return GS_OK;
 }

@@ -12,13 +12,16 @@ Here:
   (``gs_find_hits``, or ``gs_find_all`` for a negative cosine tolerance);
 * ``scorethem`` builds every hit's trial UBI in one kernel
   (``gs_orient_hits``) and scores all of them in one launch of the
-  many-grain scorer (``gs_score_many``).  The reference's sequential
-  decision loop then runs on those precomputed scores: a hit's score does
-  not depend on earlier decisions, only whether it is looked at does, so
-  the loop visits the hits in the reference's order (last first), skips
-  the same ones, and makes the same expensive follow-ups (orientation
-  variants, ``score_and_refine``, ``getind``, uniqueness) on the GPU only
-  for hits above ``minpks``.
+  many-grain scorer (``gs_score_many``).  The follow-up of every hit above
+  ``minpks`` (orientation variants within the cosine tolerance, the best
+  of them, ``score_and_refine``, the ``getind`` peak list) does not depend
+  on earlier decisions either, so it is also computed for all such hits at
+  once: variants by ``gs_orient_hits``, their scores by ``gs_score_many``,
+  the refinement as one ``gs_refine_lists`` iteration on the peak lists
+  ``gs_select_many`` extracts, and the ``getind`` lists by
+  ``gs_select_many`` again.  The reference's decision loop then visits the
+  hits in its order (last first), skips the same ones and applies the
+  uniqueness rule to those precomputed results.
 
 Results (hits, grains, scores, peak assignments) are the reference's for
 the same g-vectors, unit-cell tables and parameters; see the parity tests.
@@ -103,9 +106,12 @@ def find_hits(gv, i1, i2, cs, cosine_tol):
     return [[0.0, int(i1[a]), int(i2[b])] for a, b in zip(i, j)]
 
 
-def orient_hits(uc, ring_1, ring_2, gv, hits):
-    """Trial UBI of every hit as unitcell.orient(ring_1, gv[i], ring_2,
-    gv[j]) makes it (crange <= 0), in one kernel.  Returns (nh,3,3)."""
+def orient_hits(uc, ring_1, ring_2, gv, hits, entries=None, with_cosines=False):
+    """Trial UBI of every hit (rows [diff, i, j]) as unitcell.orient(ring_1,
+    gv[i], ring_2, gv[j]) makes it with crange <= 0, in one kernel; with
+    `entries`, the UBI from table entry entries[h] instead (the crange > 0
+    variants).  Returns (nh,3,3) UBIs, and the observed cosines when
+    with_cosines."""
     lib = _lib.load()
     dev = gv.device
     _, c2ab, matrs = uc.getanglehkls(ring_1, ring_2)
@@ -113,13 +119,42 @@ def orient_hits(uc, ring_1, ring_2, gv, hits):
     hj = _dev_i32([h[2] for h in hits], dev)
     nh = len(hits)
     ubis = torch.empty((nh, 3, 3), dtype=torch.float64, device=dev)
+    cos = torch.empty(nh, dtype=torch.float64, device=dev) if with_cosines else None
+    bsel = _dev_i32(entries, dev) if entries is not None else None
     dc = _dev_f64(np.asarray(c2ab, dtype=np.float64), dev)
     dbt = _dev_f64(np.asarray(matrs, dtype=np.float64).reshape(-1, 3, 3), dev)
     if nh:
         _lib.check(lib.gs_orient_hits(dptr(gv), gv.shape[0], dptr(hi), dptr(hj), nh, dptr(dc),
-                                      len(c2ab), dptr(dbt), dptr(ubis), None, stream_ptr()),
-                   "orient_hits")
-    return ubis
+                                      len(c2ab), dptr(dbt),
+                                      dptr(bsel) if bsel is not None else None, dptr(ubis),
+                                      None, dptr(cos) if cos is not None else None,
+                                      stream_ptr()), "orient_hits")
+    return (ubis, cos) if with_cosines else ubis
+
+
+def select_lists(ubis, gv, thr):
+    """CSR lists of the peaks each UBI indexes with the reference
+    arithmetic (drlv2 < thr), each list in peak order.  ubis (K,3,3) and gv
+    (N,3) device tensors; returns (idx int32, off int64 (K+1,)) on the
+    device."""
+    lib = _lib.load()
+    dev = gv.device
+    K, N = ubis.shape[0], gv.shape[0]
+    cnt = torch.empty(K, dtype=torch.int32, device=dev)
+    _lib.check(lib.gs_select_many(dptr(ubis), K, dptr(gv), N, float(thr), None, dptr(cnt), None,
+                                  stream_ptr()), "select_many")
+    off = torch.zeros(K + 1, dtype=torch.int64, device=dev)
+    torch.cumsum(cnt.to(torch.int64), 0, out=off[1:])
+    total = int(off[-1].item())
+    idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+    if total:
+        _lib.check(lib.gs_select_many(dptr(ubis), K, dptr(gv), N, float(thr), dptr(off),
+                                      dptr(cnt), dptr(idx), stream_ptr()), "select_many")
+        owner = torch.repeat_interleave(torch.arange(K, device=dev, dtype=torch.int64),
+                                        cnt.to(torch.int64))
+        key = torch.sort(owner * N + idx[:total].to(torch.int64)).values
+        idx = (key - owner * N).to(torch.int32)
+    return idx[:total], off
 
 
 class indexer:
@@ -280,8 +315,16 @@ class indexer:
         cImageD11.score_and_assign(UBI, g, tol, drlv2, labels, 1)
         return (labels == 1).cpu().numpy()
 
-    def scorethem(self, fitb4=False):
-        """Decide which hits become grains (ImageD11/indexing.py:762-874)."""
+    def scorethem(self, fitb4=False, batched_followup=True):
+        """Decide which hits become grains (ImageD11/indexing.py:762-874).
+
+        Every hit's trial UBI is built and scored in two launches.  With
+        batched_followup (default) the follow-up of every hit above minpks
+        (orientation variants, score_and_refine, the getind peak list) is
+        also computed for all of them at once, since none of it depends on
+        earlier decisions; the reference's decision loop then only reads
+        those results.  batched_followup=False runs the follow-ups one hit
+        at a time inside the loop, as the reference does."""
         if self.hits is None or len(self.hits) == 0:
             logging.info("No hits to score")
             return
@@ -289,9 +332,12 @@ class indexer:
             raise NotImplementedError("fitb4 (marked as not working in the reference)")
         g = self._gvd()
         tol = float(self.hkl_tol)
-        uc = self.unitcell
-        trial = orient_hits(uc, self.ring_1, self.ring_2, g, self.hits)
+        trial = orient_hits(self.unitcell, self.ring_1, self.ring_2, g, self.hits)
         npk_all = batched.score_many(trial, g, tol).cpu().numpy()
+        if batched_followup:
+            follow = self._followups(g, tol, npk_all)
+        else:
+            follow = self._followup_one
         trial = trial.cpu().numpy()
         ng = 0
         self.nuniq = 0
@@ -301,34 +347,107 @@ class indexer:
             _, i, j = self.hits[h]
             if self.ga[i] > -1 or self.ga[j] > -1 or i == j:
                 continue
-            npk = int(npk_all[h])
-            UBI = trial[h].copy()
-            if npk > self.minpks:
-                uc.orient(self.ring_1, self.gv[i, :], self.ring_2, self.gv[j, :], verbose=0,
-                          crange=abs(self.cosine_tol))
-                if len(uc.UBIlist) > 1:
-                    cand = torch.as_tensor(np.array(uc.UBIlist), device=g.device)
-                    npks = batched.score_many(cand, g, tol).cpu().numpy()
-                    choice = int(np.argmax(npks))
-                    if npks[choice] >= npk:
-                        UBI = uc.UBIlist[choice].copy()
-                        npk = int(npks[choice])
-                cImageD11.score_and_refine(UBI, g, tol)
-                ind = self.getind(UBI)
-                ga = self.ga[ind]
-                with np.errstate(invalid="ignore", divide="ignore"):
-                    uniq = np.sum(np.where(ga == -1, 1, 0)) * 1.0 / ga.shape[0]
-                if uniq > self.uniqueness:
-                    self.ga[ind] = len(self.scores) + 1
-                    self.ubis.append(UBI)
-                    self.scores.append(npk)
-                    ng += 1
-                else:
-                    self.nuniq += 1
+            if npk_all[h] <= self.minpks:
+                continue
+            UBI, npk, ind = follow(h, trial[h], int(npk_all[h]))
+            ga = self.ga[ind]
+            with np.errstate(invalid="ignore", divide="ignore"):
+                uniq = np.sum(np.where(ga == -1, 1, 0)) * 1.0 / ga.shape[0]
+            if uniq > self.uniqueness:
+                self.ga[ind] = len(self.scores) + 1
+                self.ubis.append(UBI)
+                self.scores.append(npk)
+                ng += 1
+            else:
+                self.nuniq += 1
         # the reference pops every visited hit off its list
         self.hits = self.hits[:h]
         logging.info("Number of orientations with more than %d peaks is %d",
                      self.minpks, len(self.ubis))
+
+    def _followup_one(self, h, UBI, npk):
+        """The reference's follow-up of one hit above minpks
+        (ImageD11/indexing.py:801-837): the orientation variants within
+        cosine_tol, the best-scoring one if it scores at least as well,
+        score_and_refine, then getind.  Returns (UBI, npk, peak mask)."""
+        g = self._gvd()
+        tol = float(self.hkl_tol)
+        uc = self.unitcell
+        _, i, j = self.hits[h]
+        UBI = UBI.copy()
+        uc.orient(self.ring_1, self.gv[i, :], self.ring_2, self.gv[j, :], verbose=0,
+                  crange=abs(self.cosine_tol))
+        if len(uc.UBIlist) > 1:
+            cand = torch.as_tensor(np.array(uc.UBIlist), device=g.device)
+            npks = batched.score_many(cand, g, tol).cpu().numpy()
+            choice = int(np.argmax(npks))
+            if npks[choice] >= npk:
+                UBI = uc.UBIlist[choice].copy()
+                npk = int(npks[choice])
+        cImageD11.score_and_refine(UBI, g, tol)
+        return UBI, npk, self.getind(UBI)
+
+    def _followups(self, g, tol, npk_all):
+        """The follow-up of every hit above minpks at once; returns a
+        function h -> (UBI, npk, peak indices) reading the results."""
+        hits = self.hits
+        hi = np.array([x[1] for x in hits])
+        hj = np.array([x[2] for x in hits])
+        cand = np.nonzero((npk_all > self.minpks) & (hi != hj))[0]
+        slot = np.full(len(hits), -1, np.int64)
+        slot[cand] = np.arange(len(cand))
+        if len(cand) == 0:
+            return None
+        uc = self.unitcell
+        chits = [hits[h] for h in cand]
+        ubi, cos = orient_hits(uc, self.ring_1, self.ring_2, g, chits, with_cosines=True)
+        ubi = ubi.cpu().numpy()
+        npk = npk_all[cand].astype(np.int64)
+        crange = abs(self.cosine_tol)
+        if crange > 0:
+            # unitcell.orient with crange: every table entry within crange,
+            # reduced to distinct orientations, the best one taken when it
+            # scores at least as well as the trial (ImageD11/indexing.py:
+            # 803-825, ImageD11/unitcell.py:620-666)
+            _, c2ab, _ = uc.getanglehkls(self.ring_1, self.ring_2)
+            near = abs(np.asarray(c2ab)[None, :] - cos.cpu().numpy()[:, None]) < crange
+            rows, cols = np.nonzero(near)
+            var = orient_hits(uc, self.ring_1, self.ring_2, g, [chits[r] for r in rows],
+                              entries=cols).cpu().numpy()
+            starts = np.searchsorted(rows, np.arange(len(cand) + 1))
+            lists, owners = [], []
+            for c in range(len(cand)):
+                vs = list(var[starts[c]:starts[c + 1]])
+                if len(vs) > 1:
+                    vs = unitcell_mod.ubi_equiv(vs, [np.linalg.inv(v) for v in vs])
+                if len(vs) > 1:
+                    lists.extend(vs)
+                    owners.extend([c] * len(vs))
+            if lists:
+                owners = np.array(owners)
+                sc = batched.score_many(torch.as_tensor(np.array(lists), device=g.device), g,
+                                        tol).cpu().numpy()
+                firsts = np.searchsorted(owners, np.unique(owners))
+                for c, f in zip(np.unique(owners), firsts):
+                    n = np.sum(owners == c)
+                    choice = int(np.argmax(sc[f:f + n]))
+                    if sc[f + choice] >= npk[c]:
+                        ubi[c] = lists[f + choice]
+                        npk[c] = sc[f + choice]
+        # score_and_refine of each chosen UBI over all peaks = one refine
+        # iteration on the list of peaks it indexes
+        dubi = torch.as_tensor(np.ascontiguousarray(ubi), device=g.device)
+        idx, off = select_lists(dubi, g, tol * tol)
+        refined, _, _ = batched.refine_lists(dubi, g, idx, off, tol, n_iter=1)
+        # getind: score_and_assign from drlv2 = 1, labels = 0, label 1
+        idx2, off2 = select_lists(refined, g, min(tol * tol, 1.0))
+        refined = refined.cpu().numpy()
+        idx2, off2 = idx2.cpu().numpy(), off2.cpu().numpy()
+
+        def result(h, _trial, _npk):
+            c = slot[h]
+            return refined[c].copy(), int(npk[c]), idx2[off2[c]:off2[c + 1]]
+        return result
 
     def fight_over_peaks(self):
         """Give each peak to the grain indexing it best, then refine each

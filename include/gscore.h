@@ -274,15 +274,27 @@ int gs_find_all(const double *n1, int N1, const double *n2, int N2, const double
                 double mtol, unsigned long long *counter, long long *keys, long long cap,
                 gs_stream_t stream);
 
-/* unitcell.orient with crange <= 0 + quickorient (ImageD11/unitcell.py:
- * 617-653) for nh peak pairs (hi[h], hj[h]): ubis[9h..] = the trial UBI
- * from the sorted cosine table c2ab (nc entries) and its BT matrices
- * bt[9b..]; choice[h] = b (may be NULL).  Feed ubis to gs_score_many to
- * score every trial in one launch (indexer.scorethem,
- * ImageD11/indexing.py:762-800). */
+/* unitcell.orient + quickorient (ImageD11/unitcell.py:617-653) for nh peak
+ * pairs (hi[h], hj[h]) with the sorted cosine table c2ab (nc entries) and
+ * its BT matrices bt[9b..]: ubis[9h..] = the trial UBI.  bsel NULL: b is
+ * the entry nearest the observed cosine (orient with crange <= 0, the
+ * trial of indexer.scorethem, ImageD11/indexing.py:784-800); otherwise
+ * b = bsel[h] (the crange > 0 variants).  choice[h] = b and costheta[h] =
+ * the observed cosine when not NULL.  Feed ubis to gs_score_many to score
+ * every trial in one launch. */
 int gs_orient_hits(const double *gv, int ng, const int *hi, const int *hj, int nh,
-                   const double *c2ab, int nc, const double *bt, double *ubis, int *choice,
-                   gs_stream_t stream);
+                   const double *c2ab, int nc, const double *bt, const int *bsel, double *ubis,
+                   int *choice, double *costheta, gs_stream_t stream);
+
+/* The peaks each of K UBIs indexes with the reference arithmetic
+ * (drlv2 < thr): thr = tol^2 gives the peaks score_and_refine fits
+ * (src/closest.c:287-318), min(tol^2, 1) the mask of indexer.getind
+ * (score_and_assign from drlv2 = 1, ImageD11/indexing.py:1060-1088).
+ * cursor (K ints, zeroed here) receives the counts; with off (K+1 CSR
+ * offsets from those counts) and idx, the peak indices are also written,
+ * grain g's in idx[off[g]..off[g+1]) in no particular order. */
+int gs_select_many(const double *ubis, int K, const double *gv, int ng, double thr,
+                   const long long *off, int *cursor, int *idx, gs_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -139,15 +139,51 @@ def test_indexer_simAl_20_grains(orc):
     _compare(ind, res)
 
 
-def test_indexer_synthetic_matches_oracle(orc):
-    w = simulate.indexing_workload(K=300, seed=4)
+CASES = {
+    # fcc Al-like: one orientation per table entry
+    "fcc": dict(K=300, cell=(4.05, 4.05, 4.05, 90.0, 90.0, 90.0), symmetry="F", dsmax=1.2,
+                cosine_tol=0.002, minpks=20, min_grains=250),
+    # orthorhombic P: two distinct orientation variants within cosine_tol
+    # for every follow-up (the ubi_equiv + best-variant branch)
+    "ortho": dict(K=150, cell=(4.0, 4.04, 4.09, 90.0, 90.0, 90.0), symmetry="P", dsmax=1.0,
+                  cosine_tol=0.01, minpks=15, min_grains=60),
+}
+
+
+@pytest.mark.parametrize("batched_followup", [True, False], ids=["batched", "per_hit"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_indexer_synthetic_matches_oracle(orc, case, batched_followup):
+    c = CASES[case]
+    w = simulate.indexing_workload(K=c["K"], cell=c["cell"], symmetry=c["symmetry"],
+                                   dsmax=c["dsmax"], seed=4)
     uc_gpu = U.unitcell(list(w["cell"]), w["symmetry"])
-    ind = indexing.indexer(uc_gpu, w["gv"], ring_1=0, ring_2=1, minpks=20, hkl_tol=0.02,
-                           max_grains=1000)
+    ind = indexing.indexer(uc_gpu, w["gv"], ring_1=0, ring_2=1, minpks=c["minpks"],
+                           hkl_tol=0.02, cosine_tol=c["cosine_tol"], max_grains=1000)
     ind.assigntorings()
     ind.find()
-    ind.scorethem()
+    ind.scorethem(batched_followup=batched_followup)
     uc = U.unitcell(list(w["cell"]), w["symmetry"], ext=orc)
-    res = IO.index(orc, uc, w["gv"], w["ds"], 0, 1, minpks=20, hkl_tol=0.02, max_grains=1000)
-    assert len(res["ubis"]) > 250
+    res = IO.index(orc, uc, w["gv"], w["ds"], 0, 1, minpks=c["minpks"], hkl_tol=0.02,
+                   cosine_tol=c["cosine_tol"], max_grains=1000)
+    assert len(res["ubis"]) > c["min_grains"]
     _compare(ind, res)
+
+
+def test_select_lists_match_oracle(dev, orc):
+    """The peak lists of many UBIs (score_and_refine's selection and the
+    getind mask) against score_and_assign on fresh arrays."""
+    wl = simulate.scoring_workload(K=40, N=30000, seed=12)
+    g = np.zeros((30000, 3))
+    rng = np.random.default_rng(0)
+    g[:] = rng.normal(size=(30000, 3)) * 0.6
+    ubis = wl["ubi"]
+    for tol in (0.05, 0.3, 1.5):
+        thr = min(tol * tol, 1.0)
+        idx, off = indexing.select_lists(torch.as_tensor(ubis, device=dev),
+                                         torch.as_tensor(g, device=dev), thr)
+        idx, off = idx.cpu().numpy(), off.cpu().numpy()
+        assert np.all(np.diff(off) >= 0) and off[-1] == len(idx)
+        for k, u in enumerate(ubis):
+            d, lab = np.ones(len(g)), np.zeros(len(g), np.int32)
+            orc.score_and_assign(u, g, tol, d, lab, 1)
+            assert np.array_equal(idx[off[k]:off[k + 1]], np.nonzero(lab == 1)[0]), (tol, k)
