@@ -78,6 +78,13 @@ SIGNATURES = {
     "gs_find_all": (_i, [_vp, _i, _vp, _i, _vp, _i, _d, _vp, _vp, ctypes.c_longlong, _vp]),
     "gs_orient_hits": (_i, [_vp, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gs_select_many": (_i, [_vp, _i, _vp, _i, _d, _vp, _vp, _vp, _vp]),
+    "compute_geometry": (None, [_vp, _vp, _d, _d, _d, _d, _vp, _vp, _i]),
+    "compute_xlylzl_xpos_variable": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "gs_compute_geometry_dev": (_i, [_vp, _vp, _d, _d, _d, _d, _dp, _vp, _i, _vp]),
+    "gs_compute_geometry_trig_dev": (_i, [_vp, _vp, _vp, _dp, _d, _dp, _vp, _i, _vp]),
+    "gs_compute_xlylzl_xpos_dev": (_i, [_vp, _vp, _dp, _dp, _dp, _vp, _vp, _i, _vp]),
+    "gs_sf2geometry": (_i, [_vp, _vp, _vp, _dp, _dp, _dp, _d, _d, _d, _d, _dp, _vp, _vp, _i,
+                            _vp]),
 }
 
 SCORE_CERTIFIED = 0

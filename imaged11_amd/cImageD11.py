@@ -13,6 +13,8 @@ the extension's:
     compute_gv(xlylzl, omega, omegasign, wvln, wedge, chi, t, gv)
     compute_xlylzl(s, f, p, r, dist, xlylzl)
     verify_rounding(n)
+    compute_geometry(xlylzl, omega, omegasign, wvln, wedge, chi, t, out)
+    compute_xlylzl_xpos_variable(s, f, p, r, dist, xpos, xlylzl)
     ibest, best = closest(x, v)                       # indexer.find
     quickorient(ubi, bt)                              # unitcell.orient
 
@@ -40,7 +42,8 @@ from ._lib import GscoreError, dptr, hptr, darr, stream_ptr
 
 __all__ = [
     "verify_rounding", "score", "score_and_refine", "score_and_assign", "refine_assigned",
-    "compute_gv", "compute_xlylzl", "closest", "quickorient", "cimaged11_omp_set_num_threads",
+    "compute_gv", "compute_xlylzl", "compute_geometry", "compute_xlylzl_xpos_variable",
+    "closest", "quickorient", "cimaged11_omp_set_num_threads",
     "cimaged11_omp_get_max_threads", "GscoreError",
 ]
 
@@ -340,4 +343,61 @@ def quickorient(ubi, bt):
     lib = _lib_ready()
     lib.quickorient(hptr(u), hptr(b))
     _lib.check_host_call("quickorient")
+    return None
+
+
+def compute_geometry(xlylzl, omega, omegasign, wvln, wedge, chi, t, out):
+    """Per peak (tth, eta, ds, gx, gy, gz) from lab-frame positions and omega,
+    written into out (n, 6) (``src/cdiffraction.c:31-94``; the
+    ``updateGeometry`` pass of ``ImageD11/columnfile.py:469-504``)."""
+    tt = _in_array(np.asarray(t, dtype=np.float64).reshape(-1), np.float64, (3,), "t")
+    if _is_dev(out) or _is_dev(xlylzl) or _is_dev(omega):
+        x = _dev_in(xlylzl, torch.float64, (None, 3), "xlylzl")
+        n = x.shape[0]
+        w = _dev_in(omega, torch.float64, (n,), "omega", device=x.device)
+        o = _dev_inout(out, torch.float64, (n, 6), "out")
+        lib = _lib_ready()
+        _lib.check(lib.gs_compute_geometry_dev(dptr(x), dptr(w), float(omegasign), float(wvln),
+                                               float(wedge), float(chi), darr(tt), dptr(o), n,
+                                               stream_ptr()), "compute_geometry")
+        torch.cuda.current_stream().synchronize()
+        return None
+    x = _in_array(xlylzl, np.float64, (None, 3), "xlylzl")
+    n = x.shape[0]
+    w = _in_array(omega, np.float64, (n,), "omega")
+    o = _inout_array(out, np.float64, (n, 6), "out")
+    lib = _lib_ready()
+    lib.compute_geometry(hptr(x), hptr(w), float(omegasign), float(wvln), float(wedge),
+                         float(chi), hptr(tt), hptr(o), n)
+    _lib.check_host_call("compute_geometry")
+    return None
+
+
+def compute_xlylzl_xpos_variable(s, f, p, r, dist, xpos, xlylzl):
+    """compute_xlylzl with a per-peak x offset xpos subtracted
+    (``src/cdiffraction.c:245-277``; ``Ctransform.sf2gvlocal``)."""
+    pp = _in_array(p, np.float64, (4,), "p")
+    rr = _in_array(np.asarray(r, dtype=np.float64).reshape(-1), np.float64, (9,), "r")
+    dd = _in_array(dist, np.float64, (3,), "dist")
+    if _is_dev(xlylzl) or _is_dev(s) or _is_dev(f) or _is_dev(xpos):
+        ss = _dev_in(s, torch.float64, (None,), "s")
+        n = ss.shape[0]
+        ff = _dev_in(f, torch.float64, (n,), "f", device=ss.device)
+        xp = _dev_in(xpos, torch.float64, (n,), "xpos", device=ss.device)
+        out = _dev_inout(xlylzl, torch.float64, (n, 3), "xlylzl")
+        lib = _lib_ready()
+        _lib.check(lib.gs_compute_xlylzl_xpos_dev(dptr(ss), dptr(ff), darr(pp), darr(rr),
+                                                  darr(dd), dptr(xp), dptr(out), n,
+                                                  stream_ptr()), "compute_xlylzl_xpos_variable")
+        torch.cuda.current_stream().synchronize()
+        return None
+    ss = _in_array(s, np.float64, (None,), "s")
+    n = ss.shape[0]
+    ff = _in_array(f, np.float64, (n,), "f")
+    xp = _in_array(xpos, np.float64, (n,), "xpos")
+    out = _inout_array(xlylzl, np.float64, (n, 3), "xlylzl")
+    lib = _lib_ready()
+    lib.compute_xlylzl_xpos_variable(hptr(ss), hptr(ff), hptr(pp), hptr(rr), hptr(dd), hptr(xp),
+                                     hptr(out), n)
+    _lib.check_host_call("compute_xlylzl_xpos_variable")
     return None

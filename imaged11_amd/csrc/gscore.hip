@@ -25,6 +25,7 @@
 #include "gs_adjoint.h"
 #include "gs_assign_gv.h"
 #include "gs_index.h"
+#include "gs_geometry.h"
 #include "../../include/gscore.h"
 
 namespace {
@@ -344,6 +345,35 @@ int launch_gv(const double* xl, const double* omega, const double* so, const dou
     return GS_OK;
 }
 
+int launch_geometry(const double* xl, const double* omega, const double* so, const double* co,
+                    const gs_goniometer& gp, double* out, int n, hipStream_t st) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(n));
+    if (n == 0) return GS_OK;
+    if (so)
+        hipLaunchKernelGGL(k_geometry<1>, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, st,
+                           xl, omega, so, co, gp, out, n);
+    else
+        hipLaunchKernelGGL(k_geometry<0>, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, st,
+                           xl, omega, so, co, gp, out, n);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int launch_xlylzl_xpos(const double* s, const double* f, const double* p, const double* r,
+                       const double* dist, const double* xpos, double* xl, int n,
+                       hipStream_t st) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(n));
+    if (n == 0) return GS_OK;
+    hipLaunchKernelGGL(k_xlylzl_xpos, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, st, s, f,
+                       make_detector(p, r, dist), xpos, xl, n);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
 int check_refine_args(int K, int ng, long long n_idx, int n_iter) {
     if (K < 0 || n_iter < 0 || ng < 0 || n_idx < 0)
         return set_err(GS_ERR_ARG, "negative K, n_iter, ng or n_idx");
@@ -530,6 +560,50 @@ void compute_gv(double xlylzl[][3], double omega[], double omegasign, double wvl
     if (hipMemcpyAsync(gv, dg, (size_t)n * 24, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         set_err(GS_ERR_HIP, "compute_gv: copy back failed");
+}
+
+void compute_geometry(double xlylzl[][3], double omega[], double omegasign, double wvln,
+                      double wedge, double chi, double t[3], double out[][6], int n) {
+    if (check_ng(n) != GS_OK || n == 0) return;
+    std::vector<char*> sl;
+    if (stage_slices({(size_t)n * 24, (size_t)n * 8, (size_t)n * 48}, sl) != GS_OK) return;
+    double* dx = reinterpret_cast<double*>(sl[0]);
+    double* dw = reinterpret_cast<double*>(sl[1]);
+    double* dout = reinterpret_cast<double*>(sl[2]);
+    hipStream_t st = nullptr;
+    if (hipMemcpyAsync(dx, xlylzl, (size_t)n * 24, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dw, omega, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "compute_geometry: copy to device failed");
+        return;
+    }
+    gs_goniometer gp = make_gonio(omegasign, wvln, wedge, chi, t, nullptr);
+    if (launch_geometry(dx, dw, nullptr, nullptr, gp, dout, n, st) != GS_OK) return;
+    if (hipMemcpyAsync(out, dout, (size_t)n * 48, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        set_err(GS_ERR_HIP, "compute_geometry: copy back failed");
+}
+
+void compute_xlylzl_xpos_variable(double s[], double f[], double p[4], double r[9],
+                                  double dist[3], double xpos[], double xlylzl[][3], int n) {
+    if (check_ng(n) != GS_OK || n == 0) return;
+    std::vector<char*> sl;
+    if (stage_slices({(size_t)n * 8, (size_t)n * 8, (size_t)n * 8, (size_t)n * 24}, sl) != GS_OK)
+        return;
+    double* ds = reinterpret_cast<double*>(sl[0]);
+    double* df = reinterpret_cast<double*>(sl[1]);
+    double* dp = reinterpret_cast<double*>(sl[2]);
+    double* dx = reinterpret_cast<double*>(sl[3]);
+    hipStream_t st = nullptr;
+    if (hipMemcpyAsync(ds, s, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(df, f, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dp, xpos, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "compute_xlylzl_xpos_variable: copy to device failed");
+        return;
+    }
+    if (launch_xlylzl_xpos(ds, df, p, r, dist, dp, dx, n, st) != GS_OK) return;
+    if (hipMemcpyAsync(xlylzl, dx, (size_t)n * 24, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        set_err(GS_ERR_HIP, "compute_xlylzl_xpos_variable: copy back failed");
 }
 
 void compute_xlylzl(double s[], double f[], double p[4], double r[9], double dist[3],
@@ -858,6 +932,44 @@ int gs_gv_lists(const double* xlylzl, const double* so, const double* co, int ng
     gs_goniometer gp = make_gonio(1.0, wvln, wedge, chi, nullptr, mat);
     hipLaunchKernelGGL(k_gv_lists, dim3(blocks_for(n_idx, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
                        xlylzl, so, co, ng, gp, trans, K, idx, owner, n_idx, out);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+// ---- column-file geometry ----------------------------------------------------
+
+int gs_compute_geometry_dev(const double* xlylzl, const double* omega, double omegasign,
+                            double wvln, double wedge, double chi, const double t[3], double* out,
+                            int n, gs_stream_t stream) {
+    gs_goniometer gp = make_gonio(omegasign, wvln, wedge, chi, t, nullptr);
+    return launch_geometry(xlylzl, omega, nullptr, nullptr, gp, out, n, S(stream));
+}
+
+int gs_compute_geometry_trig_dev(const double* xlylzl, const double* so, const double* co,
+                                 const double mat[9], double wvln, const double t[3], double* out,
+                                 int n, gs_stream_t stream) {
+    if (!so || !co || !mat) return set_err(GS_ERR_ARG, "gs_compute_geometry_trig_dev: NULL");
+    gs_goniometer gp = make_gonio(1.0, wvln, 0.0, 0.0, t, mat);
+    return launch_geometry(xlylzl, nullptr, so, co, gp, out, n, S(stream));
+}
+
+int gs_compute_xlylzl_xpos_dev(const double* s, const double* f, const double p[4],
+                               const double r[9], const double dist[3], const double* xpos,
+                               double* xlylzl, int n, gs_stream_t stream) {
+    return launch_xlylzl_xpos(s, f, p, r, dist, xpos, xlylzl, n, S(stream));
+}
+
+int gs_sf2geometry(const double* s, const double* f, const double* omega, const double p[4],
+                   const double r[9], const double dist[3], double omegasign, double wvln,
+                   double wedge, double chi, const double t[3], double* xlylzl_out, double* out,
+                   int n, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(n));
+    if (n == 0) return GS_OK;
+    hipLaunchKernelGGL(k_sf2geometry, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
+                       s, f, omega, make_detector(p, r, dist),
+                       make_gonio(omegasign, wvln, wedge, chi, t, nullptr), xlylzl_out, out, n);
     GS_LAUNCHED();
     return GS_OK;
 }

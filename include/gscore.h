@@ -233,6 +233,44 @@ int gs_gv_lists(const double *xlylzl, const double *so, const double *co, int ng
                 const int *idx, const int *owner, long long n_idx, double *out,
                 gs_stream_t stream);
 
+/* ---- column-file geometry ---- */
+
+/* compute_geometry (src/cdiffraction.c:31-94; f2py src/_cImageD11.pyf:534-549):
+ * per peak out[k] = (tth, eta, ds, gx, gy, gz)
+ * from lab positions and omega (columnfile.updateGeometry,
+ * ImageD11/columnfile.py:469-504).  Host pointers. */
+void compute_geometry(double xlylzl[][3], double omega[], double omegasign, double wvln,
+                      double wedge, double chi, double t[3], double out[][6], int n);
+
+/* compute_xlylzl_xpos_variable (src/cdiffraction.c:245-277; f2py
+ * src/_cImageD11.pyf:585-601): compute_xlylzl
+ * with xpos[k] subtracted from x (Ctransform.sf2gvlocal,
+ * ImageD11/transform.py:740-751).  Host pointers. */
+void compute_xlylzl_xpos_variable(double s[], double f[], double p[4], double r[9],
+                                  double dist[3], double xpos[], double xlylzl[][3], int n);
+
+/* the same two on device memory */
+int gs_compute_geometry_dev(const double *xlylzl, const double *omega, double omegasign,
+                            double wvln, double wedge, double chi, const double t[3], double *out,
+                            int n, gs_stream_t stream);
+int gs_compute_xlylzl_xpos_dev(const double *s, const double *f, const double p[4],
+                               const double r[9], const double dist[3], const double *xpos,
+                               double *xlylzl, int n, gs_stream_t stream);
+
+/* compute_geometry with per-peak sin/cos of omega and the wedge/chi matrix
+ * given (the arithmetic after the libm calls). */
+int gs_compute_geometry_trig_dev(const double *xlylzl, const double *so, const double *co,
+                                 const double mat[9], double wvln, const double t[3], double *out,
+                                 int n, gs_stream_t stream);
+
+/* compute_xlylzl -> compute_geometry fused, the updateGeometry pass of
+ * ImageD11/columnfile.py:491-504 (Ctransform.sf2xyz + xyz2geometry);
+ * xlylzl_out may be NULL. */
+int gs_sf2geometry(const double *s, const double *f, const double *omega, const double p[4],
+                   const double r[9], const double dist[3], double omegasign, double wvln,
+                   double wedge, double chi, const double t[3], double *xlylzl_out, double *out,
+                   int n, gs_stream_t stream);
+
 /* ---- indexer: ring assignment, pair search, trial orientations ---- */
 
 /* closest (src/closest.c:188-207; f2py src/_cImageD11.pyf:88-103): the index in x of

@@ -314,6 +314,60 @@ void orc_compute_xlylzl(const double *s, const double *f, const double p[4],
     }
 }
 
+/* src/cdiffraction.c:245-277 */
+void orc_compute_xlylzl_xpos_variable(const double *s, const double *f, const double p[4],
+                                      const double r[9], const double dist[3],
+                                      const double *xpos, double *xlylzl, int n) {
+    for (int i = 0; i < n; i++) {
+        double v1 = (f[i] - p[1]) * p[3];
+        double v2 = (s[i] - p[0]) * p[2];
+        xlylzl[3 * i + 0] = r[1] * v1 + r[2] * v2 + dist[0] - xpos[i];
+        xlylzl[3 * i + 1] = r[4] * v1 + r[5] * v2 + dist[1];
+        xlylzl[3 * i + 2] = r[7] * v1 + r[8] * v2 + dist[2];
+    }
+}
+
+/* one peak of compute_geometry after the libm sin/cos
+ * (src/cdiffraction.c:62-92); the DEG macro expands to 180.0 / PI. */
+static void orc_geometry_one(const double *xl, double so, double co, const double mat[9],
+                             double wvln, const double t[3], double *out) {
+    double u[3] = {co * t[0] - so * t[1], so * t[0] + co * t[1], t[2]}, o[3], d[3], k[3], v[3];
+    for (int r = 0; r < 3; r++) o[r] = mat[3 * r] * u[0] + mat[3 * r + 1] * u[1] + mat[3 * r + 2] * u[2];
+    for (int r = 0; r < 3; r++) d[r] = xl[r] - o[r];
+    double modyz = 1. / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    out[0] = 180.0 / ORC_PI * atan2(sqrt(d[1] * d[1] + d[2] * d[2]), d[0]);
+    double ds = 1. / wvln;
+    k[0] = ds * (d[0] * modyz - 1.);
+    k[1] = ds * d[1] * modyz;
+    k[2] = ds * d[2] * modyz;
+    out[1] = 180.0 / ORC_PI * atan2(-d[1], d[2]);
+    out[2] = sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+    for (int c = 0; c < 3; c++) v[c] = mat[c] * k[0] + mat[3 + c] * k[1] + mat[6 + c] * k[2];
+    out[3] = co * v[0] + so * v[1];
+    out[4] = -so * v[0] + co * v[1];
+    out[5] = v[2];
+}
+
+/* src/cdiffraction.c:31-94 (compute_geometry); out is (n, 6) */
+void orc_compute_geometry(const double *xlylzl, const double *omega, double omegasign,
+                          double wvln, double wedge, double chi, const double t[3], double *out,
+                          int n) {
+    double mat[9];
+    orc_wedge_chi_matrix(wedge, chi, mat);
+    for (int i = 0; i < n; i++) {
+        double so, co;
+        orc_sincos(ORC_PI / 180.0 * omega[i] * omegasign, &so, &co);
+        orc_geometry_one(xlylzl + 3 * i, so, co, mat, wvln, t, out + 6 * i);
+    }
+}
+
+/* compute_geometry with the per-peak sin/cos and wedge/chi matrix given */
+void orc_compute_geometry_trig(const double *xlylzl, const double *so, const double *co,
+                               const double mat[9], double wvln, const double t[3], double *out,
+                               int n) {
+    for (int i = 0; i < n; i++) orc_geometry_one(xlylzl + 3 * i, so[i], co[i], mat, wvln, t, out + 6 * i);
+}
+
 /* ---------------------------------------------------------------------
  * Loop drivers: the Python callers' per-grain loops, in C so that the
  * oracle can check full-size batched results in seconds.

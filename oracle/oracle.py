@@ -83,6 +83,13 @@ class _Base:
         self._xl = getattr(L, p + "compute_xlylzl")
         self._xl.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
         self._xl.restype = None
+        self._geom = getattr(L, p + "compute_geometry")
+        self._geom.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, _dp, _dp, ctypes.c_int]
+        self._geom.restype = None
+        self._xpos = getattr(L, p + "compute_xlylzl_xpos_variable")
+        self._xpos.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
+        self._xpos.restype = None
         self._closest = getattr(L, p + "closest")
         self._closest.argtypes = [_dp, _dp, _ip, _dp, ctypes.c_int, ctypes.c_int]
         self._closest.restype = None
@@ -142,6 +149,20 @@ class _Base:
         assert xlylzl.dtype == np.float64 and xlylzl.flags.c_contiguous
         self._xl(_d(s), _d(f), _d(p), _d(r), _d(dist), _d(xlylzl), s.shape[0])
 
+    def compute_geometry(self, xlylzl, omega, omegasign, wvln, wedge, chi, t, out):
+        xlylzl = _f64(xlylzl)
+        omega = _f64(omega)
+        t = _f64(t, (3,))
+        assert out.dtype == np.float64 and out.flags.c_contiguous and out.shape == (len(omega), 6)
+        self._geom(_d(xlylzl), _d(omega), float(omegasign), float(wvln), float(wedge),
+                   float(chi), _d(t), _d(out), omega.shape[0])
+
+    def compute_xlylzl_xpos_variable(self, s, f, p, r, dist, xpos, xlylzl):
+        s, f, xpos = _f64(s), _f64(f), _f64(xpos)
+        p, r, dist = _f64(p, (4,)), _f64(r, (9,)), _f64(dist, (3,))
+        assert xlylzl.dtype == np.float64 and xlylzl.flags.c_contiguous
+        self._xpos(_d(s), _d(f), _d(p), _d(r), _d(dist), _d(xpos), _d(xlylzl), s.shape[0])
+
     def closest(self, x, v):
         x = _f64(x)
         v = _f64(v)
@@ -200,6 +221,8 @@ class Oracle(_Base):
         L.orc_openmp_threads.restype = ctypes.c_int
         L.orc_unit_rows.argtypes = [_dp, _ip, ctypes.c_int, _dp]
         L.orc_dot3_rows.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_compute_geometry_trig.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp,
+                                                ctypes.c_int]
         L.orc_find_hits.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip,
                                     _dp]
         L.orc_find_all.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
@@ -207,6 +230,15 @@ class Oracle(_Base):
         L.orc_find_all.restype = ctypes.c_longlong
         L.orc_orient_hits.argtypes = [_dp, _ip, _ip, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
                                       _ip]
+
+    def compute_geometry_trig(self, xlylzl, so, co, mat, wvln, t):
+        xlylzl = _f64(xlylzl)
+        so, co = _f64(so), _f64(co)
+        mat, t = _f64(mat, (9,)), _f64(t, (3,))
+        out = np.empty((len(so), 6))
+        self.lib.orc_compute_geometry_trig(_d(xlylzl), _d(so), _d(co), _d(mat), float(wvln),
+                                           _d(t), _d(out), len(so))
+        return out
 
     def dot3_rows(self, x, y, mode):
         x = _f64(x, (-1, 3))
