@@ -205,12 +205,23 @@ def ken_simul(ref):
                 ra_n=np.array([a for a, _ in ra], np.int32), ra_mean=np.array([b for _, b in ra]))
 
 
+DATA_FILES = [  # (source under the reference's test/, name here)
+    ("ken_simul/simAl.gve", "simAl.gve"),  # input of testken.py's 20-grain indexing test
+    ("makemap/test.flt", "makemap_test.flt"),  # a peak table ...
+    ("makemap/test.prm", "makemap_test.prm"),  # ... its parameters ...
+    ("makemap/map.ubi", "makemap_map.ubi"),  # ... and the 36 grains mapped from it
+    ("quantix/g3.flt", "quantix_g3.flt"),  # one grain's 229 peaks ...
+    ("quantix/g3.pars", "quantix_g3.pars"),  # ... fitted parameters ...
+    ("quantix/g3.ubi", "quantix_g3.ubi"),  # ... and its UBI (g3.idx: Npeaks=229)
+]
+
+
 def copy_data_files():
-    """Data files of the reference's own tests, kept verbatim: simAl.gve
-    (test/ken_simul, the input of testken.py's 20-grain indexing test)."""
+    """Data files of the reference's own tests, kept verbatim."""
     import shutil
-    shutil.copyfile(os.path.join(REFROOT, "test", "ken_simul", "simAl.gve"),
-                    os.path.join(HERE, "simAl.gve"))
+    for src, dst in DATA_FILES:
+        shutil.copyfile(os.path.join(REFROOT, "test", src), os.path.join(HERE, dst))
+        os.chmod(os.path.join(HERE, dst), 0o644)
 
 
 def main():

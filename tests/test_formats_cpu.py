@@ -1,0 +1,144 @@
+"""Host-side formats and the column-file geometry checker, without a GPU:
+parameter files, column files (.flt), grain files (.ubi) read from the
+reference's own test data and written back; compute_geometry /
+compute_xlylzl_xpos_variable of the oracle against the reference C and
+closed forms."""
+import math
+import os
+
+import numpy as np
+
+from imaged11_amd import columnfile, geometry, grain, parameters
+from test_oracle_math import _gv_numpy, _xyz_lab
+
+G = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_parameter_file_types_and_roundtrip(tmp_path):
+    p = parameters.read_par_file(os.path.join(G, "makemap_test.prm"))
+    assert p["distance"] == 218355.233382 and isinstance(p["distance"], float)
+    assert p["o22"] == -1 and isinstance(p["o22"], int)
+    assert p["omegasign"] == 1.0
+    assert p["cell_lattice_[P,A,B,C,I,F,R]"] == "F"
+    out = tmp_path / "x.prm"
+    parameters.parameters(**p).saveparameters(str(out))
+    assert parameters.read_par_file(str(out)) == p
+    q = parameters.read_par_file(os.path.join(G, "quantix_g3.pars"))
+    assert q["omegasign"] == 1 and q["t_x"] == 35.3542632362
+
+
+def test_columnfile_read_write_roundtrip(tmp_path):
+    c = columnfile.columnfile(os.path.join(G, "makemap_test.flt"))
+    assert c.titles[:3] == ["xc", "yc", "omega"] and c.ncols == 10
+    raw = [ln for ln in open(os.path.join(G, "makemap_test.flt")) if not ln.startswith("#")]
+    assert c.nrows == len([ln for ln in raw if ln.strip()])
+    assert c.xc[0] == 1215.083488 and c.omega[0] == -88.5
+    c.parameters.set("distance", 1234.5)
+    c.addcolumn(np.arange(c.nrows, dtype=float), "labels")
+    out = tmp_path / "y.flt"
+    c.writefile(str(out))
+    d = columnfile.columnfile(str(out))
+    assert d.titles == c.titles
+    assert d.parameters.get("distance") == 1234.5
+    for t in c.titles:
+        fmt = columnfile.FORMATS.get(t, "%f")  # omega: "%.4f" (the reference's table)
+        decimals = int(fmt[2:-1]) if fmt.startswith("%.") and fmt.endswith("f") else 6
+        np.testing.assert_allclose(d[t], c[t], rtol=0, atol=0.51 * 10.0 ** -decimals)
+        if fmt == "%f":
+            np.testing.assert_array_equal(d[t], c[t])  # 6-decimal data round-trips exactly
+    # filtering and copies keep columns aligned
+    e = d.copy()
+    e.filter(e.omega > 0)
+    assert e.nrows == int((d.omega > 0).sum()) and len(e.xc) == e.nrows
+    text = open(out).read().splitlines()
+    assert text[0] == "# distance = 1234.5"
+    assert text[1].split() == ["#"] + c.titles
+
+
+def test_grain_file_roundtrip(tmp_path):
+    gs = grain.read_grain_file(os.path.join(G, "makemap_map.ubi"))
+    assert len(gs) == 36
+    assert gs[0].name.strip() == "2:test_fix.flt" and int(gs[0].npks) == 73
+    np.testing.assert_array_equal(gs[0].translation, [208.353777, -126.476401, -0.711959])
+    np.testing.assert_array_equal(gs[0].ubi[0], [3.437417, -1.651156, -1.368126])
+    out = tmp_path / "z.ubi"
+    grain.write_grain_file(str(out), gs)
+    hs = grain.read_grain_file(str(out))
+    assert len(hs) == 36
+    for a, b in zip(gs, hs):
+        np.testing.assert_array_equal(a.ubi, b.ubi)  # %.9g keeps 7-digit data
+        np.testing.assert_allclose(a.translation, b.translation, rtol=1e-5)  # %g: 6 digits
+        assert int(a.npks) == int(b.npks)
+    g3 = grain.read_grain_file(os.path.join(G, "quantix_g3.ubi"))
+    assert len(g3) == 1 and g3[0].translation is None
+
+
+def _geometry_inputs(n=5000, seed=0):
+    rng = np.random.default_rng(seed)
+    xl = np.stack([rng.uniform(1e5, 3e5, n), rng.uniform(-1e5, 1e5, n),
+                   rng.uniform(-1e5, 1e5, n)], 1)
+    return xl, rng.uniform(-360, 360, n)
+
+
+def test_compute_geometry_oracle_matches_reference(orc, ref):
+    xl, om = _geometry_inputs()
+    for w, c, t, sign in ((0, 0, (0, 0, 0), 1.0), (1.3, -2.1, (12., -40., 3.), -1.0),
+                          (-0.5, 0.7, (200, 300, -50), 1.0)):
+        a, b = np.zeros((len(om), 6)), np.zeros((len(om), 6))
+        orc.compute_geometry(xl, om, sign, 0.2, w, c, np.array(t, float), a)
+        ref.compute_geometry(xl, om, sign, 0.2, w, c, np.array(t, float), b)
+        assert np.array_equal(a.view(np.int64), b.view(np.int64))
+    rng = np.random.default_rng(1)
+    s, f, xp = rng.uniform(0, 2048, 3000), rng.uniform(0, 2048, 3000), rng.normal(size=3000)
+    p = np.array([1024.0, 1000.0, 50.0, 49.0])
+    r, dist = rng.normal(size=9), np.array([1.5e5, 3.0, -2.0])
+    a, b = np.zeros((3000, 3)), np.zeros((3000, 3))
+    orc.compute_xlylzl_xpos_variable(s, f, p, r, dist, xp, a)
+    ref.compute_xlylzl_xpos_variable(s, f, p, r, dist, xp, b)
+    assert np.array_equal(a.view(np.int64), b.view(np.int64))
+    # xpos shifts x only, by exactly xpos against compute_xlylzl
+    c0 = np.zeros((3000, 3))
+    orc.compute_xlylzl(s, f, p, r, dist, c0)
+    np.testing.assert_array_equal(a[:, 1:], c0[:, 1:])
+    np.testing.assert_allclose(a[:, 0], c0[:, 0] - xp, rtol=0, atol=1e-10)
+
+
+def test_compute_geometry_closed_forms(orc):
+    """g equals compute_gv's, d* = |g|, and tth, eta, g follow the numpy
+    formula chain of transform.compute_tth_eta_from_xyz +
+    compute_g_vectors (ImageD11/transform.py:203-239,456-526)."""
+    xl, om = _geometry_inputs(2000, seed=3)
+    for w, c, t in ((0.0, 0.0, (0., 0., 0.)), (2.0, -1.5, (30., -20., 5.))):
+        out = np.zeros((len(om), 6))
+        orc.compute_geometry(xl, om, 1.0, 0.15, w, c, np.array(t), out)
+        gv = np.zeros((len(om), 3))
+        orc.compute_gv(xl, om, 1.0, 0.15, w, c, np.array(t), gv)
+        assert np.array_equal(out[:, 3:].view(np.int64), gv.view(np.int64))
+        np.testing.assert_allclose(out[:, 2], np.sqrt((gv * gv).sum(1)), rtol=1e-14)
+        np.testing.assert_allclose(out[:, 3:], _gv_numpy(xl, om, 1.0, 0.15, w, c, np.array(t)),
+                                   rtol=1e-9, atol=1e-12)
+        # tth, eta of the lab vector from the (rotated) grain origin
+        if t == (0., 0., 0.):
+            d = xl
+            tth = np.degrees(np.arctan2(np.sqrt(d[:, 1] ** 2 + d[:, 2] ** 2), d[:, 0]))
+            eta = np.degrees(np.arctan2(-d[:, 1], d[:, 2]))
+            np.testing.assert_allclose(out[:, 0], tth, rtol=1e-13)
+            np.testing.assert_allclose(out[:, 1], eta, rtol=1e-13, atol=1e-12)
+            # Bragg: d* = 2 sin(theta) / lambda
+            np.testing.assert_allclose(out[:, 2], 2 * np.sin(np.radians(tth) / 2) / 0.15,
+                                       rtol=1e-12)
+
+
+def test_xyz_lab_matches_formula_on_fixture():
+    """The test.prm detector of the makemap fixture through the numpy
+    formula chain of transform.compute_xyz_lab, as a sanity check of the
+    parameter packing used by columnfile.updateGeometry."""
+    p = parameters.read_par_file(os.path.join(G, "makemap_test.prm"))
+    c = columnfile.columnfile(os.path.join(G, "makemap_test.flt"))
+    pd, r, dist = geometry.pack_xlylzl(p)
+    v1 = (c.yc - pd[1]) * pd[3]
+    v2 = (c.xc - pd[0]) * pd[2]
+    xl = np.stack([r[1] * v1 + r[2] * v2 + dist[0], r[4] * v1 + r[5] * v2 + dist[1],
+                   r[7] * v1 + r[8] * v2 + dist[2]], 1)
+    np.testing.assert_allclose(xl, _xyz_lab(c.xc, c.yc, p), rtol=1e-12, atol=1e-6)
+    assert math.isclose(float(np.median(np.linalg.norm(xl, axis=1))), 218355.0, rel_tol=0.05)
