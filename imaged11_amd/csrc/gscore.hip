@@ -191,14 +191,16 @@ int check_ng(long long ng) {
 // ---- launchers shared by both families ------------------------------------
 
 // Tiles of the all-pairs kernels: chunks of GS_BLOCK*P peaks times splits of
-// at most GS_GRAIN_CHUNK grains (the LDS tile), with enough tiles to fill
-// every CU several times over.
+// at most GS_GRAIN_CHUNK grains (the LDS tile).  About 128 tiles per CU:
+// with 4 resident blocks per CU that is ~32 rounds, so the partly filled
+// last round costs ~3% (at 32 per CU it cost ~10%), while a split keeps
+// at least 8 grains to amortise the tile's peak loads and window setup.
 int pair_tiles(DeviceCtx* ctx, int K, int ng, int P, int* gsplit_out, int* nsplit_out,
                unsigned* nb_out) {
     const long long nchunks = (ng + (long long)GS_BLOCK * P - 1) / ((long long)GS_BLOCK * P);
-    const long long want = 32LL * ctx->cus;
+    const long long want = 128LL * ctx->cus;
     long long nsplit = std::max<long long>(1, (want + nchunks - 1) / nchunks);
-    nsplit = std::min<long long>(nsplit, std::max(1, K / 16));
+    nsplit = std::min<long long>(nsplit, std::max(1, K / 8));
     nsplit = std::max<long long>(nsplit, (K + GS_GRAIN_CHUNK - 1) / GS_GRAIN_CHUNK);
     const int gsplit = (int)((K + nsplit - 1) / nsplit);
     nsplit = (K + gsplit - 1) / gsplit;
