@@ -89,7 +89,6 @@ SIGNATURES = {
 
 SCORE_CERTIFIED = 0
 SCORE_EXACT = 1
-SCORE_CERTIFIED_OUTLINE = 2
 
 _lib = None
 

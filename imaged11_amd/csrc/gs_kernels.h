@@ -122,16 +122,7 @@ __device__ __forceinline__ double gs_wave_max(double v) {
     return v;
 }
 
-// the exact decision for one undecided pair, kept out of line so that the
-// fast loop does not hold its values live for this rare path
-__device__ __noinline__ int gs_settle_pair(gs_ubi u, double x, double y, double z, double lo,
-                                           double hi, double tolsq) {
-    const double s = gs_drlv2_fma(u, x, y, z);
-    const bool und = (s < hi) && !(s < lo);
-    return __popcll(__ballot(und && gs_drlv2(u, x, y, z) < tolsq));
-}
-
-template <int P, bool OUTLINE>
+template <int P>
 __global__ __launch_bounds__(GS_BLOCK) void k_score_many_fast(
     const double* __restrict__ ubis, int K, int gsplit, int nsplit,
     const double* __restrict__ gv, int ng, double tolsq, int* __restrict__ counts) {
@@ -188,16 +179,12 @@ __global__ __launch_bounds__(GS_BLOCK) void k_score_many_fast(
             c += __popcll(in);
         }
         if (undecided) {  // wave-uniform and rare: settle undecided lanes exactly
-            if (OUTLINE) {
-                for (int p = 0; p < P; p++) c += gs_settle_pair(u, x[p], y[p], z[p], lo, hi, tolsq);
-            } else {
 #pragma unroll
-                for (int p = 0; p < P; p++) {
-                    const double s = gs_drlv2_fma(u, x[p], y[p], z[p]);
-                    const bool und = (s < hi) && !(s < lo);
-                    const double se = gs_drlv2(u, x[p], y[p], z[p]);
-                    c += __popcll(__ballot(und && se < tolsq));
-                }
+            for (int p = 0; p < P; p++) {
+                const double s = gs_drlv2_fma(u, x[p], y[p], z[p]);
+                const bool und = (s < hi) && !(s < lo);
+                const double se = gs_drlv2(u, x[p], y[p], z[p]);
+                c += __popcll(__ballot(und && se < tolsq));
             }
         }
         if (lane == 0 && c) atomicAdd(&lcnt[g - g0], c);

@@ -234,21 +234,13 @@ int launch_score_many(const double* ubis, int K, const double* gv, int ng, doubl
             case 16: hipLaunchKernelGGL(k_score_many<16>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
             default: hipLaunchKernelGGL(k_score_many<8>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
         }
-    } else if (mode == GS_SCORE_CERTIFIED) {
+    } else {  // GS_SCORE_CERTIFIED
         switch (P) {
-            case 1: hipLaunchKernelGGL((k_score_many_fast<1, false>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
-            case 2: hipLaunchKernelGGL((k_score_many_fast<2, false>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
-            case 4: hipLaunchKernelGGL((k_score_many_fast<4, false>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
-            case 16: hipLaunchKernelGGL((k_score_many_fast<16, false>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
-            default: hipLaunchKernelGGL((k_score_many_fast<8, false>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
-        }
-    } else {  // GS_SCORE_CERTIFIED_OUTLINE
-        switch (P) {
-            case 1: hipLaunchKernelGGL((k_score_many_fast<1, true>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
-            case 2: hipLaunchKernelGGL((k_score_many_fast<2, true>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
-            case 4: hipLaunchKernelGGL((k_score_many_fast<4, true>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
-            case 16: hipLaunchKernelGGL((k_score_many_fast<16, true>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
-            default: hipLaunchKernelGGL((k_score_many_fast<8, true>), grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 1: hipLaunchKernelGGL(k_score_many_fast<1>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 2: hipLaunchKernelGGL(k_score_many_fast<2>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 4: hipLaunchKernelGGL(k_score_many_fast<4>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 16: hipLaunchKernelGGL(k_score_many_fast<16>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            default: hipLaunchKernelGGL(k_score_many_fast<8>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
         }
     }
     GS_LAUNCHED();
@@ -693,7 +685,7 @@ int gs_clear_error(void) {
 
 int gs_score_many(const double* ubis, int K, const double* gv, int ng, double tol, int* counts,
                   int peaks_per_lane, int mode, gs_stream_t stream) {
-    if (mode != GS_SCORE_CERTIFIED && mode != GS_SCORE_EXACT && mode != GS_SCORE_CERTIFIED_OUTLINE)
+    if (mode != GS_SCORE_CERTIFIED && mode != GS_SCORE_EXACT)
         return set_err(GS_ERR_ARG, "gs_score_many: unknown mode %d", mode);
     return launch_score_many(ubis, K, gv, ng, tol, counts, peaks_per_lane, mode, S(stream));
 }

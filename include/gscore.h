@@ -106,7 +106,6 @@ void compute_xlylzl(double s[], double f[], double p[4], double r[9], double dis
  * arithmetic; EXACT uses the reference's arithmetic for every pair. */
 #define GS_SCORE_CERTIFIED 0
 #define GS_SCORE_EXACT 1
-#define GS_SCORE_CERTIFIED_OUTLINE 2 /* CERTIFIED, undecided pairs settled out of line */
 
 /* score for K UBIs (K,3,3) against ng g-vectors -> counts[K] (int32);
  * the loop indexer.score runs over trial UBIs (ImageD11/indexing.py:1090).
