@@ -236,6 +236,7 @@ def aux_index(with_cpu, K=3000, seed=11):
     """Ring-pair indexing (indexer.assigntorings + find + scorethem) of K
     fcc grains' g-vectors on the GPU, against the CPU checker's sequential
     restatement of the same search (one thread) on the same input."""
+    import numpy as np
     import torch
     from imaged11_amd import indexing, simulate, unitcell
     t0 = time.perf_counter()

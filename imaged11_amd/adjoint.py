@@ -104,12 +104,16 @@ class JointProblem:
         """Fix hkl_q = round(UBI_g . G_q) at the given state."""
         lib = _lib.load()
         lib.gs_clear_error()
+        # bound to names so that no argument's memory is recycled by the
+        # allocator before the launch (a temporary's block is free again as
+        # soon as its last reference goes)
         geo = geo.detach().contiguous()
+        ubis = ubis.detach().contiguous()
+        trans = trans.detach().contiguous()
         _lib.check(lib.gs_joint_hkl(dptr(self.sc), dptr(self.fc), dptr(self.so), dptr(self.co),
-                                    self.sc.shape[0], dptr(geo), dptr(ubis.detach().contiguous()),
-                                    dptr(trans.detach().contiguous()), self.K, dptr(self.idx),
-                                    dptr(self.owner), self.idx.numel(), dptr(self.hkl),
-                                    stream_ptr()), "joint hkl")
+                                    self.sc.shape[0], dptr(geo), dptr(ubis), dptr(trans), self.K,
+                                    dptr(self.idx), dptr(self.owner), self.idx.numel(),
+                                    dptr(self.hkl), stream_ptr()), "joint hkl")
 
     def evaluate(self, geo, ubis, trans, gspace=False, want_sums=False, lanes_per_grain=16):
         """One fused forward + reverse pass.  Returns (loss, dL/dUBI (K,3,3),

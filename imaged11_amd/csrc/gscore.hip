@@ -220,8 +220,10 @@ int launch_score_many(const double* ubis, int K, const double* gv, int ng, doubl
     GS_CHECK(fill_i32(counts, 0, K, st));
     if (K == 0 || ng == 0) return GS_OK;
     const double tolsq = tol * tol;
+    // default tile height: 4 peaks per lane for the certified kernel (fewer
+    // VGPRs, more resident waves), 8 for the exact one (measured best)
     int P = peaks_per_lane;
-    if (P != 1 && P != 2 && P != 4 && P != 8 && P != 16) P = 8;
+    if (P != 1 && P != 2 && P != 4 && P != 8 && P != 16) P = mode == GS_SCORE_EXACT ? 8 : 4;
     int gsplit, nsplit;
     unsigned nb;
     GS_CHECK(pair_tiles(ctx, K, ng, P, &gsplit, &nsplit, &nb));

@@ -109,7 +109,8 @@ void compute_xlylzl(double s[], double f[], double p[4], double r[9], double dis
 
 /* score for K UBIs (K,3,3) against ng g-vectors -> counts[K] (int32);
  * the loop indexer.score runs over trial UBIs (ImageD11/indexing.py:1090).
- * peaks_per_lane: 1, 2, 4, 8 or 16 (tile height; 0 = default 8). */
+ * peaks_per_lane: 1, 2, 4, 8 or 16 (tile height; 0 = default: 4 certified,
+ * 8 exact). */
 int gs_score_many(const double *ubis, int K, const double *gv, int ng, double tol,
                   int *counts, int peaks_per_lane, int mode, gs_stream_t stream);
 

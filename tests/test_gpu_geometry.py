@@ -45,10 +45,12 @@ def test_compute_geometry_given_trig_bitexact(dev, orc, wedge, chi, t, sign):
     mat = orc.wedge_chi_matrix(wedge, chi)
     want = orc.compute_geometry_trig(xl, so, co, mat, 0.2, np.array(t))
     lib = _lib.load()
-    T = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    # hold every input tensor until the kernel has run: a temporary's memory
+    # returns to torch's allocator as soon as its last reference goes
+    dxl, dso, dco = (torch.as_tensor(np.ascontiguousarray(a), device=dev) for a in (xl, so, co))
     out = torch.empty((len(om), 6), dtype=torch.float64, device=dev)
-    _lib.check(lib.gs_compute_geometry_trig_dev(dptr(T(xl)), dptr(T(so)), dptr(T(co)), darr(mat),
-                                                0.2, darr(t), dptr(out), len(om), stream_ptr()),
+    _lib.check(lib.gs_compute_geometry_trig_dev(dptr(dxl), dptr(dso), dptr(dco), darr(mat), 0.2,
+                                                darr(t), dptr(out), len(om), stream_ptr()),
                "geometry_trig")
     got = out.cpu().numpy()
     assert np.array_equal(bits(got[:, 2:]), bits(want[:, 2:]))
