@@ -37,6 +37,7 @@ SIGNATURES = {
     "gs_init": (_i, []),
     "gs_selftest_sincos_host": (None, [_vp, _vp, _vp, _i]),
     "gs_selftest_certified_host": (None, [_vp, _vp, _i, _d, _d, _vp]),
+    "gs_selftest_certified_fract_host": (None, [_vp, _vp, _i, _d, _d, _vp]),
     "gs_selftest_scan_host": (None, [_vp, _vp, _vp, _vp, _vp, _i, _vp, _d, _d, _vp]),
     "verify_rounding": (_i, [_i]),
     "score": (_i, [_vp, _vp, _d, _i]),

@@ -81,6 +81,51 @@ GS_HD __forceinline__ double gs_cert_window(const gs_ubi& u, double gmax, double
     return 1e-15 * span + 1e-15 + 2e-15 * tolsq;
 }
 
+// x - floor(x) in [0, 1): v_fract_f64 on the device (exact for |x| < 2^52),
+// the same value on the host
+GS_HD __forceinline__ double gs_fract(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_fract(x);
+#else
+    return x - floor(x);
+#endif
+}
+
+// The score kernel's evaluation, one instruction per pair cheaper than
+// gs_drlv2_fma: the rows are formed as h + 1/2 (the 1/2 enters the first
+// FMA of each row), f = fract(h + 1/2) so that f - 1/2 is the distance of
+// h to its nearest integer (half-up on ties, where the square is the same),
+// and sum f (f - 1) = sum (f - 1/2)^2 - 3/4 = drlv2 - 3/4 in exact
+// arithmetic, 5 instructions for the three rows.  Compared against
+// tol^2 - 3/4 +- gs_cert_window_fract.
+GS_HD __forceinline__ double gs_drlv2_fract(const gs_ubi& u, double x, double y, double z) {
+    const double h0 = __builtin_fma(u.u02, z, __builtin_fma(u.u01, y, __builtin_fma(u.u00, x, 0.5)));
+    const double h1 = __builtin_fma(u.u12, z, __builtin_fma(u.u11, y, __builtin_fma(u.u10, x, 0.5)));
+    const double h2 = __builtin_fma(u.u22, z, __builtin_fma(u.u21, y, __builtin_fma(u.u20, x, 0.5)));
+    const double f0 = gs_fract(h0), f1 = gs_fract(h1), f2 = gs_fract(h2);
+    double t = __builtin_fma(f0, f0, -f0);
+    t = __builtin_fma(f1, f1, t) - f1;
+    return __builtin_fma(f2, f2, t) - f2;
+}
+
+// Half-width of the undecided window of gs_drlv2_fract around tol^2 - 3/4.
+// Each row h + 1/2 is within gamma_3 (A_i + 1/2) of exact (A_i = sum_j
+// |u_ij||g_j|, gamma_3 <= 6.7e-16); x -> (x - round x)^2 is 1-Lipschitz,
+// so the rows move the sum by <= 6.7e-16 (U1 Gmax + 3/2); the five rounded
+// operations on terms of magnitude <= 3/4 add <= 5e-16; the reference's
+// own sum of squares <= 5e-16.  E = 1e-15 U1 Gmax + 3e-15 + 2e-15 tol^2
+// covers it with margin, the last terms also absorbing the rounding of the
+// thresholds (tol^2 - 3/4) +- E.  +inf when the bound does not apply
+// (U1 Gmax >= 2^49, where v_rndne/fract and the MAGIC rounding may differ,
+// or a non-finite Gmax).
+GS_HD __forceinline__ double gs_cert_window_fract(const gs_ubi& u, double gmax, double tolsq) {
+    const double u1 = fabs(u.u00) + fabs(u.u01) + fabs(u.u02) + fabs(u.u10) + fabs(u.u11) +
+                      fabs(u.u12) + fabs(u.u20) + fabs(u.u21) + fabs(u.u22);
+    const double span = u1 * gmax;
+    if (!(span < 562949953421312.0)) return __builtin_inf();  // 2^49
+    return 1e-15 * span + 3e-15 + 2e-15 * tolsq;
+}
+
 GS_HD __forceinline__ double gs_abs_finite(double a) {
     return a == a ? fabs(a) : 0.0;  // NaN rows never count on either path
 }

@@ -14,6 +14,7 @@
 
 #define GS_BLOCK 256
 #define GS_GRAIN_CHUNK 1024  // grains per LDS counter tile
+#define GS_SPLIT_MAX 512     // grains per split of the all-pairs score/select tiles
 #define GS_NPART 17          // n, sum drlv2, R[9], H[6]
 
 // ---------------------------------------------------------------------------
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_score_many(const double* __restric
                                                          int gsplit, int nsplit,
                                                          const double* __restrict__ gv, int ng,
                                                          double tolsq, int* __restrict__ counts) {
-    __shared__ int lcnt[GS_GRAIN_CHUNK];
+    __shared__ int lcnt[GS_SPLIT_MAX];
     const int tile = gs_xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = tile / nsplit;
     const int split = tile - chunk * nsplit;
@@ -97,8 +98,12 @@ __global__ __launch_bounds__(GS_BLOCK) void k_score_many(const double* __restric
 // ---------------------------------------------------------------------------
 // Certified fast score: the same counts as k_score_many for every input.
 //
-// h is formed with FMA and rounded with v_rndne (3 + 2 ops per row instead
-// of 5 + 3), the squares summed with FMA.  Let S_e be the sum the
+// The kernel evaluates gs_drlv2_fract (rows h + 1/2 by FMA, fract, and
+// sum f (f - 1) = drlv2 - 3/4: 19 instructions per pair with the two
+// compares) against tol^2 - 3/4 +- gs_cert_window_fract; the bound below
+// is stated for the plain FMA form gs_drlv2_fma, which the certified assign
+// kernel uses, and carries over with the extra 1/2 per row (see
+// gs_cert_window_fract).  Let S_e be the sum the
 // reference computes (separately rounded products, MAGIC rounding) and S_f
 // this one.  With A_i = sum_j |u_ij| |g_j|, each h differs from the exact
 // dot product by at most gamma_3 A_i (gamma_3 = 3u/(1-3u), u = 2^-53), so
@@ -122,12 +127,13 @@ __device__ __forceinline__ double gs_wave_max(double v) {
     return v;
 }
 
+// At 4 peaks per lane the kernel fits 64 VGPRs: 8 waves per SIMD.
 template <int P>
-__global__ __launch_bounds__(GS_BLOCK) void k_score_many_fast(
+__global__ __launch_bounds__(GS_BLOCK, P <= 4 ? 8 : 1) void k_score_many_fast(
     const double* __restrict__ ubis, int K, int gsplit, int nsplit,
     const double* __restrict__ gv, int ng, double tolsq, int* __restrict__ counts) {
-    __shared__ int lcnt[GS_GRAIN_CHUNK];
-    __shared__ double lwin[2 * GS_GRAIN_CHUNK];  // (lo, hi) per grain of the split
+    __shared__ int lcnt[GS_SPLIT_MAX];
+    __shared__ double lwin[2 * GS_SPLIT_MAX];  // (lo, hi) per grain of the split
     __shared__ double lgm[GS_BLOCK / GS_WAVE];
     const int tile = gs_xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = tile / nsplit;
@@ -158,10 +164,12 @@ __global__ __launch_bounds__(GS_BLOCK) void k_score_many_fast(
     double gmax = lgm[0];
 #pragma unroll
     for (int w = 1; w < GS_BLOCK / GS_WAVE; w++) gmax = fmax(gmax, lgm[w]);
+    const double mid = tolsq - 0.75;  // gs_drlv2_fract is drlv2 - 3/4
     for (int i = threadIdx.x; i < g1 - g0; i += GS_BLOCK) {
-        const double e = gs_cert_window(gs_load_ubi(ubis + 9 * (size_t)(g0 + i)), gmax, tolsq);
-        lwin[2 * i] = tolsq - e;
-        lwin[2 * i + 1] = tolsq + e;
+        const double e =
+            gs_cert_window_fract(gs_load_ubi(ubis + 9 * (size_t)(g0 + i)), gmax, tolsq);
+        lwin[2 * i] = mid - e;
+        lwin[2 * i + 1] = mid + e;
     }
     __syncthreads();
     gs_ubi un = gs_load_ubi(ubis + 9 * (size_t)g0);
@@ -173,7 +181,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_score_many_fast(
         unsigned long long undecided = 0;
 #pragma unroll
         for (int p = 0; p < P; p++) {
-            const double s = gs_drlv2_fma(u, x[p], y[p], z[p]);
+            const double s = gs_drlv2_fract(u, x[p], y[p], z[p]);
             const unsigned long long in = __ballot(s < lo);
             undecided |= __ballot(s < hi) ^ in;
             c += __popcll(in);
@@ -181,7 +189,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_score_many_fast(
         if (undecided) {  // wave-uniform and rare: settle undecided lanes exactly
 #pragma unroll
             for (int p = 0; p < P; p++) {
-                const double s = gs_drlv2_fma(u, x[p], y[p], z[p]);
+                const double s = gs_drlv2_fract(u, x[p], y[p], z[p]);
                 const bool und = (s < hi) && !(s < lo);
                 const double se = gs_drlv2(u, x[p], y[p], z[p]);
                 c += __popcll(__ballot(und && se < tolsq));
@@ -267,7 +275,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_assign_many(
 }
 
 // The same loop with the certified decision of k_score_many_fast: a pair
-// whose FMA drlv2 is >= tol^2 + E certainly fails `drlv2 < tol^2`, so only
+// whose fast drlv2 - 3/4 is >= tol^2 - 3/4 + E certainly fails
+// `drlv2 < tol^2`, so only
 // the reference's else branch (label == L -> -1) can apply; every other
 // pair is recomputed with the reference arithmetic and the reference's
 // rule is applied to that value, so labels, stored drlv2 and counts are
@@ -278,7 +287,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_assign_many_fast(
     const double* __restrict__ gv, int ng, double tolsq, double* __restrict__ drlv2,
     int* __restrict__ labels, int* __restrict__ counts) {
     __shared__ int lcnt[GS_GRAIN_CHUNK];
-    __shared__ double lhi[GS_GRAIN_CHUNK];  // tol^2 + E per grain of the chunk
+    __shared__ double lhi[GS_GRAIN_CHUNK];  // tol^2 - 3/4 + E per grain of the chunk
     __shared__ double lgm[GS_BLOCK / GS_WAVE];
     double x[P], y[P], z[P], d[P];
     int lab[P];
@@ -311,7 +320,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_assign_many_fast(
         const int c1 = min(K, c0 + GS_GRAIN_CHUNK);
         for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) {
             lcnt[i] = 0;
-            lhi[i] = tolsq + gs_cert_window(gs_load_ubi(ubis + 9 * (size_t)(c0 + i)), gmax, tolsq);
+            lhi[i] = (tolsq - 0.75) +
+                     gs_cert_window_fract(gs_load_ubi(ubis + 9 * (size_t)(c0 + i)), gmax, tolsq);
         }
         __syncthreads();
         for (int g = c0; g < c1; g++) {
@@ -321,7 +331,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_assign_many_fast(
             int c = 0;
 #pragma unroll
             for (int p = 0; p < P; p++) {
-                const double sf = gs_drlv2_fma(u, x[p], y[p], z[p]);
+                const double sf = gs_drlv2_fract(u, x[p], y[p], z[p]);  // drlv2 - 3/4
                 bool take = false;
                 if (!(sf >= hi)) {  // candidate (or non-finite): reference arithmetic
                     const double s = gs_drlv2(u, x[p], y[p], z[p]);

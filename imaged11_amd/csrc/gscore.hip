@@ -191,7 +191,7 @@ int check_ng(long long ng) {
 // ---- launchers shared by both families ------------------------------------
 
 // Tiles of the all-pairs kernels: chunks of GS_BLOCK*P peaks times splits of
-// at most GS_GRAIN_CHUNK grains (the LDS tile).  About 128 tiles per CU:
+// at most GS_SPLIT_MAX grains (the LDS tile).  About 128 tiles per CU:
 // with 4 resident blocks per CU that is ~32 rounds, so the partly filled
 // last round costs ~3% (at 32 per CU it cost ~10%), while a split keeps
 // at least 8 grains to amortise the tile's peak loads and window setup.
@@ -201,7 +201,7 @@ int pair_tiles(DeviceCtx* ctx, int K, int ng, int P, int* gsplit_out, int* nspli
     const long long want = 128LL * ctx->cus;
     long long nsplit = std::max<long long>(1, (want + nchunks - 1) / nchunks);
     nsplit = std::min<long long>(nsplit, std::max(1, K / 8));
-    nsplit = std::max<long long>(nsplit, (K + GS_GRAIN_CHUNK - 1) / GS_GRAIN_CHUNK);
+    nsplit = std::max<long long>(nsplit, (K + GS_SPLIT_MAX - 1) / GS_SPLIT_MAX);
     const int gsplit = (int)((K + nsplit - 1) / nsplit);
     nsplit = (K + gsplit - 1) / gsplit;
     const long long nb = nchunks * nsplit;
@@ -636,6 +636,22 @@ void gs_selftest_sincos_host(const double* x, double* s, double* c, int n) {
 // of gv, out[k] = 1 if `drlv2 < tol^2` is certain, 0 if certainly false,
 // 2 if undecided, plus 4 when the reference arithmetic says true.  gmax is
 // the bound on |gv| the window is computed for.  Checking aid only.
+void gs_selftest_certified_fract_host(const double* ubi, const double* gv, int n, double tol,
+                                      double gmax, int* out) {
+    const gs_ubi u = gs_load_ubi(ubi);
+    const double tolsq = tol * tol;
+    const double e = gs_cert_window_fract(u, gmax, tolsq);
+    const double mid = tolsq - 0.75;
+    const double lo = mid - e, hi = mid + e;
+    for (int k = 0; k < n; k++) {
+        const double* g = gv + 3 * (size_t)k;
+        double s = gs_drlv2_fract(u, g[0], g[1], g[2]);
+        int d = s < lo ? 1 : (s < hi ? 2 : 0);
+        if (gs_drlv2(u, g[0], g[1], g[2]) < tolsq) d |= 4;
+        out[k] = d;
+    }
+}
+
 void gs_selftest_certified_host(const double* ubi, const double* gv, int n, double tol,
                                 double gmax, int* out) {
     const gs_ubi u = gs_load_ubi(ubi);
@@ -831,9 +847,12 @@ int gs_graph_launch(gs_graph_t g, gs_stream_t stream) {
 
 int gs_graph_destroy(gs_graph_t g) {
     if (!g) return GS_OK;
-    if (g->exec) hipGraphExecDestroy(g->exec);
-    if (g->graph) hipGraphDestroy(g->graph);
+    hipError_t e1 = g->exec ? hipGraphExecDestroy(g->exec) : hipSuccess;
+    hipError_t e2 = g->graph ? hipGraphDestroy(g->graph) : hipSuccess;
     delete g;
+    if (e1 != hipSuccess || e2 != hipSuccess)
+        return set_err(GS_ERR_HIP, "gs_graph_destroy: %s",
+                       hipGetErrorString(e1 != hipSuccess ? e1 : e2));
     return GS_OK;
 }
 

@@ -55,6 +55,10 @@ void gs_selftest_sincos_host(const double *x, double *s, double *c, int n);
  * reference arithmetic gives true.  No other entry point calls it. */
 void gs_selftest_certified_host(const double *ubi, const double *gv, int n, double tol,
                                 double gmax, int *out);
+/* the same for the score kernel's fract form (drlv2 - 3/4 against
+ * tol^2 - 3/4 +- its window). */
+void gs_selftest_certified_fract_host(const double *ubi, const double *gv, int n, double tol,
+                                      double gmax, int *out);
 
 /* The fast-path decision of gs_gv_assign_many evaluated on the host for
  * single pairs (row k: peak xl/so/co, UBI ubis[9k..], translation

@@ -150,19 +150,21 @@ def test_simulator_invariants():
     assert gv.shape == (20000, 3) and lat.sum() == 12000
 
 
+@pytest.mark.parametrize("form", ["fma", "fract"])
 @pytest.mark.parametrize("tol,hmax", [(0.05, 3), (0.05, 40), (0.01, 10), (0.3, 200),
                                       (0.5, 5), (0.8, 20), (1e-4, 2)])
-def test_certified_decision_is_sound(tol, hmax):
-    """Whenever the certified FMA path decides, it decides as the
-    reference arithmetic does; undecided pairs are the rare exception."""
+def test_certified_decision_is_sound(tol, hmax, form):
+    """Whenever the certified path decides, it decides as the reference
+    arithmetic does; undecided pairs are the rare exception.  fma: the
+    assign kernel's form; fract: the score kernel's (drlv2 - 3/4)."""
     lib = _lib.load()
+    fn = lib.gs_selftest_certified_host if form == "fma" else lib.gs_selftest_certified_fract_host
     total_amb = 0
     for seed in range(4):
         ubi, gv = boundary_pairs(seed, 40000, tol, hmax)
         gmax = np.abs(gv).max()
         out = np.zeros(len(gv), np.int32)
-        lib.gs_selftest_certified_host(_lib.hptr(ubi), _lib.hptr(gv), len(gv), tol, gmax,
-                                       _lib.hptr(out))
+        fn(_lib.hptr(ubi), _lib.hptr(gv), len(gv), tol, gmax, _lib.hptr(out))
         dec, exact = out & 3, (out & 4) > 0
         decided = dec != 2
         assert np.array_equal(dec[decided] == 1, exact[decided])
@@ -174,16 +176,23 @@ def test_certified_decision_is_sound(tol, hmax):
         total_amb += int((~decided).sum())
 
 
-def test_certified_window_special():
+@pytest.mark.parametrize("form", ["fma", "fract"])
+def test_certified_window_special(form):
     lib = _lib.load()
+    fn = lib.gs_selftest_certified_host if form == "fma" else lib.gs_selftest_certified_fract_host
     ubi = np.eye(3)
     gv = np.array([[np.nan, 0, 0], [0.5, 0.5, 0.5], [2.0 ** 51 + 0.5, 0, 0], [0.01, 0, 0]])
     out = np.zeros(4, np.int32)
     # huge gmax -> everything undecided (exact path)
-    lib.gs_selftest_certified_host(_lib.hptr(ubi), _lib.hptr(gv), 4, 0.1, 2.0 ** 52, _lib.hptr(out))
+    fn(_lib.hptr(ubi), _lib.hptr(gv), 4, 0.1, 2.0 ** 52, _lib.hptr(out))
     assert (out[1:] & 3 == 2).all() and out[0] == 0
-    lib.gs_selftest_certified_host(_lib.hptr(ubi), _lib.hptr(gv), 4, 0.1, np.inf, _lib.hptr(out))
+    fn(_lib.hptr(ubi), _lib.hptr(gv), 4, 0.1, np.inf, _lib.hptr(out))
     assert (out[1:] & 3 == 2).all()
+    # ties at half integers: (h - round h)^2 = 1/4 either way
+    g = np.array([[0.5, 0.0, 0.0], [1.5, 2.5, -0.5], [0.25, 0.0, 0.0]])
+    out = np.zeros(3, np.int32)
+    fn(_lib.hptr(ubi), _lib.hptr(g), 3, 0.5 + 1e-9, 10.0, _lib.hptr(out))
+    assert out[0] == 1 + 4 and out[2] == 1 + 4 and out[1] & 4 == 0
 
 
 @pytest.mark.parametrize("tol,hmax", [(0.05, 2), (0.05, 30), (0.25, 10), (0.01, 3)])
