@@ -31,10 +31,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOPS_PER_PAIR = 23      # h = UBI.g: 9 mul + 6 add; 3 sub; sumsq: 3 mul + 2 add
-# FP64 VALU instructions issued per pair by the certified kernel: h with
-# mul + 2 FMA per row (9), round + subtract per row (6), sum of squares
-# mul + 2 FMA (3), two compares (2)
-VALU_OPS_PER_PAIR = 20
+# FP64 VALU instructions issued per pair by the certified kernel: rows
+# h + 1/2 with 3 FMA each (9), fract per row (3), sum f (f - 1) in 5, two
+# compares (2)
+VALU_OPS_PER_PAIR = 19
 PEAK_FP64_VALU = 78.6e12  # MI355X FP64 vector, FMA counted as 2
 PEAK_HBM = 8.0e12
 
