@@ -14,14 +14,16 @@
 // Certification.  Both the reference's G and the fast G are within
 // 90 u ds (u = 2^-53, ds = 1/wavelength) of the exact G in every component
 // when |t| <= |xl|/2 (so a - t and xl - o cannot cancel): k0 = ds (d0/|d|
-// - 1) is the only subtraction of like quantities and |d0/|d|| <= 1, and
-// everything else is a few rounded products of magnitude <= 2 ds.  h =
-// UBI.G then moves by at most ||u_i||_1 (90 u ds) + 6.7e-16 A_i per row,
-// and drlv2 by the sum of that
-// over rows (1-Lipschitz, see k_score_many_fast) plus 5e-16 from the sums
-// of squares.  E = U1 (1e-13 ds + 1e-15 Gmax) + 1e-15 + 2e-15 tol^2 bounds
-// it with margin (Gmax = 2 ds bounds |G|).  A pair whose fast drlv2 is
-// >= tol^2 + E certainly fails `drlv2 < tol^2`, so its only effect is the
+// - 1) is the only subtraction of like quantities and |d0/|d|| <= 1, the
+// fast path's 1/|a - t| (gs_rsqrt_cert) is within ~6u, and everything else
+// is a few rounded products of magnitude <= 2 ds.  h = UBI.G then moves by
+// at most ||u_i||_1 (90 u ds) + 6.7e-16 (A_i + 1/2) per row (the fast path
+// uses the fract form of the score kernel, drlv2 - 3/4, whose rows carry
+// an extra 1/2), and drlv2 by the sum of that over rows (1-Lipschitz, see
+// k_score_many_fast) plus 5e-16 from each sum of squares.  E = U1 (1e-13
+// ds + 1e-15 Gmax) + 3e-15 + 2e-15 tol^2 bounds it with margin (Gmax = 2
+// ds bounds |G|).  A pair whose fast drlv2 - 3/4 is >= tol^2 - 3/4 + E
+// certainly fails `drlv2 < tol^2`, so its only effect is the
 // reference's else branch (label == this label -> -1), which needs no
 // arithmetic.  Every other pair is recomputed with the reference's own
 // compute_gv and score arithmetic (gs_gv_one, gs_drlv2) and the reference's
@@ -47,28 +49,57 @@ GS_HD __forceinline__ void gs_scan_consts(double x0, double x1, double x2, doubl
     c[2] = ds * mat[2];
 }
 
-// fast drlv2 of one pair: G = ds (a - t)/|a - t| - c, then gs_drlv2_fma
+// 1/sqrt(x) to within a few ulp, for the certified fast path only: the
+// hardware estimate (v_rsq_f64) and two Newton steps y <- y + y/2 (1 - x
+// y^2).  From any start with relative error e <= 2^-14 the steps give
+// 1.5 e^2 + ~3u each, so the result is within ~6u of 1/sqrt(x) however
+// coarse the estimate (the hardware's is far better).  x = 0 or non-finite
+// gives NaN, which the callers send down the exact route.  On the host the
+// estimate is the float rsqrt of the value (a 24-bit start), so the host
+// self-test exercises the same steps from a worse start.
+GS_HD __forceinline__ double gs_rsqrt_cert(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    double y = __builtin_amdgcn_rsq(x);
+#else
+    double y = (double)(1.0f / sqrtf((float)x));
+    if (!(x > 0.0) || x > 1e300 || x < 1e-300) y = __builtin_nan("");
+#endif
+    for (int it = 0; it < 2; it++) {
+        const double r = __builtin_fma(-(x * y), y, 1.0);
+        y = __builtin_fma(0.5 * y, r, y);
+    }
+    return y;
+}
+
+// fast drlv2 - 3/4 of one pair: G = ds (a - t)/|a - t| - c, then the fract
+// form of the score kernel (gs_drlv2_fract)
 GS_HD __forceinline__ double gs_scan_fast_drlv2(const gs_ubi& u, const double* a,
                                                 const double* c, double t0, double t1, double t2,
                                                 double ds) {
     const double e0 = a[0] - t0, e1 = a[1] - t1, e2 = a[2] - t2;
-    const double inv = ds / sqrt(__builtin_fma(e2, e2, __builtin_fma(e1, e1, e0 * e0)));
+    const double inv = ds * gs_rsqrt_cert(__builtin_fma(e2, e2, __builtin_fma(e1, e1, e0 * e0)));
     const double G0 = __builtin_fma(e0, inv, -c[0]);
     const double G1 = __builtin_fma(e1, inv, -c[1]);
     const double G2 = __builtin_fma(e2, inv, -c[2]);
-    return gs_drlv2_fma(u, G0, G1, G2);
+    return gs_drlv2_fract(u, G0, G1, G2);
 }
 
-// half-width of the undecided window for a grain (UBI u, |t|^2 = tt) and
-// peaks with |xl|^2 >= n2min; +inf where the bound does not apply
+// half-width of the undecided window of gs_scan_fast_drlv2 around
+// tol^2 - 3/4 for a grain (UBI u, |t|^2 = tt) and peaks with |xl|^2 >=
+// n2min; +inf where the bound does not apply.  The G term is as for the
+// reference's G (90 u ds per component, inverse square root included); the
+// fract form adds the per-row 1/2 and its five roundings (3e-15, see
+// gs_cert_window_fract).
 GS_HD __forceinline__ double gs_scan_window(const gs_ubi& u, double ds, double tolsq, double tt,
                                             double n2min) {
     const double gmax = 2.0 * fabs(ds);
     const double u1 = fabs(u.u00) + fabs(u.u01) + fabs(u.u02) + fabs(u.u10) + fabs(u.u11) +
                       fabs(u.u12) + fabs(u.u20) + fabs(u.u21) + fabs(u.u22);
     if (!(u1 * gmax < 562949953421312.0) || !(4.0 * tt <= n2min)) return __builtin_inf();
-    return u1 * (1e-13 * fabs(ds) + 1e-15 * gmax) + 1e-15 + 2e-15 * tolsq;
+    return u1 * (1e-13 * fabs(ds) + 1e-15 * gmax) + 3e-15 + 2e-15 * tolsq;
 }
+
+#define GS_SCAN_CHUNK 512  // grains per LDS tile of k_gv_assign_many
 
 template <int P>
 __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
@@ -77,7 +108,9 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
     const double* __restrict__ ubis, const double* __restrict__ trans,
     const int* __restrict__ glabels, int K, double tolsq, double* __restrict__ drlv2,
     int* __restrict__ labels, int* __restrict__ counts) {
-    __shared__ int lcnt[GS_GRAIN_CHUNK];
+    __shared__ int lcnt[GS_SCAN_CHUNK];
+    __shared__ double lgr[4 * GS_SCAN_CHUNK];  // t0, t1, t2, hi per grain of the chunk
+    __shared__ double ln2[GS_BLOCK / GS_WAVE];
     __shared__ double smat[9];
     __shared__ double sds;
     gs_block_setup(gp, smat, &sds);
@@ -112,19 +145,31 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
             n2min = n2 == n2 ? fmin(n2min, n2) : n2min;
         }
     }
-    for (int o = GS_WAVE / 2; o > 0; o >>= 1) n2min = fmin(n2min, __shfl_xor(n2min, o, GS_WAVE));
     const int lane = threadIdx.x & (GS_WAVE - 1);
-    for (int c0 = 0; c0 < K; c0 += GS_GRAIN_CHUNK) {
-        const int c1 = min(K, c0 + GS_GRAIN_CHUNK);
-        for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) lcnt[i] = 0;
+    for (int o = GS_WAVE / 2; o > 0; o >>= 1) n2min = fmin(n2min, __shfl_xor(n2min, o, GS_WAVE));
+    if (lane == 0) ln2[threadIdx.x / GS_WAVE] = n2min;
+    __syncthreads();
+    for (int w = 0; w < GS_BLOCK / GS_WAVE; w++) n2min = fmin(n2min, ln2[w]);
+    const double mid = tolsq - 0.75;  // gs_scan_fast_drlv2 is drlv2 - 3/4
+    for (int c0 = 0; c0 < K; c0 += GS_SCAN_CHUNK) {
+        const int c1 = min(K, c0 + GS_SCAN_CHUNK);
+        // per grain of the chunk, once per block: translation and window
+        for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) {
+            const size_t g = (size_t)(c0 + i);
+            const double t0 = trans[3 * g], t1 = trans[3 * g + 1], t2 = trans[3 * g + 2];
+            const double tt = t0 * t0 + t1 * t1 + t2 * t2;
+            lgr[4 * i] = t0;
+            lgr[4 * i + 1] = t1;
+            lgr[4 * i + 2] = t2;
+            lgr[4 * i + 3] = mid + gs_scan_window(gs_load_ubi(ubis + 9 * g), ds, tolsq, tt, n2min);
+            lcnt[i] = 0;
+        }
         __syncthreads();
         for (int g = c0; g < c1; g++) {
             const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)g);
-            const double t0 = trans[3 * (size_t)g], t1 = trans[3 * (size_t)g + 1],
-                         t2 = trans[3 * (size_t)g + 2];
             const int L = glabels[g];
-            const double tt = t0 * t0 + t1 * t1 + t2 * t2;
-            const double hi = tolsq + gs_scan_window(u, ds, tolsq, tt, n2min);
+            const double t0 = lgr[4 * (g - c0)], t1 = lgr[4 * (g - c0) + 1],
+                         t2 = lgr[4 * (g - c0) + 2], hi = lgr[4 * (g - c0) + 3];
             int cnt = 0;
 #pragma unroll
             for (int p = 0; p < P; p++) {
@@ -138,7 +183,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
                     if (k < ng) {
                         double g0, g1, g2;
                         gs_gv_one(xl[3 * k], xl[3 * k + 1], xl[3 * k + 2], so_in[k], co_in[k],
-                                  mat, ds, t0, t1, t2, g0, g1, g2);
+                                  smat, ds, t0, t1, t2, g0, g1, g2);
                         const double se = gs_drlv2(u, g0, g1, g2);
                         if ((se < tolsq) && (se < d[p])) {
                             take = true;

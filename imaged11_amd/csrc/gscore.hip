@@ -686,8 +686,8 @@ void gs_selftest_scan_host(const double* xl, const double* so, const double* co,
         gs_scan_consts(x[0], x[1], x[2], so[k], co[k], mat, ds, a, c);
         const double n2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
         const double tt = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
-        const double hi = tolsq + gs_scan_window(u, ds, tolsq, tt, n2);
-        const double sf = gs_scan_fast_drlv2(u, a, c, t[0], t[1], t[2], ds);
+        const double hi = (tolsq - 0.75) + gs_scan_window(u, ds, tolsq, tt, n2);
+        const double sf = gs_scan_fast_drlv2(u, a, c, t[0], t[1], t[2], ds);  // drlv2 - 3/4
         int dcs = (sf >= hi) ? 1 : 2;
         double g0, g1, g2;
         gs_gv_one(x[0], x[1], x[2], so[k], co[k], mat, ds, t[0], t[1], t[2], g0, g1, g2);
