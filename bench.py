@@ -35,6 +35,11 @@ FLOPS_PER_PAIR = 23      # h = UBI.g: 9 mul + 6 add; 3 sub; sumsq: 3 mul + 2 add
 # h + 1/2 with 3 FMA each (9), fract per row (3), sum f (f - 1) in 5, two
 # compares (2)
 VALU_OPS_PER_PAIR = 19
+# FP64 VALU instructions per pair of the fused compute_gv + score_and_assign
+# kernel's certified loop, counted in its gfx950 assembly: 77 per grain step
+# of two peaks per lane (24 v_fmac, 18 v_fma, 15 v_mul, 10 v_add, 6
+# v_fract, 2 v_rsq, 2 v_cmp), plus about 6 integer VALU for the label state
+SCAN_VALU_OPS_PER_PAIR = 38.5
 PEAK_FP64_VALU = 78.6e12  # MI355X FP64 vector, FMA counted as 2
 PEAK_HBM = 8.0e12
 
@@ -181,7 +186,10 @@ def aux_refine_adjoint(threads, with_cpu):
            "config": best, "batched_ms": batched_ms, "per_grain_gpu_loop_ms": loop_ms,
            "speedup_vs_gpu_loop": loop_ms / batched_ms,
            "refine_iterations_per_s": n_iter / (batched_ms * 1e-3),
-           "pairs_per_s": M * n_iter / (batched_ms * 1e-3)}
+           "pairs_per_s": M * n_iter / (batched_ms * 1e-3),
+           # gathered refine: 28 B per list entry (int32 index + 24 B g) and
+           # 156 B per grain (UBI in and out, n, mean) per iteration
+           "algorithmic_hbm_gb_s": (28 * M + 156 * K) * n_iter / (batched_ms * 1e-3) / 1e9}
     if with_cpu:
         from concurrent.futures import ThreadPoolExecutor
         from oracle import oracle
@@ -314,8 +322,16 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
     refine_ms = _events_ms(lambda: (work.copy_(ubis), g.replay()), reps=3)
     own = pk["grain"]
     lab = labels.cpu().numpy()
+    # per peak: xl 24 B + sin/cos 16 B + (drlv2, label) read 12 B and
+    # written 12 B; per grain: UBI 72 B + translation 24 B + label 4 B +
+    # count 4 B
+    algo_bytes = 64 * N + 104 * K
     out = {"grains": K, "peaks": N, "gen_s": gen_s, "assign_ms": assign_ms,
            "assign_pairs_per_s": K * N / (assign_ms * 1e-3),
+           "assign_algorithmic_hbm_gb_s": algo_bytes / (assign_ms * 1e-3) / 1e9,
+           "assign_valu_ops_per_pair": SCAN_VALU_OPS_PER_PAIR,
+           "assign_issued_fp64_ops_frac_of_nonfma_peak":
+               SCAN_VALU_OPS_PER_PAIR * K * N / (assign_ms * 1e-3) / 39.3e12,
            "assigned": int((lab >= 0).sum()),
            "own_label_fraction": float(np.mean(lab[own >= 0] == own[own >= 0])),
            "refine_iterations": n_iter, "refine_list_entries": int(gvl.shape[0]),
