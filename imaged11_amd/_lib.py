@@ -86,6 +86,10 @@ SIGNATURES = {
     "gs_compute_xlylzl_xpos_dev": (_i, [_vp, _vp, _dp, _dp, _dp, _vp, _vp, _i, _vp]),
     "gs_sf2geometry": (_i, [_vp, _vp, _vp, _dp, _dp, _dp, _d, _d, _d, _d, _dp, _vp, _vp, _i,
                             _vp]),
+    "gs_grain_gv": (_i, [_vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _vp, _vp, _i, _dp, _dp, _d, _vp,
+                         _vp, ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp]),
+    "gs_grain_gv_host": (_i, [_vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _vp, _vp, _i, _dp, _dp, _d,
+                              _vp, _vp, ctypes.c_longlong, _vp, _vp, _vp, _vp]),
 }
 
 SCORE_CERTIFIED = 0

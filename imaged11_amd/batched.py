@@ -259,3 +259,44 @@ def gv_lists(xl, so, co, wvln, trans, idx, owner, wedge=0.0, chi=0.0, mat=None):
                                float(chi), _mat_arg(mat), dptr(trans), trans.shape[0], dptr(idx),
                                dptr(owner), idx.numel(), dptr(out), stream_ptr()), "gv_lists")
     return out
+
+
+def grain_gv(xl, so, co, omega, pars, trans, idx, owner, mats=None, slop=0.25, angles=False):
+    """refinegrains.compute_gv (``ImageD11/refinegrains.py:366-456``) for
+    every list entry at once: g-vector of peak idx[q] seen from request
+    owner[q] (translation trans[owner]).  With ``mats`` (R,3,3), the
+    omega-float branch with those fitted UBIs and slop.  Returns (gv (M,3),
+    tth, eta, omega_calc), the last three None unless ``angles``
+    (omega_calc also None without mats)."""
+    lib = _ready()
+    _need(xl, torch.float64, "xlylzl", (None, 3))
+    N = xl.shape[0]
+    _need(so, torch.float64, "so", (N,))
+    _need(co, torch.float64, "co", (N,))
+    _need(trans, torch.float64, "trans", (None, 3))
+    R = trans.shape[0]
+    _need(idx, torch.int32, "idx", (None,))
+    M = idx.numel()
+    _need(owner, torch.int32, "owner", (M,))
+    sign, wvln, wedge, chi = geometry.pack_gv(pars)
+    rb = pre = None
+    if mats is not None:
+        _need(mats, torch.float64, "mats", (R, 3, 3))
+        _need(omega, torch.float64, "omega", (N,))
+        rbv, prev = geometry.uncompute_vectors(wvln, wedge, chi)
+        rb = darr(rbv)
+        pre = None if prev is None else darr(prev.ravel())
+    dev = xl.device
+    gv = torch.empty((M, 3), dtype=torch.float64, device=dev)
+    tth = eta = omc = None
+    if angles:
+        tth = torch.empty(M, dtype=torch.float64, device=dev)
+        eta = torch.empty(M, dtype=torch.float64, device=dev)
+        if mats is not None:
+            omc = torch.empty(M, dtype=torch.float64, device=dev)
+    opt = lambda t: None if t is None else dptr(t)
+    _lib.check(lib.gs_grain_gv(dptr(xl), dptr(so), dptr(co), opt(omega), N, float(sign),
+                               float(wvln), float(wedge), float(chi), dptr(trans), opt(mats), R,
+                               rb, pre, float(slop), dptr(idx), dptr(owner), M, opt(tth),
+                               opt(eta), opt(omc), dptr(gv), stream_ptr()), "grain_gv")
+    return gv, tth, eta, omc

@@ -239,3 +239,17 @@ class newcolumnfile(columnfile):
         self.__dict__["titles"] = list(titles)
         self.__dict__["_cols"] = [np.zeros(0) for _ in titles]
         self.ncols = len(titles)
+
+
+def colfile_from_dict(c):
+    """A column file holding the equal-length arrays of a dict, titles in
+    the dict's order (``ImageD11/columnfile.py:546-555``)."""
+    titles = list(c.keys())
+    nrows = len(c[titles[0]])
+    for t in titles:
+        if len(c[t]) != nrows:
+            raise ValueError("column %s has %d rows, not %d" % (t, len(c[t]), nrows))
+    colf = newcolumnfile(titles)
+    colf.nrows = nrows
+    colf.__dict__["_cols"] = [np.ascontiguousarray(c[t], dtype=np.float64) for t in titles]
+    return colf

@@ -26,6 +26,7 @@
 #include "gs_assign_gv.h"
 #include "gs_index.h"
 #include "gs_geometry.h"
+#include "gs_grains.h"
 #include "../../include/gscore.h"
 
 namespace {
@@ -948,6 +949,73 @@ int gs_gv_lists(const double* xlylzl, const double* so, const double* co, int ng
     hipLaunchKernelGGL(k_gv_lists, dim3(blocks_for(n_idx, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
                        xlylzl, so, co, ng, gp, trans, K, idx, owner, n_idx, out);
     GS_LAUNCHED();
+    return GS_OK;
+}
+
+// ---- refinegrains driver -------------------------------------------------------
+
+int gs_grain_gv(const double* xlylzl, const double* so, const double* co, const double* omega,
+                int ng, double omegasign, double wvln, double wedge, double chi,
+                const double* trans, const double* mats, int R, const double* rb,
+                const double* pre, double slop, const int* idx, const int* owner, long long M,
+                double* tth, double* eta, double* omega_calc, double* gv, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (R < 0 || ng < 0 || M < 0) return set_err(GS_ERR_ARG, "gs_grain_gv: negative size");
+    if (M == 0) return GS_OK;
+    if (!xlylzl || !so || !co || !trans || !idx || !owner || !gv)
+        return set_err(GS_ERR_ARG, "gs_grain_gv: NULL array");
+    if (mats && (!omega || !rb))
+        return set_err(GS_ERR_ARG, "gs_grain_gv: omega-float needs omega and rb");
+    gs_goniometer gp = make_gonio(omegasign, wvln, wedge, chi, nullptr, nullptr);
+    gs_omfloat of;
+    for (int i = 0; i < 3; i++) of.rb[i] = rb ? rb[i] : 0.0;
+    for (int i = 0; i < 9; i++) of.pre[i] = pre ? pre[i] : 0.0;
+    of.has_pre = pre != nullptr;
+    of.slop = slop;
+    const dim3 grid(blocks_for(M, GS_BLOCK));
+    if (mats)
+        hipLaunchKernelGGL(k_grain_gv<1>, grid, dim3(GS_BLOCK), 0, S(stream), xlylzl, so, co, omega,
+                           ng, gp, trans, mats, R, of, idx, owner, M, tth, eta, omega_calc, gv);
+    else
+        hipLaunchKernelGGL(k_grain_gv<0>, grid, dim3(GS_BLOCK), 0, S(stream), xlylzl, so, co, omega,
+                           ng, gp, trans, mats, R, of, idx, owner, M, tth, eta, omega_calc, gv);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+// the kernel's per-entry arithmetic run on the host (no device needed):
+// host arrays, same arguments and outputs as gs_grain_gv
+int gs_grain_gv_host(const double* xlylzl, const double* so, const double* co,
+                     const double* omega, int ng, double omegasign, double wvln, double wedge,
+                     double chi, const double* trans, const double* mats, int R, const double* rb,
+                     const double* pre, double slop, const int* idx, const int* owner, long long M,
+                     double* tth, double* eta, double* omega_calc, double* gv) {
+    if (R < 0 || ng < 0 || M < 0) return set_err(GS_ERR_ARG, "gs_grain_gv_host: negative size");
+    if (mats && (!omega || !rb))
+        return set_err(GS_ERR_ARG, "gs_grain_gv_host: omega-float needs omega and rb");
+    gs_omfloat of;
+    for (int i = 0; i < 3; i++) of.rb[i] = rb ? rb[i] : 0.0;
+    for (int i = 0; i < 9; i++) of.pre[i] = pre ? pre[i] : 0.0;
+    of.has_pre = pre != nullptr;
+    of.slop = slop;
+    double mat[9];
+    gs_wedge_chi_matrix(wedge, chi, mat);
+    const double ds = 1. / wvln;
+    const double nan = std::nan("");
+    for (long long q = 0; q < M; q++) {
+        double out[6] = {nan, nan, nan, nan, nan, nan};
+        const int k = idx[q], r = owner[q];
+        if (k >= 0 && k < ng && r >= 0 && r < R)
+            gs_grain_gv_one(of, mat, ds, xlylzl[3 * (size_t)k], xlylzl[3 * (size_t)k + 1],
+                            xlylzl[3 * (size_t)k + 2], so[k], co[k],
+                            mats ? omega[k] * omegasign : 0.0, trans + 3 * (size_t)r,
+                            mats ? mats + 9 * (size_t)r : nullptr, out);
+        if (tth) tth[q] = out[0];
+        if (eta) eta[q] = out[1];
+        if (mats && omega_calc) omega_calc[q] = out[2];
+        for (int j = 0; j < 3; j++) gv[3 * q + j] = out[3 + j];
+    }
     return GS_OK;
 }
 

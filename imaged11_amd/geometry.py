@@ -93,3 +93,23 @@ def wedge_chi_matrix(wedge, chi):
     C = np.array([[1.0, 0.0, 0.0], [0.0, math.cos(c), -math.sin(c)],
                   [0.0, math.sin(c), math.cos(c)]])
     return np.dot(W, C).ravel()
+
+
+def uncompute_vectors(wavelength, wedge, chi):
+    """The two fixed inputs of the omega-float step
+    (``transform.uncompute_g_vectors``, ``ImageD11/transform.py:529-564``):
+    rb = post^T.(-1/wavelength, 0, 0) with post = wedgechi(wedge, chi) =
+    wedgemat(wedge).chimat(chi), and pre = chiwedge(chi, wedge)^T
+    (``ImageD11/gv_general.py:270-312``).  When wedge == chi == 0 the
+    reference skips both rotations: rb is then the beam itself and pre is
+    None."""
+    beam = np.array([-1.0 / wavelength, 0.0, 0.0])
+    if wedge == chi == 0:
+        return beam, None
+    cw, sw = np.cos(np.radians(wedge)), np.sin(np.radians(wedge))
+    cc, sc = np.cos(np.radians(chi)), np.sin(np.radians(chi))
+    wm = np.array([[cw, 0, sw], [0, 1, 0], [-sw, 0, cw]])
+    cm = np.array([[1, 0, 0], [0, cc, sc], [0, -sc, cc]])
+    post = np.dot(wm, cm)
+    pre = np.dot(cm, wm).T
+    return np.dot(post.T, beam), np.ascontiguousarray(pre)

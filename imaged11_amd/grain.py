@@ -17,6 +17,10 @@ class grain:
         for k, v in kwds.items():
             setattr(self, k, v)
 
+    def set_ubi(self, ubi):
+        """Replace the orientation matrix (``ImageD11/grain.py:60-64``)."""
+        self.ubi = np.array(ubi, float).reshape(3, 3)
+
     @property
     def UB(self):
         return np.linalg.inv(self.ubi)

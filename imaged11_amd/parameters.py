@@ -24,10 +24,14 @@ def _coerce(value):
 
 
 class parameters:
-    """A dictionary of named parameters with the reference's file format."""
+    """A dictionary of named parameters with the reference's file format,
+    and the list of parameters an optimiser varies with their step sizes
+    (``ImageD11/parameters.py:398-446``)."""
 
     def __init__(self, filename=None, **kwds):
         self.parameters = dict(kwds)
+        self.varylist = []
+        self.stepsizes = {}
         if filename is not None:
             self.loadparameters(filename)
 
@@ -66,6 +70,27 @@ class parameters:
 
     def addpar(self, name, value):
         self.parameters[name] = value
+
+    def get_variable_values(self):
+        """Values of the parameters in varylist, in its order."""
+        return [self.parameters[name] for name in self.varylist]
+
+    def get_variable_stepsizes(self):
+        """Step sizes of the parameters in varylist, in its order."""
+        return [self.stepsizes[name] for name in self.varylist]
+
+    def set_varylist(self, vl):
+        for v in vl:
+            if v not in self.parameters:
+                raise KeyError("no parameter %r" % (v,))
+        self.varylist = list(vl)
+
+    def set_variable_values(self, values):
+        if len(values) != len(self.varylist):
+            raise ValueError("%d values for %d varied parameters" % (len(values),
+                                                                   len(self.varylist)))
+        for name, value in zip(self.varylist, values):
+            self.parameters[name] = value
 
 
 def read_par_file(filename):

@@ -426,3 +426,20 @@ def indexing_workload(K=200, cell=(4.05, 4.05, 4.05, 90.0, 90.0, 90.0), symmetry
     g, own = np.ascontiguousarray(g[perm]), own[perm]
     return {"gv": g, "ds": np.sqrt((g * g).sum(axis=1)), "grain": own, "U": Us,
             "cell": tuple(cell), "symmetry": symmetry}
+
+
+def refinegrains_workload(K=20, N=20000, seed=7, wedge=0.0, chi=0.0, translation=100.0,
+                          spurious=0.05, omega_step=0.25, jitter_px=0.3, gmax=1.2):
+    """Grains with translations and a binned-omega peak table, the input of
+    the refinegrains driver: dict(pars, ubi[K,3,3], trans[K,3], peaks).
+    ``pars`` holds every parameter the driver reads (t_x..t_z zero)."""
+    rng = np.random.default_rng(seed)
+    pars = detector_parameters(rng, wedge=wedge, chi=chi)
+    pars.update({"t_x": 0.0, "t_y": 0.0, "t_z": 0.0})
+    ub, cells = random_grains(K, rng)
+    trans = rng.uniform(-translation, translation, size=(K, 3))
+    trans[:, 2] *= 0.25
+    peaks = simulate_peaks(ub, trans, pars, N, seed=seed + 1000, spurious=spurious, gmax=gmax,
+                           omega_step=omega_step, jitter_px=jitter_px)
+    return {"pars": pars, "ubi": np.ascontiguousarray(np.linalg.inv(ub)), "trans": trans,
+            "cells": cells, "peaks": peaks}

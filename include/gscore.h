@@ -237,6 +237,38 @@ int gs_gv_lists(const double *xlylzl, const double *so, const double *co, int ng
                 const int *idx, const int *owner, long long n_idx, double *out,
                 gs_stream_t stream);
 
+/* ---- refinegrains driver ---- */
+
+/* refinegrains.compute_gv (ImageD11/refinegrains.py:366-456) for every list
+ * entry q at once: peak idx[q] seen by request owner[q] (a grain at
+ * translation trans[owner], R requests).  Device pointers; so/co are the
+ * per-peak sin/cos of ((PI/180)*omega)*omegasign, omega the raw angles in
+ * degrees (read only with mats).
+ *   mats == NULL: observed geometry (tth, eta of
+ *     transform.compute_tth_eta_from_xyz, g of compute_g_vectors at
+ *     omega*omegasign);
+ *   mats != NULL: the omega-float branch (OmFloat = True) with fitted UBIs
+ *     mats (R,3,3): omega_calc from uncompute_g_vectors of m^-1.round(m.g),
+ *     wrapped and clipped to +-slop, and g at omega_calc.  rb (3) and pre
+ *     (9, NULL when wedge == chi == 0) are the host-side vectors
+ *     post^T.(-1/wavelength,0,0) and chiwedge^T of uncompute_g_vectors
+ *     (transform.py:529-564).
+ * tth, eta, omega_calc may be NULL; gv is (M,3).  Out-of-range entries
+ * give NaN. */
+int gs_grain_gv(const double *xlylzl, const double *so, const double *co, const double *omega,
+                int ng, double omegasign, double wvln, double wedge, double chi,
+                const double *trans, const double *mats, int R, const double *rb,
+                const double *pre, double slop, const int *idx, const int *owner, long long M,
+                double *tth, double *eta, double *omega_calc, double *gv, gs_stream_t stream);
+
+/* gs_grain_gv's per-entry arithmetic on the host, host pointers (a CPU
+ * self-test of the kernel's code; no device is touched). */
+int gs_grain_gv_host(const double *xlylzl, const double *so, const double *co,
+                     const double *omega, int ng, double omegasign, double wvln, double wedge,
+                     double chi, const double *trans, const double *mats, int R, const double *rb,
+                     const double *pre, double slop, const int *idx, const int *owner, long long M,
+                     double *tth, double *eta, double *omega_calc, double *gv);
+
 /* ---- column-file geometry ---- */
 
 /* compute_geometry (src/cdiffraction.c:31-94; f2py src/_cImageD11.pyf:534-549):

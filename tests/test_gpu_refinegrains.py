@@ -1,0 +1,187 @@
+"""The refinegrains driver on the GPU against the oracle's sequential
+restatement of the reference's per-grain loops.
+
+Tolerances: the GPU driver computes lab positions with compute_xlylzl's
+arithmetic and g-vectors with compute_gv's (the reference: numpy
+transform functions; both agree to ~1e-15 relative), and sums
+score_and_refine in a different order (1e-12 per call).  gof values are
+held to 1e-9 relative, UBIs to 1e-9, translations after the simplexes to
+1e-6 microns; labels and peak counts exactly.
+"""
+import numpy as np
+import pytest
+import torch
+
+from imaged11_amd import _lib, batched, columnfile, geometry, grain, simulate
+from imaged11_amd import refinegrains as RG
+from oracle import refinegrains_oracle as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(orc, tmp_path=None, K=10, N=None, seed=11, OmFloat=True, sym="triclinic",
+           tol=0.05, wedge=0.0, chi=0.0, via_files=False):
+    N = 600 * K if N is None else N
+    w = simulate.refinegrains_workload(K=K, N=N, seed=seed, wedge=wedge, chi=chi)
+    pk = w["peaks"]
+    rg = RG.refinegrains(tolerance=tol, OmFloat=OmFloat, OmSlop=0.25,
+                         latticesymmetry=RG.LATTICES[sym])
+    if via_files:
+        parfile, ubifile, fltfile = (str(tmp_path / n) for n in ("p.par", "g.ubi", "p.flt"))
+        from imaged11_amd import parameters
+        parameters.parameters(**w["pars"]).saveparameters(parfile)
+        grain.write_grain_file(ubifile, [grain.grain(u, t) for u, t in zip(w["ubi"], w["trans"])])
+        col = columnfile.colfile_from_dict({"sc": pk["sc"], "fc": pk["fc"], "omega": pk["omega"]})
+        col.writefile(fltfile)
+        rg.loadparameters(parfile)
+        rg.readubis(ubifile)
+        rg.loadfiltered(fltfile)
+    else:
+        rg.parameterobj.parameters.update(w["pars"])
+        for i in range(K):
+            rg.grainnames.append(i)
+            rg.ubisread[i] = w["ubi"][i]
+            rg.translationsread[i] = w["trans"][i]
+        rg.loadfiltered(columnfile.colfile_from_dict(
+            {"sc": pk["sc"], "fc": pk["fc"], "omega": pk["omega"]}))
+    rg.generate_grains()
+    sname = rg.scannames[0]
+    col = rg.scandata[sname]
+    names = rg.grainnames
+    o = R.RefineGrains(orc, dict(rg.parameterobj.parameters), col.sc, col.fc, col.omega,
+                       [rg.ubisread[n] for n in names],
+                       [rg.grains[(n, sname)].translation.copy() for n in names],
+                       tolerance=tol, latticesymmetry=getattr(R, sym), OmFloat=OmFloat,
+                       OmSlop=0.25)
+    return rg, o, sname
+
+
+def _assign_both(rg, o, sname):
+    rg.assignlabels(quiet=True)
+    o.assignlabels()
+    col = rg.scandata[sname]
+    assert np.array_equal(np.asarray(col.labels, np.int64), o.labels.astype(np.int64))
+    for n in rg.grainnames:
+        assert np.array_equal(rg.grains[(n, sname)].ind, o.grains[n].ind)
+
+
+def test_grain_gv_device_matches_host(dev, orc):
+    """gs_grain_gv on the device against its host build: the same code, so
+    only the device's sin/cos/atan2/asin differ (correctly rounded on the
+    device, within an ulp in glibc)."""
+    lib = _lib.load()
+    w = simulate.refinegrains_workload(K=16, N=9000, seed=4, wedge=1.0, chi=-2.5)
+    p, pk = w["pars"], w["peaks"]
+    xl = R.compute_xyz_lab([pk["sc"], pk["fc"]], **p).T.copy()
+    so, co = orc.sincos_omega(pk["omega"], p["omegasign"])
+    rng = np.random.default_rng(1)
+    R_ = 40
+    trans = rng.uniform(-100, 100, size=(R_, 3))
+    mats = w["ubi"][rng.integers(0, 16, R_)]
+    owner = np.sort(rng.integers(0, R_, 20000)).astype(np.int32)
+    idx = rng.integers(-2, len(xl) + 2, 20000).astype(np.int32)
+    T = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    dxl, dso, dco, dom = T(xl), T(so), T(co), T(pk["omega"])
+    dtr, dm = T(trans), T(mats)
+    didx, down = T(idx, torch.int32), T(owner, torch.int32)
+    for use_mats in (False, True):
+        gv, tth, eta, omc = batched.grain_gv(dxl, dso, dco, dom, p, dtr, didx, down,
+                                             mats=dm if use_mats else None, angles=True)
+        sign, wvln, wedge, chi = geometry.pack_gv(p)
+        rb, pre = geometry.uncompute_vectors(wvln, wedge, chi)
+        M = len(idx)
+        h = [np.zeros(M), np.zeros(M), np.zeros(M), np.zeros((M, 3))]
+        P = lambda a: None if a is None else a.ctypes.data
+        mh = np.ascontiguousarray(mats) if use_mats else None
+        assert lib.gs_grain_gv_host(P(xl), P(so), P(co), P(pk["omega"]), len(xl), sign, wvln,
+                                    wedge, chi, P(trans), P(mh), R_, _lib.darr(rb),
+                                    _lib.darr(pre.ravel()), 0.25, P(idx), P(owner), M, P(h[0]),
+                                    P(h[1]), P(h[2]) if use_mats else None, P(h[3])) == 0
+        got = gv.cpu().numpy()
+        bad = (idx < 0) | (idx >= len(xl))
+        assert np.isnan(got[bad]).all()
+        np.testing.assert_allclose(got[~bad], h[3][~bad], rtol=0,
+                                   atol=1e-13 * np.abs(h[3][~bad]).max())
+        np.testing.assert_allclose(tth.cpu().numpy()[~bad], h[0][~bad], atol=1e-11)
+        np.testing.assert_allclose(eta.cpu().numpy()[~bad], h[1][~bad], atol=1e-11)
+        if use_mats:
+            om_d = omc.cpu().numpy()[~bad]
+            assert np.isfinite(om_d).all()
+            np.testing.assert_allclose(om_d, h[2][~bad], atol=1e-9)
+
+
+@pytest.mark.parametrize("OmFloat,sym", [(False, "triclinic"), (True, "triclinic"),
+                                         (True, "cubic")])
+def test_gof_matches_oracle(dev, orc, OmFloat, sym):
+    rg, o, s = _setup(orc, OmFloat=OmFloat, sym=sym)
+    _assign_both(rg, o, s)
+    keys = list(rg.grains.keys())
+    rg.grains_to_refine = keys
+    o.grains_to_refine = [k[0] for k in keys]
+    for names, vals in ([], []), (["distance", "y_center"], [150300.0, 1019.0]), \
+            (["tilt_z", "wedge"], [0.004, 0.3]):
+        rg.parameterobj.varylist = names
+        o.varylist = names
+        rg.recompute_xlylzl = o.recompute_xlylzl = bool(names)
+        a = rg.gof(vals)
+        b = o.gof(vals)
+        assert abs(a - b) <= 1e-9 * abs(b), (names, a, b)
+        assert rg.npks == o.npks
+        for k in keys:
+            np.testing.assert_allclose(rg.grains[k].ubi, o.grains[k[0]].ubi, rtol=0, atol=1e-9)
+
+
+def test_refineubis_matches_oracle(dev, orc, tmp_path):
+    rg, o, s = _setup(orc, tmp_path, OmFloat=True, via_files=True)
+    _assign_both(rg, o, s)
+    for scoreonly in (True, False):
+        got = rg.refineubis(scoreonly=scoreonly)
+        want = o.refineubis(scoreonly=scoreonly)
+        for k, (n, m) in got.items():
+            assert n == want[k[0]][0]
+            assert abs(m - want[k[0]][1]) <= 1e-9 * abs(want[k[0]][1])
+            np.testing.assert_allclose(rg.grains[k].ubi, o.grains[k[0]].ubi, atol=1e-9)
+    assert rg.parameterobj.parameters["t_x"] == o.parameters["t_x"]
+
+
+@pytest.mark.parametrize("OmFloat", [False, True])
+def test_refinepositions_matches_oracle(dev, orc, OmFloat):
+    """All grains' simplexes stepped together give the translations (the
+    last evaluated point of each) of the reference's grain-by-grain loop."""
+    rg, o, s = _setup(orc, K=12, OmFloat=OmFloat, seed=13)
+    rng = np.random.default_rng(2)
+    for n in rg.grainnames:
+        off = rng.uniform(-15, 15, 3)
+        rg.grains[(n, s)].translation = rg.grains[(n, s)].translation + off
+        o.grains[n].translation = o.grains[n].translation + off
+    got = rg.refinepositions(maxiters=100)
+    steps = {k: rg.parameterobj.stepsizes[k] for k in ("t_x", "t_y", "t_z")}
+    want = o.refinepositions(steps, maxiters=100)
+    same_iters = 0
+    for k in got:
+        np.testing.assert_allclose(rg.grains[k].translation, o.grains[k[0]].translation,
+                                   rtol=0, atol=1e-6)
+        same_iters += got[k][2] == want[k[0]][2]
+    assert same_iters == len(got)
+    assert rg.tolerance == 0.05
+    # fewer GPU rounds than the reference's sequential objective calls
+    assert rg.evaluation_rounds < o.evaluations
+
+
+def test_fit_matches_oracle(dev, orc):
+    """fit() of distance and y_center from an offset start: the same
+    simplex over gof on the GPU and in the oracle."""
+    rg, o, s = _setup(orc, K=8, OmFloat=False, seed=17)
+    p0 = dict(rg.parameterobj.parameters)
+    for d in (rg.parameterobj.parameters, o.parameters):
+        d["distance"] = p0["distance"] + 400.0
+        d["y_center"] = p0["y_center"] - 0.7
+    rg.parameterobj.varylist = ["distance", "y_center"]
+    o.varylist = ["distance", "y_center"]
+    steps = dict(rg.parameterobj.stepsizes)
+    g1, e1, it1 = rg.fit(maxiters=40)
+    g2, e2, it2 = o.fit(steps, maxiters=40)
+    assert it1 == it2
+    np.testing.assert_allclose(g1, g2, rtol=1e-9)
+    assert abs(e1 - e2) <= 1e-9 * abs(e2)
+    assert abs(g1[0] - p0["distance"]) < 400.0  # moved towards the truth
