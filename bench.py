@@ -288,6 +288,87 @@ def aux_index(with_cpu, K=3000, seed=11):
     return out
 
 
+def aux_refinegrains(with_cpu, K=2000, maxiters=100, cpu_grains=10, seed=21):
+    """refinegrains.refinepositions for K grains (translations 15 microns
+    off) on a binned-omega peak table: all K simplexes stepped together on
+    the GPU (omega-float on), against the oracle's grain-by-grain loop (the
+    reference's call pattern, one thread) on the first `cpu_grains` grains,
+    whose per-grain time is scaled to K."""
+    import numpy as np
+    import torch
+    from imaged11_amd import columnfile, simulate
+    from imaged11_amd import refinegrains as RG
+    t0 = time.perf_counter()
+    w = simulate.refinegrains_workload(K=K, N=600 * K, seed=seed)
+    gen_s = time.perf_counter() - t0
+    pk = w["peaks"]
+    rng = np.random.default_rng(seed)
+    start = w["trans"] + rng.uniform(-15, 15, size=w["trans"].shape)
+
+    def make():
+        rg = RG.refinegrains(tolerance=0.05, OmFloat=True, OmSlop=0.25)
+        rg.parameterobj.parameters.update(w["pars"])
+        for i in range(K):
+            rg.grainnames.append(i)
+            rg.ubisread[i] = w["ubi"][i]
+            rg.translationsread[i] = start[i]
+        rg.loadfiltered(columnfile.colfile_from_dict(
+            {"sc": pk["sc"], "fc": pk["fc"], "omega": pk["omega"]}))
+        rg.generate_grains()
+        return rg
+    make().refinepositions(maxiters=3)  # warm the code paths
+    rg = make()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = rg.refinepositions(maxiters=maxiters)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    s = rg.scannames[0]
+    err = np.array([rg.grains[(i, s)].translation for i in range(K)]) - w["trans"]
+    iters = np.array([r[2] for r in res.values()])
+    out = {"grains": K, "peaks": int(len(pk["sc"])), "gen_s": gen_s, "maxiters": maxiters,
+           "gpu_s": gpu_s, "grains_per_s": K / gpu_s, "evaluation_rounds": rg.evaluation_rounds,
+           "ms_per_round": gpu_s * 1e3 / max(rg.evaluation_rounds, 1),
+           "median_iterations": float(np.median(iters)),
+           "median_translation_error_um": float(np.median(np.abs(err).max(1)))}
+    if with_cpu:
+        from oracle import oracle, refinegrains_oracle as R
+        orc = oracle.load_oracle()
+        sub = slice(0, cpu_grains)
+        o = R.RefineGrains(orc, dict(rg.parameterobj.parameters), pk["sc"], pk["fc"],
+                           pk["omega"], w["ubi"], start, tolerance=0.05, OmFloat=True,
+                           OmSlop=0.25)
+        # the sample grains' peak lists as assignlabels fixed them
+        for key in range(K)[sub]:
+            gr, ind = o.grains[key], rg.grains[(key, s)].ind
+            gr.ind = ind
+            gr.sc, gr.fc, gr.om = pk["sc"][ind], pk["fc"][ind], pk["omega"][ind]
+            gr.peaks_xyz = R.compute_xyz_lab([gr.sc, gr.fc], **o.parameters).T.copy()
+        steps = {k: rg.parameterobj.stepsizes[k] for k in ("t_x", "t_y", "t_z")}
+        o.tolerance = 1.0
+        o.varylist = ["t_x", "t_y", "t_z"]
+        worst = 0.0
+        t0 = time.perf_counter()
+        for key in range(K)[sub]:
+            # refinegrains.refinepositions' loop body for one grain
+            o.grains_to_refine = [key]
+            o.set_translation(key)
+            guess = [o.parameters[n] for n in o.varylist]
+            R.Simplex(o.gof, guess, [steps[n] for n in o.varylist]).minimize(maxiters=maxiters)
+            last = np.array([o.parameters["t_x"], o.parameters["t_y"], o.parameters["t_z"]])
+            worst = max(worst, float(np.abs(last - rg.grains[(key, s)].translation).max()))
+        cpu_s = time.perf_counter() - t0
+        per_grain = cpu_s / cpu_grains
+        same = worst < 1e-6
+        out["cpu"] = {"kind": "port", "threads": 1, "grains": cpu_grains, "seconds": cpu_s,
+                      "seconds_per_grain": per_grain,
+                      "sample": "first %d grains, one simplex each, sequential" % cpu_grains}
+        out["cpu_s_scaled_to_all_grains"] = per_grain * K
+        out["speedup_vs_cpu"] = per_grain * K / gpu_s
+        out["sample_translations_equal_gpu_within_1e-6_um"] = bool(same)
+    return out
+
+
 def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
     """Large scan: compute_gv per grain translation + score_and_assign for
     K grains x N peaks (per-peak dty, 15% spurious) in one fused launch,
@@ -462,7 +543,8 @@ def main():
     if not args.no_aux:
         out["aux"] = {"refine_5000_grains": aux_refine_adjoint(threads, not args.no_cpu),
                       "scan_20000_grains": aux_scan(threads, not args.no_cpu),
-                      "index_3000_grains": aux_index(not args.no_cpu)}
+                      "index_3000_grains": aux_index(not args.no_cpu),
+                      "refinegrains_2000_grains": aux_refinegrains(not args.no_cpu)}
     print(json.dumps(out))
 
 

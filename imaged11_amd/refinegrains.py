@@ -162,9 +162,7 @@ def compute_lp_factor(colfile, **kwds):
         colfile.addcolumn(lf(colfile.tth_per_grain, colfile.eta_per_grain), "Lorentz_per_grain")
 
 
-def compute_total_intensity(colfile, indices, tth_range, ntrim=2, quiet=False):
-    """The summary string of a grain's peak intensities
-    (refinegrains.py:919-995)."""
+def _weighted_intensity(colfile):
     t = colfile.titles
     if "sum_intensity" in t:
         raw = colfile.sum_intensity
@@ -180,15 +178,22 @@ def compute_total_intensity(colfile, indices, tth_range, ntrim=2, quiet=False):
         lor = colfile.Lorentz
     else:
         lor = np.ones(colfile.nrows, np.float32)
-    intensities = np.take(raw * lor, indices)
+    return raw * lor
+
+
+def _tth_column(colfile):
+    t = colfile.titles
+    if "tth_per_grain" in t:
+        return colfile.tth_per_grain
+    if "tth" in t:
+        return colfile.tth
+    return np.ones(colfile.nrows)
+
+
+def _intensity_summary(weighted, tth, indices, tth_range, ntrim, quiet):
+    intensities = np.take(weighted, indices)
     sigma_i = np.sum(intensities)
     if tth_range is not None:
-        if "tth_per_grain" in t:
-            tth = colfile.tth_per_grain
-        elif "tth" in t:
-            tth = colfile.tth
-        else:
-            tth = np.ones(colfile.nrows)
         tv = np.take(tth, indices)
         intensities = np.compress((tv > min(tth_range)) & (tv < max(tth_range)), intensities)
     if len(intensities) < 1:
@@ -204,6 +209,13 @@ def compute_total_intensity(colfile, indices, tth_range, ntrim=2, quiet=False):
     if not quiet:
         print(ret)
     return ret
+
+
+def compute_total_intensity(colfile, indices, tth_range, ntrim=2, quiet=False):
+    """The summary string of a grain's peak intensities (intensity times the
+    Lorentz factor, trimmed, within tth_range) (refinegrains.py:919-995)."""
+    return _intensity_summary(_weighted_intensity(colfile), _tth_column(colfile), indices,
+                              tth_range, ntrim, quiet)
 
 
 # ---------------------------------------------------------------------------
@@ -666,11 +678,12 @@ class refinegrains:
             batched.gv_assign_many(st.xl, st.so, st.co, wvln, ubis, trans, self.tolerance,
                                    drlv2, labels, grain_labels=glabels, wedge=wedge, chi=chi)
             # per-grain lists in grain order (peaks ascending in each)
-            lab_to_pos = {int(n): j for j, n in enumerate(names)}
             lab_h = labels.cpu().numpy()
-            posarr = np.full(lab_h.shape, -1, np.int64)
-            for lab, j in lab_to_pos.items():
-                posarr[lab_h == lab] = j
+            ints = np.array([int(n) for n in names], np.int64)
+            lut = np.full(int(ints.max()) + 2 if K else 1, -1, np.int64)
+            lut[ints] = np.arange(K)
+            ok = (lab_h >= 0) & (lab_h < len(lut) - 1)
+            posarr = np.where(ok, lut[np.where(ok, lab_h, len(lut) - 1)], -1)
             st.idx, st.off, _ = batched.lists_from_labels(torch.as_tensor(posarr, device=dev), K)
             st.off_host = st.off.cpu().numpy()
             # the g-vector column is compute_gv with the last grain's translation
@@ -714,8 +727,8 @@ class refinegrains:
                 if not quiet:
                     print("Grain", n, "Scan", sname, "npks=", len(ind))
             compute_lp_factor(col)
+            weighted, tthc = _weighted_intensity(col), _tth_column(col)
             for n in names:
                 gr = self.grains[(n, sname)]
-                gr.intensity_info = compute_total_intensity(col, gr.ind,
-                                                            self.intensity_tth_range,
-                                                            quiet=quiet)
+                gr.intensity_info = _intensity_summary(weighted, tthc, gr.ind,
+                                                       self.intensity_tth_range, 2, quiet)
