@@ -27,6 +27,7 @@
 #include "gs_index.h"
 #include "gs_geometry.h"
 #include "gs_grains.h"
+#include "gs_pbp.h"
 #include "../../include/gscore.h"
 
 namespace {
@@ -1016,6 +1017,40 @@ int gs_grain_gv_host(const double* xlylzl, const double* so, const double* co,
         if (mats && omega_calc) omega_calc[q] = out[2];
         for (int j = 0; j < 3; j++) gv[3 * q + j] = out[3 + j];
     }
+    return GS_OK;
+}
+
+// ---- point-by-point refinement ------------------------------------------------
+
+int gs_pbp_assign(const double* so, const double* co, const double* dty, const double* gv,
+                  int ng, const double* pts, const int* ptask, int P, const double* ubis, int T,
+                  double y0, double ystep, double tol, int* counts, const long long* task_off,
+                  int* cursor, int* out_peak, int* out_hkl, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (ng < 0 || P < 0 || T < 0) return set_err(GS_ERR_ARG, "gs_pbp_assign: negative size");
+    if (!counts) return set_err(GS_ERR_ARG, "gs_pbp_assign: counts is NULL");
+    const bool fill = task_off != nullptr;
+    if (fill && (!cursor || !out_peak || !out_hkl))
+        return set_err(GS_ERR_ARG, "gs_pbp_assign: the fill pass needs cursor and outputs");
+    hipStream_t st = S(stream);
+    if (!fill) GS_CHECK(fill_i32(counts, 0, T, st));
+    else GS_CHECK(fill_i32(cursor, 0, T, st));
+    if (ng == 0 || P == 0 || T == 0) return GS_OK;
+    if (!so || !co || !dty || !gv || !pts || !ptask || !ubis)
+        return set_err(GS_ERR_ARG, "gs_pbp_assign: NULL array");
+    const long long ny = (P + GS_PBP_POINTS - 1) / GS_PBP_POINTS;
+    if (ny > 65535) return set_err(GS_ERR_ARG, "gs_pbp_assign: too many points (%d)", P);
+    const dim3 grid(blocks_for(ng, GS_BLOCK), (unsigned)ny);
+    if (fill)
+        hipLaunchKernelGGL(k_pbp_assign<1>, grid, dim3(GS_BLOCK), 0, st, so, co, dty, gv, ng, pts,
+                           ptask, P, ubis, y0, ystep, tol * tol, counts, task_off, cursor,
+                           out_peak, out_hkl);
+    else
+        hipLaunchKernelGGL(k_pbp_assign<0>, grid, dim3(GS_BLOCK), 0, st, so, co, dty, gv, ng, pts,
+                           ptask, P, ubis, y0, ystep, tol * tol, counts, task_off, cursor,
+                           out_peak, out_hkl);
+    GS_LAUNCHED();
     return GS_OK;
 }
 

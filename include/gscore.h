@@ -269,6 +269,28 @@ int gs_grain_gv_host(const double *xlylzl, const double *so, const double *co,
                      const double *pre, double slop, const int *idx, const int *owner, long long M,
                      double *tth, double *eta, double *omega_calc, double *gv);
 
+/* ---- point-by-point refinement ---- */
+
+/* The per-voxel selection and first scoring of refine_map
+ * (ImageD11/sinograms/point_by_point.py:1432-1545, get_voxel_idx :359-368)
+ * for every (voxel, UBI) task at once.  Device pointers.  Peaks: so, co
+ * (sin, cos of omega), dty, gv (ng,3).  Voxels: pts (P,2) = (xi0, yi0) and
+ * ptask (P+1) = the voxel's range of tasks (tasks sorted by voxel); ubis
+ * (T,3,3).  A peak is assigned to a task when |y0 - xi0 sin - yi0 cos -
+ * dty| <= ystep and sum((h - round(h))^2) < tol^2, h = UBI.g, round half
+ * to even.
+ *   task_off == NULL: counting pass, counts[t] = assigned peaks of task t
+ *     (zeroed here);
+ *   task_off != NULL: filling pass with the counts of the counting pass and
+ *     their exclusive prefix sums task_off (T+1): writes out_peak and
+ *     out_hkl (3 ints) of each assigned peak at task_off[t] + a slot
+ *     (cursor (T) is caller scratch, zeroed here); the order inside a task
+ *     is unspecified. */
+int gs_pbp_assign(const double *so, const double *co, const double *dty, const double *gv,
+                  int ng, const double *pts, const int *ptask, int P, const double *ubis, int T,
+                  double y0, double ystep, double tol, int *counts, const long long *task_off,
+                  int *cursor, int *out_peak, int *out_hkl, gs_stream_t stream);
+
 /* ---- column-file geometry ---- */
 
 /* compute_geometry (src/cdiffraction.c:31-94; f2py src/_cImageD11.pyf:534-549):

@@ -1,0 +1,81 @@
+"""Point-by-point refinement on the GPU against the oracle's voxel loop.
+
+The first selection (gs_pbp_assign) is compared entry for entry: every
+task's peak list and hkl.  refine_map's results: NaN patterns, peak and
+distinct-reflection counts exactly; UBIs to 1e-9 and strain tensors to
+1e-10 (the GPU's g-vectors follow compute_gv's arithmetic, the reference
+the numpy chain: ~1e-15 relative; the fits solve normal equations where
+the reference calls lstsq)."""
+import numpy as np
+import pytest
+import torch
+
+from imaged11_amd import simulate
+from imaged11_amd import point_by_point as PP
+from oracle import pbp_oracle as P
+
+pytestmark = pytest.mark.gpu
+
+ARGS = ("refine_points", "all_pbpmap_ubis", "ri_col", "rj_col", "sx_grid", "sy_grid", "mask",
+        "sc", "fc", "eta", "sum_intensity", "sinomega", "cosomega", "omega", "dty", "dtyi",
+        "xpos", "ystep", "y0", "B0")
+
+
+@pytest.fixture(scope="module")
+def wl():
+    return simulate.pbp_workload()
+
+
+def test_compute_gve_matches_oracle(dev, wl):
+    g = PP.compute_gve(wl["sc"], wl["fc"], wl["omega"], wl["xpos"], **wl["geometry"])
+    want = P.compute_gve(wl["sc"], wl["fc"], wl["omega"], wl["xpos"], **wl["geometry"])
+    np.testing.assert_allclose(g, want, rtol=0, atol=2e-15 * np.abs(want).max())
+
+
+def test_assign_tasks_matches_oracle(dev, wl):
+    """Every voxel of the map with all of its UBIs, the kernel's lists
+    against get_voxel_idx + the first scoring of refine_map."""
+    T_ = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt,
+                                                     device=dev)
+    gv = P.compute_gve(wl["sc"], wl["fc"], wl["omega"], wl["xpos"], **wl["geometry"]).T.copy()
+    NY = wl["mask"].shape[0]
+    pts, ptask, ubis = [], [0], []
+    M = wl["all_pbpmap_ubis"].shape[2]
+    for i in range(NY):
+        for j in range(NY):
+            here = [m for m in range(M) if wl["ri_col"][m] == i and wl["rj_col"][m] == j and
+                    not np.isnan(wl["all_pbpmap_ubis"][0, 0, m])]
+            if not here:
+                continue
+            pts.append((wl["sx_grid"][i, j], wl["sy_grid"][i, j]))
+            ubis.extend(wl["all_pbpmap_ubis"][:, :, m] for m in here)
+            ptask.append(len(ubis))
+    pts, ptask, ubis = np.array(pts), np.array(ptask, np.int32), np.array(ubis)
+    off, peak, hkl = PP.assign_tasks(T_(wl["sinomega"]), T_(wl["cosomega"]), T_(wl["dty"]),
+                                     T_(gv), T_(pts), T_(ptask, torch.int32), T_(ubis),
+                                     wl["y0"], wl["ystep"], 0.1)
+    total = 0
+    for p in range(len(pts)):
+        li, _ = P.get_voxel_idx(wl["y0"], pts[p, 0], pts[p, 1], wl["sinomega"], wl["cosomega"],
+                                wl["dty"], wl["ystep"])
+        for t in range(ptask[p], ptask[p + 1]):
+            hi, m = P._assign(ubis[t], gv[li].T, 0.1)
+            sl = slice(off[t], off[t + 1])
+            assert np.array_equal(peak[sl], li[m])
+            assert np.array_equal(hkl[:, sl], hi[:, m])
+            total += m.sum()
+    assert total == len(peak) > 10000
+
+
+@pytest.mark.parametrize("tol,merge_tol,minpks", [(0.1, 0.05, 6), (0.05, 0.03, 50)])
+def test_refine_map_matches_oracle(dev, wl, tol, merge_tol, minpks):
+    kw = dict(tol=tol, merge_tol=merge_tol, min_grain_npks=minpks)
+    a = PP.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], **kw)
+    b = P.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], **kw)
+    for x, y in zip(a, b):
+        assert np.array_equal(np.isnan(x), np.isnan(y))
+    np.testing.assert_allclose(a[0], b[0], atol=1e-9, equal_nan=True)
+    np.testing.assert_allclose(a[1], b[1], atol=1e-10, equal_nan=True)
+    assert np.array_equal(a[2], b[2], equal_nan=True)
+    assert np.array_equal(a[3], b[3], equal_nan=True)
+    assert (~np.isnan(a[1][0, 0])).sum() >= 40

@@ -1,0 +1,189 @@
+"""Point-by-point refinement on the CPU: the oracle's helpers pinned as the
+reference's own tests pin them (test/test_point_by_point.py) and by closed
+forms; the product's vectorised stages against the oracle's voxel loop."""
+import numpy as np
+import pytest
+import torch
+
+from imaged11_amd import simulate
+from imaged11_amd import point_by_point as PP
+from oracle import pbp_oracle as P
+
+ARGS = ("refine_points", "all_pbpmap_ubis", "ri_col", "rj_col", "sx_grid", "sy_grid", "mask",
+        "sc", "fc", "eta", "sum_intensity", "sinomega", "cosomega", "omega", "dty", "dtyi",
+        "xpos", "ystep", "y0", "B0")
+
+
+@pytest.fixture(scope="module")
+def wl():
+    return simulate.pbp_workload()
+
+
+def test_count_unique_peaks_is_np_unique():
+    rng = np.random.default_rng(0)
+    hkl = rng.integers(0, 6, size=(3, 1000))
+    etasign = rng.integers(0, 2, size=1000) * 2 - 1
+    dtyi = rng.integers(-10, 11, size=1000)
+    want = np.unique(np.vstack((hkl, etasign, dtyi)), axis=1, return_inverse=True)[1].ravel()
+    assert np.array_equal(P.count_unique_peaks(hkl, etasign, dtyi), want)
+    assert np.array_equal(PP.count_unique_peaks(hkl, etasign, dtyi), want)
+    want = np.unique(np.vstack((hkl, etasign)), axis=1, return_inverse=True)[1].ravel()
+    assert np.array_equal(P.count_unique_peaks_no_dtyi(hkl, etasign), want)
+    assert np.array_equal(PP.count_unique_peaks_no_dtyi(hkl, etasign), want)
+
+
+def test_merge_is_weighted_group_mean():
+    rng = np.random.default_rng(1)
+    n = 500
+    hkl = rng.integers(-2, 3, size=(3, n))
+    es = rng.integers(0, 2, n) * 2 - 1
+    dt = rng.integers(0, 3, n)
+    cols = [rng.normal(size=n) for _ in range(6)]
+    w = rng.uniform(1, 5, n)
+    out = P.merge(hkl, es, dt, w, *cols)
+    groups = {}
+    for i in range(n):
+        groups.setdefault((*hkl[:, i], es[i], dt[i]), []).append(i)
+    for j, key in enumerate(sorted(groups)):
+        idx = groups[key]
+        assert np.isclose(out[0][j], w[idx].sum())
+        for c, col in enumerate(cols):
+            assert np.isclose(out[1 + c][j], np.sum(col[idx] * w[idx]) / w[idx].sum())
+    pm = PP.merge(hkl, es, dt, w, *cols)
+    for a, b in zip(out, pm):
+        assert np.array_equal(a, b)
+
+
+def test_gve_norm_and_divide_where():
+    rng = np.random.default_rng(2)
+    g = rng.random((3, 1000))
+    np.testing.assert_allclose(P.gve_norm(g), np.linalg.norm(g, axis=0), rtol=1e-15)
+    np.testing.assert_allclose(PP.gve_norm(g), np.linalg.norm(g, axis=0), rtol=1e-15)
+    a, b = rng.random((50, 50)), rng.random((50, 50))
+    out = np.ones_like(a)
+    wh = rng.integers(0, 2, (50, 50))
+    want = np.divide(a, b, out=out.copy(), where=wh != 0)
+    assert np.array_equal(P.divide_where(a, b, out, wh), want)
+    assert np.array_equal(PP.divide_where(a, b, out, wh), want)
+
+
+def test_compute_gve_is_compute_gv_with_xpos(ref):
+    """compute_gve = compute_xlylzl with x - xpos, then compute_gv with the
+    translation (the compiled reference), to 1e-13."""
+    rng = np.random.default_rng(3)
+    n = 3000
+    pars = simulate.detector_parameters(rng)
+    sc, fc = rng.uniform(0, 2048, n), rng.uniform(0, 2048, n)
+    om, xpos = rng.uniform(-180, 180, n), rng.normal(scale=200, size=n)
+    geo = {k: float(pars[k]) for k in ("distance", "y_center", "y_size", "tilt_y", "z_center",
+                                       "z_size", "tilt_z", "tilt_x", "o11", "o12", "o21", "o22",
+                                       "wedge", "chi", "wavelength")}
+    geo.update(t_x=5.0, t_y=-7.0, t_z=2.0, wedge=0.7, chi=-1.1)
+    g = P.compute_gve(sc, fc, om, xpos, **geo)
+    from imaged11_amd import geometry
+    p, r, dist = geometry.pack_xlylzl(dict(pars, **geo))
+    xl = np.zeros((n, 3))
+    ref.compute_xlylzl(sc, fc, p, r, dist, xl)
+    xl[:, 0] -= xpos
+    want = np.zeros((n, 3))
+    ref.compute_gv(xl, om, 1.0, geo["wavelength"], geo["wedge"], geo["chi"],
+                   np.array([5.0, -7.0, 2.0]), want)
+    np.testing.assert_allclose(g.T, want, rtol=0, atol=1e-13 * np.abs(want).max())
+
+
+def test_ubi_and_ucell_to_u_and_unitcell():
+    rng = np.random.default_rng(4)
+    U = simulate.random_rotations(4, rng)
+    from imaged11_amd import unitcell
+    for k, cell in enumerate(([4.05] * 3 + [90] * 3, [3.1, 4.2, 5.3, 80.0, 95.0, 101.0])):
+        B = unitcell.unitcell(cell, "P").B
+        ubi = np.linalg.inv(U[k] @ B)
+        np.testing.assert_allclose(P.ubi_to_unitcell(ubi), cell, rtol=1e-12)
+        u = P.ubi_and_ucell_to_u(ubi, np.array(cell, float))
+        np.testing.assert_allclose(u, U[k], atol=1e-12)
+        np.testing.assert_allclose(PP.ubi_and_ucell_to_u(ubi, np.array(cell, float)), u,
+                                   atol=1e-14)
+        np.testing.assert_allclose(PP.ubi_to_unitcell(ubi), P.ubi_to_unitcell(ubi), rtol=1e-14)
+
+
+def test_weighted_fit_and_strain_closed_forms():
+    """Noise-free g of a lattice strained by a symmetric eps: the weighted
+    fit returns the UBI exactly, the strain fit returns eps to first
+    order (second-order terms ~|eps|^2)."""
+    rng = np.random.default_rng(5)
+    a = 4.05
+    B0 = np.eye(3) / a
+    U = simulate.random_rotations(1, rng)[0]
+    eps = rng.normal(scale=1e-3, size=(3, 3))
+    eps = 0.5 * (eps + eps.T)
+    ub = np.linalg.inv(np.eye(3) + eps).T @ U @ B0
+    hkl = np.array(np.meshgrid(*[np.arange(-3, 4)] * 3, indexing="ij")).reshape(3, -1)
+    hkl = hkl[:, np.any(hkl != 0, axis=0)]
+    g = ub @ hkl
+    yd = rng.uniform(0, 3, hkl.shape[1])
+    _, ubifit, _, rank, _ = P.weighted_lstsq_ubi_fit(yd, g, hkl)
+    assert rank == 3
+    np.testing.assert_allclose(ubifit, np.linalg.inv(ub), atol=1e-12)
+    e = P.strain_fit(ubifit, g, hkl, yd, B0, 1e-4)
+    np.testing.assert_allclose(e, eps, atol=5e-6)
+
+
+def test_hkluniq():
+    rng = np.random.default_rng(6)
+    ubi = np.eye(3) * 4.0
+    hkl = rng.integers(-4, 5, size=(3, 400)).astype(float)
+    g = hkl / 4.0 + rng.normal(scale=0.002, size=hkl.shape)
+    eta = rng.normal(size=400)
+    m = rng.random(400) < 0.8
+    want = P.hkluniq(ubi, g[0], g[1], g[2], eta, m, 0.05, 4)
+    assert PP.hkluniq(ubi, g[0], g[1], g[2], eta, m, 0.05, 4) == want
+    assert want[0] == int(m.sum())
+
+
+def test_oracle_refine_map_recovers_truth(wl):
+    """Voxel UBIs 1e-3 rad off come back to the grain's UBI, and the
+    strained grain's tensor is recovered, to the data's noise level."""
+    U, E, N, Q = P.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], tol=0.1,
+                              merge_tol=0.05)
+    ubs, eps = wl["truth"]["ubs"], wl["truth"]["eps"]
+    done = ~np.isnan(E[0, 0])
+    assert done.sum() >= 50
+    for m in np.nonzero(done)[0]:
+        ubi = wl["all_pbpmap_ubis"][:, :, m]
+        g = int(np.argmin([np.abs(ubi @ u - np.eye(3)).max() for u in ubs]))
+        assert np.abs(U[:, :, m] @ ubs[g] - np.eye(3)).max() < 1e-4
+        assert np.abs(E[:, :, m] - (eps if g == 1 else 0)).max() < 1.5e-4
+    assert np.isnan(N[-1])  # the NaN entry is skipped
+
+
+def test_refine_map_host_stages_match_oracle(wl, monkeypatch):
+    """The product's vectorised stages (merge, re-selection, batched fits,
+    strain) with the two GPU pieces replaced by the oracle's numpy
+    equivalents: the same results as the oracle's voxel loop."""
+    monkeypatch.setattr(PP, "_dev", lambda: torch.device("cpu"))
+    monkeypatch.setattr(PP, "_gve_device", lambda sc, fc, om, xp, geo: torch.as_tensor(
+        P.compute_gve(sc.numpy(), fc.numpy(), om.numpy(), xp.numpy(), **geo).T.copy()))
+
+    def assign(so, co, dty, gv, pts, ptask, ubis, y0, ystep, tol):
+        so, co, dty, gv, pts, ptask, ubis = (x.numpy() for x in (so, co, dty, gv, pts, ptask,
+                                                                 ubis))
+        cnt, peaks, hkls = np.zeros(ubis.shape[0], np.int64), [], []
+        for p in range(len(pts)):
+            li, _ = P.get_voxel_idx(y0, pts[p, 0], pts[p, 1], so, co, dty, ystep)
+            for t in range(ptask[p], ptask[p + 1]):
+                hi, m = P._assign(ubis[t], gv[li].T, tol)
+                cnt[t] = m.sum()
+                peaks.append(li[m])
+                hkls.append(hi[:, m])
+        off = np.zeros(len(cnt) + 1, np.int64)
+        off[1:] = np.cumsum(cnt)
+        return off, np.concatenate(peaks), np.concatenate(hkls, 1)
+    monkeypatch.setattr(PP, "assign_tasks", assign)
+    a = PP.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], tol=0.1, merge_tol=0.05)
+    b = P.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], tol=0.1, merge_tol=0.05)
+    for x, y in zip(a, b):
+        assert np.array_equal(np.isnan(x), np.isnan(y))
+    np.testing.assert_allclose(a[0], b[0], atol=1e-12, equal_nan=True)
+    np.testing.assert_allclose(a[1], b[1], atol=1e-13, equal_nan=True)
+    assert np.array_equal(a[2], b[2], equal_nan=True)
+    assert np.array_equal(a[3], b[3], equal_nan=True)
