@@ -369,6 +369,82 @@ def aux_refinegrains(with_cpu, K=2000, maxiters=100, cpu_grains=10, seed=21):
     return out
 
 
+def aux_pbp(with_cpu, NY=101, grains=24, cpu_points=40):
+    """point_by_point.refine_map on an NY x NY map (a disc of voxels in
+    `grains` Voronoi grains, one detector peak per grain, reflection and
+    stage position): the whole map on
+    the GPU (selection + first scoring of every (voxel, UBI) task in one
+    kernel, the rest vectorised over tasks), against the oracle's voxel loop
+    (the reference's per-voxel code, one thread) on the first `cpu_points`
+    voxels, scaled to the map."""
+    import numpy as np
+    import torch
+    from imaged11_amd import point_by_point as PP, simulate
+    t0 = time.perf_counter()
+    w = simulate.pbp_workload(NY=NY, grains=grains, gmax=1.0, seed=41)
+    gen_s = time.perf_counter() - t0
+    keys = ("refine_points", "all_pbpmap_ubis", "ri_col", "rj_col", "sx_grid", "sy_grid", "mask",
+            "sc", "fc", "eta", "sum_intensity", "sinomega", "cosomega", "omega", "dty", "dtyi",
+            "xpos", "ystep", "y0", "B0")
+    args = [w[k] for k in keys]
+    PP.refine_map(*args, **w["geometry"])  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    U, E, N, Q = PP.refine_map(*args, **w["geometry"])
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    M = w["all_pbpmap_ubis"].shape[2]
+    voxels = int(w["mask"].sum())
+    npeaks = len(w["sc"])
+    # the selection kernel alone (both passes), timed with events
+    dev = torch.device("cuda")
+    T_ = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt,
+                                                     device=dev)
+    pts, ptask, ubis = [], [0], []
+    order = np.lexsort((w["rj_col"], w["ri_col"]))
+    last = None
+    for m in order:
+        u = w["all_pbpmap_ubis"][:, :, m]
+        if np.isnan(u[0, 0]):
+            continue
+        key = (w["ri_col"][m], w["rj_col"][m])
+        if key != last:
+            pts.append((w["sx_grid"][key], w["sy_grid"][key]))
+            ptask.append(ptask[-1])
+            last = key
+        ubis.append(u)
+        ptask[-1] += 1
+    gv = PP._gve_device(T_(w["sc"]), T_(w["fc"]), T_(w["omega"]), T_(w["xpos"]), w["geometry"])
+    dargs = (T_(w["sinomega"]), T_(w["cosomega"]), T_(w["dty"]), gv, T_(np.array(pts)),
+             T_(np.array(ptask, np.int32), torch.int32), T_(np.array(ubis)), w["y0"],
+             w["ystep"], 0.1)
+    assign_ms = _events_ms(lambda: PP.assign_tasks(*dargs), reps=3)
+    out = {"map": "%dx%d" % (NY, NY), "grains": grains, "voxels": voxels, "ubis": M,
+           "peaks": npeaks,
+           "gen_s": gen_s, "gpu_s": gpu_s, "voxels_per_s": voxels / gpu_s,
+           "selection_ms": assign_ms,
+           "selection_voxel_peak_pairs_per_s": len(pts) * npeaks / (assign_ms * 1e-3),
+           "refined": int((~np.isnan(E[0, 0])).sum())}
+    if with_cpu:
+        from oracle import pbp_oracle as P
+        inmask = [tuple(p) for p in w["refine_points"] if w["mask"][tuple(p)]]
+        sub_points = np.array(inmask[:cpu_points])
+        sargs = [sub_points if k == "refine_points" else w[k] for k in keys]
+        t0 = time.perf_counter()
+        Uc, Ec, Nc, Qc = P.refine_map(*sargs, **w["geometry"])
+        cpu_s = time.perf_counter() - t0
+        done = ~np.isnan(Nc)
+        same = bool(np.array_equal(Nc[done], N[done]) and
+                    np.allclose(Uc[:, :, done], U[:, :, done], atol=1e-9))
+        out["cpu"] = {"kind": "port", "threads": 1, "voxels": len(sub_points), "seconds": cpu_s,
+                      "sample": "first %d voxels of the map, the oracle's voxel loop" %
+                                len(sub_points)}
+        out["cpu_s_scaled_to_map"] = cpu_s / len(sub_points) * voxels
+        out["speedup_vs_cpu"] = out["cpu_s_scaled_to_map"] / gpu_s
+        out["sample_results_equal_gpu"] = same
+    return out
+
+
 def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
     """Large scan: compute_gv per grain translation + score_and_assign for
     K grains x N peaks (per-peak dty, 15% spurious) in one fused launch,
@@ -544,7 +620,8 @@ def main():
         out["aux"] = {"refine_5000_grains": aux_refine_adjoint(threads, not args.no_cpu),
                       "scan_20000_grains": aux_scan(threads, not args.no_cpu),
                       "index_3000_grains": aux_index(not args.no_cpu),
-                      "refinegrains_2000_grains": aux_refinegrains(not args.no_cpu)}
+                      "refinegrains_2000_grains": aux_refinegrains(not args.no_cpu),
+                      "pbp_refine_map_101x101": aux_pbp(not args.no_cpu)}
     print(json.dumps(out))
 
 

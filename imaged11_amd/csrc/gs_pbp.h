@@ -86,3 +86,21 @@ __global__ __launch_bounds__(GS_BLOCK) void k_pbp_assign(
         }
     }
 }
+
+// Per-segment column sums, each summed row by row from 0.0 in row order:
+// out[s][c] = sum of v[r][c] for r in [off[s], off[s+1]).  That is the
+// order in which np.bincount(labels, weights) accumulates, so grouped
+// weighted sums (refine_map's merge, :338-356) come out bit for bit as the
+// reference's, and the same on every run.  One thread per (segment,
+// column); consecutive threads read consecutive columns of a row.
+__global__ __launch_bounds__(GS_BLOCK) void k_segment_sum(const double* __restrict__ v, int C,
+                                                          const long long* __restrict__ off,
+                                                          int S, double* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
+    if (i >= (long long)S * C) return;
+    const long long s = i / C;
+    const int c = (int)(i - s * C);
+    double acc = 0.0;
+    for (long long r = off[s]; r < off[s + 1]; r++) acc += v[r * C + c];
+    out[i] = acc;
+}

@@ -1054,6 +1054,19 @@ int gs_pbp_assign(const double* so, const double* co, const double* dty, const d
     return GS_OK;
 }
 
+int gs_segment_sum(const double* v, int C, const long long* off, int nseg, double* out,
+                   gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (C < 0 || nseg < 0) return set_err(GS_ERR_ARG, "gs_segment_sum: negative size");
+    if ((long long)C * nseg == 0) return GS_OK;
+    if (!v || !off || !out) return set_err(GS_ERR_ARG, "gs_segment_sum: NULL array");
+    hipLaunchKernelGGL(k_segment_sum, dim3(blocks_for((long long)C * nseg, GS_BLOCK)),
+                       dim3(GS_BLOCK), 0, S(stream), v, C, off, nseg, out);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
 // ---- column-file geometry ----------------------------------------------------
 
 int gs_compute_geometry_dev(const double* xlylzl, const double* omega, double omegasign,

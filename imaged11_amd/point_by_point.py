@@ -173,9 +173,9 @@ def _gve_device(sc, fc, omega, xpos, geo):
 def assign_tasks(so, co, dty, gv, pts, ptask, ubis, y0, ystep, tol):
     """gs_pbp_assign, both passes: for every task t (tasks grouped by voxel:
     ptask (P+1,) offsets over the voxels pts (P,2)), the peaks local to its
-    voxel and assigned to its UBI.  Device tensors in; returns (task_off
-    (T+1,) int64 numpy, peak (E,) int64 numpy, hkl (3, E) int32 numpy),
-    peaks ascending within each task."""
+    voxel and assigned to its UBI.  Device tensors in; returns device
+    tensors (task_off (T+1,) int64, task (E,) int64, peak (E,) int64, hkl
+    (E,3) int64), ordered by task, then peak index."""
     lib = _lib.load()
     lib.gs_clear_error()
     dev = gv.device
@@ -187,33 +187,82 @@ def assign_tasks(so, co, dty, gv, pts, ptask, ubis, y0, ystep, tol):
                                  dptr(ptask), P, dptr(ubis), T, float(y0), float(ystep),
                                  float(tol), dptr(counts), None, None, None, None, stream_ptr()),
                "pbp_assign count")
-    cnt = counts.cpu().numpy().astype(np.int64)
-    off = np.zeros(T + 1, np.int64)
-    off[1:] = np.cumsum(cnt)
+    off = torch.zeros(T + 1, dtype=torch.int64, device=dev)
+    off[1:] = torch.cumsum(counts.long(), 0)
     E = int(off[-1])
     peak = torch.empty(max(E, 1), dtype=torch.int32, device=dev)
     hkl = torch.empty((max(E, 1), 3), dtype=torch.int32, device=dev)
     cursor = torch.empty(T, dtype=torch.int32, device=dev)
-    off_d = _t(off, dev, torch.int64)
     _lib.check(lib.gs_pbp_assign(dptr(so), dptr(co), dptr(dty), dptr(gv), n, dptr(pts),
                                  dptr(ptask), P, dptr(ubis), T, float(y0), float(ystep),
-                                 float(tol), dptr(counts), dptr(off_d), dptr(cursor), dptr(peak),
+                                 float(tol), dptr(counts), dptr(off), dptr(cursor), dptr(peak),
                                  dptr(hkl), stream_ptr()), "pbp_assign fill")
-    pk = peak[:E].cpu().numpy().astype(np.int64)
-    hk = hkl[:E].cpu().numpy()
-    task = np.repeat(np.arange(T), cnt)
-    order = np.lexsort((pk, task))
-    return off, pk[order], np.ascontiguousarray(hk[order].T)
+    # ascending peak index inside each task: sort (task, peak) keys
+    task = torch.repeat_interleave(torch.arange(T, device=dev), counts.long(), output_size=E)
+    _, order = torch.sort(task * max(n, 1) + peak[:E].long())
+    return off, task[order], peak[:E].long()[order], hkl[:E].long()[order]
 
 
-# ---------------------------------------------------------------------------
-# batched small linear algebra on per-task groups of rows
+def segment_sum(v, off):
+    """Column sums of the row segments off[s]..off[s+1] of v (rows, C),
+    row by row in order (gs_segment_sum: np.bincount's order)."""
+    lib = _lib.load()
+    lib.gs_clear_error()
+    v = v.contiguous()
+    S = off.numel() - 1
+    C = v.shape[1]
+    out = torch.empty((S, C), dtype=torch.float64, device=v.device)
+    _lib.check(lib.gs_segment_sum(dptr(v), C, dptr(off), S, dptr(out), stream_ptr()),
+               "segment_sum")
+    return out
 
 
-def _segment_sum(task, values, T):
-    """Per-task sums of each column of values (rows, k) in row order."""
-    return np.stack([np.bincount(task, weights=values[:, k], minlength=T)
-                     for k in range(values.shape[1])], -1)
+def _offsets(seg, S):
+    """CSR offsets of rows already ordered by segment id seg (0..S-1)."""
+    off = torch.zeros(S + 1, dtype=torch.int64, device=seg.device)
+    off[1:] = torch.cumsum(torch.bincount(seg, minlength=S), 0)
+    return off
+
+
+def _pack(cols):
+    """One int64 key per row ordering rows as the tuples of cols do
+    (lexicographic, first column most significant), or None when the value
+    ranges need more than 62 bits."""
+    lo = torch.stack([c.min() for c in cols])
+    hi = torch.stack([c.max() for c in cols])
+    lo_h, hi_h = lo.cpu().numpy(), hi.cpu().numpy()
+    bits = [max(1, int(h - l).bit_length()) for l, h in zip(lo_h, hi_h)]
+    if sum(bits) > 62:
+        return None
+    key = torch.zeros_like(cols[0])
+    for c, l, b in zip(cols, lo_h, bits):
+        key = (key << b) | (c - int(l))
+    return key
+
+
+def _groups(cols):
+    """Group ids (rows with equal tuples share one, ids in sorted tuple
+    order) and the number of groups."""
+    key = _pack(cols)
+    if key is None:
+        _, inv = torch.unique(torch.stack(cols, 1), dim=0, sorted=True, return_inverse=True)
+    else:
+        _, inv = torch.unique(key, sorted=True, return_inverse=True)
+    G = int(inv.max()) + 1 if inv.numel() else 0
+    return inv, G
+
+
+def _rows_dot(u, g):
+    """h = u.g per row, written out in row order."""
+    return torch.stack([u[:, i, 0] * g[:, 0] + u[:, i, 1] * g[:, 1] + u[:, i, 2] * g[:, 2]
+                        for i in range(3)], 1)
+
+
+def _score(u, g, tol):
+    h = _rows_dot(u, g)
+    hi = torch.round(h)
+    e = h - hi
+    return hi.long(), ((e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]) + e[:, 2] * e[:, 2]) < tol * tol
 
 
 def _lstsq_rank(ata, n_rows):
@@ -236,12 +285,6 @@ def _solve_normal(ata, atb):
     return out
 
 
-def _sym_outer(x, y=None):
-    """Per-row products x_i y_j (rows, 3, 3) flattened to (rows, 9)."""
-    y = x if y is None else y
-    return (x[:, :, None] * y[:, None, :]).reshape(len(x), -1)
-
-
 def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid, mask,
                sc, fc, eta, sum_intensity, sinomega, cosomega, omega, dty, dtyi, xpos,
                ystep, y0, B0, distance, y_center, y_size, tilt_y, z_center, z_size, tilt_z,
@@ -257,16 +300,12 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
                o21=o21, o22=o22, t_x=t_x, t_y=t_y, t_z=t_z, wedge=wedge, chi=chi,
                wavelength=wavelength)
     M = all_pbpmap_ubis.shape[2]
-    final_ubis = np.full_like(all_pbpmap_ubis, np.nan, dtype=np.float64)
-    final_eps = np.full_like(all_pbpmap_ubis, np.nan, dtype=np.float64)
+    final_ubis = np.full(all_pbpmap_ubis.shape, np.nan)
+    final_eps = np.full(all_pbpmap_ubis.shape, np.nan)
     final_npks = np.full(M, np.nan)
     final_nuniq = np.full(M, np.nan)
-    sc, fc, eta, sum_intensity = (np.asarray(a, np.float64) for a in (sc, fc, eta, sum_intensity))
-    sinomega, cosomega, omega, dty, xpos = (np.asarray(a, np.float64)
-                                            for a in (sinomega, cosomega, omega, dty, xpos))
-    dtyi = np.asarray(dtyi)
     ri_col, rj_col = np.asarray(ri_col), np.asarray(rj_col)
-    # ---- tasks: (voxel, UBI) pairs in the reference's visiting order
+    # ---- tasks: (voxel, UBI) pairs of the voxels to refine
     by_point = {}
     for m in range(M):
         by_point.setdefault((int(ri_col[m]), int(rj_col[m])), []).append(m)
@@ -275,8 +314,8 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
         ri, rj = int(ri), int(rj)
         if not mask[ri, rj]:
             continue
-        here = by_point.get((ri, rj), [])
-        valid = [m for m in here if not np.isnan(all_pbpmap_ubis[0, 0, m])]
+        valid = [m for m in by_point.get((ri, rj), [])
+                 if not np.isnan(all_pbpmap_ubis[0, 0, m])]
         if not valid:
             continue
         pts.append((sx_grid[ri, rj], sy_grid[ri, rj]))
@@ -286,139 +325,139 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
     if T == 0:
         return final_ubis, final_eps, final_npks, final_nuniq
     tmap = np.array(tmap, np.int64)
-    pts = np.array(pts, np.float64)
-    ptask = np.array(ptask, np.int32)
     tpoint = np.repeat(np.arange(len(pts)), np.diff(ptask))
-    ubis = np.ascontiguousarray(np.moveaxis(all_pbpmap_ubis[:, :, tmap], -1, 0))
-    # ---- GPU: g-vectors of every peak, selection + first scoring
-    gv_all = _gve_device(_t(sc, dev), _t(fc, dev), _t(omega, dev), _t(xpos, dev), geo)
-    off, peak, hkl = assign_tasks(_t(sinomega, dev), _t(cosomega, dev), _t(dty, dev), gv_all,
-                                  _t(pts, dev), _t(ptask, dev, torch.int32), _t(ubis, dev), y0,
-                                  ystep, tol)
-    E = len(peak)
-    task = np.repeat(np.arange(T), np.diff(off))
-    # ---- merge by (task, h, k, l, etasign, dtyi); groups in sorted key order
-    etas = np.sign(eta[peak])
-    keys = (dtyi[peak], etas, hkl[2], hkl[1], hkl[0], task)
-    order = np.lexsort(keys)
-    ks = np.stack([k[order] for k in keys[::-1]])
-    new = np.ones(E, bool)
-    new[1:] = np.any(ks[:, 1:] != ks[:, :-1], axis=0)
-    gid_sorted = np.cumsum(new) - 1
-    gid = np.empty(E, np.int64)
-    gid[order] = gid_sorted
-    G = int(gid_sorted[-1]) + 1 if E else 0
-    gtask = np.zeros(G, np.int64)
-    gtask[gid] = task
-    ghkl = np.zeros((3, G), np.int64)
-    ghkl[:, gid] = hkl
-    wI = sum_intensity[peak]
-    sI = np.bincount(gid, weights=wI, minlength=G)
-    mean = lambda x: np.bincount(gid, weights=x[peak] * wI, minlength=G) / sI
-    m_sc, m_fc, m_om, m_dty, m_xpos, m_eta = (mean(x) for x in (sc, fc, omega, dty, xpos, eta))
+    ubis_h = np.ascontiguousarray(np.moveaxis(all_pbpmap_ubis[:, :, tmap], -1, 0), np.float64)
+    D = lambda a, dt=torch.float64: _t(a, dev, dt)
+    sc_d, fc_d, om_d, xpos_d = D(sc), D(fc), D(omega), D(xpos)
+    eta_d, si_d, dty_d = D(eta), D(sum_intensity), D(dty)
+    dtyi_d = D(dtyi, torch.int64)
+    pts_d, ubis_d = D(np.array(pts)), D(ubis_h)
+    tx_d, ty_d = pts_d[D(tpoint, torch.int64), 0], pts_d[D(tpoint, torch.int64), 1]
+    # ---- selection and first scoring of every task over every peak
+    gv_all = _gve_device(sc_d, fc_d, om_d, xpos_d, geo)
+    _, task, peak, hkl = assign_tasks(D(sinomega), D(cosomega), dty_d, gv_all, pts_d,
+                                      D(np.array(ptask, np.int32), torch.int32), ubis_d, y0,
+                                      ystep, tol)
+    # ---- merge by (task, h, k, l, eta sign, dtyi): groups in sorted key
+    # order, sums in row (= ascending peak) order, as np.bincount per voxel
+    gid, G = _groups([task, hkl[:, 0], hkl[:, 1], hkl[:, 2],
+                      torch.sign(eta_d[peak]).long(), dtyi_d[peak]])
+    perm = torch.sort(gid, stable=True).indices
+    goff = _offsets(gid, G)
+    wI = si_d[peak]
+    cols = torch.stack([wI] + [x[peak] * wI for x in (sc_d, fc_d, om_d, dty_d, xpos_d, eta_d)],
+                       1)[perm]
+    sums = segment_sum(cols, goff)
+    sI = sums[:, 0]
+    m_sc, m_fc, m_om, m_dty, m_xpos, m_eta = (sums[:, k] / sI for k in range(1, 7))
+    gtask = task[perm][goff[:-1]]
     # ---- re-selection of the merged peaks for the voxel, merged g-vectors
-    gpt = pts[tpoint[gtask]]
-    yd = np.abs(y0 - gpt[:, 0] * np.sin(np.radians(m_om)) - gpt[:, 1] *
-                np.cos(np.radians(m_om)) - m_dty)
-    keep = np.nonzero(yd <= ystep)[0]
-    rtask = gtask[keep]
-    rydist = yd[keep]
-    reta = m_eta[keep]
-    g = _gve_device(_t(m_sc[keep], dev), _t(m_fc[keep], dev), _t(m_om[keep], dev),
-                    _t(m_xpos[keep], dev), geo).cpu().numpy()
-
-    def score(u):
-        h = np.einsum("rij,rj->ri", u, g)
-        hi = np.round(h).astype(np.int32)
-        e = h - hi
-        return hi, (e ** 2).sum(axis=1) < merge_tol ** 2
-    hi, sel = score(ubis[rtask])
-    npks = np.bincount(rtask, weights=sel, minlength=T).astype(np.int64)
+    rad = m_om * (np.pi / 180.0)
+    yd = torch.abs(y0 - tx_d[gtask] * torch.sin(rad) - ty_d[gtask] * torch.cos(rad) - m_dty)
+    keep = torch.nonzero(yd <= ystep).flatten()
+    rtask, rydist, reta = gtask[keep], yd[keep], m_eta[keep]
+    g = _gve_device(m_sc[keep].contiguous(), m_fc[keep].contiguous(), m_om[keep].contiguous(),
+                    m_xpos[keep].contiguous(), geo)
+    hi, sel = _score(ubis_d[rtask], g, merge_tol)
+    npks = torch.bincount(rtask[sel], minlength=T)
     fit = npks > min_grain_npks
-    # ---- weighted least-squares UBI for the tasks with enough peaks
-    rows = sel & fit[rtask]
-    w = 1.0 / (rydist + 1)
-    wg = w[:, None] * g
-    wh = w[:, None] * hi.astype(np.float64)
+    # ---- weighted least-squares UBI of every task with enough peaks
+    rows = torch.nonzero(sel & fit[rtask]).flatten()
     rt = rtask[rows]
-    ata = _segment_sum(rt, _sym_outer(wg[rows]), T).reshape(T, 3, 3)
-    atb = _segment_sum(rt, _sym_outer(wg[rows], wh[rows]), T).reshape(T, 3, 3)
+    w = 1.0 / (rydist[rows] + 1)
+    wg = w[:, None] * g[rows]
+    wh = w[:, None] * hi[rows].double()
+    prods = torch.cat([(wg[:, :, None] * wg[:, None, :]).reshape(-1, 9),
+                       (wg[:, :, None] * wh[:, None, :]).reshape(-1, 9)], 1)
+    nsum = segment_sum(prods, _offsets(rt, T)).cpu().numpy()
+    ata, atb = nsum[:, :9].reshape(T, 3, 3), nsum[:, 9:].reshape(T, 3, 3)
+    fit_h = fit.cpu().numpy()
+    npks_h = npks.cpu().numpy()
     ubifit = np.full((T, 3, 3), np.nan)
-    ubifit[fit] = np.swapaxes(_solve_normal(ata[fit], atb[fit]), -1, -2)
+    ubifit[fit_h] = np.swapaxes(_solve_normal(ata[fit_h], atb[fit_h]), -1, -2)
     rank = np.zeros(T, np.int64)
-    rank[fit] = _lstsq_rank(ata[fit], npks[fit])
-    worth = fit & (rank == 3) & np.all(np.isfinite(ubifit), axis=(1, 2))
+    rank[fit_h] = _lstsq_rank(ata[fit_h], npks_h[fit_h])
+    worth = fit_h & (rank == 3) & np.all(np.isfinite(ubifit), axis=(1, 2))
     if worth.any():
         u = ubifit[worth]
         good = (np.linalg.cond(u) < 1e14) & (np.linalg.det(u) > 0) & \
             (np.linalg.matrix_rank(u) == 3)
         worth[np.nonzero(worth)[0][~good]] = False
-    ubi_out = np.where(worth[:, None, None], ubifit, ubis)
+    ubi_out = np.where(worth[:, None, None], ubifit, ubis_h)
     # ---- re-score with the fitted UBIs, count distinct reflections
-    hi2, sel2 = score(ubi_out[rtask])
-    npk2 = np.bincount(rtask, weights=sel2, minlength=T).astype(np.int64)
-    r2 = sel2 & fit[rtask]
-    ukeys = np.stack([rtask[r2], hi2[r2, 0], hi2[r2, 1], hi2[r2, 2],
-                      np.sign(reta[r2]).astype(np.int64)])
-    nuniq = np.bincount(np.unique(ukeys, axis=1)[0], minlength=T) if r2.any() else \
-        np.zeros(T, np.int64)
-    for t in np.nonzero(fit)[0]:
+    hi2, sel2 = _score(D(ubi_out)[rtask], g, merge_tol)
+    npk2 = torch.bincount(rtask[sel2], minlength=T).cpu().numpy()
+    r2 = torch.nonzero(sel2 & fit[rtask]).flatten()
+    nuniq = np.zeros(T, np.int64)
+    if r2.numel():
+        uid, U_ = _groups([rtask[r2], hi2[r2, 0], hi2[r2, 1], hi2[r2, 2],
+                           torch.sign(reta[r2]).long()])
+        first = torch.zeros(U_, dtype=torch.int64, device=dev).scatter_(0, uid, rtask[r2])
+        nuniq = torch.bincount(first, minlength=T).cpu().numpy()
+    for t in np.nonzero(fit_h)[0]:
         m = tmap[t]
         final_ubis[:, :, m] = ubi_out[t]
         final_npks[m] = npk2[t]
         final_nuniq[m] = nuniq[t]
     # ---- strain tensors of the well-fitted tasks
     if worth.any():
-        eps = _strain_fit(worth, ubifit, rtask, sel2, g, hi2, rydist, B0, gnoise_std, T)
+        eps = _strain_fit(D(worth, torch.bool), ubifit, worth, rtask, sel2, g, hi2, rydist, B0,
+                          gnoise_std, T)
         for t in np.nonzero(worth)[0]:
             final_eps[:, :, tmap[t]] = eps[t]
     return final_ubis, final_eps, final_npks, final_nuniq
 
 
-def _strain_fit(worth, ubifit, rtask, sel2, g, hi2, rydist, B0, gnoise_std, T):
+def _strain_fit(worth_d, ubifit, worth, rtask, sel2, g, hi2, rydist, B0, gnoise_std, T):
     """The directional-strain fit of refine_map (:1624-1668) for every task
-    in ``worth`` at once."""
-    rows = sel2 & worth[rtask]
+    in ``worth`` at once: per-row terms on the device, per-task sums by
+    gs_segment_sum, the 6x6 solves on the host."""
+    dev = g.device
+    rows = torch.nonzero(sel2 & worth_d[rtask]).flatten()
     t = rtask[rows]
     gg = g[rows]
-    hk = hi2[rows].astype(np.float64)
-    ucell = ubi_to_unitcell(ubifit[worth])
+    hk = hi2[rows].double()
     U = np.full((T, 3, 3), np.nan)
-    U[worth] = ubi_and_ucell_to_u(ubifit[worth], ucell)
-    UB0 = np.matmul(U, B0)
-    gve0 = np.einsum("rij,rj->ri", UB0[t], hk)
-    gTg0 = np.sum(gg * gve0, axis=1)
-    gTg = np.sum(gg * gg, axis=1)
-    strain = (gTg0 / gTg) - 1
-    kap = gg / np.sqrt(np.sum(gg * gg, axis=1))[:, None]
-    kx, ky, kz = kap.T
-    Mrow = np.stack((kx * kx, ky * ky, kz * kz, 2 * kx * ky, 2 * kx * kz, 2 * ky * kz), 1)
+    U[worth] = ubi_and_ucell_to_u(ubifit[worth], ubi_to_unitcell(ubifit[worth]))
+    UB0 = _t(np.matmul(U, B0), dev)[t]
+    gve0 = _rows_dot(UB0, hk)
+    gTg0 = (gg[:, 0] * gve0[:, 0] + gg[:, 1] * gve0[:, 1]) + gg[:, 2] * gve0[:, 2]
+    gTg = (gg[:, 0] * gg[:, 0] + gg[:, 1] * gg[:, 1]) + gg[:, 2] * gg[:, 2]
+    strain = gTg0 / gTg - 1
+    kap = gg / torch.sqrt(gTg)[:, None]
+    kx, ky, kz = kap[:, 0], kap[:, 1], kap[:, 2]
+    Mrow = torch.stack((kx * kx, ky * ky, kz * kz, 2 * kx * ky, 2 * kx * kz, 2 * ky * kz), 1)
     w = 1. / (rydist[rows] + 1)
-    a = np.sum(gve0 * (gnoise_std ** 2) * gve0, axis=1)
-    noise = np.sqrt(divide_where(a, gTg ** 2, out=np.ones_like(gTg), wherearr=gTg))
+    a = (gve0[:, 0] * (gnoise_std ** 2) * gve0[:, 0] + gve0[:, 1] * (gnoise_std ** 2) *
+         gve0[:, 1]) + gve0[:, 2] * (gnoise_std ** 2) * gve0[:, 2]
+    noise = torch.sqrt(torch.where(gTg != 0, a / gTg ** 2, torch.ones_like(gTg)))
     w = w * (1. / noise)
-    n = np.bincount(t, minlength=T).astype(np.float64)
-    mean = np.bincount(t, weights=strain, minlength=T) / np.maximum(n, 1)
-    std = np.sqrt(np.bincount(t, weights=(strain - mean[t]) ** 2, minlength=T) / np.maximum(n, 1))
-    w = np.where(strain > mean[t] + std[t] * 3.5, 0.0, w)
-    w = np.where(strain < mean[t] - std[t] * 3.5, 0.0, w)
-    wmax = np.zeros(T)
-    np.maximum.at(wmax, t, w)
+    off = _offsets(t, T)
+    n = (off[1:] - off[:-1]).double()
+    mean = segment_sum(strain[:, None], off)[:, 0] / torch.clamp(n, min=1)
+    std = torch.sqrt(segment_sum(((strain - mean[t]) ** 2)[:, None], off)[:, 0] /
+                     torch.clamp(n, min=1))
+    w = torch.where(strain > mean[t] + std[t] * 3.5, torch.zeros_like(w), w)
+    w = torch.where(strain < mean[t] - std[t] * 3.5, torch.zeros_like(w), w)
+    wmax = torch.zeros(T, dtype=torch.float64, device=dev).scatter_reduce(0, t, w, "amax")
     w = w / wmax[t]
     aM = w[:, None] * Mrow
     b = w * strain
-    ata = _segment_sum(t, (aM[:, :, None] * aM[:, None, :]).reshape(len(t), 36), T)
-    atb = _segment_sum(t, aM * b[:, None], T)
-    ata = ata.reshape(T, 6, 6)
+    s = segment_sum(torch.cat([(aM[:, :, None] * aM[:, None, :]).reshape(-1, 36),
+                               aM * b[:, None]], 1), off).cpu().numpy()
+    ata, atb = s[:, :36].reshape(T, 6, 6), s[:, 36:]
     e = np.full((T, 6), np.nan)
     idx = np.nonzero(worth)[0]
     sol = _solve_normal(ata[idx], atb[idx][:, :, None])[:, :, 0]
     e[idx] = sol
-    # rank-deficient tasks: the minimum-norm solution, as lstsq gives it
-    for k, tt in enumerate(idx):
-        if not np.all(np.isfinite(sol[k])):
-            r = t == tt
-            e[tt] = np.linalg.lstsq(aM[r], b[r], rcond=np.finfo(float).eps * max(r.sum(), 6))[0]
+    bad = idx[~np.all(np.isfinite(sol), axis=1)]
+    if len(bad):
+        # rank-deficient tasks: the minimum-norm solution, as lstsq gives it
+        aM_h, b_h, t_h = aM.cpu().numpy(), b.cpu().numpy(), t.cpu().numpy()
+        for tt in bad:
+            r = t_h == tt
+            e[tt] = np.linalg.lstsq(aM_h[r], b_h[r],
+                                    rcond=np.finfo(float).eps * max(r.sum(), 6))[0]
     sxx, syy, szz, sxy, sxz, syz = e.T
     return np.stack([np.stack([sxx, sxy, sxz], -1), np.stack([sxy, syy, syz], -1),
                      np.stack([sxz, syz, szz], -1)], -2)

@@ -445,13 +445,14 @@ def refinegrains_workload(K=20, N=20000, seed=7, wedge=0.0, chi=0.0, translation
             "cells": cells, "peaks": peaks}
 
 
-def _voxel_spots(ub, sx, sy, pars, gmax, ystep, y0, omega_range=(0.0, 180.0)):
-    """Spots of one voxel at sample position (sx, sy) in a scanning
-    experiment: at rotation omega the voxel sits at lab (lx, ly) =
+def _grain_scan_spots(ub, sx, sy, pars, gmax, ystep, y0, omega_range=(0.0, 180.0)):
+    """Spots of the voxels (sx, sy) (arrays) of one grain in a scanning
+    experiment: at rotation omega a voxel sits at lab (lx, ly) =
     (sx cos - sy sin, sx sin + sy cos); the stage translation dty (on a grid
     of step ystep) that brings it into the beam is the grid value nearest
     y0 - ly, and the scattering origin is (lx, ly + dty - y0, 0).  Returns
-    rows (sc, fc, omega, dty, dtyi, xpos=lx)."""
+    rows (sc, fc, omega, dty, dtyi, xpos=lx, reflection, solution), the
+    last two identifying the spot (shared by every voxel of the grain)."""
     p, r, dist = geometry.pack_xlylzl(pars)
     wvln = pars["wavelength"]
     inv_len = np.linalg.norm(np.linalg.inv(ub), axis=1)
@@ -466,83 +467,115 @@ def _voxel_spots(ub, sx, sy, pars, gmax, ystep, y0, omega_range=(0.0, 180.0)):
     A, B, C = g[:, 0], -g[:, 1], -0.5 * wvln * gg
     R = np.hypot(A, B)
     ok = (R > 0) & (np.abs(C) < R)
+    refl = np.nonzero(ok)[0]
     g, A, B, C, R = g[ok], A[ok], B[ok], C[ok], R[ok]
     phi = np.arctan2(B, A)
     acos = np.arccos(C / R)
     R3 = r.reshape(3, 3)
+    sx, sy = np.asarray(sx, float)[:, None], np.asarray(sy, float)[:, None]
     rows = []
     for sgn in (1.0, -1.0):
         om = np.degrees(phi + sgn * acos)
         om = (om - omega_range[0]) % 360.0 + omega_range[0]
         inr = (om >= omega_range[0]) & (om < omega_range[1])
-        gi, om = g[inr], om[inr]
+        gi, om, ri = g[inr], om[inr], refl[inr]
         th = np.radians(om)
         co, so = np.cos(th), np.sin(th)
         k = np.stack([co * gi[:, 0] - so * gi[:, 1], so * gi[:, 0] + co * gi[:, 1], gi[:, 2]], 1)
         nhat = wvln * k
         nhat[:, 0] += 1.0
-        lx, ly = sx * co - sy * so, sx * so + sy * co
+        # origin + s nhat = dist + v1 c1 + v2 c2 for every (voxel, spot):
+        # one 3x3 matrix per spot, one right-hand side per voxel
+        Mi = np.empty((len(nhat), 3, 3))
+        Mi[:, :, 0] = nhat
+        Mi[:, :, 1] = -R3[:, 1][None, :]
+        Mi[:, :, 2] = -R3[:, 2][None, :]
+        Mi = np.linalg.inv(Mi)
+        lx, ly = sx * co - sy * so, sx * so + sy * co            # (V, S)
         dtyi = np.round((y0 - ly) / ystep)
         dty = dtyi * ystep
-        o = np.stack([lx, ly + dty - y0, np.zeros_like(lx)], 1)
-        M = np.empty((len(nhat), 3, 3))
-        M[:, :, 0] = nhat
-        M[:, :, 1] = -R3[:, 1][None, :]
-        M[:, :, 2] = -R3[:, 2][None, :]
-        sol = np.linalg.solve(M, (dist[None, :] - o)[:, :, None])[:, :, 0]
-        fc = sol[:, 1] / p[3] + p[1]
-        sc = sol[:, 2] / p[2] + p[0]
-        good = (sol[:, 0] > 0) & (sc >= 0) & (sc < 2048) & (fc >= 0) & (fc < 2048)
-        rows.append(np.stack([sc, fc, om, dty, dtyi, lx], 1)[good])
-    return np.concatenate(rows) if rows else np.zeros((0, 6))
+        rhs = np.stack([dist[0] - lx, dist[1] - (ly + dty - y0), np.full_like(lx, dist[2])], -1)
+        sol = np.einsum("sij,vsj->vsi", Mi, rhs)
+        fc = sol[..., 1] / p[3] + p[1]
+        sc = sol[..., 2] / p[2] + p[0]
+        good = (sol[..., 0] > 0) & (sc >= 0) & (sc < 2048) & (fc >= 0) & (fc < 2048)
+        full = lambda a: np.broadcast_to(a, sc.shape)
+        rows.append(np.stack([sc, fc, full(om), dty, dtyi, lx, full(ri.astype(float)),
+                              np.full(sc.shape, sgn)], -1)[good])
+    return np.concatenate(rows) if rows else np.zeros((0, 8))
 
 
-def pbp_workload(NY=9, ystep=4.0, seed=31, a=4.05, gmax=0.9, strain=1e-3, extra=0.25):
+def pbp_workload(NY=9, ystep=4.0, seed=31, a=4.05, gmax=0.9, strain=1e-3, extra=0.25,
+                 grains=2):
     """A point-by-point refinement input: an NY x NY sample grid (step
-    ystep), a disc of voxels shared by two fcc-like cubic grains (left and
-    right halves; the right one strained by a symmetric tensor of size
-    ``strain``), every voxel's spots with its stage position dty and origin
-    x position xpos.  The per-voxel UBIs to refine are the grain's exact
-    UBI rotated by ~1e-3 rad; a fraction ``extra`` of voxels also carry the
-    other grain's UBI, and one carries a NaN entry.  Returns a dict with
-    every argument refine_map takes (peak columns, map columns, grids,
+    ystep) with a disc of voxels shared by cubic grains (two: the left and
+    right halves; more: the Voronoi cells of random seeds), every grain but
+    the first strained by a symmetric tensor of size ``strain``, and the
+    spots of every voxel with the stage position dty that puts it in the
+    beam.  Voxels of one grain on the same ray (same reflection, omega and
+    dty) make one detector peak, as on a real detector: their spots are
+    summed (sum_intensity ~ voxel count) at the mean position, xpos the mean
+    origin x.  The per-voxel UBIs to refine are the grain's exact UBI
+    rotated by ~1e-3 rad; a fraction ``extra`` of voxels also carry a
+    neighbouring grain's UBI, and one carries a NaN entry.  Returns a dict
+    with every argument refine_map takes (peak columns, map columns, grids,
     mask, B0 and the detector parameters) plus the truth."""
     rng = np.random.default_rng(seed)
     pars = detector_parameters(rng)
     pars.update(distance=150000.0, wedge=0.0, chi=0.0, t_x=0.0, t_y=0.0, t_z=0.0)
-    U = random_rotations(2, rng)
+    U = random_rotations(grains, rng)
     B0 = np.eye(3) / a
-    eps = rng.normal(scale=strain, size=(3, 3))
-    eps = 0.5 * (eps + eps.T)
-    ubs = [U[0] @ B0, np.linalg.inv(np.eye(3) + eps).T @ U[1] @ B0]
+    epss, ubs = [], []
+    for k in range(grains):
+        e = rng.normal(scale=strain, size=(3, 3)) if k > 0 else np.zeros((3, 3))
+        e = 0.5 * (e + e.T)
+        epss.append(e)
+        ubs.append(np.linalg.inv(np.eye(3) + e).T @ U[k] @ B0)
     c = NY // 2
     ii, jj = np.meshgrid(np.arange(NY), np.arange(NY), indexing="ij")
     sx_grid = (ii - c) * ystep
     sy_grid = (jj - c) * ystep
     mask = (ii - c) ** 2 + (jj - c) ** 2 <= c * c
-    owner = np.where(jj < c, 0, 1)
-    rows, ubis, ri_col, rj_col = [], [], [], []
+    if grains == 2:
+        owner = np.where(jj < c, 0, 1)
+    else:
+        seeds = rng.uniform(-c, c, size=(grains, 2))
+        d2 = (ii[..., None] - c - seeds[:, 0]) ** 2 + (jj[..., None] - c - seeds[:, 1]) ** 2
+        owner = np.argmin(d2, axis=-1)
+    rows = []
+    for k in range(grains):
+        sel = mask & (owner == k)
+        if not sel.any():
+            continue
+        sp = _grain_scan_spots(ubs[k], sx_grid[sel], sy_grid[sel], pars, gmax, ystep, 0.0)
+        rows.append(np.concatenate([sp, np.full((len(sp), 1), k)], 1))
+    ubis, ri_col, rj_col = [], [], []
     for i in range(NY):
         for j in range(NY):
             if not mask[i, j]:
                 continue
             g = owner[i, j]
-            rows.append(_voxel_spots(ubs[g], sx_grid[i, j], sy_grid[i, j], pars, gmax, ystep,
-                                     0.0))
             w = rng.normal(scale=1e-3, size=3)
             Wx = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
-            for gg in ([g, 1 - g] if rng.random() < extra else [g]):
+            other = (g + 1) % grains
+            for gg in ([g, other] if rng.random() < extra else [g]):
                 ubis.append(np.linalg.inv((np.eye(3) + Wx) @ ubs[gg]))
                 ri_col.append(i)
                 rj_col.append(j)
     ubis.append(np.full((3, 3), np.nan))
     ri_col.append(c)
     rj_col.append(c)
-    pk = np.concatenate(rows)
-    n = len(pk)
-    sc, fc, om, dty, dtyi, xpos = (np.ascontiguousarray(pk[:, k]) for k in range(6))
+    spots = np.concatenate(rows)
+    # one detector peak per (grain, reflection, solution, dtyi)
+    _, lab = np.unique(spots[:, [8, 6, 7, 4]], axis=0, return_inverse=True)
+    lab = lab.ravel()
+    cnt = np.bincount(lab).astype(float)
+    mean = lambda k: np.bincount(lab, weights=spots[:, k]) / cnt
+    sc, fc, om, dty, dtyi, xpos = (mean(k) for k in range(6))
+    n = len(cnt)
     sc = sc + rng.normal(scale=0.05, size=n)
     fc = fc + rng.normal(scale=0.05, size=n)
+    dtyi = np.round(dtyi)
     pd, r, dist = geometry.pack_xlylzl(pars)
     v1 = (fc - pd[1]) * pd[3]
     v2 = (sc - pd[0]) * pd[2]
@@ -558,7 +591,9 @@ def pbp_workload(NY=9, ystep=4.0, seed=31, a=4.05, gmax=0.9, strain=1e-3, extra=
         "all_pbpmap_ubis": np.ascontiguousarray(np.moveaxis(np.array(ubis), 0, -1)),
         "ri_col": np.array(ri_col, np.int64), "rj_col": np.array(rj_col, np.int64),
         "sx_grid": sx_grid.astype(float), "sy_grid": sy_grid.astype(float), "mask": mask,
-        "sc": sc, "fc": fc, "eta": eta, "sum_intensity": rng.uniform(1.0, 10.0, n),
+        "sc": sc, "fc": fc, "eta": eta, "sum_intensity": cnt * rng.uniform(0.9, 1.1, n),
         "sinomega": np.sin(np.radians(om)), "cosomega": np.cos(np.radians(om)), "omega": om,
         "dty": dty, "dtyi": dtyi.astype(np.int64), "xpos": xpos, "ystep": ystep, "y0": 0.0,
-        "B0": B0, "geometry": geo, "truth": {"ubs": ubs, "eps": eps, "owner": owner}}
+        "B0": B0, "geometry": geo,
+        "truth": {"ubs": ubs, "eps": epss[1] if grains == 2 else epss, "epss": epss,
+                  "owner": owner}}

@@ -291,6 +291,14 @@ int gs_pbp_assign(const double *so, const double *co, const double *dty, const d
                   double y0, double ystep, double tol, int *counts, const long long *task_off,
                   int *cursor, int *out_peak, int *out_hkl, gs_stream_t stream);
 
+/* Per-segment column sums of a row-major (rows, C) array: out (S, C),
+ * out[s][c] = the sum of rows off[s] .. off[s+1]-1 of column c, added one
+ * row at a time from 0.0 (np.bincount's order: refine_map's merge,
+ * ImageD11/sinograms/point_by_point.py:338-356).  Device pointers; off is
+ * (S+1) int64. */
+int gs_segment_sum(const double *v, int C, const long long *off, int S, double *out,
+                   gs_stream_t stream);
+
 /* ---- column-file geometry ---- */
 
 /* compute_geometry (src/cdiffraction.c:31-94; f2py src/_cImageD11.pyf:534-549):

@@ -51,9 +51,10 @@ def test_assign_tasks_matches_oracle(dev, wl):
             ubis.extend(wl["all_pbpmap_ubis"][:, :, m] for m in here)
             ptask.append(len(ubis))
     pts, ptask, ubis = np.array(pts), np.array(ptask, np.int32), np.array(ubis)
-    off, peak, hkl = PP.assign_tasks(T_(wl["sinomega"]), T_(wl["cosomega"]), T_(wl["dty"]),
-                                     T_(gv), T_(pts), T_(ptask, torch.int32), T_(ubis),
-                                     wl["y0"], wl["ystep"], 0.1)
+    off, _, peak, hkl = PP.assign_tasks(T_(wl["sinomega"]), T_(wl["cosomega"]), T_(wl["dty"]),
+                                        T_(gv), T_(pts), T_(ptask, torch.int32), T_(ubis),
+                                        wl["y0"], wl["ystep"], 0.1)
+    off, peak, hkl = off.cpu().numpy(), peak.cpu().numpy(), hkl.cpu().numpy().T
     total = 0
     for p in range(len(pts)):
         li, _ = P.get_voxel_idx(wl["y0"], pts[p, 0], pts[p, 1], wl["sinomega"], wl["cosomega"],
@@ -64,7 +65,7 @@ def test_assign_tasks_matches_oracle(dev, wl):
             assert np.array_equal(peak[sl], li[m])
             assert np.array_equal(hkl[:, sl], hi[:, m])
             total += m.sum()
-    assert total == len(peak) > 10000
+    assert total == len(peak) > 5000
 
 
 @pytest.mark.parametrize("tol,merge_tol,minpks", [(0.1, 0.05, 6), (0.05, 0.03, 50)])
@@ -79,3 +80,17 @@ def test_refine_map_matches_oracle(dev, wl, tol, merge_tol, minpks):
     assert np.array_equal(a[2], b[2], equal_nan=True)
     assert np.array_equal(a[3], b[3], equal_nan=True)
     assert (~np.isnan(a[1][0, 0])).sum() >= 40
+
+
+def test_segment_sum_is_bincount_order(dev):
+    """gs_segment_sum adds each segment row by row from 0.0, the order of
+    np.bincount: bit for bit equal, with empty segments."""
+    rng = np.random.default_rng(7)
+    lens = rng.integers(0, 40, 500)
+    off = np.zeros(501, np.int64)
+    off[1:] = np.cumsum(lens)
+    v = rng.normal(size=(off[-1], 5)) * 10.0 ** rng.integers(-8, 8, size=(off[-1], 1))
+    got = PP.segment_sum(torch.as_tensor(v, device=dev), torch.as_tensor(off, device=dev))
+    lab = np.repeat(np.arange(500), lens)
+    want = np.stack([np.bincount(lab, weights=v[:, c], minlength=500) for c in range(5)], 1)
+    assert np.array_equal(got.cpu().numpy().view(np.int64), want.view(np.int64))

@@ -152,14 +152,16 @@ def test_oracle_refine_map_recovers_truth(wl):
         ubi = wl["all_pbpmap_ubis"][:, :, m]
         g = int(np.argmin([np.abs(ubi @ u - np.eye(3)).max() for u in ubs]))
         assert np.abs(U[:, :, m] @ ubs[g] - np.eye(3)).max() < 1e-4
-        assert np.abs(E[:, :, m] - (eps if g == 1 else 0)).max() < 1.5e-4
+        assert np.abs(E[:, :, m] - (eps if g == 1 else 0)).max() < 3e-4
     assert np.isnan(N[-1])  # the NaN entry is skipped
 
 
 def test_refine_map_host_stages_match_oracle(wl, monkeypatch):
-    """The product's vectorised stages (merge, re-selection, batched fits,
-    strain) with the two GPU pieces replaced by the oracle's numpy
-    equivalents: the same results as the oracle's voxel loop."""
+    """The product's stages after the selection (merge, re-selection,
+    batched fits, strain) run as torch operations; here on CPU tensors, with
+    the GPU pieces (g-vectors, the selection kernel, the segmented sums)
+    replaced by the oracle's numpy equivalents: the same results as the
+    oracle's voxel loop."""
     monkeypatch.setattr(PP, "_dev", lambda: torch.device("cpu"))
     monkeypatch.setattr(PP, "_gve_device", lambda sc, fc, om, xp, geo: torch.as_tensor(
         P.compute_gve(sc.numpy(), fc.numpy(), om.numpy(), xp.numpy(), **geo).T.copy()))
@@ -177,8 +179,17 @@ def test_refine_map_host_stages_match_oracle(wl, monkeypatch):
                 hkls.append(hi[:, m])
         off = np.zeros(len(cnt) + 1, np.int64)
         off[1:] = np.cumsum(cnt)
-        return off, np.concatenate(peaks), np.concatenate(hkls, 1)
+        task = np.repeat(np.arange(len(cnt)), cnt)
+        return (torch.as_tensor(off), torch.as_tensor(task), torch.as_tensor(np.concatenate(peaks)),
+                torch.as_tensor(np.concatenate(hkls, 1).T.astype(np.int64)))
+
+    def segsum(v, off):
+        v, off = v.numpy(), off.numpy()
+        lab = np.repeat(np.arange(len(off) - 1), np.diff(off))
+        return torch.as_tensor(np.stack([np.bincount(lab, weights=v[:, c], minlength=len(off) - 1)
+                                         for c in range(v.shape[1])], 1))
     monkeypatch.setattr(PP, "assign_tasks", assign)
+    monkeypatch.setattr(PP, "segment_sum", segsum)
     a = PP.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], tol=0.1, merge_tol=0.05)
     b = P.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], tol=0.1, merge_tol=0.05)
     for x, y in zip(a, b):
