@@ -212,3 +212,12 @@ def test_scan_fast_path_is_sound(tol, hmax):
         assert not np.any(skipped & exact_in)
         assert 0 < exact_in.sum() < len(xl)
         assert skipped.sum() > 0 and (~skipped).sum() > 0
+
+
+def test_smoke_workload_generates():
+    """smoke()'s input builds on the CPU (the simulator raises when the
+    grains give fewer spots than asked for)."""
+    import __graft_entry__ as g
+    from imaged11_amd import simulate
+    wl = simulate.scoring_workload(**g.SMOKE_WORKLOAD)
+    assert len(wl["peaks"]["sc"]) == g.SMOKE_WORKLOAD["N"]
