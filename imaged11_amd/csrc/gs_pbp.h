@@ -101,6 +101,16 @@ __global__ __launch_bounds__(GS_BLOCK) void k_segment_sum(const double* __restri
     const long long s = i / C;
     const int c = (int)(i - s * C);
     double acc = 0.0;
-    for (long long r = off[s]; r < off[s + 1]; r++) acc += v[r * C + c];
+    long long r = off[s];
+    const long long r1 = off[s + 1];
+    // loads run ahead in groups of eight; the additions stay in row order
+    for (; r + 8 <= r1; r += 8) {
+        double x[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[j] = v[(r + j) * C + c];
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc += x[j];
+    }
+    for (; r < r1; r++) acc += v[r * C + c];
     out[i] = acc;
 }

@@ -235,6 +235,10 @@ class _ScanState:
         self.so = self.co = None
         self.idx = self.off = None
         self.off_host = None
+        # per grain position (assignlabels' order): the UBIs read and the
+        # current ones, on the device
+        self.names = []
+        self.ubi_read = self.ubi_cur = None
 
     def lab(self, pars):
         """Lab positions for the current detector parameters."""
@@ -455,8 +459,11 @@ class refinegrains:
         if self.OMEGA_FLOAT:
             if ubis is None:
                 ubis = np.array([self.grains[(n, sname)].ubi for n in names], float)
-            u = torch.as_tensor(np.ascontiguousarray(ubis, dtype=np.float64),
-                                device=st.device).reshape(-1, 3, 3)
+            if isinstance(ubis, torch.Tensor):
+                u = ubis.contiguous()
+            else:
+                u = torch.as_tensor(np.ascontiguousarray(ubis, dtype=np.float64),
+                                    device=st.device).reshape(-1, 3, 3)
             ident = torch.arange(idx.numel(), dtype=torch.int32, device=st.device)
             mats, _, _ = batched.refine_lists(u, gv, ident, off_d, self.tolerance, 1)
             gv, tth, eta, omc = batched.grain_gv(st.xl, st.so, st.co, st.omega, p, trans_d, idx,
@@ -468,30 +475,51 @@ class refinegrains:
         per-request (npks, avg_drlv2) of the second score_and_refine call,
         the refined UBIs written to the grains when ``update``."""
         st = self._state[sname]
-        pos = [self._pos[sname][n] for n in names]
+        pos = np.array([self._pos[sname][n] for n in names], np.int64)
+        self._push(sname, names, pos)
+        npks, avg = self._evaluate_pos(sname, pos, trans, start, update)
+        if update:
+            self._pull(sname, names, pos)
+        return npks, avg
+
+    def _push(self, sname, names, pos):
+        """Grain objects' UBIs -> the device stack (they may have been set
+        by the caller between calls)."""
+        st = self._state[sname]
+        st.ubi_cur[torch.as_tensor(pos, device=st.device)] = torch.as_tensor(
+            np.array([self.grains[(n, sname)].ubi for n in names], float), device=st.device)
+
+    def _pull(self, sname, names, pos):
+        st = self._state[sname]
+        cur = st.ubi_cur[torch.as_tensor(pos, device=st.device)].cpu().numpy()
+        for j, n in enumerate(names):
+            self.grains[(n, sname)].set_ubi(cur[j])
+
+    def _evaluate_pos(self, sname, pos, trans, start="read", update=True):
+        """_evaluate on grain positions and the device UBI stacks: no
+        per-grain host work (the lockstep simplex calls it every round)."""
+        st = self._state[sname]
         idx, owner, off, off_d = self._requests(sname, pos)
-        gv, _, _, _ = self._gv_device(sname, idx, owner, off_d, trans, names, False)
+        pos_d = torch.as_tensor(pos, device=st.device)
+        gv, _, _, _ = self._gv_device(sname, idx, owner, off_d, trans, None, False,
+                                      ubis=st.ubi_cur[pos_d])
         ident = torch.arange(idx.numel(), dtype=torch.int32, device=st.device)
-        if start == "read":
-            u0 = np.array([self.ubisread[n] for n in names], float)
-        else:
-            u0 = np.array([self.grains[(n, sname)].ubi for n in names], float)
-        u = torch.as_tensor(u0, device=st.device)
+        u = (st.ubi_read if start == "read" else st.ubi_cur)[pos_d].contiguous()
         if self.latticesymmetry is triclinic:
             u, n, m = batched.refine_lists(u, gv, ident, off_d, self.tolerance, 2)
-            out = u.cpu().numpy()
-            npks, avg = n[1].cpu().numpy(), m[1].cpu().numpy()
+            k = 1
         else:
             u, _, _ = batched.refine_lists(u, gv, ident, off_d, self.tolerance, 1)
             u = torch.as_tensor(symmetrize(u.cpu().numpy(), self.latticesymmetry),
                                 device=st.device)
             u, n, m = batched.refine_lists(u, gv, ident, off_d, self.tolerance, 1)
-            out = symmetrize(u.cpu().numpy(), self.latticesymmetry)
-            npks, avg = n[0].cpu().numpy(), m[0].cpu().numpy()
+            u = torch.as_tensor(symmetrize(u.cpu().numpy(), self.latticesymmetry),
+                                device=st.device)
+            k = 0
         if update:
-            for j, nme in enumerate(names):
-                self.grains[(nme, sname)].set_ubi(out[j])
-        return npks.astype(np.int64), avg, out
+            st.ubi_cur[pos_d] = u
+        nm = torch.stack([n[k].double(), m[k]]).cpu().numpy()
+        return nm[0].astype(np.int64), nm[1]
 
     # -- the reference's per-grain entry points ----------------------------
     def compute_gv(self, thisgrain, update_columns=False):
@@ -546,7 +574,7 @@ class refinegrains:
         for sname, names in byscan.items():
             if self.recompute_xlylzl:
                 self._state[sname].lab(p)
-            npks, avg, _ = self._evaluate(sname, names, np.array([t] * len(names), float))
+            npks, avg = self._evaluate(sname, names, np.array([t] * len(names), float))
             for j, n in enumerate(names):
                 results[(n, sname)] = (int(npks[j]), float(avg[j]))
         for key in self.grains_to_refine:
@@ -610,15 +638,17 @@ class refinegrains:
                     continue
                 names = [k[0] for k in keys]
                 guesses = np.array([self.grains[k].translation for k in keys], float)
+                pos = np.array([self._pos[sname][n] for n in names], np.int64)
+                self._push(sname, names, pos)
 
-                def evaluate(points, which, sname=sname, names=names):
-                    sel = [names[w] for w in which]
-                    npks, avg, _ = self._evaluate(sname, sel, points)
+                def evaluate(points, which, sname=sname, pos=pos):
+                    npks, avg = self._evaluate_pos(sname, pos[which], points)
                     diffs = npks * avg
                     val = np.where(npks > 0, 1e6 * (0. + diffs) / np.maximum(npks, 1), 1e6)
                     return val
                 sb = simplex.SimplexBatch(evaluate, guesses, np.asarray(inc, float))
                 best, err, iters = sb.minimize(maxiters=maxiters)
+                self._pull(sname, names, pos)
                 for j, k in enumerate(keys):
                     self.grains[k].translation = sb.last[j].copy()
                     results[k] = (best[j], err[j], int(iters[j]))
@@ -642,8 +672,8 @@ class refinegrains:
                 continue
             names = [k[0] for k in keys]
             trans = np.array([self.grains[k].translation for k in keys], float)
-            npks, avg, _ = self._evaluate(sname, names, trans, start="current",
-                                          update=not scoreonly)
+            npks, avg = self._evaluate(sname, names, trans, start="current",
+                                       update=not scoreonly)
             for j, k in enumerate(keys):
                 out[k] = (int(npks[j]), float(avg[j]))
                 if not quiet:
@@ -670,6 +700,10 @@ class refinegrains:
             dev = st.device
             ubis = torch.as_tensor(np.array([self.grains[(n, sname)].ubi for n in names], float),
                                    device=dev).reshape(K, 3, 3)
+            st.names = names
+            st.ubi_cur = ubis.clone()
+            st.ubi_read = torch.as_tensor(np.array([self.ubisread[n] for n in names], float),
+                                          device=dev).reshape(K, 3, 3)
             trans = torch.as_tensor(np.array([self.grains[(n, sname)].translation for n in names],
                                              float), device=dev).reshape(K, 3)
             labels = torch.full((st.n,), -1, dtype=torch.int32, device=dev)
