@@ -106,8 +106,8 @@ class JointProblem:
         lib.gs_clear_error()
         # bound to names so that no argument's memory is recycled by the
         # allocator before the launch (a temporary's block is free again as
-        # soon as its last reference goes)
-        geo = geo.detach().contiguous()
+        # soon as its last reference goes); geo may come from the host
+        geo = geo.detach().to(self.sc.device, torch.float64).contiguous()
         ubis = ubis.detach().contiguous()
         trans = trans.detach().contiguous()
         _lib.check(lib.gs_joint_hkl(dptr(self.sc), dptr(self.fc), dptr(self.so), dptr(self.co),
@@ -122,10 +122,10 @@ class JointProblem:
         and the UBI gradient slot holds dE/dUB."""
         lib = _lib.load()
         lib.gs_clear_error()
-        geo = geo.detach().contiguous()
+        dev = self.sc.device
+        geo = geo.detach().to(dev, torch.float64).contiguous()
         ubis = ubis.detach().contiguous()
         trans = trans.detach().contiguous()
-        dev = self.sc.device
         gu = torch.empty((self.K, 3, 3), dtype=torch.float64, device=dev)
         gt = torch.empty((self.K, 3), dtype=torch.float64, device=dev)
         sums = torch.empty((self.K, 15), dtype=torch.float64, device=dev) if want_sums else None
@@ -146,13 +146,14 @@ class JointLoss(torch.autograd.Function):
     @staticmethod
     def forward(ctx, ubis, trans, geo, problem):
         loss, gu, gt, gg, _ = problem.evaluate(geo, ubis, trans)
+        ctx.geo_device = geo.device
         ctx.save_for_backward(gu, gt, gg)
         return loss
 
     @staticmethod
     def backward(ctx, grad):
         gu, gt, gg = ctx.saved_tensors
-        return grad * gu, grad * gt, grad * gg, None
+        return grad * gu, grad * gt, (grad * gg).to(ctx.geo_device), None
 
 
 def joint_loss(problem, ubis, trans, params, fixed):
@@ -162,40 +163,50 @@ def joint_loss(problem, ubis, trans, params, fixed):
     return JointLoss.apply(ubis, trans, geo, problem)
 
 
-def gauss_newton_ubis(sums):
+def gauss_newton_ubis(sums, ubis=None):
     """Per-grain least-squares UBI for fixed g-vectors and hkl:
-    UBI = (sum h G^T)(sum G G^T)^-1, from the (K,15) sums."""
+    UBI = (sum h G^T)(sum G G^T)^-1, from the (K,15) sums, with the
+    cofactor inverse of the symmetric 3x3 (elementwise device operations,
+    no library solver).  A grain whose sum G G^T is singular keeps its UBI
+    from ``ubis`` (as score_and_refine does; NaN when ubis is None)."""
     A = sums[:, :9].reshape(-1, 3, 3)
-    s = sums[:, 9:]
-    M = torch.stack([torch.stack([s[:, 0], s[:, 1], s[:, 2]], -1),
-                     torch.stack([s[:, 1], s[:, 3], s[:, 4]], -1),
-                     torch.stack([s[:, 2], s[:, 4], s[:, 5]], -1)], -2)
-    return torch.linalg.solve(M, A.transpose(1, 2)).transpose(1, 2)
+    s0, s1, s2, s3, s4, s5 = sums[:, 9:].unbind(1)
+    c00, c01, c02 = s3 * s5 - s4 * s4, s2 * s4 - s1 * s5, s1 * s4 - s2 * s3
+    c11, c12, c22 = s0 * s5 - s2 * s2, s1 * s2 - s0 * s4, s0 * s3 - s1 * s1
+    det = s0 * c00 + s1 * c01 + s2 * c02
+    inv = torch.stack([c00, c01, c02, c01, c11, c12, c02, c12, c22], 1).reshape(-1, 3, 3)
+    new = (A[:, :, :, None] * (inv / det[:, None, None])[:, None, :, :]).sum(2)
+    if ubis is None:
+        return new
+    return torch.where((det != 0)[:, None, None], new, ubis)
 
 
 def refine_joint(problem, ubis, trans, params, fixed, steps=50, lr=1e-3, scales=None):
     """Joint refinement: each step is one adjoint pass; the UBIs take the
     Gauss-Newton (exact least-squares) update for the current geometry and
     the geometry parameters an Adam step on dL/dparams (per-parameter step
-    scales in ``scales``).  hkl and the peak lists stay fixed.  Returns
-    (ubis, params, loss history)."""
+    scales in ``scales``).  hkl and the peak lists stay fixed.  The handful
+    of geometry numbers (and their autograd graph) live on the host; the
+    26 packed values go to the device once per step.  Returns (ubis,
+    params, loss history)."""
     ubis = ubis.detach().clone()
     names = sorted(params)
     scale = {k: float((scales or {}).get(k, 1.0)) for k in names}
+    cpu = torch.device("cpu")
     # optimise dimensionless offsets x_k with value = start + scale_k * x_k
-    start = {k: params[k].detach().clone() for k in names}
-    x = {k: torch.zeros((), dtype=torch.float64, device=ubis.device, requires_grad=True)
-         for k in names}
+    start = {k: params[k].detach().to(cpu, torch.float64).clone() for k in names}
+    x = {k: torch.zeros((), dtype=torch.float64, requires_grad=True) for k in names}
     opt = torch.optim.Adam([x[k] for k in names], lr=lr)
     history = []
     for _ in range(steps):
         cur = {k: start[k] + scale[k] * x[k] for k in names}
         geo = pack_geometry(cur, fixed)
         loss, gu, gt, gg, sums = problem.evaluate(geo, ubis, trans, want_sums=True)
+        ubis = gauss_newton_ubis(sums, ubis)
+        g_host = gg.cpu()
         history.append(float(loss))
-        ubis = gauss_newton_ubis(sums)
         opt.zero_grad()
-        geo.backward(gg)
+        geo.backward(g_host)
         opt.step()
-    final = {k: (start[k] + scale[k] * x[k]).detach() for k in names}
+    final = {k: (start[k] + scale[k] * x[k]).detach().to(params[k].device) for k in names}
     return ubis, final, history

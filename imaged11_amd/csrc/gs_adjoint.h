@@ -255,19 +255,41 @@ __global__ __launch_bounds__(GS_BLOCK) void k_joint_adjoint(
 
 // Deterministic reduction of the per-grain partials: out[0..25] = dL/dgeo
 // (slot 16 converted from d/d(1/wavelength) to d/dwavelength), out[26] =
-// loss.  One block; thread j < 27 sums column j over grains 0..K-1 in
-// order (the order of the grains, not of the waves that took them).
-__global__ void k_joint_finalize(const double* __restrict__ geo_part,
-                                 const double* __restrict__ loss_g, int K,
-                                 const double* __restrict__ geo_in, double* __restrict__ out) {
-    const int j = threadIdx.x;
-    if (j >= GS_GEO_OUT) return;
-    double s = 0.0;
-    if (j < GS_GEO) {
-        for (int g = 0; g < K; g++) s += geo_part[GS_GEO * (size_t)g + j];
-        if (j == 16) s = -s / (geo_in[16] * geo_in[16]);
-    } else {
-        for (int g = 0; g < K; g++) s += loss_g[g];
+// loss.  One block of GS_FIN_GROUPS x 32 threads: thread (r, j) sums
+// column j over the grains g = r, r + GS_FIN_GROUPS, ... in that order,
+// then thread j adds the GS_FIN_GROUPS partial sums in group order.  The
+// order depends only on K (not on which waves took which grains), so the
+// result is the same on every run.
+#define GS_FIN_GROUPS 32
+__global__ __launch_bounds__(GS_FIN_GROUPS * 32) void k_joint_finalize(
+    const double* __restrict__ geo_part, const double* __restrict__ loss_g, int K,
+    const double* __restrict__ geo_in, double* __restrict__ out) {
+    __shared__ double part[GS_FIN_GROUPS][GS_GEO_OUT];
+    const int j = threadIdx.x & 31;
+    const int r = threadIdx.x >> 5;
+    if (j < GS_GEO_OUT) {
+        double s = 0.0;
+        int g = r;
+        // loads run ahead in groups of four; additions stay in order
+        for (; g + 3 * GS_FIN_GROUPS < K; g += 4 * GS_FIN_GROUPS) {
+            double v[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const size_t gg = (size_t)g + (size_t)q * GS_FIN_GROUPS;
+                v[q] = j < GS_GEO ? geo_part[GS_GEO * gg + j] : loss_g[gg];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) s += v[q];
+        }
+        for (; g < K; g += GS_FIN_GROUPS)
+            s += j < GS_GEO ? geo_part[GS_GEO * (size_t)g + j] : loss_g[g];
+        part[r][j] = s;
     }
-    out[j] = s;
+    __syncthreads();
+    if (r == 0 && j < GS_GEO_OUT) {
+        double s = 0.0;
+        for (int q = 0; q < GS_FIN_GROUPS; q++) s += part[q][j];
+        if (j == 16) s = -s / (geo_in[16] * geo_in[16]);
+        out[j] = s;
+    }
 }

@@ -911,8 +911,8 @@ int gs_joint_adjoint(const double* sc, const double* fc, const double* so, const
     }
 #undef GS_ADJ_LPG
 #undef GS_ADJ_LAUNCH
-    hipLaunchKernelGGL(k_joint_finalize, dim3(1), dim3(GS_WAVE), 0, st, geo_part, loss_g, K, geo,
-                       out);
+    hipLaunchKernelGGL(k_joint_finalize, dim3(1), dim3(GS_FIN_GROUPS * 32), 0, st, geo_part, loss_g,
+                       K, geo, out);
     GS_LAUNCHED();
     return GS_OK;
 }

@@ -203,8 +203,9 @@ int gs_graph_destroy(gs_graph_t graph);
  * gs_joint_adjoint: per grain loss_g[K], grad_ubi[K*9], grad_t[K*3],
  * geo_part[K*26]; nsums[K*15] (optional, NULL to skip) = sum h G^T and
  * sum G G^T for a Gauss-Newton UBI update; out[27] = dL/dgeo (26, slot 16
- * is dL/dwavelength) and L, reduced over grains in grain order
- * (deterministic).  gspace != 0: the g-space objective
+ * is dL/dwavelength) and L, reduced over grains in an order fixed by K
+ * (32 interleaved groups of grains, then the groups in order:
+ * deterministic).  gspace != 0: the g-space objective
  * 1/2 sum ||UB.hkl - G||^2 (UB = UBI^-1), grad_ubi then holds dE/dUB =
  * UB.H - R with score_and_refine's R and H (src/closest.c:310-316). */
 int gs_joint_hkl(const double *sc, const double *fc, const double *so, const double *co,

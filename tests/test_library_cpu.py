@@ -221,3 +221,26 @@ def test_smoke_workload_generates():
     from imaged11_amd import simulate
     wl = simulate.scoring_workload(**g.SMOKE_WORKLOAD)
     assert len(wl["peaks"]["sc"]) == g.SMOKE_WORKLOAD["N"]
+
+
+def test_gauss_newton_ubis_closed_form():
+    """The cofactor solve of the per-grain Gauss-Newton systems against
+    numpy's solver; a singular grain keeps its UBI."""
+    import torch
+    from imaged11_amd import adjoint
+    rng = np.random.default_rng(8)
+    K = 50
+    G = rng.normal(size=(K, 40, 3))
+    h = rng.normal(size=(K, 40, 3))
+    A = np.einsum("kni,knj->kij", h, G)
+    M = np.einsum("kni,knj->kij", G, G)
+    M[3] = np.outer([1.0, 2.0, 3.0], [1.0, 2.0, 3.0])  # singular
+    sums = np.concatenate([A.reshape(K, 9), M[:, [0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]]], 1)
+    prev = rng.normal(size=(K, 3, 3))
+    got = adjoint.gauss_newton_ubis(torch.as_tensor(sums), torch.as_tensor(prev)).numpy()
+    for k in range(K):
+        if k == 3:
+            assert np.array_equal(got[k], prev[k])
+        else:
+            want = np.linalg.solve(M[k], A[k].T).T
+            np.testing.assert_allclose(got[k], want, rtol=1e-11, atol=1e-12)
