@@ -405,11 +405,14 @@ def aux_pbp(with_cpu, NY=101, grains=24, cpu_points=40):
             "xpos", "ystep", "y0", "B0")
     args = [w[k] for k in keys]
     PP.refine_map(*args, **w["geometry"])  # warm
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    U, E, N, Q = PP.refine_map(*args, **w["geometry"])
-    torch.cuda.synchronize()
-    gpu_s = time.perf_counter() - t0
+    runs = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        U, E, N, Q = PP.refine_map(*args, **w["geometry"])
+        torch.cuda.synchronize()
+        runs.append(time.perf_counter() - t0)
+    gpu_s = statistics.median(runs)
     M = w["all_pbpmap_ubis"].shape[2]
     voxels = int(w["mask"].sum())
     npeaks = len(w["sc"])
@@ -438,7 +441,7 @@ def aux_pbp(with_cpu, NY=101, grains=24, cpu_points=40):
     assign_ms = _events_ms(lambda: PP.assign_tasks(*dargs), reps=3)
     out = {"map": "%dx%d" % (NY, NY), "grains": grains, "voxels": voxels, "ubis": M,
            "peaks": npeaks,
-           "gen_s": gen_s, "gpu_s": gpu_s, "voxels_per_s": voxels / gpu_s,
+           "gen_s": gen_s, "gpu_s": gpu_s, "gpu_s_runs": runs, "voxels_per_s": voxels / gpu_s,
            "selection_ms": assign_ms,
            "selection_voxel_peak_pairs_per_s": len(pts) * npeaks / (assign_ms * 1e-3),
            "refined": int((~np.isnan(E[0, 0])).sum())}
