@@ -305,39 +305,38 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
     final_npks = np.full(M, np.nan)
     final_nuniq = np.full(M, np.nan)
     ri_col, rj_col = np.asarray(ri_col), np.asarray(rj_col)
-    # ---- tasks: (voxel, UBI) pairs of the voxels to refine
-    by_point = {}
-    for m in range(M):
-        by_point.setdefault((int(ri_col[m]), int(rj_col[m])), []).append(m)
-    pts, ptask, tmap = [], [0], []
-    for ri, rj in refine_points:
-        ri, rj = int(ri), int(rj)
-        if not mask[ri, rj]:
-            continue
-        valid = [m for m in by_point.get((ri, rj), [])
-                 if not np.isnan(all_pbpmap_ubis[0, 0, m])]
-        if not valid:
-            continue
-        pts.append((sx_grid[ri, rj], sy_grid[ri, rj]))
-        tmap.extend(valid)
-        ptask.append(len(tmap))
+    # ---- tasks: (voxel, UBI) pairs of the voxels to refine, grouped by
+    # voxel in the order of refine_points (a repeated point is refined once:
+    # the reference would write the same results twice)
+    rp = np.asarray(refine_points, np.int64).reshape(-1, 2)
+    rank = np.full(mask.shape, -1, np.int64)
+    rank[rp[::-1, 0], rp[::-1, 1]] = np.arange(len(rp))[::-1]
+    sel = np.asarray(mask, bool) & (rank >= 0)
+    inb = (ri_col >= 0) & (ri_col < mask.shape[0]) & (rj_col >= 0) & (rj_col < mask.shape[1])
+    ri_c, rj_c = np.where(inb, ri_col, 0), np.where(inb, rj_col, 0)
+    ok = inb & sel[ri_c, rj_c] & ~np.isnan(all_pbpmap_ubis[0, 0, :])
+    m_idx = np.nonzero(ok)[0]
+    prank = rank[ri_c[m_idx], rj_c[m_idx]]
+    o = np.lexsort((m_idx, prank))
+    tmap, tp = m_idx[o], prank[o]
     T = len(tmap)
     if T == 0:
         return final_ubis, final_eps, final_npks, final_nuniq
-    tmap = np.array(tmap, np.int64)
-    tpoint = np.repeat(np.arange(len(pts)), np.diff(ptask))
+    upr, counts = np.unique(tp, return_counts=True)
+    ptask = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    pts = np.stack([sx_grid[rp[upr, 0], rp[upr, 1]], sy_grid[rp[upr, 0], rp[upr, 1]]], 1)
+    tpoint = np.repeat(np.arange(len(upr)), counts)
     ubis_h = np.ascontiguousarray(np.moveaxis(all_pbpmap_ubis[:, :, tmap], -1, 0), np.float64)
     D = lambda a, dt=torch.float64: _t(a, dev, dt)
     sc_d, fc_d, om_d, xpos_d = D(sc), D(fc), D(omega), D(xpos)
     eta_d, si_d, dty_d = D(eta), D(sum_intensity), D(dty)
     dtyi_d = D(dtyi, torch.int64)
-    pts_d, ubis_d = D(np.array(pts)), D(ubis_h)
+    pts_d, ubis_d = D(pts), D(ubis_h)
     tx_d, ty_d = pts_d[D(tpoint, torch.int64), 0], pts_d[D(tpoint, torch.int64), 1]
     # ---- selection and first scoring of every task over every peak
     gv_all = _gve_device(sc_d, fc_d, om_d, xpos_d, geo)
     _, task, peak, hkl = assign_tasks(D(sinomega), D(cosomega), dty_d, gv_all, pts_d,
-                                      D(np.array(ptask, np.int32), torch.int32), ubis_d, y0,
-                                      ystep, tol)
+                                      D(ptask, torch.int32), ubis_d, y0, ystep, tol)
     # ---- merge by (task, h, k, l, eta sign, dtyi): groups in sorted key
     # order, sums in row (= ascending peak) order, as np.bincount per voxel
     gid, G = _groups([task, hkl[:, 0], hkl[:, 1], hkl[:, 2],
@@ -394,17 +393,16 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
                            torch.sign(reta[r2]).long()])
         first = torch.zeros(U_, dtype=torch.int64, device=dev).scatter_(0, uid, rtask[r2])
         nuniq = torch.bincount(first, minlength=T).cpu().numpy()
-    for t in np.nonzero(fit_h)[0]:
-        m = tmap[t]
-        final_ubis[:, :, m] = ubi_out[t]
-        final_npks[m] = npk2[t]
-        final_nuniq[m] = nuniq[t]
+    fidx = np.nonzero(fit_h)[0]
+    final_ubis[:, :, tmap[fidx]] = np.moveaxis(ubi_out[fidx], 0, -1)
+    final_npks[tmap[fidx]] = npk2[fidx]
+    final_nuniq[tmap[fidx]] = nuniq[fidx]
     # ---- strain tensors of the well-fitted tasks
     if worth.any():
         eps = _strain_fit(D(worth, torch.bool), ubifit, worth, rtask, sel2, g, hi2, rydist, B0,
                           gnoise_std, T)
-        for t in np.nonzero(worth)[0]:
-            final_eps[:, :, tmap[t]] = eps[t]
+        widx = np.nonzero(worth)[0]
+        final_eps[:, :, tmap[widx]] = np.moveaxis(eps[widx], 0, -1)
     return final_ubis, final_eps, final_npks, final_nuniq
 
 
