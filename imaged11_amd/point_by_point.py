@@ -211,6 +211,8 @@ def segment_sum(v, off):
     v = v.contiguous()
     S = off.numel() - 1
     C = v.shape[1]
+    if v.shape[0] == 0:  # no rows: every segment is empty
+        return torch.zeros((S, C), dtype=torch.float64, device=v.device)
     out = torch.empty((S, C), dtype=torch.float64, device=v.device)
     _lib.check(lib.gs_segment_sum(dptr(v), C, dptr(off), S, dptr(out), stream_ptr()),
                "segment_sum")
@@ -337,6 +339,8 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
     gv_all = _gve_device(sc_d, fc_d, om_d, xpos_d, geo)
     _, task, peak, hkl = assign_tasks(D(sinomega), D(cosomega), dty_d, gv_all, pts_d,
                                       D(ptask, torch.int32), ubis_d, y0, ystep, tol)
+    if peak.numel() == 0:  # no task has any peak: nothing is refined
+        return final_ubis, final_eps, final_npks, final_nuniq
     # ---- merge by (task, h, k, l, eta sign, dtyi): groups in sorted key
     # order, sums in row (= ascending peak) order, as np.bincount per voxel
     gid, G = _groups([task, hkl[:, 0], hkl[:, 1], hkl[:, 2],

@@ -94,3 +94,23 @@ def test_segment_sum_is_bincount_order(dev):
     lab = np.repeat(np.arange(500), lens)
     want = np.stack([np.bincount(lab, weights=v[:, c], minlength=500) for c in range(5)], 1)
     assert np.array_equal(got.cpu().numpy().view(np.int64), want.view(np.int64))
+
+
+def test_refine_map_edge_cases(dev, wl):
+    """No point to refine, nothing assigned, and too few peaks to fit: all
+    NaN, as the reference leaves them."""
+    args = {k: wl[k] for k in ARGS}
+    # only masked-out points
+    outside = [(i, j) for i in range(wl["mask"].shape[0]) for j in range(wl["mask"].shape[1])
+               if not wl["mask"][i, j]]
+    a = dict(args, refine_points=np.array(outside[:5]))
+    for r in PP.refine_map(*[a[k] for k in ARGS], **wl["geometry"]):
+        assert np.isnan(r).all()
+    # a tolerance nothing passes
+    for r in PP.refine_map(*[args[k] for k in ARGS], **wl["geometry"], tol=1e-300):
+        assert np.isnan(r).all()
+    # a peak minimum no UBI reaches: nothing fitted, nothing written
+    got = PP.refine_map(*[args[k] for k in ARGS], **wl["geometry"], min_grain_npks=10 ** 9)
+    want = P.refine_map(*[args[k] for k in ARGS], **wl["geometry"], min_grain_npks=10 ** 9)
+    for x, y in zip(got, want):
+        assert np.isnan(x).all() and np.isnan(y).all()

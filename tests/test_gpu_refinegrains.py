@@ -20,9 +20,16 @@ pytestmark = pytest.mark.gpu
 
 
 def _setup(orc, tmp_path=None, K=10, N=None, seed=11, OmFloat=True, sym="triclinic",
-           tol=0.05, wedge=0.0, chi=0.0, via_files=False):
+           tol=0.05, wedge=0.0, chi=0.0, via_files=False, empty_grain=False):
     N = 600 * K if N is None else N
     w = simulate.refinegrains_workload(K=K, N=N, seed=seed, wedge=wedge, chi=chi)
+    if empty_grain:
+        # one more grain whose orientation matches no peak
+        rng = np.random.default_rng(99)
+        u = simulate.random_rotations(1, rng)[0] / 7.77
+        w["ubi"] = np.concatenate([w["ubi"], np.linalg.inv(u)[None]])
+        w["trans"] = np.concatenate([w["trans"], np.zeros((1, 3))])
+        K += 1
     pk = w["peaks"]
     rg = RG.refinegrains(tolerance=tol, OmFloat=OmFloat, OmSlop=0.25,
                          latticesymmetry=RG.LATTICES[sym])
@@ -185,3 +192,25 @@ def test_fit_matches_oracle(dev, orc):
     np.testing.assert_allclose(g1, g2, rtol=1e-9)
     assert abs(e1 - e2) <= 1e-9 * abs(e2)
     assert abs(g1[0] - p0["distance"]) < 400.0  # moved towards the truth
+
+
+def test_grain_without_peaks(dev, orc):
+    """A grain that owns no peak: gof, refineubis and refinepositions run,
+    its npks is 0 and its UBI is left as it was, as in the oracle."""
+    rg, o, s = _setup(orc, K=6, OmFloat=True, seed=19, empty_grain=True)
+    _assign_both(rg, o, s)
+    assert len(rg.grains[(6, s)].ind) == 0
+    u0 = rg.grains[(6, s)].ubi.copy()
+    got = rg.refineubis()
+    want = o.refineubis()
+    assert got[(6, s)] == (0, 0.0) and want[6] == (0, 0.0)
+    assert np.array_equal(rg.grains[(6, s)].ubi, u0)
+    rg.grains_to_refine = [(6, s)]
+    o.grains_to_refine = [6]
+    rg.parameterobj.varylist = o.varylist = []
+    assert rg.gof([]) == o.gof([]) == 1e6
+    rg.refinepositions(maxiters=5)
+    o.refinepositions({k: rg.parameterobj.stepsizes[k] for k in ("t_x", "t_y", "t_z")},
+                      maxiters=5)
+    np.testing.assert_allclose(rg.grains[(6, s)].translation, o.grains[6].translation,
+                               atol=1e-9)
