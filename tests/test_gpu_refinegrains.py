@@ -214,3 +214,41 @@ def test_grain_without_peaks(dev, orc):
                       maxiters=5)
     np.testing.assert_allclose(rg.grains[(6, s)].translation, o.grains[6].translation,
                                atol=1e-9)
+
+
+def test_single_grain_calls_and_savegrains(dev, orc, tmp_path):
+    """compute_gv + refine for one grain as the reference's callers use
+    them, and savegrains: the file holds every grain (most peaks first),
+    and the per-grain columns hold its g-vectors and hkl."""
+    rg, o, s = _setup(orc, K=6, OmFloat=True, seed=23)
+    _assign_both(rg, o, s)
+    for n in (0, 3):
+        rg.set_translation(n, s)
+        o.set_translation(n)
+        rg.compute_gv(rg.grains[(n, s)])
+        o.compute_gv(o.grains[n])
+        np.testing.assert_allclose(rg.gv, o.gv, rtol=0, atol=1e-13 * np.abs(o.gv).max())
+        np.testing.assert_allclose(rg.grains[(n, s)].omega_calc, o.grains[n].omega_calc,
+                                   atol=1e-9)
+        a = rg.refine(rg.ubisread[n])
+        b = o.refine(o.ubisread[n])
+        assert rg.npks == o.npks
+        np.testing.assert_allclose(a, b, atol=1e-9)
+    out = str(tmp_path / "refined.ubi")
+    rg.savegrains(out, sort_npks=True)
+    back = grain.read_grain_file(out)
+    assert len(back) == 6
+    npks = [int(g.npks) for g in back]
+    assert npks == sorted(npks, reverse=True)
+    col = rg.scandata[s]
+    for n in range(6):
+        g = rg.grains[(n, s)]
+        ind = g.ind
+        gv = np.stack([col.gx[ind], col.gy[ind], col.gz[ind]], 1)
+        hr = g.ubi @ gv.T
+        # hr, kr, lr are float32 columns, as the reference makes them
+        np.testing.assert_allclose(np.stack([col.hr[ind], col.kr[ind], col.lr[ind]]), hr,
+                                   rtol=1e-6, atol=1e-6)
+        assert np.array_equal(np.stack([col.h[ind], col.k[ind], col.l[ind]]),
+                              np.floor(hr + 0.5))
+        assert 0 < g.nuniq <= len(ind)
