@@ -493,9 +493,8 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
     torch.cuda.synchronize()
     assign_ms = a.elapsed_time(b)
     gvl, idx, off = drivers.grain_gvectors(scan, trans, labels, K)
-    ident = torch.arange(gvl.shape[0], dtype=torch.int32, device=dev)
     work = ubis.clone()
-    g = refine.RefineGraph(work, gvl, ident, off, 0.05, n_iter)
+    g = refine.RefineGraph(work, gvl, None, off, 0.05, n_iter)
     refine_ms = _events_ms(lambda: (work.copy_(ubis), g.replay()), reps=3)
     own = pk["grain"]
     lab = labels.cpu().numpy()

@@ -166,20 +166,45 @@ def list_order(off):
     return torch.argsort(lengths, descending=True, stable=True).to(torch.int32)
 
 
+def gather_rows(gv, idx):
+    """rows[q] = gv[idx[q]] for every list entry, in list order, a NaN row
+    where idx[q] is outside [0, len(gv)) (never kept by the refinement, as
+    such an index is never kept)."""
+    lib = _ready()
+    _need(gv, torch.float64, "gv", (None, 3))
+    _need(idx, torch.int32, "idx", (None,))
+    rows = torch.empty((idx.numel(), 3), dtype=torch.float64, device=gv.device)
+    _lib.check(lib.gs_gather_rows(dptr(gv), gv.shape[0], dptr(idx), idx.numel(), dptr(rows),
+                                  stream_ptr()), "gather_rows")
+    return rows
+
+
 def refine_lists(ubis, gv, idx, off, tol, n_iter=2, order=None, lanes_per_grain=0,
-                 waves_per_cu=0, inplace=False):
+                 waves_per_cu=0, inplace=False, gather=None):
     """n_iter successive score_and_refine calls per grain on CSR peak lists.
 
-    ubis (K,3,3) float64; gv (N,3); idx int32 peak indices; off int64 (K+1,)
-    offsets.  Returns (ubis_out, n int32 (n_iter,K), mean float64 (n_iter,K));
-    n/mean of iteration i are those the i-th call returns (kept peaks and
-    mean drlv2 of the UBI entering that call)."""
+    ubis (K,3,3) float64; gv (N,3); idx int32 peak indices, or None when
+    row q of gv is list entry q; off int64 (K+1,) offsets.  Returns
+    (ubis_out, n int32 (n_iter,K), mean float64 (n_iter,K)); n/mean of
+    iteration i are those the i-th call returns (kept peaks and mean drlv2
+    of the UBI entering that call).
+
+    gather (default: when idx is given and n_iter >= 2) packs the list rows
+    once with `gather_rows`, so every iteration streams contiguous rows
+    instead of fetching a cache line per entry; the results are
+    bit-identical either way."""
     lib = _ready()
     _need(ubis, torch.float64, "ubis", (None, 3, 3))
     _need(gv, torch.float64, "gv", (None, 3))
     K = ubis.shape[0]
-    _need(idx, torch.int32, "idx", (None,))
     _need(off, torch.int64, "off", (K + 1,))
+    if idx is not None:
+        _need(idx, torch.int32, "idx", (None,))
+        if gather is None:
+            gather = n_iter >= 2
+        if gather:
+            gv, idx = gather_rows(gv, idx), None
+    n_idx = gv.shape[0] if idx is None else idx.numel()
     if order is None:
         order = list_order(off)
     _need(order, torch.int32, "order", (K,))
@@ -187,7 +212,8 @@ def refine_lists(ubis, gv, idx, off, tol, n_iter=2, order=None, lanes_per_grain=
     n = torch.empty((n_iter, K), dtype=torch.int32, device=gv.device)
     m = torch.empty((n_iter, K), dtype=torch.float64, device=gv.device)
     queue = torch.empty(1, dtype=torch.int32, device=gv.device)
-    _lib.check(lib.gs_refine_lists(dptr(u), K, dptr(gv), gv.shape[0], dptr(idx), idx.numel(),
+    _lib.check(lib.gs_refine_lists(dptr(u), K, dptr(gv), gv.shape[0],
+                                   None if idx is None else dptr(idx), n_idx,
                                    dptr(off), dptr(order), float(tol), int(n_iter), dptr(n),
                                    dptr(m), int(lanes_per_grain), int(waves_per_cu), dptr(queue),
                                    stream_ptr()), "refine_lists")

@@ -444,6 +444,27 @@ __global__ void k_refine_finalize(const double* __restrict__ partials, int nbloc
 }
 
 // ---------------------------------------------------------------------------
+// rows[q] = gv[idx[q]] for the n entries of CSR peak lists, a NaN row where
+// idx[q] is outside [0, ng) (a NaN row scores NaN and is never kept, as an
+// out-of-range index is never kept).
+__global__ __launch_bounds__(GS_BLOCK) void k_gather_rows(const double* __restrict__ gv, int ng,
+                                                          const int* __restrict__ idx, long long n,
+                                                          double* __restrict__ rows) {
+    const long long q = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
+    if (q >= n) return;
+    const long long k = idx[q];
+    double g0 = __builtin_nan(""), g1 = g0, g2 = g0;
+    if (k >= 0 && k < ng) {
+        g0 = gv[3 * k + 0];
+        g1 = gv[3 * k + 1];
+        g2 = gv[3 * k + 2];
+    }
+    rows[3 * q + 0] = g0;
+    rows[3 * q + 1] = g1;
+    rows[3 * q + 2] = g2;
+}
+
+// ---------------------------------------------------------------------------
 // Batched score_and_refine over CSR peak lists: a persistent grid whose
 // waves pull groups of 64/LPG grains from a work queue (grains visited in
 // `order`, longest list first), LPG lanes per grain.  Each grain runs
@@ -455,7 +476,10 @@ __global__ void k_refine_finalize(const double* __restrict__ partials, int nbloc
 // Out-of-range entries are ignored rather than trusted: a grain number
 // outside [0, K) in `order` is skipped, list bounds are clipped to
 // [0, n_idx], and peak indices outside [0, ng) are not kept.
-template <int LPG>
+// ROWS: there is no index array; list entry q is row q of gv (the rows
+// gathered once by k_gather_rows), so every iteration streams contiguous
+// rows instead of gathering one cache line per entry.
+template <int LPG, bool ROWS>
 __global__ __launch_bounds__(GS_BLOCK) void k_refine_lists(
     double* __restrict__ ubis, int K, const double* __restrict__ gv, int ng,
     const int* __restrict__ idx, long long n_idx, const long long* __restrict__ off,
@@ -486,7 +510,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_refine_lists(
 #pragma unroll
             for (int i = 0; i < GS_NPART; i++) a[i] = 0.0;
             for (long long q = o0 + sl; q < o1; q += LPG) {
-                const long long k = idx[q];
+                const long long k = ROWS ? q : (long long)idx[q];
                 if (k < 0 || k >= ng) continue;
                 double g0 = gv[3 * k + 0], g1 = gv[3 * k + 1], g2 = gv[3 * k + 2];
                 double i0, i1, i2;

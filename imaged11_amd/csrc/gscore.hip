@@ -372,9 +372,12 @@ int launch_xlylzl_xpos(const double* s, const double* f, const double* p, const 
     return GS_OK;
 }
 
-int check_refine_args(int K, int ng, long long n_idx, int n_iter) {
+int check_refine_args(int K, int ng, const int* idx, long long n_idx, int n_iter) {
     if (K < 0 || n_iter < 0 || ng < 0 || n_idx < 0)
         return set_err(GS_ERR_ARG, "negative K, n_iter, ng or n_idx");
+    if (!idx && n_idx > ng)
+        return set_err(GS_ERR_ARG, "without idx the rows (ng = %d) must cover the n_idx = %lld "
+                       "list entries", ng, n_idx);
     return GS_OK;
 }
 
@@ -391,12 +394,17 @@ int launch_refine_lists(DeviceCtx* ctx, double* ubis, int K, const double* gv, i
     long long waves = std::min<long long>(waves_needed, (long long)wpc * ctx->cus);
     int nb = (int)std::max<long long>(1, (waves + GS_BLOCK / GS_WAVE - 1) / (GS_BLOCK / GS_WAVE));
     const dim3 grid(nb), block(GS_BLOCK);
+#define GS_REFINE_LAUNCH(L, R)                                                                  \
+    hipLaunchKernelGGL((k_refine_lists<L, R>), grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, \
+                       order, tolsq, n_iter, n_out, mean_out, queue)
+    const bool rows = idx == nullptr;  // entry q is row q of gv
     switch (lpg) {
-        case 8: hipLaunchKernelGGL(k_refine_lists<8>, grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
-        case 32: hipLaunchKernelGGL(k_refine_lists<32>, grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
-        case 64: hipLaunchKernelGGL(k_refine_lists<64>, grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
-        default: hipLaunchKernelGGL(k_refine_lists<16>, grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out, queue); break;
+        case 8: if (rows) GS_REFINE_LAUNCH(8, true); else GS_REFINE_LAUNCH(8, false); break;
+        case 32: if (rows) GS_REFINE_LAUNCH(32, true); else GS_REFINE_LAUNCH(32, false); break;
+        case 64: if (rows) GS_REFINE_LAUNCH(64, true); else GS_REFINE_LAUNCH(64, false); break;
+        default: if (rows) GS_REFINE_LAUNCH(16, true); else GS_REFINE_LAUNCH(16, false); break;
     }
+#undef GS_REFINE_LAUNCH
     GS_LAUNCHED();
     return GS_OK;
 }
@@ -784,13 +792,26 @@ int gs_refine_lists(double* ubis, int K, const double* gv, int ng, const int* id
                     gs_stream_t stream) {
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
-    GS_CHECK(check_refine_args(K, ng, n_idx, n_iter));
+    GS_CHECK(check_refine_args(K, ng, idx, n_idx, n_iter));
     if (K == 0 || n_iter == 0) return GS_OK;
     hipStream_t st = S(stream);
     int* queue = queue_ws ? queue_ws : ctx->counters;
     GS_CHECK(fill_i32(queue, 0, 1, st));
     return launch_refine_lists(ctx, ubis, K, gv, ng, idx, n_idx, off, order, tol, n_iter, n_out,
                                mean_out, lanes_per_grain, waves_per_cu, queue, st);
+}
+
+int gs_gather_rows(const double* gv, int ng, const int* idx, long long n_idx, double* rows,
+                   gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (ng < 0 || n_idx < 0) return set_err(GS_ERR_ARG, "gs_gather_rows: negative ng or n_idx");
+    if (n_idx == 0) return GS_OK;
+    if (!idx || !rows) return set_err(GS_ERR_ARG, "gs_gather_rows: null idx or rows");
+    hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(n_idx, GS_BLOCK)), dim3(GS_BLOCK), 0,
+                       S(stream), gv, ng, idx, n_idx, rows);
+    GS_LAUNCHED();
+    return GS_OK;
 }
 
 // A captured multi-iteration refinement: one persistent refine launch per
@@ -811,7 +832,7 @@ int gs_refine_graph_create(gs_graph_t* out, double* ubis, int K, const double* g
     if (!queue_ws) return set_err(GS_ERR_ARG, "gs_refine_graph_create: queue_ws (n_iter ints) needed");
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
-    GS_CHECK(check_refine_args(K, ng, n_idx, n_iter));
+    GS_CHECK(check_refine_args(K, ng, idx, n_idx, n_iter));
     if (K == 0 || n_iter == 0) return set_err(GS_ERR_ARG, "gs_refine_graph_create: empty problem");
     hipStream_t st = S(stream);
     if (st == nullptr) return set_err(GS_ERR_ARG, "gs_refine_graph_create: capture needs a non-null stream");

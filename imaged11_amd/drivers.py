@@ -116,8 +116,6 @@ def grain_gvectors(scan, translations, labels, K):
 def refine_grains(ubis, gv_lists, off, tol, n_iter=2, config=None):
     """n_iter score_and_refine calls per grain on each grain's own list of
     g-vectors (gv_lists rows off[g]..off[g+1]).  Returns (ubis, n, mean)."""
-    M = gv_lists.shape[0]
-    ident = torch.arange(M, dtype=torch.int32, device=gv_lists.device)
     if config is None:
-        return batched.refine_lists(ubis, gv_lists, ident, off, tol, n_iter)
-    return refine.refine(ubis, gv_lists, ident, off, tol, n_iter, config=config)
+        return batched.refine_lists(ubis, gv_lists, None, off, tol, n_iter)
+    return refine.refine(ubis, gv_lists, None, off, tol, n_iter, config=config)

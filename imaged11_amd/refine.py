@@ -36,19 +36,31 @@ class RefineGraph:
     """n_iter score_and_refine passes over all grains, captured as a
     hipGraph on a private stream.  ``ubis`` is refined in place by each
     replay; ``n`` and ``mean`` (n_iter, K) hold the per-call results of the
-    last replay.  The tensors passed in must stay alive with the graph."""
+    last replay.  The tensors passed in must stay alive with the graph.
+
+    With an index array (and n_iter >= 2) the graph runs on the list rows
+    packed once by `batched.gather_rows`; each replay packs them again
+    first, so a replay sees the current contents of gv.  idx None: row q
+    of gv is list entry q."""
 
     def __init__(self, ubis, gv, idx, off, tol, n_iter, order=None, lanes_per_grain=0,
                  waves_per_cu=0):
         batched._need(ubis, torch.float64, "ubis", (None, 3, 3))
         batched._need(gv, torch.float64, "gv", (None, 3))
         K = ubis.shape[0]
-        batched._need(idx, torch.int32, "idx", (None,))
+        if idx is not None:
+            batched._need(idx, torch.int32, "idx", (None,))
         batched._need(off, torch.int64, "off", (K + 1,))
         if order is None:
             order = batched.list_order(off)
         batched._need(order, torch.int32, "order", (K,))
         self.ubis, self.gv, self.idx, self.off, self.order = ubis, gv, idx, off, order
+        self.rows = None
+        rows, ridx = gv, idx
+        if idx is not None and n_iter >= 2:
+            self.rows = rows = batched.gather_rows(gv, idx)
+            ridx = None
+        n_idx = rows.shape[0] if ridx is None else ridx.numel()
         dev = gv.device
         self.n = torch.zeros((n_iter, K), dtype=torch.int32, device=dev)
         self.mean = torch.zeros((n_iter, K), dtype=torch.float64, device=dev)
@@ -59,7 +71,8 @@ class RefineGraph:
         handle = ctypes.c_void_p()
         torch.cuda.current_stream(dev).synchronize()
         _lib.check(lib.gs_refine_graph_create(
-            ctypes.byref(handle), dptr(ubis), K, dptr(gv), gv.shape[0], dptr(idx), idx.numel(),
+            ctypes.byref(handle), dptr(ubis), K, dptr(rows), rows.shape[0],
+            None if ridx is None else dptr(ridx), n_idx,
             dptr(off), dptr(order), float(tol), int(n_iter), dptr(self.n), dptr(self.mean),
             int(lanes_per_grain), int(waves_per_cu), dptr(self.queue),
             stream_ptr(self.stream)), "refine graph capture")
@@ -67,7 +80,12 @@ class RefineGraph:
 
     def replay(self, stream=None):
         """Enqueue one replay on `stream` (default: torch's current stream)."""
-        _lib.check(_lib.load().gs_graph_launch(self.handle, stream_ptr(stream)), "graph launch")
+        lib = _lib.load()
+        if self.rows is not None:
+            _lib.check(lib.gs_gather_rows(dptr(self.gv), self.gv.shape[0], dptr(self.idx),
+                                          self.idx.numel(), dptr(self.rows),
+                                          stream_ptr(stream)), "gather_rows")
+        _lib.check(lib.gs_graph_launch(self.handle, stream_ptr(stream)), "graph launch")
 
     def close(self):
         if self.handle:

@@ -464,8 +464,7 @@ class refinegrains:
             else:
                 u = torch.as_tensor(np.ascontiguousarray(ubis, dtype=np.float64),
                                     device=st.device).reshape(-1, 3, 3)
-            ident = torch.arange(idx.numel(), dtype=torch.int32, device=st.device)
-            mats, _, _ = batched.refine_lists(u, gv, ident, off_d, self.tolerance, 1)
+            mats, _, _ = batched.refine_lists(u, gv, None, off_d, self.tolerance, 1)
             gv, tth, eta, omc = batched.grain_gv(st.xl, st.so, st.co, st.omega, p, trans_d, idx,
                                                  owner, mats=mats, slop=self.slop, angles=angles)
         return gv, tth, eta, omc
@@ -503,16 +502,15 @@ class refinegrains:
         pos_d = torch.as_tensor(pos, device=st.device)
         gv, _, _, _ = self._gv_device(sname, idx, owner, off_d, trans, None, False,
                                       ubis=st.ubi_cur[pos_d])
-        ident = torch.arange(idx.numel(), dtype=torch.int32, device=st.device)
         u = (st.ubi_read if start == "read" else st.ubi_cur)[pos_d].contiguous()
         if self.latticesymmetry is triclinic:
-            u, n, m = batched.refine_lists(u, gv, ident, off_d, self.tolerance, 2)
+            u, n, m = batched.refine_lists(u, gv, None, off_d, self.tolerance, 2)
             k = 1
         else:
-            u, _, _ = batched.refine_lists(u, gv, ident, off_d, self.tolerance, 1)
+            u, _, _ = batched.refine_lists(u, gv, None, off_d, self.tolerance, 1)
             u = torch.as_tensor(symmetrize(u.cpu().numpy(), self.latticesymmetry),
                                 device=st.device)
-            u, n, m = batched.refine_lists(u, gv, ident, off_d, self.tolerance, 1)
+            u, n, m = batched.refine_lists(u, gv, None, off_d, self.tolerance, 1)
             u = torch.as_tensor(symmetrize(u.cpu().numpy(), self.latticesymmetry),
                                 device=st.device)
             k = 0
@@ -546,12 +544,11 @@ class refinegrains:
         """Two score_and_refine calls on self.gv (refinegrains.py:459-498)."""
         dev = self.device
         gv = torch.as_tensor(np.ascontiguousarray(self.gv, dtype=np.float64), device=dev)
-        ident = torch.arange(gv.shape[0], dtype=torch.int32, device=dev)
         off = torch.tensor([0, gv.shape[0]], dtype=torch.int64, device=dev)
         u = torch.as_tensor(np.array(ubi, float).reshape(1, 3, 3), device=dev)
-        u, _, _ = batched.refine_lists(u, gv, ident, off, self.tolerance, 1)
+        u, _, _ = batched.refine_lists(u, gv, None, off, self.tolerance, 1)
         mat = symmetrize(u.cpu().numpy(), self.latticesymmetry)
-        u, n, m = batched.refine_lists(torch.as_tensor(mat, device=dev), gv, ident, off,
+        u, n, m = batched.refine_lists(torch.as_tensor(mat, device=dev), gv, None, off,
                                        self.tolerance, 1)
         self.npks, self.avg_drlv2 = int(n[0, 0]), float(m[0, 0])
         mat = symmetrize(u.cpu().numpy(), self.latticesymmetry)[0]

@@ -168,17 +168,32 @@ int gs_sf2gv(const double *s, const double *f, const double *omega, const double
  * used as the work-queue counter (NULL = a per-device counter of the
  * library, which calls overlapping on one device would share).  Entries
  * out of range are ignored, never dereferenced: grains outside [0, K) in
- * order, offsets outside [0, n_idx], peak indices outside [0, ng). */
+ * order, offsets outside [0, n_idx], peak indices outside [0, ng).
+ * idx may be NULL: list entry q is then row q of gv (rows gathered once by
+ * gs_gather_rows, or lists already stored contiguously), ng >= n_idx.
+ * Every iteration then reads contiguous rows instead of one cache line
+ * per entry; the sums see the same values in the same order, so the
+ * results are bit-identical to the indexed call. */
 int gs_refine_lists(double *ubis, int K, const double *gv, int ng, const int *idx,
                     long long n_idx, const long long *off, const int *order, double tol,
                     int n_iter, int *n_out, double *mean_out, int lanes_per_grain,
                     int waves_per_cu, int *queue_ws, gs_stream_t stream);
 
+/* rows[q] = gv[idx[q]] (3 doubles) for q < n_idx, a NaN row where idx[q] is
+ * outside [0, ng): the list rows of gs_refine_lists packed once, in list
+ * order.  A NaN row is never kept, as an out-of-range index is never kept.
+ * No counterpart in the reference (score_and_refine takes each grain's
+ * rows as its gv argument, src/closest.c:267-345). */
+int gs_gather_rows(const double *gv, int ng, const int *idx, long long n_idx, double *rows,
+                   gs_stream_t stream);
+
 /* The same refinement as a hipGraph: n_iter single-iteration persistent
  * launches (each a full score_and_refine pass over all grains) captured
  * once on `stream` and replayed with gs_graph_launch.  queue_ws must hold
  * n_iter device ints; all buffers must stay allocated while the graph is
- * used.  Replays start from whatever ubis holds. */
+ * used.  Replays start from whatever ubis holds.  idx may be NULL as for
+ * gs_refine_lists (then gather the rows again before a replay whenever gv
+ * has changed). */
 typedef struct gs_graph_s *gs_graph_t;
 int gs_refine_graph_create(gs_graph_t *out, double *ubis, int K, const double *gv, int ng,
                            const int *idx, long long n_idx, const long long *off,

@@ -112,3 +112,68 @@ def test_refine_lists_bad_entries_ignored(dev, orc):
                                      order=T(order_bad, dev, torch.int32))
     torch.cuda.synchronize()
     assert torch.equal(u2[1], T(ubi, dev))
+
+
+@pytest.mark.parametrize("lanes", [16, 64])
+def test_gathered_rows_bit_identical(dev, problem, lanes):
+    """The refinement on rows packed once (gather_rows, idx None) sees the
+    same values in the same order as the indexed one: bit-identical
+    results, bad indices included (a NaN row is never kept)."""
+    gv, idx, off = _dev(problem, dev)
+    idx = idx.clone()
+    idx[[5, 700, 4000]] = torch.tensor([-1, gv.shape[0], 2 ** 31 - 1], dtype=torch.int32,
+                                       device=dev)
+    start = T(problem["start"], dev)
+    ui, ni, mi = batched.refine_lists(start, gv, idx, off, 0.05, 3, lanes_per_grain=lanes,
+                                      gather=False)
+    ug, ng_, mg = batched.refine_lists(start, gv, idx, off, 0.05, 3, lanes_per_grain=lanes,
+                                       gather=True)
+    rows = batched.gather_rows(gv, idx)
+    ur, nr, mr = batched.refine_lists(start, rows, None, off, 0.05, 3, lanes_per_grain=lanes)
+    torch.cuda.synchronize()
+    assert torch.isnan(rows[[5, 700, 4000]]).all()
+    assert torch.equal(rows[6], gv[idx[6].long()])
+    for u, n, m in ((ug, ng_, mg), (ur, nr, mr)):
+        assert torch.equal(u, ui) and torch.equal(n, ni) and torch.equal(m, mi)
+
+
+def test_graph_replay_gathers_current_rows(dev, problem):
+    """A replay packs the rows again, so it sees gv as it is at the replay."""
+    gv, idx, off = _dev(problem, dev)
+    gv = gv.clone()
+    start = T(problem["start"], dev)
+    work = start.clone()
+    g = refine.RefineGraph(work, gv, idx, off, 0.05, 3)
+    assert g.rows is not None
+    g.replay()
+    torch.cuda.synchronize()
+    u1, _, _ = batched.refine_lists(start, gv, idx, off, 0.05, 3)
+    assert torch.equal(work, u1)
+    gv.mul_(1.0 + 1e-4)
+    work.copy_(start)
+    g.replay()
+    torch.cuda.synchronize()
+    u2, n2, m2 = batched.refine_lists(start, gv, idx, off, 0.05, 3)
+    assert torch.equal(work, u2) and torch.equal(g.n, n2) and torch.equal(g.mean, m2)
+    assert not torch.equal(u1, u2)
+    g.close()
+
+
+def test_rows_must_cover_the_lists(dev):
+    """Without an index the rows must cover every list entry."""
+    import ctypes
+    from imaged11_amd import _lib
+    from imaged11_amd._lib import dptr, stream_ptr
+    lib = _lib.load()
+    ubis = torch.eye(3, dtype=torch.float64, device=dev)[None].repeat(2, 1, 1)
+    rows = torch.zeros((10, 3), dtype=torch.float64, device=dev)
+    off = torch.tensor([0, 6, 11], dtype=torch.int64, device=dev)
+    order = torch.tensor([0, 1], dtype=torch.int32, device=dev)
+    n = torch.zeros((1, 2), dtype=torch.int32, device=dev)
+    m = torch.zeros((1, 2), dtype=torch.float64, device=dev)
+    q = torch.zeros(1, dtype=torch.int32, device=dev)
+    rc = lib.gs_refine_lists(dptr(ubis), 2, dptr(rows), 10, None, ctypes.c_longlong(11), dptr(off),
+                             dptr(order), 0.05, 1, dptr(n), dptr(m), 0, 0, dptr(q), stream_ptr())
+    assert rc != 0
+    assert b"cover" in lib.gs_last_error()
+    lib.gs_clear_error()
