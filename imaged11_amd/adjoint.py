@@ -76,7 +76,13 @@ def pack_geometry(params, fixed):
 class JointProblem:
     """Device-resident data of a joint refinement: the raw peak table
     (sc, fc and the per-peak sin/cos of omega), per-grain CSR peak lists and
-    the fixed integer hkl of every list entry."""
+    the fixed integer hkl of every list entry.
+
+    When every list index is a valid peak number, (sc, fc, sin, cos) are
+    copied once per list entry in list order, so each evaluation streams
+    them instead of gathering four cache lines per entry; with an
+    out-of-range index the lists stay indexed (such entries are skipped).
+    Either way the per-peak data are those of the construction."""
 
     def __init__(self, sc, fc, omega, omegasign, idx, off):
         batched._need(sc, torch.float64, "sc", (None,))
@@ -88,6 +94,11 @@ class JointProblem:
         self.sc, self.fc = sc, fc
         self.so, self.co = batched.sincos_omega(omega, omegasign)
         self.idx, self.off = idx, off
+        self.rows = None
+        m = idx.numel()
+        if m == 0 or (int(idx.min()) >= 0 and int(idx.max()) < n):
+            il = idx.long()
+            self.rows = tuple(a[il].contiguous() for a in (sc, fc, self.so, self.co))
         self.K = off.numel() - 1
         lengths = off[1:] - off[:-1]
         self.owner = torch.repeat_interleave(
@@ -110,12 +121,20 @@ class JointProblem:
         geo = geo.detach().to(self.sc.device, torch.float64).contiguous()
         ubis = ubis.detach().contiguous()
         trans = trans.detach().contiguous()
-        _lib.check(lib.gs_joint_hkl(dptr(self.sc), dptr(self.fc), dptr(self.so), dptr(self.co),
-                                    self.sc.shape[0], dptr(geo), dptr(ubis), dptr(trans), self.K,
-                                    dptr(self.idx), dptr(self.owner), self.idx.numel(),
-                                    dptr(self.hkl), stream_ptr()), "joint hkl")
+        (a, b, c, d), npk, idx = self._peaks()
+        _lib.check(lib.gs_joint_hkl(dptr(a), dptr(b), dptr(c), dptr(d), npk, dptr(geo),
+                                    dptr(ubis), dptr(trans), self.K, idx, dptr(self.owner),
+                                    self.idx.numel(), dptr(self.hkl), stream_ptr()), "joint hkl")
 
-    def evaluate(self, geo, ubis, trans, gspace=False, want_sums=False, lanes_per_grain=16):
+    def _peaks(self):
+        """(sc, fc, sin, cos), their length and the index pointer the
+        library takes: the per-entry copies without an index, or the peak
+        table with it."""
+        if self.rows is not None:
+            return self.rows, self.idx.numel(), None
+        return (self.sc, self.fc, self.so, self.co), self.sc.shape[0], dptr(self.idx)
+
+    def evaluate(self, geo, ubis, trans, gspace=False, want_sums=False, lanes_per_grain=64):
         """One fused forward + reverse pass.  Returns (loss, dL/dUBI (K,3,3),
         dL/dt (K,3), dL/dgeo (26,), sums) with sums = (K,15) Gauss-Newton
         sums or None.  In g-space mode the loss is 1/2 sum ||UB.h - G||^2
@@ -129,9 +148,10 @@ class JointProblem:
         gu = torch.empty((self.K, 3, 3), dtype=torch.float64, device=dev)
         gt = torch.empty((self.K, 3), dtype=torch.float64, device=dev)
         sums = torch.empty((self.K, 15), dtype=torch.float64, device=dev) if want_sums else None
+        (a, b, c, d), npk, idx = self._peaks()
         _lib.check(lib.gs_joint_adjoint(
-            dptr(self.sc), dptr(self.fc), dptr(self.so), dptr(self.co), self.sc.shape[0],
-            dptr(geo), dptr(ubis), dptr(trans), self.K, dptr(self.idx), self.idx.numel(),
+            dptr(a), dptr(b), dptr(c), dptr(d), npk,
+            dptr(geo), dptr(ubis), dptr(trans), self.K, idx, self.idx.numel(),
             dptr(self.off), dptr(self.hkl), dptr(self.order), int(bool(gspace)),
             dptr(self.loss_g), dptr(gu), dptr(gt), dptr(self.geo_part),
             dptr(sums) if want_sums else None, dptr(self.out), int(lanes_per_grain),

@@ -118,6 +118,7 @@ __device__ __forceinline__ void gs_joint_backward(double s, double f, double so,
 
 // Fixed integer hkl of every list entry at the current state:
 // hkl[q] = round(UBI_g . G_q).  One lane per list entry.
+template <bool ROWS>
 __global__ __launch_bounds__(GS_BLOCK) void k_joint_hkl(
     const double* __restrict__ sc, const double* __restrict__ fc, const double* __restrict__ so,
     const double* __restrict__ co, int npk, const double* __restrict__ geo_in,
@@ -129,7 +130,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_joint_hkl(
     __syncthreads();
     const long long q = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
     if (q >= n_idx) return;
-    const int k = idx[q], g = owner[q];
+    const long long k = ROWS ? q : (long long)idx[q];
+    const int g = owner[q];
     if (k < 0 || k >= npk || g < 0 || g >= K) {
         hkl[3 * q] = hkl[3 * q + 1] = hkl[3 * q + 2] = 0;
         return;
@@ -146,13 +148,16 @@ __global__ __launch_bounds__(GS_BLOCK) void k_joint_hkl(
 }
 
 // Per-grain forward + reverse sweep.  LPG lanes per grain, grains from a
-// work queue; every per-grain output is the butterfly sum over that
-// grain's lanes, so it does not depend on which wave took the grain.
+// work queue; every per-grain output is the gs_group_sum over that grain's
+// lanes, so it does not depend on which wave took the grain.
 // Outputs per grain g: loss_g[g], grad_ubi[9g..] (dL/dUBI, or dE/dUB in
 // g-space), grad_t[3g..], geo_part[26g..] (dL/dgeo restricted to grain g;
 // slot 16 holds dL/d(1/wavelength)), and with NSUMS the Gauss-Newton sums
 // sum h G^T (9) and sum G G^T (6: 00 01 02 11 12 22).
-template <int LPG, bool GSPACE, bool NSUMS>
+// ROWS: no index array; sc/fc/so/co are already in list order (entry q is
+// element q), so the pass streams them instead of gathering four cache
+// lines per entry.
+template <int LPG, bool GSPACE, bool NSUMS, bool ROWS>
 __global__ __launch_bounds__(GS_BLOCK) void k_joint_adjoint(
     const double* __restrict__ sc, const double* __restrict__ fc, const double* __restrict__ so,
     const double* __restrict__ co, int npk, const double* __restrict__ geo_in,
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_joint_adjoint(
         for (int i = 0; i < GS_GEO; i++) acc_geo[i] = 0.0;
         for (int i = 0; i < GS_NSUMS; i++) acc_n[i] = 0.0;
         for (long long q = o0 + sl; q < o1; q += LPG) {
-            const int k = idx[q];
+            const long long k = ROWS ? q : (long long)idx[q];
             if (k < 0 || k >= npk) continue;
             const double h0 = hkl[3 * q], h1 = hkl[3 * q + 1], h2 = hkl[3 * q + 2];
             gs_pair_fwd w;
@@ -236,12 +241,12 @@ __global__ __launch_bounds__(GS_BLOCK) void k_joint_adjoint(
             }
             gs_joint_backward(sc[k], fc[k], so[k], co[k], geo, w, gG, acc_geo, acc_t);
         }
-        acc_l = gs_wave_sum(acc_l, LPG);
-        for (int i = 0; i < 9; i++) acc_u[i] = gs_wave_sum(acc_u[i], LPG);
-        for (int i = 0; i < 3; i++) acc_t[i] = gs_wave_sum(acc_t[i], LPG);
-        for (int i = 0; i < GS_GEO; i++) acc_geo[i] = gs_wave_sum(acc_geo[i], LPG);
+        acc_l = gs_group_sum<LPG>(acc_l);
+        for (int i = 0; i < 9; i++) acc_u[i] = gs_group_sum<LPG>(acc_u[i]);
+        for (int i = 0; i < 3; i++) acc_t[i] = gs_group_sum<LPG>(acc_t[i]);
+        for (int i = 0; i < GS_GEO; i++) acc_geo[i] = gs_group_sum<LPG>(acc_geo[i]);
         if (NSUMS)
-            for (int i = 0; i < GS_NSUMS; i++) acc_n[i] = gs_wave_sum(acc_n[i], LPG);
+            for (int i = 0; i < GS_NSUMS; i++) acc_n[i] = gs_group_sum<LPG>(acc_n[i]);
         if (active && sl == 0) {
             loss_g[g] = acc_l;
             for (int i = 0; i < 9; i++) grad_ubi[9 * (size_t)g + i] = acc_u[i];

@@ -336,6 +336,52 @@ __device__ __forceinline__ double gs_wave_sum(double v, int width) {
     return v;
 }
 
+// A double moved across lanes by one DPP pattern (both 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double gs_dpp_f64(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// v + v of the partner lane 16 (SPAN 16) or 32 (SPAN 32) lanes away, via
+// v_permlane16_swap / v_permlane32_swap: after the swap one register holds
+// the lower partner's value and the other the upper partner's, in every
+// lane of the pair, so both lanes add the same two operands in the same
+// order.
+template <int SPAN>
+__device__ __forceinline__ double gs_swap_sum(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+    auto l = SPAN == 16 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                        : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    auto h = SPAN == 16 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                        : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    const double lower = __builtin_bit_cast(double, ((unsigned long long)h[0] << 32) | l[0]);
+    const double upper = __builtin_bit_cast(double, ((unsigned long long)h[1] << 32) | l[1]);
+    return lower + upper;
+}
+
+// Sum over aligned groups of WIDTH lanes (8, 16, 32 or 64), all lanes
+// active, every lane of a group ending with the bit-identical total: each
+// step adds two values already equal within their half-groups (IEEE
+// addition commutes).  Pairs: lane ^ 1 and ^ 2 (DPP quad_perm), the other
+// quad of an 8 (row_half_mirror), the other 8 of a row (row_mirror), the
+// other row of 32 lanes and the other 32 (permlane swaps).  All VALU: no
+// LDS round trip per step.
+template <int WIDTH>
+__device__ __forceinline__ double gs_group_sum(double v) {
+    static_assert(WIDTH == 8 || WIDTH == 16 || WIDTH == 32 || WIDTH == 64, "group width");
+    v += gs_dpp_f64<0xB1>(v);  // quad_perm [1,0,3,2]
+    v += gs_dpp_f64<0x4E>(v);  // quad_perm [2,3,0,1]
+    v += gs_dpp_f64<0x141>(v);  // row_half_mirror
+    if (WIDTH >= 16) v += gs_dpp_f64<0x140>(v);  // row_mirror
+    if (WIDTH >= 32) v = gs_swap_sum<16>(v);
+    if (WIDTH >= 64) v = gs_swap_sum<32>(v);
+    return v;
+}
+
 __device__ __forceinline__ int gs_wave_sum_int(int v, int width) {
     for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, GS_WAVE);
     return v;

@@ -16,6 +16,9 @@
 #define GS_GRAIN_CHUNK 1024  // grains per LDS counter tile
 #define GS_SPLIT_MAX 512     // grains per split of the all-pairs score/select tiles
 #define GS_NPART 17          // n, sum drlv2, R[9], H[6]
+#ifndef GS_REFINE_BATCH
+#define GS_REFINE_BATCH 8    // list rows per lane in flight in k_refine_lists (rows mode)
+#endif
 
 // ---------------------------------------------------------------------------
 // small utility kernels (used instead of hipMemset so every launch is
@@ -443,6 +446,24 @@ __global__ void k_refine_finalize(const double* __restrict__ partials, int nbloc
     }
 }
 
+// One score_and_refine term (src/closest.c:296-316): a peak within tol adds
+// to n, sum drlv2, R = sum g h^T and H = sum h h^T (the six distinct
+// entries), in the reference's expression order.
+__device__ __forceinline__ void gs_refine_add(const gs_ubi& U, double g0, double g1, double g2,
+                                              double tolsq, double* a) {
+    double i0, i1, i2;
+    double s = gs_drlv2(U, g0, g1, g2, i0, i1, i2);
+    if (s < tolsq) {
+        a[0] += 1.0;
+        a[1] += s;
+        a[2] += i0 * g0; a[3] += i1 * g0; a[4] += i2 * g0;
+        a[5] += i0 * g1; a[6] += i1 * g1; a[7] += i2 * g1;
+        a[8] += i0 * g2; a[9] += i1 * g2; a[10] += i2 * g2;
+        a[11] += i0 * i0; a[12] += i1 * i0; a[13] += i2 * i0;
+        a[14] += i1 * i1; a[15] += i2 * i1; a[16] += i2 * i2;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // rows[q] = gv[idx[q]] for the n entries of CSR peak lists, a NaN row where
 // idx[q] is outside [0, ng) (a NaN row scores NaN and is never kept, as an
@@ -469,9 +490,10 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gather_rows(const double* __restri
 // waves pull groups of 64/LPG grains from a work queue (grains visited in
 // `order`, longest list first), LPG lanes per grain.  Each grain runs
 // n_iter successive score_and_refine calls on its own list (the calls of
-// different grains are independent).  Sums are butterfly-reduced across
-// the LPG lanes, which leaves bit-identical totals in every lane, so each
-// lane solves the 3x3 system itself and keeps the updated UBI in registers.
+// different grains are independent).  Sums are reduced across the LPG
+// lanes by gs_group_sum, which leaves bit-identical totals in every lane, so
+// each lane solves the 3x3 system itself and keeps the updated UBI in
+// registers.
 // n_out/mean_out are [n_iter][K]; queue[0] must be zero at launch.
 // Out-of-range entries are ignored rather than trusted: a grain number
 // outside [0, K) in `order` is skipped, list bounds are clipped to
@@ -509,24 +531,35 @@ __global__ __launch_bounds__(GS_BLOCK) void k_refine_lists(
             double a[GS_NPART];
 #pragma unroll
             for (int i = 0; i < GS_NPART; i++) a[i] = 0.0;
-            for (long long q = o0 + sl; q < o1; q += LPG) {
-                const long long k = ROWS ? q : (long long)idx[q];
-                if (k < 0 || k >= ng) continue;
-                double g0 = gv[3 * k + 0], g1 = gv[3 * k + 1], g2 = gv[3 * k + 2];
-                double i0, i1, i2;
-                double s = gs_drlv2(U, g0, g1, g2, i0, i1, i2);
-                if (s < tolsq) {
-                    a[0] += 1.0;
-                    a[1] += s;
-                    a[2] += i0 * g0; a[3] += i1 * g0; a[4] += i2 * g0;
-                    a[5] += i0 * g1; a[6] += i1 * g1; a[7] += i2 * g1;
-                    a[8] += i0 * g2; a[9] += i1 * g2; a[10] += i2 * g2;
-                    a[11] += i0 * i0; a[12] += i1 * i0; a[13] += i2 * i0;
-                    a[14] += i1 * i1; a[15] += i2 * i1; a[16] += i2 * i2;
+            if (ROWS) {
+                // GS_REFINE_BATCH rows per lane in flight.  The row is
+                // clamped to the list's last one (in bounds: o1 <= n_idx <=
+                // ng), so the loads need no guard; the copies past the end
+                // are dropped.  The lane still adds its rows in list order.
+                for (long long q = o0 + sl; q < o1; q += (long long)GS_REFINE_BATCH * LPG) {
+                    double gg[GS_REFINE_BATCH][3];
+#pragma unroll
+                    for (int j = 0; j < GS_REFINE_BATCH; j++) {
+                        long long r = q + (long long)j * LPG;
+                        r = r < o1 ? r : o1 - 1;
+                        gg[j][0] = gv[3 * r + 0];
+                        gg[j][1] = gv[3 * r + 1];
+                        gg[j][2] = gv[3 * r + 2];
+                    }
+#pragma unroll
+                    for (int j = 0; j < GS_REFINE_BATCH; j++)
+                        if (q + (long long)j * LPG < o1)
+                            gs_refine_add(U, gg[j][0], gg[j][1], gg[j][2], tolsq, a);
+                }
+            } else {
+                for (long long q = o0 + sl; q < o1; q += LPG) {
+                    const long long k = idx[q];
+                    if (k < 0 || k >= ng) continue;
+                    gs_refine_add(U, gv[3 * k + 0], gv[3 * k + 1], gv[3 * k + 2], tolsq, a);
                 }
             }
 #pragma unroll
-            for (int i = 0; i < GS_NPART; i++) a[i] = gs_wave_sum(a[i], LPG);
+            for (int i = 0; i < GS_NPART; i++) a[i] = gs_group_sum<LPG>(a[i]);
             double R[9], H[9];
             gs_unpack_RH(a, R, H);
             gs_solve_ubi(u, R, H);

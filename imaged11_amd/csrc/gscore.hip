@@ -887,9 +887,16 @@ int gs_joint_hkl(const double* sc, const double* fc, const double* so, const dou
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
     if (npk < 0 || K < 0 || n_idx < 0) return set_err(GS_ERR_ARG, "gs_joint_hkl: negative size");
+    if (!idx && n_idx > npk)
+        return set_err(GS_ERR_ARG, "gs_joint_hkl: without idx the per-entry arrays (npk = %d) "
+                       "must cover the n_idx = %lld entries", npk, n_idx);
     if (n_idx == 0) return GS_OK;
-    hipLaunchKernelGGL(k_joint_hkl, dim3(blocks_for(n_idx, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
-                       sc, fc, so, co, npk, geo, ubis, trans, K, idx, owner, n_idx, hkl);
+    if (idx)
+        hipLaunchKernelGGL(k_joint_hkl<false>, dim3(blocks_for(n_idx, GS_BLOCK)), dim3(GS_BLOCK), 0,
+                           S(stream), sc, fc, so, co, npk, geo, ubis, trans, K, idx, owner, n_idx, hkl);
+    else
+        hipLaunchKernelGGL(k_joint_hkl<true>, dim3(blocks_for(n_idx, GS_BLOCK)), dim3(GS_BLOCK), 0,
+                           S(stream), sc, fc, so, co, npk, geo, ubis, trans, K, idx, owner, n_idx, hkl);
     GS_LAUNCHED();
     return GS_OK;
 }
@@ -903,24 +910,32 @@ int gs_joint_adjoint(const double* sc, const double* fc, const double* so, const
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
     if (npk < 0 || K < 0 || n_idx < 0) return set_err(GS_ERR_ARG, "gs_joint_adjoint: negative size");
+    if (!idx && n_idx > npk)
+        return set_err(GS_ERR_ARG, "gs_joint_adjoint: without idx the per-entry arrays (npk = %d) "
+                       "must cover the n_idx = %lld entries", npk, n_idx);
     hipStream_t st = S(stream);
     int* queue = queue_ws ? queue_ws : ctx->counters + 1;
     GS_CHECK(fill_i32(queue, 0, 1, st));
     int lpg = lanes_per_grain;
-    if (lpg != 16 && lpg != 32 && lpg != 64) lpg = 16;
+    if (lpg != 16 && lpg != 32 && lpg != 64) lpg = 64;
     const int gpw = GS_WAVE / lpg;
     long long waves = std::min<long long>((K + gpw - 1) / gpw, 8LL * ctx->cus);
     int nb = (int)std::max<long long>(1, (waves + GS_BLOCK / GS_WAVE - 1) / (GS_BLOCK / GS_WAVE));
     const dim3 grid(nb), block(GS_BLOCK);
-#define GS_ADJ_LAUNCH(L, GSP, NS)                                                              \
-    hipLaunchKernelGGL((k_joint_adjoint<L, GSP, NS>), grid, block, 0, st, sc, fc, so, co, npk, \
-                       geo, ubis, trans, K, idx, n_idx, off, hkl, order, loss_g, grad_ubi,     \
-                       grad_t, geo_part, nsums, queue)
+#define GS_ADJ_LAUNCH(L, GSP, NS)                                                                \
+    if (idx)                                                                                     \
+        hipLaunchKernelGGL((k_joint_adjoint<L, GSP, NS, false>), grid, block, 0, st, sc, fc, so, \
+                           co, npk, geo, ubis, trans, K, idx, n_idx, off, hkl, order, loss_g,    \
+                           grad_ubi, grad_t, geo_part, nsums, queue);                            \
+    else                                                                                         \
+        hipLaunchKernelGGL((k_joint_adjoint<L, GSP, NS, true>), grid, block, 0, st, sc, fc, so,  \
+                           co, npk, geo, ubis, trans, K, idx, n_idx, off, hkl, order, loss_g,    \
+                           grad_ubi, grad_t, geo_part, nsums, queue)
 #define GS_ADJ_LPG(GSP, NS)                         \
     switch (lpg) {                                  \
+        case 16: GS_ADJ_LAUNCH(16, GSP, NS); break; \
         case 32: GS_ADJ_LAUNCH(32, GSP, NS); break; \
-        case 64: GS_ADJ_LAUNCH(64, GSP, NS); break; \
-        default: GS_ADJ_LAUNCH(16, GSP, NS); break; \
+        default: GS_ADJ_LAUNCH(64, GSP, NS); break; \
     }
     if (K > 0) {
         if (gspace) {

@@ -222,7 +222,11 @@ int gs_graph_destroy(gs_graph_t graph);
  * (32 interleaved groups of grains, then the groups in order:
  * deterministic).  gspace != 0: the g-space objective
  * 1/2 sum ||UB.hkl - G||^2 (UB = UBI^-1), grad_ubi then holds dE/dUB =
- * UB.H - R with score_and_refine's R and H (src/closest.c:310-316). */
+ * UB.H - R with score_and_refine's R and H (src/closest.c:310-316).
+ * lanes_per_grain 16, 32 or 64 (anything else: 64) lanes share a grain.
+ * idx may be NULL in both calls: sc, fc, so and co are then per list entry,
+ * already in list order (entry q is element q, npk >= n_idx), and the pass
+ * streams them instead of gathering four cache lines per entry. */
 int gs_joint_hkl(const double *sc, const double *fc, const double *so, const double *co,
                  int npk, const double *geo, const double *ubis, const double *trans, int K,
                  const int *idx, const int *owner, long long n_idx, int *hkl,

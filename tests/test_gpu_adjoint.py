@@ -228,3 +228,38 @@ def test_joint_refinement_recovers_exact_geometry(exact_setup):
     assert hist[-1] < 0.05 * hist[0]
     for k in ("y_center", "z_center"):
         assert abs(float(final[k]) - truth[k]) < 0.5 * abs(start[k] - truth[k]), k
+
+
+def test_packed_entries_bit_identical_to_indexed(setup, dev):
+    """The per-entry copies (no index) give the indexed pass's values in
+    the same order: hkl, loss and every gradient bit for bit."""
+    s = setup
+    pk = s["wl"]["peaks"]
+    packed = s["prob"]
+    assert packed.rows is not None
+    indexed = adjoint.JointProblem(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev),
+                                   s["pars"]["omegasign"], T(s["idx"], dev, torch.int32),
+                                   T(s["off"], dev, torch.int64))
+    indexed.rows = None
+    indexed.assign_hkl(s["geo"], s["ubis"], s["trans"])
+    assert torch.equal(indexed.hkl, packed.hkl)
+    for gspace in (False, True):
+        a = packed.evaluate(s["geo"], s["ubis"], s["trans"], gspace=gspace, want_sums=True)
+        b = indexed.evaluate(s["geo"], s["ubis"], s["trans"], gspace=gspace, want_sums=True)
+        for x, y in zip(a, b):
+            assert torch.equal(x, y)
+
+
+def test_out_of_range_index_keeps_lists_indexed(setup, dev):
+    s = setup
+    pk = s["wl"]["peaks"]
+    idx = s["idx"].copy()
+    idx[7] = len(pk["sc"]) + 3
+    prob = adjoint.JointProblem(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev),
+                                s["pars"]["omegasign"], T(idx, dev, torch.int32),
+                                T(s["off"], dev, torch.int64))
+    assert prob.rows is None
+    prob.assign_hkl(s["geo"], s["ubis"], s["trans"])
+    loss, *_ = prob.evaluate(s["geo"], s["ubis"], s["trans"])
+    assert bool(torch.isfinite(loss))
+    assert prob.hkl[7].abs().sum() == 0
