@@ -305,25 +305,31 @@ __global__ void k_orient_hits(const double* __restrict__ gv, int ng, const int* 
     if (costheta) costheta[h] = ct;
 }
 
-// Peaks each of K UBIs indexes, with the reference's arithmetic: the pairs
-// with gs_drlv2 < thr (thr = tol^2 for score_and_refine's selection,
-// min(tol^2, 1) for getind's score_and_assign from drlv2 = 1).  cursor[g]
-// (zeroed by the caller) counts grain g's peaks; when idx is given they are
-// also written at idx[off[g] + slot] for slot < off[g+1] - off[g], in no
-// particular order.  Tiles as k_score_many: P peaks per lane in registers,
-// a split of gsplit grains per block; lane 0 of a wave takes the wave's
+// Peaks each of K UBIs indexes, decided as the reference's arithmetic
+// decides them: the pairs with gs_drlv2 < thr (thr = tol^2 for
+// score_and_refine's selection, min(tol^2, 1) for getind's score_and_assign
+// from drlv2 = 1).  cursor[g] (zeroed by the caller) counts grain g's
+// peaks; when idx is given they are also written at idx[off[g] + slot] for
+// slot < off[g+1] - off[g], in no particular order.  Tiles as
+// k_score_many_fast: P peaks per lane in registers, a split of gsplit
+// grains per block, the certified fract form against thr - 3/4 +- E
+// (gs_cert_window_fract, E per grain and block in LDS), the undecided
+// pairs settled with gs_drlv2 itself; lane 0 of a wave takes the wave's
 // slots with one atomic per grain.
 template <int P>
 __global__ __launch_bounds__(GS_BLOCK) void k_select_many(
     const double* __restrict__ ubis, int K, int gsplit, int nsplit, const double* __restrict__ gv,
     int ng, double thr, const long long* __restrict__ off, int* __restrict__ cursor,
     int* __restrict__ idx) {
+    __shared__ double lwin[2 * GS_SPLIT_MAX];  // (lo, hi) per grain of the split
+    __shared__ double lgm[GS_BLOCK / GS_WAVE];
     const int tile = gs_xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = tile / nsplit;
     const int split = tile - chunk * nsplit;
     const int g0 = split * gsplit;
     const int g1 = min(K, g0 + gsplit);
     double x[P], y[P], z[P];
+    double gm = 0.0;
     const long long base = (long long)chunk * (GS_BLOCK * P) + threadIdx.x;
 #pragma unroll
     for (int p = 0; p < P; p++) {
@@ -335,14 +341,33 @@ __global__ __launch_bounds__(GS_BLOCK) void k_select_many(
         } else {
             x[p] = y[p] = z[p] = __builtin_nan("");  // never selected
         }
+        gm = fmax(gm, fmax(gs_abs_finite(x[p]), fmax(gs_abs_finite(y[p]), gs_abs_finite(z[p]))));
     }
     const int lane = threadIdx.x & (GS_WAVE - 1);
+    gm = gs_wave_max(gm);  // +inf if any peak of the wave is infinite
+    if (lane == 0) lgm[threadIdx.x / GS_WAVE] = gm;
+    __syncthreads();
+    double gmax = lgm[0];
+#pragma unroll
+    for (int w = 1; w < GS_BLOCK / GS_WAVE; w++) gmax = fmax(gmax, lgm[w]);
+    const double mid = thr - 0.75;  // gs_drlv2_fract is drlv2 - 3/4
+    for (int i = threadIdx.x; i < g1 - g0; i += GS_BLOCK) {
+        const double e = gs_cert_window_fract(gs_load_ubi(ubis + 9 * (size_t)(g0 + i)), gmax, thr);
+        lwin[2 * i] = mid - e;
+        lwin[2 * i + 1] = mid + e;
+    }
+    __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
     for (int g = g0; g < g1; g++) {
         const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)g);
+        const double lo = lwin[2 * (g - g0)], hi = lwin[2 * (g - g0) + 1];
 #pragma unroll
         for (int p = 0; p < P; p++) {
-            const bool pass = gs_drlv2(u, x[p], y[p], z[p]) < thr;
+            const double s = gs_drlv2_fract(u, x[p], y[p], z[p]);
+            bool pass = s < lo;
+            const bool und = (s < hi) && !pass;
+            if (__ballot(und))  // wave-uniform and rare: the reference arithmetic decides
+                pass = pass || (und && gs_drlv2(u, x[p], y[p], z[p]) < thr);
             const unsigned long long m = __ballot(pass);
             if (m == 0) continue;
             int first = 0;

@@ -1299,4 +1299,56 @@ int gs_select_many(const double* ubis, int K, const double* gv, int ng, double t
     return GS_OK;
 }
 
+// The decision loop of indexer.scorethem (ImageD11/indexing.py:778-838) on
+// host arrays, over follow-ups computed beforehand for every hit: visited
+// from the last hit to the first, a hit is skipped when either peak is
+// already assigned or i == j, or when its trial score is <= minpks; its
+// follow-up's peak list then becomes a grain if the fraction of those peaks
+// not yet assigned exceeds `uniqueness` (0/0 is NaN: never).  Sequential by
+// definition (each accept marks peaks the later hits test), so it runs on
+// the host, in native code.
+int gs_index_decide(int nhits, const int* hit_i, const int* hit_j, const int* npk, int minpks,
+                    const int* cand, const int* lists, const long long* list_off, int ncand,
+                    double uniqueness, int max_grains, int label_base, int* ga, int npeaks,
+                    int* accepted, int* n_accepted, int* stop_at, int* nuniq) {
+    if (nhits < 0 || npeaks < 0 || ncand < 0)
+        return set_err(GS_ERR_ARG, "gs_index_decide: negative size");
+    if (!n_accepted || !stop_at || !nuniq)
+        return set_err(GS_ERR_ARG, "gs_index_decide: null output");
+    int h = nhits, ng = 0, nu = 0;
+    while (h > 0 && ng < max_grains) {
+        h--;
+        const int i = hit_i[h], j = hit_j[h];
+        if (i < 0 || i >= npeaks || j < 0 || j >= npeaks)
+            return set_err(GS_ERR_ARG, "gs_index_decide: hit %d names peak %d / %d outside [0, %d)",
+                           h, i, j, npeaks);
+        if (ga[i] > -1 || ga[j] > -1 || i == j) continue;
+        if (npk[h] <= minpks) continue;
+        const int c = cand[h];
+        if (c < 0 || c >= ncand)
+            return set_err(GS_ERR_ARG, "gs_index_decide: hit %d has no follow-up", h);
+        const long long a = list_off[c], b = list_off[c + 1];
+        long long free = 0;
+        for (long long q = a; q < b; q++) {
+            const int k = lists[q];
+            if (k < 0 || k >= npeaks)
+                return set_err(GS_ERR_ARG, "gs_index_decide: peak %d outside [0, %d)", k, npeaks);
+            free += ga[k] == -1;
+        }
+        const double uniq = (double)free * 1.0 / (double)(b - a);
+        if (uniq > uniqueness) {
+            const int label = label_base + ng + 1;
+            for (long long q = a; q < b; q++) ga[lists[q]] = label;
+            accepted[ng] = h;
+            ng++;
+        } else {
+            nu++;
+        }
+    }
+    *n_accepted = ng;
+    *stop_at = h;
+    *nuniq = nu;
+    return GS_OK;
+}
+
 }  // extern "C"

@@ -21,7 +21,8 @@ Here:
   ``gs_select_many`` extracts, and the ``getind`` lists by
   ``gs_select_many`` again.  The reference's decision loop then visits the
   hits in its order (last first), skips the same ones and applies the
-  uniqueness rule to those precomputed results.
+  uniqueness rule to those precomputed results, in native host code
+  (``gs_index_decide``: the loop is sequential by definition).
 
 Results (hits, grains, scores, peak assignments) are the reference's for
 the same g-vectors, unit-cell tables and parameters; see the parity tests.
@@ -155,6 +156,19 @@ def select_lists(ubis, gv, thr):
         key = torch.sort(owner * N + idx[:total].to(torch.int64)).values
         idx = (key - owner * N).to(torch.int32)
     return idx[:total], off
+
+
+class _Followups:
+    """Every qualifying hit's follow-up: slot[h] = its row (or -1), and per
+    row the refined UBI, the score, and the getind peak list idx[off[c] ..
+    off[c+1])."""
+
+    def __init__(self, slot, ubi, npk, idx, off):
+        self.slot, self.ubi, self.npk, self.idx, self.off = slot, ubi, npk, idx, off
+
+    def __call__(self, h, _trial=None, _npk=None):
+        c = self.slot[h]
+        return self.ubi[c].copy(), int(self.npk[c]), self.idx[self.off[c]:self.off[c + 1]]
 
 
 class indexer:
@@ -322,8 +336,9 @@ class indexer:
         batched_followup (default) the follow-up of every hit above minpks
         (orientation variants, score_and_refine, the getind peak list) is
         also computed for all of them at once, since none of it depends on
-        earlier decisions; the reference's decision loop then only reads
-        those results.  batched_followup=False runs the follow-ups one hit
+        earlier decisions; the reference's decision loop (gs_index_decide,
+        native) then only reads those results.  Ends, as the reference does,
+        with `refine` of the best grain (scorelastrefined, fitlastrefined).  batched_followup=False runs the follow-ups one hit
         at a time inside the loop, as the reference does."""
         if self.hits is None or len(self.hits) == 0:
             logging.info("No hits to score")
@@ -336,8 +351,10 @@ class indexer:
         npk_all = batched.score_many(trial, g, tol).cpu().numpy()
         if batched_followup:
             follow = self._followups(g, tol, npk_all)
-        else:
-            follow = self._followup_one
+            self._decide(follow, npk_all)
+            self._refine_best()
+            return
+        follow = self._followup_one
         trial = trial.cpu().numpy()
         ng = 0
         self.nuniq = 0
@@ -362,8 +379,90 @@ class indexer:
                 self.nuniq += 1
         # the reference pops every visited hit off its list
         self.hits = self.hits[:h]
+        self._refine_best()
+
+    def _decide(self, follow, npk_all):
+        """The decision loop over precomputed follow-ups, in native code
+        (gs_index_decide): the same visiting order, skips and uniqueness
+        rule as the reference's loop (ImageD11/indexing.py:778-838)."""
+        import ctypes
+        lib = _lib.load()
+        hits = self.hits
+        nh = len(hits)
+        hi = np.ascontiguousarray([x[1] for x in hits], dtype=np.int32)
+        hj = np.ascontiguousarray([x[2] for x in hits], dtype=np.int32)
+        npk_h = np.ascontiguousarray(npk_all, dtype=np.int32)
+        if follow is None:
+            slot = np.full(nh, -1, np.int32)
+            lists, off, ncand = np.zeros(1, np.int32), np.zeros(1, np.int64), 0
+        else:
+            slot = np.ascontiguousarray(follow.slot, dtype=np.int32)
+            lists = np.ascontiguousarray(follow.idx, dtype=np.int32)
+            off = np.ascontiguousarray(follow.off, dtype=np.int64)
+            ncand = len(off) - 1
+        ga = np.ascontiguousarray(self.ga, dtype=np.int32)
+        accepted = np.empty(max(nh, 1), np.int32)
+        na, stop, nu = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.gs_clear_error()
+        a = lambda x: x.ctypes.data
+        _lib.check(lib.gs_index_decide(
+            nh, a(hi), a(hj), a(npk_h), int(self.minpks), a(slot), a(lists), a(off), ncand,
+            float(self.uniqueness), int(min(self.max_grains, 2 ** 31 - 1)), len(self.scores),
+            a(ga), len(ga), a(accepted), ctypes.byref(na), ctypes.byref(stop),
+            ctypes.byref(nu)), "index_decide")
+        self.ga[:] = ga
+        for h in accepted[:na.value]:
+            c = follow.slot[h]
+            self.ubis.append(follow.ubi[c].copy())
+            self.scores.append(int(follow.npk[c]))
+        self.nuniq = nu.value
+        # the reference pops every visited hit off its list
+        self.hits = hits[:stop.value]
+
+    def _refine_best(self):
+        """The end of the reference's scorethem (ImageD11/indexing.py:
+        839-874): refine the best-scoring grain for the log."""
         logging.info("Number of orientations with more than %d peaks is %d",
                      self.minpks, len(self.ubis))
+        if len(self.ubis) > 0:
+            best = int(np.argmax(self.scores))
+            self.refine(self.ubis[best])
+            logging.info("Indexes %d peaks, with <drlv2>=%f", self.scorelastrefined,
+                         self.fitlastrefined)
+
+    def refine(self, UBI):
+        """Refine an orientation on the ring-assigned peaks it indexes and
+        rescore it (ImageD11/indexing.py:1099-1167): R = sum g h^T and H =
+        sum h h^T over the peaks with drlv2 < hkl_tol^2 on a ring, UB =
+        R H^-1, then scorelastrefined / fitlastrefined of the new UBI.
+        Returns the refined UBI (the input when H or UB is singular).
+
+        The reference rounds with floor(h + 1/2), the kernels half to even;
+        they differ only at exact half-integers, where drlv2 >= 1/4 exceeds
+        any hkl_tol^2 below 1/4, so the selected peaks are the same."""
+        g = self._gvd()
+        tol = float(self.hkl_tol)
+        if not tol * tol < 0.25:
+            raise ValueError("indexer.refine needs hkl_tol < 0.5")
+        ring = torch.as_tensor(np.asarray(self.ra) > -1, device=g.device)
+        u = torch.as_tensor(np.array(UBI, dtype=np.float64).reshape(1, 3, 3), device=g.device)
+
+        def on_rings(ubi):
+            idx, off = select_lists(ubi, g, tol * tol)
+            idx = idx[ring[idx.long()]]
+            return idx, torch.tensor([0, idx.numel()], dtype=torch.int64, device=g.device)
+        idx, off = on_rings(u)
+        if idx.numel() == 0:
+            raise Exception("No contributing reflections for" + str(UBI))
+        uo, _, _ = batched.refine_lists(u, g, idx, off, tol, n_iter=1)
+        idx, off = on_rings(uo)
+        self.scorelastrefined = int(idx.numel())
+        if idx.numel() == 0:
+            raise ValueError("No contributing reflections for ubi:\n{}\nafter "
+                             "refinement".format(UBI))
+        _, _, mean = batched.refine_lists(uo, g, idx, off, tol, n_iter=1)
+        self.fitlastrefined = float(np.sqrt(float(mean[0, 0])))
+        return uo[0].cpu().numpy()
 
     def _followup_one(self, h, UBI, npk):
         """The reference's follow-up of one hit above minpks
@@ -388,8 +487,8 @@ class indexer:
         return UBI, npk, self.getind(UBI)
 
     def _followups(self, g, tol, npk_all):
-        """The follow-up of every hit above minpks at once; returns a
-        function h -> (UBI, npk, peak indices) reading the results."""
+        """The follow-up of every hit above minpks at once (a _Followups,
+        or None when no hit qualifies)."""
         hits = self.hits
         hi = np.array([x[1] for x in hits])
         hj = np.array([x[2] for x in hits])
@@ -416,7 +515,8 @@ class indexer:
                               entries=cols).cpu().numpy()
             starts = np.searchsorted(rows, np.arange(len(cand) + 1))
             lists, owners = [], []
-            for c in range(len(cand)):
+            # a candidate with one variant keeps its trial orientation
+            for c in np.nonzero(np.diff(starts) > 1)[0]:
                 vs = list(var[starts[c]:starts[c + 1]])
                 if len(vs) > 1:
                     vs = unitcell_mod.ubi_equiv(vs, [np.linalg.inv(v) for v in vs])
@@ -427,9 +527,8 @@ class indexer:
                 owners = np.array(owners)
                 sc = batched.score_many(torch.as_tensor(np.array(lists), device=g.device), g,
                                         tol).cpu().numpy()
-                firsts = np.searchsorted(owners, np.unique(owners))
-                for c, f in zip(np.unique(owners), firsts):
-                    n = np.sum(owners == c)
+                uniq, firsts, counts = np.unique(owners, return_index=True, return_counts=True)
+                for c, f, n in zip(uniq, firsts, counts):
                     choice = int(np.argmax(sc[f:f + n]))
                     if sc[f + choice] >= npk[c]:
                         ubi[c] = lists[f + choice]
@@ -441,13 +540,8 @@ class indexer:
         refined, _, _ = batched.refine_lists(dubi, g, idx, off, tol, n_iter=1)
         # getind: score_and_assign from drlv2 = 1, labels = 0, label 1
         idx2, off2 = select_lists(refined, g, min(tol * tol, 1.0))
-        refined = refined.cpu().numpy()
-        idx2, off2 = idx2.cpu().numpy(), off2.cpu().numpy()
-
-        def result(h, _trial, _npk):
-            c = slot[h]
-            return refined[c].copy(), int(npk[c]), idx2[off2[c]:off2[c + 1]]
-        return result
+        return _Followups(slot, refined.cpu().numpy(), npk, idx2.cpu().numpy(),
+                          off2.cpu().numpy())
 
     def fight_over_peaks(self):
         """Give each peak to the grain indexing it best, then refine each

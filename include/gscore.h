@@ -420,6 +420,20 @@ int gs_orient_hits(const double *gv, int ng, const int *hi, const int *hj, int n
 int gs_select_many(const double *ubis, int K, const double *gv, int ng, double thr,
                    const long long *off, int *cursor, int *idx, gs_stream_t stream);
 
+/* The decision loop of indexer.scorethem (ImageD11/indexing.py:778-838) on
+ * HOST arrays, once every hit's follow-up is known: hits h = nhits-1 .. 0
+ * (hit_i/hit_j their two peaks, npk their trial scores, cand[h] the row of
+ * their follow-up or -1); row c's peak list is lists[list_off[c] ..
+ * list_off[c+1]).  ga (npeaks, -1 = unassigned) is updated in place: an
+ * accepted hit's peaks get label_base + (its rank) + 1.  accepted (nhits
+ * ints) receives the accepted hits in order, *n_accepted their number,
+ * *stop_at the first hit not visited (the reference pops every visited
+ * hit), *nuniq the hits rejected as not unique.  No device work. */
+int gs_index_decide(int nhits, const int *hit_i, const int *hit_j, const int *npk, int minpks,
+                    const int *cand, const int *lists, const long long *list_off, int ncand,
+                    double uniqueness, int max_grains, int label_base, int *ga, int npeaks,
+                    int *accepted, int *n_accepted, int *stop_at, int *nuniq);
+
 #ifdef __cplusplus
 }
 #endif

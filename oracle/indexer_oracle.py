@@ -73,40 +73,103 @@ def getind(orc, gv, ubi, tol):
     return labels == 1
 
 
-def scorethem(orc, uc, gv, ga, hits, ring_1, ring_2, hkl_tol, minpks, cosine_tol, uniqueness,
-              max_grains):
-    """The reference's decision loop, hit by hit from the end of the list.
-    ga is updated in place; returns (ubis, scores, hits left)."""
+def decide(hits, ga, trial_npk, follow, minpks, uniqueness, max_grains, label_base=0):
+    """The reference's decision loop (ImageD11/indexing.py:778-838), hit by
+    hit from the end of the list.  trial_npk(h) is hit h's trial score and
+    follow(h, npk) its follow-up (UBI, npk, peak selection), both evaluated
+    only when the reference would.  ga is updated in place; an accepted
+    hit's peaks get label_base + its rank + 1.  Returns (ubis, scores,
+    accepted hits, hits left, number rejected as not unique)."""
     hits = list(hits)
-    ubis, scores = [], []
-    ng = 0
-    tol = float(hkl_tol)
+    ubis, scores, accepted = [], [], []
+    ng = nuniq = 0
     while len(hits) > 0 and ng < max_grains:
         _, i, j = hits.pop()
+        h = len(hits)
         if ga[i] > -1 or ga[j] > -1 or i == j:
             continue
-        uc.orient(ring_1, gv[i, :], ring_2, gv[j, :], verbose=0)
-        npk = orc.score(uc.UBI, gv, tol)
-        UBI = uc.UBI.copy()
+        npk = trial_npk(h)
         if npk > minpks:
-            uc.orient(ring_1, gv[i, :], ring_2, gv[j, :], verbose=0, crange=abs(cosine_tol))
-            if len(uc.UBIlist) > 1:
-                npks = [orc.score(u, gv, tol) for u in uc.UBIlist]
-                choice = int(np.argmax(npks))
-                if npks[choice] >= npk:
-                    UBI = uc.UBIlist[choice].copy()
-                    npk = npks[choice]
-            orc.score_and_refine(UBI, gv, tol)
-            ind = getind(orc, gv, UBI, tol)
+            UBI, npk, ind = follow(h, npk)
             g = ga[ind]
             with np.errstate(invalid="ignore", divide="ignore"):
                 uniq = np.sum(np.where(g == -1, 1, 0)) * 1.0 / g.shape[0]
             if uniq > uniqueness:
-                ga[ind] = len(scores) + 1
+                ga[ind] = label_base + len(scores) + 1
                 ubis.append(UBI)
                 scores.append(npk)
+                accepted.append(h)
                 ng += 1
-    return ubis, scores, hits
+            else:
+                nuniq += 1
+    return ubis, scores, accepted, hits, nuniq
+
+
+def scorethem(orc, uc, gv, ga, hits, ring_1, ring_2, hkl_tol, minpks, cosine_tol, uniqueness,
+              max_grains):
+    """The reference's scorethem (ImageD11/indexing.py:762-838): the
+    decision loop with each hit's orientation, score and follow-up computed
+    when it is visited.  ga is updated in place; returns (ubis, scores,
+    hits left)."""
+    tol = float(hkl_tol)
+    trial = {}
+
+    def trial_npk(h):
+        _, i, j = hits[h]
+        uc.orient(ring_1, gv[i, :], ring_2, gv[j, :], verbose=0)
+        trial[h] = uc.UBI.copy()
+        return orc.score(uc.UBI, gv, tol)
+
+    def follow(h, npk):
+        _, i, j = hits[h]
+        UBI = trial[h]
+        uc.orient(ring_1, gv[i, :], ring_2, gv[j, :], verbose=0, crange=abs(cosine_tol))
+        if len(uc.UBIlist) > 1:
+            npks = [orc.score(u, gv, tol) for u in uc.UBIlist]
+            choice = int(np.argmax(npks))
+            if npks[choice] >= npk:
+                UBI = uc.UBIlist[choice].copy()
+                npk = npks[choice]
+        orc.score_and_refine(UBI, gv, tol)
+        return UBI, npk, getind(orc, gv, UBI, tol)
+    ubis, scores, _, left, _ = decide(hits, ga, trial_npk, follow, minpks, uniqueness,
+                                      max_grains)
+    return ubis, scores, left
+
+
+def calc_drlv2(ubi, gv):
+    """drlv2 with floor(h + 1/2) rounding (ImageD11/indexing.py:186-197)."""
+    h = np.dot(ubi, np.transpose(gv))
+    d = h - np.floor(h + 0.5)
+    return np.sum(d * d, 0)
+
+
+def refine(gv, ra, ubi, hkl_tol):
+    """indexer.refine (ImageD11/indexing.py:1099-1167): UB = R H^-1 over
+    the ring-assigned peaks with drlv2 < hkl_tol^2 (the input UBI when a
+    matrix is singular), then (refined UBI, scorelastrefined,
+    fitlastrefined)."""
+    tol = float(hkl_tol) ** 2
+    drlv2 = calc_drlv2(ubi, gv)
+    h = np.dot(ubi, np.transpose(gv))
+    hint = np.floor(h + 0.5)
+    ind = np.nonzero((drlv2 < tol) & (ra > -1))[0]
+    if len(ind) == 0:
+        raise Exception("No contributing reflections")
+    R = np.zeros((3, 3))
+    H = np.zeros((3, 3))
+    for i in ind:
+        R = R + np.outer(gv[i], hint[:, i])
+        H = H + np.outer(hint[:, i], hint[:, i])
+    try:
+        ubio = np.linalg.inv(np.dot(R, np.linalg.inv(H)))
+    except np.linalg.LinAlgError:
+        ubio = ubi
+    drlv2 = calc_drlv2(ubio, gv)
+    ind = np.nonzero((drlv2 < tol) & (ra > -1))[0]
+    if len(ind) == 0:
+        raise ValueError("No contributing reflections after refinement")
+    return ubio, len(ind), float(np.sqrt(np.sum(drlv2[ind]) / len(ind)))
 
 
 def fight_over_peaks(orc, gv, ubis, hkl_tol):

@@ -117,6 +117,13 @@ def _compare(ind, res):
     for a, b in zip(ind.ubis, res["ubis"]):
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-10)
     assert np.array_equal(ind.ga, res["ga"])
+    if res["ubis"]:
+        # the end of scorethem: indexer.refine of the best grain
+        best = int(np.argmax(res["scores"]))
+        u, n, fit = IO.refine(ind.gv, np.asarray(ind.ra), res["ubis"][best], ind.hkl_tol)
+        assert ind.scorelastrefined == n
+        assert abs(ind.fitlastrefined - fit) <= 1e-8 * fit
+        np.testing.assert_allclose(ind.refine(res["ubis"][best]), u, rtol=0, atol=1e-10)
 
 
 def test_indexer_simAl_20_grains(orc):
