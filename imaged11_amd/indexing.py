@@ -152,7 +152,7 @@ def select_lists(ubis, gv, thr):
         _lib.check(lib.gs_select_many(dptr(ubis), K, dptr(gv), N, float(thr), dptr(off),
                                       dptr(cnt), dptr(idx), stream_ptr()), "select_many")
         owner = torch.repeat_interleave(torch.arange(K, device=dev, dtype=torch.int64),
-                                        cnt.to(torch.int64))
+                                        cnt.to(torch.int64), output_size=total)
         key = torch.sort(owner * N + idx[:total].to(torch.int64)).values
         idx = (key - owner * N).to(torch.int32)
     return idx[:total], off
