@@ -376,10 +376,19 @@ __global__ __launch_bounds__(GS_BLOCK) void k_assign_many_fast(
 // solves UB = R.H^-1, UBI = UB^-1 in place and writes n and the mean drlv2.
 // MODE 0: peaks with drlv2 < tol^2 (score_and_refine, src/closest.c:267-345)
 // MODE 1: peaks with labels == label (refine_assigned, src/closest.c:416-474)
-template <int MODE>
+// FUSED (a grid of one block): the block finishes the job itself, with the
+// finalize's arithmetic, so a fused launch gives the bits of the two-pass
+// one-block launch; ubi is then read by every thread first and written by
+// thread 0 after the block's barrier.
+__device__ __forceinline__ void gs_unpack_RH(const double* t, double* R, double* H);
+__device__ __forceinline__ void gs_refine_finish(const double* tot, double* ubi, int* n_out,
+                                                 double* mean_out);
+
+template <int MODE, bool FUSED>
 __global__ __launch_bounds__(GS_BLOCK) void k_refine_partials(
-    const double* __restrict__ ubi, const double* __restrict__ gv, int ng, double tolsq,
-    const int* __restrict__ labels, int label, double* __restrict__ partials) {
+    double* ubi, const double* __restrict__ gv, int ng, double tolsq,
+    const int* __restrict__ labels, int label, double* __restrict__ partials,
+    int* __restrict__ n_out, double* __restrict__ mean_out) {
     const gs_ubi u = gs_load_ubi(ubi);
     double a[GS_NPART];
 #pragma unroll
@@ -408,10 +417,18 @@ __global__ __launch_bounds__(GS_BLOCK) void k_refine_partials(
         if (lane == 0) red[wave][i] = v;
     }
     __syncthreads();
+    __shared__ double tot[GS_NPART];
     if (threadIdx.x < GS_NPART) {
         double v = red[0][threadIdx.x];
         for (int w = 1; w < GS_BLOCK / GS_WAVE; w++) v += red[w][threadIdx.x];
-        partials[(size_t)blockIdx.x * GS_NPART + threadIdx.x] = v;
+        if (FUSED)
+            tot[threadIdx.x] = 0.0 + v;  // the finalize's sum over one block
+        else
+            partials[(size_t)blockIdx.x * GS_NPART + threadIdx.x] = v;
+    }
+    if (FUSED) {
+        __syncthreads();
+        if (threadIdx.x == 0) gs_refine_finish(tot, ubi, n_out, mean_out);
     }
 }
 
@@ -421,6 +438,20 @@ __device__ __forceinline__ void gs_unpack_RH(const double* t, double* R, double*
     H[0] = t[11]; H[1] = t[12]; H[2] = t[13];
     H[3] = t[12]; H[4] = t[14]; H[5] = t[15];
     H[6] = t[13]; H[7] = t[15]; H[8] = t[16];
+}
+
+// Solve from the 17 totals, UBI in place, n and the mean drlv2 (one thread).
+__device__ __forceinline__ void gs_refine_finish(const double* tot, double* ubi, int* n_out,
+                                                 double* mean_out) {
+    double R[9], H[9], u[9];
+    gs_unpack_RH(tot, R, H);
+    for (int i = 0; i < 9; i++) u[i] = ubi[i];
+    gs_solve_ubi(u, R, H);
+    for (int i = 0; i < 9; i++) ubi[i] = u[i];
+    int n = (int)tot[0];
+    double s = tot[1];
+    *n_out = n;
+    *mean_out = n > 0 ? s / n : s;
 }
 
 __global__ void k_refine_finalize(const double* __restrict__ partials, int nblocks,
@@ -433,17 +464,7 @@ __global__ void k_refine_finalize(const double* __restrict__ partials, int nbloc
         tot[threadIdx.x] = v;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double R[9], H[9], u[9];
-        gs_unpack_RH(tot, R, H);
-        for (int i = 0; i < 9; i++) u[i] = ubi[i];
-        gs_solve_ubi(u, R, H);
-        for (int i = 0; i < 9; i++) ubi[i] = u[i];
-        int n = (int)tot[0];
-        double s = tot[1];
-        *n_out = n;
-        *mean_out = n > 0 ? s / n : s;
-    }
+    if (threadIdx.x == 0) gs_refine_finish(tot, ubi, n_out, mean_out);
 }
 
 // One score_and_refine term (src/closest.c:296-316): a peak within tol adds

@@ -185,6 +185,45 @@ int stage_slices(std::initializer_list<size_t> sizes, std::vector<char*>& out) {
     return GS_OK;
 }
 
+// Small host-pointer calls go through pinned, device-mapped host memory,
+// one growable buffer per host thread: the kernels read the caller's data
+// and write their results there directly, so a call costs no copy
+// launches.  Used up to GS_PINNED_MAX bytes; larger inputs are copied to the
+// device staging buffer instead.
+constexpr size_t GS_PINNED_MAX = size_t(1) << 20;
+struct Pinned {
+    char* host = nullptr;
+    char* dev = nullptr;  // the same memory as the device addresses it
+    size_t cap = 0;
+};
+thread_local Pinned g_pinned;
+
+int pinned_slices(std::initializer_list<size_t> sizes, std::vector<char*>& host,
+                  std::vector<char*>& dev) {
+    size_t need = 0;
+    for (size_t b : sizes) need += slice_bytes(b);
+    if (need > g_pinned.cap) {
+        if (g_pinned.host) GS_HIP(hipHostFree(g_pinned.host));
+        g_pinned = Pinned();
+        void* h = nullptr;
+        GS_HIP(hipHostMalloc(&h, need, hipHostMallocMapped | hipHostMallocPortable));
+        void* d = nullptr;
+        GS_HIP(hipHostGetDevicePointer(&d, h, 0));
+        g_pinned.host = static_cast<char*>(h);
+        g_pinned.dev = static_cast<char*>(d);
+        g_pinned.cap = need;
+    }
+    host.clear();
+    dev.clear();
+    size_t at = 0;
+    for (size_t b : sizes) {
+        host.push_back(g_pinned.host + at);
+        dev.push_back(g_pinned.dev + at);
+        at += slice_bytes(b);
+    }
+    return GS_OK;
+}
+
 int check_ng(long long ng) {
     if (ng < 0) return set_err(GS_ERR_ARG, "negative array length %lld", ng);
     return GS_OK;
@@ -279,14 +318,24 @@ int launch_refine_one(int mode, double* ubi, const double* gv, int ng, double to
     GS_CHECK(check_ng(ng));
     const int nb = std::min(GS_MAX_PARTIAL_BLOCKS, blocks_for(ng, GS_BLOCK * 4));
     const double tolsq = tol * tol;
+    if (nb == 1) {  // one block: it finishes the job itself (same bits, one launch)
+        if (mode == 0)
+            hipLaunchKernelGGL((k_refine_partials<0, true>), dim3(1), dim3(GS_BLOCK), 0, st, ubi,
+                               gv, ng, tolsq, labels, label, nullptr, n_out, mean_out);
+        else
+            hipLaunchKernelGGL((k_refine_partials<1, true>), dim3(1), dim3(GS_BLOCK), 0, st, ubi,
+                               gv, ng, tolsq, labels, label, nullptr, n_out, mean_out);
+        GS_LAUNCHED();
+        return GS_OK;
+    }
     double* partials;
     GS_CHECK(get_partials(&partials));
     if (mode == 0)
-        hipLaunchKernelGGL(k_refine_partials<0>, dim3(nb), dim3(GS_BLOCK), 0, st, ubi, gv, ng,
-                           tolsq, labels, label, partials);
+        hipLaunchKernelGGL((k_refine_partials<0, false>), dim3(nb), dim3(GS_BLOCK), 0, st, ubi, gv,
+                           ng, tolsq, labels, label, partials, nullptr, nullptr);
     else
-        hipLaunchKernelGGL(k_refine_partials<1>, dim3(nb), dim3(GS_BLOCK), 0, st, ubi, gv, ng,
-                           tolsq, labels, label, partials);
+        hipLaunchKernelGGL((k_refine_partials<1, false>), dim3(nb), dim3(GS_BLOCK), 0, st, ubi, gv,
+                           ng, tolsq, labels, label, partials, nullptr, nullptr);
     GS_LAUNCHED();
     hipLaunchKernelGGL(k_refine_finalize, dim3(1), dim3(GS_WAVE), 0, st, partials, nb, ubi,
                        n_out, mean_out);
@@ -453,6 +502,29 @@ int verify_rounding(int n) {
 
 int score(double ubi[3][3], double gv[][3], double tol, int ng) {
     if (check_ng(ng) != GS_OK) return -1;
+    if ((size_t)ng * 24 <= GS_PINNED_MAX) {
+        // inputs read in place from pinned memory; the count is accumulated
+        // with atomics, so it lives in device memory and comes back by copy
+        std::vector<char*> h, d, sl;
+        if (pinned_slices({72, 4, (size_t)ng * 24}, h, d) != GS_OK) return -1;
+        if (stage_slices({4}, sl) != GS_OK) return -1;
+        std::memcpy(h[0], ubi, 72);
+        if (ng > 0) std::memcpy(h[2], gv, (size_t)ng * 24);
+        int* dn = reinterpret_cast<int*>(sl[0]);
+        hipStream_t st = nullptr;
+        if (launch_score_many(reinterpret_cast<double*>(d[0]), 1,
+                              reinterpret_cast<double*>(d[2]), ng, tol, dn, 0,
+                              GS_SCORE_CERTIFIED, st) != GS_OK)
+            return -1;
+        if (hipMemcpyAsync(h[1], dn, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            set_err(GS_ERR_HIP, "score: copy back failed");
+            return -1;
+        }
+        int n;
+        std::memcpy(&n, h[1], 4);
+        return n;
+    }
     std::vector<char*> sl;
     if (stage_slices({72, (size_t)ng * 24, 4}, sl) != GS_OK) return -1;
     double* du = reinterpret_cast<double*>(sl[0]);
@@ -477,6 +549,25 @@ int score(double ubi[3][3], double gv[][3], double tol, int ng) {
 static int refine_host(int mode, double* ubi, const double* gv, const int* labels, int label,
                        double tol, int* n_out, double* mean_out, int ng) {
     GS_CHECK(check_ng(ng));
+    const size_t gv_bytes = (size_t)ng * 24, lab_bytes = mode ? (size_t)ng * 4 : 0;
+    if (gv_bytes + lab_bytes <= GS_PINNED_MAX) {
+        std::vector<char*> h, d;
+        GS_CHECK(pinned_slices({72, 8, 4, gv_bytes, lab_bytes}, h, d));
+        std::memcpy(h[0], ubi, 72);
+        if (ng > 0) std::memcpy(h[3], gv, gv_bytes);
+        if (mode && ng > 0) std::memcpy(h[4], labels, lab_bytes);
+        hipStream_t st = nullptr;
+        GS_CHECK(launch_refine_one(mode, reinterpret_cast<double*>(d[0]),
+                                   reinterpret_cast<const double*>(d[3]), ng, tol,
+                                   mode ? reinterpret_cast<const int*>(d[4]) : nullptr, label,
+                                   reinterpret_cast<int*>(d[2]), reinterpret_cast<double*>(d[1]),
+                                   st));
+        GS_HIP(hipStreamSynchronize(st));
+        std::memcpy(ubi, h[0], 72);
+        std::memcpy(mean_out, h[1], 8);
+        std::memcpy(n_out, h[2], 4);
+        return GS_OK;
+    }
     std::vector<char*> sl;
     GS_CHECK(stage_slices({72, (size_t)ng * 24, 8, 4, mode ? (size_t)ng * 4 : 0}, sl));
     double* du = reinterpret_cast<double*>(sl[0]);
