@@ -171,20 +171,20 @@ def score_and_refine(ubi, gv, tol):
     (``src/closest.c:267-345``).  Returns (n, mean drlv2 before refinement)."""
     if _is_dev(ubi) or _is_dev(gv):
         g = _dev_in(gv, torch.float64, (None, 3), "gv")
-        if _is_dev(ubi):
-            host = None
-            u = _dev_inout(ubi, torch.float64, (3, 3), "ubi")
-        else:
-            host = _inout_array(ubi, np.float64, (3, 3), "ubi")
-            u = torch.as_tensor(host, device=g.device).clone()
         lib = _lib_ready()
+        if not _is_dev(ubi):  # host UBI and results, device peaks: one launch, one wait
+            host = _inout_array(ubi, np.float64, (3, 3), "ubi")
+            n = ctypes.c_int(0)
+            m = ctypes.c_double(0.0)
+            _lib.check(lib.gs_score_and_refine_host_ubi(
+                hptr(host), dptr(g), g.shape[0], float(tol), ctypes.byref(n), ctypes.byref(m),
+                stream_ptr(torch.cuda.current_stream(g.device))), "score_and_refine")
+            return n.value, m.value
+        u = _dev_inout(ubi, torch.float64, (3, 3), "ubi")
         res, n, m = _device_scalars(g.device)
         _lib.check(lib.gs_score_and_refine_dev(dptr(u), dptr(g), g.shape[0], float(tol), dptr(n),
                                                dptr(m), stream_ptr()), "score_and_refine")
-        out = _read_scalars(res)
-        if host is not None:
-            host[...] = u.cpu().numpy()
-        return out
+        return _read_scalars(res)
     u = _inout_array(ubi, np.float64, (3, 3), "ubi")
     g = _in_array(gv, np.float64, (None, 3), "gv")
     lib = _lib_ready()
@@ -234,6 +234,15 @@ def refine_assigned(ubi, gv, labels, label):
         g = _dev_in(gv, torch.float64, (None, 3), "gv")
         ng = g.shape[0]
         lab = _dev_in(labels, torch.int32, (ng,), "labels", device=g.device)
+        lib = _lib_ready()
+        if not _is_dev(ubi):  # host UBI and results, device peaks: one launch, one wait
+            u = ubi if writable else np.array(_in_array(ubi, np.float64, (3, 3), "ubi"))
+            n = ctypes.c_int(0)
+            m = ctypes.c_double(0.0)
+            _lib.check(lib.gs_refine_assigned_host_ubi(
+                hptr(u), dptr(g), ng, dptr(lab), int(label), ctypes.byref(n), ctypes.byref(m),
+                stream_ptr(torch.cuda.current_stream(g.device))), "refine_assigned")
+            return n.value, m.value
         if _is_dev(ubi) and ubi.dtype == torch.float64 and ubi.is_contiguous():
             u = _dev_inout(ubi, torch.float64, (3, 3), "ubi")
         else:

@@ -828,6 +828,36 @@ int gs_refine_assigned_dev(double* ubi, const double* gv, int ng, const int* lab
     return launch_refine_one(1, ubi, gv, ng, 0.0, labels, label, n_out, mean_out, S(stream));
 }
 
+// The UBI and the two results on the host, the peaks on the device: the
+// UBI and results pass through the thread's pinned mapped buffer, so a call
+// is one launch and one wait on the stream.
+static int refine_host_ubi(int mode, double* ubi, const double* gv, int ng, double tol,
+                           const int* labels, int label, int* n_out, double* mean_out,
+                           hipStream_t st) {
+    GS_CHECK(check_ng(ng));
+    if (!ubi || !n_out || !mean_out) return set_err(GS_ERR_ARG, "null ubi or result pointer");
+    std::vector<char*> h, d;
+    GS_CHECK(pinned_slices({72, 8, 4}, h, d));
+    std::memcpy(h[0], ubi, 72);
+    GS_CHECK(launch_refine_one(mode, reinterpret_cast<double*>(d[0]), gv, ng, tol, labels, label,
+                               reinterpret_cast<int*>(d[2]), reinterpret_cast<double*>(d[1]), st));
+    GS_HIP(hipStreamSynchronize(st));
+    std::memcpy(ubi, h[0], 72);
+    std::memcpy(mean_out, h[1], 8);
+    std::memcpy(n_out, h[2], 4);
+    return GS_OK;
+}
+
+int gs_score_and_refine_host_ubi(double* ubi, const double* gv, int ng, double tol, int* n_out,
+                                 double* mean_out, gs_stream_t stream) {
+    return refine_host_ubi(0, ubi, gv, ng, tol, nullptr, 0, n_out, mean_out, S(stream));
+}
+
+int gs_refine_assigned_host_ubi(double* ubi, const double* gv, int ng, const int* labels,
+                                int label, int* n_out, double* mean_out, gs_stream_t stream) {
+    return refine_host_ubi(1, ubi, gv, ng, 0.0, labels, label, n_out, mean_out, S(stream));
+}
+
 int gs_compute_xlylzl_dev(const double* s, const double* f, const double p[4], const double r[9],
                           const double dist[3], double* xlylzl, int n, gs_stream_t stream) {
     return launch_xlylzl(s, f, p, r, dist, xlylzl, n, S(stream));

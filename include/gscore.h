@@ -132,6 +132,14 @@ int gs_score_and_refine_dev(double *ubi, const double *gv, int ng, double tol, i
                             double *mean_out, gs_stream_t stream);
 int gs_refine_assigned_dev(double *ubi, const double *gv, int ng, const int *labels,
                            int label, int *n_out, double *mean_out, gs_stream_t stream);
+/* The same two calls with ubi (9 doubles, updated in place), n_out and
+ * mean_out on the HOST and gv (and labels) on the device: the calling
+ * pattern of a per-grain loop over device-resident peaks.  Waits for the
+ * stream before returning. */
+int gs_score_and_refine_host_ubi(double *ubi, const double *gv, int ng, double tol, int *n_out,
+                                 double *mean_out, gs_stream_t stream);
+int gs_refine_assigned_host_ubi(double *ubi, const double *gv, int ng, const int *labels,
+                                int label, int *n_out, double *mean_out, gs_stream_t stream);
 
 /* compute_xlylzl / compute_gv on device memory. */
 int gs_compute_xlylzl_dev(const double *s, const double *f, const double p[4],

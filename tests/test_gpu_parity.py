@@ -120,13 +120,17 @@ def test_refine_assigned_golden(dev, mode):
         np.testing.assert_allclose(u, s["ra_ubi"][g], rtol=0, atol=UBI_TOL)
 
 
-def test_refine_assigned_readonly_ubi(dev):
-    """ubi that is not C-contiguous float64 is copied, not overwritten."""
+@pytest.mark.parametrize("mode", ["host", "device"])
+def test_refine_assigned_readonly_ubi(dev, mode):
+    """ubi that is not C-contiguous float64 is copied, not overwritten
+    (the results are those of a writable copy)."""
     s = load("scoring_cases.npz")
+    gv = s["gv"] if mode == "host" else T(s["gv"], dev)
     u = np.asfortranarray(s["ubis"][0])
     before = u.copy()
-    gpu.refine_assigned(u, s["gv"], s["saa_rg_t2_labels"], 0)
+    got = gpu.refine_assigned(u, gv, s["saa_rg_t2_labels"], 0)
     assert np.array_equal(u, before)
+    assert got == gpu.refine_assigned(before.copy(), gv, s["saa_rg_t2_labels"], 0)
 
 
 @pytest.mark.parametrize("mode", ["host", "device"])
