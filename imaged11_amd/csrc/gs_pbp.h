@@ -114,3 +114,282 @@ __global__ __launch_bounds__(GS_BLOCK) void k_segment_sum(const double* __restri
     for (; r < r1; r++) acc += v[r * C + c];
     out[i] = acc;
 }
+
+// ---------------------------------------------------------------------------
+// The per-task least-squares fits of refine_map, one thread per task,
+// solved as np.linalg.lstsq solves them: the minimum-norm solution with
+// the singular values of the weighted design matrix below rcond * the
+// largest taken as zero, rcond = eps * max(rows, columns).  Each task's
+// rows stream into the triangular factor R of a QR decomposition (Givens
+// rotations, the right-hand sides carried along as extra columns), and R,
+// which has the design matrix's singular values, is decomposed by one-sided
+// Jacobi rotations.  Nothing squares the condition number.
+
+// Rotate one row x[0..NC+NR) into the upper-triangular [R | C].
+template <int NC, int NR>
+__device__ __forceinline__ void gs_qr_add_row(double (&R)[NC][NC + NR], double (&x)[NC + NR]) {
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const double a = R[k][k], b = x[k];
+        if (b == 0.0) continue;
+        const double r = sqrt(a * a + b * b);
+        const double c = a / r, s = b / r;
+#pragma unroll
+        for (int j = k; j < NC + NR; j++) {
+            const double rj = R[k][j], xj = x[j];
+            R[k][j] = c * rj + s * xj;
+            x[j] = c * xj - s * rj;
+        }
+    }
+}
+
+// One-sided Jacobi SVD of the N x N matrix A: on return A's column j is
+// U_j * sv[j] and V holds the right singular vectors.  At most 60 sweeps.
+template <int N>
+__device__ __forceinline__ void gs_jacobi_svd(double (&A)[N][N], double (&V)[N][N],
+                                              double (&sv)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) V[i][j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        bool rotated = false;
+#pragma unroll
+        for (int p = 0; p < N - 1; p++) {
+#pragma unroll
+            for (int q = p + 1; q < N; q++) {
+                double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; i++) {
+                    al += A[i][p] * A[i][p];
+                    be += A[i][q] * A[i][q];
+                    ga += A[i][p] * A[i][q];
+                }
+                if (!(fabs(ga) > 2.220446049250313e-16 * sqrt(al * be))) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = fabs(zeta) > 1e150 ? 0.5 / zeta
+                                                    : copysign(1.0, zeta) /
+                                                          (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+#pragma unroll
+                for (int i = 0; i < N; i++) {
+                    const double ap = A[i][p], aq = A[i][q];
+                    A[i][p] = c * ap - s * aq;
+                    A[i][q] = s * ap + c * aq;
+                    const double vp = V[i][p], vq = V[i][q];
+                    V[i][p] = c * vp - s * vq;
+                    V[i][q] = s * vp + c * vq;
+                }
+            }
+        }
+        if (!rotated) break;
+    }
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; i++) s += A[i][j] * A[i][j];
+        sv[j] = sqrt(s);
+    }
+}
+
+// Minimum-norm least-squares solution X (NC x NR) of the rows streamed into
+// [R | C], singular values <= rcond * max taken as zero; returns the rank.
+template <int NC, int NR>
+__device__ __forceinline__ int gs_lstsq_solve(double (&R)[NC][NC + NR], double rcond,
+                                              double (&X)[NC][NR]) {
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < NC; i++)
+#pragma unroll
+        for (int j = 0; j < NC + NR; j++) finite = finite && isfinite(R[i][j]);
+    if (!finite) {  // numpy's lstsq fails on such rows: no solution
+#pragma unroll
+        for (int i = 0; i < NC; i++)
+#pragma unroll
+            for (int k = 0; k < NR; k++) X[i][k] = __builtin_nan("");
+        return 0;
+    }
+    double A[NC][NC], V[NC][NC], sv[NC];
+#pragma unroll
+    for (int i = 0; i < NC; i++)
+#pragma unroll
+        for (int j = 0; j < NC; j++) A[i][j] = R[i][j];
+    gs_jacobi_svd<NC>(A, V, sv);
+    double smax = 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; j++) smax = fmax(smax, sv[j]);
+    const double cut = rcond * smax;
+#pragma unroll
+    for (int i = 0; i < NC; i++)
+#pragma unroll
+        for (int k = 0; k < NR; k++) X[i][k] = 0.0;
+    int rank = 0;
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        if (!(sv[j] > cut)) continue;
+        rank++;
+        const double inv = 1.0 / (sv[j] * sv[j]);  // A_j = U_j sv_j
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            double d = 0.0;
+#pragma unroll
+            for (int i = 0; i < NC; i++) d += A[i][j] * R[i][NC + k];
+            d *= inv;
+#pragma unroll
+            for (int i = 0; i < NC; i++) X[i][k] += V[i][j] * d;
+        }
+    }
+    return rank;
+}
+
+// Singular values of a 3x3 matrix (row-major).
+__device__ __forceinline__ void gs_sv3(const double* m, double (&sv)[3]) {
+    double A[3][3], V[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) A[i][j] = m[3 * i + j];
+    gs_jacobi_svd<3>(A, V, sv);
+}
+
+// U.B0 of a fitted UBI: the cell of UBI.UBI^T, its Busing-Levy B and
+// U = (B.UBI)^T (ImageD11/sinograms/point_by_point.py:443-490).
+__device__ __forceinline__ void gs_ubi_u_b0(const double* u, const double* B0, double* ub0) {
+    const double r2d = 180.0 / GS_PI, d2r = GS_PI / 180.0;
+    double G[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            G[3 * i + j] = u[3 * i] * u[3 * j] + u[3 * i + 1] * u[3 * j + 1] +
+                           u[3 * i + 2] * u[3 * j + 2];
+    const double a = sqrt(G[0]), b = sqrt(G[4]), c = sqrt(G[8]);
+    const double al = acos(G[5] / b / c) * r2d, be = acos(G[2] / a / c) * r2d,
+                 ga = acos(G[1] / a / b) * r2d;
+    const double ca = cos(al * d2r), cb = cos(be * d2r), cg = cos(ga * d2r);
+    double g[9] = {a * a, a * b * cg, a * c * cb, a * b * cg, b * b, b * c * ca,
+                   a * c * cb, b * c * ca, c * c};
+    if (gs_inverse3x3(g) != 0) {
+        for (int i = 0; i < 9; i++) ub0[i] = __builtin_nan("");
+        return;
+    }
+    const double as = sqrt(g[0]), bs = sqrt(g[4]), cs = sqrt(g[8]);
+    const double betas = acos(g[2] / as / cs) * r2d, gammas = acos(g[1] / as / bs) * r2d;
+    const double B[9] = {as, bs * cos(gammas * d2r), cs * cos(betas * d2r),
+                         0.0, bs * sin(gammas * d2r), -cs * sin(betas * d2r) * ca,
+                         0.0, 0.0, 1.0 / c};
+    double U[9];  // (B.u)^T
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            U[3 * j + i] = B[3 * i] * u[j] + B[3 * i + 1] * u[3 + j] + B[3 * i + 2] * u[6 + j];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            ub0[3 * i + j] = U[3 * i] * B0[j] + U[3 * i + 1] * B0[3 + j] + U[3 * i + 2] * B0[6 + j];
+}
+
+struct gs_mat9 {
+    double m[9];
+};
+
+// The UBI fit of refine_map (weighted_lstsq_ubi_fit, :408-419, and the
+// quality test of :1595-1597) for every task with fit[t] != 0: rows
+// design[r] = (w g, w hkl) for r in [off[t], off[t+1]).  Outputs per task:
+// ubifit (NaN when not fitted), lstsq's rank, worth = rank 3, cond < 1e14,
+// det > 0 and matrix_rank 3; ubi_out = ubifit when worth, else ubi_in; and
+// ub0 = U.B0 of ubifit when worth (NaN otherwise).
+__global__ __launch_bounds__(64) void k_pbp_ubi_fit(
+    const double* __restrict__ design, const long long* __restrict__ off,
+    const int* __restrict__ fit, const double* __restrict__ ubi_in, gs_mat9 B0, int T,
+    double* __restrict__ ubifit, int* __restrict__ rank_out, int* __restrict__ worth_out,
+    double* __restrict__ ubi_out, double* __restrict__ ub0) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= T) return;
+    double u[9], uo[9], b0[9];
+    for (int i = 0; i < 9; i++) {
+        u[i] = __builtin_nan("");
+        uo[i] = ubi_in[9 * (size_t)t + i];
+        b0[i] = __builtin_nan("");
+    }
+    int rank = 0, worth = 0;
+    if (fit[t]) {
+        double R[3][6];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 6; j++) R[i][j] = 0.0;
+        const long long r0 = off[t], r1 = off[t + 1];
+        for (long long r = r0; r < r1; r++) {
+            double x[6];
+#pragma unroll
+            for (int j = 0; j < 6; j++) x[j] = design[6 * r + j];
+            gs_qr_add_row<3, 3>(R, x);
+        }
+        const double n = (double)(r1 - r0);
+        double X[3][3];  // ubifit^T
+        rank = gs_lstsq_solve<3, 3>(R, 2.220446049250313e-16 * fmax(n, 3.0), X);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) u[3 * i + j] = X[j][i];
+        double sv[3];
+        gs_sv3(u, sv);
+        const double smax = fmax(sv[0], fmax(sv[1], sv[2]));
+        const double smin = fmin(sv[0], fmin(sv[1], sv[2]));
+        const double cond = smax / smin;  // inf for a singular matrix, NaN with NaN
+        int mrank = 0;
+        for (int j = 0; j < 3; j++) mrank += sv[j] > smax * 3.0 * 2.220446049250313e-16;
+        const double det = u[0] * (u[4] * u[8] - u[5] * u[7]) -
+                           u[1] * (u[3] * u[8] - u[5] * u[6]) +
+                           u[2] * (u[3] * u[7] - u[4] * u[6]);
+        worth = rank == 3 && cond < 1e14 && det > 0.0 && mrank == 3;
+        if (worth) {
+            for (int i = 0; i < 9; i++) uo[i] = u[i];
+            gs_ubi_u_b0(u, B0.m, b0);
+        }
+    }
+    for (int i = 0; i < 9; i++) {
+        ubifit[9 * (size_t)t + i] = u[i];
+        ubi_out[9 * (size_t)t + i] = uo[i];
+        ub0[9 * (size_t)t + i] = b0[i];
+    }
+    rank_out[t] = rank;
+    worth_out[t] = worth;
+}
+
+// The strain fit of refine_map (:1652-1656) for every task with
+// worth[t] != 0: rows design[r] = (w M, w strain) for r in [off[t],
+// off[t+1]), eps[t] = lstsq's minimum-norm solution (NaN for other tasks).
+__global__ __launch_bounds__(64) void k_pbp_strain_fit(const double* __restrict__ design,
+                                                       const long long* __restrict__ off,
+                                                       const int* __restrict__ worth, int T,
+                                                       double* __restrict__ eps) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= T) return;
+    double e[6];
+    for (int i = 0; i < 6; i++) e[i] = __builtin_nan("");
+    if (worth[t]) {
+        double R[6][7];
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+#pragma unroll
+            for (int j = 0; j < 7; j++) R[i][j] = 0.0;
+        const long long r0 = off[t], r1 = off[t + 1];
+        for (long long r = r0; r < r1; r++) {
+            double x[7];
+#pragma unroll
+            for (int j = 0; j < 7; j++) x[j] = design[7 * r + j];
+            gs_qr_add_row<6, 1>(R, x);
+        }
+        const double n = (double)(r1 - r0);
+        double X[6][1];
+        gs_lstsq_solve<6, 1>(R, 2.220446049250313e-16 * fmax(n, 6.0), X);
+        for (int i = 0; i < 6; i++) e[i] = X[i][0];
+    }
+    for (int i = 0; i < 6; i++) eps[6 * (size_t)t + i] = e[i];
+}

@@ -1211,6 +1211,34 @@ int gs_pbp_assign(const double* so, const double* co, const double* dty, const d
     return GS_OK;
 }
 
+int gs_pbp_ubi_fit(const double* design, const long long* off, const int* fit,
+                   const double* ubi_in, const double B0[9], int T, double* ubifit, int* rank,
+                   int* worth, double* ubi_out, double* ub0, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (T < 0) return set_err(GS_ERR_ARG, "gs_pbp_ubi_fit: negative task count");
+    if (T == 0) return GS_OK;
+    if (!B0) return set_err(GS_ERR_ARG, "gs_pbp_ubi_fit: null B0");
+    gs_mat9 b;
+    for (int i = 0; i < 9; i++) b.m[i] = B0[i];
+    hipLaunchKernelGGL(k_pbp_ubi_fit, dim3(blocks_for(T, 64)), dim3(64), 0, S(stream), design, off,
+                       fit, ubi_in, b, T, ubifit, rank, worth, ubi_out, ub0);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_pbp_strain_fit(const double* design, const long long* off, const int* worth, int T,
+                      double* eps, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (T < 0) return set_err(GS_ERR_ARG, "gs_pbp_strain_fit: negative task count");
+    if (T == 0) return GS_OK;
+    hipLaunchKernelGGL(k_pbp_strain_fit, dim3(blocks_for(T, 64)), dim3(64), 0, S(stream), design,
+                       off, worth, T, eps);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
 int gs_segment_sum(const double* v, int C, const long long* off, int nseg, double* out,
                    gs_stream_t stream) {
     DeviceCtx* ctx;

@@ -15,13 +15,15 @@ UBI) task is one kernel in two passes (``gs_pbp_assign``: count, then fill
 the assigned (peak, hkl) lists), and the g-vectors are computed on the GPU
 (``compute_gve``: ``gs_compute_xlylzl_xpos_dev`` + ``gs_compute_gv_dev``).
 Everything after the first selection works on the assigned rows only and
-is vectorised across all tasks at once: the merge is one ``np.bincount``
-per column over (task, key) groups (the reference's per-voxel bincount,
-the same sums in the same order), the least-squares fits are per-task 3x3
-and 6x6 normal equations accumulated with ``np.bincount`` and solved in one
-batched call (the reference's ``np.linalg.lstsq``: equal for full-rank
-problems up to rounding; the rank it reports is computed from the same
-singular values), and the outlier statistics are per-task sums.
+is vectorised across all tasks at once on the device: the merge is one
+ordered segmented sum per column over (task, key) groups (the reference's
+per-voxel ``np.bincount``, the same sums in the same order), the
+least-squares fits run one task per thread (``gs_pbp_ubi_fit``,
+``gs_pbp_strain_fit``: the task's weighted rows streamed into a QR factor
+by Givens rotations, the factor decomposed by one-sided Jacobi rotations,
+then ``np.linalg.lstsq``'s minimum-norm solution, rank and rcond rule, and
+the reference's cond / det / matrix_rank test of the fitted UBI), and the
+outlier statistics are per-task sums.
 """
 import numpy as np
 import torch
@@ -267,24 +269,44 @@ def _score(u, g, tol):
     return hi.long(), ((e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]) + e[:, 2] * e[:, 2]) < tol * tol
 
 
-def _lstsq_rank(ata, n_rows):
-    """Numerical rank of the (n x m) design matrix with Gram matrix ata, as
-    np.linalg.lstsq counts it: singular values above eps * max(n, m) times
-    the largest."""
-    m = ata.shape[-1]
-    s = np.sqrt(np.clip(np.linalg.eigvalsh(ata), 0, None))
-    rcond = np.finfo(np.float64).eps * np.maximum(n_rows, m)
-    return np.sum(s > (rcond * s.max(-1))[..., None], axis=-1)
+def ubi_fit(design, off, fit, ubis, B0):
+    """weighted_lstsq_ubi_fit (:408-419) and the fit-quality test
+    (:1595-1597) of every task with fit[t], on the device
+    (gs_pbp_ubi_fit): design (rows, 6) = (w g, w hkl) grouped by task with
+    CSR offsets off (T+1).  Returns (ubifit, rank, worth, ubi_out, ub0):
+    lstsq's solution and rank, the quality test, the fitted UBI where it
+    passes (else the input) and U.B0 of the fitted cell."""
+    lib = _lib.load()
+    T = ubis.shape[0]
+    dev = ubis.device
+    fit_i = fit.to(torch.int32).contiguous()
+    design = design.contiguous()
+    ubis = ubis.contiguous()
+    ubifit = torch.empty((T, 3, 3), dtype=torch.float64, device=dev)
+    ubi_out = torch.empty((T, 3, 3), dtype=torch.float64, device=dev)
+    ub0 = torch.empty((T, 3, 3), dtype=torch.float64, device=dev)
+    rank = torch.empty(T, dtype=torch.int32, device=dev)
+    worth = torch.empty(T, dtype=torch.int32, device=dev)
+    b0 = darr(np.asarray(B0, np.float64).reshape(9))
+    _lib.check(lib.gs_pbp_ubi_fit(dptr(design), dptr(off), dptr(fit_i), dptr(ubis), b0, T,
+                                  dptr(ubifit), dptr(rank), dptr(worth), dptr(ubi_out), dptr(ub0),
+                                  stream_ptr()), "pbp_ubi_fit")
+    return ubifit, rank, worth.bool(), ubi_out, ub0
 
 
-def _solve_normal(ata, atb):
-    """Least-squares solutions from the normal equations; tasks whose Gram
-    matrix is singular get NaN (the caller handles them)."""
-    out = np.full(atb.shape, np.nan)
-    ok = np.abs(np.linalg.det(ata)) > 0
-    if ok.any():
-        out[ok] = np.linalg.solve(ata[ok], atb[ok])
-    return out
+def strain_fit(design, off, worth):
+    """The strain lstsq of every task with worth[t] (:1652-1656) on the
+    device (gs_pbp_strain_fit): design (rows, 7) = (w M, w strain) grouped
+    by task.  Returns (T, 6) = (sxx, syy, szz, sxy, sxz, syz), NaN where
+    not fitted."""
+    lib = _lib.load()
+    T = worth.shape[0]
+    w_i = worth.to(torch.int32).contiguous()
+    design = design.contiguous()
+    eps = torch.empty((T, 6), dtype=torch.float64, device=worth.device)
+    _lib.check(lib.gs_pbp_strain_fit(dptr(design), dptr(off), dptr(w_i), T, dptr(eps),
+                                     stream_ptr()), "pbp_strain_fit")
+    return eps
 
 
 def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid, mask,
@@ -367,28 +389,14 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
     # ---- weighted least-squares UBI of every task with enough peaks
     rows = torch.nonzero(sel & fit[rtask]).flatten()
     rt = rtask[rows]
-    w = 1.0 / (rydist[rows] + 1)
-    wg = w[:, None] * g[rows]
-    wh = w[:, None] * hi[rows].double()
-    prods = torch.cat([(wg[:, :, None] * wg[:, None, :]).reshape(-1, 9),
-                       (wg[:, :, None] * wh[:, None, :]).reshape(-1, 9)], 1)
-    nsum = segment_sum(prods, _offsets(rt, T)).cpu().numpy()
-    ata, atb = nsum[:, :9].reshape(T, 3, 3), nsum[:, 9:].reshape(T, 3, 3)
+    w = (1.0 / (rydist[rows] + 1))[:, None]
+    design = torch.cat([w * g[rows], w * hi[rows].double()], 1)
+    _, _, worth_d, ubi_out_d, ub0_d = ubi_fit(design, _offsets(rt, T), fit, ubis_d, B0)
     fit_h = fit.cpu().numpy()
-    npks_h = npks.cpu().numpy()
-    ubifit = np.full((T, 3, 3), np.nan)
-    ubifit[fit_h] = np.swapaxes(_solve_normal(ata[fit_h], atb[fit_h]), -1, -2)
-    rank = np.zeros(T, np.int64)
-    rank[fit_h] = _lstsq_rank(ata[fit_h], npks_h[fit_h])
-    worth = fit_h & (rank == 3) & np.all(np.isfinite(ubifit), axis=(1, 2))
-    if worth.any():
-        u = ubifit[worth]
-        good = (np.linalg.cond(u) < 1e14) & (np.linalg.det(u) > 0) & \
-            (np.linalg.matrix_rank(u) == 3)
-        worth[np.nonzero(worth)[0][~good]] = False
-    ubi_out = np.where(worth[:, None, None], ubifit, ubis_h)
+    worth = worth_d.cpu().numpy()
+    ubi_out = ubi_out_d.cpu().numpy()
     # ---- re-score with the fitted UBIs, count distinct reflections
-    hi2, sel2 = _score(D(ubi_out)[rtask], g, merge_tol)
+    hi2, sel2 = _score(ubi_out_d[rtask], g, merge_tol)
     npk2 = torch.bincount(rtask[sel2], minlength=T).cpu().numpy()
     r2 = torch.nonzero(sel2 & fit[rtask]).flatten()
     nuniq = np.zeros(T, np.int64)
@@ -403,26 +411,24 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
     final_nuniq[tmap[fidx]] = nuniq[fidx]
     # ---- strain tensors of the well-fitted tasks
     if worth.any():
-        eps = _strain_fit(D(worth, torch.bool), ubifit, worth, rtask, sel2, g, hi2, rydist, B0,
-                          gnoise_std, T)
+        eps = _strain_fit(worth_d, ub0_d, rtask, sel2, g, hi2, rydist, gnoise_std, T)
         widx = np.nonzero(worth)[0]
         final_eps[:, :, tmap[widx]] = np.moveaxis(eps[widx], 0, -1)
     return final_ubis, final_eps, final_npks, final_nuniq
 
 
-def _strain_fit(worth_d, ubifit, worth, rtask, sel2, g, hi2, rydist, B0, gnoise_std, T):
+def _strain_fit(worth_d, ub0_d, rtask, sel2, g, hi2, rydist, gnoise_std, T):
     """The directional-strain fit of refine_map (:1624-1668) for every task
-    in ``worth`` at once: per-row terms on the device, per-task sums by
-    gs_segment_sum, the 6x6 solves on the host."""
+    with worth_d at once: per-row terms on the device, per-task mean and
+    standard deviation by gs_segment_sum, the lstsq solves by
+    gs_pbp_strain_fit.  Returns (T, 3, 3) strain tensors (NaN where not
+    fitted)."""
     dev = g.device
     rows = torch.nonzero(sel2 & worth_d[rtask]).flatten()
     t = rtask[rows]
     gg = g[rows]
     hk = hi2[rows].double()
-    U = np.full((T, 3, 3), np.nan)
-    U[worth] = ubi_and_ucell_to_u(ubifit[worth], ubi_to_unitcell(ubifit[worth]))
-    UB0 = _t(np.matmul(U, B0), dev)[t]
-    gve0 = _rows_dot(UB0, hk)
+    gve0 = _rows_dot(ub0_d[t], hk)
     gTg0 = (gg[:, 0] * gve0[:, 0] + gg[:, 1] * gve0[:, 1]) + gg[:, 2] * gve0[:, 2]
     gTg = (gg[:, 0] * gg[:, 0] + gg[:, 1] * gg[:, 1]) + gg[:, 2] * gg[:, 2]
     strain = gTg0 / gTg - 1
@@ -443,23 +449,8 @@ def _strain_fit(worth_d, ubifit, worth, rtask, sel2, g, hi2, rydist, B0, gnoise_
     w = torch.where(strain < mean[t] - std[t] * 3.5, torch.zeros_like(w), w)
     wmax = torch.zeros(T, dtype=torch.float64, device=dev).scatter_reduce(0, t, w, "amax")
     w = w / wmax[t]
-    aM = w[:, None] * Mrow
-    b = w * strain
-    s = segment_sum(torch.cat([(aM[:, :, None] * aM[:, None, :]).reshape(-1, 36),
-                               aM * b[:, None]], 1), off).cpu().numpy()
-    ata, atb = s[:, :36].reshape(T, 6, 6), s[:, 36:]
-    e = np.full((T, 6), np.nan)
-    idx = np.nonzero(worth)[0]
-    sol = _solve_normal(ata[idx], atb[idx][:, :, None])[:, :, 0]
-    e[idx] = sol
-    bad = idx[~np.all(np.isfinite(sol), axis=1)]
-    if len(bad):
-        # rank-deficient tasks: the minimum-norm solution, as lstsq gives it
-        aM_h, b_h, t_h = aM.cpu().numpy(), b.cpu().numpy(), t.cpu().numpy()
-        for tt in bad:
-            r = t_h == tt
-            e[tt] = np.linalg.lstsq(aM_h[r], b_h[r],
-                                    rcond=np.finfo(float).eps * max(r.sum(), 6))[0]
+    design = torch.cat([w[:, None] * Mrow, (w * strain)[:, None]], 1)
+    e = strain_fit(design, off, worth_d).cpu().numpy()
     sxx, syy, szz, sxy, sxz, syz = e.T
     return np.stack([np.stack([sxx, sxy, sxz], -1), np.stack([sxy, syy, syz], -1),
                      np.stack([sxz, syz, szz], -1)], -2)

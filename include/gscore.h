@@ -319,6 +319,27 @@ int gs_pbp_assign(const double *so, const double *co, const double *dty, const d
                   double y0, double ystep, double tol, int *counts, const long long *task_off,
                   int *cursor, int *out_peak, int *out_hkl, gs_stream_t stream);
 
+/* The least-squares fits of refine_map, one thread per task t < T, as
+ * np.linalg.lstsq solves them (minimum-norm solution, singular values of
+ * the weighted design matrix <= eps * max(rows, columns) * the largest
+ * taken as zero), rows of task t = [off[t], off[t+1]) of `design`:
+ *   gs_pbp_ubi_fit: design (rows, 6) = (w g, w hkl) (weighted_lstsq_ubi_fit,
+ *     ImageD11/sinograms/point_by_point.py:408-419) for tasks with
+ *     fit[t] != 0; ubifit (T,3,3) (NaN when not fitted), rank (T) as lstsq
+ *     returns it, worth (T) = rank 3, cond < 1e14, det > 0, matrix_rank 3
+ *     (:1595-1597), ubi_out = ubifit where worth else ubi_in, ub0 (T,3,3) =
+ *     U.B0 with U from ubifit's cell (:443-490, B0 9 host doubles) where
+ *     worth, NaN elsewhere;
+ *   gs_pbp_strain_fit: design (rows, 7) = (w M, w strain) (:1652-1656)
+ *     for tasks with worth[t] != 0; eps (T, 6) = (sxx syy szz sxy sxz syz),
+ *     NaN for the other tasks.
+ * Non-finite rows give NaN (numpy's lstsq fails on them). */
+int gs_pbp_ubi_fit(const double *design, const long long *off, const int *fit,
+                   const double *ubi_in, const double B0[9], int T, double *ubifit, int *rank,
+                   int *worth, double *ubi_out, double *ub0, gs_stream_t stream);
+int gs_pbp_strain_fit(const double *design, const long long *off, const int *worth, int T,
+                      double *eps, gs_stream_t stream);
+
 /* Per-segment column sums of a row-major (rows, C) array: out (S, C),
  * out[s][c] = the sum of rows off[s] .. off[s+1]-1 of column c, added one
  * row at a time from 0.0 (np.bincount's order: refine_map's merge,

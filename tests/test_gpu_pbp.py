@@ -114,3 +114,93 @@ def test_refine_map_edge_cases(dev, wl):
     want = P.refine_map(*[args[k] for k in ARGS], **wl["geometry"], min_grain_npks=10 ** 9)
     for x, y in zip(got, want):
         assert np.isnan(x).all() and np.isnan(y).all()
+
+
+def _T(a, dev, dt=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+
+
+def _lstsq_tasks(rng, nc, nr, T=300):
+    """Random per-task systems: full rank, rank-deficient (rows in a
+    subspace), fewer rows than columns, and scaled rows."""
+    rows, off = [], [0]
+    for t in range(T):
+        n = int(rng.integers(1, 40))
+        a = rng.normal(size=(n, nc)) * 10.0 ** rng.uniform(-2, 2)
+        kind = t % 4
+        if kind == 1:  # rank nc - 1 or less: rows in a random subspace
+            k = int(rng.integers(1, nc))
+            a = rng.normal(size=(n, k)) @ rng.normal(size=(k, nc))
+        elif kind == 2:  # exactly repeated directions
+            a = np.repeat(a[:1], n, 0) * rng.uniform(0.5, 2, size=(n, 1))
+        b = rng.normal(size=(n, nr))
+        rows.append(np.hstack([a, b]))
+        off.append(off[-1] + n)
+    return np.vstack(rows), np.array(off, np.int64)
+
+
+def test_pbp_strain_fit_is_lstsq(dev):
+    """gs_pbp_strain_fit against np.linalg.lstsq(a, b, rcond=eps*max(n, 6))
+    per task: rank-deficient and under-determined tasks get the minimum-norm
+    solution; tasks not marked stay NaN; a non-finite row gives NaN."""
+    rng = np.random.default_rng(8)
+    d, off = _lstsq_tasks(rng, 6, 1)
+    T = len(off) - 1
+    worth = np.ones(T, bool)
+    worth[::7] = False
+    d[off[5]] = np.nan  # task 5: a NaN row
+    got = PP.strain_fit(_T(d, dev), _T(off, dev, torch.int64), _T(worth, dev, torch.bool)).cpu().numpy()
+    eps = np.finfo(float).eps
+    for t in range(T):
+        if not worth[t] or t == 5:
+            assert np.isnan(got[t]).all()
+            continue
+        a, b = d[off[t]:off[t + 1], :6], d[off[t]:off[t + 1], 6]
+        want = np.linalg.lstsq(a, b, rcond=eps * max(len(a), 6))[0]
+        scale = np.abs(want).max() + 1e-300
+        assert np.abs(got[t] - want).max() <= 1e-9 * scale, t
+
+
+def test_pbp_ubi_fit_is_lstsq_and_quality_test(dev):
+    """gs_pbp_ubi_fit against np.linalg.lstsq per task (solution and rank)
+    and the reference's cond / det / matrix_rank test; ub0 = U.B0 of the
+    fitted cell (the oracle's ubi_and_ucell_to_u) where the fit is kept."""
+    rng = np.random.default_rng(9)
+    d, off = _lstsq_tasks(rng, 3, 3)
+    # half of the tasks: a lattice-like fit (rows w g, w hkl with hkl = UBI g)
+    for t in range(0, len(off) - 1, 2):
+        n = off[t + 1] - off[t]
+        ub = np.linalg.inv(np.diag([4.0, 5.0, 6.0]) + rng.normal(scale=0.1, size=(3, 3)))
+        h = rng.integers(-4, 5, size=(n, 3)).astype(float)
+        w = rng.uniform(0.3, 1.0, size=(n, 1))
+        d[off[t]:off[t + 1]] = np.hstack([w * (h @ ub.T), w * h])
+    T = len(off) - 1
+    fit = np.ones(T, bool)
+    fit[1::9] = False
+    ubis = rng.normal(size=(T, 3, 3))
+    B0 = np.eye(3) / 4.05
+    ubifit, rank, worth, ubi_out, ub0 = (x.cpu().numpy() for x in PP.ubi_fit(
+        _T(d, dev), _T(off, dev, torch.int64), _T(fit, dev, torch.bool), _T(ubis, dev), B0))
+    eps = np.finfo(float).eps
+    n_worth = 0
+    for t in range(T):
+        if not fit[t]:
+            assert not worth[t] and np.isnan(ubifit[t]).all()
+            assert np.array_equal(ubi_out[t], ubis[t])
+            continue
+        a, b = d[off[t]:off[t + 1], :3], d[off[t]:off[t + 1], 3:]
+        x, _, r, _ = np.linalg.lstsq(a, b, rcond=eps * max(len(a), 3))
+        assert rank[t] == r, t
+        u = x.T
+        np.testing.assert_allclose(ubifit[t], u, rtol=0, atol=1e-9 * (np.abs(u).max() + 1e-300))
+        ok = (r == 3 and np.linalg.cond(u) < 1e14 and np.linalg.det(u) > 0 and
+              np.linalg.matrix_rank(u) == 3)
+        assert bool(worth[t]) == ok, t
+        if ok:
+            n_worth += 1
+            np.testing.assert_array_equal(ubi_out[t], ubifit[t])
+            want = P.ubi_and_ucell_to_u(u, P.ubi_to_unitcell(u)) @ B0
+            np.testing.assert_allclose(ub0[t], want, rtol=0, atol=1e-9 * np.abs(want).max())
+        else:
+            assert np.array_equal(ubi_out[t], ubis[t]) and np.isnan(ub0[t]).all()
+    assert n_worth > 100

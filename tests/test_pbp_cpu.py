@@ -159,9 +159,9 @@ def test_oracle_refine_map_recovers_truth(wl):
 def test_refine_map_host_stages_match_oracle(wl, monkeypatch):
     """The product's stages after the selection (merge, re-selection,
     batched fits, strain) run as torch operations; here on CPU tensors, with
-    the GPU pieces (g-vectors, the selection kernel, the segmented sums)
-    replaced by the oracle's numpy equivalents: the same results as the
-    oracle's voxel loop."""
+    the GPU pieces (g-vectors, the selection kernel, the segmented sums, the
+    per-task lstsq kernels) replaced by numpy equivalents: the same results
+    as the oracle's voxel loop."""
     monkeypatch.setattr(PP, "_dev", lambda: torch.device("cpu"))
     monkeypatch.setattr(PP, "_gve_device", lambda sc, fc, om, xp, geo: torch.as_tensor(
         P.compute_gve(sc.numpy(), fc.numpy(), om.numpy(), xp.numpy(), **geo).T.copy()))
@@ -188,8 +188,39 @@ def test_refine_map_host_stages_match_oracle(wl, monkeypatch):
         lab = np.repeat(np.arange(len(off) - 1), np.diff(off))
         return torch.as_tensor(np.stack([np.bincount(lab, weights=v[:, c], minlength=len(off) - 1)
                                          for c in range(v.shape[1])], 1))
+    eps = np.finfo(float).eps
+
+    def ubi_fit(design, off, fit, ubis, B0):
+        # np.linalg.lstsq per task on its design rows, as the reference fits
+        d, off, fit, ubis = design.numpy(), off.numpy(), fit.numpy(), ubis.numpy()
+        T = len(fit)
+        ubifit = np.full((T, 3, 3), np.nan)
+        rank = np.zeros(T, np.int32)
+        worth = np.zeros(T, bool)
+        ub0 = np.full((T, 3, 3), np.nan)
+        for t in np.nonzero(fit)[0]:
+            a, b = d[off[t]:off[t + 1], :3], d[off[t]:off[t + 1], 3:]
+            x, _, rank[t], _ = np.linalg.lstsq(a, b, rcond=eps * max(len(a), 3))
+            u = ubifit[t] = x.T
+            worth[t] = (rank[t] == 3 and np.linalg.cond(u) < 1e14 and np.linalg.det(u) > 0
+                        and np.linalg.matrix_rank(u) == 3)
+            if worth[t]:
+                ub0[t] = P.ubi_and_ucell_to_u(u, P.ubi_to_unitcell(u)) @ B0
+        ubi_out = np.where(worth[:, None, None], ubifit, ubis)
+        return (torch.as_tensor(ubifit), torch.as_tensor(rank), torch.as_tensor(worth),
+                torch.as_tensor(ubi_out), torch.as_tensor(ub0))
+
+    def strain_fit(design, off, worth):
+        d, off, worth = design.numpy(), off.numpy(), worth.numpy()
+        e = np.full((len(worth), 6), np.nan)
+        for t in np.nonzero(worth)[0]:
+            a, b = d[off[t]:off[t + 1], :6], d[off[t]:off[t + 1], 6]
+            e[t] = np.linalg.lstsq(a, b, rcond=eps * max(len(a), 6))[0]
+        return torch.as_tensor(e)
     monkeypatch.setattr(PP, "assign_tasks", assign)
     monkeypatch.setattr(PP, "segment_sum", segsum)
+    monkeypatch.setattr(PP, "ubi_fit", ubi_fit)
+    monkeypatch.setattr(PP, "strain_fit", strain_fit)
     a = PP.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], tol=0.1, merge_tol=0.05)
     b = P.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], tol=0.1, merge_tol=0.05)
     for x, y in zip(a, b):
