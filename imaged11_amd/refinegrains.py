@@ -191,10 +191,15 @@ def _tth_column(colfile):
 
 
 def _intensity_summary(weighted, tth, indices, tth_range, ntrim, quiet):
-    intensities = np.take(weighted, indices)
+    return _summary_of(np.take(weighted, indices), np.take(tth, indices), tth_range, ntrim,
+                       quiet)
+
+
+def _summary_of(intensities, tv, tth_range, ntrim, quiet):
+    """The summary string of one grain's intensities (weighted, in peak
+    order) and their tth values."""
     sigma_i = np.sum(intensities)
     if tth_range is not None:
-        tv = np.take(tth, indices)
         intensities = np.compress((tv > min(tth_range)) & (tv < max(tth_range)), intensities)
     if len(intensities) < 1:
         return "no peaks"
@@ -235,6 +240,7 @@ class _ScanState:
         self.so = self.co = None
         self.idx = self.off = None
         self.off_host = None
+        self.requests = None  # (positions, lists) of the last _requests call
         # per grain position (assignlabels' order): the UBIs read and the
         # current ones, on the device
         self.names = []
@@ -419,9 +425,14 @@ class refinegrains:
     # -- the batched core --------------------------------------------------
     def _requests(self, sname, positions):
         """CSR entries of the requests: (idx, owner, off (R+1) host and
-        device) for grain positions ``positions`` (one request each)."""
+        device) for grain positions ``positions`` (one request each).  The
+        lists of the last set of positions are kept (the lockstep simplex
+        asks for the same set round after round) until assignlabels
+        rebuilds the grains' lists."""
         st = self._state[sname]
         pos = np.asarray(positions, np.int64)
+        if st.requests is not None and np.array_equal(st.requests[0], pos):
+            return st.requests[1]
         lengths = st.off_host[pos + 1] - st.off_host[pos]
         off = np.zeros(len(pos) + 1, np.int64)
         off[1:] = np.cumsum(lengths)
@@ -434,7 +445,17 @@ class refinegrains:
         off_d = torch.as_tensor(off, device=dev)
         within = torch.arange(M, device=dev) - off_d[owner]
         idx = st.idx[(starts[owner] + within)].contiguous()
-        return idx, owner.to(torch.int32), off, off_d
+        out = (idx, owner.to(torch.int32), off, off_d)
+        st.requests = (pos.copy(), out, batched.list_order(off_d))
+        return out
+
+    def _order_of(self, sname, off_d):
+        """The refinement's grain order for lists off_d (kept with the
+        lists of the last request set)."""
+        st = self._state[sname]
+        if st.requests is not None and st.requests[1][3] is off_d:
+            return st.requests[2]
+        return batched.list_order(off_d)
 
     def _geometry(self, sname, names, trans, angles=False, ubis=None):
         """g-vectors (host) of the listed grains at the given translations,
@@ -464,7 +485,8 @@ class refinegrains:
             else:
                 u = torch.as_tensor(np.ascontiguousarray(ubis, dtype=np.float64),
                                     device=st.device).reshape(-1, 3, 3)
-            mats, _, _ = batched.refine_lists(u, gv, None, off_d, self.tolerance, 1)
+            mats, _, _ = batched.refine_lists(u, gv, None, off_d, self.tolerance, 1,
+                                              order=self._order_of(sname, off_d))
             gv, tth, eta, omc = batched.grain_gv(st.xl, st.so, st.co, st.omega, p, trans_d, idx,
                                                  owner, mats=mats, slop=self.slop, angles=angles)
         return gv, tth, eta, omc
@@ -503,14 +525,15 @@ class refinegrains:
         gv, _, _, _ = self._gv_device(sname, idx, owner, off_d, trans, None, False,
                                       ubis=st.ubi_cur[pos_d])
         u = (st.ubi_read if start == "read" else st.ubi_cur)[pos_d].contiguous()
+        order = self._order_of(sname, off_d)
         if self.latticesymmetry is triclinic:
-            u, n, m = batched.refine_lists(u, gv, None, off_d, self.tolerance, 2)
+            u, n, m = batched.refine_lists(u, gv, None, off_d, self.tolerance, 2, order=order)
             k = 1
         else:
-            u, _, _ = batched.refine_lists(u, gv, None, off_d, self.tolerance, 1)
+            u, _, _ = batched.refine_lists(u, gv, None, off_d, self.tolerance, 1, order=order)
             u = torch.as_tensor(symmetrize(u.cpu().numpy(), self.latticesymmetry),
                                 device=st.device)
-            u, n, m = batched.refine_lists(u, gv, None, off_d, self.tolerance, 1)
+            u, n, m = batched.refine_lists(u, gv, None, off_d, self.tolerance, 1, order=order)
             u = torch.as_tensor(symmetrize(u.cpu().numpy(), self.latticesymmetry),
                                 device=st.device)
             k = 0
@@ -717,6 +740,7 @@ class refinegrains:
             posarr = np.where(ok, lut[np.where(ok, lab_h, len(lut) - 1)], -1)
             st.idx, st.off, _ = batched.lists_from_labels(torch.as_tensor(posarr, device=dev), K)
             st.off_host = st.off.cpu().numpy()
+            st.requests = None
             # the g-vector column is compute_gv with the last grain's translation
             last_t = trans[-1].cpu().numpy() if K else np.zeros(3)
             self.gv = batched.gvectors_trig(st.xl, st.so, st.co,
@@ -733,33 +757,36 @@ class refinegrains:
             col.addcolumn(self.gv[:, 2], "gz")
             for name in ("hr", "kr", "lr", "h", "k", "l"):
                 col.addcolumn(np.zeros(nr, np.float32), name)
-            xl_h = st.xl.cpu().numpy()
-            idx_h = st.idx.cpu().numpy()
+            idx_h = st.idx.cpu().numpy().astype(np.int64)
             # observed tth, eta of every grain's peaks with its translation
             owner = torch.repeat_interleave(torch.arange(K, device=dev),
                                             st.off[1:] - st.off[:-1]).to(torch.int32)
             _, tth, eta, _ = batched.grain_gv(st.xl, st.so, st.co, st.omega, p, trans, st.idx,
                                               owner, angles=True)
-            tth, eta = tth.cpu().numpy(), eta.cpu().numpy()
+            # every column in list order once; each grain takes its slice (the
+            # lists are disjoint: a peak has one label)
+            xl_l = st.xl[st.idx.long()].cpu().numpy()
+            sc_l, fc_l, om_l = (np.take(c, idx_h) for c in (col.sc, col.fc, col.omega))
+            col.tth_per_grain[idx_h] = tth.cpu().numpy()
+            col.eta_per_grain[idx_h] = eta.cpu().numpy()
+            bounds = st.off_host.astype(np.int64)
             for j, n in enumerate(names):
                 gr = self.grains[(n, sname)]
-                a, b = int(st.off_host[j]), int(st.off_host[j + 1])
-                ind = idx_h[a:b].astype(np.int64)
-                gr.ind = ind
-                gr.peaks_xyz = xl_h[ind]
-                gr.sc = np.take(col.sc, ind)
-                gr.fc = np.take(col.fc, ind)
-                gr.om = np.take(col.omega, ind)
+                a, b = bounds[j], bounds[j + 1]
+                gr.ind = idx_h[a:b]
+                gr.peaks_xyz = xl_l[a:b]
+                gr.sc = sc_l[a:b]
+                gr.fc = fc_l[a:b]
+                gr.om = om_l[a:b]
                 gr.omega_calc = gr.om.copy()
-                gr.npks = len(ind)
-                col.tth_per_grain[ind] = tth[a:b]
-                col.eta_per_grain[ind] = eta[a:b]
+                gr.npks = int(b - a)
                 self.set_translation(n, sname)
                 if not quiet:
-                    print("Grain", n, "Scan", sname, "npks=", len(ind))
+                    print("Grain", n, "Scan", sname, "npks=", gr.npks)
             compute_lp_factor(col)
-            weighted, tthc = _weighted_intensity(col), _tth_column(col)
-            for n in names:
-                gr = self.grains[(n, sname)]
-                gr.intensity_info = _intensity_summary(weighted, tthc, gr.ind,
-                                                       self.intensity_tth_range, 2, quiet)
+            w_l = np.take(_weighted_intensity(col), idx_h)
+            t_l = np.take(_tth_column(col), idx_h)
+            for j, n in enumerate(names):
+                a, b = bounds[j], bounds[j + 1]
+                self.grains[(n, sname)].intensity_info = _summary_of(
+                    w_l[a:b], t_l[a:b], self.intensity_tth_range, 2, quiet)
