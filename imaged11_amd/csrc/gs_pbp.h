@@ -87,12 +87,14 @@ __global__ __launch_bounds__(GS_BLOCK) void k_pbp_assign(
     }
 }
 
-// Per-segment column sums, each summed row by row from 0.0 in row order:
-// out[s][c] = sum of v[r][c] for r in [off[s], off[s+1]).  That is the
-// order in which np.bincount(labels, weights) accumulates, so grouped
-// weighted sums (refine_map's merge, :338-356) come out bit for bit as the
-// reference's, and the same on every run.  One thread per (segment,
-// column); consecutive threads read consecutive columns of a row.
+// Per-segment column reductions, row by row from 0.0 in row order:
+// out[s][c] = the sum (MAX false) or the maximum (MAX true) of v[r][c] and
+// 0.0 for r in [off[s], off[s+1]).  The sums are added in the order in which
+// np.bincount(labels, weights) accumulates, so grouped weighted sums
+// (refine_map's merge, :338-356) come out bit for bit as the reference's,
+// and the same on every run; the maximum propagates NaN.  One thread per
+// (segment, column); consecutive threads read consecutive columns of a row.
+template <bool MAX>
 __global__ __launch_bounds__(GS_BLOCK) void k_segment_sum(const double* __restrict__ v, int C,
                                                           const long long* __restrict__ off,
                                                           int S, double* __restrict__ out) {
@@ -103,15 +105,22 @@ __global__ __launch_bounds__(GS_BLOCK) void k_segment_sum(const double* __restri
     double acc = 0.0;
     long long r = off[s];
     const long long r1 = off[s + 1];
-    // loads run ahead in groups of eight; the additions stay in row order
+    auto add = [&](double x) {
+        if (!MAX) {
+            acc += x;
+        } else if (x != x || x > acc) {
+            if (acc == acc) acc = x;  // a NaN stays
+        }
+    };
+    // loads run ahead in groups of eight; the reduction stays in row order
     for (; r + 8 <= r1; r += 8) {
         double x[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) x[j] = v[(r + j) * C + c];
 #pragma unroll
-        for (int j = 0; j < 8; j++) acc += x[j];
+        for (int j = 0; j < 8; j++) add(x[j]);
     }
-    for (; r < r1; r++) acc += v[r * C + c];
+    for (; r < r1; r++) add(v[r * C + c]);
     out[i] = acc;
 }
 
@@ -140,6 +149,28 @@ __device__ __forceinline__ void gs_qr_add_row(double (&R)[NC][NC + NR], double (
             R[k][j] = c * rj + s * xj;
             x[j] = c * xj - s * rj;
         }
+    }
+}
+
+// Stream rows [r0, r1) of a row-major (rows, NC+NR) array into [R | C],
+// CHUNK rows in flight: their loads are issued together (the row index
+// clamped to r1 - 1, so always in bounds), then they are rotated in in
+// row order and the copies past r1 dropped.
+template <int NC, int NR, int CHUNK>
+__device__ __forceinline__ void gs_qr_add_rows(double (&R)[NC][NC + NR],
+                                               const double* __restrict__ rows, long long r0,
+                                               long long r1) {
+    for (long long r = r0; r < r1; r += CHUNK) {
+        double x[CHUNK][NC + NR];
+#pragma unroll
+        for (int c = 0; c < CHUNK; c++) {
+            const long long rr = r + c < r1 ? r + c : r1 - 1;
+#pragma unroll
+            for (int j = 0; j < NC + NR; j++) x[c][j] = rows[(NC + NR) * rr + j];
+        }
+#pragma unroll
+        for (int c = 0; c < CHUNK; c++)
+            if (r + c < r1) gs_qr_add_row<NC, NR>(R, x[c]);
     }
 }
 
@@ -324,12 +355,7 @@ __global__ __launch_bounds__(64) void k_pbp_ubi_fit(
 #pragma unroll
             for (int j = 0; j < 6; j++) R[i][j] = 0.0;
         const long long r0 = off[t], r1 = off[t + 1];
-        for (long long r = r0; r < r1; r++) {
-            double x[6];
-#pragma unroll
-            for (int j = 0; j < 6; j++) x[j] = design[6 * r + j];
-            gs_qr_add_row<3, 3>(R, x);
-        }
+        gs_qr_add_rows<3, 3, 8>(R, design, r0, r1);
         const double n = (double)(r1 - r0);
         double X[3][3];  // ubifit^T
         rank = gs_lstsq_solve<3, 3>(R, 2.220446049250313e-16 * fmax(n, 3.0), X);
@@ -380,12 +406,7 @@ __global__ __launch_bounds__(64) void k_pbp_strain_fit(const double* __restrict_
 #pragma unroll
             for (int j = 0; j < 7; j++) R[i][j] = 0.0;
         const long long r0 = off[t], r1 = off[t + 1];
-        for (long long r = r0; r < r1; r++) {
-            double x[7];
-#pragma unroll
-            for (int j = 0; j < 7; j++) x[j] = design[7 * r + j];
-            gs_qr_add_row<6, 1>(R, x);
-        }
+        gs_qr_add_rows<6, 1, 4>(R, design, r0, r1);
         const double n = (double)(r1 - r0);
         double X[6][1];
         gs_lstsq_solve<6, 1>(R, 2.220446049250313e-16 * fmax(n, 6.0), X);

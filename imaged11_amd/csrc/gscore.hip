@@ -1246,7 +1246,20 @@ int gs_segment_sum(const double* v, int C, const long long* off, int nseg, doubl
     if (C < 0 || nseg < 0) return set_err(GS_ERR_ARG, "gs_segment_sum: negative size");
     if ((long long)C * nseg == 0) return GS_OK;
     if (!v || !off || !out) return set_err(GS_ERR_ARG, "gs_segment_sum: NULL array");
-    hipLaunchKernelGGL(k_segment_sum, dim3(blocks_for((long long)C * nseg, GS_BLOCK)),
+    hipLaunchKernelGGL(k_segment_sum<false>, dim3(blocks_for((long long)C * nseg, GS_BLOCK)),
+                       dim3(GS_BLOCK), 0, S(stream), v, C, off, nseg, out);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_segment_max(const double* v, int C, const long long* off, int nseg, double* out,
+                   gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (C < 0 || nseg < 0) return set_err(GS_ERR_ARG, "gs_segment_max: negative size");
+    if ((long long)C * nseg == 0) return GS_OK;
+    if (!v || !off || !out) return set_err(GS_ERR_ARG, "gs_segment_max: NULL array");
+    hipLaunchKernelGGL(k_segment_sum<true>, dim3(blocks_for((long long)C * nseg, GS_BLOCK)),
                        dim3(GS_BLOCK), 0, S(stream), v, C, off, nseg, out);
     GS_LAUNCHED();
     return GS_OK;

@@ -221,6 +221,22 @@ def segment_sum(v, off):
     return out
 
 
+def segment_max(v, off):
+    """Column maxima of the row segments off[s]..off[s+1] of v (rows, C)
+    and 0.0 (gs_segment_max; an empty segment gives 0.0)."""
+    lib = _lib.load()
+    lib.gs_clear_error()
+    v = v.contiguous()
+    S = off.numel() - 1
+    C = v.shape[1]
+    if v.shape[0] == 0:
+        return torch.zeros((S, C), dtype=torch.float64, device=v.device)
+    out = torch.empty((S, C), dtype=torch.float64, device=v.device)
+    _lib.check(lib.gs_segment_max(dptr(v), C, dptr(off), S, dptr(out), stream_ptr()),
+               "segment_max")
+    return out
+
+
 def _offsets(seg, S):
     """CSR offsets of rows already ordered by segment id seg (0..S-1)."""
     off = torch.zeros(S + 1, dtype=torch.int64, device=seg.device)
@@ -420,7 +436,8 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
 def _strain_fit(worth_d, ub0_d, rtask, sel2, g, hi2, rydist, gnoise_std, T):
     """The directional-strain fit of refine_map (:1624-1668) for every task
     with worth_d at once: per-row terms on the device, per-task mean and
-    standard deviation by gs_segment_sum, the lstsq solves by
+    standard deviation by gs_segment_sum, the weight maximum by
+    gs_segment_max, the lstsq solves by
     gs_pbp_strain_fit.  Returns (T, 3, 3) strain tensors (NaN where not
     fitted)."""
     dev = g.device
@@ -447,7 +464,7 @@ def _strain_fit(worth_d, ub0_d, rtask, sel2, g, hi2, rydist, gnoise_std, T):
                      torch.clamp(n, min=1))
     w = torch.where(strain > mean[t] + std[t] * 3.5, torch.zeros_like(w), w)
     w = torch.where(strain < mean[t] - std[t] * 3.5, torch.zeros_like(w), w)
-    wmax = torch.zeros(T, dtype=torch.float64, device=dev).scatter_reduce(0, t, w, "amax")
+    wmax = segment_max(w[:, None], off)[:, 0]
     w = w / wmax[t]
     design = torch.cat([w[:, None] * Mrow, (w * strain)[:, None]], 1)
     e = strain_fit(design, off, worth_d).cpu().numpy()

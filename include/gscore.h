@@ -347,6 +347,11 @@ int gs_pbp_strain_fit(const double *design, const long long *off, const int *wor
  * (S+1) int64. */
 int gs_segment_sum(const double *v, int C, const long long *off, int S, double *out,
                    gs_stream_t stream);
+/* The same layout, out[s][c] = the maximum of 0.0 and rows off[s] ..
+ * off[s+1]-1 of column c (NaN if any is NaN): refine_map's per-task weight
+ * maximum (:1650), the 0.0 being the zeros torch reduces into. */
+int gs_segment_max(const double *v, int C, const long long *off, int S, double *out,
+                   gs_stream_t stream);
 
 /* ---- column-file geometry ---- */
 

@@ -204,3 +204,22 @@ def test_pbp_ubi_fit_is_lstsq_and_quality_test(dev):
         else:
             assert np.array_equal(ubi_out[t], ubis[t]) and np.isnan(ub0[t]).all()
     assert n_worth > 100
+
+
+def test_segment_max_is_max_with_zero(dev):
+    """gs_segment_max: the maximum of each segment's rows and 0.0 (the
+    zeros torch's scatter_reduce reduces into), NaN propagated, empty
+    segments 0.0."""
+    rng = np.random.default_rng(10)
+    lens = rng.integers(0, 30, 400)
+    off = np.zeros(401, np.int64)
+    off[1:] = np.cumsum(lens)
+    v = rng.normal(size=(off[-1], 3))
+    v[off[7]:off[8]] = -np.abs(v[off[7]:off[8]])  # all negative: 0.0
+    if lens[9]:
+        v[off[9], 1] = np.nan
+    got = PP.segment_max(_T(v, dev), _T(off, dev, torch.int64)).cpu().numpy()
+    t = np.repeat(np.arange(400), lens)
+    want = torch.zeros((400, 3), dtype=torch.float64).scatter_reduce(
+        0, torch.as_tensor(np.repeat(t[:, None], 3, 1)), torch.as_tensor(v), "amax").numpy()
+    assert np.array_equal(got, want, equal_nan=True)

@@ -219,6 +219,13 @@ def test_refine_map_host_stages_match_oracle(wl, monkeypatch):
         return torch.as_tensor(e)
     monkeypatch.setattr(PP, "assign_tasks", assign)
     monkeypatch.setattr(PP, "segment_sum", segsum)
+
+    def segmax(v, off):
+        v, off = v.numpy(), off.numpy()
+        return torch.as_tensor(np.stack([[max([0.0] + list(v[off[s]:off[s + 1], c]))
+                                          for c in range(v.shape[1])]
+                                         for s in range(len(off) - 1)]))
+    monkeypatch.setattr(PP, "segment_max", segmax)
     monkeypatch.setattr(PP, "ubi_fit", ubi_fit)
     monkeypatch.setattr(PP, "strain_fit", strain_fit)
     a = PP.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], tol=0.1, merge_tol=0.05)
