@@ -95,6 +95,8 @@ SIGNATURES = {
                          _vp, ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp]),
     "gs_pbp_assign": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _i, _vp, _i, _d, _d, _d, _vp, _vp,
                            _vp, _vp, _vp, _vp]),
+    "gs_pbp_merge_sums": (_i, [_vp, _vp, ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                               _vp, _i, _vp, _vp]),
     "gs_pbp_ubi_fit": (_i, [_vp, _vp, _vp, _vp, _dp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gs_pbp_strain_fit": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
     "gs_segment_sum": (_i, [_vp, _i, _vp, _i, _vp, _vp]),

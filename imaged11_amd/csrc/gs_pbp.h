@@ -124,6 +124,44 @@ __global__ __launch_bounds__(GS_BLOCK) void k_segment_sum(const double* __restri
     out[i] = acc;
 }
 
+// The per-peak columns the merge of refine_map averages.
+struct gs_merge_cols {
+    const double *si, *sc, *fc, *om, *dty, *xpos, *eta;
+};
+
+// refine_map's merge (:338-356) for every (task, h, k, l, sign eta, dtyi)
+// group at once: out[g] = (sum w, sum sc w, sum fc w, sum omega w, sum dty w,
+// sum xpos w, sum eta w) over rows r in [goff[g], goff[g+1]) of the grouped
+// order, row r being peak p = peak[perm[r]] with weight w = si[p].  Each
+// sum starts at 0.0 and adds in row order, and each product is x * w: the
+// gathered, multiplied, permuted and segment-summed columns of the
+// unfused form, bit for bit.  One thread per group; a peak outside [0, npk)
+// is skipped.
+__global__ __launch_bounds__(GS_BLOCK) void k_pbp_merge_sums(const long long* __restrict__ perm,
+                                                             const long long* __restrict__ peak,
+                                                             gs_merge_cols c,
+                                                             const long long* __restrict__ goff,
+                                                             int G, long long npk,
+                                                             double* __restrict__ out) {
+    const long long g = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
+    if (g >= G) return;
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long long r = goff[g]; r < goff[g + 1]; r++) {
+        const long long p = peak[perm[r]];
+        if (p < 0 || p >= npk) continue;
+        const double w = c.si[p];
+        acc[0] += w;
+        acc[1] += c.sc[p] * w;
+        acc[2] += c.fc[p] * w;
+        acc[3] += c.om[p] * w;
+        acc[4] += c.dty[p] * w;
+        acc[5] += c.xpos[p] * w;
+        acc[6] += c.eta[p] * w;
+    }
+#pragma unroll
+    for (int j = 0; j < 7; j++) out[7 * g + j] = acc[j];
+}
+
 // ---------------------------------------------------------------------------
 // The per-task least-squares fits of refine_map, one thread per task,
 // solved as np.linalg.lstsq solves them: the minimum-norm solution with

@@ -1211,6 +1211,21 @@ int gs_pbp_assign(const double* so, const double* co, const double* dty, const d
     return GS_OK;
 }
 
+int gs_pbp_merge_sums(const long long* perm, const long long* peak, long long npk,
+                      const double* si, const double* sc, const double* fc, const double* om,
+                      const double* dty, const double* xpos, const double* eta,
+                      const long long* goff, int G, double* out, gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (G < 0 || npk < 0) return set_err(GS_ERR_ARG, "gs_pbp_merge_sums: negative size");
+    if (G == 0) return GS_OK;
+    gs_merge_cols c{si, sc, fc, om, dty, xpos, eta};
+    hipLaunchKernelGGL(k_pbp_merge_sums, dim3(blocks_for(G, GS_BLOCK)), dim3(GS_BLOCK), 0,
+                       S(stream), perm, peak, c, goff, G, npk, out);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
 int gs_pbp_ubi_fit(const double* design, const long long* off, const int* fit,
                    const double* ubi_in, const double B0[9], int T, double* ubifit, int* rank,
                    int* worth, double* ubi_out, double* ub0, gs_stream_t stream) {

@@ -221,6 +221,25 @@ def segment_sum(v, off):
     return out
 
 
+def merge_sums(perm, peak, si, cols, goff):
+    """The merge's weighted sums per group (gs_pbp_merge_sums): (G, 7) =
+    sums of w and of each of the six columns (sc, fc, omega, dty, xpos,
+    eta) times w over the rows goff[g]..goff[g+1] of the grouped order (row
+    r is peak peak[perm[r]], w its intensity), in row order from 0.0."""
+    lib = _lib.load()
+    lib.gs_clear_error()
+    G = goff.numel() - 1
+    out = torch.empty((G, 7), dtype=torch.float64, device=si.device)
+    if G == 0:
+        return out
+    perm, peak, si = perm.contiguous(), peak.contiguous(), si.contiguous()
+    cols = [c.contiguous() for c in cols]
+    _lib.check(lib.gs_pbp_merge_sums(dptr(perm), dptr(peak), si.shape[0], dptr(si),
+                                     *[dptr(c) for c in cols], dptr(goff), G, dptr(out),
+                                     stream_ptr()), "pbp_merge_sums")
+    return out
+
+
 def segment_max(v, off):
     """Column maxima of the row segments off[s]..off[s+1] of v (rows, C)
     and 0.0 (gs_segment_max; an empty segment gives 0.0)."""
@@ -385,10 +404,7 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
                       torch.sign(eta_d[peak]).long(), dtyi_d[peak]])
     perm = torch.sort(gid, stable=True).indices
     goff = _offsets(gid, G)
-    wI = si_d[peak]
-    cols = torch.stack([wI] + [x[peak] * wI for x in (sc_d, fc_d, om_d, dty_d, xpos_d, eta_d)],
-                       1)[perm]
-    sums = segment_sum(cols, goff)
+    sums = merge_sums(perm, peak, si_d, (sc_d, fc_d, om_d, dty_d, xpos_d, eta_d), goff)
     sI = sums[:, 0]
     m_sc, m_fc, m_om, m_dty, m_xpos, m_eta = (sums[:, k] / sI for k in range(1, 7))
     gtask = task[perm][goff[:-1]]

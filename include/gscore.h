@@ -319,6 +319,17 @@ int gs_pbp_assign(const double *so, const double *co, const double *dty, const d
                   double y0, double ystep, double tol, int *counts, const long long *task_off,
                   int *cursor, int *out_peak, int *out_hkl, gs_stream_t stream);
 
+/* refine_map's merge (ImageD11/sinograms/point_by_point.py:338-356) for
+ * all groups at once: out (G, 7) = per group g the sums of w, sc w, fc w,
+ * omega w, dty w, xpos w, eta w over rows r in [goff[g], goff[g+1]) of the
+ * grouped order, row r being peak p = peak[perm[r]] (npk peaks) with
+ * w = si[p]; each sum from 0.0 in row order (np.bincount's order), each
+ * product x * w.  Device pointers; perm and peak int64. */
+int gs_pbp_merge_sums(const long long *perm, const long long *peak, long long npk,
+                      const double *si, const double *sc, const double *fc, const double *om,
+                      const double *dty, const double *xpos, const double *eta,
+                      const long long *goff, int G, double *out, gs_stream_t stream);
+
 /* The least-squares fits of refine_map, one thread per task t < T, as
  * np.linalg.lstsq solves them (minimum-norm solution, singular values of
  * the weighted design matrix <= eps * max(rows, columns) * the largest

@@ -223,3 +223,22 @@ def test_segment_max_is_max_with_zero(dev):
     want = torch.zeros((400, 3), dtype=torch.float64).scatter_reduce(
         0, torch.as_tensor(np.repeat(t[:, None], 3, 1)), torch.as_tensor(v), "amax").numpy()
     assert np.array_equal(got, want, equal_nan=True)
+
+
+def test_merge_sums_equal_the_unfused_columns(dev):
+    """gs_pbp_merge_sums against gathering, weighting and permuting the
+    columns then summing each group with gs_segment_sum: bit for bit."""
+    rng = np.random.default_rng(11)
+    n, E = 5000, 40000
+    peak = _T(rng.integers(0, n, E), dev, torch.int64)
+    gid = _T(rng.integers(0, 3000, E), dev, torch.int64)
+    perm = torch.sort(gid, stable=True).indices
+    counts = torch.bincount(gid, minlength=3000)
+    goff = torch.zeros(3001, dtype=torch.int64, device=dev)
+    goff[1:] = torch.cumsum(counts, 0)
+    si = _T(rng.uniform(1, 100, n), dev)
+    cols = [_T(rng.normal(size=n) * 10.0 ** rng.integers(-3, 3), dev) for _ in range(6)]
+    got = PP.merge_sums(perm, peak, si, cols, goff)
+    wI = si[peak]
+    want = PP.segment_sum(torch.stack([wI] + [c[peak] * wI for c in cols], 1)[perm], goff)
+    assert torch.equal(got, want)

@@ -226,6 +226,13 @@ def test_refine_map_host_stages_match_oracle(wl, monkeypatch):
                                           for c in range(v.shape[1])]
                                          for s in range(len(off) - 1)]))
     monkeypatch.setattr(PP, "segment_max", segmax)
+
+    def msums(perm, peak, si, cols, goff):
+        p = peak.numpy()[perm.numpy()]
+        w = si.numpy()[p]
+        v = np.stack([w] + [c.numpy()[p] * w for c in cols], 1)
+        return segsum(torch.as_tensor(v), goff)
+    monkeypatch.setattr(PP, "merge_sums", msums)
     monkeypatch.setattr(PP, "ubi_fit", ubi_fit)
     monkeypatch.setattr(PP, "strain_fit", strain_fit)
     a = PP.refine_map(*[wl[k] for k in ARGS], **wl["geometry"], tol=0.1, merge_tol=0.05)
