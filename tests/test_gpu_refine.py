@@ -177,3 +177,19 @@ def test_rows_must_cover_the_lists(dev):
     assert rc != 0
     assert b"cover" in lib.gs_last_error()
     lib.gs_clear_error()
+
+
+def test_empty_lists_rows_mode(dev):
+    """No list entries at all, gathered (n_iter >= 2) or given as rows: the
+    UBIs are kept, n = 0 and mean = 0 for every call, as score_and_refine
+    on no peaks returns."""
+    ubis = torch.eye(3, dtype=torch.float64, device=dev)[None].repeat(5, 1, 1) * 4.0
+    gv = torch.zeros((10, 3), dtype=torch.float64, device=dev)
+    idx = torch.zeros(0, dtype=torch.int32, device=dev)
+    off = torch.zeros(6, dtype=torch.int64, device=dev)
+    for g_idx, g_rows in ((idx, gv), (None, gv[:0])):
+        u, n, m = batched.refine_lists(ubis, g_rows, g_idx, off, 0.05, 3)
+        torch.cuda.synchronize()
+        assert torch.equal(u, ubis)
+        assert int(n.abs().sum()) == 0 and float(m.abs().sum()) == 0.0
+    assert batched.gather_rows(gv, idx).shape == (0, 3)
