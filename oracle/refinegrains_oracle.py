@@ -27,8 +27,10 @@ Pinning (what a plausible mistake would fail):
   noise-free simulated data, and the C calls are the reference's per-grain
   calls in the reference's order.
 ``ubi_to_cell`` / ``ubi_to_u`` / ``u_to_ubi`` stand in for the absent
-``xfab.tools`` functions of the same names (parity unpinned against xfab;
-pinned by the properties above).
+``xfab.tools`` functions of the same names; they are pinned by closed forms
+(a UBI built as (U B(cell))^-1 from known cells, cubic to triclinic, returns
+that cell and U, and u_to_ubi rebuilds it) and by the properties above;
+bit-level agreement with xfab itself cannot be checked here (xfab absent).
 """
 import math
 

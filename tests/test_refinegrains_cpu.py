@@ -232,6 +232,27 @@ def test_lattice_symmetry_projection():
                                rtol=1e-12, atol=1e-12)
 
 
+def test_cell_and_orientation_closed_forms():
+    """UBI = (U B(cell))^-1 for known cells (cubic to triclinic) and random
+    rotations: ubi_to_cell returns the cell, ubi_to_u the rotation, and
+    u_to_ubi rebuilds the UBI (the product and the oracle)."""
+    from imaged11_amd import unitcell
+    rng = np.random.default_rng(31)
+    cells = ([4.05] * 3 + [90.0] * 3, [3.0, 4.0, 5.0, 90.0, 90.0, 90.0],
+             [3.1, 3.1, 5.2, 90.0, 90.0, 120.0], [3.1, 4.2, 5.3, 80.0, 95.0, 101.0])
+    U = simulate.random_rotations(len(cells), rng)
+    for k, cell in enumerate(cells):
+        B = unitcell.unitcell(cell, "P").B
+        ubi = np.linalg.inv(U[k] @ B)
+        for mod in (RG, R):
+            np.testing.assert_allclose(np.ravel(mod.ubi_to_cell(ubi)), cell, rtol=1e-12,
+                                       atol=1e-10)
+            np.testing.assert_allclose(np.reshape(mod.ubi_to_u(ubi), (3, 3)), U[k],
+                                       atol=1e-12)
+            np.testing.assert_allclose(np.reshape(mod.u_to_ubi(U[k], np.array(cell)), (3, 3)),
+                                       ubi, rtol=1e-12, atol=1e-12)
+
+
 def test_b_matrix_is_the_unitcell_b():
     from imaged11_amd import unitcell
     for cell in ([4.05] * 3 + [90] * 3, [3.1, 4.2, 5.3, 80.0, 95.0, 101.0]):
