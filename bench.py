@@ -466,6 +466,70 @@ def aux_pbp(with_cpu, NY=101, grains=24, cpu_points=40):
     return out
 
 
+def aux_geometry(threads, with_cpu, N=8_000_000, seed=7):
+    """The updateGeometry pass of a column file (compute_xlylzl then
+    compute_geometry: xl, yl, zl, tth, eta, ds, gx, gy, gz of every peak)
+    over N raw peaks in one fused kernel (k_sf2geometry), against the
+    reference C compiled here (compute_xlylzl serial as shipped,
+    compute_geometry with OpenMP) on the same peaks.  Algorithmic HBM bytes
+    per peak: 24 in (sc, fc, omega) + 72 out."""
+    import numpy as np
+    import torch
+    from imaged11_amd import _lib, geometry, simulate
+    from imaged11_amd._lib import darr, dptr, stream_ptr
+    rng = np.random.default_rng(seed)
+    pars = simulate.detector_parameters(rng, wedge=0.3, chi=-0.2)
+    sc, fc = rng.uniform(0, 2048, N), rng.uniform(0, 2048, N)
+    om = rng.uniform(-180, 180, N)
+    t = np.array([12.0, -7.0, 3.0])
+    dev = torch.device("cuda")
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    s_d, f_d, w_d = T(sc), T(fc), T(om)
+    xl = torch.empty((N, 3), dtype=torch.float64, device=dev)
+    out = torch.empty((N, 6), dtype=torch.float64, device=dev)
+    p, r, dist = geometry.pack_xlylzl(pars)
+    sign, wvln, wedge, chi = geometry.pack_gv(pars)
+    lib = _lib.load()
+    args = (darr(p), darr(r), darr(dist), float(sign), float(wvln), float(wedge), float(chi),
+            darr(t))
+
+    def run():
+        _lib.check(lib.gs_sf2geometry(dptr(s_d), dptr(f_d), dptr(w_d), *args, dptr(xl),
+                                      dptr(out), N, stream_ptr()), "sf2geometry")
+    ms = _events_ms(run, reps=10)
+    res = {"peaks": N, "gpu_ms": ms, "peaks_per_s": N / (ms * 1e-3),
+           # 24 B in, 72 B out per peak
+           "algorithmic_hbm_gb_s": 96.0 * N / (ms * 1e-3) / 1e9}
+    if with_cpu:
+        from oracle import oracle
+        ref = oracle.load_reference()
+        kind = "reference"
+        if ref is None:
+            ref, kind = oracle.load_oracle(), "port"
+        xl_h = np.zeros((N, 3))
+        out_h = np.zeros((N, 6))
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ref.compute_xlylzl(sc, fc, p, r, dist, xl_h)
+            ref.compute_geometry(xl_h, om, sign, wvln, wedge, chi, t, out_h)
+            runs.append(time.perf_counter() - t0)
+        cpu_s = statistics.median(runs)
+        got_xl, got = xl.cpu().numpy(), out.cpu().numpy()
+        res["cpu"] = {"kind": kind, "threads": threads, "seconds": cpu_s,
+                      "sample": "all %d peaks, median of 3" % N}
+        res["speedup_vs_cpu"] = cpu_s / (ms * 1e-3)
+        # xl bit-exact; ds and g within device-vs-glibc sin/cos rounding; tth,
+        # eta within the two libraries' atan2
+        res["xl_bit_exact"] = bool(np.array_equal(got_xl, xl_h))
+        res["max_rel_diff_ds_g"] = float(np.max(np.abs(got[:, 2:] - out_h[:, 2:]) /
+                                                np.maximum(np.abs(out_h[:, 2:]), 1e-300)))
+        d = np.abs(got[:, :2] - out_h[:, :2])
+        d[:, 1] = np.minimum(d[:, 1], 360.0 - d[:, 1])  # eta: across the +-180 cut
+        res["max_abs_diff_tth_eta_deg"] = float(d.max())
+    return res
+
+
 def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
     """Large scan: compute_gv per grain translation + score_and_assign for
     K grains x N peaks (per-peak dty, 15% spurious) in one fused launch,
@@ -645,7 +709,8 @@ def main():
                       "scan_20000_grains": aux_scan(threads, not args.no_cpu),
                       "index_3000_grains": aux_index(not args.no_cpu),
                       "refinegrains_2000_grains": aux_refinegrains(not args.no_cpu),
-                      "pbp_refine_map_101x101": aux_pbp(not args.no_cpu)}
+                      "pbp_refine_map_101x101": aux_pbp(not args.no_cpu),
+                      "geometry_8e6_peaks": aux_geometry(threads, not args.no_cpu)}
         out["refine_plus_adjoint_vs_per_grain_gpu_loop"] = \
             out["aux"]["refine_5000_grains"]["speedup_refine_plus_adjoint_vs_gpu_loop"]
     print(json.dumps(out))
