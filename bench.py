@@ -558,8 +558,12 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
     torch.cuda.synchronize()
     assign_ms = a.elapsed_time(b)
     gvl, idx, off = drivers.grain_gvectors(scan, trans, labels, K)
+    # the captured iterations in the autotuner's launch shape for these lists
+    best = refine.autotune(ubis, gvl, None, off, 0.05, n_iter, reps=2)
     work = ubis.clone()
-    g = refine.RefineGraph(work, gvl, None, off, 0.05, n_iter)
+    g = refine.RefineGraph(work, gvl, None, off, 0.05, n_iter,
+                           lanes_per_grain=best["lanes_per_grain"],
+                           waves_per_cu=best["waves_per_cu"])
     refine_ms = _events_ms(lambda: (work.copy_(ubis), g.replay()), reps=3)
     own = pk["grain"]
     lab = labels.cpu().numpy()
@@ -576,7 +580,9 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
            "assigned": int((lab >= 0).sum()),
            "own_label_fraction": float(np.mean(lab[own >= 0] == own[own >= 0])),
            "refine_iterations": n_iter, "refine_list_entries": int(gvl.shape[0]),
-           "refine_graph_ms": refine_ms}
+           "refine_graph_ms": refine_ms,
+           "refine_graph_shape": {"lanes_per_grain": best["lanes_per_grain"],
+                                  "waves_per_cu": best["waves_per_cu"]}}
     if with_cpu:
         from oracle import oracle
         ref = oracle.load_reference()
