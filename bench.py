@@ -670,6 +670,11 @@ def main():
     value = pairs * args.steps / (total_ms * 1e-3)
     achieved = FLOPS_PER_PAIR * pairs / (score_ms * 1e-3)
     gpu_counts = counts.cpu().numpy()
+    # boundary-peak audit of the device-sincos g-vectors (outside the timed
+    # region): pairs within 1e-12 relative of tol^2, where a last-bit
+    # difference from the host libm's sin/cos could flip a count
+    boundary = int(batched.boundary_pairs(ubis, batched.sf2gv(sc, fc, om, pars),
+                                          args.tol).sum())
 
     out = {
         "metric": "grain\u00d7peak pairs/s scored; batched+adjoint refine vs per-grain loop, "
@@ -688,6 +693,7 @@ def main():
         "config": {"workload": "raw peaks -> compute_xlylzl -> compute_gv -> score, "
                                "%d grains x %d peaks, tol %g" % (args.grains, args.peaks, args.tol),
                    "grains": args.grains, "peaks": args.peaks, "tol": args.tol,
+                   "boundary_pairs_rel_1e-12": boundary,
                    "value_is": "the pairs/s of this workload; the batched+adjoint refine "
                                "against the per-grain GPU loop is "
                                "aux.refine_5000_grains.speedup_refine_plus_adjoint_vs_gpu_loop"},

@@ -53,6 +53,19 @@ def score_many(ubis, gv, tol, peaks_per_lane=0, out=None, exact=False, mode=None
     return out
 
 
+def boundary_pairs(ubis, gv, tol, rel=1e-12):
+    """The boundary-peak audit of g-vectors computed with the device's
+    sin/cos: per grain, the number of peaks with tol^2 (1 - rel) <= drlv2 <
+    tol^2 (1 + rel), the band in which a last-bit difference between the
+    device's and the host libm's sin/cos (hence in g) could move a peak
+    across the tolerance sphere.  Zero everywhere means the counts (and
+    labels) cannot depend on which sin/cos made the g-vectors.  Two
+    score_many launches: count(drlv2 < tol_hi^2) - count(drlv2 < tol_lo^2)."""
+    hi = score_many(ubis, gv, float(tol) * float(np.sqrt(1.0 + rel)))
+    lo = score_many(ubis, gv, float(tol) * float(np.sqrt(1.0 - rel)))
+    return hi - lo
+
+
 def assign_many(ubis, gv, tol, drlv2, labels, grain_labels=None, out=None, exact=False):
     """score_and_assign(ubis[g], gv, tol, drlv2, labels, grain_labels[g]) for
     g = 0..K-1 in order; drlv2 (N,) float64 and labels (N,) int32 are

@@ -360,3 +360,27 @@ def test_score_certified_boundary(dev, orc, tol, hmax):
                 c = batched.assign_many(u, g, tol, dd, ll, exact=ex)
                 assert int(c[0]) == n_o and np.array_equal(ll.cpu().numpy(), lab)
                 assert np.array_equal(dd.cpu().numpy().view(np.int64), d.view(np.int64))
+
+
+def test_boundary_pairs_audit(dev, orc):
+    """The boundary-peak audit counts exactly the pairs whose reference
+    drlv2 lies in [tol^2 (1 - rel), tol^2 (1 + rel)), here planted on
+    purpose; on ordinary data the band is empty."""
+    ubi, gv, _ = simulate.lattice_gvectors(seed=61, n_lattice=3000, n_random=2000)
+    tol = 0.05
+    ub = np.linalg.inv(ubi)
+    rng = np.random.default_rng(62)
+    planted = []
+    for k in range(40):  # hkl + a unit offset scaled to drlv2 = tol^2 (1 + tiny)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        h = rng.integers(-5, 6, 3) + tol * (1 + rng.uniform(-5e-13, 5e-13)) * d
+        planted.append(ub @ h)
+    g = np.vstack([gv, np.array(planted)])
+    got = batched.boundary_pairs(T(ubi[None], dev), T(g, dev), tol).cpu().numpy()
+    # the same band from the oracle's reference arithmetic
+    hi = orc.score(ubi, g, tol * np.sqrt(1 + 1e-12))
+    lo = orc.score(ubi, g, tol * np.sqrt(1 - 1e-12))
+    assert got[0] == hi - lo
+    assert got[0] >= 30  # nearly all planted pairs land in the band
+    assert batched.boundary_pairs(T(ubi[None], dev), T(gv, dev), tol).cpu().numpy()[0] == 0
