@@ -182,12 +182,14 @@ def aux_refine_adjoint(threads, with_cpu):
     if best["mode"] == "graph":
         g = refine.RefineGraph(work, gv, idx_d, off_d, tol, n_iter, order=order,
                                lanes_per_grain=best["lanes_per_grain"],
-                               waves_per_cu=best["waves_per_cu"])
+                               waves_per_cu=best["waves_per_cu"],
+                               long_min=best.get("long_min", 0))
         run = lambda: (work.copy_(ubis0), g.replay())
     else:
         run = lambda: batched.refine_lists(ubis0, gv, idx_d, off_d, tol, n_iter, order=order,
                                            lanes_per_grain=best["lanes_per_grain"],
-                                           waves_per_cu=best["waves_per_cu"])
+                                           waves_per_cu=best["waves_per_cu"],
+                                           long_min=best.get("long_min", 0))
     batched_ms = _events_ms(run, reps=5)
     # per-grain drop-in loop on the same GPU: lists gathered once, then the
     # reference's call pattern (n_iter score_and_refine calls per grain)
@@ -562,7 +564,8 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
     work = ubis.clone()
     g = refine.RefineGraph(work, gvl, None, off, 0.05, n_iter,
                            lanes_per_grain=best["lanes_per_grain"],
-                           waves_per_cu=best["waves_per_cu"])
+                           waves_per_cu=best["waves_per_cu"],
+                           long_min=best.get("long_min", 0))
     refine_ms = _events_ms(lambda: (work.copy_(ubis), g.replay()), reps=3)
     own = pk["grain"]
     lab = labels.cpu().numpy()
@@ -581,7 +584,8 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
            "refine_iterations": n_iter, "refine_list_entries": int(gvl.shape[0]),
            "refine_graph_ms": refine_ms,
            "refine_graph_shape": {"lanes_per_grain": best["lanes_per_grain"],
-                                  "waves_per_cu": best["waves_per_cu"]}}
+                                  "waves_per_cu": best["waves_per_cu"],
+                                  "long_min": best.get("long_min", 0)}}
     if with_cpu:
         from oracle import oracle
         ref = oracle.load_reference()

@@ -507,15 +507,86 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gather_rows(const double* __restri
 }
 
 // ---------------------------------------------------------------------------
+// One work item of k_refine_lists: the grains in slots base .. base +
+// 64/L - 1 of `order`, L lanes each, every grain's n_iter iterations.
+template <int L, bool ROWS>
+__device__ __forceinline__ void gs_refine_item(
+    int base, int lane, double* __restrict__ ubis, int K, const double* __restrict__ gv, int ng,
+    const int* __restrict__ idx, long long n_idx, const long long* __restrict__ off,
+    const int* __restrict__ order, double tolsq, int n_iter, int* __restrict__ n_out,
+    double* __restrict__ mean_out) {
+    const int sub = lane / L;
+    const int sl = lane - sub * L;
+    const int slot = base + sub;
+    const int gq = slot < K ? order[slot] : -1;
+    const bool active = gq >= 0 && gq < K;
+    const int g = active ? gq : 0;
+    long long o1 = active ? off[g + 1] : 0;
+    o1 = o1 < 0 ? 0 : (o1 > n_idx ? n_idx : o1);
+    long long o0 = active ? off[g] : 0;
+    o0 = o0 < 0 ? 0 : (o0 > o1 ? o1 : o0);
+    double u[9];
+    for (int i = 0; i < 9; i++) u[i] = active ? ubis[9 * (size_t)g + i] : 0.0;
+    for (int it = 0; it < n_iter; it++) {
+        const gs_ubi U = {u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8]};
+        double a[GS_NPART];
+#pragma unroll
+        for (int i = 0; i < GS_NPART; i++) a[i] = 0.0;
+        if (ROWS) {
+            // GS_REFINE_BATCH rows per lane in flight.  The row is
+            // clamped to the list's last one (in bounds: o1 <= n_idx <=
+            // ng), so the loads need no guard; the copies past the end
+            // are dropped.  The lane still adds its rows in list order.
+            for (long long q = o0 + sl; q < o1; q += (long long)GS_REFINE_BATCH * L) {
+                double gg[GS_REFINE_BATCH][3];
+#pragma unroll
+                for (int j = 0; j < GS_REFINE_BATCH; j++) {
+                    long long r = q + (long long)j * L;
+                    r = r < o1 ? r : o1 - 1;
+                    gg[j][0] = gv[3 * r + 0];
+                    gg[j][1] = gv[3 * r + 1];
+                    gg[j][2] = gv[3 * r + 2];
+                }
+#pragma unroll
+                for (int j = 0; j < GS_REFINE_BATCH; j++)
+                    if (q + (long long)j * L < o1)
+                        gs_refine_add(U, gg[j][0], gg[j][1], gg[j][2], tolsq, a);
+            }
+        } else {
+            for (long long q = o0 + sl; q < o1; q += L) {
+                const long long k = idx[q];
+                if (k < 0 || k >= ng) continue;
+                gs_refine_add(U, gv[3 * k + 0], gv[3 * k + 1], gv[3 * k + 2], tolsq, a);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < GS_NPART; i++) a[i] = gs_group_sum<L>(a[i]);
+        double R[9], H[9];
+        gs_unpack_RH(a, R, H);
+        gs_solve_ubi(u, R, H);
+        if (active && sl == 0) {
+            const int n = (int)a[0];
+            n_out[(size_t)it * K + g] = n;
+            mean_out[(size_t)it * K + g] = n > 0 ? a[1] / n : a[1];
+        }
+    }
+    if (active && sl == 0)
+        for (int i = 0; i < 9; i++) ubis[9 * (size_t)g + i] = u[i];
+}
+
 // Batched score_and_refine over CSR peak lists: a persistent grid whose
-// waves pull groups of 64/LPG grains from a work queue (grains visited in
-// `order`, longest list first), LPG lanes per grain.  Each grain runs
-// n_iter successive score_and_refine calls on its own list (the calls of
-// different grains are independent).  Sums are reduced across the LPG
-// lanes by gs_group_sum, which leaves bit-identical totals in every lane, so
-// each lane solves the 3x3 system itself and keeps the updated UBI in
-// registers.
-// n_out/mean_out are [n_iter][K]; queue[0] must be zero at launch.
+// waves pull work items from a queue: items [0, nlong) are the grains in
+// slots 0 .. nlong-1 of `order`, one per wave (64 lanes), the others groups
+// of 64/LPG grains (LPG lanes each) from slot nlong on.  With `order`
+// longest list first and nlong the number of lists of at least a given
+// length (k_count_long), the longest lists get a whole wave, so they no
+// longer set the makespan.  Each grain runs n_iter successive
+// score_and_refine calls on its own list (the calls of different grains
+// are independent).  Sums are reduced across the grain's lanes by
+// gs_group_sum, which leaves bit-identical totals in every lane, so each
+// lane solves the 3x3 system itself and keeps the updated UBI in
+// registers.  n_out/mean_out are [n_iter][K]; queue[0] must be zero at
+// launch; nlong_p NULL means nlong = 0.
 // Out-of-range entries are ignored rather than trusted: a grain number
 // outside [0, K) in `order` is skipped, list bounds are clipped to
 // [0, n_idx], and peak indices outside [0, ng) are not kept.
@@ -527,72 +598,45 @@ __global__ __launch_bounds__(GS_BLOCK) void k_refine_lists(
     double* __restrict__ ubis, int K, const double* __restrict__ gv, int ng,
     const int* __restrict__ idx, long long n_idx, const long long* __restrict__ off,
     const int* __restrict__ order, double tolsq, int n_iter, int* __restrict__ n_out,
-    double* __restrict__ mean_out, int* __restrict__ queue) {
+    double* __restrict__ mean_out, const int* __restrict__ nlong_p, int* __restrict__ queue) {
     constexpr int GPW = GS_WAVE / LPG;
     const int lane = threadIdx.x & (GS_WAVE - 1);
-    const int sub = lane / LPG;
-    const int sl = lane - sub * LPG;
+    int nlong = nlong_p ? *nlong_p : 0;
+    nlong = nlong < 0 ? 0 : (nlong > K ? K : nlong);
+    const int nitems = nlong + (K - nlong + GPW - 1) / GPW;
     for (;;) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(queue, GPW);
-        base = __shfl(base, 0, GS_WAVE);
-        if (base >= K) break;
-        const int slot = base + sub;
-        const int gq = slot < K ? order[slot] : -1;
-        const bool active = gq >= 0 && gq < K;
-        const int g = active ? gq : 0;
-        long long o1 = active ? off[g + 1] : 0;
-        o1 = o1 < 0 ? 0 : (o1 > n_idx ? n_idx : o1);
-        long long o0 = active ? off[g] : 0;
-        o0 = o0 < 0 ? 0 : (o0 > o1 ? o1 : o0);
-        double u[9];
-        for (int i = 0; i < 9; i++) u[i] = active ? ubis[9 * (size_t)g + i] : 0.0;
-        for (int it = 0; it < n_iter; it++) {
-            const gs_ubi U = {u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8]};
-            double a[GS_NPART];
-#pragma unroll
-            for (int i = 0; i < GS_NPART; i++) a[i] = 0.0;
-            if (ROWS) {
-                // GS_REFINE_BATCH rows per lane in flight.  The row is
-                // clamped to the list's last one (in bounds: o1 <= n_idx <=
-                // ng), so the loads need no guard; the copies past the end
-                // are dropped.  The lane still adds its rows in list order.
-                for (long long q = o0 + sl; q < o1; q += (long long)GS_REFINE_BATCH * LPG) {
-                    double gg[GS_REFINE_BATCH][3];
-#pragma unroll
-                    for (int j = 0; j < GS_REFINE_BATCH; j++) {
-                        long long r = q + (long long)j * LPG;
-                        r = r < o1 ? r : o1 - 1;
-                        gg[j][0] = gv[3 * r + 0];
-                        gg[j][1] = gv[3 * r + 1];
-                        gg[j][2] = gv[3 * r + 2];
-                    }
-#pragma unroll
-                    for (int j = 0; j < GS_REFINE_BATCH; j++)
-                        if (q + (long long)j * LPG < o1)
-                            gs_refine_add(U, gg[j][0], gg[j][1], gg[j][2], tolsq, a);
-                }
-            } else {
-                for (long long q = o0 + sl; q < o1; q += LPG) {
-                    const long long k = idx[q];
-                    if (k < 0 || k >= ng) continue;
-                    gs_refine_add(U, gv[3 * k + 0], gv[3 * k + 1], gv[3 * k + 2], tolsq, a);
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < GS_NPART; i++) a[i] = gs_group_sum<LPG>(a[i]);
-            double R[9], H[9];
-            gs_unpack_RH(a, R, H);
-            gs_solve_ubi(u, R, H);
-            if (active && sl == 0) {
-                const int n = (int)a[0];
-                n_out[(size_t)it * K + g] = n;
-                mean_out[(size_t)it * K + g] = n > 0 ? a[1] / n : a[1];
-            }
-        }
-        if (active && sl == 0)
-            for (int i = 0; i < 9; i++) ubis[9 * (size_t)g + i] = u[i];
+        int item = 0;
+        if (lane == 0) item = atomicAdd(queue, 1);
+        item = __shfl(item, 0, GS_WAVE);
+        if (item >= nitems) break;
+        if (item < nlong)
+            gs_refine_item<GS_WAVE, ROWS>(item, lane, ubis, K, gv, ng, idx, n_idx, off, order,
+                                          tolsq, n_iter, n_out, mean_out);
+        else
+            gs_refine_item<LPG, ROWS>(nlong + (item - nlong) * GPW, lane, ubis, K, gv, ng, idx,
+                                      n_idx, off, order, tolsq, n_iter, n_out, mean_out);
     }
+}
+
+// nlong[0] = the number of grains whose clipped list length is at least
+// long_min (one block; the count does not depend on the grain order).
+__global__ __launch_bounds__(GS_BLOCK) void k_count_long(const long long* __restrict__ off, int K,
+                                                         long long n_idx, long long long_min,
+                                                         int* __restrict__ nlong) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int c = 0;
+    for (int g = threadIdx.x; g < K; g += GS_BLOCK) {
+        long long o1 = off[g + 1];
+        o1 = o1 < 0 ? 0 : (o1 > n_idx ? n_idx : o1);
+        long long o0 = off[g];
+        o0 = o0 < 0 ? 0 : (o0 > o1 ? o1 : o0);
+        c += (o1 - o0) >= long_min;
+    }
+    atomicAdd(&cnt, c);
+    __syncthreads();
+    if (threadIdx.x == 0) nlong[0] = cnt;
 }
 
 // ---------------------------------------------------------------------------

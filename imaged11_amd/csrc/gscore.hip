@@ -433,7 +433,7 @@ int check_refine_args(int K, int ng, const int* idx, long long n_idx, int n_iter
 int launch_refine_lists(DeviceCtx* ctx, double* ubis, int K, const double* gv, int ng,
                         const int* idx, long long n_idx, const long long* off, const int* order,
                         double tol, int n_iter, int* n_out, double* mean_out, int lanes_per_grain,
-                        int waves_per_cu, int* queue, hipStream_t st) {
+                        int waves_per_cu, const int* nlong, int* queue, hipStream_t st) {
     const double tolsq = tol * tol;
     int lpg = lanes_per_grain;
     if (lpg != 8 && lpg != 16 && lpg != 32 && lpg != 64) lpg = 16;
@@ -445,7 +445,7 @@ int launch_refine_lists(DeviceCtx* ctx, double* ubis, int K, const double* gv, i
     const dim3 grid(nb), block(GS_BLOCK);
 #define GS_REFINE_LAUNCH(L, R)                                                                  \
     hipLaunchKernelGGL((k_refine_lists<L, R>), grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, \
-                       order, tolsq, n_iter, n_out, mean_out, queue)
+                       order, tolsq, n_iter, n_out, mean_out, nlong, queue)
     const bool rows = idx == nullptr;  // entry q is row q of gv
     switch (lpg) {
         case 8: if (rows) GS_REFINE_LAUNCH(8, true); else GS_REFINE_LAUNCH(8, false); break;
@@ -919,7 +919,39 @@ int gs_refine_lists(double* ubis, int K, const double* gv, int ng, const int* id
     int* queue = queue_ws ? queue_ws : ctx->counters;
     GS_CHECK(fill_i32(queue, 0, 1, st));
     return launch_refine_lists(ctx, ubis, K, gv, ng, idx, n_idx, off, order, tol, n_iter, n_out,
-                               mean_out, lanes_per_grain, waves_per_cu, queue, st);
+                               mean_out, lanes_per_grain, waves_per_cu, nullptr, queue, st);
+}
+
+int launch_count_long(const long long* off, int K, long long n_idx, long long long_min,
+                      int* nlong, hipStream_t st) {
+    hipLaunchKernelGGL(k_count_long, dim3(1), dim3(GS_BLOCK), 0, st, off, K, n_idx, long_min,
+                       nlong);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_refine_lists_long(double* ubis, int K, const double* gv, int ng, const int* idx,
+                         long long n_idx, const long long* off, const int* order, double tol,
+                         int n_iter, int* n_out, double* mean_out, int lanes_per_grain,
+                         int waves_per_cu, long long long_min, int* queue_ws,
+                         gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_refine_args(K, ng, idx, n_idx, n_iter));
+    if (K == 0 || n_iter == 0) return GS_OK;
+    hipStream_t st = S(stream);
+    // queue_ws[0] the work-queue counter, queue_ws[1] nlong (the library's
+    // per-device counters 0 and 2 without a workspace)
+    int* queue = queue_ws ? queue_ws : ctx->counters;
+    int* nlong_ws = queue_ws ? queue_ws + 1 : ctx->counters + 2;
+    GS_CHECK(fill_i32(queue, 0, 1, st));
+    const int* nlong = nullptr;
+    if (long_min > 0) {
+        GS_CHECK(launch_count_long(off, K, n_idx, long_min, nlong_ws, st));
+        nlong = nlong_ws;
+    }
+    return launch_refine_lists(ctx, ubis, K, gv, ng, idx, n_idx, off, order, tol, n_iter, n_out,
+                               mean_out, lanes_per_grain, waves_per_cu, nlong, queue, st);
 }
 
 int gs_gather_rows(const double* gv, int ng, const int* idx, long long n_idx, double* rows,
@@ -943,11 +975,11 @@ struct gs_graph_s {
     hipGraphExec_t exec = nullptr;
 };
 
-int gs_refine_graph_create(gs_graph_t* out, double* ubis, int K, const double* gv, int ng,
-                           const int* idx, long long n_idx, const long long* off, const int* order,
-                           double tol, int n_iter, int* n_out, double* mean_out,
-                           int lanes_per_grain, int waves_per_cu, int* queue_ws,
-                           gs_stream_t stream) {
+int gs_refine_graph_create_long(gs_graph_t* out, double* ubis, int K, const double* gv, int ng,
+                                const int* idx, long long n_idx, const long long* off,
+                                const int* order, double tol, int n_iter, int* n_out,
+                                double* mean_out, int lanes_per_grain, int waves_per_cu,
+                                long long long_min, int* queue_ws, gs_stream_t stream) {
     if (!out) return set_err(GS_ERR_ARG, "gs_refine_graph_create: null output handle");
     *out = nullptr;
     if (!queue_ws) return set_err(GS_ERR_ARG, "gs_refine_graph_create: queue_ws (n_iter ints) needed");
@@ -958,11 +990,17 @@ int gs_refine_graph_create(gs_graph_t* out, double* ubis, int K, const double* g
     hipStream_t st = S(stream);
     if (st == nullptr) return set_err(GS_ERR_ARG, "gs_refine_graph_create: capture needs a non-null stream");
     GS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    // queue_ws[0..n_iter) the per-iteration counters, queue_ws[n_iter] nlong
     int rc = fill_i32(queue_ws, 0, n_iter, st);
+    const int* nlong = nullptr;
+    if (rc == GS_OK && long_min > 0) {
+        rc = launch_count_long(off, K, n_idx, long_min, queue_ws + n_iter, st);
+        nlong = queue_ws + n_iter;
+    }
     for (int it = 0; rc == GS_OK && it < n_iter; it++)
         rc = launch_refine_lists(ctx, ubis, K, gv, ng, idx, n_idx, off, order, tol, 1,
                                  n_out + (size_t)it * K, mean_out + (size_t)it * K,
-                                 lanes_per_grain, waves_per_cu, queue_ws + it, st);
+                                 lanes_per_grain, waves_per_cu, nlong, queue_ws + it, st);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (rc != GS_OK) {
@@ -981,6 +1019,16 @@ int gs_refine_graph_create(gs_graph_t* out, double* ubis, int K, const double* g
     g->exec = exec;
     *out = g;
     return GS_OK;
+}
+
+int gs_refine_graph_create(gs_graph_t* out, double* ubis, int K, const double* gv, int ng,
+                           const int* idx, long long n_idx, const long long* off, const int* order,
+                           double tol, int n_iter, int* n_out, double* mean_out,
+                           int lanes_per_grain, int waves_per_cu, int* queue_ws,
+                           gs_stream_t stream) {
+    return gs_refine_graph_create_long(out, ubis, K, gv, ng, idx, n_idx, off, order, tol, n_iter,
+                                       n_out, mean_out, lanes_per_grain, waves_per_cu, 0,
+                                       queue_ws, stream);
 }
 
 int gs_graph_launch(gs_graph_t g, gs_stream_t stream) {

@@ -44,7 +44,7 @@ class RefineGraph:
     of gv is list entry q."""
 
     def __init__(self, ubis, gv, idx, off, tol, n_iter, order=None, lanes_per_grain=0,
-                 waves_per_cu=0):
+                 waves_per_cu=0, long_min=0):
         batched._need(ubis, torch.float64, "ubis", (None, 3, 3))
         batched._need(gv, torch.float64, "gv", (None, 3))
         K = ubis.shape[0]
@@ -64,17 +64,17 @@ class RefineGraph:
         dev = gv.device
         self.n = torch.zeros((n_iter, K), dtype=torch.int32, device=dev)
         self.mean = torch.zeros((n_iter, K), dtype=torch.float64, device=dev)
-        self.queue = torch.zeros(n_iter, dtype=torch.int32, device=dev)
+        self.queue = torch.zeros(n_iter + 1, dtype=torch.int32, device=dev)
         self.stream = torch.cuda.Stream(device=dev)
         lib = _lib.load()
         lib.gs_clear_error()
         handle = ctypes.c_void_p()
         torch.cuda.current_stream(dev).synchronize()
-        _lib.check(lib.gs_refine_graph_create(
+        _lib.check(lib.gs_refine_graph_create_long(
             ctypes.byref(handle), dptr(ubis), K, dptr(rows), rows.shape[0],
             None if ridx is None else dptr(ridx), n_idx,
             dptr(off), dptr(order), float(tol), int(n_iter), dptr(self.n), dptr(self.mean),
-            int(lanes_per_grain), int(waves_per_cu), dptr(self.queue),
+            int(lanes_per_grain), int(waves_per_cu), int(long_min), dptr(self.queue),
             stream_ptr(self.stream)), "refine graph capture")
         self.handle = handle
 
@@ -129,8 +129,10 @@ def _time(fn, reps):
 
 
 def autotune(ubis, gv, idx, off, tol, n_iter, reps=5, use_cache=True, verbose=False):
-    """Pick (mode, lanes_per_grain, waves_per_cu) for this problem by
-    timing every candidate with HIP events; results are cached on disk by
+    """Pick (mode, lanes_per_grain, waves_per_cu, long_min) for this
+    problem by timing every candidate with HIP events (long_min: 0, or the
+    90th / 99th percentile of the list lengths: the grains with lists at
+    least that long get a whole wave); results are cached on disk by
     `problem_key`.  Returns the best config dict (with its time in ms)."""
     key = problem_key(off, n_iter)
     path = _cache_path()
@@ -144,29 +146,35 @@ def autotune(ubis, gv, idx, off, tol, n_iter, reps=5, use_cache=True, verbose=Fa
             return cache[key]
     order = batched.list_order(off)
     work = ubis.clone()
+    lengths = (off[1:] - off[:-1]).double()
+    q = torch.quantile(lengths, torch.tensor([0.9, 0.99], dtype=torch.float64,
+                                             device=lengths.device)).cpu().numpy()
+    longs = sorted({0} | {int(round(v)) for v in q if v >= 1})
     results = []
     for lanes in LANES:
         for wpc in WAVES_PER_CU:
-            def fused():
-                work.copy_(ubis)
-                batched.refine_lists(work, gv, idx, off, tol, n_iter, order=order,
-                                     lanes_per_grain=lanes, waves_per_cu=wpc, inplace=True)
-            results.append(({"mode": "fused", "lanes_per_grain": lanes, "waves_per_cu": wpc},
-                            _time(fused, reps)))
-            g = RefineGraph(work, gv, idx, off, tol, n_iter, order=order, lanes_per_grain=lanes,
-                            waves_per_cu=wpc)
+            for lm in (longs if lanes < 64 else [0]):
+                cfg = {"lanes_per_grain": lanes, "waves_per_cu": wpc, "long_min": lm}
 
-            def graph():
-                work.copy_(ubis)
-                g.replay()
-            results.append(({"mode": "graph", "lanes_per_grain": lanes, "waves_per_cu": wpc},
-                             _time(graph, reps)))
-            g.close()
+                def fused():
+                    work.copy_(ubis)
+                    batched.refine_lists(work, gv, idx, off, tol, n_iter, order=order,
+                                         lanes_per_grain=lanes, waves_per_cu=wpc,
+                                         inplace=True, long_min=lm)
+                results.append((dict(cfg, mode="fused"), _time(fused, reps)))
+                g = RefineGraph(work, gv, idx, off, tol, n_iter, order=order,
+                                lanes_per_grain=lanes, waves_per_cu=wpc, long_min=lm)
+
+                def graph():
+                    work.copy_(ubis)
+                    g.replay()
+                results.append((dict(cfg, mode="graph"), _time(graph, reps)))
+                g.close()
     results.sort(key=lambda r: r[1])
     if verbose:
         for cfg, ms in results:
-            print("%-6s lanes %2d waves/CU %2d : %.3f ms" % (cfg["mode"], cfg["lanes_per_grain"],
-                                                             cfg["waves_per_cu"], ms))
+            print("%-6s lanes %2d waves/CU %2d long_min %5d : %.3f ms" % (
+                cfg["mode"], cfg["lanes_per_grain"], cfg["waves_per_cu"], cfg["long_min"], ms))
     best = dict(results[0][0], ms=results[0][1], key=key, tuned_at=time.time())
     cache[key] = best
     if use_cache:
@@ -188,7 +196,8 @@ def refine(ubis, gv, idx, off, tol, n_iter=2, config=None):
     if config.get("mode", "fused") == "graph":
         u = ubis.clone()
         g = RefineGraph(u, gv, idx, off, tol, n_iter, lanes_per_grain=config["lanes_per_grain"],
-                        waves_per_cu=config["waves_per_cu"])
+                        waves_per_cu=config["waves_per_cu"],
+                        long_min=config.get("long_min", 0))
         g.replay()
         n, m = g.n.clone(), g.mean.clone()
         torch.cuda.current_stream().synchronize()
@@ -196,4 +205,5 @@ def refine(ubis, gv, idx, off, tol, n_iter=2, config=None):
         return u, n, m
     return batched.refine_lists(ubis, gv, idx, off, tol, n_iter,
                                 lanes_per_grain=config["lanes_per_grain"],
-                                waves_per_cu=config["waves_per_cu"])
+                                waves_per_cu=config["waves_per_cu"],
+                                long_min=config.get("long_min", 0))

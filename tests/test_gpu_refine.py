@@ -193,3 +193,32 @@ def test_empty_lists_rows_mode(dev):
         assert torch.equal(u, ubis)
         assert int(n.abs().sum()) == 0 and float(m.abs().sum()) == 0.0
     assert batched.gather_rows(gv, idx).shape == (0, 3)
+
+
+@pytest.mark.parametrize("lanes", [16, 32])
+def test_long_lists_get_a_wave(dev, orc, problem, lanes):
+    """long_min: the grains with the longest lists refined by a whole wave,
+    the rest at `lanes` lanes per grain; against the oracle's per-grain
+    loop like every other launch shape, graph and fused alike, and the
+    count of long grains is that of the lengths."""
+    gv, idx, off = _dev(problem, dev)
+    lengths = np.diff(problem["off"])
+    long_min = int(np.quantile(lengths, 0.8))
+    n_iter = 4
+    u_o, n_o, m_o = orc.refine_lists(problem["start"], problem["gv"], problem["idx"],
+                                     problem["off"], 0.05, n_iter)
+    uf, nf, mf = batched.refine_lists(T(problem["start"], dev), gv, idx, off, 0.05, n_iter,
+                                      lanes_per_grain=lanes, long_min=long_min)
+    work = T(problem["start"], dev)
+    g = refine.RefineGraph(work, gv, idx, off, 0.05, n_iter, lanes_per_grain=lanes,
+                           long_min=long_min)
+    g.replay()
+    torch.cuda.synchronize()
+    assert int(g.queue[n_iter]) == int((lengths >= long_min).sum()) > 0
+    assert torch.equal(g.n, nf) and torch.equal(g.mean, mf) and torch.equal(work, uf)
+    g.close()
+    assert np.array_equal(nf.cpu().numpy(), n_o)
+    m = mf.cpu().numpy()
+    assert np.all(np.abs(m[0] - m_o[0]) <= MEAN_RTOL * np.abs(m_o[0]))
+    assert np.all(np.abs(m - m_o) <= CHAIN_MEAN_RTOL * np.abs(m_o))
+    np.testing.assert_allclose(uf.cpu().numpy(), u_o, rtol=0, atol=UBI_TOL)
