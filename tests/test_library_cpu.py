@@ -244,3 +244,18 @@ def test_gauss_newton_ubis_closed_form():
         else:
             want = np.linalg.solve(M[k], A[k].T).T
             np.testing.assert_allclose(got[k], want, rtol=1e-11, atol=1e-12)
+
+
+def test_ctypes_signatures_match_the_header():
+    """Every ctypes declaration has as many arguments as the prototype in
+    include/gscore.h (a mismatch would only show when the call is made)."""
+    import re
+    src = re.sub(r"/\*.*?\*/", "", open(_lib.HEADER).read(), flags=re.S)
+    arity = {}
+    for m in re.finditer(r"\b(?:int|void|double|char)\s*\*?\s*(\w+)\s*\(([^;]*?)\)\s*;", src,
+                         re.S):
+        params = m.group(2).strip()
+        arity[m.group(1)] = 0 if params in ("", "void") else len(params.split(","))
+    for name, (_, args) in _lib.SIGNATURES.items():
+        assert name in arity, name
+        assert len(args) == arity[name], (name, len(args), arity[name])
