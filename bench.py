@@ -9,7 +9,8 @@ at 50 um / 150 mm, wavelength 0.2 A, omega 0-180 deg in 0.25 deg frames,
 timed region.  value = K*N*steps / elapsed.
 
 Printed: ONE JSON line, with a roofline object for the dominant kernel
-(k_score_many_fast, FP64 vector ALU bound) timed live with HIP events on its
+(k_score_many_sieve at tol <= 0.1, else k_score_many_fast; FP64 vector ALU
+bound) timed live with HIP events on its
 stream, and a cpu_baseline object: the reference C (oracle/_ref, compiled
 from its sources) running the same pipeline on the host cores, the grain
 loop spread over threads (ctypes releases the GIL), with its per-grain
@@ -35,6 +36,13 @@ FLOPS_PER_PAIR = 23      # h = UBI.g: 9 mul + 6 add; 3 sub; sumsq: 3 mul + 2 add
 # h + 1/2 with 3 FMA each (9), fract per row (3), sum f (f - 1) in 5, two
 # compares (2)
 VALU_OPS_PER_PAIR = 19
+# the sieved score (k_score_many_sieve<4>, the default for tol <= 0.1),
+# counted in its gfx950 assembly: every pair 9 VALU (row 0 + r by three
+# FMAs, v_fract_f64, one compare; two mbcnt, the queue address and the entry
+# tag), every pair passing row 0 another 28 (the 19 above plus 9 integer
+# ones for the entry decode and addresses)
+SIEVE_STAGE1_VALU = 9
+SIEVE_SETTLE_VALU = 28
 # FP64 VALU instructions per pair of the fused compute_gv + score_and_assign
 # kernel's certified loop, counted in its gfx950 assembly: 77 per grain step
 # of two peaks per lane (24 v_fmac, 18 v_fma, 15 v_mul, 10 v_add, 6
@@ -63,14 +71,12 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_score_many_fast__traffic_1000x1e6.txt")
-
-
-def measured_traffic():
-    """HBM bytes per k_score_many_fast launch on the bench workload, from
+def measured_traffic(kernel):
+    """HBM bytes per launch of the score kernel on the bench workload, from
     the committed rocprofv3 PMC summary (null when absent)."""
+    path = os.path.join(ROOT, "profiles", kernel + "__traffic_1000x1e6.txt")
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     except OSError:
@@ -672,6 +678,19 @@ def main():
     pairs = args.grains * args.peaks
     value = pairs * args.steps / (total_ms * 1e-3)
     achieved = FLOPS_PER_PAIR * pairs / (score_ms * 1e-3)
+    if batched.score_mode(args.tol) == _lib.SCORE_SIEVE:
+        kernel = "k_score_many_sieve"
+        g = batched.sf2gv(sc, fc, om, pars)
+        passed = 0.0
+        for k0 in range(0, min(args.grains, 64), 16):  # pairs passing row 0, 64 grains
+            h0 = ubis[k0:k0 + 16, 0, :] @ g.T
+            passed += float(((h0 - torch.round(h0)).abs() < args.tol).sum())
+            del h0
+        pass_frac = passed / (min(args.grains, 64) * args.peaks)
+        valu_ops = SIEVE_STAGE1_VALU + SIEVE_SETTLE_VALU * pass_frac
+        del g
+    else:
+        kernel, pass_frac, valu_ops = "k_score_many_fast", None, VALU_OPS_PER_PAIR
     gpu_counts = counts.cpu().numpy()
     # boundary-peak audit of the device-sincos g-vectors (outside the timed
     # region): pairs within 1e-12 relative of tol^2, where a last-bit
@@ -701,17 +720,18 @@ def main():
                                "against the per-grain GPU loop is "
                                "aux.refine_5000_grains.speedup_refine_plus_adjoint_vs_gpu_loop"},
         "roofline": {
-            "kernel": "k_score_many_fast",
+            "kernel": kernel,
             "bound": "valu",
             "achieved": achieved / 1e12,
             "peak": PEAK_FP64_VALU / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_VALU,
-            "traffic": measured_traffic(),
+            "traffic": measured_traffic(kernel),
             "kernel_ms": score_ms,
             "flops_per_pair": FLOPS_PER_PAIR,
-            "valu_ops_per_pair": VALU_OPS_PER_PAIR,
-            "issued_fp64_ops_frac_of_nonfma_peak": VALU_OPS_PER_PAIR * pairs / (score_ms * 1e-3)
+            "valu_ops_per_pair": valu_ops,
+            "row0_pass_fraction": pass_frac,
+            "issued_valu_ops_frac_of_nonfma_peak": valu_ops * pairs / (score_ms * 1e-3)
             / (PEAK_FP64_VALU / 2),
         },
         "cpu_baseline": None,

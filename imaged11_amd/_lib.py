@@ -112,6 +112,7 @@ SIGNATURES = {
 
 SCORE_CERTIFIED = 0
 SCORE_EXACT = 1
+SCORE_SIEVE = 2
 
 _lib = None
 

@@ -33,12 +33,23 @@ def _ready():
     return lib
 
 
+SIEVE_TOL_MAX = 0.1  # the one-row sieve pays while ~2 tol of the pairs pass row 0
+
+
+def score_mode(tol):
+    """The faster of the certified kernels for this tolerance: the sieve
+    (row 0 of every pair first) for |tol| <= SIEVE_TOL_MAX, else the
+    all-rows certified kernel.  Both give the reference's counts."""
+    return _lib.SCORE_SIEVE if abs(float(tol)) <= SIEVE_TOL_MAX else _lib.SCORE_CERTIFIED
+
+
 def score_many(ubis, gv, tol, peaks_per_lane=0, out=None, exact=False, mode=None):
     """counts[g] = score(ubis[g], gv, tol) for every grain g.
 
     ubis (K,3,3) float64, gv (N,3) float64 -> int32 (K,).  exact=True uses
-    the reference's arithmetic for every pair instead of the certified FMA
-    path; both return the reference's counts."""
+    the reference's arithmetic for every pair; by default the certified
+    kernel score_mode(tol) picks runs; every mode returns the reference's
+    counts."""
     lib = _ready()
     _need(ubis, torch.float64, "ubis", (None, 3, 3))
     _need(gv, torch.float64, "gv", (None, 3))
@@ -47,7 +58,7 @@ def score_many(ubis, gv, tol, peaks_per_lane=0, out=None, exact=False, mode=None
         out = torch.empty(K, dtype=torch.int32, device=gv.device)
     _need(out, torch.int32, "out", (K,))
     if mode is None:
-        mode = _lib.SCORE_EXACT if exact else _lib.SCORE_CERTIFIED
+        mode = _lib.SCORE_EXACT if exact else score_mode(tol)
     _lib.check(lib.gs_score_many(dptr(ubis), K, dptr(gv), gv.shape[0], float(tol), dptr(out),
                                  int(peaks_per_lane), mode, stream_ptr()), "score_many")
     return out

@@ -126,6 +126,33 @@ GS_HD __forceinline__ double gs_cert_window_fract(const gs_ubi& u, double gmax, 
     return 1e-15 * span + 3e-15 + 2e-15 * tolsq;
 }
 
+// Radius of the one-row sieve of k_score_many_sieve: a pair whose row 0,
+// evaluated as y = h0 + r by three FMAs (r entering the first), has
+// fract(y) >= 2r lies at least r - eps from every integer (eps = the error
+// of y and of fract), and then its reference drlv2 >= tol^2, so it is not
+// counted.  Equivalently, with d = y - r - round(y - r): |d| >= r.  Let t be
+// the reference's row-0 difference h0 - round(h0) (exact for |h0| < 2^51,
+// |t| = the distance of h0 to the nearest integer).  The reference's drlv2
+// is a sum of three non-negative rounded squares that starts from fl(t^2),
+// and rounding is monotone, so drlv2 >= t^2 (1 - u).  For r < 1/2 (else
+// 2r > fract(y) always and every pair passes) y is within gamma_3 (A_0 +
+// 1/2) of h0 + r and the reference's h0 within gamma_3 A_0 of the exact
+// dot product (A_0 = sum_j |u_0j||g_j| <= (|u00| + |u01| + |u02|) Gmax),
+// fract adds <= u, and the distance is 1-Lipschitz: ||d| - |t|| <= 6.7e-16
+// A_0 + 5e-16.  With r = tol (1 + 1e-15) + 1e-15 A_0 + 2e-15, |d| >= r
+// gives |t| >= tol (1 + 1e-15) and t^2 (1 - u) > tol^2; the margins also
+// absorb the rounding of
+// sqrt(tol^2) and of r itself.  +inf where gs_cert_window_fract is +inf
+// (every pair then reaches the exact recomputation).  A NaN d or r lets the
+// pair through, so non-finite inputs are decided by the full test.
+GS_HD __forceinline__ double gs_sieve_radius(const gs_ubi& u, double gmax, double tolsq) {
+    const double a0 = fabs(u.u00) + fabs(u.u01) + fabs(u.u02);
+    const double u1 = a0 + fabs(u.u10) + fabs(u.u11) + fabs(u.u12) + fabs(u.u20) + fabs(u.u21) +
+                      fabs(u.u22);
+    if (!(u1 * gmax < 562949953421312.0)) return __builtin_inf();  // 2^49
+    return sqrt(tolsq) * (1.0 + 1e-15) + 1e-15 * (a0 * gmax) + 2e-15;
+}
+
 GS_HD __forceinline__ double gs_abs_finite(double a) {
     return a == a ? fabs(a) : 0.0;  // NaN rows never count on either path
 }

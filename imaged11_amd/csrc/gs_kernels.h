@@ -15,6 +15,7 @@
 #define GS_BLOCK 256
 #define GS_GRAIN_CHUNK 1024  // grains per LDS counter tile
 #define GS_SPLIT_MAX 512     // grains per split of the all-pairs score/select tiles
+#define GS_SIEVE_SPLIT_MAX 128  // grains per split of k_score_many_sieve (UBI table in LDS)
 #define GS_NPART 17          // n, sum drlv2, R[9], H[6]
 #ifndef GS_REFINE_BATCH
 #define GS_REFINE_BATCH 8    // list rows per lane in flight in k_refine_lists (rows mode)
@@ -203,6 +204,137 @@ __global__ __launch_bounds__(GS_BLOCK, P <= 4 ? 8 : 1) void k_score_many_fast(
     __syncthreads();
     for (int i = threadIdx.x; i < g1 - g0; i += GS_BLOCK) {
         int c = lcnt[i];
+        if (c) atomicAdd(&counts[g0 + i], c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sieved score: the counts of k_score_many for every input, with most pairs
+// decided by one row.
+//
+// At tol = 0.05 a random pair has |h0 - round h0| < tol with probability
+// ~0.1, and any pair outside that band is not counted (gs_sieve_radius).
+// Stage one evaluates only row 0 for each pair of the grain step: y = h0 + r
+// by three FMAs (r enters the first), and the pair passes unless
+// fract(y) >= 2r, i.e. unless h0 is at least r from every integer (five
+// FP64 instructions; a NaN passes); the survivors (grain, peak) are
+// appended to a per-wave queue in LDS (ballot + mbcnt positions).  Whenever
+// the queue holds a wave's worth, 64 survivors are settled at once, one per
+// lane, by the full certified test of k_score_many_fast (its window in LDS,
+// the reference arithmetic for undecided pairs) with the UBI from an LDS
+// table and the g-vector re-read through the cache.  The queue is drained
+// from its top, so entries never move; capacity 64 (P + 1) covers the
+// at most 63 left over plus one grain step's 64 P.  Survivors are counted
+// with LDS atomics per grain (rare: only true hits), then one global atomic
+// per (block, grain).
+//
+// Queue entry: (grain offset in the split) << 9 | p << 6 | lane.
+__device__ __forceinline__ void gs_sieve_settle(int e, const double* __restrict__ gvc, long long nrem,
+                                                int wave, const double* lubi, const double* lwin,
+                                                double tolsq, int* lcnt) {
+    const int gl = e >> 9;
+    const long long k = (long long)((e >> 6) & 7) * GS_BLOCK + wave * GS_WAVE + (e & 63);
+    if (k >= nrem) return;  // NaN padding rows of the last chunk
+    const double x = gvc[3 * k + 0], y = gvc[3 * k + 1], z = gvc[3 * k + 2];
+    const double* ur = lubi + 9 * gl;
+    const gs_ubi u = {ur[0], ur[1], ur[2], ur[3], ur[4], ur[5], ur[6], ur[7], ur[8]};
+    const double s = gs_drlv2_fract(u, x, y, z);
+    bool in = s < lwin[2 * gl];
+    if (!in && s < lwin[2 * gl + 1]) in = gs_drlv2(u, x, y, z) < tolsq;  // undecided: exact
+    if (in) atomicAdd(&lcnt[gl], 1);
+}
+
+template <int P>
+__global__ __launch_bounds__(GS_BLOCK, 8) void k_score_many_sieve(
+    const double* __restrict__ ubis, int K, int gsplit, int nsplit,
+    const double* __restrict__ gv, int ng, double tolsq, int* __restrict__ counts) {
+    static_assert(P >= 1 && P <= 8, "the queue entry holds p in three bits");
+    constexpr int NW = GS_BLOCK / GS_WAVE;
+    constexpr int QCAP = GS_WAVE * (P + 1);
+    __shared__ int lcnt[GS_SIEVE_SPLIT_MAX];
+    __shared__ double lubi[9 * GS_SIEVE_SPLIT_MAX];
+    __shared__ double lwin[2 * GS_SIEVE_SPLIT_MAX];  // (lo, hi) per grain of the split
+    __shared__ double lrad[2 * GS_SIEVE_SPLIT_MAX];  // (r, 2r) per grain of the split
+    __shared__ double lgm[NW];
+    __shared__ int lq[NW * QCAP];
+    const int tile = gs_xcd_remap(blockIdx.x, gridDim.x);
+    const int chunk = tile / nsplit;
+    const int split = tile - chunk * nsplit;
+    const int g0 = split * gsplit;
+    const int ns = min(K, g0 + gsplit) - g0;  // grains of this split, <= GS_SIEVE_SPLIT_MAX
+
+    double x[P], y[P], z[P];
+    double gm = 0.0;
+    const long long c0 = (long long)chunk * (GS_BLOCK * P);  // first peak of the chunk
+    const long long nrem = (long long)ng - c0;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const long long k = (long long)p * GS_BLOCK + threadIdx.x;
+        if (k < nrem) {
+            x[p] = gv[3 * (c0 + k) + 0];
+            y[p] = gv[3 * (c0 + k) + 1];
+            z[p] = gv[3 * (c0 + k) + 2];
+        } else {
+            x[p] = y[p] = z[p] = __builtin_nan("");  // passes the sieve, skipped when settled
+        }
+        gm = fmax(gm, fmax(gs_abs_finite(x[p]), fmax(gs_abs_finite(y[p]), gs_abs_finite(z[p]))));
+    }
+    const int lane = threadIdx.x & (GS_WAVE - 1);
+    const int wave = threadIdx.x / GS_WAVE;
+    gm = gs_wave_max(gm);
+    if (lane == 0) lgm[wave] = gm;
+    __syncthreads();
+    double gmax = lgm[0];
+#pragma unroll
+    for (int w = 1; w < NW; w++) gmax = fmax(gmax, lgm[w]);
+    const double mid = tolsq - 0.75;  // gs_drlv2_fract is drlv2 - 3/4
+    for (int i = threadIdx.x; i < ns; i += GS_BLOCK) {
+        const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)(g0 + i));
+        const double e = gs_cert_window_fract(u, gmax, tolsq);
+        lcnt[i] = 0;
+        lwin[2 * i] = mid - e;
+        lwin[2 * i + 1] = mid + e;
+        const double r = gs_sieve_radius(u, gmax, tolsq);
+        lrad[2 * i] = r;
+        lrad[2 * i + 1] = 2.0 * r;
+        double* ur = lubi + 9 * i;
+        ur[0] = u.u00; ur[1] = u.u01; ur[2] = u.u02;
+        ur[3] = u.u10; ur[4] = u.u11; ur[5] = u.u12;
+        ur[6] = u.u20; ur[7] = u.u21; ur[8] = u.u22;
+    }
+    __syncthreads();
+    int* q = lq + __builtin_amdgcn_readfirstlane(wave) * QCAP;  // scalar base
+    const double* gvc = gv + 3 * c0;
+    int qn = 0;  // wave-uniform
+    for (int gl = 0; gl < ns; gl++) {
+        const double* ug = ubis + 9 * (size_t)(g0 + gl);
+        const double u00 = ug[0], u01 = ug[1], u02 = ug[2];
+        const double r = lrad[2 * gl], r2 = lrad[2 * gl + 1];
+        const int tag = gl << 9;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const double h = __builtin_fma(u02, z[p], __builtin_fma(u01, y[p], __builtin_fma(u00, x[p], r)));
+            const bool sv = !(gs_fract(h) >= r2);
+            const unsigned long long m = __ballot(sv);
+            if (sv) {  // rank among the survivors, then the scalar top of the queue
+                const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+                    (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                (q + qn)[pos] = tag | (p << 6) | lane;
+            }
+            qn += __popcll(m);
+        }
+        while (qn >= GS_WAVE) {  // settle the top 64 entries
+            qn -= GS_WAVE;
+            __builtin_amdgcn_wave_barrier();
+            gs_sieve_settle(q[qn + lane], gvc, nrem, wave, lubi, lwin, tolsq, lcnt);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < qn) gs_sieve_settle(q[lane], gvc, nrem, wave, lubi, lwin, tolsq, lcnt);
+    __syncthreads();
+    for (int i = threadIdx.x; i < ns; i += GS_BLOCK) {
+        const int c = lcnt[i];
         if (c) atomicAdd(&counts[g0 + i], c);
     }
 }

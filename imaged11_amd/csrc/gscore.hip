@@ -237,12 +237,12 @@ int check_ng(long long ng) {
 // last round costs ~3% (at 32 per CU it cost ~10%), while a split keeps
 // at least 8 grains to amortise the tile's peak loads and window setup.
 int pair_tiles(DeviceCtx* ctx, int K, int ng, int P, int* gsplit_out, int* nsplit_out,
-               unsigned* nb_out) {
+               unsigned* nb_out, int split_max = GS_SPLIT_MAX) {
     const long long nchunks = (ng + (long long)GS_BLOCK * P - 1) / ((long long)GS_BLOCK * P);
     const long long want = 128LL * ctx->cus;
     long long nsplit = std::max<long long>(1, (want + nchunks - 1) / nchunks);
     nsplit = std::min<long long>(nsplit, std::max(1, K / 8));
-    nsplit = std::max<long long>(nsplit, (K + GS_SPLIT_MAX - 1) / GS_SPLIT_MAX);
+    nsplit = std::max<long long>(nsplit, (K + split_max - 1) / split_max);
     const int gsplit = (int)((K + nsplit - 1) / nsplit);
     nsplit = (K + gsplit - 1) / gsplit;
     const long long nb = nchunks * nsplit;
@@ -265,11 +265,21 @@ int launch_score_many(const double* ubis, int K, const double* gv, int ng, doubl
     // VGPRs, more resident waves), 8 for the exact one (measured best)
     int P = peaks_per_lane;
     if (P != 1 && P != 2 && P != 4 && P != 8 && P != 16) P = mode == GS_SCORE_EXACT ? 8 : 4;
+    if (mode == GS_SCORE_SIEVE && P > 8) P = 8;
     int gsplit, nsplit;
     unsigned nb;
-    GS_CHECK(pair_tiles(ctx, K, ng, P, &gsplit, &nsplit, &nb));
+    GS_CHECK(pair_tiles(ctx, K, ng, P, &gsplit, &nsplit, &nb,
+                        mode == GS_SCORE_SIEVE ? GS_SIEVE_SPLIT_MAX : GS_SPLIT_MAX));
     const dim3 grid(nb), block(GS_BLOCK);
-    if (mode == GS_SCORE_EXACT) {
+    if (mode == GS_SCORE_SIEVE) {
+        if (gsplit > GS_SIEVE_SPLIT_MAX) return set_err(GS_ERR_ARG, "sieve split %d too large", gsplit);
+        switch (P) {
+            case 1: hipLaunchKernelGGL(k_score_many_sieve<1>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 2: hipLaunchKernelGGL(k_score_many_sieve<2>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            case 8: hipLaunchKernelGGL(k_score_many_sieve<8>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+            default: hipLaunchKernelGGL(k_score_many_sieve<4>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
+        }
+    } else if (mode == GS_SCORE_EXACT) {
         switch (P) {
             case 1: hipLaunchKernelGGL(k_score_many<1>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
             case 2: hipLaunchKernelGGL(k_score_many<2>, grid, block, 0, st, ubis, K, gsplit, nsplit, gv, ng, tolsq, counts); break;
@@ -804,7 +814,7 @@ int gs_clear_error(void) {
 
 int gs_score_many(const double* ubis, int K, const double* gv, int ng, double tol, int* counts,
                   int peaks_per_lane, int mode, gs_stream_t stream) {
-    if (mode != GS_SCORE_CERTIFIED && mode != GS_SCORE_EXACT)
+    if (mode != GS_SCORE_CERTIFIED && mode != GS_SCORE_EXACT && mode != GS_SCORE_SIEVE)
         return set_err(GS_ERR_ARG, "gs_score_many: unknown mode %d", mode);
     return launch_score_many(ubis, K, gv, ng, tol, counts, peaks_per_lane, mode, S(stream));
 }

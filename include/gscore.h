@@ -104,17 +104,20 @@ void compute_xlylzl(double s[], double f[], double p[4], double r[9], double dis
 
 /* ---- 2. device-pointer entry points --------------------------------- */
 
-/* score modes: both give the reference's counts for every input.
+/* score modes: all give the reference's counts for every input.
  * CERTIFIED evaluates h with FMA and decides each pair from a proven error
  * bound, recomputing the few undecidable pairs with the reference's own
  * arithmetic; EXACT uses the reference's arithmetic for every pair. */
 #define GS_SCORE_CERTIFIED 0
 #define GS_SCORE_EXACT 1
+/* gs_score_many only: row 0 of every pair first; the pairs it cannot
+ * exclude (|h0 - round h0| near or below tol) take the CERTIFIED test. */
+#define GS_SCORE_SIEVE 2
 
 /* score for K UBIs (K,3,3) against ng g-vectors -> counts[K] (int32);
  * the loop indexer.score runs over trial UBIs (ImageD11/indexing.py:1090).
  * peaks_per_lane: 1, 2, 4, 8 or 16 (tile height; 0 = default: 4 certified,
- * 8 exact). */
+ * 8 exact; the sieve takes at most 8). */
 int gs_score_many(const double *ubis, int K, const double *gv, int ng, double tol,
                   int *counts, int peaks_per_lane, int mode, gs_stream_t stream);
 

@@ -217,21 +217,43 @@ def _oracle_gv(orc, wl, t=np.zeros(3)):
     return xl, gv
 
 
-@pytest.mark.parametrize("exact", [False, True])
+SCORE_MODES = [_lib.SCORE_CERTIFIED, _lib.SCORE_EXACT, _lib.SCORE_SIEVE]
+
+
+@pytest.mark.parametrize("mode", SCORE_MODES)
 @pytest.mark.parametrize("ppl", [1, 2, 4, 8])
-def test_score_many_pipeline(dev, orc, workload, ppl, exact):
+def test_score_many_pipeline(dev, orc, workload, ppl, mode):
     wl = workload
     _, gv = _oracle_gv(orc, wl)
     want = orc.score_many(wl["ubi"], gv, 0.05)
     pk = wl["peaks"]
     g = batched.sf2gv(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev), wl["pars"])
     got = batched.score_many(T(wl["ubi"], dev), g, 0.05, peaks_per_lane=ppl,
-                             exact=exact).cpu().numpy()
+                             mode=mode).cpu().numpy()
     assert np.array_equal(got, want)
     # and on the oracle's own g-vectors (no sincos difference at all)
     got2 = batched.score_many(T(wl["ubi"], dev), T(gv, dev), 0.05, peaks_per_lane=ppl,
-                              exact=exact)
+                              mode=mode)
     assert np.array_equal(got2.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("tol", [0.01, 0.05, 0.2, 0.6, 1.0])
+def test_score_sieve_tolerances_and_ragged_sizes(dev, orc, workload, tol):
+    """The one-row sieve at wide and narrow tolerances (from ~2% to every
+    pair passing the first row) and at grain/peak counts that leave partial
+    splits, chunks and queues: the oracle's counts."""
+    wl = workload
+    _, gv = _oracle_gv(orc, wl)
+    u, g = T(wl["ubi"], dev), T(gv, dev)
+    want = orc.score_many(wl["ubi"], gv, tol)
+    for ppl in (1, 4, 8):
+        got = batched.score_many(u, g, tol, peaks_per_lane=ppl, mode=_lib.SCORE_SIEVE)
+        assert np.array_equal(got.cpu().numpy(), want)
+    for K, N in [(1, 1), (7, 63), (129, 1000), (len(u), 4097)]:
+        want = orc.score_many(wl["ubi"][:K], gv[:N], tol)
+        got = batched.score_many(u[:K].contiguous(), g[:N].contiguous(), tol,
+                                 mode=_lib.SCORE_SIEVE)
+        assert np.array_equal(got.cpu().numpy(), want)
 
 
 def test_sf2gv_matches_separate_kernels(dev, orc, workload):
@@ -323,6 +345,16 @@ def test_edge_cases(dev):
         assert np.array_equal(d1.view(np.int64), d2.view(np.int64)) and np.array_equal(l1, l2)
     K0 = batched.score_many(T(np.zeros((0, 3, 3)), dev), T(gv, dev), 0.1)
     assert K0.numel() == 0
+    # every score mode on the same rows, one block and many, with finite,
+    # tiny, infinite and huge UBIs (the sieve lets NaN rows through to the
+    # full test and opens completely where the error bound does not apply)
+    for tol in (0.0, 0.1, 0.8, 1.5, 1e10, -0.1):
+        for ub in (np.eye(3), np.eye(3) * 1e-300, np.full((3, 3), np.inf), np.eye(3) * 1e40):
+            for rows in (gv, np.tile(gv, (300, 1))):
+                want = orc.score(ub, rows, tol)
+                for m in SCORE_MODES:
+                    assert batched.score_many(T(ub[None], dev), T(rows, dev), tol,
+                                              mode=m).item() == want
 
 
 def test_full_scoring_workload_counts(dev, orc):
@@ -348,9 +380,9 @@ def test_score_certified_boundary(dev, orc, tol, hmax):
         u = T(ubi[None], dev)
         g = T(gv, dev)
         for ppl in (1, 4, 8):
-            fast = batched.score_many(u, g, tol, peaks_per_lane=ppl).item()
-            exact = batched.score_many(u, g, tol, peaks_per_lane=ppl, exact=True).item()
-            assert fast == exact == want
+            got = [batched.score_many(u, g, tol, peaks_per_lane=ppl, mode=m).item()
+                   for m in SCORE_MODES]
+            assert got == [want] * len(SCORE_MODES)
         # the assignment state machine on the same boundary pairs
         for d0 in (1.0, tol * tol * (1 + 1e-13)):
             d, lab = np.full(len(gv), d0), np.zeros(len(gv), np.int32)
