@@ -16,6 +16,9 @@
 #define GS_GRAIN_CHUNK 1024  // grains per LDS counter tile
 #define GS_SPLIT_MAX 512     // grains per split of the all-pairs score/select tiles
 #define GS_SIEVE_SPLIT_MAX 128  // grains per split of k_score_many_sieve (UBI table in LDS)
+#ifndef GS_SIEVE_TILES_PER_CU
+#define GS_SIEVE_TILES_PER_CU 64  // (chunk, split) tiles per CU of k_score_many_sieve (32: 0.413 ms, 64: 0.403, 128: 0.416, 256: 0.447 at 1000 x 1e6)
+#endif
 #define GS_NPART 17          // n, sum drlv2, R[9], H[6]
 #ifndef GS_REFINE_BATCH
 #define GS_REFINE_BATCH 8    // list rows per lane in flight in k_refine_lists (rows mode)

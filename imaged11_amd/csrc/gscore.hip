@@ -237,9 +237,9 @@ int check_ng(long long ng) {
 // last round costs ~3% (at 32 per CU it cost ~10%), while a split keeps
 // at least 8 grains to amortise the tile's peak loads and window setup.
 int pair_tiles(DeviceCtx* ctx, int K, int ng, int P, int* gsplit_out, int* nsplit_out,
-               unsigned* nb_out, int split_max = GS_SPLIT_MAX) {
+               unsigned* nb_out, int split_max = GS_SPLIT_MAX, int tiles_per_cu = 128) {
     const long long nchunks = (ng + (long long)GS_BLOCK * P - 1) / ((long long)GS_BLOCK * P);
-    const long long want = 128LL * ctx->cus;
+    const long long want = (long long)tiles_per_cu * ctx->cus;
     long long nsplit = std::max<long long>(1, (want + nchunks - 1) / nchunks);
     nsplit = std::min<long long>(nsplit, std::max(1, K / 8));
     nsplit = std::max<long long>(nsplit, (K + split_max - 1) / split_max);
@@ -268,8 +268,11 @@ int launch_score_many(const double* ubis, int K, const double* gv, int ng, doubl
     if (mode == GS_SCORE_SIEVE && P > 8) P = 8;
     int gsplit, nsplit;
     unsigned nb;
-    GS_CHECK(pair_tiles(ctx, K, ng, P, &gsplit, &nsplit, &nb,
-                        mode == GS_SCORE_SIEVE ? GS_SIEVE_SPLIT_MAX : GS_SPLIT_MAX));
+    if (mode == GS_SCORE_SIEVE)
+        GS_CHECK(pair_tiles(ctx, K, ng, P, &gsplit, &nsplit, &nb, GS_SIEVE_SPLIT_MAX,
+                            GS_SIEVE_TILES_PER_CU));
+    else
+        GS_CHECK(pair_tiles(ctx, K, ng, P, &gsplit, &nsplit, &nb));
     const dim3 grid(nb), block(GS_BLOCK);
     if (mode == GS_SCORE_SIEVE) {
         if (gsplit > GS_SIEVE_SPLIT_MAX) return set_err(GS_ERR_ARG, "sieve split %d too large", gsplit);
