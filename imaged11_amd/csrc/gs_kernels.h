@@ -16,6 +16,7 @@
 #define GS_GRAIN_CHUNK 1024  // grains per LDS counter tile
 #define GS_SPLIT_MAX 512     // grains per split of the all-pairs score/select tiles
 #define GS_SIEVE_SPLIT_MAX 128  // grains per split of k_score_many_sieve (UBI table in LDS)
+#define GS_SIEVE_THR_MAX 0.01  // gs_select_many takes the sieve for thr <= 0.01 (tol <= 0.1)
 #ifndef GS_SIEVE_TILES_PER_CU
 #define GS_SIEVE_TILES_PER_CU 64  // (chunk, split) tiles per CU of k_score_many_sieve (32: 0.413 ms, 64: 0.403, 128: 0.416, 256: 0.447 at 1000 x 1e6)
 #endif
@@ -232,9 +233,17 @@ __global__ __launch_bounds__(GS_BLOCK, P <= 4 ? 8 : 1) void k_score_many_fast(
 // per (block, grain).
 //
 // Queue entry: (grain offset in the split) << 9 | p << 6 | lane.
+//
+// SELECT = true is the sieved k_select_many: a settled pair that passes takes
+// a slot of its grain with one global atomic on cursor[g] (passing pairs are
+// the rare true hits) and, when idx is given, writes its peak index at
+// idx[off[g] + slot] for slot < off[g+1] - off[g].
+template <bool SELECT>
 __device__ __forceinline__ void gs_sieve_settle(int e, const double* __restrict__ gvc, long long nrem,
                                                 int wave, const double* lubi, const double* lwin,
-                                                double tolsq, int* lcnt) {
+                                                double tolsq, int* lcnt, int g0, long long c0,
+                                                const long long* __restrict__ off,
+                                                int* __restrict__ cursor, int* __restrict__ idx) {
     const int gl = e >> 9;
     const long long k = (long long)((e >> 6) & 7) * GS_BLOCK + wave * GS_WAVE + (e & 63);
     if (k >= nrem) return;  // NaN padding rows of the last chunk
@@ -244,13 +253,22 @@ __device__ __forceinline__ void gs_sieve_settle(int e, const double* __restrict_
     const double s = gs_drlv2_fract(u, x, y, z);
     bool in = s < lwin[2 * gl];
     if (!in && s < lwin[2 * gl + 1]) in = gs_drlv2(u, x, y, z) < tolsq;  // undecided: exact
-    if (in) atomicAdd(&lcnt[gl], 1);
+    if (!in) return;
+    if constexpr (SELECT) {
+        const int g = g0 + gl;
+        const int slot = atomicAdd(&cursor[g], 1);
+        if (idx && slot < off[g + 1] - off[g]) idx[off[g] + slot] = (int)(c0 + k);
+    } else {
+        atomicAdd(&lcnt[gl], 1);
+    }
 }
 
-template <int P>
+template <int P, bool SELECT = false>
 __global__ __launch_bounds__(GS_BLOCK, 8) void k_score_many_sieve(
     const double* __restrict__ ubis, int K, int gsplit, int nsplit,
-    const double* __restrict__ gv, int ng, double tolsq, int* __restrict__ counts) {
+    const double* __restrict__ gv, int ng, double tolsq, int* __restrict__ counts,
+    const long long* __restrict__ off = nullptr, int* __restrict__ cursor = nullptr,
+    int* __restrict__ idx = nullptr) {
     static_assert(P >= 1 && P <= 8, "the queue entry holds p in three bits");
     constexpr int NW = GS_BLOCK / GS_WAVE;
     constexpr int QCAP = GS_WAVE * (P + 1);
@@ -329,12 +347,16 @@ __global__ __launch_bounds__(GS_BLOCK, 8) void k_score_many_sieve(
         while (qn >= GS_WAVE) {  // settle the top 64 entries
             qn -= GS_WAVE;
             __builtin_amdgcn_wave_barrier();
-            gs_sieve_settle(q[qn + lane], gvc, nrem, wave, lubi, lwin, tolsq, lcnt);
+            gs_sieve_settle<SELECT>(q[qn + lane], gvc, nrem, wave, lubi, lwin, tolsq, lcnt, g0, c0,
+                                    off, cursor, idx);
             __builtin_amdgcn_wave_barrier();
         }
     }
     __builtin_amdgcn_wave_barrier();
-    if (lane < qn) gs_sieve_settle(q[lane], gvc, nrem, wave, lubi, lwin, tolsq, lcnt);
+    if (lane < qn)
+        gs_sieve_settle<SELECT>(q[lane], gvc, nrem, wave, lubi, lwin, tolsq, lcnt, g0, c0, off,
+                                cursor, idx);
+    if constexpr (SELECT) return;
     __syncthreads();
     for (int i = threadIdx.x; i < ns; i += GS_BLOCK) {
         const int c = lcnt[i];

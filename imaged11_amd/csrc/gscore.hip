@@ -1530,6 +1530,15 @@ int gs_select_many(const double* ubis, int K, const double* gv, int ng, double t
     constexpr int P = 4;
     int gsplit, nsplit;
     unsigned nb;
+    if (thr <= GS_SIEVE_THR_MAX) {  // few pairs pass row 0: the sieved selection
+        GS_CHECK(pair_tiles(ctx, K, ng, P, &gsplit, &nsplit, &nb, GS_SIEVE_SPLIT_MAX,
+                            GS_SIEVE_TILES_PER_CU));
+        if (gsplit > GS_SIEVE_SPLIT_MAX) return set_err(GS_ERR_ARG, "sieve split %d too large", gsplit);
+        hipLaunchKernelGGL((k_score_many_sieve<P, true>), dim3(nb), dim3(GS_BLOCK), 0, S(stream),
+                           ubis, K, gsplit, nsplit, gv, ng, thr, nullptr, off, cursor, idx);
+        GS_LAUNCHED();
+        return GS_OK;
+    }
     GS_CHECK(pair_tiles(ctx, K, ng, P, &gsplit, &nsplit, &nb));
     hipLaunchKernelGGL(k_select_many<P>, dim3(nb), dim3(GS_BLOCK), 0, S(stream), ubis, K, gsplit,
                        nsplit, gv, ng, thr, off, cursor, idx);

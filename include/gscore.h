@@ -483,7 +483,8 @@ int gs_orient_hits(const double *gv, int ng, const int *hi, const int *hj, int n
  * (score_and_assign from drlv2 = 1, ImageD11/indexing.py:1060-1088).
  * cursor (K ints, zeroed here) receives the counts; with off (K+1 CSR
  * offsets from those counts) and idx, the peak indices are also written,
- * grain g's in idx[off[g]..off[g+1]) in no particular order. */
+ * grain g's in idx[off[g]..off[g+1]) in no particular order.  For
+ * thr <= 0.01 the one-row sieve of GS_SCORE_SIEVE runs first. */
 int gs_select_many(const double *ubis, int K, const double *gv, int ng, double thr,
                    const long long *off, int *cursor, int *idx, gs_stream_t stream);
 

@@ -184,7 +184,9 @@ def test_select_lists_match_oracle(dev, orc):
     rng = np.random.default_rng(0)
     g[:] = rng.normal(size=(30000, 3)) * 0.6
     ubis = wl["ubi"]
-    for tol in (0.05, 0.3, 1.5):
+    # tol <= 0.1 takes the sieved selection (row 0 first), above it the
+    # all-rows certified kernel
+    for tol in (0.01, 0.05, 0.1, 0.3, 1.5):
         thr = min(tol * tol, 1.0)
         idx, off = indexing.select_lists(torch.as_tensor(ubis, device=dev),
                                          torch.as_tensor(g, device=dev), thr)
@@ -194,3 +196,18 @@ def test_select_lists_match_oracle(dev, orc):
             d, lab = np.ones(len(g)), np.zeros(len(g), np.int32)
             orc.score_and_assign(u, g, tol, d, lab, 1)
             assert np.array_equal(idx[off[k]:off[k + 1]], np.nonzero(lab == 1)[0]), (tol, k)
+
+
+@pytest.mark.parametrize("tol,hmax", [(0.05, 3), (0.05, 40), (1e-4, 2)])
+def test_select_lists_boundary_pairs(dev, orc, tol, hmax):
+    """Pairs within 1e-17..1e-9 relative of tol^2 through the sieved
+    selection: the lists hold exactly the peaks the reference arithmetic
+    counts."""
+    from helpers import boundary_pairs
+    for seed in range(2):
+        ubi, gv = boundary_pairs(seed, 60000, tol, hmax)
+        d, lab = np.ones(len(gv)), np.zeros(len(gv), np.int32)
+        orc.score_and_assign(ubi, gv, tol, d, lab, 1)
+        idx, off = indexing.select_lists(torch.as_tensor(ubi[None], device=dev),
+                                         torch.as_tensor(gv, device=dev), tol * tol)
+        assert np.array_equal(idx.cpu().numpy(), np.nonzero(lab == 1)[0])
