@@ -3,6 +3,7 @@ every declared symbol, the host logic of the drop-in (f2py argument rules),
 the accuracy of the device sincos algorithm (its host build), the
 simulator's invariants.  No kernel is launched."""
 import math
+import os
 
 import mpmath
 import numpy as np
@@ -259,3 +260,17 @@ def test_ctypes_signatures_match_the_header():
     for name, (_, args) in _lib.SIGNATURES.items():
         assert name in arity, name
         assert len(args) == arity[name], (name, len(args), arity[name])
+
+
+def test_mode_constants_match_the_header():
+    """The score-mode values the Python layer passes are the header's."""
+    import re
+    text = open(os.path.join(os.path.dirname(__file__), "..", "include", "gscore.h")).read()
+    defs = {m.group(1): int(m.group(2))
+            for m in re.finditer(r"#define GS_SCORE_(\w+) (\d+)", text)}
+    assert defs == {"CERTIFIED": _lib.SCORE_CERTIFIED, "EXACT": _lib.SCORE_EXACT,
+                    "SIEVE": _lib.SCORE_SIEVE}
+    from imaged11_amd import batched
+    assert batched.score_mode(0.05) == _lib.SCORE_SIEVE
+    assert batched.score_mode(-0.1) == _lib.SCORE_SIEVE
+    assert batched.score_mode(0.2) == _lib.SCORE_CERTIFIED
