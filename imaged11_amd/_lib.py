@@ -113,6 +113,14 @@ SIGNATURES = {
 SCORE_CERTIFIED = 0
 SCORE_EXACT = 1
 SCORE_SIEVE = 2
+SIEVE_TOL_MAX = 0.1  # the one-row sieve pays while ~2 tol of the pairs pass row 0
+
+
+def score_mode(tol):
+    """The faster of the certified score kernels for this tolerance: the
+    sieve (row 0 of every pair first) for |tol| <= SIEVE_TOL_MAX, else the
+    all-rows certified kernel.  Both give the reference's counts."""
+    return SCORE_SIEVE if abs(float(tol)) <= SIEVE_TOL_MAX else SCORE_CERTIFIED
 
 _lib = None
 

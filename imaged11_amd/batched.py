@@ -33,14 +33,7 @@ def _ready():
     return lib
 
 
-SIEVE_TOL_MAX = 0.1  # the one-row sieve pays while ~2 tol of the pairs pass row 0
-
-
-def score_mode(tol):
-    """The faster of the certified kernels for this tolerance: the sieve
-    (row 0 of every pair first) for |tol| <= SIEVE_TOL_MAX, else the
-    all-rows certified kernel.  Both give the reference's counts."""
-    return _lib.SCORE_SIEVE if abs(float(tol)) <= SIEVE_TOL_MAX else _lib.SCORE_CERTIFIED
+from ._lib import SIEVE_TOL_MAX, score_mode  # noqa: E402  (the default score kernel)
 
 
 def score_many(ubis, gv, tol, peaks_per_lane=0, out=None, exact=False, mode=None):

@@ -145,7 +145,7 @@ def score(ubi, gv, tol):
         lib = _lib_ready()
         out = torch.empty(1, dtype=torch.int32, device=g.device)
         _lib.check(lib.gs_score_many(dptr(u), 1, dptr(g), g.shape[0], float(tol), dptr(out), 0,
-                                     _lib.SCORE_CERTIFIED, stream_ptr()), "score")
+                                     _lib.score_mode(tol), stream_ptr()), "score")
         return int(out.item())
     u = _in_array(ubi, np.float64, (3, 3), "ubi")
     g = _in_array(gv, np.float64, (None, 3), "gv")
