@@ -50,6 +50,7 @@ SIGNATURES = {
     "gs_assign_many": (_i, [_vp, _vp, _i, _vp, _i, _d, _vp, _vp, _vp, _i, _vp]),
     "gs_score_and_refine_dev": (_i, [_vp, _vp, _i, _d, _vp, _vp, _vp]),
     "gs_refine_assigned_dev": (_i, [_vp, _vp, _i, _vp, _i, _vp, _vp, _vp]),
+    "gs_score_and_refine_dev_noalloc": (_i, [_vp, _vp, _i, _d, _vp, _vp, _vp]),
     "gs_score_and_refine_host_ubi": (_i, [_vp, _vp, _i, _d, _vp, _vp, _vp]),
     "gs_refine_assigned_host_ubi": (_i, [_vp, _vp, _i, _vp, _i, _vp, _vp, _vp]),
     "gs_compute_xlylzl_dev": (_i, [_vp, _vp, _dp, _dp, _dp, _vp, _i, _vp]),

@@ -30,6 +30,13 @@
 #include "gs_pbp.h"
 #include "../../include/gscore.h"
 
+// Scratch for the entry point that allocates nothing: the block partials
+// of the one-grain refinement (1024 blocks x 17 doubles = 136 KiB), one copy
+// per GPU, held until the process ends.  Its use is explained beside
+// scratch_ptr() below.
+#define GS_SCRATCH_DOUBLES (1024 * GS_NPART)
+static __device__ __align__(256) double scratch[GS_SCRATCH_DOUBLES];
+
 namespace {
 
 thread_local std::string g_err;
@@ -324,12 +331,17 @@ int launch_assign_many(const double* ubis, const int* glabels, int K, const doub
     return GS_OK;
 }
 
-int launch_refine_one(int mode, double* ubi, const double* gv, int ng, double tol,
-                      const int* labels, int label, int* n_out, double* mean_out, hipStream_t st) {
-    DeviceCtx* ctx;
-    GS_CHECK(get_ctx(&ctx));
-    GS_CHECK(check_ng(ng));
-    const int nb = std::min(GS_MAX_PARTIAL_BLOCKS, blocks_for(ng, GS_BLOCK * 4));
+// Blocks of the one-grain refinement: one per 4·GS_BLOCK peaks, at most
+// GS_MAX_PARTIAL_BLOCKS (each block strides over the peaks), so the block
+// partials never need more than GS_MAX_PARTIAL_BLOCKS · GS_NPART doubles.
+int refine_one_blocks(int ng) { return std::min(GS_MAX_PARTIAL_BLOCKS, blocks_for(ng, GS_BLOCK * 4)); }
+
+// The launches of the one-grain refinement with a given block-partials
+// buffer (GS_MAX_PARTIAL_BLOCKS · GS_NPART doubles; unused by one block).
+int refine_one_launch(int mode, double* ubi, const double* gv, int ng, double tol,
+                      const int* labels, int label, int* n_out, double* mean_out,
+                      double* partials, hipStream_t st) {
+    const int nb = refine_one_blocks(ng);
     const double tolsq = tol * tol;
     if (nb == 1) {  // one block: it finishes the job itself (same bits, one launch)
         if (mode == 0)
@@ -341,8 +353,6 @@ int launch_refine_one(int mode, double* ubi, const double* gv, int ng, double to
         GS_LAUNCHED();
         return GS_OK;
     }
-    double* partials;
-    GS_CHECK(get_partials(&partials));
     if (mode == 0)
         hipLaunchKernelGGL((k_refine_partials<0, false>), dim3(nb), dim3(GS_BLOCK), 0, st, ubi, gv,
                            ng, tolsq, labels, label, partials, nullptr, nullptr);
@@ -353,6 +363,51 @@ int launch_refine_one(int mode, double* ubi, const double* gv, int ng, double to
     hipLaunchKernelGGL(k_refine_finalize, dim3(1), dim3(GS_WAVE), 0, st, partials, nb, ubi,
                        n_out, mean_out);
     GS_LAUNCHED();
+    return GS_OK;
+}
+
+int launch_refine_one(int mode, double* ubi, const double* gv, int ng, double tol,
+                      const int* labels, int label, int* n_out, double* mean_out, hipStream_t st) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(ng));
+    double* partials = nullptr;
+    if (refine_one_blocks(ng) > 1) GS_CHECK(get_partials(&partials));
+    return refine_one_launch(mode, ubi, gv, ng, tol, labels, label, n_out, mean_out, partials, st);
+}
+
+// ---- scratch without allocation ----------------------------------------------
+//
+// A fixed device array, one copy per GPU, that holds the block partials of
+// the one-grain refinement for gs_score_and_refine_dev_noalloc, whose
+// callers cannot allocate (code under graph capture, serving loops).  It
+// holds the partials of the largest grid that refinement launches, so no
+// peak count is too large for it.  Host code never passes the array's name
+// to a kernel: its device address, looked up once per device with
+// hipGetSymbolAddress, is kept in g_scratch_ptr.
+static_assert(GS_SCRATCH_DOUBLES == GS_MAX_PARTIAL_BLOCKS * GS_NPART,
+              "scratch holds the partials of the largest one-grain refinement grid");
+
+constexpr int GS_SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[GS_SCRATCH_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch` (looked up on the first call on
+// that device).  Allocates nothing.
+int scratch_ptr(double** out) {
+    int dev = 0;
+    GS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= GS_SCRATCH_MAX_DEVICES)
+        return set_err(GS_ERR_ARG, "device %d: the scratch table holds %d devices", dev,
+                       GS_SCRATCH_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        GS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_err(GS_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
     return GS_OK;
 }
 
@@ -490,7 +545,9 @@ int gs_device_count(void) {
 
 int gs_init(void) {
     DeviceCtx* ctx;
-    return get_ctx(&ctx);
+    GS_CHECK(get_ctx(&ctx));
+    double* sp;
+    return scratch_ptr(&sp);  // the no-allocation entry point's scratch on this device
 }
 
 // ---- reference prototypes on host pointers ---------------------------------
@@ -839,6 +896,28 @@ int gs_score_and_refine_dev(double* ubi, const double* gv, int ng, double tol, i
 int gs_refine_assigned_dev(double* ubi, const double* gv, int ng, const int* labels, int label,
                            int* n_out, double* mean_out, gs_stream_t stream) {
     return launch_refine_one(1, ubi, gv, ng, 0.0, labels, label, n_out, mean_out, S(stream));
+}
+
+// gs_score_and_refine_dev without any allocation: the block partials live
+// in the library's fixed `scratch` array of the current device instead of
+// a buffer allocated on a thread's first call.  Same kernels, same grid,
+// so the same bits as gs_score_and_refine_dev.  Nothing in the scratch has
+// to start at zero (every block writes its partials before the finishing
+// launch reads them); calls that overlap on one GPU share the array and
+// race.
+int gs_score_and_refine_dev_noalloc(double* ubi, const double* gv, int ng, double tol,
+                                    int* n_out, double* mean_out, gs_stream_t stream) {
+    if (ng < 0) return set_err(GS_ERR_ARG, "gs_score_and_refine_dev_noalloc: negative ng %d", ng);
+    if (!ubi || !n_out || !mean_out || (ng > 0 && !gv))
+        return set_err(GS_ERR_ARG, "gs_score_and_refine_dev_noalloc: null pointer");
+    const int nb = refine_one_blocks(ng);
+    if ((long long)nb * GS_NPART > GS_SCRATCH_DOUBLES)
+        return set_err(GS_ERR_ARG, "gs_score_and_refine_dev_noalloc: %d block partials exceed "
+                       "the scratch of %d doubles", nb, GS_SCRATCH_DOUBLES);
+    double* partials;
+    GS_CHECK(scratch_ptr(&partials));
+    return refine_one_launch(0, ubi, gv, ng, tol, nullptr, 0, n_out, mean_out, partials,
+                             S(stream));
 }
 
 // The UBI and the two results on the host, the peaks on the device: the

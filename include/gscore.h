@@ -135,6 +135,19 @@ int gs_score_and_refine_dev(double *ubi, const double *gv, int ng, double tol, i
                             double *mean_out, gs_stream_t stream);
 int gs_refine_assigned_dev(double *ubi, const double *gv, int ng, const int *labels,
                            int label, int *n_out, double *mean_out, gs_stream_t stream);
+/* gs_score_and_refine_dev for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  It calls no hipMalloc,
+ * hipFree or allocating routine: the block partials (at most 1024 blocks x
+ * 17 doubles; every ng >= 0 fits, since the grid is capped at 1024
+ * blocks that stride over the peaks) live in a fixed 136 KiB device array
+ * of the library, one per GPU, held until the process ends and looked up
+ * by gs_init() or the first call on a device.  Same results, bit for bit,
+ * as gs_score_and_refine_dev.  Calls that overlap on one GPU share that
+ * array and race: serialise them (one stream, or events).  Rejects a
+ * negative ng or a null pointer with GS_ERR_ARG. */
+int gs_score_and_refine_dev_noalloc(double *ubi, const double *gv, int ng, double tol,
+                                    int *n_out, double *mean_out, gs_stream_t stream);
 /* The same two calls with ubi (9 doubles, updated in place), n_out and
  * mean_out on the HOST and gv (and labels) on the device: the calling
  * pattern of a per-grain loop over device-resident peaks.  Waits for the
