@@ -16,10 +16,14 @@ from its sources) running the same pipeline on the host cores, the grain
 loop spread over threads (ctypes releases the GIL), with its per-grain
 counts compared to the GPU's.
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 50] [--no-cpu] [--no-aux]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 50] [--warmup-s 1.5]
+                    [--no-cpu] [--no-aux]
 
-The warmup is long enough for the GPU clock to settle: the first few
-launches of the score kernel run up to 20% slower than the steady state.
+Warmup: the --warmup steps, then more until --warmup-s seconds of steps
+have run, whatever --warmup says: the GPU clock ramps up over the first
+launches after idle (they run 15-25% slower), and the timed region should
+see the steady clock.  The JSON line states both (warmup, warmup_s,
+warmup_steps_run).
 """
 import argparse
 import json
@@ -57,6 +61,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--warmup-s", type=float, default=1.5,
+                    help="keep warming up until this many seconds of steps have run")
     ap.add_argument("--grains", type=int, default=1000)
     ap.add_argument("--peaks", type=int, default=1_000_000)
     ap.add_argument("--tol", type=float, default=0.05)
@@ -69,6 +75,25 @@ def parse():
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def host_cpus(requested=0):
+    """(threads, facts) for the host baselines: the CPUs this process may
+    run on (its affinity mask), capped by OMP_NUM_THREADS when that is set
+    (the CPU share a box gives to one GPU), unless --cpu-threads says
+    otherwise.  facts records nproc, the affinity count and the cap."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    cap = int(env) if env.isdigit() and int(env) > 0 else None
+    threads = requested or (min(affinity, cap) if cap else affinity)
+    return threads, {"nproc": nproc, "affinity_cpus": affinity, "omp_num_threads": cap,
+                     "threads_used": threads,
+                     "rule": "--cpu-threads if given, else the affinity CPUs capped by "
+                             "OMP_NUM_THREADS"}
 
 
 def measured_traffic(kernel):
@@ -225,16 +250,37 @@ def aux_refine_adjoint(threads, with_cpu):
 
         def one(k):
             u = start[k].copy()
-            for _ in range(n_iter):
-                ref.score_and_refine(u, sub[k], tol)
-            return u
+            nm = [ref.score_and_refine(u, sub[k], tol) for _ in range(n_iter)]
+            return u, nm
         with ThreadPoolExecutor(max_workers=threads) as pool:
             t0 = time.perf_counter()
-            list(pool.map(one, range(K)))
+            res = list(pool.map(one, range(K)))
             cpu_ms = (time.perf_counter() - t0) * 1e3
         out["cpu_reference_ms"] = cpu_ms
         out["cpu_threads"] = threads
         out["speedup_vs_cpu"] = cpu_ms / batched_ms
+        # parity against the reference's per-grain loop (outside the timed
+        # region): every iteration's n bit-exact, UBIs within 1e-10 of the
+        # largest entry, the first call's mean within 1e-12 relative (later
+        # calls start from UBIs that already differ by rounding)
+        u_g, n_g, m_g = batched.refine_lists(ubis0, gv, idx_d, off_d, tol, n_iter, order=order,
+                                             lanes_per_grain=best["lanes_per_grain"],
+                                             waves_per_cu=best["waves_per_cu"],
+                                             long_min=best.get("long_min", 0))
+        u_g, n_g, m_g = u_g.cpu().numpy(), n_g.cpu().numpy(), m_g.cpu().numpy()
+        u_c = np.array([r[0] for r in res])
+        n_c = np.array([[nm[0] for nm in r[1]] for r in res]).T
+        m_c = np.array([[nm[1] for nm in r[1]] for r in res]).T
+        rel = np.abs(m_g - m_c) / np.maximum(np.abs(m_c), 1e-300)
+        ubi_err = float(np.max(np.abs(u_g - u_c).max(axis=(1, 2)) /
+                               np.abs(u_c).max(axis=(1, 2))))
+        out["parity_vs_reference"] = {
+            "n_bit_exact_all_iterations": bool(np.array_equal(n_g, n_c)),
+            "max_ubi_rel_err": ubi_err, "ubi_within_1e-10": ubi_err <= 1e-10,
+            "max_mean_rel_err_first_call": float(rel[0].max()),
+            "max_mean_rel_err_later_calls": float(rel[1:].max()) if n_iter > 1 else None,
+            "sample": "all %d grains x %d iterations against the reference C's "
+                      "score_and_refine loop" % (K, n_iter)}
     # adjoint on the same lists (translations zero), then 50 joint steps
     prob = adjoint.JointProblem(T(pk["sc"]), T(pk["fc"]), T(pk["omega"]), pars["omegasign"],
                                 idx_d, off_d)
@@ -537,13 +583,16 @@ def aux_geometry(threads, with_cpu, N=8_000_000, seed=7):
     return res
 
 
-def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
+def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_peaks=50_000):
     """Large scan: compute_gv per grain translation + score_and_assign for
     K grains x N peaks (per-peak dty, 15% spurious) in one fused launch,
     then n_iter captured score_and_refine iterations on the assigned peaks.
-    The CPU leg runs the reference's compute_gv + score_and_assign for the
-    first `cpu_grains` grains over all peaks (its own OpenMP loops, as
-    assignlabels calls them) and reports the per-pair rate."""
+    Timed with the parity mode's sin/cos; the fast mode's assignment is run
+    beside it (untimed) and compared.  The CPU leg runs the reference's
+    compute_gv + score_and_assign for every grain over a seeded sample of
+    `cpu_peaks` peaks (its own OpenMP loops, as assignlabels calls them),
+    reports the per-pair rate and checks the GPU's labels and drlv2 on
+    those peaks."""
     import numpy as np
     import torch
     from imaged11_amd import drivers, geometry, refine, simulate
@@ -554,7 +603,16 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
     gen_s = time.perf_counter() - t0
     pk = wl["peaks"]
     T = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
-    scan = drivers.Scan(T(pk["sc"]), T(pk["fc"]), T(pk["omega"]), wl["pars"], dty=T(pk["dty"]))
+    setup = {}
+    scans = {}
+    for mode in ("parity", "fast"):  # parity: sin/cos of omega from the host's glibc
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        scans[mode] = drivers.Scan(T(pk["sc"]), T(pk["fc"]), T(pk["omega"]), wl["pars"],
+                                   dty=T(pk["dty"]), parity=mode == "parity")
+        torch.cuda.synchronize()
+        setup[mode] = (time.perf_counter() - t0) * 1e3
+    scan = scans["parity"]
     ubis, trans = T(wl["ubi"]), T(wl["trans"])
     drivers.assignlabels(scan, ubis[:64].contiguous(), trans[:64].contiguous(), 0.05)  # warm
     torch.cuda.synchronize()
@@ -564,6 +622,21 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
     b.record()
     torch.cuda.synchronize()
     assign_ms = a.elapsed_time(b)
+    # the same assignment from the fast mode's sin/cos (outside the timed region)
+    lab_f, d2_f, cnt_f = drivers.assignlabels(scans["fast"], ubis, trans, 0.05)
+    lab_p_h, d2_p_h = labels.cpu().numpy(), drlv2.cpu().numpy()
+    lab_f_h, d2_f_h = lab_f.cpu().numpy(), d2_f.cpu().numpy()
+    relf = np.abs(d2_f_h - d2_p_h) / np.maximum(np.abs(d2_p_h), 1e-300)
+    fast_vs_parity = {
+        "labels_equal": bool(np.array_equal(lab_f_h, lab_p_h)),
+        "counts_equal": bool(torch.equal(cnt_f, counts)),
+        "drlv2_bit_equal_fraction": float(np.mean(d2_f_h == d2_p_h)),
+        "drlv2_max_rel_diff": float(relf.max()),
+        "drlv2_peaks_rel_diff_above_1e-12": int((relf > 1e-12).sum()),
+        "sincos_bit_equal_fraction": float(
+            (torch.eq(scans["fast"].so, scan.so) & torch.eq(scans["fast"].co, scan.co))
+            .double().mean())}
+    del scans["fast"], lab_f, d2_f
     gvl, idx, off = drivers.grain_gvectors(scan, trans, labels, K)
     # the captured iterations in the autotuner's launch shape for these lists
     best = refine.autotune(ubis, gvl, None, off, 0.05, n_iter, reps=2)
@@ -574,12 +647,14 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
                            long_min=best.get("long_min", 0))
     refine_ms = _events_ms(lambda: (work.copy_(ubis), g.replay()), reps=3)
     own = pk["grain"]
-    lab = labels.cpu().numpy()
+    lab = lab_p_h
     # per peak: xl 24 B + sin/cos 16 B + (drlv2, label) read 12 B and
     # written 12 B; per grain: UBI 72 B + translation 24 B + label 4 B +
     # count 4 B
     algo_bytes = 64 * N + 104 * K
     out = {"grains": K, "peaks": N, "gen_s": gen_s, "assign_ms": assign_ms,
+           "trig_mode_timed": "parity (sin/cos of omega from the host's glibc)",
+           "scan_setup_ms": setup,
            "assign_pairs_per_s": K * N / (assign_ms * 1e-3),
            "assign_algorithmic_hbm_gb_s": algo_bytes / (assign_ms * 1e-3) / 1e9,
            "assign_valu_ops_per_pair": SCAN_VALU_OPS_PER_PAIR,
@@ -587,6 +662,7 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
                SCAN_VALU_OPS_PER_PAIR * K * N / (assign_ms * 1e-3) / 39.3e12,
            "assigned": int((lab >= 0).sum()),
            "own_label_fraction": float(np.mean(lab[own >= 0] == own[own >= 0])),
+           "fast_mode_vs_parity_mode": fast_vs_parity,
            "refine_iterations": n_iter, "refine_list_entries": int(gvl.shape[0]),
            "refine_graph_ms": refine_ms,
            "refine_graph_shape": {"lanes_per_grain": best["lanes_per_grain"],
@@ -600,20 +676,33 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_grains=24):
             ref, kind = oracle.load_oracle(), "port"
         else:
             ref.set_threads(threads)
-        xl = scan.xl.cpu().numpy()
+        # every grain over a seeded sample of peaks: the reference's
+        # assignlabels loop (compute_gv with the grain's translation, then
+        # score_and_assign) from labels -1 / drlv2 1.  A peak's label and
+        # drlv2 depend on that peak alone, so the sample's final state must
+        # equal the GPU's on the same peaks, bit for bit.
+        rs = np.random.default_rng(55)
+        sel = np.sort(rs.choice(N, size=min(cpu_peaks, N), replace=False))
+        xl = scan.xl.cpu().numpy()[sel]
+        om = np.ascontiguousarray(pk["omega"][sel])
         sign, wvln, wedge, chi = geometry.pack_gv(wl["pars"])
-        d = np.ones(N)
-        lb = np.full(N, -1, np.int32)
-        gv = np.empty((N, 3))
+        d = np.ones(len(sel))
+        lb = np.full(len(sel), -1, np.int32)
+        gv = np.empty((len(sel), 3))
         t0 = time.perf_counter()
-        for k in range(cpu_grains):
-            ref.compute_gv(xl, pk["omega"], sign, wvln, wedge, chi, wl["trans"][k], gv)
+        for k in range(K):
+            ref.compute_gv(xl, om, sign, wvln, wedge, chi, wl["trans"][k], gv)
             ref.score_and_assign(wl["ubi"][k], gv, 0.05, d, lb, k)
         cpu_s = time.perf_counter() - t0
-        out["cpu"] = {"kind": kind, "threads": threads, "grains": cpu_grains,
-                      "seconds": cpu_s, "pairs_per_s": cpu_grains * N / cpu_s,
-                      "sample": "first %d grains x all %d peaks" % (cpu_grains, N)}
+        out["cpu"] = {"kind": kind, "threads": threads, "grains": K, "peaks": len(sel),
+                      "seconds": cpu_s, "pairs_per_s": K * len(sel) / cpu_s,
+                      "sample": "all %d grains x a seeded sample of %d peaks" % (K, len(sel))}
         out["speedup_vs_cpu"] = out["assign_pairs_per_s"] / out["cpu"]["pairs_per_s"]
+        out["parity_vs_reference"] = {
+            "labels_bit_exact": bool(np.array_equal(lab_p_h[sel], lb)),
+            "drlv2_bit_exact": bool(np.array_equal(d2_p_h[sel].view(np.int64), d.view(np.int64))),
+            "fast_mode_labels_equal": bool(np.array_equal(lab_f_h[sel], lb)),
+            "sample": "the CPU leg's %d peaks, final state after all %d grains" % (len(sel), K)}
     g.close()
     return out
 
@@ -660,9 +749,14 @@ def main():
         if ev is not None:
             ev[1].record(stream)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+    t_warm = time.perf_counter()
+    warm_steps = 0
+    while warm_steps < args.warmup or time.perf_counter() - t_warm < args.warmup_s:
+        for _ in range(10):
+            step()
+        warm_steps += 10
+        torch.cuda.synchronize()
+    warm_s = time.perf_counter() - t_warm
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(args.steps)]
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -697,6 +791,10 @@ def main():
     # difference from the host libm's sin/cos could flip a count
     boundary = int(batched.boundary_pairs(ubis, batched.sf2gv(sc, fc, om, pars),
                                           args.tol).sum())
+    # the same counts from the parity mode's g-vectors (sin/cos of omega from
+    # the host's glibc: the reference's compute_gv bit for bit), untimed
+    parity_counts = batched.score_many(ubis, batched.sf2gv(sc, fc, om, pars, parity=True),
+                                       args.tol).cpu().numpy()
 
     out = {
         "metric": "grain\u00d7peak pairs/s scored; batched+adjoint refine vs per-grain loop, "
@@ -706,6 +804,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_s": warm_s,
+        "warmup_steps_run": warm_steps,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -716,6 +816,8 @@ def main():
                                "%d grains x %d peaks, tol %g" % (args.grains, args.peaks, args.tol),
                    "grains": args.grains, "peaks": args.peaks, "tol": args.tol,
                    "boundary_pairs_rel_1e-12": boundary,
+                   "trig_mode_timed": "fast (device sincos)",
+                   "counts_equal_parity_mode": bool(np.array_equal(parity_counts, gpu_counts)),
                    "value_is": "the pairs/s of this workload; the batched+adjoint refine "
                                "against the per-grain GPU loop is "
                                "aux.refine_5000_grains.speedup_refine_plus_adjoint_vs_gpu_loop"},
@@ -736,9 +838,10 @@ def main():
         },
         "cpu_baseline": None,
     }
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads, cpu_facts = host_cpus(args.cpu_threads)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args.tol, threads, gpu_counts)
+        out["cpu_baseline"]["host"] = cpu_facts
     if not args.no_aux:
         out["aux"] = {"refine_5000_grains": aux_refine_adjoint(threads, not args.no_cpu),
                       "scan_20000_grains": aux_scan(threads, not args.no_cpu),

@@ -57,6 +57,10 @@ SIGNATURES = {
     "gs_compute_gv_dev": (_i, [_vp, _vp, _d, _d, _d, _d, _dp, _vp, _i, _vp]),
     "gs_compute_gv_trig_dev": (_i, [_vp, _vp, _vp, _dp, _d, _dp, _vp, _i, _vp]),
     "gs_sincos_omega": (_i, [_vp, _d, _vp, _vp, _i, _vp]),
+    "gs_sincos_omega_host": (_i, [_vp, _d, _vp, _vp, _i]),
+    "gs_wedge_chi_matrix_host": (_i, [_d, _d, _vp]),
+    "gs_sincos_omega_parity": (_i, [_vp, _d, _vp, _vp, _i, _vp]),
+    "gs_sf2gv_trig": (_i, [_vp, _vp, _vp, _vp, _dp, _dp, _dp, _dp, _d, _dp, _vp, _vp, _i, _vp]),
     "gs_sf2gv": (_i, [_vp, _vp, _vp, _dp, _dp, _dp, _d, _d, _d, _d, _dp, _vp, _vp, _vp, _vp, _i,
                       _vp]),
     "gs_refine_lists": (_i, [_vp, _i, _vp, _i, _vp, ctypes.c_longlong, _vp, _vp, _d, _i, _vp, _vp,
@@ -79,7 +83,8 @@ SIGNATURES = {
     "gs_joint_hkl": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _i, _vp, _vp, ctypes.c_longlong,
                           _vp, _vp]),
     "gs_joint_adjoint": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _i, _vp, ctypes.c_longlong,
-                              _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
+                              _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp,
+                              _vp]),
     "gs_graph_destroy": (_i, [_vp]),
     "closest": (None, [_vp, _vp, _ip, _dp, _i, _i]),
     "quickorient": (None, [_vp, _vp]),

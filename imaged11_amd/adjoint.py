@@ -28,14 +28,16 @@ FREE = ("distance", "y_center", "z_center", "tilt_x", "tilt_y", "tilt_z", "wavel
         "chi")
 
 
-def pack_geometry(params, fixed):
+def pack_geometry(params, fixed, parity=False):
     """26-vector [p(4), r(9), dist(3), wavelength, mat(9)] from a dict of
     0-d tensors ``params`` (any of FREE) and plain numbers ``fixed`` (the
     rest of an ImageD11 parameter dict: pixel sizes, o11..o22, ...).
 
     Same conventions as transform.Ctransform (``ImageD11/transform.py:
     685-693``) and the wedge/chi matrix of compute_gv
-    (``src/cdiffraction.c:119-142``); differentiable in ``params``."""
+    (``src/cdiffraction.c:119-142``); differentiable in ``params``.
+    parity=True gives mat the values of the reference's glibc sincos
+    (``batched.wedge_chi_matrix``) while its gradient stays torch's."""
     q = geometry.full_parameters(fixed)
     dev = next(iter(params.values())).device if params else torch.device("cpu")
 
@@ -70,6 +72,10 @@ def pack_geometry(params, fixed):
                      torch.stack([zero, torch.cos(c), -torch.sin(c)]),
                      torch.stack([zero, torch.sin(c), torch.cos(c)])])
     mat = (W @ C).reshape(9)
+    if parity:
+        exact = torch.as_tensor(batched.wedge_chi_matrix(float(P("wedge")), float(P("chi"))),
+                                dtype=torch.float64, device=dev).reshape(9)
+        mat = mat + (exact - mat).detach()
     return torch.cat([p, r, dist, P("wavelength").reshape(1), mat])
 
 
@@ -82,9 +88,13 @@ class JointProblem:
     copied once per list entry in list order, so each evaluation streams
     them instead of gathering four cache lines per entry; with an
     out-of-range index the lists stay indexed (such entries are skipped).
-    Either way the per-peak data are those of the construction."""
+    Either way the per-peak data are those of the construction.
 
-    def __init__(self, sc, fc, omega, omegasign, idx, off):
+    parity=True takes sin/cos of omega from the host's glibc, the values
+    the reference's compute_gv uses (``batched.sincos_omega``); otherwise
+    the device's correctly rounded sincos."""
+
+    def __init__(self, sc, fc, omega, omegasign, idx, off, parity=False):
         batched._need(sc, torch.float64, "sc", (None,))
         n = sc.shape[0]
         batched._need(fc, torch.float64, "fc", (n,))
@@ -92,7 +102,8 @@ class JointProblem:
         batched._need(idx, torch.int32, "idx", (None,))
         batched._need(off, torch.int64, "off", (None,))
         self.sc, self.fc = sc, fc
-        self.so, self.co = batched.sincos_omega(omega, omegasign)
+        self.so, self.co = batched.sincos_omega(omega, omegasign, parity=parity)
+        self.parity = bool(parity)
         self.idx, self.off = idx, off
         self.rows = None
         m = idx.numel()
@@ -106,6 +117,7 @@ class JointProblem:
         self.order = batched.list_order(off)
         self.hkl = torch.zeros((idx.numel(), 3), dtype=torch.int32, device=sc.device)
         self.queue = torch.zeros(1, dtype=torch.int32, device=sc.device)
+        self.fin = torch.empty(128 * (GEO + 1), dtype=torch.float64, device=sc.device)
         dev = sc.device
         self.loss_g = torch.empty(self.K, dtype=torch.float64, device=dev)
         self.geo_part = torch.empty((self.K, GEO), dtype=torch.float64, device=dev)
@@ -134,7 +146,7 @@ class JointProblem:
             return self.rows, self.idx.numel(), None
         return (self.sc, self.fc, self.so, self.co), self.sc.shape[0], dptr(self.idx)
 
-    def evaluate(self, geo, ubis, trans, gspace=False, want_sums=False, lanes_per_grain=64):
+    def evaluate(self, geo, ubis, trans, gspace=False, want_sums=False, lanes_per_grain=32):
         """One fused forward + reverse pass.  Returns (loss, dL/dUBI (K,3,3),
         dL/dt (K,3), dL/dgeo (26,), sums) with sums = (K,15) Gauss-Newton
         sums or None.  In g-space mode the loss is 1/2 sum ||UB.h - G||^2
@@ -155,7 +167,7 @@ class JointProblem:
             dptr(self.off), dptr(self.hkl), dptr(self.order), int(bool(gspace)),
             dptr(self.loss_g), dptr(gu), dptr(gt), dptr(self.geo_part),
             dptr(sums) if want_sums else None, dptr(self.out), int(lanes_per_grain),
-            dptr(self.queue), stream_ptr()), "joint adjoint")
+            dptr(self.queue), dptr(self.fin), stream_ptr()), "joint adjoint")
         out = self.out.clone()
         return out[GEO], gu, gt, out[:GEO], sums
 
@@ -179,7 +191,7 @@ class JointLoss(torch.autograd.Function):
 def joint_loss(problem, ubis, trans, params, fixed):
     """L as a differentiable function of UBIs, translations and the
     physical geometry parameters in ``params`` (dict of 0-d tensors)."""
-    geo = pack_geometry(params, fixed)
+    geo = pack_geometry(params, fixed, parity=problem.parity)
     return JointLoss.apply(ubis, trans, geo, problem)
 
 
@@ -220,7 +232,7 @@ def refine_joint(problem, ubis, trans, params, fixed, steps=50, lr=1e-3, scales=
     history = []
     for _ in range(steps):
         cur = {k: start[k] + scale[k] * x[k] for k in names}
-        geo = pack_geometry(cur, fixed)
+        geo = pack_geometry(cur, fixed, parity=problem.parity)
         loss, gu, gt, gg, sums = problem.evaluate(geo, ubis, trans, want_sums=True)
         ubis = gauss_newton_ubis(sums, ubis)
         g_host = gg.cpu()

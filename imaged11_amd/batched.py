@@ -5,13 +5,15 @@ grain (``indexing.py:818-820`` for score, ``refinegrains.py:718-734`` for
 compute_gv + score_and_assign, ``refinegrains.py:459-498`` for
 score_and_refine), these functions run the whole loop in one kernel launch
 with the same per-call semantics.  Every function takes CUDA (HIP) tensors,
-enqueues on torch's current stream and returns without synchronising.
+enqueues on torch's current stream and returns without synchronising,
+except the parity mode's host trig (``parity=True``), which returns once
+its values are on the device.
 """
 import numpy as np
 import torch
 
 from . import _lib, geometry
-from ._lib import dptr, darr, stream_ptr
+from ._lib import dptr, darr, hptr, stream_ptr
 
 
 def _need(t, dtype, name, shape=None):
@@ -108,14 +110,20 @@ def xlylzl(sc, fc, p, r, dist, out=None):
     return out
 
 
-def gvectors(xl, omega, omegasign, wvln, wedge, chi, t=(0.0, 0.0, 0.0), out=None):
-    """compute_gv on the device: (N,3) lab positions, (N,) omega -> (N,3)."""
+def gvectors(xl, omega, omegasign, wvln, wedge, chi, t=(0.0, 0.0, 0.0), out=None,
+             parity=False):
+    """compute_gv on the device: (N,3) lab positions, (N,) omega -> (N,3).
+    parity=True takes sin/cos from the host's glibc (see sincos_omega):
+    the reference's g-vectors, bit for bit."""
     lib = _ready()
     _need(xl, torch.float64, "xlylzl", (None, 3))
     n = xl.shape[0]
     _need(omega, torch.float64, "omega", (n,))
     if out is None:
         out = torch.empty((n, 3), dtype=torch.float64, device=xl.device)
+    if parity:
+        so, co = sincos_omega(omega, omegasign, parity=True)
+        return gvectors_trig(xl, so, co, wedge_chi_matrix(wedge, chi), wvln, t=t, out=out)
     _lib.check(lib.gs_compute_gv_dev(dptr(xl), dptr(omega), float(omegasign), float(wvln),
                                      float(wedge), float(chi), darr(t), dptr(out), n, stream_ptr()),
                "gvectors")
@@ -124,8 +132,7 @@ def gvectors(xl, omega, omegasign, wvln, wedge, chi, t=(0.0, 0.0, 0.0), out=None
 
 def gvectors_trig(xl, so, co, mat, wvln, t=(0.0, 0.0, 0.0), out=None):
     """compute_gv arithmetic with the per-peak sin/cos and wedge/chi matrix
-    supplied (used to check the device arithmetic independently of how the
-    sines were evaluated)."""
+    supplied (the parity mode's, or any other)."""
     lib = _ready()
     _need(xl, torch.float64, "xlylzl", (None, 3))
     n = xl.shape[0]
@@ -139,22 +146,44 @@ def gvectors_trig(xl, so, co, mat, wvln, t=(0.0, 0.0, 0.0), out=None):
     return out
 
 
-def sincos_omega(omega, omegasign):
-    """Per-peak sin, cos of ((PI/180)*omega)*omegasign."""
+def sincos_omega(omega, omegasign, parity=False):
+    """Per-peak sin, cos of ((PI/180)*omega)*omegasign.
+
+    parity=False (the fast mode): the device's correctly rounded sincos.
+    parity=True: glibc's sincos on the host, the function the reference's
+    compute_gv calls (``src/cdiffraction.c:146-147``), so the values are
+    the reference's bit for bit; omega goes down to the host and (so, co)
+    come back up (16 B/peak over PCIe, once per peak table).  The two
+    differ in the last bit for a few tenths of a percent of angles, where
+    glibc is not correctly rounded."""
     lib = _ready()
     _need(omega, torch.float64, "omega", (None,))
     so = torch.empty_like(omega)
     co = torch.empty_like(omega)
-    _lib.check(lib.gs_sincos_omega(dptr(omega), float(omegasign), dptr(so), dptr(co),
-                                   omega.shape[0], stream_ptr()), "sincos_omega")
+    fn = lib.gs_sincos_omega_parity if parity else lib.gs_sincos_omega
+    _lib.check(fn(dptr(omega), float(omegasign), dptr(so), dptr(co), omega.shape[0],
+                  stream_ptr()), "sincos_omega")
     return so, co
 
 
-def sf2gv(sc, fc, omega, pars, t=(0.0, 0.0, 0.0), keep=False):
+def wedge_chi_matrix(wedge, chi):
+    """The (3,3) wedge/chi matrix of compute_gv (``src/cdiffraction.c:119-142``)
+    from glibc's sincos on the host: the reference's values, bit for bit.
+    Used by both modes (the per-peak omega trig is what the modes choose)."""
+    m = np.zeros(9)
+    _lib.check(_ready().gs_wedge_chi_matrix_host(float(wedge), float(chi), hptr(m)),
+               "wedge_chi_matrix")
+    return m.reshape(3, 3)
+
+
+def sf2gv(sc, fc, omega, pars, t=(0.0, 0.0, 0.0), keep=False, parity=False):
     """Raw peaks (sc, fc, omega) to g-vectors in one fused pass, with the
     detector/goniometer of an ImageD11 parameter dict (``Ctransform.sf2gv``,
     ``ImageD11/transform.py``).  keep=True also returns the lab positions
-    and per-peak sin/cos: (gv, xl, so, co)."""
+    and per-peak sin/cos: (gv, xl, so, co).  parity=True: sin/cos of omega
+    and the wedge/chi matrix from the host's glibc (see sincos_omega), so
+    the g-vectors are the reference's compute_xlylzl + compute_gv, bit for
+    bit; otherwise the device sincos in the same pass."""
     lib = _ready()
     _need(sc, torch.float64, "sc", (None,))
     n = sc.shape[0]
@@ -163,6 +192,14 @@ def sf2gv(sc, fc, omega, pars, t=(0.0, 0.0, 0.0), keep=False):
     p, r, dist = geometry.pack_xlylzl(pars)
     sign, wvln, wedge, chi = geometry.pack_gv(pars)
     gv = torch.empty((n, 3), dtype=torch.float64, device=sc.device)
+    if parity:
+        so, co = sincos_omega(omega, sign, parity=True)
+        xl = torch.empty((n, 3), dtype=torch.float64, device=sc.device) if keep else None
+        _lib.check(lib.gs_sf2gv_trig(dptr(sc), dptr(fc), dptr(so), dptr(co), darr(p), darr(r),
+                                     darr(dist), darr(np.ravel(wedge_chi_matrix(wedge, chi))),
+                                     float(wvln), darr(t), dptr(xl) if keep else None, dptr(gv),
+                                     n, stream_ptr()), "sf2gv")
+        return (gv, xl, so, co) if keep else gv
     xl = so = co = None
     if keep:
         xl = torch.empty((n, 3), dtype=torch.float64, device=sc.device)

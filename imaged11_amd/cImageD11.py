@@ -268,30 +268,62 @@ def refine_assigned(ubi, gv, labels, label):
     return npk.value, m.value
 
 
-def compute_gv(xlylzl, omega, omegasign, wvln, wedge, chi, t, gv):
-    """g-vectors from lab-frame spot positions (``src/cdiffraction.c:112-168``),
-    written into gv (n, 3)."""
+def _gv_call(what, width, xlylzl, omega, omegasign, wvln, wedge, chi, t, out, parity):
+    """compute_gv (width 3) / compute_geometry (width 6) on host arrays or
+    device tensors.  parity=True: sin/cos of omega, wedge and chi from the
+    host's glibc, as the reference computes them, so the output is the
+    reference's bit for bit (tth and eta of compute_geometry aside, which
+    use the device atan2); parity=False: the device sincos."""
     tt = _in_array(np.asarray(t, dtype=np.float64).reshape(-1), np.float64, (3,), "t")
-    if _is_dev(gv) or _is_dev(xlylzl) or _is_dev(omega):
-        x = _dev_in(xlylzl, torch.float64, (None, 3), "xlylzl")
+    on_dev = _is_dev(out) or _is_dev(xlylzl) or _is_dev(omega)
+    if on_dev or not parity:
+        host_out = None if on_dev else _inout_array(out, np.float64, (None, width), what)
+        dev = next((a.device for a in (out, xlylzl, omega) if _is_dev(a)), torch.device("cuda"))
+        x = _dev_in(xlylzl, torch.float64, (None, 3), "xlylzl", device=dev)
         n = x.shape[0]
-        w = _dev_in(omega, torch.float64, (n,), "omega", device=x.device)
-        out = _dev_inout(gv, torch.float64, (n, 3), "gv")
+        w = _dev_in(omega, torch.float64, (n,), "omega", device=dev)
+        if on_dev:
+            o = _dev_inout(out, torch.float64, (n, width), what)
+        else:
+            if host_out.shape[0] != n:
+                raise ValueError("%s: expected shape %s, got %s" % (what, (n, width),
+                                                                     host_out.shape))
+            o = torch.empty((n, width), dtype=torch.float64, device=x.device)
         lib = _lib_ready()
-        _lib.check(lib.gs_compute_gv_dev(dptr(x), dptr(w), float(omegasign), float(wvln),
-                                         float(wedge), float(chi), darr(tt), dptr(out), n,
-                                         stream_ptr()), "compute_gv")
+        if parity:
+            from . import batched
+            so, co = batched.sincos_omega(w, omegasign, parity=True)
+            mat = darr(batched.wedge_chi_matrix(wedge, chi).ravel())
+            fn = lib.gs_compute_gv_trig_dev if width == 3 else lib.gs_compute_geometry_trig_dev
+            _lib.check(fn(dptr(x), dptr(so), dptr(co), mat, float(wvln), darr(tt), dptr(o), n,
+                          stream_ptr()), what)
+        else:
+            fn = lib.gs_compute_gv_dev if width == 3 else lib.gs_compute_geometry_dev
+            _lib.check(fn(dptr(x), dptr(w), float(omegasign), float(wvln), float(wedge),
+                          float(chi), darr(tt), dptr(o), n, stream_ptr()), what)
         torch.cuda.current_stream().synchronize()
+        if host_out is not None:
+            host_out[...] = o.cpu().numpy()
         return None
     x = _in_array(xlylzl, np.float64, (None, 3), "xlylzl")
     n = x.shape[0]
     w = _in_array(omega, np.float64, (n,), "omega")
-    out = _inout_array(gv, np.float64, (n, 3), "gv")
+    o = _inout_array(out, np.float64, (n, width), what)
     lib = _lib_ready()
-    lib.compute_gv(hptr(x), hptr(w), float(omegasign), float(wvln), float(wedge), float(chi),
-                   hptr(tt), hptr(out), n)
-    _lib.check_host_call("compute_gv")
+    fn = lib.compute_gv if width == 3 else lib.compute_geometry
+    fn(hptr(x), hptr(w), float(omegasign), float(wvln), float(wedge), float(chi), hptr(tt),
+       hptr(o), n)
+    _lib.check_host_call(what)
     return None
+
+
+def compute_gv(xlylzl, omega, omegasign, wvln, wedge, chi, t, gv, parity=True):
+    """g-vectors from lab-frame spot positions (``src/cdiffraction.c:112-168``),
+    written into gv (n, 3).  parity=True (the default): the reference's
+    values bit for bit (sin/cos from the host's glibc, as the reference
+    computes them); parity=False: the device sincos, faster for device
+    tensors, last-bit differences for a few tenths of a percent of peaks."""
+    return _gv_call("compute_gv", 3, xlylzl, omega, omegasign, wvln, wedge, chi, t, gv, parity)
 
 
 def compute_xlylzl(s, f, p, r, dist, xlylzl):
@@ -355,31 +387,14 @@ def quickorient(ubi, bt):
     return None
 
 
-def compute_geometry(xlylzl, omega, omegasign, wvln, wedge, chi, t, out):
+def compute_geometry(xlylzl, omega, omegasign, wvln, wedge, chi, t, out, parity=True):
     """Per peak (tth, eta, ds, gx, gy, gz) from lab-frame positions and omega,
     written into out (n, 6) (``src/cdiffraction.c:31-94``; the
-    ``updateGeometry`` pass of ``ImageD11/columnfile.py:469-504``)."""
-    tt = _in_array(np.asarray(t, dtype=np.float64).reshape(-1), np.float64, (3,), "t")
-    if _is_dev(out) or _is_dev(xlylzl) or _is_dev(omega):
-        x = _dev_in(xlylzl, torch.float64, (None, 3), "xlylzl")
-        n = x.shape[0]
-        w = _dev_in(omega, torch.float64, (n,), "omega", device=x.device)
-        o = _dev_inout(out, torch.float64, (n, 6), "out")
-        lib = _lib_ready()
-        _lib.check(lib.gs_compute_geometry_dev(dptr(x), dptr(w), float(omegasign), float(wvln),
-                                               float(wedge), float(chi), darr(tt), dptr(o), n,
-                                               stream_ptr()), "compute_geometry")
-        torch.cuda.current_stream().synchronize()
-        return None
-    x = _in_array(xlylzl, np.float64, (None, 3), "xlylzl")
-    n = x.shape[0]
-    w = _in_array(omega, np.float64, (n,), "omega")
-    o = _inout_array(out, np.float64, (n, 6), "out")
-    lib = _lib_ready()
-    lib.compute_geometry(hptr(x), hptr(w), float(omegasign), float(wvln), float(wedge),
-                         float(chi), hptr(tt), hptr(o), n)
-    _lib.check_host_call("compute_geometry")
-    return None
+    ``updateGeometry`` pass of ``ImageD11/columnfile.py:469-504``).
+    parity as for compute_gv (ds and g bit for bit in parity mode; tth and
+    eta use the device's atan2, within a few ulps of glibc's)."""
+    return _gv_call("compute_geometry", 6, xlylzl, omega, omegasign, wvln, wedge, chi, t, out,
+                    parity)
 
 
 def compute_xlylzl_xpos_variable(s, f, p, r, dist, xpos, xlylzl):

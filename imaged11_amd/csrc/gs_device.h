@@ -247,17 +247,17 @@ GS_HD __forceinline__ gs_dd gs_dd_sub(gs_dd a, gs_dd b) {
 // Cody-Waite reduction by pi/2 held in four pieces (exact for |k| < 2^20),
 // then Taylor series on |r| <= pi/4 with the leading terms in
 // double-double.  Arguments beyond 2^19*pi use the device library.
-GS_HD __noinline__ void gs_sincos(double x, double* s_out, double* c_out) {
-    if (!(fabs(x) < 823549.6)) {  // 2^19 * pi, also catches NaN/Inf
-        *s_out = sin(x);
-        *c_out = cos(x);
-        return;
-    }
-    if (x == 0.0) {  // keeps the sign of a zero argument
-        *s_out = x;
-        *c_out = 1.0;
-        return;
-    }
+// Inlined and returned by value, so the pair stays in registers (a call
+// through pointers put it in per-lane scratch memory).
+struct gs_sc {
+    double s, c;
+};
+
+GS_HD __forceinline__ gs_sc gs_sincos_v(double x) {
+    if (!(fabs(x) < 823549.6))  // 2^19 * pi, also catches NaN/Inf
+        return {sin(x), cos(x)};
+    if (x == 0.0)  // keeps the sign of a zero argument
+        return {x, 1.0};
     double kf = rint(x * GS_2_OVER_PI);
     int q = ((int)kf) & 3;
     gs_dd r;
@@ -287,33 +287,47 @@ GS_HD __noinline__ void gs_sincos(double x, double* s_out, double* c_out) {
     gs_dd cr = gs_dd_add_small({1.0, 0.0}, gs_dd_mul(r2, C));
     double sv = sr.hi + sr.lo;
     double cv = cr.hi + cr.lo;
-    double so, co;
     switch (q) {
-        case 0: so = sv; co = cv; break;
-        case 1: so = cv; co = -sv; break;
-        case 2: so = -sv; co = -cv; break;
-        default: so = -cv; co = sv; break;
+        case 0: return {sv, cv};
+        case 1: return {cv, -sv};
+        case 2: return {-sv, -cv};
+        default: return {-cv, sv};
     }
-    *s_out = so;
-    *c_out = co;
+}
+
+GS_HD __forceinline__ void gs_sincos(double x, double* s_out, double* c_out) {
+    const gs_sc r = gs_sincos_v(x);
+    *s_out = r.s;
+    *c_out = r.c;
 }
 
 // ---------------------------------------------------------------------------
 // Diffraction geometry (src/cdiffraction.c).
 
+// the wedge and chi arguments as the reference forms them: sin(wedge * RAD)
+// with RAD = PI / 180.0 unparenthesised is sin((wedge*PI)/180.0)
+// (src/cdiffraction.c:7,119-120,130-131)
+GS_HD __forceinline__ double gs_wedge_rad(double a) { return a * GS_PI / 180.0; }
+
 // mat = W(wedge).C(chi), formed with the operand order of the matmat macro
 // (src/cdiffraction.h:20-29) from sin((wedge*PI)/180.0) etc.
 // (src/cdiffraction.c:119-142).
-GS_HD __forceinline__ void gs_wedge_chi_matrix(double wedge, double chi, double* mat) {
-    double sw, cw, sc, cc;
-    gs_sincos(wedge * GS_PI / 180.0, &sw, &cw);
-    gs_sincos(chi * GS_PI / 180.0, &sc, &cc);
+// The matrix from the four sines and cosines.
+GS_HD __forceinline__ void gs_wedge_chi_from_trig(double sw, double cw, double sc, double cc,
+                                                  double* mat) {
     double w[9] = {cw, 0.0, -sw, 0., 1.0, 0., sw, 0.0, cw};
     double c[9] = {1., 0.0, 0., 0., cc, -sc, 0., sc, cc};
     for (int col = 0; col < 3; col++)
         for (int row = 0; row < 3; row++)
             mat[3 * col + row] = c[row] * w[3 * col + 0] + c[row + 3] * w[3 * col + 1] +
                                  c[row + 6] * w[3 * col + 2];
+}
+
+GS_HD __forceinline__ void gs_wedge_chi_matrix(double wedge, double chi, double* mat) {
+    double sw, cw, sc, cc;
+    gs_sincos(gs_wedge_rad(wedge), &sw, &cw);
+    gs_sincos(gs_wedge_rad(chi), &sc, &cc);
+    gs_wedge_chi_from_trig(sw, cw, sc, cc, mat);
 }
 
 // the omega argument exactly as the reference forms it:

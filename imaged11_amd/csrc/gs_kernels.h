@@ -849,6 +849,24 @@ __global__ __launch_bounds__(GS_BLOCK) void k_sincos_omega(const double* __restr
     co[i] = b;
 }
 
+// A block's GS_BLOCK rows of three doubles, row t held by thread t (rows
+// past n, i.e. t >= n - i0, are not stored), written to out[3*i0 ..] through
+// LDS: consecutive threads store consecutive doubles, so each wave writes
+// whole contiguous 128-byte lines instead of three strided 8-byte stores per
+// lane.  Every thread of the block must call it (it synchronises).
+__device__ __forceinline__ void gs_store_rows3(double* __restrict__ out, long long i0, long long n,
+                                               double a, double b, double c,
+                                               double* __restrict__ lds) {
+    const int t = threadIdx.x;
+    __syncthreads();  // lds may still be read by a previous call
+    lds[3 * t + 0] = a;
+    lds[3 * t + 1] = b;
+    lds[3 * t + 2] = c;
+    __syncthreads();
+    const long long rows = n - i0 < GS_BLOCK ? n - i0 : GS_BLOCK;
+    for (int j = t; j < 3 * rows; j += GS_BLOCK) out[3 * i0 + j] = lds[j];
+}
+
 // compute_gv (src/cdiffraction.c:112-168).  TRIG: sin/cos per peak are
 // read from so/co instead of being evaluated from omega.
 template <bool TRIG>
@@ -859,55 +877,60 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv(const double* __restrict__ xl,
                                                  gs_goniometer gp, double* __restrict__ gv, int n) {
     __shared__ double smat[9];
     __shared__ double sds;
+    __shared__ double rows[3 * GS_BLOCK];
     gs_block_setup(gp, smat, &sds);
-    const long long i = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    double so, co;
-    if (TRIG) {
-        so = so_in[i];
-        co = co_in[i];
-    } else {
-        gs_sincos(gs_omega_rad(omega[i], gp.omegasign), &so, &co);
+    const long long i0 = (long long)blockIdx.x * GS_BLOCK;
+    const long long i = i0 + threadIdx.x;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if (i < n) {
+        double so, co;
+        if (TRIG) {
+            so = so_in[i];
+            co = co_in[i];
+        } else {
+            gs_sincos(gs_omega_rad(omega[i], gp.omegasign), &so, &co);
+        }
+        double mat[9];
+        for (int j = 0; j < 9; j++) mat[j] = smat[j];
+        gs_gv_one(xl[3 * i + 0], xl[3 * i + 1], xl[3 * i + 2], so, co, mat, sds, gp.t[0],
+                  gp.t[1], gp.t[2], g0, g1, g2);
     }
-    double mat[9];
-    for (int j = 0; j < 9; j++) mat[j] = smat[j];
-    double g0, g1, g2;
-    gs_gv_one(xl[3 * i + 0], xl[3 * i + 1], xl[3 * i + 2], so, co, mat, sds, gp.t[0], gp.t[1],
-              gp.t[2], g0, g1, g2);
-    gv[3 * i + 0] = g0;
-    gv[3 * i + 1] = g1;
-    gv[3 * i + 2] = g2;
+    gs_store_rows3(gv, i0, n, g0, g1, g2, rows);
 }
 
 // compute_xlylzl -> compute_gv fused: raw (s, f, omega) to g-vectors in one
 // pass; the lab positions and per-peak sin/cos are optionally stored too.
+// TRIG: sin/cos per peak are read from so_in/co_in (omega is not read).
+template <bool TRIG>
 __global__ __launch_bounds__(GS_BLOCK) void k_sf2gv(
     const double* __restrict__ s, const double* __restrict__ f, const double* __restrict__ omega,
-    gs_detector det, gs_goniometer gp, double* __restrict__ xl_out, double* __restrict__ so_out,
+    const double* __restrict__ so_in, const double* __restrict__ co_in, gs_detector det,
+    gs_goniometer gp, double* __restrict__ xl_out, double* __restrict__ so_out,
     double* __restrict__ co_out, double* __restrict__ gv, int n) {
     __shared__ double smat[9];
     __shared__ double sds;
+    __shared__ double rows[3 * GS_BLOCK];
     gs_block_setup(gp, smat, &sds);
-    const long long i = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    double x0, x1, x2;
-    gs_xlylzl_one(s[i], f[i], det.p, det.r, det.dist, x0, x1, x2);
-    double so, co;
-    gs_sincos(gs_omega_rad(omega[i], gp.omegasign), &so, &co);
-    double mat[9];
-    for (int j = 0; j < 9; j++) mat[j] = smat[j];
-    double g0, g1, g2;
-    gs_gv_one(x0, x1, x2, so, co, mat, sds, gp.t[0], gp.t[1], gp.t[2], g0, g1, g2);
-    if (xl_out) {
-        xl_out[3 * i + 0] = x0;
-        xl_out[3 * i + 1] = x1;
-        xl_out[3 * i + 2] = x2;
+    const long long i0 = (long long)blockIdx.x * GS_BLOCK;
+    const long long i = i0 + threadIdx.x;
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if (i < n) {
+        gs_xlylzl_one(s[i], f[i], det.p, det.r, det.dist, x0, x1, x2);
+        double so, co;
+        if (TRIG) {
+            so = so_in[i];
+            co = co_in[i];
+        } else {
+            gs_sincos(gs_omega_rad(omega[i], gp.omegasign), &so, &co);
+        }
+        double mat[9];
+        for (int j = 0; j < 9; j++) mat[j] = smat[j];
+        gs_gv_one(x0, x1, x2, so, co, mat, sds, gp.t[0], gp.t[1], gp.t[2], g0, g1, g2);
+        if (so_out) {
+            so_out[i] = so;
+            co_out[i] = co;
+        }
     }
-    if (so_out) {
-        so_out[i] = so;
-        co_out[i] = co;
-    }
-    gv[3 * i + 0] = g0;
-    gv[3 * i + 1] = g1;
-    gv[3 * i + 2] = g2;
+    if (xl_out) gs_store_rows3(xl_out, i0, n, x0, x1, x2, rows);  // uniform branch
+    gs_store_rows3(gv, i0, n, g0, g1, g2, rows);
 }

@@ -19,7 +19,10 @@
 #include <initializer_list>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include <sched.h>
 
 #include "gs_kernels.h"
 #include "gs_adjoint.h"
@@ -71,12 +74,13 @@ int set_err(int code, const char* fmt, ...) {
         if (c_ != GS_OK) return c_; \
     } while (0)
 
-// Per-device state: a few int counters (default work queues), allocated
-// once per device.
+// Per-device state: a few int counters (default work queues) and the
+// adjoint's default reduction workspace, allocated once per device.
 struct DeviceCtx {
     bool ready = false;
     int cus = 0;
     int* counters = nullptr;  // GS_NCOUNTERS ints
+    double* fin = nullptr;    // GS_FIN_BLOCKS * GS_GEO_OUT doubles
 };
 
 constexpr int GS_MAX_PARTIAL_BLOCKS = 1024;
@@ -97,6 +101,7 @@ int get_ctx(DeviceCtx** out) {
     if (!c.ready) {
         GS_HIP(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
         GS_HIP(hipMalloc(&c.counters, sizeof(int) * GS_NCOUNTERS));
+        GS_HIP(hipMalloc(&c.fin, sizeof(double) * GS_FIN_BLOCKS * GS_GEO_OUT));
         c.ready = true;
     }
     *out = &c;
@@ -411,6 +416,73 @@ int scratch_ptr(double** out) {
     return GS_OK;
 }
 
+// ---- host libm trig: the parity mode ----------------------------------------
+//
+// The reference evaluates sin(RAD * omega[i] * omegasign) and cos(...) per
+// peak (src/cdiffraction.c:146-147, RAD = PI / 180.0 unparenthesised), and
+// sin/cos(wedge * RAD), sin/cos(chi * RAD) once (:119-120, :130-131).  gcc -O2
+// compiles each such pair into ONE glibc sincos() call of the same argument.
+// glibc's sincos is not correctly rounded everywhere and differs from any
+// device algorithm in the last bit for a few tenths of a percent of
+// arguments, so the parity mode calls that same sincos here, on the host,
+// with the same expression (this translation unit is compiled with
+// -ffp-contract=off; PI/180.0 folds to the same double as in gcc), and the
+// kernels take the results as input.
+
+// Host threads for n independent peaks: the CPUs this process may run on,
+// or OMP_NUM_THREADS when set (as the reference's OpenMP loop would use),
+// at most one per 16384 peaks.
+int host_threads(long long n) {
+    long long want = 0;
+    if (const char* env = std::getenv("OMP_NUM_THREADS")) want = std::atoll(env);
+    if (want <= 0) {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        want = sched_getaffinity(0, sizeof(set), &set) == 0 ? CPU_COUNT(&set) : 1;
+    }
+    want = std::min<long long>(want, 256);
+    want = std::min<long long>(want, n / 16384 + 1);
+    return (int)std::max<long long>(1, want);
+}
+
+void sincos_omega_rows(const double* omega, double omegasign, double* so, double* co,
+                       long long a, long long b) {
+    for (long long i = a; i < b; i++) sincos(GS_PI / 180.0 * omega[i] * omegasign, so + i, co + i);
+}
+
+// so[i], co[i] = glibc sincos(((PI/180.0)*omega[i])*omegasign), the peaks
+// split into contiguous ranges over host threads (each value is computed
+// the same way whichever thread computes it).
+void sincos_omega_libm(const double* omega, double omegasign, double* so, double* co,
+                       long long n) {
+    const int nt = host_threads(n);
+    if (nt <= 1) {
+        sincos_omega_rows(omega, omegasign, so, co, 0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    long long next = n / nt;  // first row no thread has taken; this thread takes [0, n/nt)
+    try {
+        for (int t = 1; t < nt; t++) {
+            const long long a = n * t / nt, b = n * (t + 1) / nt;
+            pool.emplace_back(sincos_omega_rows, omega, omegasign, so, co, a, b);
+            next = b;
+        }
+    } catch (...) {  // no more threads could start: this thread takes the rest
+    }
+    sincos_omega_rows(omega, omegasign, so, co, 0, n / nt);
+    if (next < n) sincos_omega_rows(omega, omegasign, so, co, next, n);
+    for (auto& th : pool) th.join();
+}
+
+// The wedge/chi matrix of compute_gv from glibc's sincos (src/cdiffraction.c:119-142).
+void wedge_chi_matrix_libm(double wedge, double chi, double* mat) {
+    double sw, cw, sc, cc;
+    sincos(gs_wedge_rad(wedge), &sw, &cw);
+    sincos(gs_wedge_rad(chi), &sc, &cc);
+    gs_wedge_chi_from_trig(sw, cw, sc, cc, mat);
+}
+
 gs_detector make_detector(const double* p, const double* r, const double* dist) {
     gs_detector d;
     for (int i = 0; i < 4; i++) d.p[i] = p[i];
@@ -708,46 +780,55 @@ int score_and_assign(double ubi[3][3], double gv[][3], double tol, double* drlv2
     return h;
 }
 
-void compute_gv(double xlylzl[][3], double omega[], double omegasign, double wvln, double wedge,
-                double chi, double t[3], double gv[][3], int n) {
+// compute_gv / compute_geometry on host arrays: sin/cos of every omega and
+// the wedge/chi matrix from the host's glibc, as the reference computes
+// them, the rest on the GPU (so the outputs are the reference's bits).
+// `width` doubles of output per peak: 3 (gv) or 6 (tth, eta, ds, g).
+static void host_gv_call(const char* what, int width, double xlylzl[][3], double omega[],
+                         double omegasign, double wvln, double wedge, double chi, double t[3],
+                         double* out, int n) {
     if (check_ng(n) != GS_OK || n == 0) return;
-    std::vector<char*> sl;
-    if (stage_slices({(size_t)n * 24, (size_t)n * 8, (size_t)n * 24}, sl) != GS_OK) return;
+    std::vector<char*> sl, h, d;
+    const size_t nb8 = (size_t)n * 8;
+    if (stage_slices({nb8 * 3, nb8, nb8, nb8 * width}, sl) != GS_OK) return;
+    if (pinned_slices({nb8, nb8}, h, d) != GS_OK) return;
     double* dx = reinterpret_cast<double*>(sl[0]);
-    double* dw = reinterpret_cast<double*>(sl[1]);
-    double* dg = reinterpret_cast<double*>(sl[2]);
+    double* dso = reinterpret_cast<double*>(sl[1]);
+    double* dco = reinterpret_cast<double*>(sl[2]);
+    double* dout = reinterpret_cast<double*>(sl[3]);
+    double* hso = reinterpret_cast<double*>(h[0]);
+    double* hco = reinterpret_cast<double*>(h[1]);
     hipStream_t st = nullptr;
-    if (hipMemcpyAsync(dx, xlylzl, (size_t)n * 24, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(dw, omega, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
-        set_err(GS_ERR_HIP, "compute_gv: copy to device failed");
+    if (hipMemcpyAsync(dx, xlylzl, nb8 * 3, hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "%s: copy to device failed", what);
         return;
     }
-    gs_goniometer gp = make_gonio(omegasign, wvln, wedge, chi, t, nullptr);
-    if (launch_gv(dx, dw, nullptr, nullptr, gp, dg, n, st) != GS_OK) return;
-    if (hipMemcpyAsync(gv, dg, (size_t)n * 24, hipMemcpyDeviceToHost, st) != hipSuccess ||
+    sincos_omega_libm(omega, omegasign, hso, hco, n);
+    if (hipMemcpyAsync(dso, hso, nb8, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dco, hco, nb8, hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_err(GS_ERR_HIP, "%s: copy to device failed", what);
+        return;
+    }
+    double mat[9];
+    wedge_chi_matrix_libm(wedge, chi, mat);
+    gs_goniometer gp = make_gonio(omegasign, wvln, wedge, chi, t, mat);
+    if ((width == 3 ? launch_gv(dx, nullptr, dso, dco, gp, dout, n, st)
+                    : launch_geometry(dx, nullptr, dso, dco, gp, dout, n, st)) != GS_OK)
+        return;
+    if (hipMemcpyAsync(out, dout, nb8 * width, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
-        set_err(GS_ERR_HIP, "compute_gv: copy back failed");
+        set_err(GS_ERR_HIP, "%s: copy back failed", what);
+}
+
+void compute_gv(double xlylzl[][3], double omega[], double omegasign, double wvln, double wedge,
+                double chi, double t[3], double gv[][3], int n) {
+    host_gv_call("compute_gv", 3, xlylzl, omega, omegasign, wvln, wedge, chi, t, &gv[0][0], n);
 }
 
 void compute_geometry(double xlylzl[][3], double omega[], double omegasign, double wvln,
                       double wedge, double chi, double t[3], double out[][6], int n) {
-    if (check_ng(n) != GS_OK || n == 0) return;
-    std::vector<char*> sl;
-    if (stage_slices({(size_t)n * 24, (size_t)n * 8, (size_t)n * 48}, sl) != GS_OK) return;
-    double* dx = reinterpret_cast<double*>(sl[0]);
-    double* dw = reinterpret_cast<double*>(sl[1]);
-    double* dout = reinterpret_cast<double*>(sl[2]);
-    hipStream_t st = nullptr;
-    if (hipMemcpyAsync(dx, xlylzl, (size_t)n * 24, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(dw, omega, (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
-        set_err(GS_ERR_HIP, "compute_geometry: copy to device failed");
-        return;
-    }
-    gs_goniometer gp = make_gonio(omegasign, wvln, wedge, chi, t, nullptr);
-    if (launch_geometry(dx, dw, nullptr, nullptr, gp, dout, n, st) != GS_OK) return;
-    if (hipMemcpyAsync(out, dout, (size_t)n * 48, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        set_err(GS_ERR_HIP, "compute_geometry: copy back failed");
+    host_gv_call("compute_geometry", 6, xlylzl, omega, omegasign, wvln, wedge, chi, t,
+                 &out[0][0], n);
 }
 
 void compute_xlylzl_xpos_variable(double s[], double f[], double p[4], double r[9],
@@ -982,6 +1063,65 @@ int gs_sincos_omega(const double* omega, double omegasign, double* so, double* c
     return GS_OK;
 }
 
+int gs_sincos_omega_host(const double* omega, double omegasign, double* so, double* co, int n) {
+    GS_CHECK(check_ng(n));
+    if (n == 0) return GS_OK;
+    if (!omega || !so || !co) return set_err(GS_ERR_ARG, "gs_sincos_omega_host: null array");
+    sincos_omega_libm(omega, omegasign, so, co, n);
+    return GS_OK;
+}
+
+int gs_wedge_chi_matrix_host(double wedge, double chi, double* mat) {
+    if (!mat) return set_err(GS_ERR_ARG, "gs_wedge_chi_matrix_host: null mat");
+    wedge_chi_matrix_libm(wedge, chi, mat);
+    return GS_OK;
+}
+
+// Device omega -> device so/co through the host's glibc: chunks of peaks
+// go down to pinned host memory, their sincos is taken there, and the
+// results go back up.  Returns when so/co are written.
+int gs_sincos_omega_parity(const double* omega, double omegasign, double* so, double* co, int n,
+                           gs_stream_t stream) {
+    GS_CHECK(check_ng(n));
+    if (n == 0) return GS_OK;
+    if (!omega || !so || !co) return set_err(GS_ERR_ARG, "gs_sincos_omega_parity: null array");
+    hipStream_t st = S(stream);
+    constexpr long long CHUNK = 1LL << 21;  // 48 MiB of pinned memory at most
+    const long long c = std::min<long long>(CHUNK, n);
+    std::vector<char*> h, d;
+    GS_CHECK(pinned_slices({(size_t)c * 8, (size_t)c * 8, (size_t)c * 8}, h, d));
+    double* hw = reinterpret_cast<double*>(h[0]);
+    double* hs = reinterpret_cast<double*>(h[1]);
+    double* hc = reinterpret_cast<double*>(h[2]);
+    for (long long a = 0; a < n; a += c) {
+        const long long m = std::min<long long>(c, n - a);
+        GS_HIP(hipMemcpyAsync(hw, omega + a, m * 8, hipMemcpyDeviceToHost, st));
+        GS_HIP(hipStreamSynchronize(st));  // also: the previous chunk's uploads are done
+        sincos_omega_libm(hw, omegasign, hs, hc, m);
+        GS_HIP(hipMemcpyAsync(so + a, hs, m * 8, hipMemcpyHostToDevice, st));
+        GS_HIP(hipMemcpyAsync(co + a, hc, m * 8, hipMemcpyHostToDevice, st));
+    }
+    GS_HIP(hipStreamSynchronize(st));
+    return GS_OK;
+}
+
+int gs_sf2gv_trig(const double* s, const double* f, const double* so, const double* co,
+                  const double p[4], const double r[9], const double dist[3], const double mat[9],
+                  double wvln, const double t[3], double* xlylzl_out, double* gv, int n,
+                  gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    GS_CHECK(check_ng(n));
+    if (!so || !co || !mat) return set_err(GS_ERR_ARG, "gs_sf2gv_trig: null so/co/mat");
+    if (n == 0) return GS_OK;
+    gs_goniometer gp = make_gonio(1.0, wvln, 0.0, 0.0, t, mat);
+    hipLaunchKernelGGL(k_sf2gv<true>, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
+                       s, f, nullptr, so, co, make_detector(p, r, dist), gp, xlylzl_out, nullptr,
+                       nullptr, gv, n);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
 int gs_sf2gv(const double* s, const double* f, const double* omega, const double p[4],
              const double r[9], const double dist[3], double omegasign, double wvln, double wedge,
              double chi, const double t[3], double* xlylzl_out, double* so_out, double* co_out,
@@ -993,8 +1133,9 @@ int gs_sf2gv(const double* s, const double* f, const double* omega, const double
         return set_err(GS_ERR_ARG, "gs_sf2gv: so_out and co_out must both be given or both NULL");
     if (n == 0) return GS_OK;
     gs_goniometer gp = make_gonio(omegasign, wvln, wedge, chi, t, nullptr);
-    hipLaunchKernelGGL(k_sf2gv, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream), s, f,
-                       omega, make_detector(p, r, dist), gp, xlylzl_out, so_out, co_out, gv, n);
+    hipLaunchKernelGGL(k_sf2gv<false>, dim3(blocks_for(n, GS_BLOCK)), dim3(GS_BLOCK), 0,
+                       S(stream), s, f, omega, nullptr, nullptr, make_detector(p, r, dist), gp,
+                       xlylzl_out, so_out, co_out, gv, n);
     GS_LAUNCHED();
     return GS_OK;
 }
@@ -1167,7 +1308,7 @@ int gs_joint_adjoint(const double* sc, const double* fc, const double* so, const
                      const int* idx, long long n_idx, const long long* off, const int* hkl,
                      const int* order, int gspace, double* loss_g, double* grad_ubi,
                      double* grad_t, double* geo_part, double* nsums, double* out,
-                     int lanes_per_grain, int* queue_ws, gs_stream_t stream) {
+                     int lanes_per_grain, int* queue_ws, double* fin_ws, gs_stream_t stream) {
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
     if (npk < 0 || K < 0 || n_idx < 0) return set_err(GS_ERR_ARG, "gs_joint_adjoint: negative size");
@@ -1180,7 +1321,8 @@ int gs_joint_adjoint(const double* sc, const double* fc, const double* so, const
     int lpg = lanes_per_grain;
     if (lpg != 16 && lpg != 32 && lpg != 64) lpg = 64;
     const int gpw = GS_WAVE / lpg;
-    long long waves = std::min<long long>((K + gpw - 1) / gpw, 8LL * ctx->cus);
+    // persistent waves: enough to fill every SIMD at the kernel's occupancy
+    long long waves = std::min<long long>((K + gpw - 1) / gpw, 32LL * ctx->cus);
     int nb = (int)std::max<long long>(1, (waves + GS_BLOCK / GS_WAVE - 1) / (GS_BLOCK / GS_WAVE));
     const dim3 grid(nb), block(GS_BLOCK);
 #define GS_ADJ_LAUNCH(L, GSP, NS)                                                                \
@@ -1208,8 +1350,13 @@ int gs_joint_adjoint(const double* sc, const double* fc, const double* so, const
     }
 #undef GS_ADJ_LPG
 #undef GS_ADJ_LAUNCH
-    hipLaunchKernelGGL(k_joint_finalize, dim3(1), dim3(GS_FIN_GROUPS * 32), 0, st, geo_part, loss_g,
-                       K, geo, out);
+    const int fb = gs_fin_blocks(K);
+    double* fin = fin_ws ? fin_ws : ctx->fin;
+    hipLaunchKernelGGL(k_joint_reduce1, dim3(fb), dim3(GS_FIN_GROUPS * 32), 0, st, geo_part,
+                       loss_g, K, fin);
+    GS_LAUNCHED();
+    hipLaunchKernelGGL(k_joint_reduce2, dim3(1), dim3(GS_FIN_GROUPS * 32), 0, st, fin, fb, geo,
+                       out);
     GS_LAUNCHED();
     return GS_OK;
 }

@@ -76,17 +76,27 @@ def assign_peaks_to_grains(ubis, gv, tol):
 class Scan:
     """A peak table on the GPU with the per-peak quantities that do not
     depend on grains: lab positions (after removing any per-peak sample
-    y-translation dty from yl) and sin/cos of omega."""
+    y-translation dty from yl), sin/cos of omega and the wedge/chi matrix.
 
-    def __init__(self, sc, fc, omega, pars, dty=None):
+    parity=True takes sin/cos of omega from the host's glibc, as the
+    reference's compute_gv does (``batched.sincos_omega``): then every
+    g-vector, label and drlv2 computed from this scan is the reference's,
+    bit for bit.  parity=False (the fast mode) uses the device sincos,
+    which differs from glibc's in the last bit for a few tenths of a
+    percent of angles; labels only differ where a pair sits within ~1e-15
+    relative of the tolerance.  The wedge/chi matrix is glibc's in both."""
+
+    def __init__(self, sc, fc, omega, pars, dty=None, parity=False):
         dev = sc.device
         self.pars = geometry.full_parameters(pars)
-        self.gv0, self.xl, self.so, self.co = batched.sf2gv(sc, fc, omega, self.pars, keep=True)
+        self.parity = bool(parity)
+        sign, wvln, wedge, chi = geometry.pack_gv(self.pars)
+        self.mat = batched.wedge_chi_matrix(wedge, chi)
+        self.gv0, self.xl, self.so, self.co = batched.sf2gv(sc, fc, omega, self.pars, keep=True,
+                                                            parity=parity)
         if dty is not None:
             self.xl[:, 1] -= dty
-            sign, wvln, wedge, chi = geometry.pack_gv(self.pars)
-            self.gv0 = batched.gvectors_trig(self.xl, self.so, self.co,
-                                             geometry.wedge_chi_matrix(wedge, chi), wvln)
+            self.gv0 = batched.gvectors_trig(self.xl, self.so, self.co, self.mat, wvln)
         self.n = sc.shape[0]
         self.device = dev
 
@@ -98,8 +108,7 @@ def assignlabels(scan, ubis, translations, tol, grain_labels=None):
     drlv2 = torch.ones(scan.n, dtype=torch.float64, device=dev)
     sign, wvln, wedge, chi = geometry.pack_gv(scan.pars)
     counts = batched.gv_assign_many(scan.xl, scan.so, scan.co, wvln, ubis, translations, tol,
-                                    drlv2, labels, grain_labels=grain_labels, wedge=wedge,
-                                    chi=chi)
+                                    drlv2, labels, grain_labels=grain_labels, mat=scan.mat)
     return labels, drlv2, counts
 
 
@@ -109,7 +118,7 @@ def grain_gvectors(scan, translations, labels, K):
     idx, off, owner = batched.lists_from_labels(labels, K)
     sign, wvln, wedge, chi = geometry.pack_gv(scan.pars)
     gvl = batched.gv_lists(scan.xl, scan.so, scan.co, wvln, translations, idx, owner,
-                           wedge=wedge, chi=chi)
+                           mat=scan.mat)
     return gvl, idx, off
 
 

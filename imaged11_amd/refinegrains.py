@@ -230,9 +230,10 @@ class _ScanState:
     """Device-resident peaks of one scan and the per-grain lists that
     assignlabels fixed."""
 
-    def __init__(self, col, device):
+    def __init__(self, col, device, parity=False):
         T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
         self.device = device
+        self.parity = parity
         self.sc, self.fc, self.omega = T(col.sc), T(col.fc), T(col.omega)
         self.n = int(col.nrows)
         self.xl = None
@@ -253,7 +254,8 @@ class _ScanState:
 
     def trig(self, omegasign):
         if self.sign != float(omegasign):
-            self.so, self.co = batched.sincos_omega(self.omega, float(omegasign))
+            self.so, self.co = batched.sincos_omega(self.omega, float(omegasign),
+                                                    parity=self.parity)
             self.sign = float(omegasign)
 
 
@@ -281,11 +283,15 @@ class refinegrains:
                  "y_size": 0.2, "z_size": 0.2}
 
     def __init__(self, tolerance=0.01, intensity_tth_range=(6.1, 6.3),
-                 latticesymmetry=triclinic, OmFloat=True, OmSlop=0.25, device=None):
+                 latticesymmetry=triclinic, OmFloat=True, OmSlop=0.25, device=None,
+                 parity=False):
         _lib.load()
         _lib.require_gpu()
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self.OMEGA_FLOAT = OmFloat
+        # parity=True: sin/cos of omega from the host's glibc (the values
+        # the reference's compute_gv uses, bit for bit); else the device's
+        self.parity = bool(parity)
         self.slop = OmSlop
         self.tolerance = tolerance
         self.grainnames = []
@@ -339,7 +345,7 @@ class refinegrains:
         if "fc" not in col.titles:
             col.addcolumn(col.yc.copy(), "fc")
         self.scandata[name] = col
-        self._state[name] = _ScanState(col, self.device)
+        self._state[name] = _ScanState(col, self.device, self.parity)
         return name
 
     def generate_grains(self):
@@ -730,7 +736,8 @@ class refinegrains:
             drlv2 = torch.ones(st.n, dtype=torch.float64, device=dev)
             glabels = torch.as_tensor(np.array([int(n) for n in names], np.int32), device=dev)
             batched.gv_assign_many(st.xl, st.so, st.co, wvln, ubis, trans, self.tolerance,
-                                   drlv2, labels, grain_labels=glabels, wedge=wedge, chi=chi)
+                                   drlv2, labels, grain_labels=glabels,
+                                   mat=batched.wedge_chi_matrix(wedge, chi))
             # per-grain lists in grain order (peaks ascending in each)
             lab_h = labels.cpu().numpy()
             ints = np.array([int(n) for n in names], np.int64)
@@ -744,7 +751,7 @@ class refinegrains:
             # the g-vector column is compute_gv with the last grain's translation
             last_t = trans[-1].cpu().numpy() if K else np.zeros(3)
             self.gv = batched.gvectors_trig(st.xl, st.so, st.co,
-                                            geometry.wedge_chi_matrix(wedge, chi), wvln,
+                                            batched.wedge_chi_matrix(wedge, chi), wvln,
                                             t=tuple(last_t)).cpu().numpy()
             col.addcolumn(lab_h, "labels")
             col.addcolumn(drlv2.cpu().numpy(), "drlv2")

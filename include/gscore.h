@@ -94,7 +94,11 @@ int score_and_assign(double ubi[3][3], double gv[][3], double tol, double *drlv2
 void refine_assigned(double ubi[3][3], double gv[][3], int *labels, int label, int *npk,
                      double *sumdrlv2, int ng);
 
-/* src/cdiffraction.c:112-168 — g-vectors from lab positions and omega. */
+/* src/cdiffraction.c:112-168 — g-vectors from lab positions and omega.
+ * sin/cos of omega, wedge and chi come from the host's glibc sincos() with
+ * the reference's expressions (gcc -O2 turns each sin/cos pair of the
+ * reference into that one call), the rest runs on the GPU: the output is
+ * the reference's, bit for bit. */
 void compute_gv(double xlylzl[][3], double omega[], double omegasign, double wvln,
                 double wedge, double chi, double t[3], double gv[][3], int n);
 
@@ -171,9 +175,27 @@ int gs_compute_gv_trig_dev(const double *xlylzl, const double *so, const double 
                            const double mat[9], double wvln, const double t[3], double *gv,
                            int n, gs_stream_t stream);
 
-/* sin and cos of ((PI/180)*omega)*omegasign per peak. */
+/* sin and cos of ((PI/180)*omega)*omegasign per peak, on the device (the
+ * fast mode: the device's correctly rounded sincos, which differs from
+ * glibc's in the last bit for a few tenths of a percent of arguments). */
 int gs_sincos_omega(const double *omega, double omegasign, double *so, double *co, int n,
                     gs_stream_t stream);
+
+/* The parity mode's trig: glibc sincos() on the host, with the reference's
+ * expressions (src/cdiffraction.c:146-147 for omega, :119-120 and :130-131
+ * for wedge and chi; gcc -O2 compiles each sin/cos pair there into one
+ * sincos call), so sin/cos, and every g-vector computed from them on the
+ * GPU, are the reference's bits.  The peaks are split over the CPUs the
+ * process may run on (OMP_NUM_THREADS when set).
+ * gs_sincos_omega_host: host arrays.
+ * gs_wedge_chi_matrix_host: the 3x3 matrix mat (row-major) of compute_gv
+ *   (src/cdiffraction.c:121-142) for wedge, chi in degrees.
+ * gs_sincos_omega_parity: DEVICE omega in, DEVICE so/co out, through
+ *   pinned host memory on `stream`; returns when so/co are written. */
+int gs_sincos_omega_host(const double *omega, double omegasign, double *so, double *co, int n);
+int gs_wedge_chi_matrix_host(double wedge, double chi, double *mat);
+int gs_sincos_omega_parity(const double *omega, double omegasign, double *so, double *co, int n,
+                           gs_stream_t stream);
 
 /* compute_xlylzl -> compute_gv fused (ImageD11/transform.py:729-738,
  * Ctransform.sf2gv); xlylzl_out and so_out/co_out may be NULL. */
@@ -181,6 +203,13 @@ int gs_sf2gv(const double *s, const double *f, const double *omega, const double
              const double r[9], const double dist[3], double omegasign, double wvln,
              double wedge, double chi, const double t[3], double *xlylzl_out, double *so_out,
              double *co_out, double *gv, int n, gs_stream_t stream);
+
+/* gs_sf2gv with the per-peak sin/cos (so, co) and the wedge/chi matrix mat
+ * (9 host doubles) given, e.g. by the parity mode above. */
+int gs_sf2gv_trig(const double *s, const double *f, const double *so, const double *co,
+                  const double p[4], const double r[9], const double dist[3],
+                  const double mat[9], double wvln, const double t[3], double *xlylzl_out,
+                  double *gv, int n, gs_stream_t stream);
 
 /* Batched score_and_refine: grain g refines on the peaks gv[idx[off[g]]]
  * .. gv[idx[off[g+1]-1]], n_iter successive calls (refinegrains.refine,
@@ -262,8 +291,14 @@ int gs_graph_destroy(gs_graph_t graph);
  * geo_part[K*26]; nsums[K*15] (optional, NULL to skip) = sum h G^T and
  * sum G G^T for a Gauss-Newton UBI update; out[27] = dL/dgeo (26, slot 16
  * is dL/dwavelength) and L, reduced over grains in an order fixed by K
- * (32 interleaved groups of grains, then the groups in order:
- * deterministic).  gspace != 0: the g-space objective
+ * (contiguous ranges of grains, interleaved groups within each, then the
+ * ranges in order: deterministic).  queue_ws: 1 int (the work queue) or
+ * NULL for the library's per-device counter; fin_ws: 128 * 27 doubles of
+ * reduction workspace or NULL for the library's per-device buffer (calls
+ * overlapping on one device must pass their own).  The pair's forward
+ * pass (compute_xlylzl, compute_gv) uses FMA and a reciprocal square root,
+ * so G is within a few ulps of compute_gv's (gs_joint_hkl keeps
+ * compute_gv's own arithmetic).  gspace != 0: the g-space objective
  * 1/2 sum ||UB.hkl - G||^2 (UB = UBI^-1), grad_ubi then holds dE/dUB =
  * UB.H - R with score_and_refine's R and H (src/closest.c:310-316).
  * lanes_per_grain 16, 32 or 64 (anything else: 64) lanes share a grain.
@@ -279,7 +314,7 @@ int gs_joint_adjoint(const double *sc, const double *fc, const double *so, const
                      const int *idx, long long n_idx, const long long *off, const int *hkl,
                      const int *order, int gspace, double *loss_g, double *grad_ubi,
                      double *grad_t, double *geo_part, double *nsums, double *out,
-                     int lanes_per_grain, int *queue_ws, gs_stream_t stream);
+                     int lanes_per_grain, int *queue_ws, double *fin_ws, gs_stream_t stream);
 
 /* ---- compute_gv per grain + score_and_assign, all grains in order ----
  * The loop of refinegrains.assignlabels (ImageD11/refinegrains.py:

@@ -263,3 +263,52 @@ def test_out_of_range_index_keeps_lists_indexed(setup, dev):
     loss, *_ = prob.evaluate(s["geo"], s["ubis"], s["trans"])
     assert bool(torch.isfinite(loss))
     assert prob.hkl[7].abs().sum() == 0
+
+
+def test_parity_mode_inputs_are_the_references(setup, orc, dev):
+    """JointProblem(parity=True): sin/cos of omega and the wedge/chi matrix
+    are glibc's (the reference's values, bit for bit); the loss then equals
+    the oracle's objective on the oracle's own trig within 1e-10 (the
+    adjoint's own FMA / reciprocal-square-root arithmetic and summation
+    order) and the geometry gradient keeps flowing through mat."""
+    s = setup
+    pk = s["wl"]["peaks"]
+    pars = s["pars"]
+    prob = adjoint.JointProblem(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev),
+                                pars["omegasign"], T(s["idx"], dev, torch.int32),
+                                T(s["off"], dev, torch.int64), parity=True)
+    so, co = orc.sincos_omega(pk["omega"], pars["omegasign"])
+    assert np.array_equal(prob.so.cpu().numpy().view(np.int64), so.view(np.int64))
+    assert np.array_equal(prob.co.cpu().numpy().view(np.int64), co.view(np.int64))
+    params = {k: v.clone().requires_grad_(True) for k, v in s["params"].items()}
+    geo = adjoint.pack_geometry(params, pars, parity=True)
+    g = geo.detach().cpu().numpy()
+    assert np.array_equal(g[17:].view(np.int64),
+                          orc.wedge_chi_matrix(pars["wedge"], pars["chi"]).view(np.int64))
+    prob.assign_hkl(geo, s["ubis"], s["trans"])
+    loss = adjoint.joint_loss(prob, s["ubis"], s["trans"], params, pars)
+    want = orc.joint_loss(pk["sc"], pk["fc"], so, co, g[:4], g[4:13], g[13:16], g[16], g[17:],
+                          s["wl"]["ubi"], s["wl"]["trans"], s["idx"], s["off"],
+                          prob.hkl.cpu().numpy())
+    assert abs(float(loss) - want) <= 1e-10 * abs(want)
+    loss.backward()
+    assert params["wedge"].grad is not None and float(params["wedge"].grad) != 0.0
+
+
+def test_gauss_newton_mode_ubi_gradient_matches_direct(setup):
+    """With the Gauss-Newton sums requested, dL/dUBI is formed from them,
+    2 (UBI . sum G G^T - sum h G^T), instead of being accumulated pair by
+    pair: the two agree to rounding, and the loss, translation and
+    geometry gradients are the same computation in both modes."""
+    s = setup
+    l1, gu1, gt1, gg1, _ = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"])
+    l2, gu2, gt2, gg2, sums = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"],
+                                                want_sums=True)
+    assert abs(float(l1) - float(l2)) <= 1e-14 * abs(float(l1))
+    scale = float(gu1.abs().max())
+    assert float((gu2 - gu1).abs().max()) <= 1e-9 * scale
+    torch.testing.assert_close(gt2, gt1, rtol=1e-12, atol=1e-12 * float(gt1.abs().max()))
+    torch.testing.assert_close(gg2, gg1, rtol=1e-12, atol=1e-12 * float(gg1.abs().max()))
+    # and the sums themselves: sum G G^T is symmetric positive semi-definite
+    GG = sums[:, 9:]
+    assert bool((GG[:, 0] >= 0).all() and (GG[:, 3] >= 0).all() and (GG[:, 5] >= 0).all())

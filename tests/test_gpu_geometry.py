@@ -1,9 +1,9 @@
 """compute_geometry, compute_xlylzl_xpos_variable and the column-file
 geometry update on the GPU against the oracle.
 
-d* and g are bit-exact given the per-peak sin/cos (the device sincos is
-correctly rounded, glibc's is not everywhere, so the full path is compared
-within 1e-13 relative).  tth and eta go through atan2, whose device and
+d* and g are bit-exact given the per-peak sin/cos and in the drop-in's
+parity mode (sin/cos from the host's glibc); with the device sincos
+(correctly rounded, glibc's is not everywhere) within 1e-13 relative.  tth and eta go through atan2, whose device and
 glibc versions are both within about an ulp of the true value but not
 always equal: they are compared within 4 ulps."""
 import os
@@ -65,11 +65,18 @@ def test_compute_geometry_dropin(dev, orc, wedge, chi, t, sign):
     orc.compute_geometry(xl, om, sign, 0.2, wedge, chi, np.array(t), want)
     got = np.zeros((len(om), 6))
     cImageD11.compute_geometry(xl, om, sign, 0.2, wedge, chi, t, got)
-    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-13)
+    # parity mode (the default): ds and g bit for bit, tth/eta within 4 ulps
+    assert np.array_equal(bits(got[:, 2:]), bits(want[:, 2:]))
+    assert ulps(got[:, 0], want[:, 0]).max() <= 4
+    assert ulps(got[:, 1], want[:, 1]).max() <= 4
     got_d = torch.zeros((len(om), 6), dtype=torch.float64, device=dev)
     cImageD11.compute_geometry(torch.as_tensor(xl, device=dev), torch.as_tensor(om, device=dev),
                                sign, 0.2, wedge, chi, t, got_d)
     assert np.array_equal(bits(got_d.cpu().numpy()), bits(got))
+    # the device sincos: within 1e-13 relative
+    fast = np.zeros((len(om), 6))
+    cImageD11.compute_geometry(xl, om, sign, 0.2, wedge, chi, t, fast, parity=False)
+    np.testing.assert_allclose(fast, want, rtol=1e-13, atol=1e-13)
     with pytest.raises(ValueError):
         cImageD11.compute_geometry(xl, om, sign, 0.2, wedge, chi, t, np.zeros((len(om), 5)))
 

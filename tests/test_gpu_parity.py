@@ -5,9 +5,10 @@ written by the reference C.
 Bars: counts, labels, drlv2 of score_and_assign and everything of
 compute_xlylzl bit-exact; score_and_refine n exact, mean drlv2 within
 1e-12 relative, refined UBI within 1e-10 (summation order of R differs);
-compute_gv bit-exact given the same sin/cos, and within 2 ulp of the
-reference with the device's correctly rounded sincos (glibc's sincos, which
-the reference calls, is not correctly rounded for ~0.16% of arguments)."""
+compute_gv bit-exact in the parity mode (sin/cos from the host's glibc, as
+the reference computes them) and given the same sin/cos, and within 8 ulps
+with the device's correctly rounded sincos (glibc's sincos, which the
+reference calls, is not correctly rounded for ~0.16% of arguments)."""
 import os
 
 import numpy as np
@@ -176,10 +177,12 @@ def test_sincos_device_equals_host_build(dev):
 
 
 @pytest.mark.parametrize("mode", ["host", "device"])
-def test_compute_gv_golden_ulps(dev, mode):
+def test_compute_gv_golden_parity(dev, mode):
+    """The drop-in compute_gv in its default parity mode (host glibc
+    sincos) reproduces the reference-written golden g-vectors bit for bit,
+    on host arrays and on device tensors, over 216 wedge/chi/sign/t sets."""
     s = load("geometry_cases.npz")
     outs = iter(s["gv_out"])
-    frac = []
     for wedge, chi, sign, wvln, tx, ty, tz in s["gv_sets"]:
         for x, w in ((s["gv_xyz"], s["gv_omega"]), (s["gv_xyz_rand"], s["gv_omega_rand"])):
             want = next(outs)
@@ -190,11 +193,47 @@ def test_compute_gv_golden_ulps(dev, mode):
                 gt = torch.zeros(want.shape, dtype=torch.float64, device=dev)
                 gpu.compute_gv(T(x, dev), T(w, dev), sign, wvln, wedge, chi, (tx, ty, tz), gt)
                 g = gt.cpu().numpy()
+            assert np.array_equal(g.view(np.int64), want.view(np.int64))
+
+
+@pytest.mark.parametrize("mode", ["host", "device"])
+def test_compute_gv_golden_fast_ulps(dev, mode):
+    """parity=False (device sincos): within 8 ulps of the golden vectors,
+    identical for most peaks."""
+    s = load("geometry_cases.npz")
+    outs = iter(s["gv_out"])
+    frac = []
+    for wedge, chi, sign, wvln, tx, ty, tz in s["gv_sets"]:
+        for x, w in ((s["gv_xyz"], s["gv_omega"]), (s["gv_xyz_rand"], s["gv_omega_rand"])):
+            want = next(outs)
+            if mode == "host":
+                g = np.zeros_like(want)
+                gpu.compute_gv(x, w, sign, wvln, wedge, chi, (tx, ty, tz), g, parity=False)
+            else:
+                gt = torch.zeros(want.shape, dtype=torch.float64, device=dev)
+                gpu.compute_gv(T(x, dev), T(w, dev), sign, wvln, wedge, chi, (tx, ty, tz), gt,
+                               parity=False)
+                g = gt.cpu().numpy()
             scale = np.abs(want).max(axis=1, keepdims=True)
             err = np.abs(g - want) / scale
             assert err.max() <= 8 * 2.0 ** -53
             frac.append(np.mean(np.any(g != want, axis=1)))
     assert np.mean(frac) < 0.05
+
+
+def test_parity_trig_on_device_tensors(dev, orc):
+    """batched.sincos_omega(parity=True) moves omega through the host's
+    glibc and back: bit for bit the oracle's sincos (pinned against the
+    reference), over more than one 2^21-peak chunk."""
+    rng = np.random.default_rng(5)
+    omega = rng.uniform(-360, 360, (1 << 21) + 12345)
+    so, co = batched.sincos_omega(T(omega, dev), -1.0, parity=True)
+    want_s, want_c = orc.sincos_omega(omega, -1.0)
+    assert np.array_equal(so.cpu().numpy().view(np.int64), want_s.view(np.int64))
+    assert np.array_equal(co.cpu().numpy().view(np.int64), want_c.view(np.int64))
+    e = torch.empty(0, dtype=torch.float64, device=dev)
+    s0, c0 = batched.sincos_omega(e, 1.0, parity=True)
+    assert s0.numel() == 0 and c0.numel() == 0
 
 
 # ---- batched kernels against the oracle --------------------------------------
@@ -254,6 +293,28 @@ def test_score_sieve_tolerances_and_ragged_sizes(dev, orc, workload, tol):
         got = batched.score_many(u[:K].contiguous(), g[:N].contiguous(), tol,
                                  mode=_lib.SCORE_SIEVE)
         assert np.array_equal(got.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("wedge,chi", [(0.0, 0.0), (0.8, -1.7)])
+def test_sf2gv_parity_equals_reference_chain(dev, orc, workload, wedge, chi):
+    """sf2gv(parity=True): compute_xlylzl -> compute_gv of the reference,
+    bit for bit (the oracle's chain, pinned against the compiled reference)."""
+    pk = workload["peaks"]
+    pars = dict(workload["pars"], wedge=wedge, chi=chi)
+    p, r, dist = geometry.pack_xlylzl(pars)
+    sign, wvln, _, _ = geometry.pack_gv(pars)
+    t = (11.0, -7.0, 3.5)
+    n = len(pk["sc"])
+    xl_o = np.zeros((n, 3))
+    orc.compute_xlylzl(pk["sc"], pk["fc"], p, r, dist, xl_o)
+    gv_o = np.zeros((n, 3))
+    orc.compute_gv(xl_o, pk["omega"], sign, wvln, wedge, chi, np.array(t), gv_o)
+    sc, fc, om = T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev)
+    gv, xl, so, co = batched.sf2gv(sc, fc, om, pars, t=t, keep=True, parity=True)
+    assert np.array_equal(xl.cpu().numpy().view(np.int64), xl_o.view(np.int64))
+    assert np.array_equal(gv.cpu().numpy().view(np.int64), gv_o.view(np.int64))
+    g2 = batched.gvectors(xl, om, sign, wvln, wedge, chi, t=t, parity=True)
+    assert torch.equal(g2, gv)
 
 
 def test_sf2gv_matches_separate_kernels(dev, orc, workload):

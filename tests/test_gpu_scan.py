@@ -53,14 +53,19 @@ def test_gv_assign_bitexact_given_trig(dev, orc, scan, d0, l0):
     assert want.sum() > 0.5 * n
 
 
-def test_assignlabels_driver(dev, orc, scan):
-    """The production path (device sincos, device wedge/chi matrix) gives
-    the reference loop's labels on this data."""
+@pytest.mark.parametrize("parity", [True, False], ids=["parity", "fast"])
+def test_assignlabels_driver(dev, orc, scan, parity):
+    """The shipped scan path (drivers.Scan + assignlabels).  Parity mode
+    (sin/cos of omega from the host's glibc, as the reference): labels,
+    drlv2 and per-call counts bit for bit the reference loop's.  Fast mode
+    (device sincos): the same labels and counts on this data, drlv2 within
+    the last bits of sin/cos."""
     s = scan
     pk = s["wl"]["peaks"]
     sc = drivers.Scan(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev), s["pars"],
-                      dty=T(pk["dty"], dev))
+                      dty=T(pk["dty"], dev), parity=parity)
     np.testing.assert_array_equal(sc.xl.cpu().numpy(), s["xl"])
+    assert np.array_equal(sc.mat.ravel().view(np.int64), s["mat"].ravel().view(np.int64))
     K = len(s["wl"]["ubi"])
     labels, drlv2, counts = drivers.assignlabels(sc, T(s["wl"]["ubi"], dev),
                                                  T(s["wl"]["trans"], dev), 0.05)
@@ -70,9 +75,12 @@ def test_assignlabels_driver(dev, orc, scan):
                               s["wl"]["trans"], np.arange(K), 0.05, d, lab)
     assert np.array_equal(counts.cpu().numpy(), want)
     assert np.array_equal(labels.cpu().numpy(), lab)
-    # drlv2 within the last bits of sin/cos of omega (device sincos is
-    # correctly rounded, glibc's is not for ~0.16% of angles)
-    np.testing.assert_allclose(drlv2.cpu().numpy(), d, rtol=1e-9, atol=1e-18)
+    if parity:
+        assert np.array_equal(sc.so.cpu().numpy().view(np.int64), s["so"].view(np.int64))
+        assert np.array_equal(drlv2.cpu().numpy().view(np.int64), d.view(np.int64))
+    else:
+        # device sincos is correctly rounded, glibc's is not for ~0.16% of angles
+        np.testing.assert_allclose(drlv2.cpu().numpy(), d, rtol=1e-9, atol=1e-18)
     # most peaks go back to the grain that made them
     own = pk["grain"]
     real = own >= 0
@@ -80,10 +88,13 @@ def test_assignlabels_driver(dev, orc, scan):
 
 
 def test_refine_after_assignment(dev, orc, scan):
+    """Parity mode: the grains' g-vector lists are the reference's compute_gv
+    bit for bit, and the refinement follows the reference's per-grain
+    score_and_refine calls (n exact, UBI within 1e-10)."""
     s = scan
     pk = s["wl"]["peaks"]
     sc = drivers.Scan(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev), s["pars"],
-                      dty=T(pk["dty"], dev))
+                      dty=T(pk["dty"], dev), parity=True)
     K = len(s["wl"]["ubi"])
     ubis = T(s["wl"]["ubi"], dev)
     trans = T(s["wl"]["trans"], dev)
@@ -99,8 +110,8 @@ def test_refine_after_assignment(dev, orc, scan):
         # the grain's g-vectors with its translation
         want_g = orc.compute_gv_trig(s["xl"][sel], s["so"][sel], s["co"][sel], s["mat"],
                                      s["wvln"], s["wl"]["trans"][g])
-        np.testing.assert_allclose(gvl_h[off_h[g]:off_h[g + 1]], want_g, rtol=0,
-                                   atol=1e-14)
+        assert np.array_equal(gvl_h[off_h[g]:off_h[g + 1]].view(np.int64),
+                              want_g.view(np.int64))
         uu = s["wl"]["ubi"][g].copy()
         for it in range(2):
             nn, mm = orc.score_and_refine(uu, gvl_h[off_h[g]:off_h[g + 1]], 0.05)

@@ -77,6 +77,50 @@ def test_device_sincos_vs_glibc_sincos(orc):
     assert np.abs(s - so).max() <= 2.0 ** -53 and np.abs(c - co).max() <= 2.0 ** -53
 
 
+def _same_bits(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 300001])
+def test_parity_trig_is_glibc_sincos(orc, monkeypatch, n):
+    """The parity mode's host trig (libgscore's own code, no oracle) equals
+    the oracle's glibc sincos of the reference expression bit for bit, the
+    oracle's being pinned against the compiled reference's compute_gv; with
+    several host threads (300001 peaks over up to 8 threads) and with one."""
+    rng = np.random.default_rng(n)
+    omega = rng.uniform(-720, 720, n)
+    omega[: min(n, 4)] = [0.0, -0.0, 90.0, 180.0][: min(n, 4)]
+    lib = _lib.load()
+    for threads in ("8", "1"):
+        monkeypatch.setenv("OMP_NUM_THREADS", threads)
+        for sign in (1.0, -1.0):
+            so = np.full(n, np.nan)
+            co = np.full(n, np.nan)
+            _lib.check(lib.gs_sincos_omega_host(_lib.hptr(omega), sign, _lib.hptr(so),
+                                                _lib.hptr(co), n), "sincos_omega_host")
+            want_s, want_c = orc.sincos_omega(omega, sign)
+            assert _same_bits(so, want_s) and _same_bits(co, want_c)
+
+
+def test_parity_wedge_chi_matrix_is_the_references(orc):
+    lib = _lib.load()
+    for wedge, chi in [(0.0, 0.0), (0.3, -0.7), (-2.5, 11.0), (45.0, 90.0), (1e-9, 179.99)]:
+        m = np.zeros(9)
+        _lib.check(lib.gs_wedge_chi_matrix_host(wedge, chi, _lib.hptr(m)), "wedge_chi")
+        assert _same_bits(m, orc.wedge_chi_matrix(wedge, chi).ravel())
+
+
+def test_parity_trig_rejects_bad_arguments():
+    lib = _lib.load()
+    x = np.zeros(4)
+    assert lib.gs_sincos_omega_host(_lib.hptr(x), 1.0, None, _lib.hptr(x), 4) == 1
+    assert lib.gs_sincos_omega_host(_lib.hptr(x), 1.0, _lib.hptr(x), _lib.hptr(x), -1) == 1
+    assert lib.gs_wedge_chi_matrix_host(0.0, 0.0, None) == 1
+    lib.gs_clear_error()
+
+
 def test_dropin_argument_rules():
     """f2py rules of src/_cImageD11.pyf: intent(inout) arrays must have the
     exact dtype and be C-contiguous; shapes are checked; checked before any
