@@ -114,6 +114,18 @@ class columnfile:
         self.__dict__["_cols"] = [c[mask] for c in self._cols]
         self.nrows = int(mask.sum())
 
+    def copyrows(self, rows):
+        """A new columnfile with the selected rows (a boolean mask or
+        indices) and a copy of the parameters (ImageD11/columnfile.py:381-392)."""
+        c = columnfile(new=True)
+        c.__dict__["titles"] = list(self.titles)
+        c.__dict__["_cols"] = [x[rows] for x in self._cols]
+        c.nrows = len(c._cols[0]) if c._cols else 0
+        c.ncols = self.ncols
+        c.parameters = Parameters(**self.parameters.parameters)
+        c.filename = self.filename
+        return c
+
     def reorder(self, indices):
         self.__dict__["_cols"] = [c[indices] for c in self._cols]
 

@@ -227,8 +227,11 @@ __device__ __forceinline__ gs_adj_entry gs_adj_fetch(
 // ROWS: no index array; sc/fc/so/co are already in list order (entry q is
 // element q), so the pass streams them instead of gathering four cache
 // lines per entry.
+#ifndef GS_ADJ_MIN_BLOCKS
+#define GS_ADJ_MIN_BLOCKS 1  // resident blocks per CU the register budget must allow
+#endif
 template <int LPG, bool GSPACE, bool NSUMS, bool ROWS>
-__global__ __launch_bounds__(GS_BLOCK) void k_joint_adjoint(
+__global__ __launch_bounds__(GS_BLOCK, GS_ADJ_MIN_BLOCKS) void k_joint_adjoint(
     const double* __restrict__ sc, const double* __restrict__ fc, const double* __restrict__ so,
     const double* __restrict__ co, int npk, const double* __restrict__ geo,
     const double* __restrict__ ubis, const double* __restrict__ trans, int K,

@@ -28,6 +28,7 @@ Results (hits, grains, scores, peak assignments) are the reference's for
 the same g-vectors, unit-cell tables and parameters; see the parity tests.
 """
 import logging
+import math
 
 import numpy as np
 import torch
@@ -205,6 +206,7 @@ class indexer:
         self.scores = []
         self.hits = []
         self.omega = None
+        self.wedge = 0.0
         self._gv_dev = None
 
     # parameters given as strings, as from ImageD11's parameter files
@@ -274,6 +276,11 @@ class indexer:
         self.ds = rows[:, 5].copy()
         self.eta = rows[:, 6].copy()
         self.omega = rows[:, 7].copy()
+        self.omega_fullrange = find_omega_ranges(self.omega)
+        if self.wavelength > 0:
+            self.tth = np.arcsin(self.ds * self.wavelength / 2) * 360 / math.pi
+        else:
+            self.tth = np.zeros(len(self.ds))
         self.gv = np.ascontiguousarray(rows[:, :3])
         self.allgv = self.gv.copy()
         self.ga = np.zeros(len(self.ds), np.int32) - 1
@@ -549,7 +556,351 @@ class indexer:
         from . import drivers
         g = self._gvd()
         ubis = torch.as_tensor(np.array(self.ubis), device=g.device)
-        labels, drlv2, gas = drivers.fight_over_peaks(ubis, g, self.hkl_tol)
+        labels, drlv2, gas = drivers.fight_over_peaks(ubis.reshape(-1, 3, 3), g, self.hkl_tol)
         self.ga = labels.cpu().numpy()
         self.gas = gas.cpu().numpy()
+        self.drlv2 = drlv2.cpu().numpy()
         return self.ga
+
+    def score_all_pairs(self, n=None, rmulmax=None, rings_to_use=None):
+        """Search every pair of rings in turn (ImageD11/indexing.py:577-631):
+        the rings holding peaks (or those of rings_to_use that do), without
+        those whose multiplicity exceeds rmulmax, paired with each other in
+        both orders and with themselves, sorted by (product of the
+        multiplicities, product of the ring occupations, ring_1, ring_2) so
+        that the sparsest pairs come first; for each pair `find` and, when
+        it gives hits, `scorethem`.  Each pair's peaks exclude those earlier
+        pairs assigned, so the pairs run in this order; within a pair every
+        hit is found, oriented and scored in one launch each.  n: stop after
+        the n-th pair that gave hits (the reference's count)."""
+        self.assigntorings()
+        if rings_to_use is not None:
+            rings = [r for r in rings_to_use if (self.ra == r).sum() > 0]
+        else:
+            rings = sorted(set(int(r) for r in self.ra if r >= 0))
+        uc = self.unitcell
+        mults = {r: len(uc.ringhkls[uc.ringds[r]]) for r in rings}
+        if rmulmax is not None:
+            rings = [r for r in rings if mults[r] <= rmulmax]
+        pairs = sorted((int(mults[r1] * mults[r2]), int(self.na[r1] * self.na[r2]), r1, r2)
+                       for r1 in rings for r2 in rings)
+        self.tried = 0
+        self.npairs = len(pairs)
+        self.stop = False
+        k = 0
+        for _, _, r1, r2 in pairs:
+            k += 1
+            self.ring_1 = r1
+            self.ring_2 = r2
+            self.find()
+            if len(self.hits) == 0:
+                continue
+            self.scorethem()
+            self.tried += 1
+            if self.stop:
+                break
+            if n is not None and k > n:
+                break
+            logging.info("Tried r1=%d r2=%d attempt %d of %d, got %d grains", r1, r2,
+                         self.tried, len(pairs), len(self.ubis))
+        logging.info("Tested %d pairs and found %d grains so far", self.tried, len(self.ubis))
+
+    def saveubis(self, filename):
+        """Write the UBIs found (ImageD11/indexing.py:1170-1174)."""
+        write_ubi_file(filename, self.ubis)
+
+    def saveindexing(self, filename, tol=None):
+        """Write the indexing report (ImageD11/indexing.py:907-1058): each
+        grain's peak count and mean drlv from `fight_over_peaks` followed
+        by `refine_assigned` of a copy of its UBI (the UBIs are not
+        changed), its UBI, cell, U and B, and per assigned peak its hkl,
+        drlv, detector position and observed/calculated omega, eta, tth;
+        then every ring-assigned peak with its grain.  Peak assignment and
+        refine_assigned run on the GPU, the text formatting on the host."""
+        from . import transform
+        self.gv = np.ascontiguousarray(self.gv)
+        self.fight_over_peaks()
+        g = self._gvd()
+        labels = torch.as_tensor(np.ascontiguousarray(self.ga, np.int32), device=g.device)
+        npeaks = len(self.ra)
+        allind = np.arange(npeaks)
+        tthcalc = np.zeros(npeaks, float)
+        etacalc = np.zeros(npeaks, float)
+        omegacalc = np.zeros(npeaks, float)
+        xp = getattr(self, "xp", None)
+        yp = getattr(self, "yp", None)
+        uinverses = []
+        with open(filename, "w") as f:
+            for i, ubi in enumerate(self.ubis):
+                ubi = np.asarray(ubi, float)
+                uinverses.append(np.linalg.inv(ubi))
+                npk, mdrlv = cImageD11.refine_assigned(ubi.copy(), g, labels, i)
+                assert npk == self.gas[i]
+                f.write("Grain: %d   Npeaks=%d   <drlv>=%f\n" % (i, self.gas[i], np.sqrt(mdrlv)))
+                f.write("UBI:\n" + str(ubi) + "\n")
+                f.write("Cell pars: ")
+                cellpars = ubitocellpars(ubi)
+                for v in cellpars[:3]:
+                    f.write("%10.6f " % v)
+                for v in cellpars[3:]:
+                    f.write("%10.3f " % v)
+                f.write("\n")
+                f.write("U:\n" + str(ubitoU(ubi)) + "\n")
+                f.write("B:\n" + str(ubitoB(ubi)) + "\n")
+                ind = allind[self.ga == i]
+                h = np.dot(ubi, self.gv[ind].T)
+                hint = np.floor(h + 0.5)
+                gint = np.dot(uinverses[-1], hint)
+                f.write("Peak   (  h       k       l      )   drlv             x       y ")
+                if self.wavelength < 0:
+                    f.write("\n")
+                else:
+                    f.write("   Omega_obs Omega_calc   Eta_obs Eta_calc   tth_obs tth_calc\n")
+                    tc, ec, oc = transform.uncompute_g_vectors(gint, self.wavelength,
+                                                               wedge=self.wedge)
+                for col, j in enumerate(ind):
+                    f.write("%-6d ( % 6.4f % 6.4f % 6.4f ) % 12.8f "
+                            % (j, h[0, col], h[1, col], h[2, col], np.sqrt(self.drlv2[j])))
+                    f.write(" % 7.1f % 7.1f " % (xp[j] if xp is not None else 0.0,
+                                                 yp[j] if yp is not None else 0.0))
+                    if self.wavelength < 0:
+                        f.write("\n")
+                    else:
+                        to = math.asin(self.wavelength * self.ds[j] / 2) * 360 / math.pi
+                        eo = mod_360(self.eta[j], 0)
+                        oo = self.omega[j]
+                        w = int(np.argmin([abs(ec[0][col] - eo), abs(ec[1][col] - eo)]))
+                        ec1 = ec[w][col]
+                        oc1 = mod_360(oc[w][col], oo)
+                        f.write("  % 9.4f % 9.4f     % 9.4f % 9.4f   % 9.4f % 9.4f"
+                                % (oo, oc1, eo, ec1, to, tc[col]))
+                        etacalc[j] = ec1
+                        omegacalc[j] = oc1
+                        tthcalc[j] = tc[col]
+                    if self.ra[j] == -1:
+                        f.write(" *** was not assigned to ring\n")
+                    else:
+                        f.write("\n")
+                f.write("\n\n")
+            in_rings = allind[np.asarray(self.ra) > -1]
+            f.write("\n\nAnd now listing via peaks which were assigned to rings\n")
+            nleft = 0
+            npk = 0
+            omega = self.omega if self.omega is not None else np.zeros(npeaks)
+            eta = getattr(self, "eta", np.zeros(npeaks))
+            tth = getattr(self, "tth", np.zeros(npeaks))
+            for peak in in_rings:
+                gp = self.gv[peak, :]
+                f.write("\nPeak= %-5d Ring= %-5d gv=[ % -6.4f % -6.4f % -6.4f ]   omega= % 9.4f"
+                        "   eta= % 9.4f   tth= % 9.4f\n"
+                        % (peak, self.ra[peak], gp[0], gp[1], gp[2], omega[peak], eta[peak],
+                           tth[peak]))
+                if self.ga[peak] != -1:
+                    m = self.ga[peak]
+                    hi = np.dot(self.ubis[m], gp)
+                    hint = np.floor(hi + 0.5).astype(int)
+                    f.write("Grain %-5d (%3d,%3d,%3d)" % (m, hint[0], hint[1], hint[2]))
+                    f.write("  ( % -6.4f % -6.4f % -6.4f )  " % (hi[0], hi[1], hi[2]))
+                    f.write(" omega= % 9.4f   eta= %9.4f   tth= %9.4f\n"
+                            % (omegacalc[peak], etacalc[peak], tthcalc[peak]))
+                    npk += 1
+                else:
+                    if len(self.ubis) > 0:
+                        f.write("Peak not assigned\n")
+                    else:
+                        f.write("Peak not assigned, no grains found\n")
+                    nleft += 1
+            f.write("\n\nTotal number of peaks was %d\n" % (self.gv.shape[0]))
+            f.write("Peaks assigned to grains %d\n" % npk)
+            f.write("Peaks assigned to rings but remaining unindexed %d\n" % nleft)
+            f.write("Peaks not assigned to rings at all %d\n"
+                    % int(np.sum(np.asarray(self.ra) == -1)))
+
+
+# ---- file helpers and the column-file drivers --------------------------------
+
+def write_ubi_file(filename, ubilist):
+    """3x3 matrices, one row per line ("%f %f %f"), a blank line after each
+    matrix (ImageD11/indexing.py:107-114)."""
+    with open(filename, "w") as f:
+        for u in ubilist:
+            f.write("%f %f %f\n" % (u[0][0], u[0][1], u[0][2]))
+            f.write("%f %f %f\n" % (u[1][0], u[1][1], u[1][2]))
+            f.write("%f %f %f\n\n" % (u[2][0], u[2][1], u[2][2]))
+
+
+def readubis(ubifile):
+    """The 3x3 matrices of a file written by write_ubi_file, skipping '#'
+    lines (ImageD11/indexing.py:89-104)."""
+    out, rows = [], []
+    with open(ubifile) as f:
+        for line in f:
+            if line.startswith("#"):
+                continue
+            vals = [float(x) for x in line.split()]
+            if len(vals) == 3:
+                rows.append(vals)
+            if len(rows) == 3:
+                out.append(np.array(rows))
+                rows = []
+    return out
+
+
+def ubitocellpars(ubi):
+    """(a, b, c, alpha, beta, gamma) of a UBI (ImageD11/indexing.py:117-128)."""
+    g = np.dot(ubi, np.transpose(ubi))
+    a, b, c = math.sqrt(g[0, 0]), math.sqrt(g[1, 1]), math.sqrt(g[2, 2])
+    return (a, b, c, math.degrees(math.acos(g[1, 2] / b / c)),
+            math.degrees(math.acos(g[0, 2] / a / c)), math.degrees(math.acos(g[0, 1] / a / b)))
+
+
+def ubitoU(ubi):
+    """The orientation U with inv(UBI) = U.B (ImageD11/indexing.py:131-138)."""
+    from .refinegrains import ubi_to_u
+    return ubi_to_u(ubi)
+
+
+def ubitoB(ubi):
+    """B from the Cholesky factor of UBI.UBI^T (ImageD11/indexing.py:166-169)."""
+    g = np.dot(ubi, np.transpose(ubi))
+    return np.transpose(np.linalg.inv(np.linalg.cholesky(g)))
+
+
+def mod_360(theta, target):
+    """theta plus the multiple of 360 that brings it within 180 of target
+    (ImageD11/indexing.py:172-183)."""
+    diff = theta - target
+    while diff < -180:
+        theta = theta + 360
+        diff = theta - target
+    while diff > 180:
+        theta = theta - 360
+        diff = theta - target
+    return theta
+
+
+def find_omega_ranges(omega):
+    """Whole degrees of omega that hold a peak (ImageD11/indexing.py:1254-1264)."""
+    iomega = np.round(np.asarray(omega) % 360).astype(int)
+    return int((np.bincount(iomega) > 0).sum())
+
+
+def indexer_from_colfile(colfile, **kwds):
+    """An indexer on a column file's g-vectors with the unit cell and
+    wavelength of its parameters (ImageD11/indexing.py:278-288)."""
+    uc = unitcell_mod.unitcell_from_parameters(colfile.parameters)
+    w = float(colfile.parameters.get("wavelength"))
+    gv = np.array((colfile.gx, colfile.gy, colfile.gz), float)
+    kwds.update({"unitcell": uc, "wavelength": w, "gv": gv.T})
+    ind = indexer(**kwds)
+    if "omega" in colfile.titles:
+        ind.omega_fullrange = find_omega_ranges(colfile.omega)
+    ind.colfile = colfile
+    return ind
+
+
+def indexer_from_colfile_and_ucell(colfile, ucell, **kwds):
+    """indexer_from_colfile with a given unit cell; the wavelength from
+    kwds or the column file's parameters (ImageD11/indexing.py:291-308)."""
+    w = kwds.get("wavelength")
+    if w is None:
+        w = colfile.parameters.get("wavelength")
+        if w is None:
+            raise KeyError("Wavelength must be provided either in kwds or colfile.parameters")
+    gv = np.array((colfile.gx, colfile.gy, colfile.gz), float)
+    kwds.update({"unitcell": ucell, "wavelength": float(w), "gv": gv.T})
+    ind = indexer(**kwds)
+    if "omega" in colfile.titles:
+        ind.omega_fullrange = find_omega_ranges(colfile.omega)
+    ind.colfile = colfile
+    return ind
+
+
+def index(colf, npk_tol=((400, 0.01), (200, 0.02)), cosine_tol=math.cos(math.radians(90 - 0.1)),
+          ds_tol=0.004, max_grains=1000, rmulmax=10, rings_to_use=None, maxpairs=None,
+          log_level=3):
+    """score_all_pairs for each (minpks, hkl_tol) of npk_tol on a column
+    file's peaks (ImageD11/indexing.py:1267-1306).  Returns the indexer."""
+    ind = indexer_from_colfile(colf, ds_tol=ds_tol, max_grains=max_grains,
+                               cosine_tol=cosine_tol)
+    ind.assigntorings()
+    for minpks, tol in npk_tol:
+        ind.minpks = minpks
+        ind.hkl_tol = tol
+        ind.score_all_pairs(n=maxpairs, rmulmax=rmulmax, rings_to_use=rings_to_use)
+    return ind
+
+
+def do_index(cf, dstol=0.005, hkl_tols=(0.01, 0.025, 0.05), fracs=(0.9, 0.7),
+             cosine_tol=math.cos(math.radians(90 - 0.25)), max_grains=1000, forgen=(),
+             foridx=(), unitcell=None, **kwds):
+    """Index a column file (ImageD11/indexing.py:1309-1433): keep the peaks
+    on the rings in foridx, then for each frac (minpks = frac x the peaks
+    expected per grain) and each hkl_tol, search every pair (i <= j) of the
+    rings in forgen, sparsest first.  Returns (grains, indexer)."""
+    from .grain import grain as Grain
+    for ringid in forgen:
+        if ringid not in foridx:
+            raise ValueError("All rings in forgen must be in foridx!")
+    if cosine_tol < 0:
+        import warnings
+        warnings.warn("cosine_tol given as a negative number, are you sure about that?")
+    logging.info("Indexing %d peaks", cf.nrows)
+
+    def make(colfile):
+        if unitcell is not None:
+            ind = indexer_from_colfile_and_ucell(colfile, ucell=unitcell, **kwds)
+        else:
+            ind = indexer_from_colfile(colfile)
+        ind.ds_tol = dstol
+        ind.assigntorings()
+        return ind
+    ind = make(cf)
+    pkmask = np.zeros(cf.nrows, bool)
+    for i in foridx:
+        pkmask |= ind.ra == i
+    cf_for_indexing = cf.copyrows(pkmask)
+    cf_for_indexing.parameters = cf.parameters
+    ind = make(cf_for_indexing)
+    omega_range = getattr(ind, "omega_fullrange", -1)
+    if omega_range < 0:
+        omega_range = 180
+    n_peaks_expected = 0
+    rings = []
+    for i, dstar in enumerate(ind.unitcell.ringds):
+        if ind.na[i] > 0 and i in foridx:
+            multiplicity = len(ind.unitcell.ringhkls[ind.unitcell.ringds[i]])
+            n_peaks_expected += int(multiplicity * omega_range / 180.0)
+            if i in forgen:
+                rings.append((ind.na[i], multiplicity, i))
+    rings.sort()
+    print("{} peaks expected".format(n_peaks_expected))
+    print("Trying these rings (counts, multiplicity, ring number): {}".format(rings))
+    ind.cosine_tol = cosine_tol
+    ind.max_grains = max_grains
+    threadb4 = cImageD11.cimaged11_omp_get_max_threads()
+    try:
+        cImageD11.cimaged11_omp_set_num_threads(1)
+        for frac in fracs:
+            ind.minpks = n_peaks_expected * frac
+            for ind.hkl_tol in hkl_tols:
+                for i in range(len(rings)):
+                    for j in range(i, len(rings)):
+                        ind.ring_1 = rings[i][2]
+                        ind.ring_2 = rings[j][2]
+                        ind.find()
+                        if len(ind.hits) > 0:
+                            ind.scorethem()
+                print(frac, ind.hkl_tol, len(ind.ubis))
+    finally:
+        cImageD11.cimaged11_omp_set_num_threads(threadb4)
+    grains = [Grain(ubi) for ubi in ind.ubis]
+    if unitcell is not None:
+        for g in grains:
+            g.ref_unitcell = unitcell
+    for ginc, g in enumerate(grains):
+        try:
+            g.name = g.ref_unitcell.name + ":" + str(ginc)
+        except (NameError, KeyError, AttributeError):
+            g.name = str(ginc)
+    return grains, ind

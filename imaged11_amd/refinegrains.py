@@ -35,6 +35,7 @@ functions of the same names, restated: U from UB = U.B with B the
 upper-triangular factor of the reciprocal metric).
 """
 import math
+import re
 
 import numpy as np
 import torch
@@ -131,6 +132,96 @@ def ubi_to_u(ubi):
 def u_to_ubi(u, cells):
     """UBI = inv(U.B(cell))."""
     return np.linalg.inv(np.matmul(u, b_matrix(cells)))
+
+
+# ---- symmetry-equivalent orientation choice (ImageD11/sym_u.py) -------------
+# The rotation groups makeuniq chooses within, as generator operators in the
+# "x,y,z" notation of ImageD11/sym_u.py:175-225 (cubic P432, hexagonal
+# P6/mmm, trigonal P321, rhombohedral R3, tetragonal P4, orthorhombic P222,
+# monoclinic with the 2-fold along c, a or b, triclinic).  The generators
+# fix the order in which the group's elements are generated, which decides
+# between equal traces.
+GROUP_GENERATORS = {
+    "cubic": ("z,x,y", "-y,x,z"),
+    "hexagonal": ("-y,x-y,z", "-x,-y,z", "y,x,-z"),
+    "trigonal": ("y,-x-y,z", "y,x,-z"),
+    "rhombohedralP": ("z,x,y", "-z,-y,-x"),
+    "tetragonal": ("-y,x,z", "-x,y,-z"),
+    "orthorhombic": ("-x,-y,z", "-x,y,-z"),
+    "monoclinic_c": ("-x,-y,z",),
+    "monoclinic_a": ("x,-y,-z",),
+    "monoclinic_b": ("-x,y,-z",),
+    "triclinic": ("x,y,z",),
+}
+GROUP_GENERATORS["trigonalP"] = GROUP_GENERATORS["rhombohedralP"]
+
+
+def symmetry_operator(spec):
+    """The 3x3 matrix of an operator such as "-y,x-y,z": row i is the image
+    of the i-th unit vector (the convention of sym_u.m_from_string,
+    ImageD11/sym_u.py:13-22).  Parsed, not evaluated: each component is a
+    signed sum of x, y and z."""
+    parts = [p.replace(" ", "") for p in spec.split(",")]
+    if len(parts) != 3:
+        raise ValueError("operator %r needs three components" % spec)
+    m = np.zeros((3, 3))
+    for j, part in enumerate(parts):
+        pos = 0
+        for sign, var in re.findall(r"([+-]?)([xyz])", part):
+            m["xyz".index(var), j] += -1.0 if sign == "-" else 1.0
+            pos += len(sign) + 1
+        if pos != len(part):
+            raise ValueError("cannot read component %r of %r" % (part, spec))
+    return m
+
+
+_GROUPS = {}
+
+
+def point_group(name):
+    """The elements of a rotation group, in the order the reference's
+    group.additem / makegroup build them (ImageD11/sym_u.py:64-116): the
+    identity, then each generator not yet present followed by the closure
+    (every product a.b of the list so far, in list order, appended when new;
+    equality within 1e-5)."""
+    if name not in GROUP_GENERATORS:
+        raise ValueError("%s not in %s" % (name, sorted(GROUP_GENERATORS)))
+    if name in _GROUPS:
+        return _GROUPS[name]
+    elems = [np.identity(3)]
+    member = lambda m: any(np.allclose(e, m, rtol=1e-5, atol=1e-5) for e in elems)
+    for spec in GROUP_GENERATORS[name]:
+        m = symmetry_operator(spec)
+        if not member(m):
+            elems.append(m)
+        new = True
+        while new:
+            i = 0
+            while i < len(elems):
+                j = 0
+                while j < len(elems):
+                    c = np.dot(elems[i], elems[j])
+                    new = not member(c)
+                    if new:
+                        elems.append(c)
+                    j += 1
+                i += 1
+    _GROUPS[name] = elems
+    return elems
+
+
+def find_uniq_u(u, ops):
+    """Among o.u for the group's operators o, the one with the largest
+    trace, keeping u itself unless some o.u is strictly larger and the
+    first of equal ones (ImageD11/sym_u.py:229-239)."""
+    best = np.asarray(u, float)
+    tmax = np.trace(best)
+    for o in ops:
+        cand = np.dot(o, u)
+        t = np.trace(cand)
+        if t > tmax:
+            best, tmax = cand, t
+    return np.array(best)
 
 
 def symmetrize(ubis, latticesymmetry):
@@ -422,6 +513,22 @@ class refinegrains:
 
     def getgrains(self):
         return list(self.grains.keys())
+
+    def makeuniq(self, symmetry):
+        """Replace every UBI read and every grain's UBI by its equivalent
+        under the rotation group `symmetry` (a name of GROUP_GENERATORS)
+        with the largest trace (ImageD11/refinegrains.py:288-300).  The
+        device copies of the UBIs read are refreshed; the grains' own UBIs
+        reach the device at the next evaluation."""
+        ops = point_group(symmetry)
+        for k in list(self.ubisread.keys()):
+            self.ubisread[k] = find_uniq_u(self.ubisread[k], ops)
+        for k in list(self.grains.keys()):
+            self.grains[k].set_ubi(find_uniq_u(self.grains[k].ubi, ops))
+        for st in self._state.values():
+            if st.ubi_read is not None and st.names:
+                st.ubi_read = torch.as_tensor(np.array([self.ubisread[n] for n in st.names],
+                                                       float), device=st.device)
 
     def set_translation(self, gr, sc):
         t = self.grains[(gr, sc)].translation

@@ -383,3 +383,13 @@ def cellfromstring(s, ext=None):
     latt = [float(x) for x in items[0:6]]
     symm = items[6] if len(items) > 6 else "P"
     return unitcell(latt, symm, ext=ext)
+
+
+def unitcell_from_parameters(pars):
+    """The unit cell named by a parameters object or dict: cell__a,
+    cell__b, cell__c, cell_alpha, cell_beta, cell_gamma and the lattice
+    symbol cell_lattice_[P,A,B,C,I,F,R] (ImageD11/unitcell.py:835-840)."""
+    get = pars.get
+    names = ("_a", "_b", "_c", "alpha", "beta", "gamma")
+    return unitcell([float(get("cell_%s" % n)) for n in names],
+                    get("cell_lattice_[P,A,B,C,I,F,R]"))

@@ -1,9 +1,9 @@
 """ORACLE — test infrastructure only.
 
 Sequential restatement of the indexer's ring-pair search
-(``ImageD11/indexing.py``: ``assigntorings`` :437-457, ``find`` :633-728,
-``scorethem`` :762-874, ``getind`` :1060-1088, ``fight_over_peaks``
-:876-905) on the CPU checker: every ``closest``, ``quickorient``,
+(``ImageD11/indexing.py``: ``assigntorings`` :437-457, ``score_all_pairs``
+:577-631, ``find`` :633-728, ``scorethem`` :762-874, ``getind``
+:1060-1088, ``fight_over_peaks`` :876-905) on the CPU checker: every ``closest``, ``quickorient``,
 ``score``, ``score_and_refine`` and ``score_and_assign`` goes to the C
 restatement (``liboracle.so``), and the loops are the reference's loops,
 one hit at a time, with no precomputation.  The unit-cell tables come from
@@ -106,7 +106,7 @@ def decide(hits, ga, trial_npk, follow, minpks, uniqueness, max_grains, label_ba
 
 
 def scorethem(orc, uc, gv, ga, hits, ring_1, ring_2, hkl_tol, minpks, cosine_tol, uniqueness,
-              max_grains):
+              max_grains, label_base=0):
     """The reference's scorethem (ImageD11/indexing.py:762-838): the
     decision loop with each hit's orientation, score and follow-up computed
     when it is visited.  ga is updated in place; returns (ubis, scores,
@@ -133,7 +133,7 @@ def scorethem(orc, uc, gv, ga, hits, ring_1, ring_2, hkl_tol, minpks, cosine_tol
         orc.score_and_refine(UBI, gv, tol)
         return UBI, npk, getind(orc, gv, UBI, tol)
     ubis, scores, _, left, _ = decide(hits, ga, trial_npk, follow, minpks, uniqueness,
-                                      max_grains)
+                                      max_grains, label_base)
     return ubis, scores, left
 
 
@@ -194,3 +194,100 @@ def index(orc, uc, gv, ds, ring_1, ring_2, cosine_tol=0.002, minpks=10, hkl_tol=
     ubis, scores, left = scorethem(orc, uc, gv, ga, hits, ring_1, ring_2, hkl_tol, minpks,
                                    cosine_tol, uniqueness, max_grains)
     return dict(ra=ra, ga=ga, cs=cs, hits=hits, ubis=ubis, scores=scores, left=left)
+
+
+def score_all_pairs(orc, uc, gv, ds, n=None, rmulmax=None, rings_to_use=None,
+                    cosine_tol=0.002, minpks=10, hkl_tol=0.01, ds_tol=0.005, uniqueness=0.5,
+                    max_grains=100):
+    """The reference's indexer.score_all_pairs (ImageD11/indexing.py:577-631):
+    every ordered pair of occupied rings (and each ring with itself),
+    sorted by (product of multiplicities, product of occupations, r1, r2),
+    one find + scorethem each, the peaks assigned by earlier pairs left
+    out of later finds.  Returns a dict of ra, ga, ubis, scores and the
+    ring pairs tried."""
+    uc.makerings(np.amax(ds), tol=ds_tol)
+    ra = assigntorings(ds, uc.ringds, ds_tol)
+    na = np.bincount(ra[ra >= 0], minlength=len(uc.ringds))
+    ga = np.zeros(len(gv), np.int32) - 1
+    if rings_to_use is not None:
+        rings = [r for r in rings_to_use if (ra == r).sum() > 0]
+    else:
+        rings = sorted(set(int(r) for r in ra if r >= 0))
+    mults = {r: len(uc.ringhkls[uc.ringds[r]]) for r in rings}
+    if rmulmax is not None:
+        rings = [r for r in rings if mults[r] <= rmulmax]
+    pairs = sorted((int(mults[a] * mults[b]), int(na[a] * na[b]), a, b)
+                   for a in rings for b in rings)
+    ubis, scores, tried = [], [], []
+    k = 0
+    for _, _, r1, r2 in pairs:
+        k += 1
+        cs = ring_cosines(uc, r1, r2)
+        hits = find(orc, gv, ra, ga, r1, r2, cs, cosine_tol)
+        if len(hits) == 0:
+            continue
+        u, sc, _ = scorethem(orc, uc, gv, ga, hits, r1, r2, hkl_tol, minpks, cosine_tol,
+                             uniqueness, max_grains, label_base=len(scores))
+        ubis += u
+        scores += sc
+        tried.append((r1, r2))
+        if n is not None and k > n:
+            break
+    return dict(ra=ra, ga=ga, ubis=ubis, scores=scores, tried=tried)
+
+
+def find_omega_ranges(omega):
+    """Whole degrees of omega holding a peak (ImageD11/indexing.py:1254-1264)."""
+    iomega = np.round(np.asarray(omega) % 360).astype(int)
+    return int((np.bincount(iomega) > 0).sum())
+
+
+def do_index(orc, uc, gv, omega, dstol=0.005, hkl_tols=(0.01, 0.025, 0.05), fracs=(0.9, 0.7),
+             cosine_tol=np.cos(np.radians(90 - 0.25)), max_grains=1000, forgen=(), foridx=(),
+             uniqueness=0.5):
+    """The search loop of the reference's do_index (ImageD11/indexing.py:
+    1309-1433) on g-vectors: keep the peaks on the rings in foridx, then
+    for each frac and hkl_tol every pair i <= j of the rings in forgen
+    (sorted by (occupation, multiplicity, ring)), find + scorethem with
+    minpks = frac x the peaks expected per grain.  ds is |g| as the
+    indexer computes it from g-vectors.  Returns a dict of the subset's
+    rows, ga, ubis and scores."""
+    ds = np.sqrt((gv * gv).sum(axis=1))
+    uc.makerings(np.amax(ds), tol=dstol)
+    ra = assigntorings(ds, uc.ringds, dstol)
+    keep = np.zeros(len(gv), bool)
+    for r in foridx:
+        keep |= ra == r
+    gv2 = np.ascontiguousarray(gv[keep])
+    ds2 = np.sqrt((gv2 * gv2).sum(axis=1))
+    uc.makerings(np.amax(ds2), tol=dstol)
+    ra2 = assigntorings(ds2, uc.ringds, dstol)
+    na = np.bincount(ra2[ra2 >= 0], minlength=len(uc.ringds))
+    omega_range = find_omega_ranges(omega[keep]) if omega is not None else 180
+    expected = 0
+    rings = []
+    for i in range(len(uc.ringds)):
+        if na[i] > 0 and i in foridx:
+            mult = len(uc.ringhkls[uc.ringds[i]])
+            expected += int(mult * omega_range / 180.0)
+            if i in forgen:
+                rings.append((na[i], mult, i))
+    rings.sort()
+    ga = np.zeros(len(gv2), np.int32) - 1
+    ubis, scores = [], []
+    for frac in fracs:
+        minpks = expected * frac
+        for tol in hkl_tols:
+            for i in range(len(rings)):
+                for j in range(i, len(rings)):
+                    r1, r2 = rings[i][2], rings[j][2]
+                    cs = ring_cosines(uc, r1, r2)
+                    hits = find(orc, gv2, ra2, ga, r1, r2, cs, cosine_tol)
+                    if len(hits) == 0:
+                        continue
+                    u, sc, _ = scorethem(orc, uc, gv2, ga, hits, r1, r2, tol, minpks,
+                                         cosine_tol, uniqueness, max_grains,
+                                         label_base=len(scores))
+                    ubis += u
+                    scores += sc
+    return dict(keep=keep, ga=ga, ubis=ubis, scores=scores, expected=expected, rings=rings)

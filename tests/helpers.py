@@ -21,6 +21,24 @@ def boundary_pairs(seed, n, tol, hmax):
     return ubi, gv
 
 
+def row0_boundary_pairs(seed, n, tol, hmax):
+    """g-vectors whose row 0 alone sits at the tolerance edge: h = integer +
+    (+-tol*(1 + d), 0, 0), |d| log-uniform in [1e-17, 1e-9], either sign,
+    so the sieve's row-0 cut-off (gs_sieve_radius) is tested at its edge;
+    large hmax makes the rounding of h0 (the radius margin) matter."""
+    rng = np.random.default_rng(seed)
+    U = simulate.random_rotations(1, rng)[0]
+    ub = simulate.ub_from_cell(U, *rng.uniform(2.5, 12.0, 3))
+    ubi = np.ascontiguousarray(np.linalg.inv(ub))
+    ints = rng.integers(-hmax, hmax + 1, size=(n, 3)).astype(float)
+    d = 10.0 ** rng.uniform(-17, -9, n) * rng.choice([-1.0, 1.0], n)
+    off = tol * (1.0 + d) * rng.choice([-1.0, 1.0], n)
+    h = ints.copy()
+    h[:, 0] += off
+    gv = np.ascontiguousarray(h @ ub.T)
+    return ubi, gv
+
+
 def scan_boundary_pairs(seed, n, tol, hmax, wedge=1.0, chi=-2.0, wvln=0.2):
     """Single (peak, grain) pairs for the fused compute_gv + assign path
     whose drlv2 lies very close to tol^2: a peak position xl on a ray from

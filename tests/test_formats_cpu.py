@@ -142,3 +142,23 @@ def test_xyz_lab_matches_formula_on_fixture():
                    r[7] * v1 + r[8] * v2 + dist[2]], 1)
     np.testing.assert_allclose(xl, _xyz_lab(c.xc, c.yc, p), rtol=1e-12, atol=1e-6)
     assert math.isclose(float(np.median(np.linalg.norm(xl, axis=1))), 218355.0, rel_tol=0.05)
+
+
+def test_copyrows_and_unitcell_from_parameters():
+    """columnfile.copyrows keeps the selected rows and a copy of the
+    parameters (ImageD11/columnfile.py:381-392); unitcell_from_parameters
+    reads cell__a .. cell_gamma and the lattice symbol
+    (ImageD11/unitcell.py:835-840)."""
+    from imaged11_amd import columnfile, unitcell
+    cf = columnfile.colfile_from_dict({"a": np.arange(5.0), "b": np.arange(5.0) * 2})
+    cf.setparameters({"cell__a": 4.05, "cell__b": 4.05, "cell__c": 4.05, "cell_alpha": 90.0,
+                      "cell_beta": 90.0, "cell_gamma": 90.0,
+                      "cell_lattice_[P,A,B,C,I,F,R]": "F", "wavelength": 0.2})
+    c2 = cf.copyrows(np.array([True, False, True, False, True]))
+    assert c2.nrows == 3 and list(c2.a) == [0.0, 2.0, 4.0] and list(c2.b) == [0.0, 4.0, 8.0]
+    assert c2.parameters.get("wavelength") == 0.2
+    c2.parameters.set("wavelength", 0.3)
+    assert cf.parameters.get("wavelength") == 0.2
+    uc = unitcell.unitcell_from_parameters(cf.parameters)
+    assert list(uc.lattice_parameters) == [4.05, 4.05, 4.05, 90.0, 90.0, 90.0]
+    assert uc.symmetry == "F"

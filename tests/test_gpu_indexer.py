@@ -184,9 +184,10 @@ def test_select_lists_match_oracle(dev, orc):
     rng = np.random.default_rng(0)
     g[:] = rng.normal(size=(30000, 3)) * 0.6
     ubis = wl["ubi"]
-    # tol <= 0.1 takes the sieved selection (row 0 first), above it the
-    # all-rows certified kernel
-    for tol in (0.01, 0.05, 0.1, 0.3, 1.5):
+    # tol^2 <= 0.01 takes the sieved selection (row 0 first), above it the
+    # all-rows certified kernel: 0.0999 is the sieve's side of the limit,
+    # 0.1 (tol^2 = 0.010000000000000002) the other
+    for tol in (0.01, 0.05, 0.0999, 0.1, 0.3, 1.5):
         thr = min(tol * tol, 1.0)
         idx, off = indexing.select_lists(torch.as_tensor(ubis, device=dev),
                                          torch.as_tensor(g, device=dev), thr)
@@ -211,3 +212,65 @@ def test_select_lists_boundary_pairs(dev, orc, tol, hmax):
         idx, off = indexing.select_lists(torch.as_tensor(ubi[None], device=dev),
                                          torch.as_tensor(gv, device=dev), tol * tol)
         assert np.array_equal(idx.cpu().numpy(), np.nonzero(lab == 1)[0])
+
+
+def test_score_all_pairs_simAl_and_reports(orc, tmp_path):
+    """test/ken_simul/test_all_pairs.py through the GPU indexer:
+    score_all_pairs() with the default parameters finds 20 grains on
+    simAl.gve, the same grains, scores and peak assignments as the
+    checker's sequential restatement; saveindexing writes each grain's
+    Npeaks as fight_over_peaks + refine_assigned give them (test.idx's
+    counts, as a multiset: test.idx lists test.ubi's order), and
+    saveubis writes the UBIs."""
+    ind = indexing.indexer()
+    ind.readgvfile(os.path.join(HERE, "golden", "simAl.gve"))
+    ind.score_all_pairs()
+    assert len(ind.ubis) == 20
+    cell, _, rows = read_gve()
+    uc = U.cellfromstring(cell, ext=orc)
+    gv = np.ascontiguousarray(rows[:, :3])
+    res = IO.score_all_pairs(orc, uc, gv, rows[:, 5].copy())
+    assert ind.scores == res["scores"]
+    assert np.array_equal(ind.ga, res["ga"])
+    for a, b in zip(ind.ubis, res["ubis"]):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-10)
+    idx_file = str(tmp_path / "all.idx")
+    ind.saveindexing(idx_file)
+    npeaks = [int(line.split("Npeaks=")[1].split()[0]) for line in open(idx_file)
+              if line.startswith("Grain:")]
+    _, _, gas = IO.fight_over_peaks(orc, gv, res["ubis"], 0.01)
+    assert npeaks == gas.tolist()
+    k = np.load(os.path.join(HERE, "golden", "ken_simul.npz"))
+    assert sorted(npeaks) == sorted(k["idx_npeaks"].tolist())
+    text = open(idx_file).read()
+    assert text.count("Peak   (  h       k       l      )") == 20
+    assert "Total number of peaks was %d" % len(gv) in text
+    ubi_file = str(tmp_path / "all.ubi")
+    ind.saveubis(ubi_file)
+    back = indexing.readubis(ubi_file)
+    np.testing.assert_allclose(np.array(back), np.array(ind.ubis), rtol=0, atol=5e-7)
+
+
+def test_do_index_matches_checker(orc):
+    """do_index on a column file of simAl.gve's g-vectors (unit cell and
+    wavelength in its parameters): the peaks of rings 0-5 kept, orientations
+    generated from rings 0-2 for each (frac, hkl_tol); the same 20 grains,
+    scores and assignments as the checker's restatement of the loop."""
+    from imaged11_amd import columnfile
+    cell, _, rows = read_gve()
+    a = [float(x) for x in cell.split()[:6]]
+    cf = columnfile.colfile_from_dict({"gx": rows[:, 0], "gy": rows[:, 1], "gz": rows[:, 2],
+                                       "omega": rows[:, 7]})
+    cf.setparameters({"cell__a": a[0], "cell__b": a[1], "cell__c": a[2], "cell_alpha": a[3],
+                      "cell_beta": a[4], "cell_gamma": a[5],
+                      "cell_lattice_[P,A,B,C,I,F,R]": "F", "wavelength": 0.153636})
+    grains, ind = indexing.do_index(cf, forgen=(0, 1, 2), foridx=(0, 1, 2, 3, 4, 5))
+    uc = U.cellfromstring(cell, ext=orc)
+    res = IO.do_index(orc, uc, np.ascontiguousarray(rows[:, :3]), rows[:, 7], forgen=(0, 1, 2),
+                      foridx=(0, 1, 2, 3, 4, 5))
+    assert len(grains) == 20 == len(res["ubis"])
+    assert ind.scores == res["scores"]
+    assert np.array_equal(ind.ga, res["ga"])
+    for g, b in zip(grains, res["ubis"]):
+        np.testing.assert_allclose(g.ubi, b, rtol=0, atol=1e-10)
+    assert [g.name for g in grains] == [str(i) for i in range(20)]

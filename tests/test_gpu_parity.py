@@ -455,6 +455,29 @@ def test_score_certified_boundary(dev, orc, tol, hmax):
                 assert np.array_equal(dd.cpu().numpy().view(np.int64), d.view(np.int64))
 
 
+@pytest.mark.parametrize("tol,hmax", [(0.05, 3), (0.05, 200), (0.1, 50), (1e-4, 2)])
+def test_score_sieve_row0_edge(dev, orc, tol, hmax):
+    """Pairs whose row 0 alone lies within 1e-17..1e-9 relative of tol from
+    an integer (rows 1 and 2 exact integers): the sieve's own row-0 cut-off
+    at its edge, at 1, 4 and 8 peaks per lane, counts as the oracle; the
+    sieved selection (tol^2 <= 0.01) lists what score_and_assign marks."""
+    from helpers import row0_boundary_pairs
+    for seed in range(3):
+        ubi, gv = row0_boundary_pairs(seed, 60000, tol, hmax)
+        want = orc.score(ubi, gv, tol)
+        u = T(ubi[None], dev)
+        g = T(gv, dev)
+        for ppl in (1, 4, 8):
+            got = batched.score_many(u, g, tol, peaks_per_lane=ppl, mode=_lib.SCORE_SIEVE)
+            assert got.item() == want, (seed, ppl)
+        if tol * tol <= 0.01:
+            from imaged11_amd import indexing
+            d, lab = np.ones(len(gv)), np.zeros(len(gv), np.int32)
+            orc.score_and_assign(ubi, gv, tol, d, lab, 1)
+            idx, off = indexing.select_lists(u, g, tol * tol)
+            assert np.array_equal(idx.cpu().numpy(), np.nonzero(lab == 1)[0])
+
+
 def test_boundary_pairs_audit(dev, orc):
     """The boundary-peak audit counts exactly the pairs whose reference
     drlv2 lies in [tol^2 (1 - rel), tol^2 (1 + rel)), here planted on

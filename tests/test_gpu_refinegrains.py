@@ -252,3 +252,27 @@ def test_single_grain_calls_and_savegrains(dev, orc, tmp_path):
         assert np.array_equal(np.stack([col.h[ind], col.k[ind], col.l[ind]]),
                               np.floor(hr + 0.5))
         assert 0 < g.nuniq <= len(ind)
+
+
+def test_makeuniq_cubic(dev, orc):
+    """makeuniq('cubic') replaces every UBI read and every grain's UBI by
+    its cubic equivalent of largest trace (ImageD11/refinegrains.py:
+    288-300); the lattice is cubic, so the refined grains index the same
+    peaks: labels, per-grain peak counts and gof are unchanged."""
+    rg, o, sname = _setup(orc, K=8, OmFloat=False)
+    rg.assignlabels(quiet=True)
+    col = rg.scandata[sname]
+    lab0 = np.asarray(col.labels).copy()
+    rg.grains_to_refine = [(n, sname) for n in rg.grainnames]
+    rg.parameterobj.varylist = []
+    gof0 = rg.gof([])
+    read0 = {k: v.copy() for k, v in rg.ubisread.items()}
+    rg.makeuniq("cubic")
+    ops = RG.point_group("cubic")
+    for k, v in rg.ubisread.items():
+        traces = [np.trace(m @ read0[k]) for m in ops]
+        assert np.isclose(np.trace(v), max(traces), rtol=0, atol=1e-12)
+    rg.assignlabels(quiet=True)
+    assert np.array_equal(np.asarray(col.labels), lab0)
+    gof1 = rg.gof([])
+    assert abs(gof1 - gof0) <= 1e-9 * abs(gof0)

@@ -180,3 +180,84 @@ def test_oracle_indexer_finds_20_grains(gve, orc):
     _, _, gas = IO.fight_over_peaks(orc, gv, res["ubis"], 0.01)
     k = np.load(os.path.join(HERE, "golden", "ken_simul.npz"))
     assert Counter(gas.tolist()) == Counter(k["idx_npeaks"].tolist())
+
+
+def _simal(orc):
+    cell, _, rows = read_gve()
+    uc = U.cellfromstring(cell, ext=orc)
+    return uc, np.ascontiguousarray(rows[:, :3]), rows[:, 5].copy()
+
+
+def test_oracle_score_all_pairs_simAl(orc):
+    """test/ken_simul/test_all_pairs.py: indexer().score_all_pairs() on
+    simAl.gve with the default parameters finds 20 grains; fight_over_peaks
+    with them gives the peak counts of test.idx (as a multiset: test.idx
+    lists test.ubi's grains in another order)."""
+    uc, gv, ds = _simal(orc)
+    res = IO.score_all_pairs(orc, uc, gv, ds)
+    assert len(res["ubis"]) == 20
+    _, _, gas = IO.fight_over_peaks(orc, gv, res["ubis"], 0.01)
+    k = np.load(os.path.join(HERE, "golden", "ken_simul.npz"))
+    assert sorted(gas.tolist()) == sorted(k["idx_npeaks"].tolist())
+    # every grain is one of test.ubi's, up to the cubic setting: same peaks
+    lab_t, _, _ = IO.fight_over_peaks(orc, gv, list(k["ubis"]), 0.01)
+    lab_f, _, _ = IO.fight_over_peaks(orc, gv, res["ubis"], 0.01)
+    pairs = Counter(zip(lab_t[lab_t >= 0].tolist(), lab_f[lab_t >= 0].tolist()))
+    assert len(pairs) == 20 and len({a for a, _ in pairs}) == 20
+
+
+def test_uncompute_g_vectors_matches_checker():
+    """The indexer report's angle conversion (imaged11_amd.transform) equals
+    the checker's restatement of transform.uncompute_g_vectors, which the
+    reference's own test/test_uncomputegv.py cases pin, and it inverts the
+    forward g(tth, eta, omega) of transform.compute_g_vectors."""
+    from imaged11_amd import transform as T
+    from oracle import refinegrains_oracle as R
+    rng = np.random.default_rng(3)
+    n = 500
+    tth = rng.uniform(1, 30, n)
+    eta = rng.uniform(-180, 180, n)
+    om = rng.uniform(-180, 180, n)
+    for wvl, wedge, chi in [(0.2, 0.0, 0.0), (0.3, 1.2, -0.7), (0.15, -2.0, 3.0)]:
+        g = R.compute_g_vectors(tth, eta, om, wvl, wedge=wedge, chi=chi)
+        t1, e1, o1 = T.uncompute_g_vectors(g, wvl, wedge=wedge, chi=chi)
+        t2, e2, o2 = R.uncompute_g_vectors(g, wvl, wedge=wedge, chi=chi)
+        np.testing.assert_allclose(t1, t2, rtol=0, atol=1e-10)
+        for a, b in zip(e1 + o1, e2 + o2):
+            d = np.abs(a - b)
+            d = np.minimum(d, 360 - d)
+            assert d.max() < 1e-9
+        np.testing.assert_allclose(t1, tth, rtol=0, atol=1e-9)
+        # one of the two solutions is the (eta, omega) that made g
+        de = [np.minimum(np.abs(e - eta), 360 - np.abs(e - eta)) for e in e1]
+        do = [np.minimum(np.abs(o - om), 360 - np.abs(o - om)) for o in o1]
+        best = np.minimum(np.maximum(de[0], do[0]), np.maximum(de[1], do[1]))
+        assert best.max() < 1e-8
+
+
+def test_indexer_file_helpers(tmp_path):
+    """write_ubi_file / readubis (the reference's "%f" rows and blank line),
+    ubitocellpars and ubitoB against closed forms, mod_360, and
+    find_omega_ranges."""
+    from imaged11_amd import indexing as I
+    rng = np.random.default_rng(5)
+    ubis = [rng.normal(size=(3, 3)) * 3 for _ in range(3)]
+    p = str(tmp_path / "x.ubi")
+    I.write_ubi_file(p, ubis)
+    text = open(p).read()
+    assert text.count("\n\n") == 3 and text.splitlines()[0] == "%f %f %f" % tuple(ubis[0][0])
+    back = I.readubis(p)
+    assert len(back) == 3
+    np.testing.assert_allclose(np.array(back), np.array(ubis), rtol=0, atol=5e-7)
+    # a cubic cell rotated: a = 4.05, angles 90
+    th = 0.3
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    ubi = 4.05 * R.T
+    cp = I.ubitocellpars(ubi)
+    np.testing.assert_allclose(cp, [4.05, 4.05, 4.05, 90, 90, 90], rtol=0, atol=1e-12)
+    B = I.ubitoB(ubi)
+    np.testing.assert_allclose(B @ B.T, np.linalg.inv(ubi @ ubi.T), rtol=0, atol=1e-14)
+    assert abs(B[1, 0]) < 1e-15 and abs(B[2, 0]) < 1e-15 and abs(B[2, 1]) < 1e-15
+    assert I.mod_360(350.0, 0.0) == -10.0 and I.mod_360(-350.0, 0.0) == 10.0
+    assert I.mod_360(10.0, 720.0) == 730.0
+    assert I.find_omega_ranges(np.array([0.2, 0.4, 10.6, 359.7, -0.3])) == 3

@@ -300,3 +300,49 @@ def test_oracle_refinepositions_recovers_translations(orc):
     res = rg.refinepositions({"t_x": 0.2, "t_y": 0.2, "t_z": 0.2}, maxiters=400)
     for g in range(4):
         assert np.abs(rg.grains[g].translation - w["trans"][g]).max() < 2.0, (g, res[g])
+
+
+def test_point_groups_and_unique_orientation():
+    """makeuniq's groups: the reference's own sym_u.test() assertions
+    (operator matrices, determinants, group orders, the cubic unique choice
+    of a 90-degree rotation), the cubic group against its closed form (the
+    24 signed permutation matrices of determinant +1), and find_uniq_u
+    against a brute-force maximum of the trace."""
+    import itertools
+    from imaged11_amd import refinegrains as RG
+    assert np.allclose(RG.symmetry_operator("x,y,z"), np.identity(3))
+    assert np.allclose(RG.symmetry_operator("-y,x,z"), [[0, 1, 0], [-1, 0, 0], [0, 0, 1]])
+    assert np.allclose(RG.symmetry_operator("-y,y-x,z"), [[0, -1, 0], [-1, 1, 0], [0, 0, 1]])
+    for op in ("x,y,z", "-y,x-y,z", "-y,x,z"):
+        assert np.linalg.det(RG.symmetry_operator(op)) == 1.0
+    orders = {"cubic": 24, "hexagonal": 12, "trigonal": 6, "tetragonal": 8, "orthorhombic": 4,
+              "monoclinic_a": 2, "monoclinic_b": 2, "monoclinic_c": 2, "triclinic": 1,
+              "rhombohedralP": 6}
+    for name, n in orders.items():
+        assert len(RG.point_group(name)) == n, name
+    closed = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            m = np.zeros((3, 3))
+            for i, j in enumerate(perm):
+                m[i, j] = signs[i]
+            if np.linalg.det(m) > 0:
+                closed.append(m)
+    cub = RG.point_group("cubic")
+    assert len(closed) == 24
+    for m in closed:
+        assert sum(np.allclose(m, c) for c in cub) == 1
+    assert np.allclose(RG.find_uniq_u(np.array([[0., 1, 0], [-1, 0, 0], [0, 0, 1]]), cub),
+                       np.identity(3))
+    with pytest.raises(ValueError):
+        RG.point_group("nonsense")
+    with pytest.raises(ValueError):
+        RG.symmetry_operator("2x,y,z")
+    rng = np.random.default_rng(8)
+    from imaged11_amd import simulate
+    for U in simulate.random_rotations(50, rng):
+        ubi = np.linalg.inv(U * 4.05)
+        got = RG.find_uniq_u(ubi, cub)
+        traces = [np.trace(m @ ubi) for m in closed] + [np.trace(ubi)]
+        assert np.isclose(np.trace(got), max(traces), rtol=0, atol=1e-12)
+        assert any(np.allclose(got, m @ ubi) for m in closed)
