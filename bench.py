@@ -52,6 +52,11 @@ SIEVE_SETTLE_VALU = 28
 # of two peaks per lane (24 v_fmac, 18 v_fma, 15 v_mul, 10 v_add, 6
 # v_fract, 2 v_rsq, 2 v_cmp), plus about 6 integer VALU for the label state
 SCAN_VALU_OPS_PER_PAIR = 38.5
+# FP64 flops per list entry of the adjoint pass (hkl-space loss with the
+# Gauss-Newton sums), FMA = 2, counted from its source: forward
+# compute_xlylzl + compute_gv 75, residual and loss 24, dL/dG 18, the
+# Gauss-Newton sums 30, reverse sweep 128
+ADJ_FLOPS_PER_ENTRY = 275
 PEAK_FP64_VALU = 78.6e12  # MI355X FP64 vector, FMA counted as 2
 PEAK_HBM = 8.0e12
 
@@ -71,6 +76,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the batched-refinement and adjoint measurements")
+    ap.add_argument("--profile", choices=("refine", "adjoint"), default=None,
+                    help="only run one workload a few times for a profiler (no timing "
+                         "line): the 5000-grain batched refinement or its adjoint")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -300,8 +308,19 @@ def aux_refine_adjoint(threads, with_cpu):
     _, _, hist = adjoint.refine_joint(prob, ubis0, trans, p2, pars, steps=50, lr=0.05,
                                       scales=scales)
     torch.cuda.synchronize()
+    # roofline of the evaluation (the pass over pieces and the per-grain
+    # finish, bracketed together by the events): per list entry the pass
+    # streams 44 algorithmic bytes (sc, fc, sin, cos: 32 B, hkl: 12 B) and
+    # does ADJ_FLOPS_PER_ENTRY FP64 flops (FMA = 2)
+    adj_s = adj_ms * 1e-3
     out["adjoint"] = {"eval_ms": adj_ms, "pairs": M,
-                      "pairs_per_s": M / (adj_ms * 1e-3),
+                      "pairs_per_s": M / adj_s,
+                      "pieces": prob.n_pieces, "piece_len": prob.piece_len,
+                      "roofline": {"bound": "hbm", "achieved": 44 * M / adj_s / 1e9,
+                                   "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                                   "frac": 44 * M / adj_s / PEAK_HBM,
+                                   "fp64_tflops": ADJ_FLOPS_PER_ENTRY * M / adj_s / 1e12,
+                                   "fp64_frac": ADJ_FLOPS_PER_ENTRY * M / adj_s / PEAK_FP64_VALU},
                       "joint_steps": 50, "joint_refine_ms": (time.perf_counter() - t0) * 1e3,
                       "loss_first": hist[0], "loss_last": hist[-1]}
     out["batched_refine_plus_adjoint_ms"] = batched_ms + adj_ms
@@ -707,6 +726,53 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_peaks=50_00
     return out
 
 
+def profile_workload(which, copy_mb, launches=5):
+    """The refinement or adjoint workload of aux_refine_adjoint (5000 grains,
+    lognormal lists from a 2e6-peak table), launched a few times with a
+    fixed shape, for rocprofv3: the batched refinement is 10 fused
+    score_and_refine iterations at 16 lanes per grain, 4 waves per CU and
+    lists of 853 entries or more on a whole wave (the shape the autotuner
+    picks on MI355X); the adjoint is one evaluation with the Gauss-Newton
+    sums at the default pieces and lanes."""
+    import numpy as np
+    import torch
+    from imaged11_amd import adjoint, batched, simulate
+
+    dev = torch.device("cuda:0")
+    if copy_mb:
+        src = torch.ones(copy_mb * 131072, dtype=torch.float64, device=dev)
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
+    K, N = 5000, 2_000_000
+    wl = simulate.scoring_workload(K=K, N=N, seed=3)
+    idx, off = simulate.refine_lists(wl["peaks"]["grain"], K, seed=3)
+    pk, pars = wl["peaks"], wl["pars"]
+    T = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    rng = np.random.default_rng(4)
+    ubis0 = T(wl["ubi"] @ (np.eye(3) + rng.normal(scale=1e-3, size=(K, 3, 3))))
+    idx_d, off_d = T(idx, torch.int32), T(off, torch.int64)
+    if which == "refine":
+        gv = batched.sf2gv(T(pk["sc"]), T(pk["fc"]), T(pk["omega"]), pars)
+        order = batched.list_order(off_d)
+        for _ in range(launches):
+            batched.refine_lists(ubis0, gv, idx_d, off_d, 0.05, 10, order=order,
+                                 lanes_per_grain=16, waves_per_cu=4, long_min=853)
+    else:
+        prob = adjoint.JointProblem(T(pk["sc"]), T(pk["fc"]), T(pk["omega"]), pars["omegasign"],
+                                    idx_d, off_d)
+        params = {k: torch.tensor(float(pars[k]), dtype=torch.float64, device=dev)
+                  for k in adjoint.FREE[:8]}
+        trans = torch.zeros((K, 3), dtype=torch.float64, device=dev)
+        geo = adjoint.pack_geometry(params, pars)
+        prob.assign_hkl(geo, ubis0, trans)
+        for _ in range(launches):
+            prob.evaluate(geo, ubis0, trans, want_sums=True)
+    torch.cuda.synchronize()
+    print(json.dumps({"profiled": which, "launches": launches, "grains": K, "peaks": N,
+                      "list_entries": int(off[-1])}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -722,6 +788,9 @@ def main():
     _lib.require_gpu()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.profile:
+        profile_workload(args.profile, args.copy_mb)
+        return
 
     t0 = time.perf_counter()
     wl = simulate.scoring_workload(K=args.grains, N=args.peaks, seed=args.seed)

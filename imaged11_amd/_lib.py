@@ -82,9 +82,12 @@ SIGNATURES = {
                          _vp, _vp]),
     "gs_joint_hkl": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _i, _vp, _vp, ctypes.c_longlong,
                           _vp, _vp]),
+    "gs_joint_schedule": (_i, [_vp, _i, ctypes.c_longlong, _i, _vp, _vp, _vp, ctypes.c_longlong,
+                               _vp]),
+    "gs_joint_workspace_doubles": (ctypes.c_longlong, [ctypes.c_longlong]),
     "gs_joint_adjoint": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _i, _vp, ctypes.c_longlong,
-                              _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp,
-                              _vp]),
+                              _vp, _vp, _vp, _vp, _vp, ctypes.c_longlong, _i, _i, _vp, _vp, _vp,
+                              _vp, _vp, _vp, _i, _vp, _vp]),
     "gs_graph_destroy": (_i, [_vp]),
     "closest": (None, [_vp, _vp, _ip, _dp, _i, _i]),
     "quickorient": (None, [_vp, _vp]),

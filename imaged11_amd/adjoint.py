@@ -9,14 +9,16 @@ Python loop over grains per objective evaluation.  Here the objective
 
 with G_q = compute_gv(compute_xlylzl(sc, fc), omega) for grain g's
 translation and assignments/hkl held fixed, is evaluated together with its
-exact gradient in ONE kernel pass (``gs_joint_adjoint``: a hand-written
-reverse sweep through compute_gv and compute_xlylzl per (grain, peak)
-pair).  `JointLoss` exposes it as a ``torch.autograd.Function`` of the
+exact gradient in one pass over the list entries (``gs_joint_adjoint``: a
+hand-written reverse sweep through compute_gv and compute_xlylzl per
+(grain, peak) pair) and one launch that adds up each grain's pieces.  `JointLoss` exposes it as a ``torch.autograd.Function`` of the
 UBIs, the translations and the 26 packed geometry numbers; `pack_geometry`
 maps the physical parameters (distance, y/z centre, tilts, wavelength,
 wedge, chi) onto that packing with differentiable torch operations, so
 autograd carries the gradient to them.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -92,9 +94,13 @@ class JointProblem:
 
     parity=True takes sin/cos of omega from the host's glibc, the values
     the reference's compute_gv uses (``batched.sincos_omega``); otherwise
-    the device's correctly rounded sincos."""
+    the device's correctly rounded sincos.
 
-    def __init__(self, sc, fc, omega, omegasign, idx, off, parity=False):
+    Each evaluation runs over pieces of at most ``piece_len`` consecutive
+    list entries (``gs_joint_schedule``, longest first), so a long list is
+    shared by several waves instead of setting the pass's duration."""
+
+    def __init__(self, sc, fc, omega, omegasign, idx, off, parity=False, piece_len=512):
         batched._need(sc, torch.float64, "sc", (None,))
         n = sc.shape[0]
         batched._need(fc, torch.float64, "fc", (n,))
@@ -114,14 +120,15 @@ class JointProblem:
         lengths = off[1:] - off[:-1]
         self.owner = torch.repeat_interleave(
             torch.arange(self.K, dtype=torch.int32, device=sc.device), lengths)
-        self.order = batched.list_order(off)
         self.hkl = torch.zeros((idx.numel(), 3), dtype=torch.int32, device=sc.device)
-        self.queue = torch.zeros(1, dtype=torch.int32, device=sc.device)
-        self.fin = torch.empty(128 * (GEO + 1), dtype=torch.float64, device=sc.device)
         dev = sc.device
+        self.piece_len = int(piece_len)
+        self.piece_off, self.piece_grain, self.slots = schedule(off, idx.numel(), self.piece_len)
+        self.n_pieces = self.slots.numel()
+        lib = _lib.load()
+        self.ws = torch.empty(max(1, int(lib.gs_joint_workspace_doubles(self.n_pieces))),
+                              dtype=torch.float64, device=dev)
         self.loss_g = torch.empty(self.K, dtype=torch.float64, device=dev)
-        self.geo_part = torch.empty((self.K, GEO), dtype=torch.float64, device=dev)
-        self.out = torch.empty(GEO + 1, dtype=torch.float64, device=dev)
 
     def assign_hkl(self, geo, ubis, trans):
         """Fix hkl_q = round(UBI_g . G_q) at the given state."""
@@ -146,30 +153,78 @@ class JointProblem:
             return self.rows, self.idx.numel(), None
         return (self.sc, self.fc, self.so, self.co), self.sc.shape[0], dptr(self.idx)
 
-    def evaluate(self, geo, ubis, trans, gspace=False, want_sums=False, lanes_per_grain=32):
+    def _static_args(self):
+        """The launch arguments that do not change between evaluations
+        (formed again if the per-entry copies are switched off)."""
+        cached = getattr(self, "_args", None)
+        if cached is not None and cached[0] is self.rows:
+            return cached[1], cached[2]
+        (a, b, c, d), npk, idx = self._peaks()
+        head = (dptr(a), dptr(b), dptr(c), dptr(d), npk)
+        mid = (self.K, idx, self.idx.numel(), dptr(self.off), dptr(self.hkl),
+               dptr(self.piece_off), dptr(self.piece_grain), dptr(self.slots), self.n_pieces,
+               self.piece_len)
+        self._args = (self.rows, head, mid)
+        return head, mid
+
+    def evaluate(self, geo, ubis, trans, gspace=False, want_sums=False, lanes_per_piece=16,
+                 geo_part=None):
         """One fused forward + reverse pass.  Returns (loss, dL/dUBI (K,3,3),
         dL/dt (K,3), dL/dgeo (26,), sums) with sums = (K,15) Gauss-Newton
         sums or None.  In g-space mode the loss is 1/2 sum ||UB.h - G||^2
-        and the UBI gradient slot holds dE/dUB."""
+        and the UBI gradient slot holds dE/dUB.  geo_part: an optional
+        (K,26) tensor that receives each grain's share of dL/dgeo.  The
+        outputs are views of one new allocation per call."""
         lib = _lib.load()
-        lib.gs_clear_error()
         dev = self.sc.device
-        geo = geo.detach().to(dev, torch.float64).contiguous()
-        ubis = ubis.detach().contiguous()
-        trans = trans.detach().contiguous()
-        gu = torch.empty((self.K, 3, 3), dtype=torch.float64, device=dev)
-        gt = torch.empty((self.K, 3), dtype=torch.float64, device=dev)
-        sums = torch.empty((self.K, 15), dtype=torch.float64, device=dev) if want_sums else None
-        (a, b, c, d), npk, idx = self._peaks()
+        geo, ubis, trans = (_dev64(x, dev) for x in (geo, ubis, trans))
+        K = self.K
+        n = 12 * K + GEO + 1 + (15 * K if want_sums else 0)
+        buf = torch.empty(n, dtype=torch.float64, device=dev)
+        gu = buf[:9 * K].view(K, 3, 3)
+        gt = buf[9 * K:12 * K].view(K, 3)
+        out = buf[12 * K:12 * K + GEO + 1]
+        sums = buf[12 * K + GEO + 1:].view(K, 15) if want_sums else None
+        if geo_part is not None:
+            batched._need(geo_part, torch.float64, "geo_part", (K, GEO))
+        head, mid = self._static_args()
         _lib.check(lib.gs_joint_adjoint(
-            dptr(a), dptr(b), dptr(c), dptr(d), npk,
-            dptr(geo), dptr(ubis), dptr(trans), self.K, idx, self.idx.numel(),
-            dptr(self.off), dptr(self.hkl), dptr(self.order), int(bool(gspace)),
-            dptr(self.loss_g), dptr(gu), dptr(gt), dptr(self.geo_part),
-            dptr(sums) if want_sums else None, dptr(self.out), int(lanes_per_grain),
-            dptr(self.queue), dptr(self.fin), stream_ptr()), "joint adjoint")
-        out = self.out.clone()
+            *head, dptr(geo), dptr(ubis), dptr(trans), *mid, int(bool(gspace)),
+            dptr(self.loss_g), dptr(gu), dptr(gt),
+            dptr(geo_part) if geo_part is not None else None,
+            dptr(sums) if want_sums else None, dptr(out), int(lanes_per_piece),
+            dptr(self.ws), stream_ptr()), "joint adjoint")
         return out[GEO], gu, gt, out[:GEO], sums
+
+
+def _dev64(x, dev):
+    """x as a contiguous float64 tensor on dev without a gradient (the
+    tensor itself when it already is one)."""
+    x = x.detach()
+    if x.device != dev or x.dtype != torch.float64:
+        x = x.to(dev, torch.float64)
+    return x if x.is_contiguous() else x.contiguous()
+
+
+def schedule(off, n_idx, piece_len):
+    """The pieces of a pass (``gs_joint_schedule``, host C): device tensors
+    piece_off (K+1, int64), piece_grain and slots (int32)."""
+    lib = _lib.load()
+    lib.gs_clear_error()
+    dev = off.device
+    off_h = off.detach().to("cpu", torch.int64).contiguous()
+    K = off_h.numel() - 1
+    piece_off = torch.empty(K + 1, dtype=torch.int64)
+    n = ctypes.c_longlong(0)
+    _lib.check(lib.gs_joint_schedule(off_h.data_ptr(), K, int(n_idx), int(piece_len),
+                                     piece_off.data_ptr(), None, None, 0, ctypes.byref(n)),
+               "joint schedule")
+    grain = torch.empty(max(1, n.value), dtype=torch.int32)
+    slots = torch.empty(max(1, n.value), dtype=torch.int32)
+    _lib.check(lib.gs_joint_schedule(off_h.data_ptr(), K, int(n_idx), int(piece_len),
+                                     piece_off.data_ptr(), grain.data_ptr(), slots.data_ptr(),
+                                     n.value, ctypes.byref(n)), "joint schedule")
+    return (piece_off.to(dev), grain[:n.value].to(dev), slots[:n.value].to(dev))
 
 
 class JointLoss(torch.autograd.Function):

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -80,7 +81,8 @@ struct DeviceCtx {
     bool ready = false;
     int cus = 0;
     int* counters = nullptr;  // GS_NCOUNTERS ints
-    double* fin = nullptr;    // GS_FIN_BLOCKS * GS_GEO_OUT doubles
+    double* adj_ws = nullptr;  // the adjoint's workspace when the caller passes none
+    long long adj_cap = 0;     // its size in doubles
 };
 
 constexpr int GS_MAX_PARTIAL_BLOCKS = 1024;
@@ -101,10 +103,24 @@ int get_ctx(DeviceCtx** out) {
     if (!c.ready) {
         GS_HIP(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
         GS_HIP(hipMalloc(&c.counters, sizeof(int) * GS_NCOUNTERS));
-        GS_HIP(hipMalloc(&c.fin, sizeof(double) * GS_FIN_BLOCKS * GS_GEO_OUT));
         c.ready = true;
     }
     *out = &c;
+    return GS_OK;
+}
+
+// The device's adjoint workspace, grown to at least `doubles` (calls that
+// overlap on one device must bring their own).
+int adj_workspace(DeviceCtx* ctx, long long doubles, double** out) {
+    std::lock_guard<std::mutex> lock(g_ctx_mutex);
+    if (ctx->adj_cap < doubles) {
+        if (ctx->adj_ws) GS_HIP(hipFree(ctx->adj_ws));
+        ctx->adj_ws = nullptr;
+        ctx->adj_cap = 0;
+        GS_HIP(hipMalloc(&ctx->adj_ws, sizeof(double) * (size_t)doubles));
+        ctx->adj_cap = doubles;
+    }
+    *out = ctx->adj_ws;
     return GS_OK;
 }
 
@@ -1303,44 +1319,107 @@ int gs_joint_hkl(const double* sc, const double* fc, const double* so, const dou
     return GS_OK;
 }
 
+long long gs_joint_workspace_doubles(long long n_pieces) {
+    return n_pieces < 0 ? 0 : gs_adj_ws_doubles(n_pieces);
+}
+
+int gs_joint_schedule(const long long* off, int K, long long n_idx, int piece_len,
+                      long long* piece_off, int* piece_grain, int* slots, long long cap,
+                      long long* n_pieces) {
+    if (K < 0 || n_idx < 0) return set_err(GS_ERR_ARG, "gs_joint_schedule: negative size");
+    if (piece_len < 1) return set_err(GS_ERR_ARG, "gs_joint_schedule: piece_len %d < 1", piece_len);
+    if (!piece_off || !n_pieces || (K > 0 && !off))
+        return set_err(GS_ERR_ARG, "gs_joint_schedule: null pointer");
+    // list bounds clipped as the kernels clip them
+    auto bounds = [&](int g, long long& o0, long long& o1) {
+        o1 = off[g + 1] < 0 ? 0 : (off[g + 1] > n_idx ? n_idx : off[g + 1]);
+        o0 = off[g] < 0 ? 0 : (off[g] > o1 ? o1 : off[g]);
+    };
+    long long n = 0;
+    piece_off[0] = 0;
+    for (int g = 0; g < K; g++) {
+        long long o0, o1;
+        bounds(g, o0, o1);
+        n += (o1 - o0 + piece_len - 1) / piece_len;
+        piece_off[g + 1] = n;
+    }
+    *n_pieces = n;
+    if (!piece_grain && !slots) return GS_OK;  // a query of the count
+    if (!piece_grain || !slots) return set_err(GS_ERR_ARG, "gs_joint_schedule: null pointer");
+    if (n > cap)
+        return set_err(GS_ERR_ARG, "gs_joint_schedule: %lld pieces exceed the capacity %lld", n,
+                       cap);
+    if (n > INT_MAX) return set_err(GS_ERR_ARG, "gs_joint_schedule: %lld pieces", n);
+    // launch order: by length, longest first, ties in piece order (a
+    // counting sort over the lengths 1 .. piece_len)
+    std::vector<long long> start((size_t)piece_len + 2, 0);
+    for (int g = 0; g < K; g++) {
+        long long o0, o1;
+        bounds(g, o0, o1);
+        for (long long p = piece_off[g]; p < piece_off[g + 1]; p++) {
+            piece_grain[p] = g;
+            const long long q0 = o0 + (p - piece_off[g]) * piece_len;
+            const long long len = std::min<long long>(piece_len, o1 - q0);
+            start[(size_t)(piece_len - len) + 1]++;
+        }
+    }
+    for (int b = 1; b <= piece_len; b++) start[b] += start[b - 1];
+    for (int g = 0; g < K; g++) {
+        long long o0, o1;
+        bounds(g, o0, o1);
+        for (long long p = piece_off[g]; p < piece_off[g + 1]; p++) {
+            const long long q0 = o0 + (p - piece_off[g]) * piece_len;
+            const long long len = std::min<long long>(piece_len, o1 - q0);
+            slots[start[(size_t)(piece_len - len)]++] = (int)p;
+        }
+    }
+    return GS_OK;
+}
+
 int gs_joint_adjoint(const double* sc, const double* fc, const double* so, const double* co,
                      int npk, const double* geo, const double* ubis, const double* trans, int K,
                      const int* idx, long long n_idx, const long long* off, const int* hkl,
-                     const int* order, int gspace, double* loss_g, double* grad_ubi,
-                     double* grad_t, double* geo_part, double* nsums, double* out,
-                     int lanes_per_grain, int* queue_ws, double* fin_ws, gs_stream_t stream) {
+                     const long long* piece_off, const int* piece_grain, const int* slots,
+                     long long n_pieces, int piece_len, int gspace, double* loss_g,
+                     double* grad_ubi, double* grad_t, double* geo_part, double* nsums,
+                     double* out, int lanes_per_piece, double* ws, gs_stream_t stream) {
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
-    if (npk < 0 || K < 0 || n_idx < 0) return set_err(GS_ERR_ARG, "gs_joint_adjoint: negative size");
+    if (npk < 0 || K < 0 || n_idx < 0 || n_pieces < 0)
+        return set_err(GS_ERR_ARG, "gs_joint_adjoint: negative size");
     if (!idx && n_idx > npk)
         return set_err(GS_ERR_ARG, "gs_joint_adjoint: without idx the per-entry arrays (npk = %d) "
                        "must cover the n_idx = %lld entries", npk, n_idx);
+    if (piece_len < 1) return set_err(GS_ERR_ARG, "gs_joint_adjoint: piece_len %d < 1", piece_len);
+    if (!geo || !out || !piece_off || (K > 0 && (!off || !loss_g || !grad_ubi || !grad_t)) ||
+        (n_pieces > 0 && (!piece_grain || !slots || !hkl)))
+        return set_err(GS_ERR_ARG, "gs_joint_adjoint: null pointer");
+    if (n_pieces > INT_MAX) return set_err(GS_ERR_ARG, "gs_joint_adjoint: %lld pieces", n_pieces);
     hipStream_t st = S(stream);
-    int* queue = queue_ws ? queue_ws : ctx->counters + 1;
-    GS_CHECK(fill_i32(queue, 0, 1, st));
-    int lpg = lanes_per_grain;
+    int lpg = lanes_per_piece;
     if (lpg != 16 && lpg != 32 && lpg != 64) lpg = 64;
-    const int gpw = GS_WAVE / lpg;
-    // persistent waves: enough to fill every SIMD at the kernel's occupancy
-    long long waves = std::min<long long>((K + gpw - 1) / gpw, 32LL * ctx->cus);
-    int nb = (int)std::max<long long>(1, (waves + GS_BLOCK / GS_WAVE - 1) / (GS_BLOCK / GS_WAVE));
-    const dim3 grid(nb), block(GS_BLOCK);
-#define GS_ADJ_LAUNCH(L, GSP, NS)                                                                \
-    if (idx)                                                                                     \
-        hipLaunchKernelGGL((k_joint_adjoint<L, GSP, NS, false>), grid, block, 0, st, sc, fc, so, \
-                           co, npk, geo, ubis, trans, K, idx, n_idx, off, hkl, order, loss_g,    \
-                           grad_ubi, grad_t, geo_part, nsums, queue);                            \
-    else                                                                                         \
-        hipLaunchKernelGGL((k_joint_adjoint<L, GSP, NS, true>), grid, block, 0, st, sc, fc, so,  \
-                           co, npk, geo, ubis, trans, K, idx, n_idx, off, hkl, order, loss_g,    \
-                           grad_ubi, grad_t, geo_part, nsums, queue)
+    if (!ws) GS_CHECK(adj_workspace(ctx, gs_adj_ws_doubles(n_pieces), &ws));
+    double* part = ws;
+    double* blk = ws + n_pieces * GS_ADJ_STRIDE;
+    const long long nb = gs_adj_blocks(n_pieces, lpg);
+    if (nb > INT_MAX) return set_err(GS_ERR_ARG, "gs_joint_adjoint: too many pieces");
+    const dim3 grid((unsigned)nb), block(GS_BLOCK);
+#define GS_ADJ_LAUNCH(L, GSP, NS)                                                                 \
+    if (idx)                                                                                      \
+        hipLaunchKernelGGL((k_joint_pieces<L, GSP, NS, false>), grid, block, 0, st, sc, fc, so,   \
+                           co, npk, geo, ubis, trans, K, idx, n_idx, off, hkl, piece_off,         \
+                           piece_grain, slots, n_pieces, piece_len, part, blk);                   \
+    else                                                                                          \
+        hipLaunchKernelGGL((k_joint_pieces<L, GSP, NS, true>), grid, block, 0, st, sc, fc, so,    \
+                           co, npk, geo, ubis, trans, K, idx, n_idx, off, hkl, piece_off,         \
+                           piece_grain, slots, n_pieces, piece_len, part, blk)
 #define GS_ADJ_LPG(GSP, NS)                         \
     switch (lpg) {                                  \
         case 16: GS_ADJ_LAUNCH(16, GSP, NS); break; \
         case 32: GS_ADJ_LAUNCH(32, GSP, NS); break; \
         default: GS_ADJ_LAUNCH(64, GSP, NS); break; \
     }
-    if (K > 0) {
+    if (nb > 0) {
         if (gspace) {
             if (nsums) { GS_ADJ_LPG(true, true) } else { GS_ADJ_LPG(true, false) }
         } else {
@@ -1350,13 +1429,17 @@ int gs_joint_adjoint(const double* sc, const double* fc, const double* so, const
     }
 #undef GS_ADJ_LPG
 #undef GS_ADJ_LAUNCH
-    const int fb = gs_fin_blocks(K);
-    double* fin = fin_ws ? fin_ws : ctx->fin;
-    hipLaunchKernelGGL(k_joint_reduce1, dim3(fb), dim3(GS_FIN_GROUPS * 32), 0, st, geo_part,
-                       loss_g, K, fin);
-    GS_LAUNCHED();
-    hipLaunchKernelGGL(k_joint_reduce2, dim3(1), dim3(GS_FIN_GROUPS * 32), 0, st, fin, fb, geo,
-                       out);
+    const dim3 fgrid((unsigned)(gs_fin_grain_blocks(K) + 1)), fblock(GS_FIN_THREADS);
+#define GS_FIN_LAUNCH(GSP, NS)                                                                  \
+    hipLaunchKernelGGL((k_joint_finish<GSP, NS>), fgrid, fblock, 0, st, part, piece_off,         \
+                       n_pieces, K, ubis, geo, loss_g, grad_ubi, grad_t, geo_part, nsums, blk, nb, \
+                       out)
+    if (gspace) {
+        if (nsums) { GS_FIN_LAUNCH(true, true); } else { GS_FIN_LAUNCH(true, false); }
+    } else {
+        if (nsums) { GS_FIN_LAUNCH(false, true); } else { GS_FIN_LAUNCH(false, false); }
+    }
+#undef GS_FIN_LAUNCH
     GS_LAUNCHED();
     return GS_OK;
 }

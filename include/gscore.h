@@ -287,34 +287,50 @@ int gs_graph_destroy(gs_graph_t graph);
  *
  * gs_joint_hkl: hkl[3q..] = round(UBI_g . G_q) for every list entry, owner[q]
  * = the grain of entry q.
- * gs_joint_adjoint: per grain loss_g[K], grad_ubi[K*9], grad_t[K*3],
- * geo_part[K*26]; nsums[K*15] (optional, NULL to skip) = sum h G^T and
- * sum G G^T for a Gauss-Newton UBI update; out[27] = dL/dgeo (26, slot 16
- * is dL/dwavelength) and L, reduced over grains in an order fixed by K
- * (contiguous ranges of grains, interleaved groups within each, then the
- * ranges in order: deterministic).  queue_ws: 1 int (the work queue) or
- * NULL for the library's per-device counter; fin_ws: 128 * 27 doubles of
- * reduction workspace or NULL for the library's per-device buffer (calls
- * overlapping on one device must pass their own).  The pair's forward
- * pass (compute_xlylzl, compute_gv) uses FMA and a reciprocal square root,
- * so G is within a few ulps of compute_gv's (gs_joint_hkl keeps
- * compute_gv's own arithmetic).  gspace != 0: the g-space objective
- * 1/2 sum ||UB.hkl - G||^2 (UB = UBI^-1), grad_ubi then holds dE/dUB =
- * UB.H - R with score_and_refine's R and H (src/closest.c:310-316).
- * lanes_per_grain 16, 32 or 64 (anything else: 64) lanes share a grain.
- * idx may be NULL in both calls: sc, fc, so and co are then per list entry,
- * already in list order (entry q is element q, npk >= n_idx), and the pass
- * streams them instead of gathering four cache lines per entry. */
+ * gs_joint_schedule (host arrays): the pieces of a pass.  Every grain's
+ * list [off[g], off[g+1]) (clipped to [0, n_idx]) is cut into pieces of at
+ * most piece_len consecutive entries, numbered in grain order: grain g
+ * owns pieces piece_off[g] .. piece_off[g+1]-1 (piece_off: K+1 entries),
+ * piece_grain[p] = the grain of piece p, and slots[] = the pieces by
+ * length, longest first (ties in piece order).  *n_pieces = the count;
+ * with piece_grain and slots both NULL only piece_off and the count are
+ * written (a query), otherwise both need room for the count (cap).
+ * gs_joint_workspace_doubles(n_pieces): the doubles of workspace that
+ * gs_joint_adjoint needs for n_pieces pieces (any lane count).
+ * gs_joint_adjoint: one forward + reverse pass over the pieces of a
+ * schedule (piece_len as given to gs_joint_schedule), lanes_per_piece 16,
+ * 32 or 64 (anything else: 64) lanes per piece; then per grain loss_g[K],
+ * grad_ubi[K*9], grad_t[K*3], and when not NULL geo_part[K*26] (dL/dgeo of
+ * grain g, slot 16 = dL/d(1/wavelength)) and nsums[K*15] = sum h G^T and
+ * sum G G^T (00 01 02 11 12 22) for a Gauss-Newton UBI update; out[27] =
+ * dL/dgeo (26, slot 16 is dL/dwavelength) and L.  Every sum is taken in an
+ * order fixed by the schedule (deterministic).  ws: the workspace, or NULL
+ * for the library's per-device buffer (calls overlapping on one device must
+ * pass their own).  The pair's forward pass (compute_xlylzl, compute_gv)
+ * uses FMA and a reciprocal square root, so G is within a few ulps of
+ * compute_gv's (gs_joint_hkl keeps compute_gv's own arithmetic).
+ * gspace != 0: the g-space objective 1/2 sum ||UB.hkl - G||^2 (UB =
+ * UBI^-1), grad_ubi then holds dE/dUB = UB.H - R with score_and_refine's R
+ * and H (src/closest.c:310-316).
+ * idx may be NULL in gs_joint_hkl and gs_joint_adjoint: sc, fc, so and co
+ * are then per list entry, already in list order (entry q is element q,
+ * npk >= n_idx), and the pass streams them instead of gathering four cache
+ * lines per entry. */
 int gs_joint_hkl(const double *sc, const double *fc, const double *so, const double *co,
                  int npk, const double *geo, const double *ubis, const double *trans, int K,
                  const int *idx, const int *owner, long long n_idx, int *hkl,
                  gs_stream_t stream);
+int gs_joint_schedule(const long long *off, int K, long long n_idx, int piece_len,
+                      long long *piece_off, int *piece_grain, int *slots, long long cap,
+                      long long *n_pieces);
+long long gs_joint_workspace_doubles(long long n_pieces);
 int gs_joint_adjoint(const double *sc, const double *fc, const double *so, const double *co,
                      int npk, const double *geo, const double *ubis, const double *trans, int K,
                      const int *idx, long long n_idx, const long long *off, const int *hkl,
-                     const int *order, int gspace, double *loss_g, double *grad_ubi,
-                     double *grad_t, double *geo_part, double *nsums, double *out,
-                     int lanes_per_grain, int *queue_ws, double *fin_ws, gs_stream_t stream);
+                     const long long *piece_off, const int *piece_grain, const int *slots,
+                     long long n_pieces, int piece_len, int gspace, double *loss_g,
+                     double *grad_ubi, double *grad_t, double *geo_part, double *nsums,
+                     double *out, int lanes_per_piece, double *ws, gs_stream_t stream);
 
 /* ---- compute_gv per grain + score_and_assign, all grains in order ----
  * The loop of refinegrains.assignlabels (ImageD11/refinegrains.py:

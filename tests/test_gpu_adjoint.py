@@ -312,3 +312,33 @@ def test_gauss_newton_mode_ubi_gradient_matches_direct(setup):
     # and the sums themselves: sum G G^T is symmetric positive semi-definite
     GG = sums[:, 9:]
     assert bool((GG[:, 0] >= 0).all() and (GG[:, 3] >= 0).all() and (GG[:, 5] >= 0).all())
+
+
+@pytest.mark.parametrize("piece_len", [16, 100, 4096])
+def test_pieces_and_lanes_give_the_same_results(setup, dev, piece_len):
+    """The pass over pieces: any piece length and lane count gives the
+    same per-grain loss, gradients and sums (to the rounding of a
+    different summation order), each grain's geo_part sums to the total
+    dL/dgeo, and grain g's loss_g is the sum of its own entries' terms
+    (one piece per grain at piece_len 4096 is the plain per-grain sum)."""
+    s = setup
+    pk = s["wl"]["peaks"]
+    ref = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"], want_sums=True)
+    ref_lg = s["prob"].loss_g.clone()
+    prob = adjoint.JointProblem(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev),
+                                s["pars"]["omegasign"], T(s["idx"], dev, torch.int32),
+                                T(s["off"], dev, torch.int64), piece_len=piece_len)
+    lengths = np.diff(s["off"])
+    assert prob.n_pieces == int((-(-lengths // piece_len)).sum())
+    prob.hkl.copy_(s["prob"].hkl)
+    for lanes in (16, 32, 64):
+        part = torch.empty((120, adjoint.GEO), dtype=torch.float64, device=dev)
+        got = prob.evaluate(s["geo"], s["ubis"], s["trans"], want_sums=True,
+                            lanes_per_piece=lanes, geo_part=part)
+        for x, y in zip(got, ref):
+            torch.testing.assert_close(x, y, rtol=1e-11, atol=1e-11 * float(y.abs().max()))
+        torch.testing.assert_close(prob.loss_g, ref_lg, rtol=1e-11, atol=0)
+        total = got[3].clone()
+        total[16] = total[16] * -(s["geo"][16] ** 2)  # back to d/d(1/wavelength)
+        torch.testing.assert_close(part.sum(0), total, rtol=1e-10,
+                                   atol=1e-10 * float(total.abs().max()))

@@ -297,7 +297,7 @@ def test_ctypes_signatures_match_the_header():
     import re
     src = re.sub(r"/\*.*?\*/", "", open(_lib.HEADER).read(), flags=re.S)
     arity = {}
-    for m in re.finditer(r"\b(?:int|void|double|char)\s*\*?\s*(\w+)\s*\(([^;]*?)\)\s*;", src,
+    for m in re.finditer(r"\b(?:int|void|double|char|long long)\s*\*?\s*(\w+)\s*\(([^;]*?)\)\s*;", src,
                          re.S):
         params = m.group(2).strip()
         arity[m.group(1)] = 0 if params in ("", "void") else len(params.split(","))
@@ -318,3 +318,80 @@ def test_mode_constants_match_the_header():
     assert batched.score_mode(0.05) == _lib.SCORE_SIEVE
     assert batched.score_mode(-0.1) == _lib.SCORE_SIEVE
     assert batched.score_mode(0.2) == _lib.SCORE_CERTIFIED
+
+
+def _schedule(off, n_idx, piece_len):
+    """gs_joint_schedule through ctypes on host arrays: (piece_off,
+    piece_grain, slots)."""
+    import ctypes
+    lib = _lib.load()
+    lib.gs_clear_error()
+    off = np.ascontiguousarray(off, np.int64)
+    K = len(off) - 1
+    po = np.zeros(K + 1, np.int64)
+    n = ctypes.c_longlong(-1)
+    _lib.check(lib.gs_joint_schedule(_lib.hptr(off), K, n_idx, piece_len, _lib.hptr(po), None,
+                                     None, 0, ctypes.byref(n)), "schedule query")
+    grain = np.full(max(n.value, 1), -7, np.int32)
+    slots = np.full(max(n.value, 1), -7, np.int32)
+    _lib.check(lib.gs_joint_schedule(_lib.hptr(off), K, n_idx, piece_len, _lib.hptr(po),
+                                     _lib.hptr(grain), _lib.hptr(slots), n.value, ctypes.byref(n)),
+               "schedule")
+    return po, grain[:n.value], slots[:n.value]
+
+
+@pytest.mark.parametrize("piece_len", [1, 7, 256])
+def test_joint_schedule_covers_every_list(piece_len):
+    """The pieces of the adjoint pass: each grain's clipped list is covered
+    by consecutive pieces of at most piece_len entries, in grain order, and
+    the launch order is a permutation sorted by length (longest first, ties
+    in piece order).  Lists with bounds outside [0, n_idx] are clipped the
+    way the kernels clip them; empty lists own no piece."""
+    rng = np.random.default_rng(8)
+    lengths = rng.integers(0, 3 * piece_len + 2, 40)
+    lengths[[0, 5, 6]] = 0
+    off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    n_idx = int(off[-1])
+    off[-1] = n_idx + 10  # past the end: clipped to n_idx
+    off[-2] = min(off[-2], off[-3])  # a list shorter than nothing: empty
+    po, grain, slots = _schedule(off, n_idx, piece_len)
+    K = len(off) - 1
+    o1 = np.clip(off[1:], 0, n_idx)
+    o0 = np.minimum(np.clip(off[:-1], 0, None), o1)
+    ln = o1 - o0
+    assert po[0] == 0 and np.array_equal(np.diff(po), -(-ln // piece_len))
+    assert np.array_equal(grain, np.repeat(np.arange(K), np.diff(po)))
+    # the pieces of grain g tile [o0, o1) in order
+    plen = np.empty(len(grain), np.int64)
+    for g in range(K):
+        ps = np.arange(po[g], po[g + 1])
+        q0 = o0[g] + (ps - po[g]) * piece_len
+        plen[ps] = np.minimum(piece_len, o1[g] - q0)
+        assert plen[ps].sum() == ln[g] and (plen[ps] >= 1).all()
+    assert np.array_equal(np.sort(slots), np.arange(len(grain)))
+    want = sorted(range(len(grain)), key=lambda p: (-plen[p], p))
+    assert np.array_equal(slots, want)
+
+
+def test_joint_schedule_edges():
+    lib = _lib.load()
+    po, grain, slots = _schedule(np.zeros(1, np.int64), 0, 16)  # K = 0
+    assert np.array_equal(po, [0]) and len(grain) == 0
+    po, grain, slots = _schedule(np.array([0, 0, 0]), 0, 16)  # empty lists
+    assert np.array_equal(po, [0, 0, 0]) and len(slots) == 0
+    import ctypes
+    off = np.array([0, 40], np.int64)
+    po = np.zeros(2, np.int64)
+    g = np.zeros(8, np.int32)
+    n = ctypes.c_longlong(0)
+    lib.gs_clear_error()
+    assert lib.gs_joint_schedule(_lib.hptr(off), 1, 40, 4, _lib.hptr(po), _lib.hptr(g),
+                                 _lib.hptr(g), 8, ctypes.byref(n)) != 0
+    assert n.value == 10 and b"capacity" in lib.gs_last_error()
+    lib.gs_clear_error()
+    assert lib.gs_joint_schedule(_lib.hptr(off), 1, 40, 0, _lib.hptr(po), None, None, 0,
+                                 ctypes.byref(n)) != 0
+    lib.gs_clear_error()
+    # workspace: per piece 48 doubles, per block of the pass 20
+    assert lib.gs_joint_workspace_doubles(0) == 0
+    assert lib.gs_joint_workspace_doubles(10) == 10 * 48 + 3 * 20
