@@ -222,13 +222,15 @@ def aux_refine_adjoint(threads, with_cpu):
         g = refine.RefineGraph(work, gv, idx_d, off_d, tol, n_iter, order=order,
                                lanes_per_grain=best["lanes_per_grain"],
                                waves_per_cu=best["waves_per_cu"],
-                               long_min=best.get("long_min", 0))
+                               long_min=best.get("long_min", 0),
+                               mid_min=best.get("mid_min", 0))
         run = lambda: (work.copy_(ubis0), g.replay())
     else:
         run = lambda: batched.refine_lists(ubis0, gv, idx_d, off_d, tol, n_iter, order=order,
                                            lanes_per_grain=best["lanes_per_grain"],
                                            waves_per_cu=best["waves_per_cu"],
-                                           long_min=best.get("long_min", 0))
+                                           long_min=best.get("long_min", 0),
+                                           mid_min=best.get("mid_min", 0))
     batched_ms = _events_ms(run, reps=5)
     # per-grain drop-in loop on the same GPU: lists gathered once, then the
     # reference's call pattern (n_iter score_and_refine calls per grain)
@@ -274,7 +276,8 @@ def aux_refine_adjoint(threads, with_cpu):
         u_g, n_g, m_g = batched.refine_lists(ubis0, gv, idx_d, off_d, tol, n_iter, order=order,
                                              lanes_per_grain=best["lanes_per_grain"],
                                              waves_per_cu=best["waves_per_cu"],
-                                             long_min=best.get("long_min", 0))
+                                             long_min=best.get("long_min", 0),
+                                             mid_min=best.get("mid_min", 0))
         u_g, n_g, m_g = u_g.cpu().numpy(), n_g.cpu().numpy(), m_g.cpu().numpy()
         u_c = np.array([r[0] for r in res])
         n_c = np.array([[nm[0] for nm in r[1]] for r in res]).T
@@ -663,7 +666,8 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_peaks=50_00
     g = refine.RefineGraph(work, gvl, None, off, 0.05, n_iter,
                            lanes_per_grain=best["lanes_per_grain"],
                            waves_per_cu=best["waves_per_cu"],
-                           long_min=best.get("long_min", 0))
+                           long_min=best.get("long_min", 0),
+                           mid_min=best.get("mid_min", 0))
     refine_ms = _events_ms(lambda: (work.copy_(ubis), g.replay()), reps=3)
     own = pk["grain"]
     lab = lab_p_h
@@ -686,7 +690,8 @@ def aux_scan(threads, with_cpu, K=20000, N=8_000_000, n_iter=20, cpu_peaks=50_00
            "refine_graph_ms": refine_ms,
            "refine_graph_shape": {"lanes_per_grain": best["lanes_per_grain"],
                                   "waves_per_cu": best["waves_per_cu"],
-                                  "long_min": best.get("long_min", 0)}}
+                                  "long_min": best.get("long_min", 0),
+                                  "mid_min": best.get("mid_min", 0)}}
     if with_cpu:
         from oracle import oracle
         ref = oracle.load_reference()

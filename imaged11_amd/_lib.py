@@ -66,10 +66,11 @@ SIGNATURES = {
     "gs_refine_lists": (_i, [_vp, _i, _vp, _i, _vp, ctypes.c_longlong, _vp, _vp, _d, _i, _vp, _vp,
                              _i, _i, _vp, _vp]),
     "gs_refine_lists_long": (_i, [_vp, _i, _vp, _i, _vp, ctypes.c_longlong, _vp, _vp, _d, _i,
-                                  _vp, _vp, _i, _i, ctypes.c_longlong, _vp, _vp]),
+                                  _vp, _vp, _i, _i, ctypes.c_longlong, ctypes.c_longlong, _vp,
+                                  _vp]),
     "gs_refine_graph_create_long": (_i, [ctypes.POINTER(_vp), _vp, _i, _vp, _i, _vp,
                                          ctypes.c_longlong, _vp, _vp, _d, _i, _vp, _vp, _i, _i,
-                                         ctypes.c_longlong, _vp, _vp]),
+                                         ctypes.c_longlong, ctypes.c_longlong, _vp, _vp]),
     "gs_gather_rows": (_i, [_vp, _i, _vp, ctypes.c_longlong, _vp, _vp]),
     "gs_index_decide": (_i, [_i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _i, _d, _i, _i, _vp, _i,
                              _vp, _vp, _vp, _vp]),

@@ -234,7 +234,7 @@ def gather_rows(gv, idx):
 
 
 def refine_lists(ubis, gv, idx, off, tol, n_iter=2, order=None, lanes_per_grain=0,
-                 waves_per_cu=0, inplace=False, gather=None, long_min=0):
+                 waves_per_cu=0, inplace=False, gather=None, long_min=0, mid_min=0):
     """n_iter successive score_and_refine calls per grain on CSR peak lists.
 
     ubis (K,3,3) float64; gv (N,3); idx int32 peak indices, or None when
@@ -247,8 +247,9 @@ def refine_lists(ubis, gv, idx, off, tol, n_iter=2, order=None, lanes_per_grain=
     once with `gather_rows`, so every iteration streams contiguous rows
     instead of fetching a cache line per entry; the results are
     bit-identical either way.  long_min > 0: grains whose list has at
-    least that many entries get a whole wave each (gs_refine_lists_long),
-    so the longest lists stop setting the time."""
+    least that many entries get a whole wave each, mid_min > 0: those with
+    at least that many get 32 lanes each (gs_refine_lists_long), so the
+    longest lists stop setting the time."""
     lib = _ready()
     _need(ubis, torch.float64, "ubis", (None, 3, 3))
     _need(gv, torch.float64, "gv", (None, 3))
@@ -267,12 +268,12 @@ def refine_lists(ubis, gv, idx, off, tol, n_iter=2, order=None, lanes_per_grain=
     u = ubis if inplace else ubis.clone()
     n = torch.empty((n_iter, K), dtype=torch.int32, device=gv.device)
     m = torch.empty((n_iter, K), dtype=torch.float64, device=gv.device)
-    queue = torch.empty(2, dtype=torch.int32, device=gv.device)
+    queue = torch.empty(3, dtype=torch.int32, device=gv.device)
     _lib.check(lib.gs_refine_lists_long(dptr(u), K, dptr(gv), gv.shape[0],
                                         None if idx is None else dptr(idx), n_idx,
                                         dptr(off), dptr(order), float(tol), int(n_iter), dptr(n),
                                         dptr(m), int(lanes_per_grain), int(waves_per_cu),
-                                        int(long_min), dptr(queue), stream_ptr()),
+                                        int(long_min), int(mid_min), dptr(queue), stream_ptr()),
                "refine_lists")
     return u, n, m
 

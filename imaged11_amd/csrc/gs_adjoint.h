@@ -49,7 +49,7 @@ __device__ __forceinline__ void gs_joint_forward(double s, double f, double so, 
                                                  const double* __restrict__ geo, double ds,
                                                  double t0, double t1, double t2,
                                                  gs_pair_fwd& w) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const double* p = geo;
     const double* r = geo + 4;
     const double* dist = geo + 13;
@@ -91,7 +91,7 @@ __device__ __forceinline__ void gs_joint_backward(double so, double co,
                                                   const double* __restrict__ geo, double ds,
                                                   const gs_pair_fwd& w, const double* gG,
                                                   gs_geo_acc& acc, double* gt) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const double* mat = geo + 17;
     // G = Omega(omega) v
     const double gv[3] = {co * gG[0] - so * gG[1], so * gG[0] + co * gG[1], gG[2]};
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_joint_pieces(
     const int* __restrict__ hkl, const long long* __restrict__ piece_off,
     const int* __restrict__ piece_grain, const int* __restrict__ slots, long long n_pieces,
     int piece_len, double* __restrict__ part, double* __restrict__ blk) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     constexpr bool DERIVE_U = NSUMS && !GSPACE;  // dL/dUBI from the Gauss-Newton sums
     constexpr int GPW = GS_WAVE / LPG;
     constexpr int WPB = GS_BLOCK / GS_WAVE;
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(GS_FIN_THREADS) void k_joint_finish(
     double* __restrict__ loss_g, double* __restrict__ grad_ubi, double* __restrict__ grad_t,
     double* __restrict__ geo_part, double* __restrict__ nsums, const double* __restrict__ blk,
     long long nblk, double* __restrict__ out) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     constexpr bool DERIVE_U = NSUMS && !GSPACE;
     __shared__ double s[GS_FIN_GRAINS][GS_ADJ_STRIDE];
     const int t = threadIdx.x;

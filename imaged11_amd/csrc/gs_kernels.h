@@ -665,17 +665,18 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gather_rows(const double* __restri
 
 // ---------------------------------------------------------------------------
 // One work item of k_refine_lists: the grains in slots base .. base +
-// 64/L - 1 of `order`, L lanes each, every grain's n_iter iterations.
+// 64/L - 1 of `order` below `limit` (<= K), L lanes each, every grain's
+// n_iter iterations.
 template <int L, bool ROWS>
 __device__ __forceinline__ void gs_refine_item(
-    int base, int lane, double* __restrict__ ubis, int K, const double* __restrict__ gv, int ng,
-    const int* __restrict__ idx, long long n_idx, const long long* __restrict__ off,
-    const int* __restrict__ order, double tolsq, int n_iter, int* __restrict__ n_out,
-    double* __restrict__ mean_out) {
+    int base, int limit, int lane, double* __restrict__ ubis, int K,
+    const double* __restrict__ gv, int ng, const int* __restrict__ idx, long long n_idx,
+    const long long* __restrict__ off, const int* __restrict__ order, double tolsq, int n_iter,
+    int* __restrict__ n_out, double* __restrict__ mean_out) {
     const int sub = lane / L;
     const int sl = lane - sub * L;
     const int slot = base + sub;
-    const int gq = slot < K ? order[slot] : -1;
+    const int gq = slot < limit ? order[slot] : -1;
     const bool active = gq >= 0 && gq < K;
     const int g = active ? gq : 0;
     long long o1 = active ? off[g + 1] : 0;
@@ -732,18 +733,20 @@ __device__ __forceinline__ void gs_refine_item(
 }
 
 // Batched score_and_refine over CSR peak lists: a persistent grid whose
-// waves pull work items from a queue: items [0, nlong) are the grains in
-// slots 0 .. nlong-1 of `order`, one per wave (64 lanes), the others groups
-// of 64/LPG grains (LPG lanes each) from slot nlong on.  With `order`
-// longest list first and nlong the number of lists of at least a given
-// length (k_count_long), the longest lists get a whole wave, so they no
-// longer set the makespan.  Each grain runs n_iter successive
-// score_and_refine calls on its own list (the calls of different grains
-// are independent).  Sums are reduced across the grain's lanes by
-// gs_group_sum, which leaves bit-identical totals in every lane, so each
-// lane solves the 3x3 system itself and keeps the updated UBI in
+// waves pull work items from a queue.  The slots of `order` fall into
+// three tiers: [0, n64) one grain per item on 64 lanes, [n64, n32) two
+// grains per item on 32 lanes, [n32, K) 64/LPG grains per item on LPG
+// lanes, with n64 = tiers[0] and n32 = tiers[1] (clipped to 0 <= n64 <=
+// n32 <= K; tiers NULL: every grain on LPG lanes).  With `order` longest
+// list first and the tier counts the numbers of lists of at least given
+// lengths (k_count_tiers), each list gets lanes in proportion to its
+// length, so no list sets the makespan.  Each grain runs n_iter
+// successive score_and_refine calls on its own list (the calls of
+// different grains are independent).  Sums are reduced across the grain's
+// lanes by gs_group_sum, which leaves bit-identical totals in every lane,
+// so each lane solves the 3x3 system itself and keeps the updated UBI in
 // registers.  n_out/mean_out are [n_iter][K]; queue[0] must be zero at
-// launch; nlong_p NULL means nlong = 0.
+// launch.
 // Out-of-range entries are ignored rather than trusted: a grain number
 // outside [0, K) in `order` is skipped, list bounds are clipped to
 // [0, n_idx], and peak indices outside [0, ng) are not kept.
@@ -755,45 +758,63 @@ __global__ __launch_bounds__(GS_BLOCK) void k_refine_lists(
     double* __restrict__ ubis, int K, const double* __restrict__ gv, int ng,
     const int* __restrict__ idx, long long n_idx, const long long* __restrict__ off,
     const int* __restrict__ order, double tolsq, int n_iter, int* __restrict__ n_out,
-    double* __restrict__ mean_out, const int* __restrict__ nlong_p, int* __restrict__ queue) {
+    double* __restrict__ mean_out, const int* __restrict__ tiers, int* __restrict__ queue) {
     constexpr int GPW = GS_WAVE / LPG;
     const int lane = threadIdx.x & (GS_WAVE - 1);
-    int nlong = nlong_p ? *nlong_p : 0;
-    nlong = nlong < 0 ? 0 : (nlong > K ? K : nlong);
-    const int nitems = nlong + (K - nlong + GPW - 1) / GPW;
+    int n64 = tiers ? tiers[0] : 0;
+    n64 = n64 < 0 ? 0 : (n64 > K ? K : n64);
+    int n32 = tiers ? tiers[1] : n64;
+    n32 = n32 < n64 ? n64 : (n32 > K ? K : n32);
+    const int items64 = n64;
+    const int items32 = (n32 - n64 + 1) / 2;
+    const int nitems = items64 + items32 + (K - n32 + GPW - 1) / GPW;
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(queue, 1);
         item = __shfl(item, 0, GS_WAVE);
         if (item >= nitems) break;
-        if (item < nlong)
-            gs_refine_item<GS_WAVE, ROWS>(item, lane, ubis, K, gv, ng, idx, n_idx, off, order,
-                                          tolsq, n_iter, n_out, mean_out);
+        if (item < items64)
+            gs_refine_item<GS_WAVE, ROWS>(item, n64, lane, ubis, K, gv, ng, idx, n_idx, off,
+                                          order, tolsq, n_iter, n_out, mean_out);
+        else if (item < items64 + items32)
+            gs_refine_item<32, ROWS>(n64 + 2 * (item - items64), n32, lane, ubis, K, gv, ng,
+                                     idx, n_idx, off, order, tolsq, n_iter, n_out, mean_out);
         else
-            gs_refine_item<LPG, ROWS>(nlong + (item - nlong) * GPW, lane, ubis, K, gv, ng, idx,
-                                      n_idx, off, order, tolsq, n_iter, n_out, mean_out);
+            gs_refine_item<LPG, ROWS>(n32 + (item - items64 - items32) * GPW, K, lane, ubis, K,
+                                      gv, ng, idx, n_idx, off, order, tolsq, n_iter, n_out,
+                                      mean_out);
     }
 }
 
-// nlong[0] = the number of grains whose clipped list length is at least
-// long_min (one block; the count does not depend on the grain order).
-__global__ __launch_bounds__(GS_BLOCK) void k_count_long(const long long* __restrict__ off, int K,
-                                                         long long n_idx, long long long_min,
-                                                         int* __restrict__ nlong) {
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
+// tiers[0] = the number of grains whose clipped list length is at least
+// long_min (0 when long_min <= 0), tiers[1] = the number at least mid_min
+// (tiers[0] when mid_min <= 0).  One block; the counts do not depend on
+// the grain order.
+__global__ __launch_bounds__(GS_BLOCK) void k_count_tiers(const long long* __restrict__ off, int K,
+                                                          long long n_idx, long long long_min,
+                                                          long long mid_min,
+                                                          int* __restrict__ tiers) {
+    __shared__ int cnt[2];
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
     __syncthreads();
-    int c = 0;
+    int c64 = 0, c32 = 0;
     for (int g = threadIdx.x; g < K; g += GS_BLOCK) {
         long long o1 = off[g + 1];
         o1 = o1 < 0 ? 0 : (o1 > n_idx ? n_idx : o1);
         long long o0 = off[g];
         o0 = o0 < 0 ? 0 : (o0 > o1 ? o1 : o0);
-        c += (o1 - o0) >= long_min;
+        const long long len = o1 - o0;
+        const bool is64 = long_min > 0 && len >= long_min;
+        c64 += is64;
+        c32 += mid_min > 0 ? (len >= mid_min || is64) : is64;
     }
-    atomicAdd(&cnt, c);
+    atomicAdd(&cnt[0], c64);
+    atomicAdd(&cnt[1], c32);
     __syncthreads();
-    if (threadIdx.x == 0) nlong[0] = cnt;
+    if (threadIdx.x == 0) {
+        tiers[0] = cnt[0];
+        tiers[1] = cnt[1];
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -589,19 +589,20 @@ int check_refine_args(int K, int ng, const int* idx, long long n_idx, int n_iter
 int launch_refine_lists(DeviceCtx* ctx, double* ubis, int K, const double* gv, int ng,
                         const int* idx, long long n_idx, const long long* off, const int* order,
                         double tol, int n_iter, int* n_out, double* mean_out, int lanes_per_grain,
-                        int waves_per_cu, const int* nlong, int* queue, hipStream_t st) {
+                        int waves_per_cu, const int* tiers, int* queue, hipStream_t st) {
     const double tolsq = tol * tol;
     int lpg = lanes_per_grain;
     if (lpg != 8 && lpg != 16 && lpg != 32 && lpg != 64) lpg = 16;
     int wpc = waves_per_cu > 0 ? waves_per_cu : 8;
     const int gpw = GS_WAVE / lpg;
-    long long waves_needed = (K + gpw - 1) / gpw;
+    // work items: at most one per grain with tiers (counted on the device)
+    long long waves_needed = tiers ? K : (K + gpw - 1) / gpw;
     long long waves = std::min<long long>(waves_needed, (long long)wpc * ctx->cus);
     int nb = (int)std::max<long long>(1, (waves + GS_BLOCK / GS_WAVE - 1) / (GS_BLOCK / GS_WAVE));
     const dim3 grid(nb), block(GS_BLOCK);
 #define GS_REFINE_LAUNCH(L, R)                                                                  \
     hipLaunchKernelGGL((k_refine_lists<L, R>), grid, block, 0, st, ubis, K, gv, ng, idx, n_idx, off, \
-                       order, tolsq, n_iter, n_out, mean_out, nlong, queue)
+                       order, tolsq, n_iter, n_out, mean_out, tiers, queue)
     const bool rows = idx == nullptr;  // entry q is row q of gv
     switch (lpg) {
         case 8: if (rows) GS_REFINE_LAUNCH(8, true); else GS_REFINE_LAUNCH(8, false); break;
@@ -1171,10 +1172,10 @@ int gs_refine_lists(double* ubis, int K, const double* gv, int ng, const int* id
                                mean_out, lanes_per_grain, waves_per_cu, nullptr, queue, st);
 }
 
-int launch_count_long(const long long* off, int K, long long n_idx, long long long_min,
-                      int* nlong, hipStream_t st) {
-    hipLaunchKernelGGL(k_count_long, dim3(1), dim3(GS_BLOCK), 0, st, off, K, n_idx, long_min,
-                       nlong);
+int launch_count_tiers(const long long* off, int K, long long n_idx, long long long_min,
+                       long long mid_min, int* tiers, hipStream_t st) {
+    hipLaunchKernelGGL(k_count_tiers, dim3(1), dim3(GS_BLOCK), 0, st, off, K, n_idx, long_min,
+                       mid_min, tiers);
     GS_LAUNCHED();
     return GS_OK;
 }
@@ -1182,25 +1183,25 @@ int launch_count_long(const long long* off, int K, long long n_idx, long long lo
 int gs_refine_lists_long(double* ubis, int K, const double* gv, int ng, const int* idx,
                          long long n_idx, const long long* off, const int* order, double tol,
                          int n_iter, int* n_out, double* mean_out, int lanes_per_grain,
-                         int waves_per_cu, long long long_min, int* queue_ws,
+                         int waves_per_cu, long long long_min, long long mid_min, int* queue_ws,
                          gs_stream_t stream) {
     DeviceCtx* ctx;
     GS_CHECK(get_ctx(&ctx));
     GS_CHECK(check_refine_args(K, ng, idx, n_idx, n_iter));
     if (K == 0 || n_iter == 0) return GS_OK;
     hipStream_t st = S(stream);
-    // queue_ws[0] the work-queue counter, queue_ws[1] nlong (the library's
-    // per-device counters 0 and 2 without a workspace)
+    // queue_ws[0] the work-queue counter, queue_ws[1..2] the tier counts
+    // (the library's per-device counters 0 and 2..3 without a workspace)
     int* queue = queue_ws ? queue_ws : ctx->counters;
-    int* nlong_ws = queue_ws ? queue_ws + 1 : ctx->counters + 2;
+    int* tiers_ws = queue_ws ? queue_ws + 1 : ctx->counters + 2;
     GS_CHECK(fill_i32(queue, 0, 1, st));
-    const int* nlong = nullptr;
-    if (long_min > 0) {
-        GS_CHECK(launch_count_long(off, K, n_idx, long_min, nlong_ws, st));
-        nlong = nlong_ws;
+    const int* tiers = nullptr;
+    if (long_min > 0 || mid_min > 0) {
+        GS_CHECK(launch_count_tiers(off, K, n_idx, long_min, mid_min, tiers_ws, st));
+        tiers = tiers_ws;
     }
     return launch_refine_lists(ctx, ubis, K, gv, ng, idx, n_idx, off, order, tol, n_iter, n_out,
-                               mean_out, lanes_per_grain, waves_per_cu, nlong, queue, st);
+                               mean_out, lanes_per_grain, waves_per_cu, tiers, queue, st);
 }
 
 int gs_gather_rows(const double* gv, int ng, const int* idx, long long n_idx, double* rows,
@@ -1228,7 +1229,8 @@ int gs_refine_graph_create_long(gs_graph_t* out, double* ubis, int K, const doub
                                 const int* idx, long long n_idx, const long long* off,
                                 const int* order, double tol, int n_iter, int* n_out,
                                 double* mean_out, int lanes_per_grain, int waves_per_cu,
-                                long long long_min, int* queue_ws, gs_stream_t stream) {
+                                long long long_min, long long mid_min, int* queue_ws,
+                                gs_stream_t stream) {
     if (!out) return set_err(GS_ERR_ARG, "gs_refine_graph_create: null output handle");
     *out = nullptr;
     if (!queue_ws) return set_err(GS_ERR_ARG, "gs_refine_graph_create: queue_ws (n_iter ints) needed");
@@ -1239,17 +1241,18 @@ int gs_refine_graph_create_long(gs_graph_t* out, double* ubis, int K, const doub
     hipStream_t st = S(stream);
     if (st == nullptr) return set_err(GS_ERR_ARG, "gs_refine_graph_create: capture needs a non-null stream");
     GS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    // queue_ws[0..n_iter) the per-iteration counters, queue_ws[n_iter] nlong
+    // queue_ws[0..n_iter) the per-iteration counters, queue_ws[n_iter..+1]
+    // the tier counts
     int rc = fill_i32(queue_ws, 0, n_iter, st);
-    const int* nlong = nullptr;
-    if (rc == GS_OK && long_min > 0) {
-        rc = launch_count_long(off, K, n_idx, long_min, queue_ws + n_iter, st);
-        nlong = queue_ws + n_iter;
+    const int* tiers = nullptr;
+    if (rc == GS_OK && (long_min > 0 || mid_min > 0)) {
+        rc = launch_count_tiers(off, K, n_idx, long_min, mid_min, queue_ws + n_iter, st);
+        tiers = queue_ws + n_iter;
     }
     for (int it = 0; rc == GS_OK && it < n_iter; it++)
         rc = launch_refine_lists(ctx, ubis, K, gv, ng, idx, n_idx, off, order, tol, 1,
                                  n_out + (size_t)it * K, mean_out + (size_t)it * K,
-                                 lanes_per_grain, waves_per_cu, nlong, queue_ws + it, st);
+                                 lanes_per_grain, waves_per_cu, tiers, queue_ws + it, st);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (rc != GS_OK) {
@@ -1276,7 +1279,7 @@ int gs_refine_graph_create(gs_graph_t* out, double* ubis, int K, const double* g
                            int lanes_per_grain, int waves_per_cu, int* queue_ws,
                            gs_stream_t stream) {
     return gs_refine_graph_create_long(out, ubis, K, gv, ng, idx, n_idx, off, order, tol, n_iter,
-                                       n_out, mean_out, lanes_per_grain, waves_per_cu, 0,
+                                       n_out, mean_out, lanes_per_grain, waves_per_cu, 0, 0,
                                        queue_ws, stream);
 }
 

@@ -12,8 +12,9 @@ indices into one HBM-resident g-vector table, and the n_iter successive
 * ``"graph"``: as n_iter single-iteration persistent launches captured
   once into a hipGraph and replayed (`RefineGraph`),
 
-with launch parameters (lanes per grain, resident waves per CU) picked by
-`autotune` and cached per problem shape.
+with launch parameters (lanes per grain, resident waves per CU, the list
+lengths above which a grain gets 32 or 64 lanes) picked by `autotune` and
+cached per problem shape.
 """
 import ctypes
 import json
@@ -44,7 +45,7 @@ class RefineGraph:
     of gv is list entry q."""
 
     def __init__(self, ubis, gv, idx, off, tol, n_iter, order=None, lanes_per_grain=0,
-                 waves_per_cu=0, long_min=0):
+                 waves_per_cu=0, long_min=0, mid_min=0):
         batched._need(ubis, torch.float64, "ubis", (None, 3, 3))
         batched._need(gv, torch.float64, "gv", (None, 3))
         K = ubis.shape[0]
@@ -64,7 +65,7 @@ class RefineGraph:
         dev = gv.device
         self.n = torch.zeros((n_iter, K), dtype=torch.int32, device=dev)
         self.mean = torch.zeros((n_iter, K), dtype=torch.float64, device=dev)
-        self.queue = torch.zeros(n_iter + 1, dtype=torch.int32, device=dev)
+        self.queue = torch.zeros(n_iter + 2, dtype=torch.int32, device=dev)
         self.stream = torch.cuda.Stream(device=dev)
         lib = _lib.load()
         lib.gs_clear_error()
@@ -74,8 +75,8 @@ class RefineGraph:
             ctypes.byref(handle), dptr(ubis), K, dptr(rows), rows.shape[0],
             None if ridx is None else dptr(ridx), n_idx,
             dptr(off), dptr(order), float(tol), int(n_iter), dptr(self.n), dptr(self.mean),
-            int(lanes_per_grain), int(waves_per_cu), int(long_min), dptr(self.queue),
-            stream_ptr(self.stream)), "refine graph capture")
+            int(lanes_per_grain), int(waves_per_cu), int(long_min), int(mid_min),
+            dptr(self.queue), stream_ptr(self.stream)), "refine graph capture")
         self.handle = handle
 
     def replay(self, stream=None):
@@ -113,7 +114,8 @@ def problem_key(off, n_iter):
     b = lambda v: int(2 ** round(np.log2(max(float(v), 1.0))))
     props = torch.cuda.get_device_properties(off.device)
     arch = getattr(props, "gcnArchName", props.name)
-    return "%s|K%d|L%d-%d-%d|it%d" % (arch, b(off.numel() - 1), b(q[0]), b(q[1]), b(q[2]), n_iter)
+    return "%s|K%d|L%d-%d-%d|it%d|t3" % (arch, b(off.numel() - 1), b(q[0]), b(q[1]), b(q[2]),
+                                         n_iter)
 
 
 def _time(fn, reps):
@@ -129,10 +131,11 @@ def _time(fn, reps):
 
 
 def autotune(ubis, gv, idx, off, tol, n_iter, reps=5, use_cache=True, verbose=False):
-    """Pick (mode, lanes_per_grain, waves_per_cu, long_min) for this
-    problem by timing every candidate with HIP events (long_min: 0, or the
-    90th / 99th percentile of the list lengths: the grains with lists at
-    least that long get a whole wave); results are cached on disk by
+    """Pick (mode, lanes_per_grain, waves_per_cu, long_min, mid_min) for
+    this problem by timing every candidate with HIP events (long_min: 0, or
+    the 90th / 99th percentile of the list lengths: the grains with lists
+    at least that long get a whole wave; mid_min: 0, or the 50th / 75th
+    percentile: those get 32 lanes); results are cached on disk by
     `problem_key`.  Returns the best config dict (with its time in ms)."""
     key = problem_key(off, n_iter)
     path = _cache_path()
@@ -147,23 +150,28 @@ def autotune(ubis, gv, idx, off, tol, n_iter, reps=5, use_cache=True, verbose=Fa
     order = batched.list_order(off)
     work = ubis.clone()
     lengths = (off[1:] - off[:-1]).double()
-    q = torch.quantile(lengths, torch.tensor([0.9, 0.99], dtype=torch.float64,
+    q = torch.quantile(lengths, torch.tensor([0.5, 0.75, 0.9, 0.99], dtype=torch.float64,
                                              device=lengths.device)).cpu().numpy()
-    longs = sorted({0} | {int(round(v)) for v in q if v >= 1})
+    longs = sorted({0} | {int(round(v)) for v in q[2:] if v >= 1})
+    mids = sorted({0} | {int(round(v)) for v in q[:2] if v >= 1})
     results = []
     for lanes in LANES:
+        tiers = [(lm, mm) for lm in (longs if lanes < 64 else [0])
+                 for mm in (mids if lanes < 32 else [0]) if not (lm and mm >= lm)]
         for wpc in WAVES_PER_CU:
-            for lm in (longs if lanes < 64 else [0]):
-                cfg = {"lanes_per_grain": lanes, "waves_per_cu": wpc, "long_min": lm}
+            for lm, mm in tiers:
+                cfg = {"lanes_per_grain": lanes, "waves_per_cu": wpc, "long_min": lm,
+                       "mid_min": mm}
 
                 def fused():
                     work.copy_(ubis)
                     batched.refine_lists(work, gv, idx, off, tol, n_iter, order=order,
                                          lanes_per_grain=lanes, waves_per_cu=wpc,
-                                         inplace=True, long_min=lm)
+                                         inplace=True, long_min=lm, mid_min=mm)
                 results.append((dict(cfg, mode="fused"), _time(fused, reps)))
                 g = RefineGraph(work, gv, idx, off, tol, n_iter, order=order,
-                                lanes_per_grain=lanes, waves_per_cu=wpc, long_min=lm)
+                                lanes_per_grain=lanes, waves_per_cu=wpc, long_min=lm,
+                                mid_min=mm)
 
                 def graph():
                     work.copy_(ubis)
@@ -173,8 +181,9 @@ def autotune(ubis, gv, idx, off, tol, n_iter, reps=5, use_cache=True, verbose=Fa
     results.sort(key=lambda r: r[1])
     if verbose:
         for cfg, ms in results:
-            print("%-6s lanes %2d waves/CU %2d long_min %5d : %.3f ms" % (
-                cfg["mode"], cfg["lanes_per_grain"], cfg["waves_per_cu"], cfg["long_min"], ms))
+            print("%-6s lanes %2d waves/CU %2d long_min %5d mid_min %5d : %.3f ms" % (
+                cfg["mode"], cfg["lanes_per_grain"], cfg["waves_per_cu"], cfg["long_min"],
+                cfg["mid_min"], ms))
     best = dict(results[0][0], ms=results[0][1], key=key, tuned_at=time.time())
     cache[key] = best
     if use_cache:
@@ -197,7 +206,7 @@ def refine(ubis, gv, idx, off, tol, n_iter=2, config=None):
         u = ubis.clone()
         g = RefineGraph(u, gv, idx, off, tol, n_iter, lanes_per_grain=config["lanes_per_grain"],
                         waves_per_cu=config["waves_per_cu"],
-                        long_min=config.get("long_min", 0))
+                        long_min=config.get("long_min", 0), mid_min=config.get("mid_min", 0))
         g.replay()
         n, m = g.n.clone(), g.mean.clone()
         torch.cuda.current_stream().synchronize()
@@ -206,4 +215,5 @@ def refine(ubis, gv, idx, off, tol, n_iter=2, config=None):
     return batched.refine_lists(ubis, gv, idx, off, tol, n_iter,
                                 lanes_per_grain=config["lanes_per_grain"],
                                 waves_per_cu=config["waves_per_cu"],
-                                long_min=config.get("long_min", 0))
+                                long_min=config.get("long_min", 0),
+                                mid_min=config.get("mid_min", 0))

@@ -232,17 +232,19 @@ int gs_refine_lists(double *ubis, int K, const double *gv, int ng, const int *id
                     int n_iter, int *n_out, double *mean_out, int lanes_per_grain,
                     int waves_per_cu, int *queue_ws, gs_stream_t stream);
 
-/* The same, with the grains whose list has at least long_min entries
- * (counted on the device) refined by a whole wave each and the others
- * lanes_per_grain lanes each: with `order` longest list first, the longest
- * lists no longer set the makespan.  queue_ws: 2 device ints (the counter
- * and that count; NULL = the library's per-device counters).  long_min 0
- * is gs_refine_lists.  A long grain's sums are reduced over 64 lanes, so
- * its result can differ from the lanes_per_grain launch by rounding. */
+/* The same with lanes in proportion to list length: the grains whose list
+ * has at least long_min entries (counted on the device) are refined by a
+ * whole wave each, those with at least mid_min by 32 lanes each, the others
+ * by lanes_per_grain lanes each.  With `order` longest list first, no list
+ * sets the makespan.  queue_ws: 3 device ints (the counter and the two
+ * counts; NULL = the library's per-device counters).  long_min = mid_min =
+ * 0 is gs_refine_lists; 0 switches a tier off.  A grain's sums are reduced
+ * over its lanes, so a grain in another tier can differ from the
+ * lanes_per_grain launch by rounding. */
 int gs_refine_lists_long(double *ubis, int K, const double *gv, int ng, const int *idx,
                          long long n_idx, const long long *off, const int *order, double tol,
                          int n_iter, int *n_out, double *mean_out, int lanes_per_grain,
-                         int waves_per_cu, long long long_min, int *queue_ws,
+                         int waves_per_cu, long long long_min, long long mid_min, int *queue_ws,
                          gs_stream_t stream);
 
 /* rows[q] = gv[idx[q]] (3 doubles) for q < n_idx, a NaN row where idx[q] is
@@ -266,12 +268,13 @@ int gs_refine_graph_create(gs_graph_t *out, double *ubis, int K, const double *g
                            const int *order, double tol, int n_iter, int *n_out,
                            double *mean_out, int lanes_per_grain, int waves_per_cu,
                            int *queue_ws, gs_stream_t stream);
-/* The graph of gs_refine_lists_long: queue_ws holds n_iter + 1 ints. */
+/* The graph of gs_refine_lists_long: queue_ws holds n_iter + 2 ints. */
 int gs_refine_graph_create_long(gs_graph_t *out, double *ubis, int K, const double *gv, int ng,
                                 const int *idx, long long n_idx, const long long *off,
                                 const int *order, double tol, int n_iter, int *n_out,
                                 double *mean_out, int lanes_per_grain, int waves_per_cu,
-                                long long long_min, int *queue_ws, gs_stream_t stream);
+                                long long long_min, long long mid_min, int *queue_ws,
+                                gs_stream_t stream);
 int gs_graph_launch(gs_graph_t graph, gs_stream_t stream);
 int gs_graph_destroy(gs_graph_t graph);
 

@@ -222,3 +222,37 @@ def test_long_lists_get_a_wave(dev, orc, problem, lanes):
     assert np.all(np.abs(m[0] - m_o[0]) <= MEAN_RTOL * np.abs(m_o[0]))
     assert np.all(np.abs(m - m_o) <= CHAIN_MEAN_RTOL * np.abs(m_o))
     np.testing.assert_allclose(uf.cpu().numpy(), u_o, rtol=0, atol=UBI_TOL)
+
+
+@pytest.mark.parametrize("lanes,long_q,mid_q", [(8, 0.9, 0.5), (16, 0.8, 0.4), (16, None, 0.6),
+                                                (32, 0.9, 0.5)])
+def test_three_lane_tiers(dev, orc, problem, lanes, long_q, mid_q):
+    """Lanes in proportion to list length: lists of at least long_min
+    entries on 64 lanes, at least mid_min on 32, the rest on `lanes`
+    (mid_min only, or mid tier wider than `lanes`, included).  Against the
+    oracle's per-grain loop, graph equal to fused bit for bit, and the two
+    tier counts those of the lengths."""
+    gv, idx, off = _dev(problem, dev)
+    lengths = np.diff(problem["off"])
+    long_min = int(np.quantile(lengths, long_q)) if long_q else 0
+    mid_min = int(np.quantile(lengths, mid_q))
+    n_iter = 4
+    u_o, n_o, m_o = orc.refine_lists(problem["start"], problem["gv"], problem["idx"],
+                                     problem["off"], 0.05, n_iter)
+    uf, nf, mf = batched.refine_lists(T(problem["start"], dev), gv, idx, off, 0.05, n_iter,
+                                      lanes_per_grain=lanes, long_min=long_min, mid_min=mid_min)
+    work = T(problem["start"], dev)
+    g = refine.RefineGraph(work, gv, idx, off, 0.05, n_iter, lanes_per_grain=lanes,
+                           long_min=long_min, mid_min=mid_min)
+    g.replay()
+    torch.cuda.synchronize()
+    n64 = int((lengths >= long_min).sum()) if long_min else 0
+    n32 = int(((lengths >= mid_min) | ((lengths >= long_min) if long_min else False)).sum())
+    assert int(g.queue[n_iter]) == n64 and int(g.queue[n_iter + 1]) == n32 > n64
+    assert torch.equal(g.n, nf) and torch.equal(g.mean, mf) and torch.equal(work, uf)
+    g.close()
+    assert np.array_equal(nf.cpu().numpy(), n_o)
+    m = mf.cpu().numpy()
+    assert np.all(np.abs(m[0] - m_o[0]) <= MEAN_RTOL * np.abs(m_o[0]))
+    assert np.all(np.abs(m - m_o) <= CHAIN_MEAN_RTOL * np.abs(m_o))
+    np.testing.assert_allclose(uf.cpu().numpy(), u_o, rtol=0, atol=UBI_TOL)
