@@ -176,10 +176,17 @@ def cpu_baseline(wl, tol, threads, gpu_counts):
     }
 
 
-def _events_ms(fn, reps=3):
+def _events_ms(fn, reps=3, warm_s=0.2):
+    """Mean ms per call of fn over `reps` back-to-back calls, bracketed by
+    HIP events on the current stream, after calls for at least warm_s
+    seconds (a leg that follows CPU-only work starts on a lowered clock)."""
     import torch
-    fn()
-    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while True:
+        fn()
+        torch.cuda.synchronize()
+        if time.perf_counter() - t0 >= warm_s:
+            break
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record()
     for _ in range(reps):
@@ -231,7 +238,7 @@ def aux_refine_adjoint(threads, with_cpu):
                                            waves_per_cu=best["waves_per_cu"],
                                            long_min=best.get("long_min", 0),
                                            mid_min=best.get("mid_min", 0))
-    batched_ms = _events_ms(run, reps=5)
+    batched_ms = _events_ms(run, reps=20)
     # per-grain drop-in loop on the same GPU: lists gathered once, then the
     # reference's call pattern (n_iter score_and_refine calls per grain)
     lists = [gv[idx_d[off[k]:off[k + 1]].long()].contiguous() for k in range(K)]
@@ -300,7 +307,7 @@ def aux_refine_adjoint(threads, with_cpu):
     trans = torch.zeros((K, 3), dtype=torch.float64, device=dev)
     geo = adjoint.pack_geometry(params, pars)
     prob.assign_hkl(geo, ubis0, trans)
-    adj_ms = _events_ms(lambda: prob.evaluate(geo, ubis0, trans, want_sums=True), reps=5)
+    adj_ms = _events_ms(lambda: prob.evaluate(geo, ubis0, trans, want_sums=True), reps=50)
     p2 = dict(params, distance=params["distance"] + 100.0)
     scales = {k: 1.0 for k in p2} | {"distance": 100.0, "tilt_x": 1e-3, "tilt_y": 1e-3,
                                      "tilt_z": 1e-3, "wavelength": 1e-5, "wedge": 0.01}
@@ -735,10 +742,10 @@ def profile_workload(which, copy_mb, launches=5):
     """The refinement or adjoint workload of aux_refine_adjoint (5000 grains,
     lognormal lists from a 2e6-peak table), launched a few times with a
     fixed shape, for rocprofv3: the batched refinement is 10 fused
-    score_and_refine iterations at 16 lanes per grain, 4 waves per CU and
-    lists of 853 entries or more on a whole wave (the shape the autotuner
-    picks on MI355X); the adjoint is one evaluation with the Gauss-Newton
-    sums at the default pieces and lanes."""
+    score_and_refine iterations at 16 lanes per grain, 4 waves per CU,
+    lists of 853 entries or more on 64 lanes and of 524 or more on 32 (the
+    shape the autotuner picks on MI355X); the adjoint is one evaluation
+    with the Gauss-Newton sums at the default pieces and lanes."""
     import numpy as np
     import torch
     from imaged11_amd import adjoint, batched, simulate
@@ -762,7 +769,8 @@ def profile_workload(which, copy_mb, launches=5):
         order = batched.list_order(off_d)
         for _ in range(launches):
             batched.refine_lists(ubis0, gv, idx_d, off_d, 0.05, 10, order=order,
-                                 lanes_per_grain=16, waves_per_cu=4, long_min=853)
+                                 lanes_per_grain=16, waves_per_cu=4, long_min=853,
+                                 mid_min=524)
     else:
         prob = adjoint.JointProblem(T(pk["sc"]), T(pk["fc"]), T(pk["omega"]), pars["omegasign"],
                                     idx_d, off_d)
@@ -902,6 +910,9 @@ def main():
             "peak": PEAK_FP64_VALU / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_VALU,
+            "frac_is": "algorithmic: flops_per_pair for every pair decided, over the FP64 "
+                       "peak; the kernel issues valu_ops_per_pair instructions per pair, "
+                       "and issued_valu_ops_frac_of_nonfma_peak is their issue rate",
             "traffic": measured_traffic(kernel),
             "kernel_ms": score_ms,
             "flops_per_pair": FLOPS_PER_PAIR,

@@ -788,28 +788,45 @@ __global__ __launch_bounds__(GS_BLOCK) void k_refine_lists(
 
 // tiers[0] = the number of grains whose clipped list length is at least
 // long_min (0 when long_min <= 0), tiers[1] = the number at least mid_min
-// (tiers[0] when mid_min <= 0).  One block; the counts do not depend on
-// the grain order.
-__global__ __launch_bounds__(GS_BLOCK) void k_count_tiers(const long long* __restrict__ off, int K,
-                                                          long long n_idx, long long long_min,
-                                                          long long mid_min,
-                                                          int* __restrict__ tiers) {
+// (tiers[0] when mid_min <= 0); queue[0] = 0 (the refine launch's work
+// counter, cleared here so that no separate fill is needed; NULL: none).
+// One block of GS_TIER_THREADS; each thread loads its grains' offsets
+// GS_TIER_BATCH at a time, so K up to GS_TIER_THREADS x GS_TIER_BATCH costs
+// one memory latency; each wave counts with ballots and adds its totals
+// to LDS once.  The counts do not depend on the grain order.
+#define GS_TIER_THREADS 1024
+#define GS_TIER_BATCH 8
+__global__ __launch_bounds__(GS_TIER_THREADS) void k_count_tiers(
+    const long long* __restrict__ off, int K, long long n_idx, long long long_min,
+    long long mid_min, int* __restrict__ tiers, int* __restrict__ queue) {
     __shared__ int cnt[2];
     if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0 && queue) queue[0] = 0;
     __syncthreads();
     int c64 = 0, c32 = 0;
-    for (int g = threadIdx.x; g < K; g += GS_BLOCK) {
-        long long o1 = off[g + 1];
-        o1 = o1 < 0 ? 0 : (o1 > n_idx ? n_idx : o1);
-        long long o0 = off[g];
-        o0 = o0 < 0 ? 0 : (o0 > o1 ? o1 : o0);
-        const long long len = o1 - o0;
-        const bool is64 = long_min > 0 && len >= long_min;
-        c64 += is64;
-        c32 += mid_min > 0 ? (len >= mid_min || is64) : is64;
+    for (int g0 = threadIdx.x; g0 < K; g0 += GS_TIER_THREADS * GS_TIER_BATCH) {
+        long long a[GS_TIER_BATCH], b[GS_TIER_BATCH];
+#pragma unroll
+        for (int j = 0; j < GS_TIER_BATCH; j++) {
+            const int g = g0 + j * GS_TIER_THREADS;
+            a[j] = g < K ? off[g] : 0;
+            b[j] = g < K ? off[g + 1] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < GS_TIER_BATCH; j++) {
+            long long o1 = b[j] < 0 ? 0 : (b[j] > n_idx ? n_idx : b[j]);
+            long long o0 = a[j] < 0 ? 0 : (a[j] > o1 ? o1 : a[j]);
+            const long long len = o1 - o0;  // 0 past K: counted in no tier
+            const bool is64 = long_min > 0 && len >= long_min;
+            const bool is32 = mid_min > 0 ? (len >= mid_min || is64) : is64;
+            c64 += __popcll(__ballot(is64));
+            c32 += __popcll(__ballot(is32));
+        }
     }
-    atomicAdd(&cnt[0], c64);
-    atomicAdd(&cnt[1], c32);
+    if ((threadIdx.x & (GS_WAVE - 1)) == 0) {  // c64, c32: the wave's totals
+        atomicAdd(&cnt[0], c64);
+        atomicAdd(&cnt[1], c32);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         tiers[0] = cnt[0];

@@ -1172,10 +1172,11 @@ int gs_refine_lists(double* ubis, int K, const double* gv, int ng, const int* id
                                mean_out, lanes_per_grain, waves_per_cu, nullptr, queue, st);
 }
 
+// The tier counts; queue (may be NULL) is cleared by the same launch.
 int launch_count_tiers(const long long* off, int K, long long n_idx, long long long_min,
-                       long long mid_min, int* tiers, hipStream_t st) {
-    hipLaunchKernelGGL(k_count_tiers, dim3(1), dim3(GS_BLOCK), 0, st, off, K, n_idx, long_min,
-                       mid_min, tiers);
+                       long long mid_min, int* tiers, int* queue, hipStream_t st) {
+    hipLaunchKernelGGL(k_count_tiers, dim3(1), dim3(GS_TIER_THREADS), 0, st, off, K, n_idx,
+                       long_min, mid_min, tiers, queue);
     GS_LAUNCHED();
     return GS_OK;
 }
@@ -1194,11 +1195,12 @@ int gs_refine_lists_long(double* ubis, int K, const double* gv, int ng, const in
     // (the library's per-device counters 0 and 2..3 without a workspace)
     int* queue = queue_ws ? queue_ws : ctx->counters;
     int* tiers_ws = queue_ws ? queue_ws + 1 : ctx->counters + 2;
-    GS_CHECK(fill_i32(queue, 0, 1, st));
     const int* tiers = nullptr;
-    if (long_min > 0 || mid_min > 0) {
-        GS_CHECK(launch_count_tiers(off, K, n_idx, long_min, mid_min, tiers_ws, st));
+    if (long_min > 0 || mid_min > 0) {  // the counting launch also clears the queue
+        GS_CHECK(launch_count_tiers(off, K, n_idx, long_min, mid_min, tiers_ws, queue, st));
         tiers = tiers_ws;
+    } else {
+        GS_CHECK(fill_i32(queue, 0, 1, st));
     }
     return launch_refine_lists(ctx, ubis, K, gv, ng, idx, n_idx, off, order, tol, n_iter, n_out,
                                mean_out, lanes_per_grain, waves_per_cu, tiers, queue, st);
@@ -1246,7 +1248,8 @@ int gs_refine_graph_create_long(gs_graph_t* out, double* ubis, int K, const doub
     int rc = fill_i32(queue_ws, 0, n_iter, st);
     const int* tiers = nullptr;
     if (rc == GS_OK && (long_min > 0 || mid_min > 0)) {
-        rc = launch_count_tiers(off, K, n_idx, long_min, mid_min, queue_ws + n_iter, st);
+        rc = launch_count_tiers(off, K, n_idx, long_min, mid_min, queue_ws + n_iter, nullptr,
+                                st);
         tiers = queue_ws + n_iter;
     }
     for (int it = 0; rc == GS_OK && it < n_iter; it++)
