@@ -342,3 +342,103 @@ def test_pieces_and_lanes_give_the_same_results(setup, dev, piece_len):
         total[16] = total[16] * -(s["geo"][16] ** 2)  # back to d/d(1/wavelength)
         torch.testing.assert_close(part.sum(0), total, rtol=1e-10,
                                    atol=1e-10 * float(total.abs().max()))
+
+
+@pytest.mark.parametrize("piece_len,lanes", [(1, 16), (7, 32), (512, 64), (64, 16)])
+def test_pieces_ragged_and_empty_lists(setup, dev, orc, piece_len, lanes):
+    """Ragged lists around the piece boundaries: empty lists first, in the
+    middle and last, one entry, exactly piece_len, piece_len + 1.  Every
+    grain's loss against the oracle's objective on that grain alone, empty
+    grains with all-zero outputs, and the totals the sums of the grains."""
+    s = setup
+    pk = s["wl"]["peaks"]
+    rng = np.random.default_rng(7)
+    lens = [0, 0, 1, piece_len, piece_len + 1, 0, 2 * piece_len + 3, 37, 0, 5, 0]
+    K = len(lens)
+    idx = rng.integers(0, len(pk["sc"]), int(sum(lens))).astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    gs = rng.integers(0, 120, K)
+    ubis = s["ubis"][torch.as_tensor(gs, device=dev)].contiguous()
+    trans = s["trans"][torch.as_tensor(gs, device=dev)].contiguous()
+    prob = adjoint.JointProblem(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev),
+                                s["pars"]["omegasign"], T(idx, dev, torch.int32),
+                                T(off, dev, torch.int64), piece_len=piece_len)
+    assert prob.n_pieces == sum(-(-n // piece_len) for n in lens)
+    prob.assign_hkl(s["geo"], ubis, trans)
+    part = torch.empty((K, adjoint.GEO), dtype=torch.float64, device=dev)
+    loss, gu, gt, gg, sums = prob.evaluate(s["geo"], ubis, trans, want_sums=True,
+                                           lanes_per_piece=lanes, geo_part=part)
+    lg = prob.loss_g.cpu().numpy()
+    geo = s["geo"].cpu().numpy()
+    so, co = prob.so.cpu().numpy(), prob.co.cpu().numpy()
+    hkl = prob.hkl.cpu().numpy()
+    u_h, t_h = ubis.cpu().numpy(), trans.cpu().numpy()
+    for g in range(K):
+        if lens[g] == 0:
+            assert lg[g] == 0.0
+            for x in (gu[g], gt[g], sums[g], part[g]):
+                assert bool((x == 0).all())
+            continue
+        want = orc.joint_loss(pk["sc"], pk["fc"], so, co, geo[:4], geo[4:13], geo[13:16],
+                              geo[16], geo[17:], u_h[g:g + 1], t_h[g:g + 1], idx,
+                              off[g:g + 2], hkl)
+        assert abs(lg[g] - want) <= 1e-10 * abs(want) + 1e-300, (g, lg[g], want)
+    assert abs(float(loss) - lg.sum()) <= 1e-12 * abs(lg.sum())
+    total = gg.clone()
+    total[16] = total[16] * -(s["geo"][16] ** 2)
+    torch.testing.assert_close(part.sum(0), total, rtol=1e-10,
+                               atol=1e-10 * float(total.abs().max()))
+
+
+def test_no_grains_and_no_entries(setup, dev):
+    """K = 0, and grains whose lists are all empty: zero loss and
+    gradients, no pieces, nothing launched over pieces."""
+    s = setup
+    pk = s["wl"]["peaks"]
+    for off in (np.zeros(1, np.int64), np.zeros(4, np.int64)):
+        K = len(off) - 1
+        prob = adjoint.JointProblem(T(pk["sc"], dev), T(pk["fc"], dev), T(pk["omega"], dev),
+                                    s["pars"]["omegasign"], T(np.zeros(0), dev, torch.int32),
+                                    T(off, dev, torch.int64))
+        assert prob.n_pieces == 0
+        ubis = s["ubis"][:K].contiguous()
+        trans = s["trans"][:K].contiguous()
+        prob.assign_hkl(s["geo"], ubis, trans)
+        loss, gu, gt, gg, sums = prob.evaluate(s["geo"], ubis, trans, want_sums=True)
+        assert float(loss) == 0.0 and bool((gg == 0).all())
+        assert gu.shape == (K, 3, 3) and bool((gu == 0).all()) and bool((sums == 0).all())
+
+
+def test_joint_adjoint_rejects_bad_arguments(setup, dev):
+    """The C entry point refuses what it cannot run: negative sizes, a
+    piece length below 1, missing outputs, per-entry arrays shorter than the
+    lists; every refusal is a status and a message, nothing launched."""
+    from imaged11_amd import _lib
+    from imaged11_amd._lib import dptr, stream_ptr
+    lib = _lib.load()
+    p = setup["prob"]
+    K = p.K
+    (a, b, c, d), npk, idx = p._peaks()
+    geo = setup["geo"].contiguous()
+    ubis, trans = setup["ubis"], setup["trans"]
+    gu = torch.empty((K, 3, 3), dtype=torch.float64, device=dev)
+    gt = torch.empty((K, 3), dtype=torch.float64, device=dev)
+    out = torch.empty(27, dtype=torch.float64, device=dev)
+
+    def call(**kw):
+        args = dict(npk=npk, K=K, n_idx=p.idx.numel(), n_pieces=p.n_pieces,
+                    piece_len=p.piece_len, grad_ubi=dptr(gu), out=dptr(out))
+        args.update(kw)
+        lib.gs_clear_error()
+        return lib.gs_joint_adjoint(
+            dptr(a), dptr(b), dptr(c), dptr(d), args["npk"], dptr(geo), dptr(ubis),
+            dptr(trans), args["K"], idx, args["n_idx"], dptr(p.off), dptr(p.hkl),
+            dptr(p.piece_off), dptr(p.piece_grain), dptr(p.slots), args["n_pieces"],
+            args["piece_len"], 0, dptr(p.loss_g), args["grad_ubi"], dptr(gt), None, None,
+            args["out"], 16, dptr(p.ws), stream_ptr())
+    assert call() == 0
+    for bad in (dict(K=-1), dict(n_idx=-1), dict(n_pieces=-1), dict(piece_len=0),
+                dict(grad_ubi=None), dict(out=None), dict(npk=p.idx.numel() - 1)):
+        assert call(**bad) != 0, bad
+        assert _lib.last_error(), bad
+    lib.gs_clear_error()
