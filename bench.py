@@ -239,6 +239,16 @@ def aux_refine_adjoint(threads, with_cpu):
                                            long_min=best.get("long_min", 0),
                                            mid_min=best.get("mid_min", 0))
     batched_ms = _events_ms(run, reps=20)
+    # the same iterations captured as one hipGraph of per-iteration
+    # persistent launches (the graph mode), in the tuned shape
+    gwork = ubis0.clone()
+    graph = refine.RefineGraph(gwork, gv, idx_d, off_d, tol, n_iter, order=order,
+                               lanes_per_grain=best["lanes_per_grain"],
+                               waves_per_cu=best["waves_per_cu"],
+                               long_min=best.get("long_min", 0),
+                               mid_min=best.get("mid_min", 0))
+    graph_ms = _events_ms(lambda: (gwork.copy_(ubis0), graph.replay()), reps=20)
+    graph.close()
     # per-grain drop-in loop on the same GPU: lists gathered once, then the
     # reference's call pattern (n_iter score_and_refine calls per grain)
     lists = [gv[idx_d[off[k]:off[k + 1]].long()].contiguous() for k in range(K)]
@@ -251,7 +261,8 @@ def aux_refine_adjoint(threads, with_cpu):
             gpu_c.score_and_refine(u, lists[k], tol)
     loop_ms = (time.perf_counter() - t0) * 1e3
     out = {"grains": K, "peaks": N, "list_entries": M, "n_iter": n_iter, "gen_s": gen_s,
-           "config": best, "batched_ms": batched_ms, "per_grain_gpu_loop_ms": loop_ms,
+           "config": best, "batched_ms": batched_ms, "graph_mode_ms": graph_ms,
+           "per_grain_gpu_loop_ms": loop_ms,
            "speedup_vs_gpu_loop": loop_ms / batched_ms,
            "refine_iterations_per_s": n_iter / (batched_ms * 1e-3),
            "pairs_per_s": M * n_iter / (batched_ms * 1e-3),
@@ -335,8 +346,11 @@ def aux_refine_adjoint(threads, with_cpu):
                       "loss_first": hist[0], "loss_last": hist[-1]}
     out["batched_refine_plus_adjoint_ms"] = batched_ms + adj_ms
     out["speedup_refine_plus_adjoint_vs_gpu_loop"] = loop_ms / (batched_ms + adj_ms)
+    out["speedup_graph_mode_plus_adjoint_vs_gpu_loop"] = loop_ms / (graph_ms + adj_ms)
     if "cpu_reference_ms" in out:
         out["speedup_refine_plus_adjoint_vs_cpu"] = out["cpu_reference_ms"] / (batched_ms + adj_ms)
+        out["speedup_graph_mode_plus_adjoint_vs_cpu"] = out["cpu_reference_ms"] / (graph_ms +
+                                                                                  adj_ms)
     return out
 
 
