@@ -114,7 +114,11 @@ int get_ctx(DeviceCtx** out) {
 int adj_workspace(DeviceCtx* ctx, long long doubles, double** out) {
     std::lock_guard<std::mutex> lock(g_ctx_mutex);
     if (ctx->adj_cap < doubles) {
-        if (ctx->adj_ws) GS_HIP(hipFree(ctx->adj_ws));
+        // an earlier launch may still be reading the old buffer
+        if (ctx->adj_ws) {
+            GS_HIP(hipDeviceSynchronize());
+            GS_HIP(hipFree(ctx->adj_ws));
+        }
         ctx->adj_ws = nullptr;
         ctx->adj_cap = 0;
         GS_HIP(hipMalloc(&ctx->adj_ws, sizeof(double) * (size_t)doubles));
@@ -1341,13 +1345,14 @@ int gs_joint_schedule(const long long* off, int K, long long n_idx, int piece_le
         o1 = off[g + 1] < 0 ? 0 : (off[g + 1] > n_idx ? n_idx : off[g + 1]);
         o0 = off[g] < 0 ? 0 : (off[g] > o1 ? o1 : off[g]);
     };
-    long long n = 0;
+    long long n = 0, longest = 0;
     piece_off[0] = 0;
     for (int g = 0; g < K; g++) {
         long long o0, o1;
         bounds(g, o0, o1);
         n += (o1 - o0 + piece_len - 1) / piece_len;
         piece_off[g + 1] = n;
+        longest = std::max(longest, o1 - o0);
     }
     *n_pieces = n;
     if (!piece_grain && !slots) return GS_OK;  // a query of the count
@@ -1357,8 +1362,9 @@ int gs_joint_schedule(const long long* off, int K, long long n_idx, int piece_le
                        cap);
     if (n > INT_MAX) return set_err(GS_ERR_ARG, "gs_joint_schedule: %lld pieces", n);
     // launch order: by length, longest first, ties in piece order (a
-    // counting sort over the lengths 1 .. piece_len)
-    std::vector<long long> start((size_t)piece_len + 2, 0);
+    // counting sort over the lengths 1 .. cap, cap = the longest piece)
+    const long long cap_len = std::min<long long>(piece_len, longest);
+    std::vector<long long> start((size_t)cap_len + 2, 0);
     for (int g = 0; g < K; g++) {
         long long o0, o1;
         bounds(g, o0, o1);
@@ -1366,17 +1372,17 @@ int gs_joint_schedule(const long long* off, int K, long long n_idx, int piece_le
             piece_grain[p] = g;
             const long long q0 = o0 + (p - piece_off[g]) * piece_len;
             const long long len = std::min<long long>(piece_len, o1 - q0);
-            start[(size_t)(piece_len - len) + 1]++;
+            start[(size_t)(cap_len - len) + 1]++;
         }
     }
-    for (int b = 1; b <= piece_len; b++) start[b] += start[b - 1];
+    for (long long b = 1; b <= cap_len; b++) start[(size_t)b] += start[(size_t)b - 1];
     for (int g = 0; g < K; g++) {
         long long o0, o1;
         bounds(g, o0, o1);
         for (long long p = piece_off[g]; p < piece_off[g + 1]; p++) {
             const long long q0 = o0 + (p - piece_off[g]) * piece_len;
             const long long len = std::min<long long>(piece_len, o1 - q0);
-            slots[start[(size_t)(piece_len - len)]++] = (int)p;
+            slots[start[(size_t)(cap_len - len)]++] = (int)p;
         }
     }
     return GS_OK;

@@ -375,6 +375,11 @@ def test_joint_schedule_covers_every_list(piece_len):
 
 def test_joint_schedule_edges():
     lib = _lib.load()
+    # a piece length far beyond every list: one piece per non-empty list,
+    # and the sort's buckets sized by the longest list, not piece_len
+    po, grain, slots = _schedule(np.array([0, 5, 5, 12], np.int64), 12, 2**31 - 1)
+    assert np.array_equal(po, [0, 1, 1, 2]) and np.array_equal(grain, [0, 2])
+    assert np.array_equal(slots, [1, 0])
     po, grain, slots = _schedule(np.zeros(1, np.int64), 0, 16)  # K = 0
     assert np.array_equal(po, [0]) and len(grain) == 0
     po, grain, slots = _schedule(np.array([0, 0, 0]), 0, 16)  # empty lists
