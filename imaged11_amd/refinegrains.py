@@ -350,6 +350,73 @@ class _ScanState:
             self.sign = float(omegasign)
 
 
+class _RoundGraph:
+    """One round of the lockstep position simplex for every grain of a scan
+    at once, captured as a hipGraph: compute_gv (with the omega-float step
+    when on) and two score_and_refine iterations for all grains at the
+    translations in ``trans``, then the refined UBIs stored for the grains
+    whose ``mask`` entry is set (the others are computed and dropped: a
+    grain's result depends on its own request only, so the selected grains
+    get the bits the eager round gives them).  ``out`` holds (npks, mean
+    drlv2) of every grain.  Triclinic lattices only (the other symmetries
+    project the UBIs on the host between the two iterations)."""
+
+    def __init__(self, rg, sname, pos):
+        st = rg._state[sname]
+        dev = st.device
+        self.rg, self.sname = rg, sname
+        self.pos = np.asarray(pos, np.int64)
+        R = len(self.pos)
+        self.pos_d = torch.as_tensor(self.pos, device=dev)
+        self.trans = torch.zeros((R, 3), dtype=torch.float64, device=dev)
+        self.mask = torch.zeros(R, dtype=torch.bool, device=dev)
+        self.trans_h = torch.zeros((R, 3), dtype=torch.float64, pin_memory=True)
+        self.mask_h = torch.zeros(R, dtype=torch.bool, pin_memory=True)
+        self.out_h = torch.zeros((2, R), dtype=torch.float64, pin_memory=True)
+        # every lazily built input exists before capture: the lists of
+        # these positions, their order, sin/cos of omega
+        st.trig(rg.parameterobj.parameters["omegasign"])
+        _, _, _, off_d = rg._requests(sname, self.pos)
+        rg._order_of(sname, off_d)
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):  # one eager pass (mask clear: no update)
+            self._body()
+        torch.cuda.current_stream(dev).wait_stream(side)
+        torch.cuda.synchronize(dev)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = self._body()
+
+    def _body(self):
+        rg, st = self.rg, self.rg._state[self.sname]
+        idx, owner, off, off_d = rg._requests(self.sname, self.pos)
+        gv, _, _, _ = rg._gv_device(self.sname, idx, owner, off_d, self.trans, None, False,
+                                    ubis=st.ubi_cur[self.pos_d])
+        u = st.ubi_read[self.pos_d].contiguous()
+        u, n, m = batched.refine_lists(u, gv, None, off_d, rg.tolerance, 2,
+                                       order=rg._order_of(self.sname, off_d))
+        st.ubi_cur[self.pos_d] = torch.where(self.mask[:, None, None], u,
+                                             st.ubi_cur[self.pos_d])
+        return torch.stack([n[1].double(), m[1]])
+
+    def __call__(self, trans, which):
+        """(npks, mean drlv2) of the grains `which` (indices into pos) at
+        their rows of trans (R,3); their UBIs updated as the eager round
+        updates them."""
+        self.trans_h.numpy()[:] = trans
+        mh = self.mask_h.numpy()
+        mh[:] = False
+        mh[which] = True
+        self.trans.copy_(self.trans_h, non_blocking=True)
+        self.mask.copy_(self.mask_h, non_blocking=True)
+        self.graph.replay()
+        self.out_h.copy_(self.out, non_blocking=True)
+        torch.cuda.current_stream(self.trans.device).synchronize()
+        nm = self.out_h.numpy()
+        return nm[0][which].astype(np.int64), nm[1][which].copy()
+
+
 class refinegrains:
     """Refine grain orientations, positions and detector parameters
     (``ImageD11/refinegrains.py:137-820``), batched over grains on the GPU.
@@ -404,6 +471,9 @@ class refinegrains:
         self._pos = {}
         self.npks = 0
         self.avg_drlv2 = 0.0
+        # refinepositions replays one captured evaluation per simplex round
+        # (triclinic lattices); False runs each round's launches eagerly
+        self.round_graphs = True
 
     # -- files -------------------------------------------------------------
     def loadparameters(self, filename):
@@ -586,8 +656,11 @@ class refinegrains:
         st = self._state[sname]
         p = self.parameterobj.parameters
         st.trig(p["omegasign"])
-        trans_d = torch.as_tensor(np.ascontiguousarray(trans, dtype=np.float64),
-                                  device=st.device).reshape(-1, 3)
+        if isinstance(trans, torch.Tensor):
+            trans_d = trans.reshape(-1, 3)
+        else:
+            trans_d = torch.as_tensor(np.ascontiguousarray(trans, dtype=np.float64),
+                                      device=st.device).reshape(-1, 3)
         gv, tth, eta, omc = batched.grain_gv(st.xl, st.so, st.co, st.omega, p, trans_d, idx,
                                              owner, angles=angles)
         if self.OMEGA_FLOAT:
@@ -774,8 +847,22 @@ class refinegrains:
                 pos = np.array([self._pos[sname][n] for n in names], np.int64)
                 self._push(sname, names, pos)
 
-                def evaluate(points, which, sname=sname, pos=pos):
-                    npks, avg = self._evaluate_pos(sname, pos[which], points)
+                if self.round_graphs and self.latticesymmetry is triclinic:
+                    # every round replays one captured evaluation of all
+                    # the scan's grains, the ones not asked for at their
+                    # latest points (computed and dropped)
+                    rounds = _RoundGraph(self, sname, pos)
+                    latest = guesses.copy()
+
+                    def run(points, which, latest=latest, rounds=rounds):
+                        latest[which] = points
+                        return rounds(latest, which)
+                else:
+                    def run(points, which, sname=sname, pos=pos):
+                        return self._evaluate_pos(sname, pos[which], points)
+
+                def evaluate(points, which, run=run):
+                    npks, avg = run(points, which)
                     diffs = npks * avg
                     val = np.where(npks > 0, 1e6 * (0. + diffs) / np.maximum(npks, 1), 1e6)
                     return val

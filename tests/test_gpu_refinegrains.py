@@ -276,3 +276,27 @@ def test_makeuniq_cubic(dev, orc):
     assert np.array_equal(np.asarray(col.labels), lab0)
     gof1 = rg.gof([])
     assert abs(gof1 - gof0) <= 1e-9 * abs(gof0)
+
+
+@pytest.mark.parametrize("OmFloat", [False, True])
+def test_refinepositions_round_graph_equals_eager(dev, orc, OmFloat):
+    """The captured round (every grain evaluated, the asked-for ones kept)
+    replayed ~100 times with the simplex's host work in between gives the
+    eager rounds' translations, UBIs, errors and iteration counts bit for
+    bit."""
+    res = []
+    for graphs in (True, False):
+        rg, o, s = _setup(orc, K=12, OmFloat=OmFloat, seed=13)
+        rng = np.random.default_rng(2)
+        for n in rg.grainnames:
+            rg.grains[(n, s)].translation = rg.grains[(n, s)].translation + rng.uniform(-15, 15, 3)
+        rg.round_graphs = graphs
+        got = rg.refinepositions(maxiters=100)
+        res.append((got, {k: (rg.grains[k].translation.copy(), rg.grains[k].ubi.copy())
+                          for k in got}, rg.evaluation_rounds))
+    (g1, s1, r1), (g2, s2, r2) = res
+    assert r1 == r2 > 50
+    for k in g1:
+        assert g1[k][1:] == g2[k][1:]
+        assert np.array_equal(np.asarray(g1[k][0]), np.asarray(g2[k][0]))
+        assert np.array_equal(s1[k][0], s2[k][0]) and np.array_equal(s1[k][1], s2[k][1])
