@@ -511,6 +511,12 @@ gs_detector make_detector(const double* p, const double* r, const double* dist) 
     return d;
 }
 
+// The goniometer of a launch.  Without a given matrix the wedge/chi
+// matrix is formed here, on the host, by the same correctly rounded
+// sincos the device would run (gs_sincos_v compiles for both, and a
+// correctly rounded result is unique), so every block reads it from its
+// arguments instead of one thread per block computing two sincos while
+// the others wait.
 gs_goniometer make_gonio(double omegasign, double wvln, double wedge, double chi, const double* t,
                          const double* mat) {
     gs_goniometer g;
@@ -519,8 +525,11 @@ gs_goniometer make_gonio(double omegasign, double wvln, double wedge, double chi
     g.wedge = wedge;
     g.chi = chi;
     for (int i = 0; i < 3; i++) g.t[i] = t ? t[i] : 0.0;
-    g.mat_given = mat != nullptr;
-    for (int i = 0; i < 9; i++) g.mat[i] = mat ? mat[i] : 0.0;
+    g.mat_given = 1;
+    if (mat)
+        for (int i = 0; i < 9; i++) g.mat[i] = mat[i];
+    else
+        gs_wedge_chi_matrix(wedge, chi, g.mat);
     return g;
 }
 

@@ -164,10 +164,13 @@ def test_compute_gv_bitexact_given_trig(dev, orc):
 
 def test_sincos_device_equals_host_build(dev):
     """The device sincos and the same code compiled for the host agree bit
-    for bit (the host build is checked against 113-bit sinq/cosq on CPU)."""
+    for bit (the host build is checked against 113-bit sinq/cosq on CPU),
+    also for the small wedge/chi angles, whose matrix the launches now form
+    on the host."""
     rng = np.random.default_rng(0)
     omega = np.concatenate([rng.uniform(-1000, 1000, 200000), (np.arange(1440) + 0.5) * 0.25,
-                            np.arange(-720, 721, 1.0), [0.0, -0.0, 1e-300, 90.0, 180.0]])
+                            np.arange(-720, 721, 1.0), [0.0, -0.0, 1e-300, 90.0, 180.0],
+                            rng.uniform(-10, 10, 50000), rng.normal(scale=1e-3, size=5000)])
     so, co = batched.sincos_omega(T(omega, dev), 1.0)
     x = np.pi / 180.0 * omega * 1.0
     hs, hc = np.zeros_like(x), np.zeros_like(x)
