@@ -41,12 +41,14 @@ FLOPS_PER_PAIR = 23      # h = UBI.g: 9 mul + 6 add; 3 sub; sumsq: 3 mul + 2 add
 # compares (2)
 VALU_OPS_PER_PAIR = 19
 # the sieved score (k_score_many_sieve<4>, the default for tol <= 0.1),
-# counted in its gfx950 assembly: every pair 9 VALU (row 0 + r by three
-# FMAs, v_fract_f64, one compare; two mbcnt, the queue address and the entry
-# tag), every pair passing row 0 another 28 (the 19 above plus 9 integer
-# ones for the entry decode and addresses)
-SIEVE_STAGE1_VALU = 9
-SIEVE_SETTLE_VALU = 28
+# counted in its gfx950 assembly: every pair 8.5 VALU (row 0 + r by three
+# FMAs, v_fract_f64, one compare; two mbcnt and the queue address; one
+# entry per grain step for all four slots), every pair passing row 0
+# another 26.5 (17 FP64 of the certified test, the entry decode, addresses
+# and compares).  The SQ counters count 12.3 per pair in all, the tile
+# setup and the queue drains included.
+SIEVE_STAGE1_VALU = 8.5
+SIEVE_SETTLE_VALU = 26.5
 # FP64 VALU instructions per pair of the fused compute_gv + score_and_assign
 # kernel's certified loop, counted in its gfx950 assembly: 77 per grain step
 # of two peaks per lane (24 v_fmac, 18 v_fma, 15 v_mul, 10 v_add, 6

@@ -222,30 +222,34 @@ __global__ __launch_bounds__(GS_BLOCK, P <= 4 ? 8 : 1) void k_score_many_fast(
 // by three FMAs (r enters the first), and the pair passes unless
 // fract(y) >= 2r, i.e. unless h0 is at least r from every integer (five
 // FP64 instructions; a NaN passes); the survivors (grain, peak) are
-// appended to a per-wave queue in LDS (ballot + mbcnt positions).  Whenever
-// the queue holds a wave's worth, 64 survivors are settled at once, one per
-// lane, by the full certified test of k_score_many_fast (its window in LDS,
-// the reference arithmetic for undecided pairs) with the UBI from an LDS
-// table and the g-vector re-read through the cache.  The queue is drained
-// from its top, so entries never move; capacity 64 (P + 1) covers the
-// at most 63 left over plus one grain step's 64 P.  Survivors are counted
-// with LDS atomics per grain (rare: only true hits), then one global atomic
-// per (block, grain).
+// appended to per-wave queues in LDS, one per peak slot p of the lanes
+// (ballot + mbcnt positions).  Whenever a queue holds a wave's worth, 64
+// survivors are settled at once, one per lane, by the full certified test
+// of k_score_many_fast (its window in LDS, the reference arithmetic for
+// undecided pairs) with the UBI from an LDS table and the g-vector re-read
+// through the cache.  A queue is drained from its top, so entries never
+// move; capacity 128 covers the at most 63 left over plus one grain step's
+// 64.  Survivors are counted with LDS atomics per grain (rare: only true
+// hits), then one global atomic per (block, grain).
 //
-// Queue entry: (grain offset in the split) << 9 | p << 6 | lane.
+// Queue entry (16 bits): (grain offset in the split) << 6 | lane; the slot
+// p is the queue's, so a grain step forms one entry value for all its
+// slots.
 //
 // SELECT = true is the sieved k_select_many: a settled pair that passes takes
 // a slot of its grain with one global atomic on cursor[g] (passing pairs are
 // the rare true hits) and, when idx is given, writes its peak index at
 // idx[off[g] + slot] for slot < off[g+1] - off[g].
+// kbase: the first peak (in the chunk) of the queue's slot for this wave,
+// p GS_BLOCK + 64 wave.
 template <bool SELECT>
 __device__ __forceinline__ void gs_sieve_settle(int e, const double* __restrict__ gvc, long long nrem,
-                                                int wave, const double* lubi, const double* lwin,
+                                                int kbase, const double* lubi, const double* lwin,
                                                 double tolsq, int* lcnt, int g0, long long c0,
                                                 const long long* __restrict__ off,
                                                 int* __restrict__ cursor, int* __restrict__ idx) {
-    const int gl = e >> 9;
-    const long long k = (long long)((e >> 6) & 7) * GS_BLOCK + wave * GS_WAVE + (e & 63);
+    const int gl = e >> 6;
+    const long long k = kbase + (e & 63);
     if (k >= nrem) return;  // NaN padding rows of the last chunk
     const double x = gvc[3 * k + 0], y = gvc[3 * k + 1], z = gvc[3 * k + 2];
     const double* ur = lubi + 9 * gl;
@@ -269,15 +273,16 @@ __global__ __launch_bounds__(GS_BLOCK, 8) void k_score_many_sieve(
     const double* __restrict__ gv, int ng, double tolsq, int* __restrict__ counts,
     const long long* __restrict__ off = nullptr, int* __restrict__ cursor = nullptr,
     int* __restrict__ idx = nullptr) {
-    static_assert(P >= 1 && P <= 8, "the queue entry holds p in three bits");
+    static_assert(P >= 1 && P <= 8, "at most 8 peak slots per lane");
+    static_assert(GS_SIEVE_SPLIT_MAX <= 1024, "a 16-bit queue entry holds the grain offset");
     constexpr int NW = GS_BLOCK / GS_WAVE;
-    constexpr int QCAP = GS_WAVE * (P + 1);
+    constexpr int QCAP = 2 * GS_WAVE;  // per slot: 63 left over + one grain step's 64
     __shared__ int lcnt[GS_SIEVE_SPLIT_MAX];
     __shared__ double lubi[9 * GS_SIEVE_SPLIT_MAX];
     __shared__ double lwin[2 * GS_SIEVE_SPLIT_MAX];  // (lo, hi) per grain of the split
     __shared__ double lrad[2 * GS_SIEVE_SPLIT_MAX];  // (r, 2r) per grain of the split
     __shared__ double lgm[NW];
-    __shared__ int lq[NW * QCAP];
+    __shared__ unsigned short lq[NW * P * QCAP];
     const int tile = gs_xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = tile / nsplit;
     const int split = tile - chunk * nsplit;
@@ -324,14 +329,17 @@ __global__ __launch_bounds__(GS_BLOCK, 8) void k_score_many_sieve(
         ur[6] = u.u20; ur[7] = u.u21; ur[8] = u.u22;
     }
     __syncthreads();
-    int* q = lq + __builtin_amdgcn_readfirstlane(wave) * QCAP;  // scalar base
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    unsigned short* q = lq + wv * (P * QCAP);  // scalar base; slot p's queue at q + p QCAP
     const double* gvc = gv + 3 * c0;
-    int qn = 0;  // wave-uniform
+    int qn[P];  // wave-uniform fill of each slot's queue
+#pragma unroll
+    for (int p = 0; p < P; p++) qn[p] = 0;
     for (int gl = 0; gl < ns; gl++) {
         const double* ug = ubis + 9 * (size_t)(g0 + gl);
         const double u00 = ug[0], u01 = ug[1], u02 = ug[2];
         const double r = lrad[2 * gl], r2 = lrad[2 * gl + 1];
-        const int tag = gl << 9;
+        const unsigned short e = (unsigned short)((gl << 6) | lane);  // this step's entry, every slot
 #pragma unroll
         for (int p = 0; p < P; p++) {
             const double h = __builtin_fma(u02, z[p], __builtin_fma(u01, y[p], __builtin_fma(u00, x[p], r)));
@@ -340,22 +348,28 @@ __global__ __launch_bounds__(GS_BLOCK, 8) void k_score_many_sieve(
             if (sv) {  // rank among the survivors, then the scalar top of the queue
                 const unsigned pos = __builtin_amdgcn_mbcnt_hi(
                     (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                (q + qn)[pos] = tag | (p << 6) | lane;
+                (q + p * QCAP + qn[p])[pos] = e;
             }
-            qn += __popcll(m);
+            qn[p] += __popcll(m);
         }
-        while (qn >= GS_WAVE) {  // settle the top 64 entries
-            qn -= GS_WAVE;
-            __builtin_amdgcn_wave_barrier();
-            gs_sieve_settle<SELECT>(q[qn + lane], gvc, nrem, wave, lubi, lwin, tolsq, lcnt, g0, c0,
-                                    off, cursor, idx);
-            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            if (qn[p] >= GS_WAVE) {  // settle the top 64 entries of slot p
+                qn[p] -= GS_WAVE;
+                __builtin_amdgcn_wave_barrier();
+                gs_sieve_settle<SELECT>(q[p * QCAP + qn[p] + lane], gvc, nrem,
+                                        p * GS_BLOCK + wv * GS_WAVE, lubi, lwin, tolsq, lcnt, g0,
+                                        c0, off, cursor, idx);
+                __builtin_amdgcn_wave_barrier();
+            }
         }
     }
     __builtin_amdgcn_wave_barrier();
-    if (lane < qn)
-        gs_sieve_settle<SELECT>(q[lane], gvc, nrem, wave, lubi, lwin, tolsq, lcnt, g0, c0, off,
-                                cursor, idx);
+#pragma unroll
+    for (int p = 0; p < P; p++)
+        if (lane < qn[p])
+            gs_sieve_settle<SELECT>(q[p * QCAP + lane], gvc, nrem, p * GS_BLOCK + wv * GS_WAVE,
+                                    lubi, lwin, tolsq, lcnt, g0, c0, off, cursor, idx);
     if constexpr (SELECT) return;
     __syncthreads();
     for (int i = threadIdx.x; i < ns; i += GS_BLOCK) {
