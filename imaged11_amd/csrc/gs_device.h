@@ -423,6 +423,72 @@ __device__ __forceinline__ double gs_group_sum(double v) {
     return v;
 }
 
+// ---------------------------------------------------------------------------
+// gs_solve_ubi for groups of 16 or more lanes that all hold the same R and
+// H (the totals of gs_group_sum), bit for bit the same results.  Written
+// per lane, every lane of the wave issues all 15 divisions of the two
+// inverses (each some ten instructions, five of them slow); here lane i
+// of each row of 16 (i < 9) divides element i only, and row_newbcast DPP
+// moves hand the nine quotients to the whole row.  lr: the lane's index
+// within its row (lane & 15).
+
+// v of lane J of the lane's row of 16
+template <int J>
+__device__ __forceinline__ double gs_row_bcast(double v) {
+    return gs_dpp_f64<0x150 + J>(v);  // row_newbcast:J
+}
+
+// q[i] = num[i] / den for i < 9 in every lane of the row (den the same in
+// all lanes of the row), each quotient IEEE division.
+__device__ __forceinline__ void gs_div9_row(const double* num, double den, double* q, int lr) {
+    // num[lr] (num[8] for lr >= 8) by a tree of selects on the bits of lr
+    // (a chain of compares is turned into an indexed load from memory)
+    const bool b0 = lr & 1, b1 = lr & 2, b2 = lr & 4, b3 = lr & 8;
+    const double s01 = b0 ? num[1] : num[0], s23 = b0 ? num[3] : num[2];
+    const double s45 = b0 ? num[5] : num[4], s67 = b0 ? num[7] : num[6];
+    const double s03 = b1 ? s23 : s01, s47 = b1 ? s67 : s45;
+    const double s07 = b2 ? s47 : s03;
+    const double x = b3 ? num[8] : s07;
+    const double r = x / den;
+    q[0] = gs_row_bcast<0>(r); q[1] = gs_row_bcast<1>(r); q[2] = gs_row_bcast<2>(r);
+    q[3] = gs_row_bcast<3>(r); q[4] = gs_row_bcast<4>(r); q[5] = gs_row_bcast<5>(r);
+    q[6] = gs_row_bcast<6>(r); q[7] = gs_row_bcast<7>(r); q[8] = gs_row_bcast<8>(r);
+}
+
+// gs_inverse3x3 with the divisions shared out over the row: the same
+// cofactors, determinant and quotients.
+__device__ __forceinline__ int gs_inverse3x3_row(double* A, int lr) {
+    const double c0 = A[8] * A[4] - A[7] * A[5];
+    const double c1 = A[8] * A[1] - A[7] * A[2];
+    const double c2 = A[5] * A[1] - A[4] * A[2];
+    const double det = A[0] * c0 - A[3] * c1 + A[6] * c2;
+    if (!(det != 0.)) return -1;
+    const double num[9] = {c0,
+                           -c1,
+                           c2,
+                           -(A[8] * A[3] - A[6] * A[5]),
+                           (A[8] * A[0] - A[6] * A[2]),
+                           -(A[5] * A[0] - A[3] * A[2]),
+                           (A[7] * A[3] - A[6] * A[4]),
+                           -(A[7] * A[0] - A[6] * A[1]),
+                           (A[4] * A[0] - A[3] * A[1])};
+    gs_div9_row(num, det, A, lr);
+    return 0;
+}
+
+__device__ __forceinline__ void gs_solve_ubi_row(double* ubi, const double* R, double* H, int lr) {
+    double UB[9];
+    for (int i = 0; i < 9; i++) UB[i] = 0.;
+    const int k = gs_inverse3x3_row(H, lr);
+    if (k == 0) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int l = 0; l < 3; l++) UB[3 * i + j] = UB[3 * i + j] + R[3 * i + l] * H[3 * l + j];
+    }
+    if (k == 0 && gs_inverse3x3_row(UB, lr) == 0)
+        for (int i = 0; i < 9; i++) ubi[i] = UB[i];
+}
+
 __device__ __forceinline__ int gs_wave_sum_int(int v, int width) {
     for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, GS_WAVE);
     return v;

@@ -677,6 +677,32 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gather_rows(const double* __restri
     rows[3 * q + 2] = g2;
 }
 
+// Rows q, q + L, ..., q + (B-1) L of a lane into buf, each clamped to the
+// list's last row o1 - 1 (in bounds: o1 <= the number of rows); nothing is
+// loaded when q >= o1.
+template <int B, int L>
+__device__ __forceinline__ void gs_rows_load(double (*buf)[3], const double* __restrict__ rows,
+                                             long long q, long long o1) {
+    if (!(q < o1)) return;
+#pragma unroll
+    for (int j = 0; j < B; j++) {
+        long long r = q + (long long)j * L;
+        r = r < o1 ? r : o1 - 1;
+        buf[j][0] = rows[3 * r + 0];
+        buf[j][1] = rows[3 * r + 1];
+        buf[j][2] = rows[3 * r + 2];
+    }
+}
+
+// The terms of the rows in buf that lie below o1, in row order.
+template <int B, int L>
+__device__ __forceinline__ void gs_rows_add(const gs_ubi& U, const double (*buf)[3], long long q,
+                                            long long o1, double tolsq, double* a) {
+#pragma unroll
+    for (int j = 0; j < B; j++)
+        if (q + (long long)j * L < o1) gs_refine_add(U, buf[j][0], buf[j][1], buf[j][2], tolsq, a);
+}
+
 // ---------------------------------------------------------------------------
 // One work item of k_refine_lists: the grains in slots base .. base +
 // 64/L - 1 of `order` below `limit` (<= K), L lanes each, every grain's
@@ -699,31 +725,30 @@ __device__ __forceinline__ void gs_refine_item(
     o0 = o0 < 0 ? 0 : (o0 > o1 ? o1 : o0);
     double u[9];
     for (int i = 0; i < 9; i++) u[i] = active ? ubis[9 * (size_t)g + i] : 0.0;
+    double X[GS_REFINE_BATCH][3], Y[GS_REFINE_BATCH][3];  // rows mode only
+    if (ROWS) gs_rows_load<GS_REFINE_BATCH, L>(X, gv, o0 + sl, o1);
     for (int it = 0; it < n_iter; it++) {
         const gs_ubi U = {u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8]};
         double a[GS_NPART];
 #pragma unroll
         for (int i = 0; i < GS_NPART; i++) a[i] = 0.0;
         if (ROWS) {
-            // GS_REFINE_BATCH rows per lane in flight.  The row is
-            // clamped to the list's last one (in bounds: o1 <= n_idx <=
-            // ng), so the loads need no guard; the copies past the end
-            // are dropped.  The lane still adds its rows in list order.
-            for (long long q = o0 + sl; q < o1; q += (long long)GS_REFINE_BATCH * L) {
-                double gg[GS_REFINE_BATCH][3];
-#pragma unroll
-                for (int j = 0; j < GS_REFINE_BATCH; j++) {
-                    long long r = q + (long long)j * L;
-                    r = r < o1 ? r : o1 - 1;
-                    gg[j][0] = gv[3 * r + 0];
-                    gg[j][1] = gv[3 * r + 1];
-                    gg[j][2] = gv[3 * r + 2];
-                }
-#pragma unroll
-                for (int j = 0; j < GS_REFINE_BATCH; j++)
-                    if (q + (long long)j * L < o1)
-                        gs_refine_add(U, gg[j][0], gg[j][1], gg[j][2], tolsq, a);
+            // two buffers of GS_REFINE_BATCH rows: the next batch is in
+            // flight while this one is added; the lane adds its rows in
+            // list order
+            long long q = o0 + sl;
+            while (q < o1) {
+                const long long qn = q + (long long)GS_REFINE_BATCH * L;
+                gs_rows_load<GS_REFINE_BATCH, L>(Y, gv, qn, o1);
+                gs_rows_add<GS_REFINE_BATCH, L>(U, X, q, o1, tolsq, a);
+                if (!(qn < o1)) break;
+                q = qn + (long long)GS_REFINE_BATCH * L;
+                gs_rows_load<GS_REFINE_BATCH, L>(X, gv, q, o1);
+                gs_rows_add<GS_REFINE_BATCH, L>(U, Y, qn, o1, tolsq, a);
             }
+            // the next iteration's first rows, in flight during the sums
+            // and the solve
+            if (it + 1 < n_iter) gs_rows_load<GS_REFINE_BATCH, L>(X, gv, o0 + sl, o1);
         } else {
             for (long long q = o0 + sl; q < o1; q += L) {
                 const long long k = idx[q];
@@ -735,7 +760,12 @@ __device__ __forceinline__ void gs_refine_item(
         for (int i = 0; i < GS_NPART; i++) a[i] = gs_group_sum<L>(a[i]);
         double R[9], H[9];
         gs_unpack_RH(a, R, H);
-        gs_solve_ubi(u, R, H);
+        // groups of whole DPP rows share each inverse's divisions out
+        // over the row (the same results as gs_solve_ubi)
+        if constexpr (L >= 16)
+            gs_solve_ubi_row(u, R, H, lane & 15);
+        else
+            gs_solve_ubi(u, R, H);
         if (active && sl == 0) {
             const int n = (int)a[0];
             n_out[(size_t)it * K + g] = n;
