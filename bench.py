@@ -959,12 +959,20 @@ def main():
         out["cpu_baseline"] = cpu_baseline(wl, args.tol, threads, gpu_counts)
         out["cpu_baseline"]["host"] = cpu_facts
     if not args.no_aux:
-        out["aux"] = {"refine_5000_grains": aux_refine_adjoint(threads, not args.no_cpu),
-                      "scan_20000_grains": aux_scan(threads, not args.no_cpu),
-                      "index_3000_grains": aux_index(not args.no_cpu),
-                      "refinegrains_2000_grains": aux_refinegrains(not args.no_cpu),
-                      "pbp_refine_map_101x101": aux_pbp(not args.no_cpu),
-                      "geometry_8e6_peaks": aux_geometry(threads, not args.no_cpu)}
+        with_cpu = not args.no_cpu
+        legs = (("refine_5000_grains", lambda: aux_refine_adjoint(threads, with_cpu)),
+                ("scan_20000_grains", lambda: aux_scan(threads, with_cpu)),
+                ("index_3000_grains", lambda: aux_index(with_cpu)),
+                ("refinegrains_2000_grains", lambda: aux_refinegrains(with_cpu)),
+                ("pbp_refine_map_101x101", lambda: aux_pbp(with_cpu)),
+                ("geometry_8e6_peaks", lambda: aux_geometry(threads, with_cpu)))
+        out["aux"] = {}
+        for name, leg in legs:
+            t0 = time.perf_counter()
+            out["aux"][name] = leg()
+            # progress on stderr; the JSON line stays the only stdout
+            print("bench: %s done in %.1f s" % (name, time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
         out["refine_plus_adjoint_vs_per_grain_gpu_loop"] = \
             out["aux"]["refine_5000_grains"]["speedup_refine_plus_adjoint_vs_gpu_loop"]
     print(json.dumps(out))
