@@ -36,6 +36,7 @@ SIGNATURES = {
     "gs_device_count": (_i, []),
     "gs_init": (_i, []),
     "gs_selftest_sincos_host": (None, [_vp, _vp, _vp, _i]),
+    "gs_selftest_sincos_series_host": (None, [_vp, _vp, _vp, _i, _vp]),
     "gs_selftest_certified_host": (None, [_vp, _vp, _i, _d, _d, _vp]),
     "gs_selftest_certified_fract_host": (None, [_vp, _vp, _i, _d, _d, _vp]),
     "gs_selftest_scan_host": (None, [_vp, _vp, _vp, _vp, _vp, _i, _vp, _d, _d, _vp]),

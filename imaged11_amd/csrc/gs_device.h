@@ -241,33 +241,32 @@ GS_HD __forceinline__ gs_dd gs_dd_sub(gs_dd a, gs_dd b) {
     return gs_fast_two_sum(s.hi, s.lo);
 }
 
-// sin and cos of a double argument x (radians), each rounded once from a
-// double-double value accurate to ~2^-80 relative, so the results are the
-// correctly rounded values except in extremely rare near-midpoint cases.
-// Cody-Waite reduction by pi/2 held in four pieces (exact for |k| < 2^20),
-// then Taylor series on |r| <= pi/4 with the leading terms in
-// double-double.  Arguments beyond 2^19*pi use the device library.
-// Inlined and returned by value, so the pair stays in registers (a call
-// through pointers put it in per-lane scratch memory).
+// sin and cos of a double argument x (radians), correctly rounded except
+// in extremely rare near-midpoint cases.  Cody-Waite reduction by pi/2
+// held in four pieces (exact for |k| < 2^20) to r = rh + rl, |r| <= pi/4;
+// then a table path that settles almost every argument, and for the rest
+// Taylor series on r with the leading terms in double-double, accurate to
+// ~2^-80 relative and rounded once.  Arguments beyond 2^19*pi use the
+// device library.  Inlined and returned by value, so the pair stays in
+// registers (a call through pointers put it in per-lane scratch memory).
 struct gs_sc {
     double s, c;
 };
 
-GS_HD __forceinline__ gs_sc gs_sincos_v(double x) {
-    if (!(fabs(x) < 823549.6))  // 2^19 * pi, also catches NaN/Inf
-        return {sin(x), cos(x)};
-    if (x == 0.0)  // keeps the sign of a zero argument
-        return {x, 1.0};
-    double kf = rint(x * GS_2_OVER_PI);
-    int q = ((int)kf) & 3;
-    gs_dd r;
-    {
-        double t = x - kf * GS_PIO2_1;  // exact: k*PIO2_1 has <= 53 bits, Sterbenz
-        gs_dd p2 = gs_two_prod(kf, GS_PIO2_2);
-        r = gs_dd_sub({t, 0.0}, p2);
-        r = gs_dd_sub(r, {kf * GS_PIO2_3, 0.0});  // exact product (33+20 bits)
-        r = gs_dd_sub(r, {kf * GS_PIO2_4, 0.0});
-    }
+// x = k pi/2 + r, r as a double-double; *q = k mod 4
+GS_HD __forceinline__ gs_dd gs_reduce_pio2(double x, int* q) {
+    const double kf = rint(x * GS_2_OVER_PI);
+    *q = ((int)kf) & 3;
+    const double t = x - kf * GS_PIO2_1;  // exact: k*PIO2_1 has <= 53 bits, Sterbenz
+    const gs_dd p2 = gs_two_prod(kf, GS_PIO2_2);
+    gs_dd r = gs_dd_sub({t, 0.0}, p2);
+    r = gs_dd_sub(r, {kf * GS_PIO2_3, 0.0});  // exact product (33+20 bits)
+    return gs_dd_sub(r, {kf * GS_PIO2_4, 0.0});
+}
+
+// sin r and cos r for |r| <= pi/4 by the series, each rounded once from a
+// double-double value accurate to ~2^-80 relative
+GS_HD __forceinline__ gs_sc gs_sincos_series(gs_dd r) {
     gs_dd r2 = gs_dd_mul(r, r);
     double z = r2.hi;
     // sin(r) = r * (1 + r2 * S(r2)),   cos(r) = 1 + r2 * C(r2)
@@ -285,13 +284,105 @@ GS_HD __forceinline__ gs_sc gs_sincos_v(double x) {
     for (int i = 4; i >= 0; i--) C = gs_dd_add_small({GS_COS_DD[i][0], GS_COS_DD[i][1]}, gs_dd_mul(C, r2));
     gs_dd sr = gs_dd_mul(r, gs_dd_add_small({1.0, 0.0}, gs_dd_mul(r2, S)));
     gs_dd cr = gs_dd_add_small({1.0, 0.0}, gs_dd_mul(r2, C));
-    double sv = sr.hi + sr.lo;
-    double cv = cr.hi + cr.lo;
+    return {sr.hi + sr.lo, cr.hi + cr.lo};
+}
+
+// Half the spacing of the doubles next to y on the side toward zero:
+// 2^(e-53) for 2^e <= |y| < 2^(e+1), 2^(e-54) when |y| = 2^e; 0 where that
+// is not a normal double (the caller then takes the series).
+GS_HD __forceinline__ double gs_half_spacing(double y) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, y) & 0x7fffffffffffffffull;
+    const unsigned long long ex = b >> 52;
+    const unsigned long long sh = (b & 0xfffffffffffffull) ? 53ull : 54ull;
+    return ex > sh ? __builtin_bit_cast(double, (ex - sh) << 52) : 0.0;
+}
+
+// y = yh + yl (|yl| <= half an ulp of yh) within e of the exact value v:
+// RN(v) = yh when |yl| + e stays below the half spacing next to yh (the
+// computed sum rounds up to it whenever the exact sum exceeds it).
+GS_HD __forceinline__ bool gs_rounds_to(double yh, double yl, double e) {
+    return fabs(yl) + e < gs_half_spacing(yh);
+}
+
+// The table path.  With a = j/128 the nearest multiple to rh and b = r - a
+// (|b| <= 1/256), sin r = Sa + Ca b + Sa (cos b - 1) + Ca (sin b - b) and
+// cos r = Ca - Sa b + Ca (cos b - 1) - Sa (sin b - b), Sa and Ca from the
+// double-double table, b^2, Ca bh and Sa (-b^2/2) exact by FMA, the rest
+// in double, smallest terms first.  Bounds on the error of y = yh + yl
+// (b's part rl, the table's 2^-106, the short series' truncation b^9/9!
+// and rounding, the summation): below 2^-76 absolute and, for j = 0,
+// 2^-69 relative for sin, with |sin r| >= sin(1/256) for j != 0, so
+// 2^-68 relative; below 2^-78 absolute for cos, with cos r >= 0.7.  The
+// tests use 2^-66 and 2^-72 relative.  Returns false (the caller takes the
+// series) when either value is not settled by gs_rounds_to.
+GS_HD __forceinline__ bool gs_sincos_table(gs_dd r, gs_sc* out) {
+    const double rh = r.hi, rl = r.lo;
+    const double jf = rint(rh * GS_TBL_STEP);
+    const double aj = fabs(jf);
+    if (!(aj < GS_TBL_N)) return false;
+    const double* tb = GS_SINCOS_TBL[(int)aj];
+    const double bh = rh - jf * (1.0 / GS_TBL_STEP);  // exact (Sterbenz, or j = 0)
+    const double sah = jf < 0.0 ? -tb[0] : tb[0], sal = jf < 0.0 ? -tb[1] : tb[1];
+    const double cah = tb[2], cal = tb[3];
+    // b^2 = q + qe exactly; cos b - 1 = cbh + cbl; sin b - b = sb
+    const double q = bh * bh, qe = __builtin_fma(bh, bh, -q);
+    const double cbh = -0.5 * q;
+    const double cbl = __builtin_fma(-bh, rl, -0.5 * qe) +
+                       q * q * (GS_TBL_COS_POLY[0] + q * (GS_TBL_COS_POLY[1] + q * GS_TBL_COS_POLY[2]));
+    const double sb = bh * q * (GS_TBL_SIN_POLY[0] + q * (GS_TBL_SIN_POLY[1] + q * GS_TBL_SIN_POLY[2])) +
+                      cbh * rl;
+    // sin r
+    const double p = cah * bh, pe = __builtin_fma(cah, bh, -p);
+    const double m = sah * cbh, me = __builtin_fma(sah, cbh, -m);
+    const gs_dd s1 = gs_two_sum(sah, p);
+    const gs_dd s2 = gs_two_sum(s1.hi, m);
+    double lo = s1.lo + s2.lo + sal + pe + me + cah * rl + cal * bh + sal * cbh;
+    lo = lo + sah * cbl;
+    lo = lo + cah * sb;
+    const gs_dd ys = gs_fast_two_sum(s2.hi, lo);
+    // cos r
+    const double p2 = -sah * bh, pe2 = __builtin_fma(-sah, bh, -p2);
+    const double m2 = cah * cbh, me2 = __builtin_fma(cah, cbh, -m2);
+    const gs_dd c1 = gs_two_sum(cah, p2);
+    const gs_dd c2 = gs_two_sum(c1.hi, m2);
+    double lc = c1.lo + c2.lo + cal + pe2 + me2 - sah * rl - sal * bh + cal * cbh;
+    lc = lc + cah * cbl;
+    lc = lc - sah * sb;
+    const gs_dd yc = gs_fast_two_sum(c2.hi, lc);
+    out->s = ys.hi;
+    out->c = yc.hi;
+    return gs_rounds_to(ys.hi, ys.lo, fabs(ys.hi) * 0x1p-66) &&
+           gs_rounds_to(yc.hi, yc.lo, fabs(yc.hi) * 0x1p-72);
+}
+
+GS_HD __forceinline__ gs_sc gs_sincos_v(double x) {
+    if (!(fabs(x) < 823549.6))  // 2^19 * pi, also catches NaN/Inf
+        return {sin(x), cos(x)};
+    if (x == 0.0)  // keeps the sign of a zero argument
+        return {x, 1.0};
+    int q;
+    const gs_dd r = gs_reduce_pio2(x, &q);
+    gs_sc v;
+    if (!gs_sincos_table(r, &v)) v = gs_sincos_series(r);
     switch (q) {
-        case 0: return {sv, cv};
-        case 1: return {cv, -sv};
-        case 2: return {-sv, -cv};
-        default: return {-cv, sv};
+        case 0: return {v.s, v.c};
+        case 1: return {v.c, -v.s};
+        case 2: return {-v.s, -v.c};
+        default: return {-v.c, v.s};
+    }
+}
+
+// The series alone (the path the table path falls back to), for tests.
+GS_HD __forceinline__ gs_sc gs_sincos_series_v(double x) {
+    if (!(fabs(x) < 823549.6)) return {sin(x), cos(x)};
+    if (x == 0.0) return {x, 1.0};
+    int q;
+    const gs_sc v = gs_sincos_series(gs_reduce_pio2(x, &q));
+    switch (q) {
+        case 0: return {v.s, v.c};
+        case 1: return {v.c, -v.s};
+        case 2: return {-v.s, -v.c};
+        default: return {-v.c, v.s};
     }
 }
 

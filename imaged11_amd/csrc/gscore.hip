@@ -913,6 +913,24 @@ void gs_selftest_sincos_host(const double* x, double* s, double* c, int n) {
     for (int i = 0; i < n; i++) gs_sincos(x[i], &s[i], &c[i]);
 }
 
+// The series path of that algorithm alone (what its table path falls back
+// to), compiled for the host; *n_table (when not NULL) = how many of the n
+// arguments the table path settles.  Checking aid only.
+void gs_selftest_sincos_series_host(const double* x, double* s, double* c, int n, int* n_table) {
+    int settled = 0;
+    for (int i = 0; i < n; i++) {
+        const gs_sc v = gs_sincos_series_v(x[i]);
+        s[i] = v.s;
+        c[i] = v.c;
+        if (fabs(x[i]) < 823549.6 && x[i] != 0.0) {
+            int q;
+            gs_sc w;
+            settled += gs_sincos_table(gs_reduce_pio2(x[i], &q), &w) ? 1 : 0;
+        }
+    }
+    if (n_table) *n_table = settled;
+}
+
 // The certified score decision (gs_drlv2_fma + gs_cert_window) compiled
 // for the host, next to the reference arithmetic (gs_drlv2): for each row
 // of gv, out[k] = 1 if `drlv2 < tol^2` is certain, 0 if certainly false,

@@ -48,6 +48,10 @@ int gs_init(void);               /* allocate the per-device state now */
 /* The device sin/cos algorithm evaluated on the host CPU, for checking its
  * accuracy without a GPU; no other entry point calls it. */
 void gs_selftest_sincos_host(const double *x, double *s, double *c, int n);
+/* The same algorithm's series path alone (the path its table path falls
+ * back to), on the host; *n_table = how many arguments the table path
+ * settles.  Checking aid only. */
+void gs_selftest_sincos_series_host(const double *x, double *s, double *c, int n, int *n_table);
 
 /* The certified score decision evaluated on the host next to the
  * reference arithmetic, for checking the error bound without a GPU:

@@ -256,3 +256,60 @@ def test_three_lane_tiers(dev, orc, problem, lanes, long_q, mid_q):
     assert np.all(np.abs(m[0] - m_o[0]) <= MEAN_RTOL * np.abs(m_o[0]))
     assert np.all(np.abs(m - m_o) <= CHAIN_MEAN_RTOL * np.abs(m_o))
     np.testing.assert_allclose(uf.cpu().numpy(), u_o, rtol=0, atol=UBI_TOL)
+
+
+@pytest.fixture(scope="module")
+def dyadic():
+    """Data whose score_and_refine sums are exact in any order: g-vectors
+    on a 1/16 grid (|g| <= 2), integer UBIs, tol 0.2, so h, its rounding,
+    drlv2 (multiples of 1/256) and every term of n, sum drlv2, R and H are
+    small dyadic numbers whose sums over a list never round.  The totals a
+    launch forms are then the reference's whatever the lanes and their
+    order, and one iteration's UBIs must equal the oracle's bit for bit:
+    that pins the 3x3 solve (two inverses, each with nine IEEE divisions),
+    including the one that shares the divisions out over a DPP row."""
+    rng = np.random.default_rng(41)
+    K = 240
+    ubis = rng.integers(-2, 3, size=(K, 3, 3)).astype(np.float64) + 5.0 * np.eye(3)
+    lengths = np.clip(np.exp(rng.normal(np.log(300), 0.8, size=K)), 20, 2000).astype(np.int64)
+    blocks = []
+    for k in range(K):
+        # 70% near the grain's lattice (UBI^-1 hkl put on the 1/16 grid),
+        # the rest anywhere on the grid
+        n_lat = int(0.7 * lengths[k])
+        hkl = rng.integers(-4, 5, size=(n_lat, 3))
+        lat = np.rint(16.0 * hkl @ np.linalg.inv(ubis[k]).T) / 16.0
+        blocks += [lat, rng.integers(-16, 17, size=(lengths[k] - n_lat, 3)) / 16.0]
+    gv = np.concatenate(blocks)
+    perm = rng.permutation(len(gv))  # lists scattered over the table
+    gv_table = np.empty_like(gv)
+    gv_table[perm] = gv
+    idx = perm.astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    gv = gv_table
+    return dict(gv=gv, ubis=ubis, idx=idx, off=off, lengths=lengths)
+
+
+@pytest.mark.parametrize("lanes,long_q,mid_q", [(8, None, None), (16, None, None), (32, None, None),
+                                                (64, None, None), (16, 0.9, 0.6), (8, 0.9, 0.6)])
+def test_exact_sums_bit_identical_to_oracle(dev, orc, dyadic, lanes, long_q, mid_q):
+    d = dyadic
+    lengths = d["lengths"]
+    long_min = int(np.quantile(lengths, long_q)) if long_q else 0
+    mid_min = int(np.quantile(lengths, mid_q)) if mid_q else 0
+    u_o, n_o, m_o = orc.refine_lists(d["ubis"], d["gv"], d["idx"], d["off"], 0.2, 1)
+    assert np.median(n_o) >= 50 and np.mean(m_o > 0) > 0.9  # kept peaks off the lattice too
+    assert not np.array_equal(u_o, d["ubis"])
+    gv, idx, off = T(d["gv"], dev), T(d["idx"], dev, torch.int32), T(d["off"], dev, torch.int64)
+    uf, nf, mf = batched.refine_lists(T(d["ubis"], dev), gv, idx, off, 0.2, 1,
+                                      lanes_per_grain=lanes, long_min=long_min, mid_min=mid_min)
+    assert np.array_equal(nf.cpu().numpy(), n_o)
+    assert np.array_equal(mf.cpu().numpy().view(np.int64), m_o.view(np.int64))
+    assert np.array_equal(uf.cpu().numpy().view(np.int64), u_o.view(np.int64))
+    work = T(d["ubis"], dev)
+    g = refine.RefineGraph(work, gv, idx, off, 0.2, 1, lanes_per_grain=lanes, long_min=long_min,
+                           mid_min=mid_min)
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(work, uf)
+    g.close()

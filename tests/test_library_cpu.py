@@ -400,3 +400,28 @@ def test_joint_schedule_edges():
     # workspace: per piece 48 doubles, per block of the pass 20
     assert lib.gs_joint_workspace_doubles(0) == 0
     assert lib.gs_joint_workspace_doubles(10) == 10 * 48 + 3 * 20
+
+
+def test_device_sincos_table_path_equals_series():
+    """The table path of the device sincos settles almost every argument
+    and, where it does, returns what the ~2^-80-accurate series returns:
+    the whole function is bit-identical to the series alone on omega
+    grids, wide ranges, tiny arguments and arguments next to multiples of
+    pi/2 and to the table's nodes."""
+    import ctypes
+    lib = _lib.load()
+    rng = np.random.default_rng(5)
+    n = 200000
+    for x in (np.pi / 180.0 * rng.uniform(-720, 720, n), rng.uniform(-823549, 823549, n),
+              np.exp(rng.uniform(np.log(1e-300), 0.0, n)) * rng.choice([-1, 1], n),
+              (np.arange(n) % 2000 - 1000) * (np.pi / 2) * (1 + rng.normal(scale=1e-15, size=n)),
+              rng.integers(-101, 102, n) / 128.0 + rng.integers(-50, 50, n) * np.pi / 2):
+        x = np.ascontiguousarray(x)
+        s, c = _host_sincos(x)
+        s2, c2 = np.zeros_like(x), np.zeros_like(x)
+        settled = ctypes.c_int(0)
+        lib.gs_selftest_sincos_series_host(_lib.hptr(x), _lib.hptr(s2), _lib.hptr(c2), n,
+                                           ctypes.byref(settled))
+        assert np.array_equal(s.view(np.int64), s2.view(np.int64))
+        assert np.array_equal(c.view(np.int64), c2.view(np.int64))
+        assert settled.value > 0.97 * n
