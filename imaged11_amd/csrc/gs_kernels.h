@@ -640,7 +640,11 @@ __global__ void k_refine_finalize(const double* __restrict__ partials, int nbloc
 
 // One score_and_refine term (src/closest.c:296-316): a peak within tol adds
 // to n, sum drlv2, R = sum g h^T and H = sum h h^T (the six distinct
-// entries), in the reference's expression order.
+// entries).  The keep decision and drlv2 are the reference's arithmetic;
+// the products go into the sums by FMA (the batched kernel's sums are
+// split over lanes, so their rounding is not the reference's anyway): H's
+// products of integers below 2^26 are exact, so its sums are the same,
+// and R's terms are each rounded once instead of twice.
 __device__ __forceinline__ void gs_refine_add(const gs_ubi& U, double g0, double g1, double g2,
                                               double tolsq, double* a) {
     double i0, i1, i2;
@@ -648,11 +652,14 @@ __device__ __forceinline__ void gs_refine_add(const gs_ubi& U, double g0, double
     if (s < tolsq) {
         a[0] += 1.0;
         a[1] += s;
-        a[2] += i0 * g0; a[3] += i1 * g0; a[4] += i2 * g0;
-        a[5] += i0 * g1; a[6] += i1 * g1; a[7] += i2 * g1;
-        a[8] += i0 * g2; a[9] += i1 * g2; a[10] += i2 * g2;
-        a[11] += i0 * i0; a[12] += i1 * i0; a[13] += i2 * i0;
-        a[14] += i1 * i1; a[15] += i2 * i1; a[16] += i2 * i2;
+        a[2] = __builtin_fma(i0, g0, a[2]); a[3] = __builtin_fma(i1, g0, a[3]);
+        a[4] = __builtin_fma(i2, g0, a[4]); a[5] = __builtin_fma(i0, g1, a[5]);
+        a[6] = __builtin_fma(i1, g1, a[6]); a[7] = __builtin_fma(i2, g1, a[7]);
+        a[8] = __builtin_fma(i0, g2, a[8]); a[9] = __builtin_fma(i1, g2, a[9]);
+        a[10] = __builtin_fma(i2, g2, a[10]);
+        a[11] = __builtin_fma(i0, i0, a[11]); a[12] = __builtin_fma(i1, i0, a[12]);
+        a[13] = __builtin_fma(i2, i0, a[13]); a[14] = __builtin_fma(i1, i1, a[14]);
+        a[15] = __builtin_fma(i2, i1, a[15]); a[16] = __builtin_fma(i2, i2, a[16]);
     }
 }
 
