@@ -90,7 +90,7 @@ def parse():
 def host_cpus(requested=0):
     """(threads, facts) for the host baselines: the CPUs this process may
     run on (its affinity mask), capped by OMP_NUM_THREADS when that is set
-    (the CPU share a box gives to one GPU), unless --cpu-threads says
+    (the CPU share given to one GPU), unless --cpu-threads says
     otherwise.  facts records nproc, the affinity count and the cap."""
     nproc = os.cpu_count() or 1
     try:
@@ -820,9 +820,9 @@ def profile_workload(which, copy_mb, launches=5):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every GPU box here has one, and the "
-              "multi-GPU path is design-only (peak sharding + an int32 all-reduce "
-              "of the K counts). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU; the multi-GPU path is design-only (peak "
+              "sharding + an int32 all-reduce of the K counts). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
