@@ -271,6 +271,22 @@ def aux_refine_adjoint(threads, with_cpu):
            # gathered refine: 28 B per list entry (int32 index + 24 B g) and
            # 156 B per grain (UBI in and out, n, mean) per iteration
            "algorithmic_hbm_gb_s": (28 * M + 156 * K) * n_iter / (batched_ms * 1e-3) / 1e9}
+    # roofline of the call (gather and tier count included): per list entry
+    # and iteration the reference's drlv2 (23 flops), per kept entry the
+    # n and sum-drlv2 adds and the 15 R and H terms by FMA (32 flops); the
+    # kept counts from one more call outside the timed region
+    n_kept = batched.refine_lists(ubis0, gv, idx_d, off_d, tol, n_iter, order=order,
+                                  lanes_per_grain=best["lanes_per_grain"],
+                                  waves_per_cu=best["waves_per_cu"],
+                                  long_min=best.get("long_min", 0),
+                                  mid_min=best.get("mid_min", 0))[1]
+    flops = 23.0 * M * n_iter + 32.0 * float(n_kept.sum())
+    out["roofline"] = {"bound": "valu", "achieved": flops / (batched_ms * 1e-3) / 1e12,
+                       "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
+                       "frac": flops / (batched_ms * 1e-3) / PEAK_FP64_VALU,
+                       "kept_fraction": float(n_kept.sum()) / (M * n_iter),
+                       "frac_is": "the whole call (gather and tier count included) over the FP64 "
+                                  "peak; the kernel runs one resident wave per SIMD"}
     if with_cpu:
         from concurrent.futures import ThreadPoolExecutor
         from oracle import oracle
