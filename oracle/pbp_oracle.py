@@ -20,7 +20,12 @@ geometry against ``transform``), ``compute_gve`` against the compiled
 reference's ``compute_xlylzl_xpos_variable`` + ``compute_gv``, ``merge``
 against a dictionary grouping, ``weighted_lstsq_ubi_fit`` and the strain
 fit against exact (noise-free) lattices with a known strain, and
-``ubi_and_ucell_to_u`` against U.B = UBI^-1.
+``ubi_and_ucell_to_u`` against U.B = UBI^-1.  ``refine_map`` as a whole
+(which voxels and peaks it selects, the merge keys, the order of its
+stages) is checked only through those helpers and through a synthetic map
+whose imposed UBIs and strain it recovers to the noise level: beyond that
+its parity is unpinned (the reference's numba code cannot run here and
+ships no output for a whole map).
 """
 import numpy as np
 
