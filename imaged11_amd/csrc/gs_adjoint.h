@@ -64,7 +64,7 @@ __device__ __forceinline__ void gs_joint_forward(double s, double f, double so, 
     for (int i = 0; i < 3; i++)
         d[i] = (r[3 * i + 1] * v1 + r[3 * i + 2] * v2 + dist[i]) -
                (mat[3 * i] * w.u[0] + mat[3 * i + 1] * w.u[1] + mat[3 * i + 2] * w.u[2]);
-    w.rn = gs_rsqrt_cert(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    w.rn = gs_rsqrt_cert<2>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     for (int i = 0; i < 3; i++) w.nh[i] = d[i] * w.rn;
     const double k0 = ds * (w.nh[0] - 1.0), k1 = ds * w.nh[1], k2 = ds * w.nh[2];
     double v[3];

@@ -11,18 +11,19 @@
 // M_k is orthogonal.  a_k, c_k are per-peak constants, so a pair costs one
 // subtraction, a norm and a scaling before h = UBI.G.
 //
-// Certification.  Both the reference's G and the fast G are within
-// 90 u ds (u = 2^-53, ds = 1/wavelength) of the exact G in every component
-// when |t| <= |xl|/2 (so a - t and xl - o cannot cancel): k0 = ds (d0/|d|
-// - 1) is the only subtraction of like quantities and |d0/|d|| <= 1, the
-// fast path's 1/|a - t| (gs_rsqrt_cert) is within ~6u, and everything else
-// is a few rounded products of magnitude <= 2 ds.  h = UBI.G then moves by
-// at most ||u_i||_1 (90 u ds) + 6.7e-16 (A_i + 1/2) per row (the fast path
+// Certification.  The reference's G is within 90 u ds (u = 2^-53, ds =
+// 1/wavelength) of the exact G in every component when |t| <= |xl|/2 (so
+// a - t and xl - o cannot cancel): k0 = ds (d0/|d| - 1) is the only
+// subtraction of like quantities and |d0/|d|| <= 1, and everything else is
+// a few rounded products of magnitude <= 2 ds.  The fast G is within
+// ~290 u ds: the same terms plus its 1/|a - t| (gs_rsqrt_cert<1>, within
+// 1.5 2^-46 + 3u ~ 200u).  h = UBI.G then moves by
+// at most ||u_i||_1 (290 u ds) + 6.7e-16 (A_i + 1/2) per row (the fast path
 // uses the fract form of the score kernel, drlv2 - 3/4, whose rows carry
 // an extra 1/2), and drlv2 by the sum of that over rows (1-Lipschitz, see
 // k_score_many_fast) plus 5e-16 from each sum of squares.  E = U1 (1e-13
 // ds + 1e-15 Gmax) + 3e-15 + 2e-15 tol^2 bounds it with margin (Gmax = 2
-// ds bounds |G|).  A pair whose fast drlv2 - 3/4 is >= tol^2 - 3/4 + E
+// ds bounds |G|; 290 u = 3.2e-14).  A pair whose fast drlv2 - 3/4 is >= tol^2 - 3/4 + E
 // certainly fails `drlv2 < tol^2`, so its only effect is the
 // reference's else branch (label == this label -> -1), which needs no
 // arithmetic.  Every other pair is recomputed with the reference's own
@@ -49,14 +50,17 @@ GS_HD __forceinline__ void gs_scan_consts(double x0, double x1, double x2, doubl
     c[2] = ds * mat[2];
 }
 
-// 1/sqrt(x) to within a few ulp, for the certified fast path only: the
-// hardware estimate (v_rsq_f64) and two Newton steps y <- y + y/2 (1 - x
-// y^2).  From any start with relative error e <= 2^-14 the steps give
-// 1.5 e^2 + ~3u each, so the result is within ~6u of 1/sqrt(x) however
-// coarse the estimate (the hardware's is far better).  x = 0 or non-finite
-// gives NaN, which the callers send down the exact route.  On the host the
-// estimate is the float rsqrt of the value (a 24-bit start), so the host
-// self-test exercises the same steps from a worse start.
+// 1/sqrt(x) to within a few ulp: the hardware estimate (v_rsq_f64, 2^29
+// ulp = 2^-23 relative by the ISA's statement; measured at most 2^-24.2
+// over 4.2e6 arguments spread over 200 binades) and STEPS Newton steps
+// y <- y + y/2 (1 - x y^2).  From a start with relative error e <= 2^-14
+// each step gives 1.5 e^2 + ~3u, so two steps leave ~6u however coarse
+// the estimate, and one step from the hardware's leaves 1.5 2^-46 + 3u
+// (~200u).  x = 0 or non-finite gives NaN, which the callers send down
+// their exact routes.  On the host the estimate is the float rsqrt of the
+// value (a 24-bit start, e <= 2^-23 as well), so the host self-test
+// covers the same bounds.
+template <int STEPS = 2>
 GS_HD __forceinline__ double gs_rsqrt_cert(double x) {
 #ifdef __HIP_DEVICE_COMPILE__
     double y = __builtin_amdgcn_rsq(x);
@@ -64,7 +68,7 @@ GS_HD __forceinline__ double gs_rsqrt_cert(double x) {
     double y = (double)(1.0f / sqrtf((float)x));
     if (!(x > 0.0) || x > 1e300 || x < 1e-300) y = __builtin_nan("");
 #endif
-    for (int it = 0; it < 2; it++) {
+    for (int it = 0; it < STEPS; it++) {
         const double r = __builtin_fma(-(x * y), y, 1.0);
         y = __builtin_fma(0.5 * y, r, y);
     }
@@ -77,7 +81,8 @@ GS_HD __forceinline__ double gs_scan_fast_drlv2(const gs_ubi& u, const double* a
                                                 const double* c, double t0, double t1, double t2,
                                                 double ds) {
     const double e0 = a[0] - t0, e1 = a[1] - t1, e2 = a[2] - t2;
-    const double inv = ds * gs_rsqrt_cert(__builtin_fma(e2, e2, __builtin_fma(e1, e1, e0 * e0)));
+    const double inv =
+        ds * gs_rsqrt_cert<1>(__builtin_fma(e2, e2, __builtin_fma(e1, e1, e0 * e0)));
     const double G0 = __builtin_fma(e0, inv, -c[0]);
     const double G1 = __builtin_fma(e1, inv, -c[1]);
     const double G2 = __builtin_fma(e2, inv, -c[2]);
@@ -86,8 +91,8 @@ GS_HD __forceinline__ double gs_scan_fast_drlv2(const gs_ubi& u, const double* a
 
 // half-width of the undecided window of gs_scan_fast_drlv2 around
 // tol^2 - 3/4 for a grain (UBI u, |t|^2 = tt) and peaks with |xl|^2 >=
-// n2min; +inf where the bound does not apply.  The G term is as for the
-// reference's G (90 u ds per component, inverse square root included); the
+// n2min; +inf where the bound does not apply.  The G term covers the fast
+// G (290 u ds per component, inverse square root included); the
 // fract form adds the per-row 1/2 and its five roundings (3e-15, see
 // gs_cert_window_fract).
 GS_HD __forceinline__ double gs_scan_window(const gs_ubi& u, double ds, double tolsq, double tt,
