@@ -90,6 +90,8 @@ SIGNATURES = {
     "gs_joint_adjoint": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _i, _vp, ctypes.c_longlong,
                               _vp, _vp, _vp, _vp, _vp, ctypes.c_longlong, _i, _i, _vp, _vp, _vp,
                               _vp, _vp, _vp, _i, _vp, _vp]),
+    "gs_gauss_newton_ubis": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "gs_selftest_gn_ubis_host": (None, [_vp, _vp, _i, _vp]),
     "gs_graph_destroy": (_i, [_vp]),
     "closest": (None, [_vp, _vp, _ip, _dp, _i, _i]),
     "quickorient": (None, [_vp, _vp]),

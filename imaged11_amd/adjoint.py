@@ -18,6 +18,7 @@ wedge, chi) onto that packing with differentiable torch operations, so
 autograd carries the gradient to them.
 """
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -75,7 +76,8 @@ def pack_geometry(params, fixed, parity=False):
                      torch.stack([zero, torch.sin(c), torch.cos(c)])])
     mat = (W @ C).reshape(9)
     if parity:
-        exact = torch.as_tensor(batched.wedge_chi_matrix(float(P("wedge")), float(P("chi"))),
+        exact = torch.as_tensor(batched.wedge_chi_matrix(float(P("wedge").detach()),
+                                                         float(P("chi").detach())),
                                 dtype=torch.float64, device=dev).reshape(9)
         mat = mat + (exact - mat).detach()
     return torch.cat([p, r, dist, P("wavelength").reshape(1), mat])
@@ -250,50 +252,158 @@ def joint_loss(problem, ubis, trans, params, fixed):
     return JointLoss.apply(ubis, trans, geo, problem)
 
 
-def gauss_newton_ubis(sums, ubis=None):
+def gauss_newton_ubis(sums, ubis=None, out=None):
     """Per-grain least-squares UBI for fixed g-vectors and hkl:
-    UBI = (sum h G^T)(sum G G^T)^-1, from the (K,15) sums, with the
-    cofactor inverse of the symmetric 3x3 (elementwise device operations,
-    no library solver).  A grain whose sum G G^T is singular keeps its UBI
-    from ``ubis`` (as score_and_refine does; NaN when ubis is None)."""
-    A = sums[:, :9].reshape(-1, 3, 3)
-    s0, s1, s2, s3, s4, s5 = sums[:, 9:].unbind(1)
-    c00, c01, c02 = s3 * s5 - s4 * s4, s2 * s4 - s1 * s5, s1 * s4 - s2 * s3
-    c11, c12, c22 = s0 * s5 - s2 * s2, s1 * s2 - s0 * s4, s0 * s3 - s1 * s1
-    det = s0 * c00 + s1 * c01 + s2 * c02
-    inv = torch.stack([c00, c01, c02, c01, c11, c12, c02, c12, c22], 1).reshape(-1, 3, 3)
-    new = (A[:, :, :, None] * (inv / det[:, None, None])[:, None, :, :]).sum(2)
-    if ubis is None:
-        return new
-    return torch.where((det != 0)[:, None, None], new, ubis)
+    UBI = (sum h G^T)(sum G G^T)^-1 from the (K,15) sums of an evaluation
+    (``gs_gauss_newton_ubis``, one device thread per grain, cofactor
+    inverse).  A grain whose sum G G^T is singular keeps its UBI from
+    ``ubis`` (as score_and_refine does; NaN when ubis is None).  out: a
+    (K,3,3) float64 device tensor to write (may be ``ubis``), else a new
+    one."""
+    lib = _lib.load()
+    K = sums.shape[0]
+    batched._need(sums, torch.float64, "sums", (K, 15))
+    if ubis is not None:
+        batched._need(ubis, torch.float64, "ubis", (K, 3, 3))
+    if out is None:
+        out = torch.empty((K, 3, 3), dtype=torch.float64, device=sums.device)
+    batched._need(out, torch.float64, "out", (K, 3, 3))
+    _lib.check(lib.gs_gauss_newton_ubis(dptr(sums), dptr(ubis) if ubis is not None else None,
+                                        K, dptr(out), stream_ptr()), "gauss_newton_ubis")
+    return out
+
+
+def _rot(axis, a):
+    """Rotation by a about axis 0/1/2 with the sign conventions of
+    geometry.detector_rotation_matrix, and its derivative d/da."""
+    c, s = math.cos(a), math.sin(a)
+    if axis == 0:
+        m = [[1.0, 0.0, 0.0], [0.0, c, -s], [0.0, s, c]]
+        d = [[0.0, 0.0, 0.0], [0.0, -s, -c], [0.0, c, -s]]
+    elif axis == 1:
+        m = [[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]]
+        d = [[-s, 0.0, c], [0.0, 0.0, 0.0], [-c, 0.0, -s]]
+    else:
+        m = [[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]]
+        d = [[-s, -c, 0.0], [c, -s, 0.0], [0.0, 0.0, 0.0]]
+    return np.array(m), np.array(d)
+
+
+# slots of the packed geometry that depend on one parameter alone, linearly
+_LINEAR_SLOT = {"z_center": 0, "y_center": 1, "z_size": 2, "y_size": 3, "distance": 13,
+                "wavelength": 16}
+
+
+def pack_geometry_host(values, fixed, parity=False):
+    """``pack_geometry`` on the host without autograd: the 26 packed
+    numbers for the plain-number parameters ``values`` (names of FREE, or
+    z_size/y_size; the rest from ``fixed``) and their derivatives in closed
+    form, as (geo (26,), jac (26, len(values)) in the order of ``values``).
+    Tilts enter r = Rx.Ry.Rz.flip, wedge and chi (degrees) mat = W.C
+    (``ImageD11/transform.py:685-693``, ``src/cdiffraction.c:119-142``);
+    parity=True takes mat's values from the host's glibc sincos as
+    pack_geometry does."""
+    q = geometry.full_parameters(fixed)
+    names = list(values)
+    for k in names:
+        q[k] = float(values[k])
+    flip = geometry.flip_matrix(q["o11"], q["o12"], q["o21"], q["o22"])
+    rx, dx = _rot(0, q["tilt_x"])
+    ry, dy = _rot(1, q["tilt_y"])
+    rz, dz = _rot(2, q["tilt_z"])
+    deg = np.pi / 180.0
+    w, c = q["wedge"] * deg, q["chi"] * deg
+    cw, sw, cc, sc = math.cos(w), math.sin(w), math.cos(c), math.sin(c)
+    W = np.array([[cw, 0.0, -sw], [0.0, 1.0, 0.0], [sw, 0.0, cw]])
+    dW = np.array([[-sw, 0.0, -cw], [0.0, 0.0, 0.0], [cw, 0.0, -sw]]) * deg
+    C = np.array([[1.0, 0.0, 0.0], [0.0, cc, -sc], [0.0, sc, cc]])
+    dC = np.array([[0.0, 0.0, 0.0], [0.0, -sc, -cc], [0.0, cc, -sc]]) * deg
+    geo = np.empty(GEO)
+    geo[0:4] = (q["z_center"], q["y_center"], q["z_size"], q["y_size"])
+    geo[4:13] = (rx @ ry @ rz @ flip).ravel()
+    geo[13:16] = (q["distance"], 0.0, 0.0)
+    geo[16] = q["wavelength"]
+    geo[17:26] = (batched.wedge_chi_matrix(q["wedge"], q["chi"]) if parity else W @ C).ravel()
+    jac = np.zeros((GEO, len(names)))
+    for j, k in enumerate(names):
+        if k in _LINEAR_SLOT:
+            jac[_LINEAR_SLOT[k], j] = 1.0
+        elif k == "tilt_x":
+            jac[4:13, j] = (dx @ ry @ rz @ flip).ravel()
+        elif k == "tilt_y":
+            jac[4:13, j] = (rx @ dy @ rz @ flip).ravel()
+        elif k == "tilt_z":
+            jac[4:13, j] = (rx @ ry @ dz @ flip).ravel()
+        elif k == "wedge":
+            jac[17:26, j] = (dW @ C).ravel()
+        elif k == "chi":
+            jac[17:26, j] = (W @ dC).ravel()
+        else:
+            raise ValueError("pack_geometry_host: %r is not a geometry parameter" % (k,))
+    return geo, jac
+
+
+class Adam:
+    """Adam on a float64 vector (Kingma & Ba 2015, with bias correction),
+    in the arithmetic of ``torch.optim.Adam``'s default single-tensor
+    update: m <- m + (1 - b1)(g - m), v <- b2 v + (1 - b2) g^2,
+    x <- x - lr/(1 - b1^t) . m / (sqrt(v)/sqrt(1 - b2^t) + eps)."""
+
+    def __init__(self, n, lr, betas=(0.9, 0.999), eps=1e-8):
+        self.lr, self.b1, self.b2, self.eps = float(lr), betas[0], betas[1], eps
+        self.m = np.zeros(n)
+        self.v = np.zeros(n)
+        self.t = 0
+
+    def step(self, x, g):
+        """x after one step with gradient g (a new array)."""
+        self.t += 1
+        self.m += (1.0 - self.b1) * (g - self.m)
+        self.v = self.v * self.b2 + (1.0 - self.b2) * (g * g)
+        step_size = self.lr / (1.0 - self.b1 ** self.t)
+        denom = np.sqrt(self.v) / math.sqrt(1.0 - self.b2 ** self.t) + self.eps
+        return x - step_size * (self.m / denom)
 
 
 def refine_joint(problem, ubis, trans, params, fixed, steps=50, lr=1e-3, scales=None):
     """Joint refinement: each step is one adjoint pass; the UBIs take the
-    Gauss-Newton (exact least-squares) update for the current geometry and
-    the geometry parameters an Adam step on dL/dparams (per-parameter step
-    scales in ``scales``).  hkl and the peak lists stay fixed.  The handful
-    of geometry numbers (and their autograd graph) live on the host; the
-    26 packed values go to the device once per step.  Returns (ubis,
-    params, loss history)."""
-    ubis = ubis.detach().clone()
+    Gauss-Newton (exact least-squares) update for the current geometry
+    (``gs_gauss_newton_ubis``, in place on the device) and the geometry
+    parameters an Adam step on dL/dparams (per-parameter step scales in
+    ``scales``).  hkl and the peak lists stay fixed.  The handful of
+    geometry numbers live on the host: the 26 packed values and their
+    Jacobian come from ``pack_geometry_host``, go to the device once per
+    step, and dL/dgeo with the loss come back in one copy, the step's only
+    synchronisation.  Returns (ubis, params, loss history)."""
+    dev = problem.sc.device
+    ubis = _dev64(ubis, dev).clone()
+    trans = _dev64(trans, dev)
     names = sorted(params)
-    scale = {k: float((scales or {}).get(k, 1.0)) for k in names}
-    cpu = torch.device("cpu")
+    scale = np.array([float((scales or {}).get(k, 1.0)) for k in names])
     # optimise dimensionless offsets x_k with value = start + scale_k * x_k
-    start = {k: params[k].detach().to(cpu, torch.float64).clone() for k in names}
-    x = {k: torch.zeros((), dtype=torch.float64, requires_grad=True) for k in names}
-    opt = torch.optim.Adam([x[k] for k in names], lr=lr)
+    start = np.array([float(params[k]) for k in names])
+    x = np.zeros(len(names))
+    opt = Adam(len(names), lr)
+    pinned = dev.type == "cuda"
+    geo_h = torch.empty(GEO, dtype=torch.float64, pin_memory=pinned)
+    back_h = torch.empty(GEO + 1, dtype=torch.float64, pin_memory=pinned)
+    geo_d = torch.empty(GEO, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev) if pinned else None
     history = []
     for _ in range(steps):
-        cur = {k: start[k] + scale[k] * x[k] for k in names}
-        geo = pack_geometry(cur, fixed, parity=problem.parity)
-        loss, gu, gt, gg, sums = problem.evaluate(geo, ubis, trans, want_sums=True)
-        ubis = gauss_newton_ubis(sums, ubis)
-        g_host = gg.cpu()
-        history.append(float(loss))
-        opt.zero_grad()
-        geo.backward(g_host)
-        opt.step()
-    final = {k: (start[k] + scale[k] * x[k]).detach().to(params[k].device) for k in names}
+        cur = start + scale * x
+        geo, jac = pack_geometry_host(dict(zip(names, cur)), fixed, parity=problem.parity)
+        geo_h.numpy()[:] = geo
+        geo_d.copy_(geo_h, non_blocking=True)
+        _, _, _, gg, sums = problem.evaluate(geo_d, ubis, trans, want_sums=True)
+        gauss_newton_ubis(sums, ubis, out=ubis)
+        # gg (26) and the loss are adjacent in the evaluation's buffer
+        back_h.copy_(gg.as_strided((GEO + 1,), (1,)), non_blocking=True)
+        if stream is not None:
+            stream.synchronize()
+        back = back_h.numpy()
+        history.append(float(back[GEO]))
+        x = opt.step(x, (jac.T @ back[:GEO]) * scale)
+    final = {k: torch.tensor(float(v), dtype=torch.float64, device=params[k].device)
+             for k, v in zip(names, start + scale * x)}
     return ubis, final, history

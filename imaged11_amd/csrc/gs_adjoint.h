@@ -554,3 +554,41 @@ __global__ __launch_bounds__(GS_FIN_THREADS) void k_joint_finish(
         out[GS_GEO] = tot[0];
     }
 }
+
+// The per-grain Gauss-Newton UBI for fixed g-vectors and hkl: the UBI that
+// minimises sum ||UBI.G - h||^2 is A M^-1 with A = sum h G^T (row-major)
+// and M = sum G G^T (00 01 02 11 12 22), the 15 sums k_joint_finish
+// writes.  M^-1 = adj(M)/det by cofactors (M is symmetric).  A grain with
+// det == 0 keeps prev (NaN when there is none), as score_and_refine leaves
+// a UBI whose normal matrix is singular (src/closest.c:320-336, where the
+// fit is of UB in g-space).  out may be prev: every read precedes the
+// first write.
+GS_HD __forceinline__ void gs_gn_ubi(const double* __restrict__ n, const double* prev,
+                                     double* out) {
+    const double s0 = n[9], s1 = n[10], s2 = n[11], s3 = n[12], s4 = n[13], s5 = n[14];
+    const double c00 = s3 * s5 - s4 * s4, c01 = s2 * s4 - s1 * s5, c02 = s1 * s4 - s2 * s3;
+    const double c11 = s0 * s5 - s2 * s2, c12 = s1 * s2 - s0 * s4, c22 = s0 * s3 - s1 * s1;
+    const double det = s0 * c00 + s1 * c01 + s2 * c02;
+    double u[9];
+    if (det == 0.0) {
+        for (int i = 0; i < 9; i++) u[i] = prev ? prev[i] : __builtin_nan("");
+    } else {
+        const double i00 = c00 / det, i01 = c01 / det, i02 = c02 / det;
+        const double i11 = c11 / det, i12 = c12 / det, i22 = c22 / det;
+        const double inv[9] = {i00, i01, i02, i01, i11, i12, i02, i12, i22};
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++)
+                u[3 * r + c] = n[3 * r] * inv[c] + n[3 * r + 1] * inv[3 + c] +
+                               n[3 * r + 2] * inv[6 + c];
+    }
+    for (int i = 0; i < 9; i++) out[i] = u[i];
+}
+
+// one thread per grain
+__global__ __launch_bounds__(GS_BLOCK) void k_gn_ubis(const double* __restrict__ nsums,
+                                                      const double* prev, int K, double* out) {
+    const int g = (int)blockIdx.x * GS_BLOCK + (int)threadIdx.x;
+    if (g >= K) return;
+    gs_gn_ubi(nsums + GS_NSUMS * (size_t)g, prev ? prev + 9 * (size_t)g : nullptr,
+              out + 9 * (size_t)g);
+}

@@ -996,6 +996,12 @@ void gs_selftest_scan_host(const double* xl, const double* so, const double* co,
     }
 }
 
+void gs_selftest_gn_ubis_host(const double* nsums, const double* ubis, int K, double* out) {
+    for (int g = 0; g < K; g++)
+        gs_gn_ubi(nsums + GS_NSUMS * (size_t)g, ubis ? ubis + 9 * (size_t)g : nullptr,
+                  out + 9 * (size_t)g);
+}
+
 int gs_clear_error(void) {
     g_err.clear();
     return GS_OK;
@@ -1479,6 +1485,19 @@ int gs_joint_adjoint(const double* sc, const double* fc, const double* so, const
         if (nsums) { GS_FIN_LAUNCH(false, true); } else { GS_FIN_LAUNCH(false, false); }
     }
 #undef GS_FIN_LAUNCH
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
+int gs_gauss_newton_ubis(const double* nsums, const double* ubis, int K, double* out,
+                         gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (K < 0) return set_err(GS_ERR_ARG, "gs_gauss_newton_ubis: K = %d < 0", K);
+    if (K == 0) return GS_OK;
+    if (!nsums || !out) return set_err(GS_ERR_ARG, "gs_gauss_newton_ubis: null pointer");
+    hipLaunchKernelGGL(k_gn_ubis, dim3(blocks_for(K, GS_BLOCK)), dim3(GS_BLOCK), 0, S(stream),
+                       nsums, ubis, K, out);
     GS_LAUNCHED();
     return GS_OK;
 }

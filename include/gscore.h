@@ -73,6 +73,10 @@ void gs_selftest_scan_host(const double *xl, const double *so, const double *co,
                            const double *ubis, const double *trans, int n, const double *mat,
                            double wvln, double tol, int *out);
 
+/* gs_gauss_newton_ubis's per-grain update evaluated on the host, for
+ * checking it without a GPU.  No other entry point calls it. */
+void gs_selftest_gn_ubis_host(const double *nsums, const double *ubis, int K, double *out);
+
 /* ---- 1. reference prototypes (host pointers) ------------------------ */
 
 /* src/closest.c:100-119 — MAGIC rounding vs floor(x+0.5) on 600 points,
@@ -338,6 +342,17 @@ int gs_joint_adjoint(const double *sc, const double *fc, const double *so, const
                      long long n_pieces, int piece_len, int gspace, double *loss_g,
                      double *grad_ubi, double *grad_t, double *geo_part, double *nsums,
                      double *out, int lanes_per_piece, double *ws, gs_stream_t stream);
+
+/* gs_gauss_newton_ubis: the Gauss-Newton UBI step of a joint refinement,
+ * after a gs_joint_adjoint call that wrote nsums.  Per grain out[9g..] =
+ * (sum h G^T)(sum G G^T)^-1 from nsums[15g..] (the least-squares UBI for
+ * the fixed hkl and the current g-vectors), by the cofactor inverse; a
+ * grain whose sum G G^T has determinant 0 keeps ubis[9g..] (NaN when ubis
+ * is NULL), as score_and_refine keeps a UBI it cannot solve for
+ * (src/closest.c:320-336, which fits UB in g-space instead).  out may be
+ * ubis.  Device pointers, stream-ordered. */
+int gs_gauss_newton_ubis(const double *nsums, const double *ubis, int K, double *out,
+                         gs_stream_t stream);
 
 /* ---- compute_gv per grain + score_and_assign, all grains in order ----
  * The loop of refinegrains.assignlabels (ImageD11/refinegrains.py:

@@ -186,6 +186,77 @@ def test_joint_refinement_noisy_data(setup):
     assert hist[-1] <= 1.05 * best_loss
 
 
+def test_gauss_newton_kernel_against_solver(setup, dev):
+    """gs_gauss_newton_ubis on the device against numpy's solver of each
+    grain's normal equations, on the sums of a real evaluation and on
+    random ones with a singular grain; in place gives the same bits."""
+    s = setup
+    _, _, _, _, sums = s["prob"].evaluate(s["geo"], s["ubis"], s["trans"], want_sums=True)
+    rng = np.random.default_rng(9)
+    K = 300
+    G = rng.normal(size=(K, 30, 3))
+    h = rng.normal(size=(K, 30, 3))
+    A = np.einsum("kni,knj->kij", h, G)
+    M = np.einsum("kni,knj->kij", G, G)
+    M[5] = np.outer([1.0, 2.0, 3.0], [1.0, 2.0, 3.0])  # singular
+    rand = np.concatenate([A.reshape(K, 9), M[:, [0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]]], 1)
+    for sm, prev in ((sums, s["ubis"]), (T(rand, dev), T(rng.normal(size=(K, 3, 3)), dev))):
+        got = adjoint.gauss_newton_ubis(sm, prev).cpu().numpy()
+        n = sm.cpu().numpy()
+        Ah = n[:, :9].reshape(-1, 3, 3)
+        Mh = n[:, [9, 10, 11, 10, 12, 13, 11, 13, 14]].reshape(-1, 3, 3)
+        for k in range(len(n)):
+            if np.linalg.det(Mh[k]) == 0.0:
+                assert np.array_equal(got[k], prev[k].cpu().numpy())
+                continue
+            want = np.linalg.solve(Mh[k], Ah[k].T).T
+            np.testing.assert_allclose(got[k], want, rtol=1e-10,
+                                       atol=1e-12 * np.abs(want).max())
+        inplace = prev.clone()
+        adjoint.gauss_newton_ubis(sm, inplace, out=inplace)
+        assert np.array_equal(inplace.cpu().numpy(), got)
+    assert bool(torch.isnan(adjoint.gauss_newton_ubis(T(rand, dev))[5]).all())
+
+
+def test_refine_joint_follows_the_autograd_loop(setup):
+    """refine_joint (closed-form packing Jacobian, host Adam, device
+    Gauss-Newton step) against the same algorithm written with torch
+    autograd through pack_geometry and torch.optim.Adam: losses, final
+    parameters and UBIs agree over 12 steps."""
+    s = setup
+    names = ("chi", "distance", "tilt_x", "wavelength", "wedge", "y_center")
+    start = {k: float(v) for k, v in s["params"].items()}
+    start.update(distance=start["distance"] + 120.0, y_center=start["y_center"] - 0.5,
+                 tilt_x=start["tilt_x"] + 1e-4, wedge=start["wedge"] + 0.01)
+    scales = {"distance": 100.0, "tilt_x": 1e-3, "wavelength": 1e-5, "wedge": 0.01,
+              "chi": 0.01, "y_center": 1.0}
+    dev = s["ubis"].device
+    params = {k: torch.tensor(start[k], dtype=torch.float64, device=dev) for k in names}
+    ubis, final, hist = adjoint.refine_joint(s["prob"], s["ubis"], s["trans"], params,
+                                             dict(s["pars"]), steps=12, lr=0.05, scales=scales)
+    # the autograd loop
+    x = {k: torch.zeros((), dtype=torch.float64, requires_grad=True) for k in sorted(names)}
+    opt = torch.optim.Adam([x[k] for k in sorted(names)], lr=0.05, foreach=False)
+    u = s["ubis"].clone()
+    want = []
+    for _ in range(12):
+        cur = {k: start[k] + scales[k] * x[k] for k in names}
+        geo = adjoint.pack_geometry(cur, s["pars"])
+        loss, _, _, gg, sums = s["prob"].evaluate(geo.detach(), u, s["trans"], want_sums=True)
+        u = adjoint.gauss_newton_ubis(sums, u)
+        want.append(float(loss))
+        opt.zero_grad()
+        geo.backward(gg.cpu())
+        opt.step()
+    np.testing.assert_allclose(hist, want, rtol=1e-9)
+    for k in names:
+        v = start[k] + scales[k] * float(x[k])
+        assert abs(float(final[k]) - v) <= 1e-9 * scales[k], k
+    np.testing.assert_allclose(ubis.cpu().numpy(), u.cpu().numpy(), rtol=0,
+                               atol=1e-9 * float(u.abs().max()))
+    assert hist[-1] < hist[0]
+
+
 @pytest.fixture(scope="module")
 def exact_setup(dev):
     """Noise-free spots (no pixel jitter, exact omega, no spurious peaks):

@@ -269,10 +269,11 @@ def test_smoke_workload_generates():
 
 
 def test_gauss_newton_ubis_closed_form():
-    """The cofactor solve of the per-grain Gauss-Newton systems against
-    numpy's solver; a singular grain keeps its UBI."""
-    import torch
-    from imaged11_amd import adjoint
+    """The cofactor solve of the per-grain Gauss-Newton systems (the
+    device function of gs_gauss_newton_ubis, built for the host) against
+    numpy's solver; a singular grain keeps its UBI, or gets NaN without
+    one."""
+    lib = _lib.load()
     rng = np.random.default_rng(8)
     K = 50
     G = rng.normal(size=(K, 40, 3))
@@ -282,13 +283,85 @@ def test_gauss_newton_ubis_closed_form():
     M[3] = np.outer([1.0, 2.0, 3.0], [1.0, 2.0, 3.0])  # singular
     sums = np.concatenate([A.reshape(K, 9), M[:, [0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]]], 1)
     prev = rng.normal(size=(K, 3, 3))
-    got = adjoint.gauss_newton_ubis(torch.as_tensor(sums), torch.as_tensor(prev)).numpy()
+    got = np.empty((K, 3, 3))
+    lib.gs_selftest_gn_ubis_host(_lib.hptr(sums), _lib.hptr(prev), K, _lib.hptr(got))
+    nan = np.empty((K, 3, 3))
+    lib.gs_selftest_gn_ubis_host(_lib.hptr(sums), None, K, _lib.hptr(nan))
+    assert np.all(np.isnan(nan[3])) and np.array_equal(nan[:3], got[:3])
     for k in range(K):
         if k == 3:
             assert np.array_equal(got[k], prev[k])
         else:
             want = np.linalg.solve(M[k], A[k].T).T
             np.testing.assert_allclose(got[k], want, rtol=1e-11, atol=1e-12)
+
+
+def test_pack_geometry_host_matches_autograd():
+    """The closed-form packing and Jacobian of pack_geometry_host against
+    pack_geometry's values and torch autograd's derivatives of them, at
+    random geometries (wedge and chi non-zero), both trig modes; the
+    values against geometry.pack_xlylzl and the wedge/chi matrix too."""
+    import torch
+    from imaged11_amd import adjoint, geometry
+    rng = np.random.default_rng(21)
+    names = list(adjoint.FREE) + ["z_size", "y_size"]
+    for trial in range(6):
+        fixed = geometry.full_parameters({"o11": -1.0 if trial % 2 else 1.0, "o22": -1.0})
+        vals = {"distance": rng.uniform(5e3, 2e5), "y_center": rng.uniform(500, 1500),
+                "z_center": rng.uniform(500, 1500), "tilt_x": rng.normal(scale=0.02),
+                "tilt_y": rng.normal(scale=0.02), "tilt_z": rng.normal(scale=0.02),
+                "wavelength": rng.uniform(0.1, 0.3), "wedge": rng.normal(scale=2.0),
+                "chi": rng.normal(scale=2.0), "z_size": rng.uniform(40, 60),
+                "y_size": rng.uniform(40, 60)}
+        parity = trial >= 3
+        geo, jac = adjoint.pack_geometry_host(vals, fixed, parity=parity)
+        assert jac.shape == (adjoint.GEO, len(names))
+
+        def f(*xs):
+            ps = {k: x for k, x in zip(names, xs)}
+            return adjoint.pack_geometry(ps, fixed, parity=parity)
+        xs = tuple(torch.tensor(vals[k], dtype=torch.float64) for k in names)
+        want = f(*xs).numpy()
+        np.testing.assert_allclose(geo, want, rtol=0, atol=1e-15)
+        q = dict(fixed, **vals)
+        p, r, dist = geometry.pack_xlylzl(q)
+        np.testing.assert_array_equal(geo[:4], p)
+        np.testing.assert_allclose(geo[4:13], r, rtol=0, atol=1e-15)
+        np.testing.assert_array_equal(geo[13:16], dist)
+        assert geo[16] == vals["wavelength"]
+        np.testing.assert_allclose(geo[17:], geometry.wedge_chi_matrix(vals["wedge"], vals["chi"]),
+                                   rtol=0, atol=1e-15)
+        J = torch.autograd.functional.jacobian(f, xs)
+        J = np.stack([j.numpy() for j in J], 1)
+        np.testing.assert_allclose(jac, J, rtol=1e-13, atol=1e-16)
+        # and against central differences of the host packing itself
+        for j, k in enumerate(names):
+            h = 1e-6 * max(abs(vals[k]), 1.0)
+            up = adjoint.pack_geometry_host(dict(vals, **{k: vals[k] + h}), fixed)[0]
+            dn = adjoint.pack_geometry_host(dict(vals, **{k: vals[k] - h}), fixed)[0]
+            np.testing.assert_allclose(jac[:, j], (up - dn) / (2 * h), rtol=0, atol=1e-8)
+    import pytest
+    with pytest.raises(ValueError):
+        adjoint.pack_geometry_host({"omegasign": 1.0}, {})
+
+
+def test_adam_matches_torch():
+    """The host Adam of refine_joint against torch.optim.Adam over 40 steps
+    of random gradients (8 parameters)."""
+    import torch
+    from imaged11_amd import adjoint
+    rng = np.random.default_rng(22)
+    x0 = rng.normal(size=8)
+    grads = rng.normal(size=(40, 8)) * np.logspace(-6, 3, 8)
+    opt = adjoint.Adam(8, 0.05)
+    x = x0.copy()
+    t = torch.tensor(x0, requires_grad=True)
+    topt = torch.optim.Adam([t], lr=0.05, foreach=False)
+    for g in grads:
+        x = opt.step(x, g)
+        t.grad = torch.tensor(g)
+        topt.step()
+        np.testing.assert_allclose(x, t.detach().numpy(), rtol=1e-14, atol=1e-15)
 
 
 def test_ctypes_signatures_match_the_header():
