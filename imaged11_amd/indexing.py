@@ -46,6 +46,80 @@ def _dev_i32(a, device):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
 
 
+class HitList:
+    """The hits of a search: the reference's list of [diff, i, j] rows
+    (cosine difference, ring-1 peak, ring-2 peak; ``ImageD11/indexing.py:
+    693-728``), held as three numpy columns so that the batched steps take
+    them without a Python loop.  Indexing a row gives the reference's
+    [float, int, int] list; a slice or an index array gives a HitList; it
+    compares equal to a list of the same rows and supports the list
+    operations the reference applies to its hits (len, iteration, pop,
+    append)."""
+
+    __slots__ = ("diff", "i", "j")
+
+    def __init__(self, diff=(), i=(), j=()):
+        self.diff = np.ascontiguousarray(diff, dtype=np.float64)
+        self.i = np.ascontiguousarray(i, dtype=np.int64)
+        self.j = np.ascontiguousarray(j, dtype=np.int64)
+        if not (self.diff.shape == self.i.shape == self.j.shape and self.i.ndim == 1):
+            raise ValueError("HitList: three 1-d columns of one length")
+
+    @classmethod
+    def from_rows(cls, rows):
+        rows = list(rows)
+        return cls([r[0] for r in rows], [r[1] for r in rows], [r[2] for r in rows])
+
+    def __len__(self):
+        return self.i.shape[0]
+
+    def __getitem__(self, k):
+        if isinstance(k, (slice, np.ndarray, list)):
+            return HitList(self.diff[k], self.i[k], self.j[k])
+        return [float(self.diff[k]), int(self.i[k]), int(self.j[k])]
+
+    def __iter__(self):
+        return iter(self.tolist())
+
+    def tolist(self):
+        return [list(r) for r in zip(self.diff.tolist(), self.i.tolist(), self.j.tolist())]
+
+    def __eq__(self, other):
+        if isinstance(other, HitList):
+            return (np.array_equal(self.diff, other.diff) and np.array_equal(self.i, other.i)
+                    and np.array_equal(self.j, other.j))
+        try:
+            return self.tolist() == [list(r) for r in other]
+        except TypeError:
+            return NotImplemented
+
+    __hash__ = None
+
+    def pop(self):
+        if len(self) == 0:
+            raise IndexError("pop from empty HitList")
+        row = self[len(self) - 1]
+        self.diff, self.i, self.j = self.diff[:-1], self.i[:-1], self.j[:-1]
+        return row
+
+    def append(self, row):
+        d, i, j = row
+        self.diff = np.append(self.diff, float(d))
+        self.i = np.append(self.i, int(i))
+        self.j = np.append(self.j, int(j))
+
+    def __repr__(self):
+        return "HitList(%r)" % (self.tolist(),)
+
+
+def _hit_columns(hits):
+    """(ring-1 peaks, ring-2 peaks) of a HitList or of a list of rows."""
+    if isinstance(hits, HitList):
+        return hits.i, hits.j
+    return (np.array([h[1] for h in hits], dtype=np.int64),
+            np.array([h[2] for h in hits], dtype=np.int64))
+
+
 def ring_cosines(uc, ring_1, ring_2):
     """Distinct cosines between the hkls of two rings, largest first, with
     +-1 removed and values within 1e-5 merged (ImageD11/indexing.py:
@@ -65,8 +139,9 @@ def ring_cosines(uc, ring_1, ring_2):
 
 def find_hits(gv, i1, i2, cs, cosine_tol):
     """The hits of indexer.find for ring-1 peaks i1 and ring-2 peaks i2
-    (ImageD11/indexing.py:693-728) as [diff, i, j] rows in the reference's
-    order.  gv: (N,3) float64 device tensor; i1, i2: int index arrays."""
+    (ImageD11/indexing.py:693-728) as a HitList of [diff, i, j] rows in the
+    reference's order.  gv: (N,3) float64 device tensor; i1, i2: int index
+    arrays."""
     lib = _lib.load()
     _lib.require_gpu()
     dev = gv.device
@@ -75,7 +150,7 @@ def find_hits(gv, i1, i2, cs, cosine_tol):
     cs = np.ascontiguousarray(cs, dtype=np.float64)
     N1, N2, nv = len(i1), len(i2), len(cs)
     if N1 == 0 or N2 == 0:
-        return []
+        return HitList()
     n1 = torch.empty((N1, 3), dtype=torch.float64, device=dev)
     n2 = torch.empty((N2, 3), dtype=torch.float64, device=dev)
     d1, d2 = _dev_i32(i1, dev), _dev_i32(i2, dev)
@@ -91,25 +166,25 @@ def find_hits(gv, i1, i2, cs, cosine_tol):
                                     dptr(best), stream_ptr()), "find_hits")
         ib, best = ib.cpu().numpy(), best.cpu().numpy()
         sel = np.nonzero(best < tol)[0]
-        return [[float(best[i]), int(i1[i]), int(i2[ib[i]])] for i in sel]
+        return HitList(best[sel], i1[sel], i2[ib[sel]])
     mtol = -tol
     counter = torch.zeros(1, dtype=torch.int64, device=dev)
     _lib.check(lib.gs_find_all(dptr(n1), N1, dptr(n2), N2, dptr(dcs), nv, mtol, dptr(counter),
                                None, 0, stream_ptr()), "find_all")
     n = int(counter.item())
     if n == 0:
-        return []
+        return HitList()
     keys = torch.empty(n, dtype=torch.int64, device=dev)
     _lib.check(lib.gs_find_all(dptr(n1), N1, dptr(n2), N2, dptr(dcs), nv, mtol, dptr(counter),
                                dptr(keys), n, stream_ptr()), "find_all")
     keys = torch.sort(keys).values.cpu().numpy()
     j = keys % N2
     i = keys // (nv * N2)
-    return [[0.0, int(i1[a]), int(i2[b])] for a, b in zip(i, j)]
+    return HitList(np.zeros(n), i1[i], i2[j])
 
 
 def orient_hits(uc, ring_1, ring_2, gv, hits, entries=None, with_cosines=False):
-    """Trial UBI of every hit (rows [diff, i, j]) as unitcell.orient(ring_1,
+    """Trial UBI of every hit (a HitList or rows [diff, i, j]) as unitcell.orient(ring_1,
     gv[i], ring_2, gv[j]) makes it with crange <= 0, in one kernel; with
     `entries`, the UBI from table entry entries[h] instead (the crange > 0
     variants).  Returns (nh,3,3) UBIs, and the observed cosines when
@@ -117,8 +192,8 @@ def orient_hits(uc, ring_1, ring_2, gv, hits, entries=None, with_cosines=False):
     lib = _lib.load()
     dev = gv.device
     _, c2ab, matrs = uc.getanglehkls(ring_1, ring_2)
-    hi = _dev_i32([h[1] for h in hits], dev)
-    hj = _dev_i32([h[2] for h in hits], dev)
+    ci, cj = _hit_columns(hits)
+    hi, hj = _dev_i32(ci, dev), _dev_i32(cj, dev)
     nh = len(hits)
     ubis = torch.empty((nh, 3, 3), dtype=torch.float64, device=dev)
     cos = torch.empty(nh, dtype=torch.float64, device=dev) if with_cosines else None
@@ -204,7 +279,7 @@ class indexer:
         self.eta_range = eta_range
         self.ubis = []
         self.scores = []
-        self.hits = []
+        self.hits = HitList()
         self.omega = None
         self.wedge = 0.0
         self._gv_dev = None
@@ -312,7 +387,7 @@ class indexer:
         iall = np.arange(self.gv.shape[0])
         i1 = iall[(self.ra == self.ring_1) & (self.ga == -1)]
         i2 = iall[(self.ra == self.ring_2) & (self.ga == -1)]
-        self.hits = []
+        self.hits = HitList()
         if len(i1) == 0 or len(i2) == 0:
             logging.info("no peaks left for those rings")
             return
@@ -396,8 +471,9 @@ class indexer:
         lib = _lib.load()
         hits = self.hits
         nh = len(hits)
-        hi = np.ascontiguousarray([x[1] for x in hits], dtype=np.int32)
-        hj = np.ascontiguousarray([x[2] for x in hits], dtype=np.int32)
+        ci, cj = _hit_columns(hits)
+        hi = np.ascontiguousarray(ci, dtype=np.int32)
+        hj = np.ascontiguousarray(cj, dtype=np.int32)
         npk_h = np.ascontiguousarray(npk_all, dtype=np.int32)
         if follow is None:
             slot = np.full(nh, -1, np.int32)
@@ -496,16 +572,15 @@ class indexer:
     def _followups(self, g, tol, npk_all):
         """The follow-up of every hit above minpks at once (a _Followups,
         or None when no hit qualifies)."""
-        hits = self.hits
-        hi = np.array([x[1] for x in hits])
-        hj = np.array([x[2] for x in hits])
+        hits = self.hits if isinstance(self.hits, HitList) else HitList.from_rows(self.hits)
+        hi, hj = hits.i, hits.j
         cand = np.nonzero((npk_all > self.minpks) & (hi != hj))[0]
         slot = np.full(len(hits), -1, np.int64)
         slot[cand] = np.arange(len(cand))
         if len(cand) == 0:
             return None
         uc = self.unitcell
-        chits = [hits[h] for h in cand]
+        chits = hits[cand]
         ubi, cos = orient_hits(uc, self.ring_1, self.ring_2, g, chits, with_cosines=True)
         ubi = ubi.cpu().numpy()
         npk = npk_all[cand].astype(np.int64)
@@ -518,7 +593,7 @@ class indexer:
             _, c2ab, _ = uc.getanglehkls(self.ring_1, self.ring_2)
             near = abs(np.asarray(c2ab)[None, :] - cos.cpu().numpy()[:, None]) < crange
             rows, cols = np.nonzero(near)
-            var = orient_hits(uc, self.ring_1, self.ring_2, g, [chits[r] for r in rows],
+            var = orient_hits(uc, self.ring_1, self.ring_2, g, chits[rows],
                               entries=cols).cpu().numpy()
             starts = np.searchsorted(rows, np.arange(len(cand) + 1))
             lists, owners = [], []

@@ -261,3 +261,31 @@ def test_indexer_file_helpers(tmp_path):
     assert I.mod_360(350.0, 0.0) == -10.0 and I.mod_360(-350.0, 0.0) == 10.0
     assert I.mod_360(10.0, 720.0) == 730.0
     assert I.find_omega_ranges(np.array([0.2, 0.4, 10.6, 359.7, -0.3])) == 3
+
+
+def test_hitlist_behaves_as_the_reference_list():
+    """HitList against a plain list of the same [diff, i, j] rows under the
+    operations the reference and its callers apply to indexer.hits."""
+    from imaged11_amd.indexing import HitList, _hit_columns
+    rows = [[0.001 * k, 3 * k, 7 * k + 1] for k in range(9)]
+    h = HitList([r[0] for r in rows], [r[1] for r in rows], [r[2] for r in rows])
+    assert h == rows and rows == h and h == HitList.from_rows(rows)
+    assert h != rows[:-1] and len(h) == 9
+    assert h[4] == rows[4] and isinstance(h[4][1], int) and isinstance(h[4][0], float)
+    assert h[np.int64(2)] == rows[2]
+    assert h[:5] == rows[:5] and isinstance(h[:5], HitList)
+    assert h[np.array([1, 5, 8])] == [rows[1], rows[5], rows[8]]
+    assert [r for r in h] == rows
+    ci, cj = _hit_columns(h)
+    ci2, cj2 = _hit_columns(rows)
+    assert np.array_equal(ci, ci2) and np.array_equal(cj, cj2)
+    assert h.pop() == rows.pop() and h == rows
+    h.append([0.5, 11, 12])
+    rows.append([0.5, 11, 12])
+    assert h == rows
+    e = HitList()
+    assert len(e) == 0 and e == [] and [] == e
+    with pytest.raises(IndexError):
+        e.pop()
+    with pytest.raises(ValueError):
+        HitList([0.0], [1, 2], [3])
