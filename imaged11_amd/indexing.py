@@ -209,26 +209,72 @@ def orient_hits(uc, ring_1, ring_2, gv, hits, entries=None, with_cosines=False):
     return (ubis, cos) if with_cosines else ubis
 
 
-def select_lists(ubis, gv, thr):
+def select_lists(ubis, gv, thr, counts=None, sort=True):
     """CSR lists of the peaks each UBI indexes with the reference
-    arithmetic (drlv2 < thr), each list in peak order.  ubis (K,3,3) and gv
-    (N,3) device tensors; returns (idx int32, off int64 (K+1,)) on the
-    device."""
+    arithmetic (drlv2 < thr).  ubis (K,3,3) and gv (N,3) device tensors;
+    returns (idx int32, off int64 (K+1,)) on the device.
+
+    ``gs_select_many`` counts in one pass and writes the lists in a second.
+    counts (host ints, one per UBI): the expected list lengths, e.g. the
+    scores of the same UBIs at the same tolerance, or upper bounds; the
+    lists are then written in the first pass into that much room, and the
+    pass's own counts decide: equal counts are the result, smaller ones
+    are packed together on the device, and a longer list than its room
+    (a bound too small) sends the whole call to a second pass with the
+    counts the first one found.  sort=True puts each list in peak order
+    (the order of the reference's selection, and a fixed order for the
+    sums of a refinement); sort=False leaves the order of the atomic slots,
+    for callers that use a list as a set."""
     lib = _lib.load()
     dev = gv.device
     K, N = ubis.shape[0], gv.shape[0]
     cnt = torch.empty(K, dtype=torch.int32, device=dev)
-    _lib.check(lib.gs_select_many(dptr(ubis), K, dptr(gv), N, float(thr), None, dptr(cnt), None,
-                                  stream_ptr()), "select_many")
-    off = torch.zeros(K + 1, dtype=torch.int64, device=dev)
-    torch.cumsum(cnt.to(torch.int64), 0, out=off[1:])
-    total = int(off[-1].item())
-    idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
-    if total:
-        _lib.check(lib.gs_select_many(dptr(ubis), K, dptr(gv), N, float(thr), dptr(off),
-                                      dptr(cnt), dptr(idx), stream_ptr()), "select_many")
+
+    def fill(off, total, count=False):
+        # count=True: the pass also gives the counts (cnt), so it runs
+        # even when there is no room at all
+        idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        if total or count:
+            _lib.check(lib.gs_select_many(dptr(ubis), K, dptr(gv), N, float(thr), dptr(off),
+                                          dptr(cnt), dptr(idx), stream_ptr()), "select_many")
+        return idx
+
+    def offsets(c):
+        off_h = np.zeros(K + 1, np.int64)
+        np.cumsum(c, out=off_h[1:])
+        return torch.as_tensor(off_h, device=dev), int(off_h[-1])
+
+    room = None if counts is None else np.asarray(counts, dtype=np.int64).reshape(K)
+    if room is not None and K:
+        off, total = offsets(room)
+        idx = fill(off, total, count=True)
+        got = cnt.cpu().numpy().astype(np.int64)
+        if np.array_equal(got, room):
+            pass
+        elif np.all(got <= room):
+            # pack: entry q of list g moves from off[g] + q to new_off[g] + q
+            new_off, new_total = offsets(got)
+            owner = torch.repeat_interleave(torch.arange(K, device=dev),
+                                            torch.as_tensor(room, device=dev), output_size=total)
+            q = torch.arange(total, device=dev) - off[owner]
+            dest = torch.where(q < cnt.to(torch.int64)[owner], new_off[owner] + q,
+                               torch.full_like(q, new_total))
+            packed = torch.empty(new_total + 1, dtype=torch.int32, device=dev)
+            packed[dest] = idx[:total]
+            idx, off, total = packed, new_off, new_total
+        else:
+            off, total = offsets(got)
+            idx = fill(off, total)
+    else:
+        _lib.check(lib.gs_select_many(dptr(ubis), K, dptr(gv), N, float(thr), None, dptr(cnt),
+                                      None, stream_ptr()), "select_many")
+        off = torch.zeros(K + 1, dtype=torch.int64, device=dev)
+        torch.cumsum(cnt.to(torch.int64), 0, out=off[1:])
+        total = int(off[-1].item())
+        idx = fill(off, total)
+    if sort and total:
         owner = torch.repeat_interleave(torch.arange(K, device=dev, dtype=torch.int64),
-                                        cnt.to(torch.int64), output_size=total)
+                                        (off[1:] - off[:-1]), output_size=total)
         key = torch.sort(owner * N + idx[:total].to(torch.int64)).values
         idx = (key - owner * N).to(torch.int32)
     return idx[:total], off
@@ -494,10 +540,10 @@ class indexer:
             a(ga), len(ga), a(accepted), ctypes.byref(na), ctypes.byref(stop),
             ctypes.byref(nu)), "index_decide")
         self.ga[:] = ga
-        for h in accepted[:na.value]:
-            c = follow.slot[h]
-            self.ubis.append(follow.ubi[c].copy())
-            self.scores.append(int(follow.npk[c]))
+        if na.value:
+            rows = follow.slot[accepted[:na.value]]
+            self.ubis.extend(follow.ubi[rows])  # fancy indexing copies: one (3,3) array each
+            self.scores.extend(follow.npk[rows].tolist())
         self.nuniq = nu.value
         # the reference pops every visited hit off its list
         self.hits = hits[:stop.value]
@@ -618,10 +664,15 @@ class indexer:
         # score_and_refine of each chosen UBI over all peaks = one refine
         # iteration on the list of peaks it indexes
         dubi = torch.as_tensor(np.ascontiguousarray(ubi), device=g.device)
-        idx, off = select_lists(dubi, g, tol * tol)
+        # npk are the scores of these UBIs at tol, i.e. the lengths of
+        # their drlv2 < tol^2 lists
+        idx, off = select_lists(dubi, g, tol * tol, counts=npk)
         refined, _, _ = batched.refine_lists(dubi, g, idx, off, tol, n_iter=1)
-        # getind: score_and_assign from drlv2 = 1, labels = 0, label 1
-        idx2, off2 = select_lists(refined, g, min(tol * tol, 1.0))
+        # getind: score_and_assign from drlv2 = 1, labels = 0, label 1; the
+        # decision loop uses each list as a set, so any order does, and the
+        # refined UBI's list is about as long as the trial's
+        room = npk + np.maximum(16, npk // 4)
+        idx2, off2 = select_lists(refined, g, min(tol * tol, 1.0), counts=room, sort=False)
         return _Followups(slot, refined.cpu().numpy(), npk, idx2.cpu().numpy(),
                           off2.cpu().numpy())
 

@@ -199,6 +199,35 @@ def test_select_lists_match_oracle(dev, orc):
             assert np.array_equal(idx[off[k]:off[k + 1]], np.nonzero(lab == 1)[0]), (tol, k)
 
 
+def test_select_lists_with_counts_and_bounds(dev):
+    """select_lists given the list lengths (one pass), over-large room (the
+    lists packed together), too little room for some lists (the second
+    pass) and sort=False (the same sets): each equals the two-pass call,
+    on both selection kernels."""
+    wl = simulate.scoring_workload(K=60, N=40000, seed=13)
+    rng = np.random.default_rng(1)
+    g = torch.as_tensor(rng.normal(size=(40000, 3)) * 0.6, device=dev)
+    u = torch.as_tensor(wl["ubi"], device=dev)
+    for tol in (0.05, 0.3):
+        thr = tol * tol
+        idx, off = indexing.select_lists(u, g, thr)
+        idx, off = idx.cpu().numpy(), off.cpu().numpy()
+        n = np.diff(off)
+        assert n.sum() > 100 and np.all(n > 0)
+        small = n.copy()
+        small[::7] -= 1
+        for counts in (n, n + rng.integers(0, 50, len(n)), small, np.zeros_like(n)):
+            for sort in (True, False):
+                i2, o2 = indexing.select_lists(u, g, thr, counts=counts, sort=sort)
+                i2, o2 = i2.cpu().numpy(), o2.cpu().numpy()
+                assert np.array_equal(o2, off)
+                if sort:
+                    assert np.array_equal(i2, idx)
+                else:
+                    for k in range(len(n)):
+                        assert np.array_equal(np.sort(i2[o2[k]:o2[k + 1]]), idx[off[k]:off[k + 1]])
+
+
 @pytest.mark.parametrize("tol,hmax", [(0.05, 3), (0.05, 40), (1e-4, 2)])
 def test_select_lists_boundary_pairs(dev, orc, tol, hmax):
     """Pairs within 1e-17..1e-9 relative of tol^2 through the sieved
