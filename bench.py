@@ -79,9 +79,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the batched-refinement and adjoint measurements")
-    ap.add_argument("--profile", choices=("refine", "adjoint"), default=None,
+    ap.add_argument("--profile", choices=("refine", "adjoint", "scan"), default=None,
                     help="only run one workload a few times for a profiler (no timing "
-                         "line): the 5000-grain batched refinement or its adjoint")
+                         "line): the 5000-grain batched refinement or its adjoint, or "
+                         "the 20000-grain x 8e6-peak scan")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -793,10 +794,12 @@ def profile_workload(which, copy_mb, launches=5):
     score_and_refine iterations at 16 lanes per grain, 4 waves per CU,
     lists of 853 entries or more on 64 lanes and of 524 or more on 32 (the
     shape the autotuner picks on MI355X); the adjoint is one evaluation
-    with the Gauss-Newton sums at the default pieces and lanes."""
+    with the Gauss-Newton sums at the default pieces and lanes.  "scan":
+    aux_scan's parity-mode assignlabels of 20000 grains over 8e6 peaks,
+    twice."""
     import numpy as np
     import torch
-    from imaged11_amd import adjoint, batched, simulate
+    from imaged11_amd import adjoint, batched, drivers, simulate
 
     dev = torch.device("cuda:0")
     if copy_mb:
@@ -804,6 +807,20 @@ def profile_workload(which, copy_mb, launches=5):
         torch.empty_like(src).copy_(src)
         torch.cuda.synchronize()
         del src
+    if which == "scan":
+        K, N = 20000, 8_000_000
+        wl = simulate.scan_workload(K=K, N=N, seed=5, device="cuda")
+        pk = wl["peaks"]
+        T = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+        scan = drivers.Scan(T(pk["sc"]), T(pk["fc"]), T(pk["omega"]), wl["pars"],
+                            dty=T(pk["dty"]), parity=True)
+        ubis, trans = T(wl["ubi"]), T(wl["trans"])
+        for _ in range(2):
+            drivers.assignlabels(scan, ubis, trans, 0.05)
+        torch.cuda.synchronize()
+        print(json.dumps({"profiled": which, "launches": 2, "grains": K, "peaks": N}),
+              flush=True)
+        return
     K, N = 5000, 2_000_000
     wl = simulate.scoring_workload(K=K, N=N, seed=3)
     idx, off = simulate.refine_lists(wl["peaks"]["grain"], K, seed=3)
