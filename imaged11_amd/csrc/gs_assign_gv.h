@@ -175,35 +175,41 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
             const int L = glabels[g];
             const double t0 = lgr[4 * (g - c0)], t1 = lgr[4 * (g - c0) + 1],
                          t2 = lgr[4 * (g - c0) + 2], hi = lgr[4 * (g - c0) + 3];
-            int cnt = 0;
 #pragma unroll
             for (int p = 0; p < P; p++) {
                 const double a[3] = {ax[p], ay[p], az[p]}, c[3] = {cx[p], cy[p], cz[p]};
                 const double sf = gs_scan_fast_drlv2(u, a, c, t0, t1, t2, ds);
                 // candidates (and non-finite values) take the exact route
                 const bool cand = !(sf >= hi);
-                bool take = false;
-                if (cand) {
+                // a certain failure only matters to a peak holding this
+                // grain's label (else branch of the update: label -> -1);
+                // such peaks are rare, so one compare per pair decides
+                // wave-wide whether any lane needs the update
+                // (the empty asm statements keep the compiler from turning
+                // this branch into selects and from hoisting the exact
+                // route's products with t into the common path)
+                if (__ballot(lab[p] == L)) {
+                    asm volatile("");
+                    if (!cand && lab[p] == L) lab[p] = -1;
+                }
+                if (cand) {  // rare: the reference arithmetic decides
                     const long long k = base + (long long)p * GS_BLOCK;
                     if (k < ng) {
-                        double g0, g1, g2;
+                        double e0 = t0, e1 = t1, e2 = t2, g0, g1, g2;
+                        asm volatile("" : "+v"(e0), "+v"(e1), "+v"(e2));
                         gs_gv_one(xl[3 * k], xl[3 * k + 1], xl[3 * k + 2], so_in[k], co_in[k],
-                                  smat, ds, t0, t1, t2, g0, g1, g2);
+                                  smat, ds, e0, e1, e2, g0, g1, g2);
                         const double se = gs_drlv2(u, g0, g1, g2);
                         if ((se < tolsq) && (se < d[p])) {
-                            take = true;
                             lab[p] = L;
                             d[p] = se;
+                            atomicAdd(&lcnt[g - c0], 1);
                         } else if (lab[p] == L) {
                             lab[p] = -1;
                         }
                     }
-                } else if (lab[p] == L) {
-                    lab[p] = -1;
                 }
-                cnt += __popcll(__ballot(take));
             }
-            if (lane == 0 && cnt) atomicAdd(&lcnt[g - c0], cnt);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < c1 - c0; i += GS_BLOCK) {
