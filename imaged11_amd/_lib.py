@@ -92,6 +92,11 @@ SIGNATURES = {
                               _vp, _vp, _vp, _i, _vp, _vp]),
     "gs_gauss_newton_ubis": (_i, [_vp, _vp, _i, _vp, _vp]),
     "gs_selftest_gn_ubis_host": (None, [_vp, _vp, _i, _vp]),
+    "gs_simplex_order": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _vp,
+                              _vp]),
+    "gs_simplex_reflected": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp,
+                                  _vp, _vp]),
+    "gs_simplex_trial": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gs_graph_destroy": (_i, [_vp]),
     "closest": (None, [_vp, _vp, _ip, _dp, _i, _i]),
     "quickorient": (None, [_vp, _vp]),

@@ -1913,6 +1913,171 @@ int gs_select_many(const double* ubis, int K, const double* gv, int ng, double t
     return GS_OK;
 }
 
+// One iteration of B Nelder-Mead simplexes stepped in lockstep, split at
+// the points where the objective is called (ImageD11/simplex.py:115-227,
+// the reference's one-problem loop; the batched driver around these calls
+// is SimplexBatch in imaged11_amd/simplex.py).  S (B, n+3, n): the n+1
+// vertices, the centroid and the reflected point of every problem; E
+// (B, n+1) their errors.  Comparisons follow the reference's `<` / `>`
+// (a NaN never wins), sums run in vertex order, and no product is fused
+// into an addition, so every value is the one the reference computes.
+//
+// gs_simplex_order: per active problem (done[b] == 0) the highest, lowest
+// and second-highest vertices, the convergence test (standard deviation of
+// the errors / n <= epsilon: done[b] = 1), lowest/iters bookkeeping, and
+// for the problems that go on (a[b] = 1) the centroid of all vertices but
+// the highest into S[b, n+1] and the reflected point into S[b, n+2].
+int gs_simplex_order(int B, int n, double* S, const double* E, unsigned char* done,
+                     long long* lowest, long long* iters, int it, double epsilon, double kR,
+                     long long* hi, long long* lo, long long* second, unsigned char* a,
+                     int* n_active) {
+    if (B < 0 || n < 1 || n > 64 || !S || !E || !done || !lowest || !iters || !hi || !lo || !second || !a ||
+        !n_active)
+        return set_err(GS_ERR_ARG, "gs_simplex_order: bad arguments");
+    const int nv = n + 1, ns = n + 3;
+    int na = 0;
+    for (int b = 0; b < B; b++) {
+        a[b] = 0;
+        if (done[b]) continue;
+        const double* e = E + (size_t)b * nv;
+        long long h = 0, l = 0;
+        for (int v = 1; v < nv; v++) {
+            if (e[v] > e[h]) h = v;
+            if (e[v] < e[l]) l = v;
+        }
+        long long sec = l;
+        for (int v = 0; v < nv; v++)
+            if (v != h && v != sec && e[v] > e[sec]) sec = v;
+        double tot = 0.0;
+        for (int v = 0; v < nv; v++) tot = tot + e[v];
+        const double mean = tot / (double)nv;
+        double ss = 0.0;
+        for (int v = 0; v < nv; v++) {
+            const double d = e[v] - mean;
+            ss = ss + d * d;
+        }
+        hi[b] = h;
+        lo[b] = l;
+        second[b] = sec;
+        lowest[b] = l;
+        iters[b] = it;
+        if (std::sqrt(ss / (double)n) <= epsilon) {
+            done[b] = 1;
+            continue;
+        }
+        a[b] = 1;
+        na++;
+        double* sb = S + (size_t)b * ns * n;
+        double* cen = sb + (size_t)(n + 1) * n;
+        double* refl = sb + (size_t)(n + 2) * n;
+        const double* sh = sb + (size_t)h * n;
+        for (int j = 0; j < n; j++) {
+            double c = 0.0;
+            for (int v = 0; v < nv; v++)
+                if (v != h) c = c + sb[(size_t)v * n + j];
+            cen[j] = c / (double)n;
+        }
+        for (int j = 0; j < n; j++) {
+            const double x = kR * sh[j];
+            const double y = (1.0 - kR) * cen[j];
+            refl[j] = x + y;
+        }
+    }
+    *n_active = na;
+    return GS_OK;
+}
+
+// gs_simplex_reflected: fr[b] = the error at the reflected point of every
+// active problem.  The reflected point replaces the highest vertex when it
+// is lower; then expansion (fr <= the lowest error) or contraction (fr >=
+// the second-highest error) is chosen and its point written to trial[b]
+// (step[b] = 1, expand[b] = 1 for an expansion).
+int gs_simplex_reflected(int B, int n, double* S, double* E, const unsigned char* a,
+                         const long long* hi, const long long* lo, const long long* second,
+                         const double* fr, double kE, double kC, double* trial,
+                         unsigned char* expand, unsigned char* step, int* n_step) {
+    if (B < 0 || n < 1 || n > 64 || !S || !E || !a || !hi || !lo || !second || !fr || !trial || !expand ||
+        !step || !n_step)
+        return set_err(GS_ERR_ARG, "gs_simplex_reflected: bad arguments");
+    const int nv = n + 1, ns = n + 3;
+    int nt = 0;
+    for (int b = 0; b < B; b++) {
+        expand[b] = 0;
+        step[b] = 0;
+        if (!a[b]) continue;
+        double* e = E + (size_t)b * nv;
+        double* sb = S + (size_t)b * ns * n;
+        const double* cen = sb + (size_t)(n + 1) * n;
+        const double* refl = sb + (size_t)(n + 2) * n;
+        const long long h = hi[b];
+        const double f = fr[b];
+        if (f < e[h]) {
+            e[h] = f;
+            for (int j = 0; j < n; j++) sb[(size_t)h * n + j] = refl[j];
+        }
+        const bool ex = f <= e[lo[b]];
+        const bool co = !ex && f >= e[second[b]];
+        double* t = trial + (size_t)b * n;
+        for (int j = 0; j < n; j++) {
+            if (ex) {
+                const double x = kE * refl[j];
+                const double y = (1.0 - kE) * cen[j];
+                t[j] = x + y;
+            } else {
+                const double x = kC * sb[(size_t)h * n + j];
+                const double y = (1.0 - kC) * cen[j];
+                t[j] = x + y;
+            }
+        }
+        expand[b] = ex;
+        step[b] = ex || co;
+        nt += step[b];
+    }
+    *n_step = nt;
+    return GS_OK;
+}
+
+// gs_simplex_trial: fb[b] = the error at trial[b] of every problem with
+// step[b].  The trial point replaces the highest vertex when it is lower;
+// a contraction that fails shrinks every vertex but the lowest halfway
+// towards it (shrink[b] = 1); the caller then evaluates those vertices in
+// vertex order.
+int gs_simplex_trial(int B, int n, double* S, double* E, const unsigned char* step,
+                     const unsigned char* expand, const long long* hi, const long long* lo,
+                     const double* fb, const double* trial, unsigned char* shrink,
+                     int* n_shrink) {
+    if (B < 0 || n < 1 || n > 64 || !S || !E || !step || !expand || !hi || !lo || !fb || !trial ||
+        !shrink || !n_shrink)
+        return set_err(GS_ERR_ARG, "gs_simplex_trial: bad arguments");
+    const int nv = n + 1, ns = n + 3;
+    int nk = 0;
+    for (int b = 0; b < B; b++) {
+        shrink[b] = 0;
+        if (!step[b]) continue;
+        double* e = E + (size_t)b * nv;
+        double* sb = S + (size_t)b * ns * n;
+        const long long h = hi[b], l = lo[b];
+        const double* t = trial + (size_t)b * n;
+        const bool take = fb[b] < e[h];
+        if (take) {
+            e[h] = fb[b];
+            for (int j = 0; j < n; j++) sb[(size_t)h * n + j] = t[j];
+        }
+        if (expand[b] || take) continue;
+        shrink[b] = 1;
+        nk++;
+        double base[64];
+        for (int j = 0; j < n; j++) base[j] = sb[(size_t)l * n + j];
+        for (int v = 0; v < nv; v++) {
+            if (v == l) continue;
+            for (int j = 0; j < n; j++)
+                sb[(size_t)v * n + j] = 0.5 * (sb[(size_t)v * n + j] + base[j]);
+        }
+    }
+    *n_shrink = nk;
+    return GS_OK;
+}
+
 // The decision loop of indexer.scorethem (ImageD11/indexing.py:778-838) on
 // host arrays, over follow-ups computed beforehand for every hit: visited
 // from the last hit to the first, a hit is skipped when either peak is

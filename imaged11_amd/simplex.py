@@ -24,7 +24,11 @@ reference's own evaluation sequences.  The only arithmetic difference is
 libm's pow, which can differ from the correctly rounded product by one
 rounding in rare cases, and only the convergence test reads it.
 """
+import ctypes
+
 import numpy as np
+
+from . import _lib
 
 
 class SimplexBatch:
@@ -62,72 +66,49 @@ class SimplexBatch:
         return vals
 
     def minimize(self, epsilon=0.0001, maxiters=250):
-        """Returns (points (B,n), errors (B,), iterations (B,))."""
+        """Returns (points (B,n), errors (B,), iterations (B,)).  Each
+        iteration's bookkeeping between the objective calls runs in native
+        code over all problems (gs_simplex_order, gs_simplex_reflected,
+        gs_simplex_trial); the calls themselves go to the batched
+        objective as before."""
+        lib = _lib.load()
         B, n = self.B, self.n
         S, E = self.vertices, self.errors
+        assert S.flags.c_contiguous and E.flags.c_contiguous
         ar = np.arange(B)
-        done = np.zeros(B, dtype=bool)
-        lowest = np.full(B, -1)
+        done = np.zeros(B, dtype=np.uint8)
+        lowest = np.full(B, -1, dtype=np.int64)
         iters = np.zeros(B, dtype=np.int64)
+        hi, lo, second = (np.zeros(B, dtype=np.int64) for _ in range(3))
+        a, expand, step, shrink = (np.zeros(B, dtype=np.uint8) for _ in range(4))
+        fr = np.full(B, np.nan)
+        fb = np.full(B, np.nan)
+        trial = np.zeros((B, n))
+        cnt = ctypes.c_int()
+        p = lambda x: x.ctypes.data
         for it in range(maxiters):
-            act = ~done
-            if not act.any():
+            if done.all():
                 break
-            hi = np.zeros(B, dtype=np.int64)
-            lo = np.zeros(B, dtype=np.int64)
-            for v in range(1, n + 1):
-                hi = np.where(E[:, v] > E[ar, hi], v, hi)
-                lo = np.where(E[:, v] < E[ar, lo], v, lo)
-            second = lo.copy()
-            for v in range(n + 1):
-                upd = (v != hi) & (v != second) & (E[:, v] > E[ar, second])
-                second = np.where(upd, v, second)
-            tot = np.zeros(B)
-            for v in range(n + 1):
-                tot = tot + E[:, v]
-            mean = tot / (n + 1)
-            ss = np.zeros(B)
-            for v in range(n + 1):
-                d = E[:, v] - mean
-                ss = ss + d * d
-            conv = act & (np.sqrt(ss / n) <= epsilon)
-            lowest[act] = lo[act]
-            iters[act] = it
-            done |= conv
-            a = act & ~conv
-            if not a.any():
+            _lib.check(lib.gs_simplex_order(B, n, p(S), p(E), p(done), p(lowest), p(iters), it,
+                                            float(epsilon), self.kR, p(hi), p(lo), p(second),
+                                            p(a), ctypes.byref(cnt)), "simplex")
+            if cnt.value == 0:
                 continue
-            # centroid of all vertices but the highest, in vertex order
-            csum = np.zeros((B, n))
-            for v in range(n + 1):
-                csum = np.where((v != hi)[:, None], csum + S[:, v, :], csum)
-            cen = csum / n
-            S[a, n + 1] = cen[a]
-            refl = self.kR * S[ar, hi] + (1 - self.kR) * cen
-            S[a, n + 2] = refl[a]
-            fr = np.full(B, np.nan)
-            fr[a] = self._call(refl[a], a)
-            acc = a & (fr < E[ar, hi])
-            E[acc, hi[acc]] = fr[acc]
-            S[acc, hi[acc]] = S[acc, n + 2]
-            expand = a & (fr <= E[ar, lo])
-            contract = a & ~expand & (fr >= E[ar, second])
-            trial = np.where(expand[:, None], self.kE * refl + (1 - self.kE) * cen,
-                             self.kC * S[ar, hi] + (1 - self.kC) * cen)
-            step = expand | contract
-            fb = np.full(B, np.nan)
-            fb[step] = self._call(trial[step], step)
-            take = step & (fb < E[ar, hi])
-            E[take, hi[take]] = fb[take]
-            S[take, hi[take]] = trial[take]
-            shrink = contract & ~take
-            if shrink.any():
-                base = S[ar, lo]
+            am = a.view(bool)
+            fr[am] = self._call(S[am, n + 2], am)
+            _lib.check(lib.gs_simplex_reflected(B, n, p(S), p(E), p(a), p(hi), p(lo), p(second),
+                                                p(fr), self.kE, self.kC, p(trial), p(expand),
+                                                p(step), ctypes.byref(cnt)), "simplex")
+            sm = step.view(bool)
+            if cnt.value:
+                fb[sm] = self._call(trial[sm], sm)
+            _lib.check(lib.gs_simplex_trial(B, n, p(S), p(E), p(step), p(expand), p(hi), p(lo),
+                                            p(fb), p(trial), p(shrink), ctypes.byref(cnt)),
+                       "simplex")
+            if cnt.value:
+                km = shrink.view(bool)
                 for v in range(n + 1):
-                    m = shrink & (lo != v)
-                    S[m, v] = 0.5 * (S[m, v] + base[m])
-                for v in range(n + 1):
-                    m = shrink & (lo != v)
+                    m = km & (lo != v)
                     if m.any():
                         E[m, v] = self._call(S[m, v], m)
         # the lowest vertex of the last ordering; before any iteration the

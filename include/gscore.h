@@ -574,6 +574,34 @@ int gs_orient_hits(const double *gv, int ng, const int *hi, const int *hj, int n
 int gs_select_many(const double *ubis, int K, const double *gv, int ng, double thr,
                    const long long *off, int *cursor, int *idx, gs_stream_t stream);
 
+/* One iteration of B Nelder-Mead simplexes of n <= 64 variables stepped in
+ * lockstep, on HOST arrays, split where the objective is called
+ * (ImageD11/simplex.py:115-227, the reference's one-problem loop, which
+ * refinegrains.refinepositions runs per grain).  S (B, n+3, n): vertices,
+ * centroid, reflected point; E (B, n+1): vertex errors; flags are bytes.
+ * gs_simplex_order: per problem not done, hi/lo/second vertex, the
+ * convergence test (done), lowest/iters = lo/it, and for the problems that
+ * go on (a = 1, *n_active of them) the centroid and the reflected point.
+ * The caller evaluates the reflected points into fr, then
+ * gs_simplex_reflected accepts a lower one and writes the expansion or
+ * contraction point into trial (step = 1, expand for an expansion); the
+ * caller evaluates them into fb, then gs_simplex_trial accepts a lower one
+ * and shrinks the simplexes whose contraction failed (shrink = 1; the
+ * caller evaluates their non-lowest vertices in vertex order).  No device
+ * work; every value is the reference's, comparison and rounding alike. */
+int gs_simplex_order(int B, int n, double *S, const double *E, unsigned char *done,
+                     long long *lowest, long long *iters, int it, double epsilon, double kR,
+                     long long *hi, long long *lo, long long *second, unsigned char *a,
+                     int *n_active);
+int gs_simplex_reflected(int B, int n, double *S, double *E, const unsigned char *a,
+                         const long long *hi, const long long *lo, const long long *second,
+                         const double *fr, double kE, double kC, double *trial,
+                         unsigned char *expand, unsigned char *step, int *n_step);
+int gs_simplex_trial(int B, int n, double *S, double *E, const unsigned char *step,
+                     const unsigned char *expand, const long long *hi, const long long *lo,
+                     const double *fb, const double *trial, unsigned char *shrink,
+                     int *n_shrink);
+
 /* The decision loop of indexer.scorethem (ImageD11/indexing.py:778-838) on
  * HOST arrays, once every hit's follow-up is known: hits h = nhits-1 .. 0
  * (hit_i/hit_j their two peaks, npk their trial scores, cand[h] the row of
