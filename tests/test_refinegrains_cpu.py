@@ -82,6 +82,49 @@ def test_simplex_batch_lockstep_equals_sequential():
     assert sb.rounds < sum(len(c) for c, _ in seq)
 
 
+def test_simplex_batch_nan_ties_and_limits():
+    """Objectives with NaN regions and flat plateaus (ties in every
+    comparison) stepped together equal the sequential oracle simplex call
+    for call; more than 64 variables is refused."""
+    import math
+    rng = np.random.default_rng(3)
+    B = 24
+    gs = rng.normal(size=(B, 3))
+
+    def fn(b, a):
+        x, y, z = a
+        if b % 3 == 0 and x > 0.7:
+            return float("nan")
+        if b % 3 == 1:
+            return float(math.floor(2 * (x * x + y * y + z * z)))  # plateaus
+        return (x - 0.3) ** 2 + 2 * (y + 0.1) ** 2 + 0.5 * z * z
+    seq = []
+    for b in range(B):
+        calls = []
+
+        def rec(a, b=b, calls=calls):
+            calls.append(list(map(float, a)))
+            return fn(b, list(a))
+        seq.append((calls, R.Simplex(rec, list(gs[b]), [0.4] * 3).minimize(maxiters=60)))
+    got = [[] for _ in range(B)]
+
+    def ev(pts, which):
+        out = []
+        for p, b in zip(pts, which):
+            got[b].append(list(map(float, p)))
+            out.append(fn(b, list(p)))
+        return out
+    P, E, I = simplex.SimplexBatch(ev, gs, [0.4] * 3).minimize(maxiters=60)
+    for b in range(B):
+        calls, (res, err, it) = seq[b]
+        assert got[b] == calls, b
+        assert list(P[b]) == list(res) and I[b] == it
+        assert E[b] == err or (math.isnan(E[b]) and math.isnan(err))
+    with pytest.raises(RuntimeError):
+        simplex.SimplexBatch(lambda p, w: np.zeros(len(w)), np.zeros((2, 65)),
+                             np.ones(65)).minimize(maxiters=2)
+
+
 # ---- transform chain -------------------------------------------------------
 
 @pytest.mark.parametrize("wedge,chi", [(0.0, 0.0), (1.5, -2.0), (-3.0, 0.0)])
