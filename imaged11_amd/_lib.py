@@ -122,6 +122,7 @@ SIGNATURES = {
                                _vp, _i, _vp, _vp]),
     "gs_pbp_ubi_fit": (_i, [_vp, _vp, _vp, _vp, _dp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gs_pbp_strain_fit": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
+    "gs_pbp_rows_dot": (_i, [_vp, _vp, ctypes.c_longlong, _vp, _d, _vp, _vp, _vp, _vp]),
     "gs_segment_sum": (_i, [_vp, _i, _vp, _i, _vp, _vp]),
     "gs_segment_max": (_i, [_vp, _i, _vp, _i, _vp, _vp]),
     "gs_grain_gv_host": (_i, [_vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _vp, _vp, _i, _dp, _dp, _d,

@@ -452,3 +452,36 @@ __global__ __launch_bounds__(64) void k_pbp_strain_fit(const double* __restrict_
     }
     for (int i = 0; i < 6; i++) eps[6 * (size_t)t + i] = e[i];
 }
+
+// Row products of refine_map's merged-peak scoring and strain fit: per row
+// r, h = U g with U = ubis[task[r]] (or row r's own 3x3 when task is
+// NULL), each component ((u_i0 g_0 + u_i1 g_1) + u_i2 g_2) with every
+// product and sum rounded on its own (ubi.dot(gve_voxel_merged) and
+// U.dot(B0).dot(hkl), ImageD11/sinograms/point_by_point.py:1582,1611,1639,
+// in the row order and arithmetic of the device implementation they
+// replace).  With hkl given: hkl = h rounded half to even (np.round) and
+// ok = ((e_0^2 + e_1^2) + e_2^2) < tolsq for e = h - hkl (:1581-1590).
+__global__ __launch_bounds__(GS_BLOCK) void k_pbp_rows_dot(
+    const double* __restrict__ ubis, const long long* __restrict__ task, long long R,
+    const double* __restrict__ g, double tolsq, double* __restrict__ h,
+    long long* __restrict__ hkl, unsigned char* __restrict__ ok) {
+    const long long r = (long long)blockIdx.x * GS_BLOCK + threadIdx.x;
+    if (r >= R) return;
+    const double* u = ubis + 9 * (size_t)(task ? task[r] : r);
+    const double g0 = g[3 * r], g1 = g[3 * r + 1], g2 = g[3 * r + 2];
+    double e2 = 0.0;
+    for (int i = 0; i < 3; i++) {
+        const double a = u[3 * i] * g0;
+        const double b = u[3 * i + 1] * g1;
+        const double c = u[3 * i + 2] * g2;
+        const double v = (a + b) + c;
+        if (h) h[3 * r + i] = v;
+        if (hkl) {
+            const double k = rint(v);
+            const double e = v - k;
+            e2 = i == 0 ? e * e : e2 + e * e;
+            hkl[3 * r + i] = (long long)k;
+        }
+    }
+    if (ok) ok[r] = e2 < tolsq;
+}

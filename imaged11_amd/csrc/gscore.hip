@@ -1682,6 +1682,21 @@ int gs_pbp_strain_fit(const double* design, const long long* off, const int* wor
     return GS_OK;
 }
 
+int gs_pbp_rows_dot(const double* ubis, const long long* task, long long R, const double* g,
+                    double tolsq, double* h, long long* hkl, unsigned char* ok,
+                    gs_stream_t stream) {
+    DeviceCtx* ctx;
+    GS_CHECK(get_ctx(&ctx));
+    if (R < 0) return set_err(GS_ERR_ARG, "gs_pbp_rows_dot: negative row count");
+    if (R == 0) return GS_OK;
+    if (!ubis || !g || (!h && !hkl) || (hkl && !ok) || (ok && !hkl))
+        return set_err(GS_ERR_ARG, "gs_pbp_rows_dot: NULL array");
+    hipLaunchKernelGGL(k_pbp_rows_dot, dim3(blocks_for(R, GS_BLOCK)), dim3(GS_BLOCK), 0,
+                       S(stream), ubis, task, R, g, tolsq, h, hkl, ok);
+    GS_LAUNCHED();
+    return GS_OK;
+}
+
 int gs_segment_sum(const double* v, int C, const long long* off, int nseg, double* out,
                    gs_stream_t stream) {
     DeviceCtx* ctx;

@@ -291,17 +291,32 @@ def _groups(cols):
     return inv, G
 
 
-def _rows_dot(u, g):
-    """h = u.g per row, written out in row order."""
-    return torch.stack([u[:, i, 0] * g[:, 0] + u[:, i, 1] * g[:, 1] + u[:, i, 2] * g[:, 2]
-                        for i in range(3)], 1)
+def _rows_dot(ubis, task, g):
+    """h = U.g per row with U = ubis[task[r]] (gs_pbp_rows_dot: each
+    component ((u0 g0 + u1 g1) + u2 g2), every operation rounded on its
+    own).  ubis (T,3,3) float64, task (R,) int64, g (R,3) float64, all on
+    the device; returns (R,3)."""
+    ubis, task, g = ubis.contiguous(), task.to(torch.int64).contiguous(), g.contiguous()
+    lib = _lib.load()
+    R = g.shape[0]
+    h = torch.empty((R, 3), dtype=torch.float64, device=g.device)
+    _lib.check(lib.gs_pbp_rows_dot(dptr(ubis), dptr(task), R, dptr(g), 0.0, dptr(h), None,
+                                   None, stream_ptr()), "pbp_rows_dot")
+    return h
 
 
-def _score(u, g, tol):
-    h = _rows_dot(u, g)
-    hi = torch.round(h)
-    e = h - hi
-    return hi.long(), ((e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]) + e[:, 2] * e[:, 2]) < tol * tol
+def _score(ubis, task, g, tol):
+    """(hkl (R,3) int64, ok (R,) bool) of every row: h = ubis[task[r]].g
+    rounded half to even, ok = the squared distance to it < tol^2
+    (gs_pbp_rows_dot)."""
+    ubis, task, g = ubis.contiguous(), task.to(torch.int64).contiguous(), g.contiguous()
+    lib = _lib.load()
+    R = g.shape[0]
+    hkl = torch.empty((R, 3), dtype=torch.int64, device=g.device)
+    ok = torch.empty(R, dtype=torch.bool, device=g.device)
+    _lib.check(lib.gs_pbp_rows_dot(dptr(ubis), dptr(task), R, dptr(g), float(tol * tol), None,
+                                   dptr(hkl), dptr(ok), stream_ptr()), "pbp_rows_dot")
+    return hkl, ok
 
 
 def ubi_fit(design, off, fit, ubis, B0):
@@ -415,7 +430,7 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
     rtask, rydist, reta = gtask[keep], yd[keep], m_eta[keep]
     g = _gve_device(m_sc[keep].contiguous(), m_fc[keep].contiguous(), m_om[keep].contiguous(),
                     m_xpos[keep].contiguous(), geo)
-    hi, sel = _score(ubis_d[rtask], g, merge_tol)
+    hi, sel = _score(ubis_d, rtask, g, merge_tol)
     npks = torch.bincount(rtask[sel], minlength=T)
     fit = npks > min_grain_npks
     # ---- weighted least-squares UBI of every task with enough peaks
@@ -428,7 +443,7 @@ def refine_map(refine_points, all_pbpmap_ubis, ri_col, rj_col, sx_grid, sy_grid,
     worth = worth_d.cpu().numpy()
     ubi_out = ubi_out_d.cpu().numpy()
     # ---- re-score with the fitted UBIs, count distinct reflections
-    hi2, sel2 = _score(ubi_out_d[rtask], g, merge_tol)
+    hi2, sel2 = _score(ubi_out_d, rtask, g, merge_tol)
     npk2 = torch.bincount(rtask[sel2], minlength=T).cpu().numpy()
     r2 = torch.nonzero(sel2 & fit[rtask]).flatten()
     nuniq = np.zeros(T, np.int64)
@@ -461,7 +476,7 @@ def _strain_fit(worth_d, ub0_d, rtask, sel2, g, hi2, rydist, gnoise_std, T):
     t = rtask[rows]
     gg = g[rows]
     hk = hi2[rows].double()
-    gve0 = _rows_dot(ub0_d[t], hk)
+    gve0 = _rows_dot(ub0_d, t, hk)
     gTg0 = (gg[:, 0] * gve0[:, 0] + gg[:, 1] * gve0[:, 1]) + gg[:, 2] * gve0[:, 2]
     gTg = (gg[:, 0] * gg[:, 0] + gg[:, 1] * gg[:, 1]) + gg[:, 2] * gg[:, 2]
     strain = gTg0 / gTg - 1

@@ -459,6 +459,17 @@ int gs_pbp_ubi_fit(const double *design, const long long *off, const int *fit,
 int gs_pbp_strain_fit(const double *design, const long long *off, const int *worth, int T,
                       double *eps, gs_stream_t stream);
 
+/* Row products of refine_map's merged-peak scoring and strain fit
+ * (ImageD11/sinograms/point_by_point.py:1581-1590,1611-1614,1639): per row
+ * r < R, h[3r+i] = ((u_i0 g_0 + u_i1 g_1) + u_i2 g_2), every product and
+ * sum rounded on its own, with u = ubis[9 task[r] ..] (or ubis[9r ..] when
+ * task is NULL) and g = g[3r ..]; h may be NULL.  With hkl and ok given
+ * (both or neither): hkl = h rounded half to even, ok[r] = ((e_0^2 +
+ * e_1^2) + e_2^2) < tolsq with e = h - hkl.  Device pointers. */
+int gs_pbp_rows_dot(const double *ubis, const long long *task, long long R, const double *g,
+                    double tolsq, double *h, long long *hkl, unsigned char *ok,
+                    gs_stream_t stream);
+
 /* Per-segment column sums of a row-major (rows, C) array: out (S, C),
  * out[s][c] = the sum of rows off[s] .. off[s+1]-1 of column c, added one
  * row at a time from 0.0 (np.bincount's order: refine_map's merge,

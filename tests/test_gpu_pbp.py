@@ -242,3 +242,33 @@ def test_merge_sums_equal_the_unfused_columns(dev):
     wI = si[peak]
     want = PP.segment_sum(torch.stack([wI] + [c[peak] * wI for c in cols], 1)[perm], goff)
     assert torch.equal(got, want)
+
+
+def test_rows_dot_kernel_bit_exact(dev):
+    """gs_pbp_rows_dot against the same row products in numpy, bit for bit:
+    h = ubis[task].g, its half-to-even rounding and the tolerance test, with
+    rows exactly on half-integers and on the tolerance."""
+    import torch
+    from imaged11_amd import point_by_point as PP
+    rng = np.random.default_rng(17)
+    T, R = 50, 20000
+    ubis = rng.normal(size=(T, 3, 3)) * 3
+    task = rng.integers(0, T, R)
+    g = rng.normal(size=(R, 3))
+    g[:10] = 0.0
+    ubis[0] = np.eye(3)
+    task[:10] = 0
+    g[:10, 0] = [0.5, 1.5, 2.5, -0.5, 0.05, 0.049999999999999996, 0.0500000001, 1.0, 3.25, -2.5]
+    u = ubis[task]
+    want = np.stack([(u[:, i, 0] * g[:, 0] + u[:, i, 1] * g[:, 1]) + u[:, i, 2] * g[:, 2]
+                     for i in range(3)], 1)
+    hw = np.round(want)
+    e = want - hw
+    okw = ((e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]) + e[:, 2] * e[:, 2]) < 0.05 * 0.05
+    D = lambda a: torch.as_tensor(a, device=dev)
+    h = PP._rows_dot(D(ubis), D(task), D(g)).cpu().numpy()
+    assert np.array_equal(h.view(np.int64), want.view(np.int64))
+    hk, ok = PP._score(D(ubis), D(task), D(g), 0.05)
+    assert np.array_equal(hk.cpu().numpy(), hw.astype(np.int64))
+    assert np.array_equal(ok.cpu().numpy(), okw)
+    assert okw[:10].tolist() == [False] * 4 + [False, True, False, True, False, False]

@@ -227,6 +227,21 @@ def test_refine_map_host_stages_match_oracle(wl, monkeypatch):
                                          for s in range(len(off) - 1)]))
     monkeypatch.setattr(PP, "segment_max", segmax)
 
+    def rows_dot(ubis, task, g):
+        # h = U.g per row, ((u0 g0 + u1 g1) + u2 g2) as gs_pbp_rows_dot
+        u, g = ubis.numpy()[task.numpy()], g.numpy()
+        return torch.as_tensor(np.stack([(u[:, i, 0] * g[:, 0] + u[:, i, 1] * g[:, 1]) +
+                                         u[:, i, 2] * g[:, 2] for i in range(3)], 1))
+
+    def score(ubis, task, g, tol):
+        h = rows_dot(ubis, task, g).numpy()
+        hi = np.round(h)
+        e = h - hi
+        ok = ((e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]) + e[:, 2] * e[:, 2]) < tol * tol
+        return torch.as_tensor(hi.astype(np.int64)), torch.as_tensor(ok)
+    monkeypatch.setattr(PP, "_rows_dot", rows_dot)
+    monkeypatch.setattr(PP, "_score", score)
+
     def msums(perm, peak, si, cols, goff):
         p = peak.numpy()[perm.numpy()]
         w = si.numpy()[p]
