@@ -50,11 +50,11 @@ VALU_OPS_PER_PAIR = 19
 SIEVE_STAGE1_VALU = 8.5
 SIEVE_SETTLE_VALU = 26.5
 # FP64 VALU instructions per pair of the fused compute_gv + score_and_assign
-# kernel's certified loop, counted in its gfx950 assembly: 68 per grain step
-# of two peaks per lane (22 v_fmac, 16 v_fma, 8 v_mul, 10 v_add, 6
+# kernel's certified loop, counted in its gfx950 assembly: 70 per grain step
+# of two peaks per lane (22 v_fmac, 16 v_fma, 8 v_mul, 12 v_add, 6
 # v_fract, 2 v_rsq, 4 v_cmp; one Newton step after each v_rsq_f64), plus
 # 2.5 integer VALU per step (the label test of each peak, one LDS address)
-SCAN_VALU_OPS_PER_PAIR = 34.0
+SCAN_VALU_OPS_PER_PAIR = 35.0
 # FP64 flops per list entry of the adjoint pass (hkl-space loss with the
 # Gauss-Newton sums), FMA = 2, counted from its source: forward
 # compute_xlylzl + compute_gv 75, residual and loss 24, dL/dG 18, the

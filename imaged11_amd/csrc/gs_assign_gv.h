@@ -26,7 +26,11 @@
 // ds bounds |G|; 290 u = 3.2e-14).  A pair whose fast drlv2 - 3/4 is >= tol^2 - 3/4 + E
 // certainly fails `drlv2 < tol^2`, so its only effect is the
 // reference's else branch (label == this label -> -1), which needs no
-// arithmetic.  Every other pair is recomputed with the reference's own
+// arithmetic.  The same holds, with the same E, against the peak's current
+// drlv2 d < tol^2: fast drlv2 - 3/4 >= d - 3/4 + E certainly fails
+// `drlv2 < d`, so the exact route is taken only below min(tol^2, d) - 3/4
+// + E (the values there are no larger than at tol^2, so E still bounds
+// their rounding).  Every other pair is recomputed with the reference's own
 // compute_gv and score arithmetic (gs_gv_one, gs_drlv2) and the reference's
 // rule is applied to that exact value, so labels and stored drlv2 are
 // those of the reference loop.  A NaN or infinite fast drlv2 is never
@@ -123,8 +127,11 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
     for (int j = 0; j < 9; j++) mat[j] = smat[j];
     const double ds = sds;
 
-    // per-peak constants of the fast path and the (drlv2, label) state
-    double ax[P], ay[P], az[P], cx[P], cy[P], cz[P], d[P];
+    // per-peak constants of the fast path and the (drlv2, label) state;
+    // thm = min(tol^2, drlv2) - 3/4: a pair can change the state through
+    // the reference arithmetic only if its drlv2 is below both (else it
+    // takes the update's else branch, which needs no arithmetic)
+    double ax[P], ay[P], az[P], cx[P], cy[P], cz[P], d[P], thm[P];
     int lab[P];
     double n2min = __builtin_inf();
     const long long base = (long long)blockIdx.x * (GS_BLOCK * P) + threadIdx.x;
@@ -133,6 +140,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
         const long long k = base + (long long)p * GS_BLOCK;
         ax[p] = ay[p] = az[p] = cx[p] = cy[p] = cz[p] = 0.0;
         d[p] = 0.0;
+        thm[p] = tolsq - 0.75;
         lab[p] = -1;
         if (k < ng) {
             const double x0 = xl[3 * k], x1 = xl[3 * k + 1], x2 = xl[3 * k + 2];
@@ -145,6 +153,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
             cy[p] = c[1];
             cz[p] = c[2];
             d[p] = drlv2[k];
+            thm[p] = fmin(tolsq, d[p]) - 0.75;  // NaN drlv2: tol^2 (never below it)
             lab[p] = labels[k];
             const double n2 = x0 * x0 + x1 * x1 + x2 * x2;
             n2min = n2 == n2 ? fmin(n2min, n2) : n2min;
@@ -155,7 +164,6 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
     if (lane == 0) ln2[threadIdx.x / GS_WAVE] = n2min;
     __syncthreads();
     for (int w = 0; w < GS_BLOCK / GS_WAVE; w++) n2min = fmin(n2min, ln2[w]);
-    const double mid = tolsq - 0.75;  // gs_scan_fast_drlv2 is drlv2 - 3/4
     for (int c0 = 0; c0 < K; c0 += GS_SCAN_CHUNK) {
         const int c1 = min(K, c0 + GS_SCAN_CHUNK);
         // per grain of the chunk, once per block: translation and window
@@ -166,7 +174,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
             lgr[4 * i] = t0;
             lgr[4 * i + 1] = t1;
             lgr[4 * i + 2] = t2;
-            lgr[4 * i + 3] = mid + gs_scan_window(gs_load_ubi(ubis + 9 * g), ds, tolsq, tt, n2min);
+            lgr[4 * i + 3] = gs_scan_window(gs_load_ubi(ubis + 9 * g), ds, tolsq, tt, n2min);
             lcnt[i] = 0;
         }
         __syncthreads();
@@ -174,13 +182,15 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
             const gs_ubi u = gs_load_ubi(ubis + 9 * (size_t)g);
             const int L = glabels[g];
             const double t0 = lgr[4 * (g - c0)], t1 = lgr[4 * (g - c0) + 1],
-                         t2 = lgr[4 * (g - c0) + 2], hi = lgr[4 * (g - c0) + 3];
+                         t2 = lgr[4 * (g - c0) + 2], win = lgr[4 * (g - c0) + 3];
 #pragma unroll
             for (int p = 0; p < P; p++) {
                 const double a[3] = {ax[p], ay[p], az[p]}, c[3] = {cx[p], cy[p], cz[p]};
                 const double sf = gs_scan_fast_drlv2(u, a, c, t0, t1, t2, ds);
-                // candidates (and non-finite values) take the exact route
-                const bool cand = !(sf >= hi);
+                // candidates (and non-finite values) take the exact route:
+                // the pairs whose drlv2 may be below both tol^2 and the
+                // peak's current drlv2 (sf is drlv2 - 3/4 within win)
+                const bool cand = !(sf >= thm[p] + win);
                 // a certain failure only matters to a peak holding this
                 // grain's label (else branch of the update: label -> -1);
                 // such peaks are rare, so one compare per pair decides
@@ -203,6 +213,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_gv_assign_many(
                         if ((se < tolsq) && (se < d[p])) {
                             lab[p] = L;
                             d[p] = se;
+                            thm[p] = se - 0.75;
                             atomicAdd(&lcnt[g - c0], 1);
                         } else if (lab[p] == L) {
                             lab[p] = -1;

@@ -53,6 +53,38 @@ def test_gv_assign_bitexact_given_trig(dev, orc, scan, d0, l0):
     assert want.sum() > 0.5 * n
 
 
+@pytest.mark.parametrize("ulps", [0, 1, -1, 4096])
+def test_gv_assign_state_at_the_drlv2_boundary(dev, orc, scan, ulps):
+    """Peaks whose current drlv2 equals (or is within a few ulps of) the
+    drlv2 one grain of the scan gives them, the rest below tol^2 at random:
+    the fast path's filter against the current drlv2 leaves labels and
+    drlv2 bit-exact to the reference loop."""
+    s = scan
+    n, K = len(s["xl"]), len(s["wl"]["ubi"])
+    glab = np.arange(K, dtype=np.int32)
+    rng = np.random.default_rng(7 + ulps)
+    gs = 5
+    d1, l1 = np.ones(n), np.full(n, -1, np.int32)
+    orc.gv_assign_many(s["xl"], s["so"], s["co"], s["mat"], s["wvln"],
+                       s["wl"]["ubi"][gs:gs + 1].copy(), s["wl"]["trans"][gs:gs + 1].copy(),
+                       glab[gs:gs + 1].copy(), 0.05, d1, l1)
+    hit = l1 == gs
+    assert hit.sum() > 50
+    d0 = rng.uniform(0.0, 0.0025, n)
+    d0[hit] = d1[hit] * (1.0 + ulps * 2.0 ** -52)
+    l0 = np.where(rng.uniform(size=n) < 0.5, rng.integers(0, K, n), -1).astype(np.int32)
+    d, lab = d0.copy(), l0.copy()
+    want = orc.gv_assign_many(s["xl"], s["so"], s["co"], s["mat"], s["wvln"], s["wl"]["ubi"],
+                              s["wl"]["trans"], glab, 0.05, d, lab)
+    dd, ll = T(d0, dev), T(l0, dev, torch.int32)
+    got = batched.gv_assign_many(T(s["xl"], dev), T(s["so"], dev), T(s["co"], dev), s["wvln"],
+                                 T(s["wl"]["ubi"], dev), T(s["wl"]["trans"], dev), 0.05, dd, ll,
+                                 grain_labels=T(glab, dev, torch.int32), mat=s["mat"])
+    assert np.array_equal(got.cpu().numpy(), want)
+    assert np.array_equal(ll.cpu().numpy(), lab)
+    assert np.array_equal(dd.cpu().numpy().view(np.int64), d.view(np.int64))
+
+
 @pytest.mark.parametrize("parity", [True, False], ids=["parity", "fast"])
 def test_assignlabels_driver(dev, orc, scan, parity):
     """The shipped scan path (drivers.Scan + assignlabels).  Parity mode
